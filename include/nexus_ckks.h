@@ -1,0 +1,147 @@
+/*
+ * nexus_ckks.h -- C ABI of the MI355X-native CKKS evaluation engine.
+ *
+ * Drop-in boundary: these entry points replace the operations NEXUS's
+ * CKKSEvaluator facade reaches into its HE library for (reference
+ * cuda/src/ckks_evaluator.cuh:11-499 over Phantom's include/evaluate.cuh, and
+ * src/ckks_evaluator.h:12-103 over SEAL's seal::Evaluator).  Plain pointers and
+ * sizes only; every call is stream-ordered on the context's HIP stream and
+ * returns NX_OK or a negative status (message in nx_last_error()).
+ *
+ * Levels are Phantom-style chain indices (reference
+ * cuda/thirdparty/phantom-fhe/include/ciphertext.h:193-195): 0 = key level
+ * (all primes incl. the special prime), 1 = top data level (L primes), L = one
+ * prime left.  Residue arrays are [poly][limb][N] uint64 in limb order q_0..
+ * (reference SEAL PolyIter / Phantom ciphertext.h:41,222-228).
+ */
+#ifndef NEXUS_CKKS_H
+#define NEXUS_CKKS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct nx_context nx_context;
+typedef struct nx_ciphertext nx_ciphertext;
+typedef struct nx_plaintext nx_plaintext;
+typedef struct nx_secret_key nx_secret_key;
+typedef struct nx_kswitch_keys nx_kswitch_keys;
+
+enum {
+    NX_OK = 0,
+    NX_ERR_INVALID_ARGUMENT = -1, /* std::invalid_argument in the reference */
+    NX_ERR_LOGIC = -2,            /* std::logic_error */
+    NX_ERR_OUT_OF_RANGE = -3,     /* std::out_of_range */
+    NX_ERR_DEVICE = -4,           /* HIP runtime error */
+    NX_ERR_NO_MEMORY = -5,
+    NX_ERR_UNKNOWN = -9
+};
+
+/* Thread-local message of the last failing call on this thread. */
+const char *nx_last_error(void);
+/* Number of visible GPUs (0 without a GPU). */
+int nx_device_count(int *count);
+int nx_device_synchronize(void);
+
+/* ---- context: replaces EncryptionParameters + CoeffModulus::Create + SEALContext
+ * (src/main.cpp:44-49) / PhantomContext (cuda/src/main.cu:264-270,
+ * phantom-fhe/src/context.cu:125-238).  bit_sizes lists data primes then the
+ * special prime; hamming_weight 0 = uniform ternary secret
+ * (SEAL-4.1-bs encryptionparams.h:188-231); sparse_slots 0 = N/2. */
+int nx_context_create(int log_n, const int *bit_sizes, size_t count, uint32_t hamming_weight, size_t sparse_slots,
+                      nx_context **out);
+int nx_context_destroy(nx_context *ctx);
+int nx_context_info(const nx_context *ctx, int *log_n, size_t *data_primes, size_t *slot_count);
+/* moduli: L+1 words (q_0..q_{L-1}, p) */
+int nx_context_moduli(const nx_context *ctx, uint64_t *moduli);
+int nx_context_synchronize(nx_context *ctx);
+/* Galois element of a rotation step (SEAL util/galois.cpp:53-90, generator 5). */
+int nx_galois_elt_from_step(const nx_context *ctx, int step, uint32_t *elt);
+/* Device memory held by the context's cache: in use / cached, bytes. */
+int nx_context_memory(const nx_context *ctx, size_t *in_use, size_t *cached);
+
+/* ---- ciphertexts / plaintexts (value types with deep copy; reference
+ * PhantomCiphertext / seal::Ciphertext) */
+int nx_ciphertext_create(nx_context *ctx, nx_ciphertext **out);
+int nx_ciphertext_destroy(nx_ciphertext *ct);
+int nx_ciphertext_copy(nx_ciphertext *dst, const nx_ciphertext *src);
+int nx_ciphertext_upload(nx_ciphertext *ct, const uint64_t *data, size_t size, size_t chain_index, double scale,
+                         int ntt_form);
+int nx_ciphertext_download(const nx_ciphertext *ct, uint64_t *data);
+int nx_ciphertext_info(const nx_ciphertext *ct, size_t *size, size_t *chain_index, size_t *coeff_modulus_size,
+                       double *scale, int *ntt_form);
+int nx_ciphertext_set_scale(nx_ciphertext *ct, double scale);
+
+int nx_plaintext_create(nx_context *ctx, nx_plaintext **out);
+int nx_plaintext_destroy(nx_plaintext *pt);
+int nx_plaintext_copy(nx_plaintext *dst, const nx_plaintext *src);
+int nx_plaintext_upload(nx_plaintext *pt, const uint64_t *data, size_t chain_index, double scale, int ntt_form);
+int nx_plaintext_download(const nx_plaintext *pt, uint64_t *data);
+int nx_plaintext_info(const nx_plaintext *pt, size_t *chain_index, size_t *coeff_modulus_size, double *scale);
+int nx_plaintext_set_scale(nx_plaintext *pt, double scale);
+
+/* ---- keys (reference KeyGenerator, src/main.cpp:51-58; PhantomSecretKey
+ * cuda/src/main.cu:272-278).  Key-switching keys are device resident;
+ * elt 0 addresses the relinearisation key of a set. */
+int nx_secret_key_generate(nx_context *ctx, uint64_t seed, nx_secret_key **out);
+int nx_secret_key_from_coeffs(nx_context *ctx, const int8_t *coeffs, nx_secret_key **out);
+int nx_secret_key_coeffs(const nx_secret_key *sk, int8_t *coeffs);
+int nx_secret_key_destroy(nx_secret_key *sk);
+int nx_kswitch_keys_create(nx_context *ctx, nx_kswitch_keys **out);
+int nx_kswitch_keys_destroy(nx_kswitch_keys *keys);
+int nx_relin_key_generate(nx_context *ctx, const nx_secret_key *sk, uint64_t seed, nx_kswitch_keys *keys);
+int nx_galois_keys_generate(nx_context *ctx, const nx_secret_key *sk, const uint32_t *elts, size_t count,
+                            uint64_t seed, nx_kswitch_keys *keys);
+/* key material [L digits][2][L+1][N] words */
+int nx_kswitch_key_upload(nx_kswitch_keys *keys, uint32_t elt, const uint64_t *data);
+int nx_kswitch_key_download(const nx_kswitch_keys *keys, uint32_t elt, uint64_t *data);
+int nx_kswitch_keys_has(const nx_kswitch_keys *keys, uint32_t elt, int *present);
+
+/* ---- encryptor / decryptor (reference cuda/src/ckks_evaluator.cuh:78-94,396-419) */
+int nx_encrypt_symmetric(nx_context *ctx, const nx_secret_key *sk, const nx_plaintext *pt, uint64_t seed,
+                         nx_ciphertext *out);
+int nx_decrypt(nx_context *ctx, const nx_secret_key *sk, const nx_ciphertext *ct, nx_plaintext *out);
+
+/* ---- encoder (reference Encoder, cuda/src/ckks_evaluator.cuh:11-76;
+ * SEAL ckks.h:457-760) */
+int nx_encode(nx_context *ctx, const double *re, const double *im, size_t count, size_t chain_index, double scale,
+              nx_plaintext *out);
+int nx_encode_constant(nx_context *ctx, double value, size_t chain_index, double scale, nx_plaintext *out);
+int nx_decode(nx_context *ctx, const nx_plaintext *pt, double *re, double *im, size_t count);
+
+/* ---- evaluator (reference Evaluator, cuda/src/ckks_evaluator.cuh:115-394;
+ * SEAL evaluator.cpp).  dest may alias any input. */
+int nx_add(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest);
+int nx_sub(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest);
+int nx_negate(const nx_ciphertext *a, nx_ciphertext *dest);
+int nx_add_plain(const nx_ciphertext *a, const nx_plaintext *p, nx_ciphertext *dest);
+int nx_sub_plain(const nx_ciphertext *a, const nx_plaintext *p, nx_ciphertext *dest);
+int nx_multiply_plain(const nx_ciphertext *a, const nx_plaintext *p, nx_ciphertext *dest);
+int nx_multiply(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest);
+int nx_square(const nx_ciphertext *a, nx_ciphertext *dest);
+int nx_relinearize(const nx_ciphertext *a, const nx_kswitch_keys *relin_keys, nx_ciphertext *dest);
+int nx_rescale_to_next(const nx_ciphertext *a, nx_ciphertext *dest);
+int nx_mod_switch_to_next(const nx_ciphertext *a, nx_ciphertext *dest);
+int nx_mod_switch_to(const nx_ciphertext *a, size_t chain_index, nx_ciphertext *dest);
+int nx_plain_mod_switch_to(const nx_plaintext *a, size_t chain_index, nx_plaintext *dest);
+int nx_apply_galois(const nx_ciphertext *a, uint32_t elt, const nx_kswitch_keys *galois_keys, nx_ciphertext *dest);
+int nx_rotate_vector(const nx_ciphertext *a, int steps, const nx_kswitch_keys *galois_keys, nx_ciphertext *dest);
+int nx_complex_conjugate(const nx_ciphertext *a, const nx_kswitch_keys *galois_keys, nx_ciphertext *dest);
+int nx_transform_to_ntt(const nx_ciphertext *a, nx_ciphertext *dest);
+int nx_transform_from_ntt(const nx_ciphertext *a, nx_ciphertext *dest);
+int nx_add_many(const nx_ciphertext *const *cts, size_t count, nx_ciphertext *dest);
+/* SEAL-4.1-bs Evaluator extensions (evaluator.cpp:395-418): the constant is
+ * encoded at the ciphertext's scale */
+int nx_multiply_const(const nx_ciphertext *a, double value, nx_ciphertext *dest);
+int nx_add_const(const nx_ciphertext *a, double value, nx_ciphertext *dest);
+/* multiply by x^exponent (coefficient shift; reference matrix_mul.cu:84-127) */
+int nx_multiply_power_of_x(const nx_ciphertext *a, int64_t exponent, nx_ciphertext *dest);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEXUS_CKKS_H */

@@ -1,0 +1,455 @@
+"""MI355X-native CKKS evaluation engine (Python view of the C ABI).
+
+Thin ctypes layer over ``libnexus_ckks.so`` (include/nexus_ckks.h).  There is no
+Python or CPU fallback: importing this package loads the HIP library, and any
+call that needs the GPU fails with :class:`NexusError` when none is present.
+"""
+import atexit
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnexus_ckks.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError("libnexus_ckks.so is not built; run `python -m nexus_amd.build`")
+
+_lib = ctypes.CDLL(LIB_PATH)
+
+u64p = ctypes.POINTER(ctypes.c_uint64)
+f64p = ctypes.POINTER(ctypes.c_double)
+vp = ctypes.c_void_p
+sz = ctypes.c_size_t
+
+
+class NexusError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (status %d)" % (msg, code))
+        self.code = code
+
+
+_ERR_TYPES = {-1: ValueError, -2: RuntimeError, -3: IndexError}
+
+
+def _check(rc):
+    if rc != 0:
+        msg = _lib.nx_last_error().decode()
+        raise NexusError(rc, msg)
+
+
+def _sig(name, args):
+    f = getattr(_lib, name)
+    f.restype = ctypes.c_int
+    f.argtypes = args
+    return f
+
+
+_lib.nx_last_error.restype = ctypes.c_char_p
+_lib.nx_last_error.argtypes = []
+_sig("nx_device_count", [ctypes.POINTER(ctypes.c_int)])
+_sig("nx_device_synchronize", [])
+_sig("nx_context_create", [ctypes.c_int, ctypes.POINTER(ctypes.c_int), sz, ctypes.c_uint32, sz, ctypes.POINTER(vp)])
+_sig("nx_context_destroy", [vp])
+_sig("nx_context_info", [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(sz), ctypes.POINTER(sz)])
+_sig("nx_context_moduli", [vp, u64p])
+_sig("nx_context_synchronize", [vp])
+_sig("nx_galois_elt_from_step", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)])
+_sig("nx_context_memory", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)])
+for _n in ("ciphertext", "plaintext"):
+    _sig("nx_%s_create" % _n, [vp, ctypes.POINTER(vp)])
+    _sig("nx_%s_destroy" % _n, [vp])
+    _sig("nx_%s_copy" % _n, [vp, vp])
+    _sig("nx_%s_download" % _n, [vp, u64p])
+    _sig("nx_%s_set_scale" % _n, [vp, ctypes.c_double])
+_sig("nx_ciphertext_upload", [vp, u64p, sz, sz, ctypes.c_double, ctypes.c_int])
+_sig("nx_ciphertext_info", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz),
+                            ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)])
+_sig("nx_plaintext_upload", [vp, u64p, sz, ctypes.c_double, ctypes.c_int])
+_sig("nx_plaintext_info", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double)])
+_sig("nx_secret_key_generate", [vp, ctypes.c_uint64, ctypes.POINTER(vp)])
+_sig("nx_secret_key_from_coeffs", [vp, ctypes.POINTER(ctypes.c_int8), ctypes.POINTER(vp)])
+_sig("nx_secret_key_coeffs", [vp, ctypes.POINTER(ctypes.c_int8)])
+_sig("nx_secret_key_destroy", [vp])
+_sig("nx_kswitch_keys_create", [vp, ctypes.POINTER(vp)])
+_sig("nx_kswitch_keys_destroy", [vp])
+_sig("nx_relin_key_generate", [vp, vp, ctypes.c_uint64, vp])
+_sig("nx_galois_keys_generate", [vp, vp, ctypes.POINTER(ctypes.c_uint32), sz, ctypes.c_uint64, vp])
+_sig("nx_kswitch_key_upload", [vp, ctypes.c_uint32, u64p])
+_sig("nx_kswitch_key_download", [vp, ctypes.c_uint32, u64p])
+_sig("nx_kswitch_keys_has", [vp, ctypes.c_uint32, ctypes.POINTER(ctypes.c_int)])
+_sig("nx_encrypt_symmetric", [vp, vp, vp, ctypes.c_uint64, vp])
+_sig("nx_decrypt", [vp, vp, vp, vp])
+_sig("nx_encode", [vp, f64p, f64p, sz, sz, ctypes.c_double, vp])
+_sig("nx_encode_constant", [vp, ctypes.c_double, sz, ctypes.c_double, vp])
+_sig("nx_decode", [vp, vp, f64p, f64p, sz])
+for _n in ("add", "sub", "multiply"):
+    _sig("nx_" + _n, [vp, vp, vp])
+for _n in ("add_plain", "sub_plain", "multiply_plain"):
+    _sig("nx_" + _n, [vp, vp, vp])
+for _n in ("negate", "square", "rescale_to_next", "mod_switch_to_next", "transform_to_ntt", "transform_from_ntt"):
+    _sig("nx_" + _n, [vp, vp])
+_sig("nx_relinearize", [vp, vp, vp])
+_sig("nx_mod_switch_to", [vp, sz, vp])
+_sig("nx_plain_mod_switch_to", [vp, sz, vp])
+_sig("nx_apply_galois", [vp, ctypes.c_uint32, vp, vp])
+_sig("nx_rotate_vector", [vp, ctypes.c_int, vp, vp])
+_sig("nx_complex_conjugate", [vp, vp, vp])
+_sig("nx_add_many", [ctypes.POINTER(vp), sz, vp])
+_sig("nx_multiply_const", [vp, ctypes.c_double, vp])
+_sig("nx_add_const", [vp, ctypes.c_double, vp])
+_sig("nx_multiply_power_of_x", [vp, ctypes.c_int64, vp])
+
+
+def lib():
+    return _lib
+
+
+# Handles are released by their destructors while the interpreter runs; once
+# it starts shutting down, nothing calls into the library any more (the
+# process exit reclaims device memory), so teardown order cannot matter.
+_state = {"alive": True}
+atexit.register(lambda: _state.__setitem__("alive", False))
+
+
+def _release(obj, fn, _st=_state):
+    h = getattr(obj, "h", None)
+    if h and _st["alive"]:
+        fn(h)
+    obj.h = None
+
+
+def device_count():
+    c = ctypes.c_int()
+    _check(_lib.nx_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def _u64(a):
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    return a, a.ctypes.data_as(u64p)
+
+
+class Context:
+    """CKKS parameters + device tables (SEALContext / PhantomContext)."""
+
+    def __init__(self, log_n, bit_sizes, hamming_weight=0, sparse_slots=0):
+        bits = (ctypes.c_int * len(bit_sizes))(*bit_sizes)
+        h = vp()
+        _check(_lib.nx_context_create(log_n, bits, len(bit_sizes), hamming_weight, sparse_slots, ctypes.byref(h)))
+        self.h = h
+        self.log_n = log_n
+        self.n = 1 << log_n
+        mods = np.zeros(len(bit_sizes), dtype=np.uint64)
+        _check(_lib.nx_context_moduli(self.h, mods.ctypes.data_as(u64p)))
+        self.moduli = [int(x) for x in mods]
+        self.L = len(bit_sizes) - 1
+        self.slot_count = self.n // 2
+
+    def __del__(self, _fn=_lib.nx_context_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def limbs_of(self, chain_index):
+        return self.L + 1 - chain_index
+
+    def synchronize(self):
+        _check(_lib.nx_context_synchronize(self.h))
+
+    def elt_from_step(self, step):
+        e = ctypes.c_uint32()
+        _check(_lib.nx_galois_elt_from_step(self.h, step, ctypes.byref(e)))
+        return e.value
+
+    def memory(self):
+        a, b = sz(), sz()
+        _check(_lib.nx_context_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+
+class Ciphertext:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = vp()
+        _check(_lib.nx_ciphertext_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self, _fn=_lib.nx_ciphertext_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def info(self):
+        s, ci, cm, sc, nt = sz(), sz(), sz(), ctypes.c_double(), ctypes.c_int()
+        _check(_lib.nx_ciphertext_info(self.h, ctypes.byref(s), ctypes.byref(ci), ctypes.byref(cm),
+                                       ctypes.byref(sc), ctypes.byref(nt)))
+        return dict(size=s.value, chain_index=ci.value, limbs=cm.value, scale=sc.value, ntt=bool(nt.value))
+
+    @property
+    def scale(self):
+        return self.info()["scale"]
+
+    @scale.setter
+    def scale(self, v):
+        _check(_lib.nx_ciphertext_set_scale(self.h, v))
+
+    @property
+    def chain_index(self):
+        return self.info()["chain_index"]
+
+    @property
+    def coeff_modulus_size(self):
+        return self.info()["limbs"]
+
+    @classmethod
+    def from_array(cls, ctx, data, chain_index, scale, ntt_form=True):
+        """data: (size, limbs, n) uint64 residues."""
+        ct = cls(ctx)
+        arr, p = _u64(data)
+        _check(_lib.nx_ciphertext_upload(ct.h, p, arr.shape[0], chain_index, scale, 1 if ntt_form else 0))
+        return ct
+
+    def to_array(self):
+        i = self.info()
+        out = np.zeros((i["size"], i["limbs"], self.ctx.n), dtype=np.uint64)
+        _check(_lib.nx_ciphertext_download(self.h, out.ctypes.data_as(u64p)))
+        return out
+
+    def copy(self):
+        c = Ciphertext(self.ctx)
+        _check(_lib.nx_ciphertext_copy(c.h, self.h))
+        return c
+
+
+class Plaintext:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = vp()
+        _check(_lib.nx_plaintext_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self, _fn=_lib.nx_plaintext_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def info(self):
+        ci, cm, sc = sz(), sz(), ctypes.c_double()
+        _check(_lib.nx_plaintext_info(self.h, ctypes.byref(ci), ctypes.byref(cm), ctypes.byref(sc)))
+        return dict(chain_index=ci.value, limbs=cm.value, scale=sc.value)
+
+    @property
+    def scale(self):
+        return self.info()["scale"]
+
+    @scale.setter
+    def scale(self, v):
+        _check(_lib.nx_plaintext_set_scale(self.h, v))
+
+    @classmethod
+    def from_array(cls, ctx, data, chain_index, scale, ntt_form=True):
+        pt = cls(ctx)
+        arr, p = _u64(data)
+        _check(_lib.nx_plaintext_upload(pt.h, p, chain_index, scale, 1 if ntt_form else 0))
+        return pt
+
+    def to_array(self):
+        i = self.info()
+        out = np.zeros((i["limbs"], self.ctx.n), dtype=np.uint64)
+        _check(_lib.nx_plaintext_download(self.h, out.ctypes.data_as(u64p)))
+        return out
+
+
+class SecretKey:
+    def __init__(self, ctx, seed=None, coeffs=None):
+        self.ctx = ctx
+        h = vp()
+        if coeffs is not None:
+            c = np.ascontiguousarray(coeffs, dtype=np.int8)
+            _check(_lib.nx_secret_key_from_coeffs(ctx.h, c.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)),
+                                                  ctypes.byref(h)))
+        else:
+            _check(_lib.nx_secret_key_generate(ctx.h, seed or 0, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self, _fn=_lib.nx_secret_key_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def coeffs(self):
+        out = np.zeros(self.ctx.n, dtype=np.int8)
+        _check(_lib.nx_secret_key_coeffs(self.h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int8))))
+        return out
+
+
+class KSwitchKeys:
+    """A set of device-resident key-switching keys (relin = element 0)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        h = vp()
+        _check(_lib.nx_kswitch_keys_create(ctx.h, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self, _fn=_lib.nx_kswitch_keys_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def generate_relin(self, sk, seed=1):
+        _check(_lib.nx_relin_key_generate(self.ctx.h, sk.h, seed, self.h))
+        return self
+
+    def generate_galois(self, sk, elts, seed=2):
+        arr = (ctypes.c_uint32 * len(elts))(*elts)
+        _check(_lib.nx_galois_keys_generate(self.ctx.h, sk.h, arr, len(elts), seed, self.h))
+        return self
+
+    def key_array(self, elt):
+        L = self.ctx.L
+        out = np.zeros((L, 2, L + 1, self.ctx.n), dtype=np.uint64)
+        _check(_lib.nx_kswitch_key_download(self.h, elt, out.ctypes.data_as(u64p)))
+        return out
+
+    def upload(self, elt, data):
+        arr, p = _u64(data)
+        _check(_lib.nx_kswitch_key_upload(self.h, elt, p))
+
+    def has(self, elt):
+        v = ctypes.c_int()
+        _check(_lib.nx_kswitch_keys_has(self.h, elt, ctypes.byref(v)))
+        return bool(v.value)
+
+
+# ---------------------------------------------------------------- encoder / crypto
+def encode(ctx, values, scale, chain_index=1):
+    v = np.asarray(values)
+    re = np.ascontiguousarray(np.real(v), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(v), dtype=np.float64) if np.iscomplexobj(v) else None
+    pt = Plaintext(ctx)
+    _check(_lib.nx_encode(ctx.h, re.ctypes.data_as(f64p), im.ctypes.data_as(f64p) if im is not None else None,
+                          len(re), chain_index, scale, pt.h))
+    return pt
+
+
+def encode_constant(ctx, value, scale, chain_index=1):
+    pt = Plaintext(ctx)
+    _check(_lib.nx_encode_constant(ctx.h, value, chain_index, scale, pt.h))
+    return pt
+
+
+def decode(ctx, pt, count=None, complex_out=False):
+    count = count or ctx.slot_count
+    re = np.zeros(count)
+    im = np.zeros(count)
+    _check(_lib.nx_decode(ctx.h, pt.h, re.ctypes.data_as(f64p), im.ctypes.data_as(f64p), count))
+    return re + 1j * im if complex_out else re
+
+
+def encrypt(ctx, sk, pt, seed=0):
+    ct = Ciphertext(ctx)
+    _check(_lib.nx_encrypt_symmetric(ctx.h, sk.h, pt.h, seed, ct.h))
+    return ct
+
+
+def decrypt(ctx, sk, ct):
+    pt = Plaintext(ctx)
+    _check(_lib.nx_decrypt(ctx.h, sk.h, ct.h, pt.h))
+    return pt
+
+
+# ---------------------------------------------------------------- evaluator
+def _out(ctx, dest):
+    return dest if dest is not None else Ciphertext(ctx)
+
+
+def _binop(name):
+    f = getattr(_lib, "nx_" + name)
+
+    def op(a, b, dest=None):
+        d = _out(a.ctx, dest)
+        _check(f(a.h, b.h, d.h))
+        return d
+
+    op.__name__ = name
+    return op
+
+
+def _unop(name):
+    f = getattr(_lib, "nx_" + name)
+
+    def op(a, dest=None):
+        d = _out(a.ctx, dest)
+        _check(f(a.h, d.h))
+        return d
+
+    op.__name__ = name
+    return op
+
+
+add = _binop("add")
+sub = _binop("sub")
+multiply = _binop("multiply")
+add_plain = _binop("add_plain")
+sub_plain = _binop("sub_plain")
+multiply_plain = _binop("multiply_plain")
+negate = _unop("negate")
+square = _unop("square")
+rescale_to_next = _unop("rescale_to_next")
+mod_switch_to_next = _unop("mod_switch_to_next")
+transform_to_ntt = _unop("transform_to_ntt")
+transform_from_ntt = _unop("transform_from_ntt")
+
+
+def relinearize(a, relin_keys, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_relinearize(a.h, relin_keys.h, d.h))
+    return d
+
+
+def mod_switch_to(a, chain_index, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_mod_switch_to(a.h, chain_index, d.h))
+    return d
+
+
+def plain_mod_switch_to(p, chain_index):
+    d = Plaintext(p.ctx)
+    _check(_lib.nx_plain_mod_switch_to(p.h, chain_index, d.h))
+    return d
+
+
+def apply_galois(a, elt, galois_keys, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_apply_galois(a.h, elt, galois_keys.h, d.h))
+    return d
+
+
+def rotate_vector(a, steps, galois_keys, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_rotate_vector(a.h, steps, galois_keys.h, d.h))
+    return d
+
+
+def complex_conjugate(a, galois_keys, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_complex_conjugate(a.h, galois_keys.h, d.h))
+    return d
+
+
+def add_many(cts, dest=None):
+    d = _out(cts[0].ctx, dest)
+    arr = (vp * len(cts))(*[c.h for c in cts])
+    _check(_lib.nx_add_many(arr, len(cts), d.h))
+    return d
+
+
+def multiply_const(a, value, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_multiply_const(a.h, value, d.h))
+    return d
+
+
+def add_const(a, value, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_add_const(a.h, value, d.h))
+    return d
+
+
+def multiply_power_of_x(a, exponent, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_multiply_power_of_x(a.h, exponent, d.h))
+    return d

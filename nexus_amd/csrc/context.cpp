@@ -1,0 +1,258 @@
+#include "context.h"
+
+#include <algorithm>
+#include <cmath>
+
+#include "numth.h"
+
+namespace nx {
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        int code = e == hipErrorOutOfMemory ? kNoMemory : kDeviceError;
+        fail(code, std::string("HIP error: ") + hipGetErrorString(e) + " in " + what);
+    }
+}
+
+// ---------------------------------------------------------------- memory
+DeviceCache::~DeviceCache() { trim(); }
+
+void *DeviceCache::get(size_t bytes)
+{
+    if (!bytes) return nullptr;
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = free_.find(bytes);
+        if (it != free_.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            cached_ -= bytes;
+            in_use_ += bytes;
+            return p;
+        }
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        // release cached blocks and retry once
+        (void)hipGetLastError();
+        trim();
+        NX_HIP(hipMalloc(&p, bytes));
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    in_use_ += bytes;
+    return p;
+}
+
+void DeviceCache::put(void *p, size_t bytes)
+{
+    if (!p) return;
+    std::lock_guard<std::mutex> g(mu_);
+    free_[bytes].push_back(p);
+    in_use_ -= bytes;
+    cached_ += bytes;
+}
+
+void DeviceCache::trim()
+{
+    std::lock_guard<std::mutex> g(mu_);
+    if (!free_.empty()) (void)hipDeviceSynchronize();
+    for (auto &kv : free_)
+        for (void *p : kv.second) (void)hipFree(p);
+    free_.clear();
+    cached_ = 0;
+}
+
+DevBuf::DevBuf(Context *ctx, size_t words) { reset(ctx, words); }
+
+DevBuf &DevBuf::operator=(DevBuf &&o) noexcept
+{
+    if (this != &o) {
+        release();
+        cache_ = std::move(o.cache_);
+        p_ = o.p_;
+        words_ = o.words_;
+        o.p_ = nullptr;
+        o.words_ = 0;
+    }
+    return *this;
+}
+
+void DevBuf::reset(Context *ctx, size_t words)
+{
+    if (p_ && cache_ == ctx->cache && words == words_) return;
+    release();
+    cache_ = ctx->cache;
+    words_ = words;
+    p_ = words ? (uint64_t *)cache_->get(words * 8) : nullptr;
+}
+
+void DevBuf::release()
+{
+    if (p_ && cache_) cache_->put(p_, words_ * 8);
+    p_ = nullptr;
+    words_ = 0;
+    cache_.reset();
+}
+
+// ---------------------------------------------------------------- context
+Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size_t sparse)
+    : logn(logn_), n(size_t(1) << logn_), hamming_weight(hw), sparse_slots(sparse)
+{
+    require(logn >= 1 && logn <= 17, "poly_modulus_degree is invalid");
+    require(bit_sizes.size() >= 2 && bit_sizes.size() <= (size_t)kMaxPrimes,
+            "coeff modulus needs at least one data prime and one special prime");
+    moduli = coeff_modulus_create(n, bit_sizes);
+    L = moduli.size() - 1;
+    if (sparse_slots == 0) sparse_slots = n / 2;
+    require(sparse_slots <= n / 2 && (sparse_slots & (sparse_slots - 1)) == 0, "sparse_slots is invalid");
+    for (uint64_t q : moduli) mods.push_back(make_modconst(q));
+
+    int dev = 0;
+    NX_HIP(hipGetDevice(&dev));
+    NX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+
+    const size_t P = moduli.size();
+    std::vector<ulonglong2> fwd(P * n), inv(P * n), ninv(P);
+    fwd_host.resize(P);
+    inv_host.resize(P);
+    for (size_t i = 0; i < P; i++) {
+        const uint64_t q = moduli[i];
+        ntt_root_tables(logn, q, fwd_host[i], inv_host[i]);
+        for (size_t k = 0; k < n; k++) {
+            fwd[i * n + k] = make_ulonglong2(fwd_host[i][k], shoup_quotient(fwd_host[i][k], q));
+            inv[i * n + k] = make_ulonglong2(inv_host[i][k], shoup_quotient(inv_host[i][k], q));
+        }
+        uint64_t ni = host_inv_mod(n % q, q);
+        ninv[i] = make_ulonglong2(ni, shoup_quotient(ni, q));
+    }
+    // rescale constants: row `l` (limbs before the rescale), column i < l-1
+    std::vector<ulonglong2> resc(P * P, make_ulonglong2(0, 0));
+    for (size_t l = 2; l <= L; l++) {
+        const uint64_t ql = moduli[l - 1];
+        for (size_t i = 0; i + 1 < l; i++) {
+            const uint64_t qi = moduli[i];
+            uint64_t v = host_inv_mod(ql % qi, qi);
+            resc[l * P + i] = make_ulonglong2(v, shoup_quotient(v, qi));
+        }
+    }
+    std::vector<ulonglong2> pinv(P, make_ulonglong2(0, 0));
+    for (size_t i = 0; i < L; i++) {
+        const uint64_t qi = moduli[i];
+        uint64_t v = host_inv_mod(moduli[L] % qi, qi);
+        pinv[i] = make_ulonglong2(v, shoup_quotient(v, qi));
+    }
+
+    NX_HIP(hipMalloc(&d_fwd, P * n * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_inv, P * n * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_ninv, P * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_mod, P * sizeof(ModConst)));
+    NX_HIP(hipMalloc(&d_rescale_inv, P * P * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_p_inv, P * sizeof(ulonglong2)));
+    NX_HIP(hipMemcpy(d_fwd, fwd.data(), P * n * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_inv, inv.data(), P * n * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_ninv, ninv.data(), P * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_mod, mods.data(), P * sizeof(ModConst), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_rescale_inv, resc.data(), P * P * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_p_inv, pinv.data(), P * sizeof(ulonglong2), hipMemcpyHostToDevice));
+
+    tables.logn = logn;
+    tables.num_primes = (int)P;
+    tables.fwd = d_fwd;
+    tables.inv = d_inv;
+    tables.ninv = d_ninv;
+    tables.mod = d_mod;
+}
+
+Context::~Context()
+{
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (auto &kv : galois_) (void)hipFree(kv.second);
+    (void)hipFree(d_fwd);
+    (void)hipFree(d_inv);
+    (void)hipFree(d_ninv);
+    (void)hipFree(d_mod);
+    (void)hipFree(d_rescale_inv);
+    (void)hipFree(d_p_inv);
+    cache->trim();
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+int Context::total_bits(size_t limbs) const
+{
+    // significant bits of q_0 * ... * q_{limbs-1} (SEAL total_coeff_modulus_bit_count)
+    std::vector<uint64_t> prod(limbs + 1, 0);
+    prod[0] = 1;
+    for (size_t i = 0; i < limbs; i++) {
+        u128 carry = 0;
+        for (size_t w = 0; w <= limbs; w++) {
+            u128 t = (u128)prod[w] * moduli[i] + carry;
+            prod[w] = (uint64_t)t;
+            carry = t >> 64;
+        }
+    }
+    for (size_t w = limbs + 1; w-- > 0;)
+        if (prod[w]) return (int)(w * 64 + 64 - __builtin_clzll(prod[w]));
+    return 0;
+}
+
+uint32_t Context::elt_from_step(int step) const
+{
+    // thirdparty/SEAL-4.1-bs/native/src/seal/util/galois.cpp:53-90, generator 5 (galois.h:169)
+    const uint32_t m = 2 * (uint32_t)n;
+    if (step == 0) return m - 1;
+    const bool neg = step < 0;
+    const uint32_t pos = (uint32_t)(neg ? -(int64_t)step : step);
+    require(pos < n / 2, "step count too large");
+    uint32_t s = neg ? (uint32_t)(n / 2) - pos : pos;
+    uint64_t e = 1;
+    while (s--) e = (e * 5) & (m - 1);
+    return (uint32_t)e;
+}
+
+const uint32_t *Context::galois_table(uint32_t elt)
+{
+    std::lock_guard<std::mutex> g(mu_);
+    auto it = galois_.find(elt);
+    if (it != galois_.end()) return it->second;
+    require((elt & 1) && elt < 2 * n, "Galois element is not valid");
+    // util/galois.cpp:18-51
+    std::vector<uint32_t> t(n);
+    const uint32_t mask = (uint32_t)n - 1;
+    for (uint32_t i = (uint32_t)n; i < 2 * (uint32_t)n; i++) {
+        uint32_t rev = reverse_bits(i, logn + 1);
+        uint64_t raw = ((uint64_t)elt * rev) >> 1;
+        t[i - n] = reverse_bits((uint32_t)(raw & mask), logn);
+    }
+    uint32_t *d = nullptr;
+    NX_HIP(hipMalloc(&d, n * sizeof(uint32_t)));
+    NX_HIP(hipMemcpy(d, t.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    galois_[elt] = d;
+    return d;
+}
+
+// ---------------------------------------------------------------- NTT helpers
+static NttBatch make_batch(uint64_t *data, size_t polys, size_t limbs, size_t poly_stride)
+{
+    NttBatch b;
+    b.data = data;
+    b.poly_stride = poly_stride;
+    b.polys = (uint32_t)polys;
+    b.limbs = (uint32_t)limbs;
+    b.prime_by_poly = 0;
+    for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)i;
+    return b;
+}
+
+void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride)
+{
+    ntt_forward(c.tables, make_batch(data, polys, limbs, poly_stride), c.stream);
+}
+
+void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride)
+{
+    ntt_inverse(c.tables, make_batch(data, polys, limbs, poly_stride), c.stream);
+}
+
+}  // namespace nx

@@ -1,0 +1,214 @@
+// Host-side engine state: CKKS parameter chain, device tables, device memory,
+// ciphertext / plaintext / key objects.  Internal to libnexus_ckks.so; the
+// public surface is the C ABI in include/nexus_ckks.h.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "kernels.h"
+#include "modarith.h"
+
+namespace nx {
+
+// ---------------------------------------------------------------- errors
+enum ErrCode { kOk = 0, kInvalidArgument = -1, kLogicError = -2, kOutOfRange = -3, kDeviceError = -4,
+               kNoMemory = -5 };
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string &msg) { throw Error(code, msg); }
+inline void require(bool ok, const char *msg, int code = kInvalidArgument)
+{
+    if (!ok) fail(code, msg);
+}
+
+void hip_check(hipError_t e, const char *what);
+#define NX_HIP(x) ::nx::hip_check((x), #x)
+
+// ---------------------------------------------------------------- memory
+// Size-bucketed cache over hipMalloc.  All work of one context is ordered on
+// one stream, so a block released after its last enqueued use can be handed to
+// the next allocation without waiting: the reuse is ordered after it.
+class DeviceCache {
+public:
+    ~DeviceCache();
+    void *get(size_t bytes);
+    void put(void *p, size_t bytes);
+    void trim();
+    size_t bytes_in_use() const { return in_use_; }
+    size_t bytes_cached() const { return cached_; }
+
+private:
+    std::unordered_map<size_t, std::vector<void *>> free_;
+    size_t in_use_ = 0, cached_ = 0;
+    std::mutex mu_;
+};
+
+class Context;
+
+// Device buffer from a context's cache.  The buffer shares ownership of the
+// cache, so releasing it stays valid even after its context is gone.
+class DevBuf {
+public:
+    DevBuf() = default;
+    DevBuf(Context *ctx, size_t words);
+    ~DevBuf() { release(); }
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    DevBuf(DevBuf &&o) noexcept { *this = std::move(o); }
+    DevBuf &operator=(DevBuf &&o) noexcept;
+    void reset(Context *ctx, size_t words);  // reallocate when the size changes
+    void release();
+    uint64_t *get() const { return p_; }
+    size_t words() const { return words_; }
+
+private:
+    std::shared_ptr<DeviceCache> cache_;
+    uint64_t *p_ = nullptr;
+    size_t words_ = 0;
+};
+
+// ---------------------------------------------------------------- objects
+struct Ciphertext {
+    Context *ctx = nullptr;
+    size_t size = 0;         // number of polynomials (2 or 3)
+    size_t limbs = 0;        // RNS limbs (data primes q_0..q_{limbs-1})
+    double scale = 1.0;
+    bool ntt = true;
+    DevBuf data;             // [size][limbs][n]
+};
+
+struct Plaintext {
+    Context *ctx = nullptr;
+    size_t limbs = 0;
+    double scale = 1.0;
+    bool ntt = true;
+    DevBuf data;             // [limbs][n]
+};
+
+struct SecretKey {
+    Context *ctx = nullptr;
+    std::vector<int8_t> coeff;  // ternary coefficients (host copy, for export)
+    DevBuf ntt;                 // [L+1][n], NTT form at the key level
+};
+
+// Key-switching keys: one entry per Galois element (0 = relinearisation).
+// Layout per entry: [L digits][2 polys][L+1 limbs][n], NTT form.
+struct KSwitchKeys {
+    Context *ctx = nullptr;
+    std::map<uint32_t, DevBuf> keys;
+};
+
+// ---------------------------------------------------------------- context
+class Context {
+public:
+    Context(int logn, const std::vector<int> &bit_sizes, uint32_t hamming_weight, size_t sparse_slots);
+    ~Context();
+
+    int logn;
+    size_t n;
+    size_t L;                          // data primes at the top level
+    std::vector<uint64_t> moduli;      // q_0..q_{L-1}, p (special prime last)
+    std::vector<ModConst> mods;
+    uint32_t hamming_weight;
+    size_t sparse_slots;
+    hipStream_t stream = nullptr;
+    std::shared_ptr<DeviceCache> cache = std::make_shared<DeviceCache>();
+
+    // device tables
+    DeviceTables tables;
+    ulonglong2 *d_fwd = nullptr, *d_inv = nullptr, *d_ninv = nullptr;
+    ModConst *d_mod = nullptr;
+    // rescale: d_rescale_inv[l] = {q_{l-1}^-1 mod q_i, Shoup} for i < l-1 (row l of a (L+1)x(L+1) table)
+    ulonglong2 *d_rescale_inv = nullptr;
+    // ModDown: {p^-1 mod q_i, Shoup}
+    ulonglong2 *d_p_inv = nullptr;
+
+    // chain helpers
+    size_t chain_index_of(size_t limbs) const { return L + 1 - limbs; }
+    size_t limbs_of(size_t chain_index) const
+    {
+        require(chain_index >= 1 && chain_index <= L, "chain_index out of range", kOutOfRange);
+        return L + 1 - chain_index;
+    }
+    int total_bits(size_t limbs) const;  // bit count of q_0 ... q_{limbs-1}
+    const ulonglong2 *rescale_inv(size_t limbs) const { return d_rescale_inv + limbs * (L + 1); }
+
+    // Galois permutation tables (NTT domain), built on first use
+    const uint32_t *galois_table(uint32_t elt);
+    uint32_t elt_from_step(int step) const;
+
+    // host copies of SEAL-ordered root tables (encoder uses complex roots, not these)
+    std::vector<std::vector<uint64_t>> fwd_host, inv_host;
+
+private:
+    std::map<uint32_t, uint32_t *> galois_;
+    std::mutex mu_;
+};
+
+// ---------------------------------------------------------------- operations
+// NTT of a [polys][limbs][n] buffer whose limb i uses prime i.
+void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
+void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
+
+void ct_resize(Ciphertext &ct, Context *c, size_t size, size_t limbs);
+void ct_copy(Ciphertext &dst, const Ciphertext &src);
+void pt_copy(Plaintext &dst, const Plaintext &src);
+
+void op_add(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst);
+void op_sub(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst);
+void op_negate(const Ciphertext &a, Ciphertext &dst);
+void op_add_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst);
+void op_sub_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst);
+void op_multiply_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst);
+void op_multiply(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst);
+void op_square(const Ciphertext &a, Ciphertext &dst);
+void op_relinearize(const Ciphertext &a, const KSwitchKeys &rk, Ciphertext &dst);
+void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst);
+void op_mod_switch_to_next(const Ciphertext &a, Ciphertext &dst);
+void op_mod_switch_to(const Ciphertext &a, size_t chain_index, Ciphertext &dst);
+void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &dst);
+void op_apply_galois(const Ciphertext &a, uint32_t elt, const KSwitchKeys &gk, Ciphertext &dst);
+void op_rotate(const Ciphertext &a, int steps, const KSwitchKeys &gk, Ciphertext &dst);
+void op_conjugate(const Ciphertext &a, const KSwitchKeys &gk, Ciphertext &dst);
+void op_to_ntt(const Ciphertext &a, Ciphertext &dst);
+void op_from_ntt(const Ciphertext &a, Ciphertext &dst);
+void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst);
+void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &dst);
+
+// key switch of `target` (limbs x n, NTT) under key `key` added into ct (2 polys)
+void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key);
+
+// encoder (host FFT + device NTT) -- encoder.cpp
+void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,
+                   Plaintext &out);
+void encode_constant(Context &c, double value, size_t limbs, double scale, Plaintext &out);
+void decode(Context &c, const Plaintext &pt, double *re, double *im, size_t count);
+
+// keys -- keys.cpp
+void secret_key_generate(Context &c, uint64_t seed, SecretKey &sk);
+void secret_key_from_coeffs(Context &c, const int8_t *coeff, SecretKey &sk);
+void relin_key_generate(Context &c, const SecretKey &sk, uint64_t seed, KSwitchKeys &out);
+void galois_keys_generate(Context &c, const SecretKey &sk, const uint32_t *elts, size_t count, uint64_t seed,
+                          KSwitchKeys &out);
+void encrypt_symmetric(Context &c, const SecretKey &sk, const Plaintext &pt, uint64_t seed, Ciphertext &out);
+void decrypt(Context &c, const SecretKey &sk, const Ciphertext &ct, Plaintext &out);
+
+inline bool are_close(double a, double b)
+{
+    double f = std::max(std::max(std::fabs(a), std::fabs(b)), 1.0);
+    return std::fabs(a - b) < 2.220446049250313e-16 * f;
+}
+
+}  // namespace nx

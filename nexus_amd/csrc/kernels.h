@@ -1,0 +1,112 @@
+// Launchers for the gfx950 kernels.  Every launcher is stream-ordered on the
+// stream it is given and performs no host synchronisation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.h"
+
+namespace nx {
+
+constexpr int kMaxPrimes = 64;
+
+// Device-resident per-prime tables of one context.
+struct DeviceTables {
+    int logn;
+    int num_primes;
+    const ulonglong2 *fwd;   // [prime][n] {psi power, Shoup quotient}, SEAL bit-reversed order
+    const ulonglong2 *inv;   // [prime][n] {inverse psi power, Shoup quotient}, SEAL scrambled order
+    const ulonglong2 *ninv;  // [prime] {n^-1 mod q, Shoup quotient}
+    const ModConst *mod;     // [prime]
+};
+
+// A batch of in-place NTTs over polys x limbs RNS limbs of n words each.
+// Limb `i` of poly `p` lives at data + p * poly_stride + i * n.  Its prime is
+// prime_of[i] (prime_by_poly = 0) or prime_of[p] (prime_by_poly = 1).
+struct NttBatch {
+    uint64_t *data;
+    uint64_t poly_stride;
+    uint32_t polys;
+    uint32_t limbs;
+    uint32_t prime_by_poly;
+    uint8_t prime_of[kMaxPrimes];
+};
+
+// Forward negacyclic NTT: natural order in, SEAL bit-reversed order out, [0, q).
+void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s);
+// Inverse: bit-reversed in, natural order out, multiplied by n^-1, [0, q).
+void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s);
+
+// ---------------------------------------------------------------- dyadic ops
+// All pointers address [polys][limbs][n] arrays; limb i uses prime i.
+void poly_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t polys,
+              uint32_t limbs, hipStream_t s);
+void poly_sub(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t polys,
+              uint32_t limbs, hipStream_t s);
+void poly_negate(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t polys, uint32_t limbs,
+                 hipStream_t s);
+// out[p][i] = a[p][i] * pt[i]
+void poly_mul_plain(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *pt, uint32_t polys,
+                    uint32_t limbs, hipStream_t s);
+// (a0,a1) x (b0,b1) -> (a0b0, a0b1+a1b0, a1b1); out may alias neither input.
+void poly_tensor(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t limbs,
+                 hipStream_t s);
+// out[p][i] = a[p][i] * w[i] with per-limb Shoup constants {w, w'} (host array, passed by value)
+void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,
+                     uint32_t polys, uint32_t limbs, hipStream_t s);
+// out[p][i][c] = a[p][i][perm[c]]
+void poly_permute(uint64_t *out, const uint64_t *a, const uint32_t *perm, uint32_t polys, uint32_t limbs, int logn,
+                  hipStream_t s);
+// Fill limb i of each poly with host_value[i] (host array, passed by value).
+void poly_fill_limbs(uint64_t *out, const uint64_t *host_value, uint32_t polys, uint32_t limbs, int logn,
+                     hipStream_t s);
+// out[i][c] = a[i][c] + v[i] on one poly (host array of residues, passed by value)
+void poly_add_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *host_v,
+                     uint32_t limbs, hipStream_t s);
+
+// ---------------------------------------------------------------- RNS ops
+// Rescale / ModDown tail, SEAL-exact rounding.
+//  last: coefficient-domain limb (already INTT'd) of prime `last_prime`, one per poly
+//  out[p][i] = (x[p][i] - NTT_i(centered(last[p]) mod q_i)) * inv[i]   for i < limbs
+// Step 1 writes the centered residues mod q_i into tmp[p][i]; the caller NTTs tmp;
+// step 2 combines.  `acc_add` (may be null) receives out instead: acc[p][i] += result.
+void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last, uint64_t last_stride,
+                     uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s);
+void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
+                 const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
+                 hipStream_t s);
+
+// Key-switch ModUp: for every output prime I in out_primes (count = nout) and
+// digit j < l, T[I][j] = (coef[j] mod q_I) when I != j, or ntt_src[j] when the
+// output prime is the digit's own prime.  coef = INTT(c2) limbs, [l][n].
+void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const uint64_t *ntt_src, uint32_t l,
+                     const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+// Inner product: acc[k][I][c] = sum_j T[I][j][c] * key[j][k][kp(I)][c] mod q_kp(I)
+// key layout [digits][2][key_limbs][n].
+void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t l,
+                      uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+
+// ---------------------------------------------------------------- keys
+// Deterministic counter-based sampling (not SEAL's PRNG).
+void sample_uniform(const DeviceTables &t, uint64_t *out, uint32_t polys, uint32_t limbs, const uint8_t *primes_host,
+                    uint64_t seed, hipStream_t s);
+// Centered binomial (eta = 21) small error, written as residues in all limbs.
+void sample_cbd(const DeviceTables &t, uint64_t *out, uint32_t limbs, const uint8_t *primes_host, uint64_t seed,
+                hipStream_t s);
+// Signed small coefficients (int8) -> residues for each limb.
+void small_to_rns(const DeviceTables &t, uint64_t *out, const int8_t *coef, uint32_t limbs,
+                  const uint8_t *primes_host, hipStream_t s);
+// Signed 64-bit coefficients -> residues.
+void i64_to_rns(const DeviceTables &t, uint64_t *out, const int64_t *coef, uint32_t limbs,
+                const uint8_t *primes_host, hipStream_t s);
+// out = a * b + c  per limb (c may be null), all NTT domain, limb i uses primes[i]
+void poly_mul_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, const uint64_t *c,
+                  uint32_t limbs, const uint8_t *primes_host, int negate_product, hipStream_t s);
+// out[c] += a[c] * w (mod q_prime) for one limb; w = {value, Shoup quotient}
+void limb_add_scaled(const DeviceTables &t, uint64_t *out, const uint64_t *a, ulonglong2 w, uint32_t prime,
+                     hipStream_t s);
+// Negacyclic shift of coefficient-domain limbs: out = a * x^k mod (x^n + 1), 0 <= k < 2n.
+void poly_negacyclic_shift(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t k, uint32_t polys,
+                           uint32_t limbs, hipStream_t s);
+
+}  // namespace nx

@@ -1,0 +1,346 @@
+// Negacyclic NTT / INTT over RNS limbs for gfx950.
+//
+// Transform definition (SEAL, util/ntt.cpp:241-477 and util/dwthandler.h):
+//  forward: Cooley-Tukey stages m = 1, 2, ..., n/2 with gap n/(2m); butterfly
+//           block i of stage m uses fwd[m + i]; natural order in, bit-reversed out.
+//  inverse: Gentleman-Sande stages m = n/2, ..., 1 with gap n/(2m); block i of
+//           stage m uses inv[n - 2m + 1 + i]; followed by multiplication by n^-1.
+// Outputs are fully reduced, so they are bit-identical to SEAL's however the
+// lazy reductions inside are scheduled.
+//
+// Mapping.  n <= 2^13: one workgroup holds a whole limb in LDS.  n >= 2^14: the
+// limb is viewed as an N1 x N2 matrix (index = t * N2 + c).  The first log2(N1)
+// forward stages only pair elements of the same column c (gap >= N2), the last
+// log2(N2) stages only pair elements of the same row t; the inverse runs rows
+// first, then columns.  Each pass stages a tile of 16 columns (or 8 contiguous
+// rows) through LDS and runs radix-8/4/2 register butterflies between
+// barriers, 256 threads (four wave64) per workgroup.  Row twiddles for row r at
+// local stage s are the contiguous block fwd[2^s (N1 + r) ...].
+#include "kernels.h"
+
+namespace nx {
+
+namespace {
+
+enum Role { kWhole = 0, kCol = 1, kRow = 2 };
+
+struct NttKArgs {
+    uint64_t *data;
+    uint64_t poly_stride;
+    uint32_t limbs;
+    uint32_t logn;
+    uint32_t prime_by_poly;
+    const ulonglong2 *tw;
+    const ModConst *mod;
+    const ulonglong2 *ninv;
+    uint8_t prime_of[kMaxPrimes];
+};
+
+constexpr int kThreads = 256;
+
+template <int LOGS, int ROLE>
+struct Geometry {
+    static constexpr int S = 1 << LOGS;
+    static constexpr int C = ROLE == kCol ? 16 : (ROLE == kRow ? (S >= 2048 ? 1 : 2048 / S) : 1);
+    static constexpr int TILE = S * C;
+};
+
+// LDS address of local position `pos` of sub-transform `cc`.
+template <int ROLE, int S, int C>
+__device__ __forceinline__ int lds_addr(int cc, int pos)
+{
+    return ROLE == kCol ? pos * C + cc : cc * S + pos;
+}
+
+// Forward radix-2^K step over local stages s .. s+K-1.
+template <int LOGS, int ROLE, int K>
+__device__ __forceinline__ void fwd_step(uint64_t *lds, int s, const ulonglong2 *tw, uint64_t q, uint32_t m0,
+                                         uint32_t row0)
+{
+    using G = Geometry<LOGS, ROLE>;
+    constexpr int R = 1 << K;
+    constexpr int GROUPS_PER = G::S / R;
+    constexpr int GROUPS = G::TILE / R;
+    const uint64_t two_q = 2 * q;
+    const int B = G::S >> s;
+    const int D = B >> K;
+    for (int g = threadIdx.x; g < GROUPS; g += kThreads) {
+        int cc, gg;
+        if (ROLE == kCol) {
+            cc = g % G::C;
+            gg = g / G::C;
+        } else {
+            cc = g / GROUPS_PER;
+            gg = g % GROUPS_PER;
+        }
+        const int blockid = gg / D;
+        const int o = gg - blockid * D;
+        const uint32_t mrow = (ROLE == kRow) ? (m0 + row0 + cc) : m0;
+        uint64_t v[R];
+        int addr[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            addr[t] = lds_addr<ROLE, G::S, G::C>(cc, blockid * B + o + t * D);
+            v[t] = lds[addr[t]];
+        }
+#pragma unroll
+        for (int d = 0; d < K; d++) {
+            const int h = 1 << (K - 1 - d);
+#pragma unroll
+            for (int j = 0; j < (1 << d); j++) {
+                const uint32_t b = (uint32_t)blockid * (1u << d) + j;
+                const ulonglong2 w = tw[((uint32_t)1 << (s + d)) * mrow + b];
+#pragma unroll
+                for (int u = 0; u < h; u++) {
+                    const int t = j * 2 * h + u;
+                    uint64_t x = v[t];
+                    x = x >= two_q ? x - two_q : x;
+                    const uint64_t y = shoup_lazy(v[t + h], w.x, w.y, q);
+                    v[t] = x + y;
+                    v[t + h] = x - y + two_q;
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < R; t++) lds[addr[t]] = v[t];
+    }
+}
+
+// Inverse radix-2^K step over local stages s .. s+K-1 (gaps 2^s .. 2^(s+K-1)).
+template <int LOGS, int ROLE, int K>
+__device__ __forceinline__ void inv_step(uint64_t *lds, int s, const ulonglong2 *tw, uint64_t q, uint32_t n,
+                                         uint32_t mx, uint32_t row0)
+{
+    using G = Geometry<LOGS, ROLE>;
+    constexpr int R = 1 << K;
+    constexpr int GROUPS_PER = G::S / R;
+    constexpr int GROUPS = G::TILE / R;
+    const uint64_t two_q = 2 * q;
+    const int so = 1 << s;
+    for (int g = threadIdx.x; g < GROUPS; g += kThreads) {
+        int cc, gg;
+        if (ROLE == kCol) {
+            cc = g % G::C;
+            gg = g / G::C;
+        } else {
+            cc = g / GROUPS_PER;
+            gg = g % GROUPS_PER;
+        }
+        const int o = gg & (so - 1);
+        const int blockid = gg >> s;
+        const int base = blockid * (so << K) + o;
+        const uint32_t r = row0 + cc;
+        uint64_t v[R];
+        int addr[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            addr[t] = lds_addr<ROLE, G::S, G::C>(cc, base + t * so);
+            v[t] = lds[addr[t]];
+        }
+#pragma unroll
+        for (int d = 0; d < K; d++) {
+            const int sd = s + d;
+            const uint32_t first = n - (mx >> sd) + 1 + (ROLE == kRow ? r * ((uint32_t)G::S >> (sd + 1)) : 0u);
+#pragma unroll
+            for (int j = 0; j < (R >> (d + 1)); j++) {
+                const uint32_t b = (uint32_t)blockid * (uint32_t)(R >> (d + 1)) + j;
+                const ulonglong2 w = tw[first + b];
+#pragma unroll
+                for (int u = 0; u < (1 << d); u++) {
+                    const int t = j * (2 << d) + u;
+                    const uint64_t x = v[t], y = v[t + (1 << d)];
+                    uint64_t sum = x + y;
+                    v[t] = sum >= two_q ? sum - two_q : sum;
+                    v[t + (1 << d)] = shoup_lazy(x - y + two_q, w.x, w.y, q);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < R; t++) lds[addr[t]] = v[t];
+    }
+}
+
+template <int LOGS, int ROLE, bool FWD>
+__global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
+{
+    using G = Geometry<LOGS, ROLE>;
+    __shared__ uint64_t lds[G::TILE];
+    const uint32_t bidx = blockIdx.y;
+    const uint32_t poly = bidx / a.limbs, limb = bidx - poly * a.limbs;
+    const uint32_t prime = a.prime_by_poly ? a.prime_of[poly] : a.prime_of[limb];
+    const uint32_t n = 1u << a.logn;
+    uint64_t *data = a.data + poly * a.poly_stride + (uint64_t)limb * n;
+    const ulonglong2 *tw = a.tw + (uint64_t)prime * n;
+    const uint64_t q = a.mod[prime].q;
+    const uint32_t N2 = ROLE == kCol ? (n >> LOGS) : (ROLE == kRow ? (uint32_t)G::S : n);
+    const uint32_t N1 = n / N2;
+    const uint32_t first = blockIdx.x * G::C;  // first column / row of the tile
+
+    // ---- load
+    for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
+        uint64_t gidx;
+        if (ROLE == kCol) {
+            const int k = e / G::C, cc = e % G::C;
+            gidx = (uint64_t)k * N2 + first + cc;
+        } else if (ROLE == kRow) {
+            gidx = (uint64_t)first * G::S + e;
+        } else {
+            gidx = e;
+        }
+        lds[e] = data[gidx];
+    }
+    __syncthreads();
+
+    if (FWD) {
+        const uint32_t m0 = ROLE == kRow ? N1 : 1u;
+        int s = 0;
+        if constexpr (LOGS >= 3) {
+#pragma unroll
+            for (int st = 0; st < LOGS / 3; st++) {
+                fwd_step<LOGS, ROLE, 3>(lds, s, tw, q, m0, first);
+                s += 3;
+                __syncthreads();
+            }
+        }
+        if constexpr (LOGS % 3 == 2) {
+            fwd_step<LOGS, ROLE, 2>(lds, s, tw, q, m0, first);
+            __syncthreads();
+        } else if constexpr (LOGS % 3 == 1) {
+            fwd_step<LOGS, ROLE, 1>(lds, s, tw, q, m0, first);
+            __syncthreads();
+        }
+    } else {
+        const uint32_t mx = ROLE == kCol ? N1 : n;
+        int s = 0;
+        if constexpr (LOGS >= 3) {
+#pragma unroll
+            for (int st = 0; st < LOGS / 3; st++) {
+                inv_step<LOGS, ROLE, 3>(lds, s, tw, q, n, mx, first);
+                s += 3;
+                __syncthreads();
+            }
+        }
+        if constexpr (LOGS % 3 == 2) {
+            inv_step<LOGS, ROLE, 2>(lds, s, tw, q, n, mx, first);
+            __syncthreads();
+        } else if constexpr (LOGS % 3 == 1) {
+            inv_step<LOGS, ROLE, 1>(lds, s, tw, q, n, mx, first);
+            __syncthreads();
+        }
+    }
+
+    // ---- store (final reduction on the last pass)
+    const bool last_pass = FWD ? (ROLE != kCol) : (ROLE != kRow);
+    const ulonglong2 ni = a.ninv[prime];
+    for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
+        uint64_t gidx;
+        if (ROLE == kCol) {
+            const int k = e / G::C, cc = e % G::C;
+            gidx = (uint64_t)k * N2 + first + cc;
+        } else if (ROLE == kRow) {
+            gidx = (uint64_t)first * G::S + e;
+        } else {
+            gidx = e;
+        }
+        uint64_t v = lds[e];
+        if (last_pass) {
+            if (FWD) {
+                v = v >= 2 * q ? v - 2 * q : v;
+                v = v >= q ? v - q : v;
+            } else {
+                v = shoup(v, ni.x, ni.y, q);
+            }
+        }
+        data[gidx] = v;
+    }
+}
+
+template <int LOGS, int ROLE, bool FWD>
+void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
+{
+    using G = Geometry<LOGS, ROLE>;
+    const uint32_t n = 1u << a.logn;
+    const uint32_t tiles = ROLE == kWhole ? 1u : (n / G::TILE);
+    dim3 grid(tiles, batch);
+    hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD>), grid, dim3(kThreads), 0, s, a);
+}
+
+template <bool FWD>
+void launch_whole(const NttKArgs &a, uint32_t batch, hipStream_t s)
+{
+    switch (a.logn) {
+#define NX_CASE(L) \
+    case L: launch_one<L, kWhole, FWD>(a, batch, s); break;
+        NX_CASE(1) NX_CASE(2) NX_CASE(3) NX_CASE(4) NX_CASE(5) NX_CASE(6) NX_CASE(7)
+        NX_CASE(8) NX_CASE(9) NX_CASE(10) NX_CASE(11) NX_CASE(12) NX_CASE(13)
+#undef NX_CASE
+    default: break;
+    }
+}
+
+template <int ROLE, bool FWD>
+void launch_pass(int logs, const NttKArgs &a, uint32_t batch, hipStream_t s)
+{
+    switch (logs) {
+    case 7: launch_one<7, ROLE, FWD>(a, batch, s); break;
+    case 8: launch_one<8, ROLE, FWD>(a, batch, s); break;
+    case 9: launch_one<9, ROLE, FWD>(a, batch, s); break;
+    default: break;
+    }
+}
+
+NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
+{
+    NttKArgs a;
+    a.data = b.data;
+    a.poly_stride = b.poly_stride;
+    a.limbs = b.limbs;
+    a.logn = (uint32_t)t.logn;
+    a.prime_by_poly = b.prime_by_poly;
+    a.tw = fwd ? t.fwd : t.inv;
+    a.mod = t.mod;
+    a.ninv = t.ninv;
+    for (int i = 0; i < kMaxPrimes; i++) a.prime_of[i] = b.prime_of[i];
+    return a;
+}
+
+}  // namespace
+
+// Split of log2(n) into column (N1) and row (N2) passes for n >= 2^14.
+static inline void split_logn(int logn, int &l1, int &l2)
+{
+    l1 = logn / 2;
+    l2 = logn - l1;
+}
+
+void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
+{
+    const uint32_t batch = b.polys * b.limbs;
+    if (!batch) return;
+    NttKArgs a = make_args(t, b, true);
+    if (t.logn <= 13) {
+        launch_whole<true>(a, batch, s);
+    } else {
+        int l1, l2;
+        split_logn(t.logn, l1, l2);
+        launch_pass<kCol, true>(l1, a, batch, s);
+        launch_pass<kRow, true>(l2, a, batch, s);
+    }
+}
+
+void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s)
+{
+    const uint32_t batch = b.polys * b.limbs;
+    if (!batch) return;
+    NttKArgs a = make_args(t, b, false);
+    if (t.logn <= 13) {
+        launch_whole<false>(a, batch, s);
+    } else {
+        int l1, l2;
+        split_logn(t.logn, l1, l2);
+        launch_pass<kRow, false>(l2, a, batch, s);
+        launch_pass<kCol, false>(l1, a, batch, s);
+    }
+}
+
+}  // namespace nx

@@ -1,0 +1,465 @@
+// Dyadic (coefficient-wise) RNS kernels: HBM-bound integer work.  Grid y walks
+// (poly, limb) pairs so the modulus is uniform per workgroup; grid x covers the
+// n coefficients of a limb, one coefficient per lane.
+#include "kernels.h"
+
+namespace nx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct PrimeMap {
+    uint8_t p[kMaxPrimes];
+};
+
+struct LimbWords {
+    uint64_t v[kMaxPrimes];
+};
+
+struct LimbShoup {
+    ulonglong2 w[kMaxPrimes];
+};
+
+inline dim3 grid_for(int logn, uint32_t rows)
+{
+    uint32_t n = 1u << logn;
+    return dim3((n + kBlock - 1) / kBlock, rows);
+}
+
+__global__ void k_add(uint64_t *out, const uint64_t *a, const uint64_t *b, const ModConst *mod, uint32_t limbs,
+                      uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t q = mod[limb].q;
+    const uint64_t idx = (uint64_t)row * n + c;
+    out[idx] = add_mod(a[idx], b[idx], q);
+}
+
+__global__ void k_sub(uint64_t *out, const uint64_t *a, const uint64_t *b, const ModConst *mod, uint32_t limbs,
+                      uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t q = mod[limb].q;
+    const uint64_t idx = (uint64_t)row * n + c;
+    out[idx] = sub_mod(a[idx], b[idx], q);
+}
+
+__global__ void k_negate(uint64_t *out, const uint64_t *a, const ModConst *mod, uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t q = mod[limb].q;
+    const uint64_t idx = (uint64_t)row * n + c;
+    out[idx] = neg_mod(a[idx], q);
+}
+
+__global__ void k_mul_plain(uint64_t *out, const uint64_t *a, const uint64_t *pt, const ModConst *mod,
+                            uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const ModConst m = mod[limb];
+    const uint64_t idx = (uint64_t)row * n + c;
+    out[idx] = mul_mod(a[idx], pt[(uint64_t)limb * n + c], m);
+}
+
+__global__ void k_tensor(uint64_t *out, const uint64_t *a, const uint64_t *b, const ModConst *mod, uint32_t limbs,
+                         uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const ModConst m = mod[limb];
+    const uint64_t ps = (uint64_t)limbs * n;
+    const uint64_t idx = (uint64_t)limb * n + c;
+    const uint64_t a0 = a[idx], a1 = a[ps + idx], b0 = b[idx], b1 = b[ps + idx];
+    out[idx] = mul_mod(a0, b0, m);
+    // a0*b1 + a1*b0 as one 129-bit sum reduced once
+    uint64_t lo1 = a0 * b1, hi1 = mulhi64(a0, b1);
+    uint64_t lo2 = a1 * b0, hi2 = mulhi64(a1, b0);
+    uint64_t lo = lo1 + lo2;
+    uint64_t hi = hi1 + hi2 + (lo < lo1);
+    out[ps + idx] = barrett128(hi, lo, m);
+    out[2 * ps + idx] = mul_mod(a1, b1, m);
+}
+
+__global__ void k_mul_scalar(uint64_t *out, const uint64_t *a, LimbShoup scal, const ModConst *mod, uint32_t limbs,
+                             uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t q = mod[limb].q;
+    const ulonglong2 w = scal.w[limb];
+    const uint64_t idx = (uint64_t)row * n + c;
+    out[idx] = shoup(a[idx], w.x, w.y, q);
+}
+
+__global__ void k_permute(uint64_t *out, const uint64_t *a, const uint32_t *perm, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint64_t row = blockIdx.y;
+    out[row * n + c] = a[row * n + perm[c]];
+}
+
+__global__ void k_fill_limbs(uint64_t *out, LimbWords value, uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    out[(uint64_t)row * n + c] = value.v[limb];
+}
+
+__global__ void k_add_scalar(uint64_t *out, const uint64_t *a, LimbWords v, const ModConst *mod, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const uint64_t idx = (uint64_t)limb * n + c;
+    out[idx] = add_mod(a[idx], v.v[limb], mod[limb].q);
+}
+
+// ---- rescale / ModDown
+__global__ void k_center_last(uint64_t *tmp, const uint64_t *last, uint64_t last_stride, const ModConst *mod,
+                              uint32_t last_prime, uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;
+    const uint32_t poly = row / limbs, limb = row - poly * limbs;
+    const uint64_t p = mod[last_prime].q;
+    const uint64_t half = p >> 1;
+    uint64_t v = last[poly * last_stride + c] + half;
+    v = v >= p ? v - p : v;  // [0, p): (x + (p-1)/2) mod p
+    const ModConst mi = mod[limb];
+    const uint64_t r = sub_mod(barrett64(v, mi), barrett64(half, mi), mi.q);
+    tmp[(uint64_t)row * n + c] = r;
+}
+
+__global__ void k_sub_mul(uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
+                          const uint64_t *tmp, const ulonglong2 *inv, const ModConst *mod, uint32_t limbs, uint32_t n,
+                          int accumulate)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;
+    const uint32_t poly = row / limbs, limb = row - poly * limbs;
+    const uint64_t q = mod[limb].q;
+    const ulonglong2 w = inv[limb];
+    const uint64_t xv = x[poly * x_stride + (uint64_t)limb * n + c];
+    const uint64_t d = sub_mod(xv, tmp[(uint64_t)row * n + c], q);
+    uint64_t r = shoup(d, w.x, w.y, q);
+    uint64_t *o = out + poly * out_stride + (uint64_t)limb * n + c;
+    if (accumulate) r = add_mod(*o, r, q);
+    *o = r;
+}
+
+// ---- key switching
+__global__ void k_modup_expand(uint64_t *T, const uint64_t *coef, const ModConst *mod, PrimeMap outp, uint32_t l,
+                               uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // I * l + j
+    const uint32_t I = row / l, j = row - I * l;
+    const ModConst m = mod[outp.p[I]];
+    const uint64_t v = coef[(uint64_t)j * n + c];
+    T[(uint64_t)row * n + c] = v >= m.q ? barrett64(v, m) : v;
+}
+
+__global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key, const ModConst *mod, PrimeMap outp,
+                           uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t I = blockIdx.y;
+    const uint32_t kp = outp.p[I];
+    const ModConst m = mod[kp];
+    uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+    const uint64_t *t = T + (uint64_t)I * l * n + c;
+    const uint64_t *k0 = key + (uint64_t)kp * n + c;
+    const uint64_t kstride = (uint64_t)key_limbs * n;  // between poly 0 and 1 of a digit
+    for (uint32_t j = 0; j < l; j++) {
+        const uint64_t tv = t[(uint64_t)j * n];
+        const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
+        const uint64_t a = kj[0], b = kj[kstride];
+        uint64_t plo = tv * a, phi = mulhi64(tv, a);
+        lo0 += plo;
+        hi0 += phi + (lo0 < plo);
+        plo = tv * b;
+        phi = mulhi64(tv, b);
+        lo1 += plo;
+        hi1 += phi + (lo1 < plo);
+    }
+    acc[(uint64_t)I * n + c] = barrett128(hi0, lo0, m);
+    acc[((uint64_t)nout + I) * n + c] = barrett128(hi1, lo1, m);
+}
+
+// ---- sampling (counter-based; deterministic given the seed)
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ uint64_t rand64(uint64_t seed, uint64_t ctr)
+{
+    return mix64(seed ^ mix64(ctr + 0x9e3779b97f4a7c15ull));
+}
+
+__global__ void k_sample_uniform(uint64_t *out, const ModConst *mod, PrimeMap pm, uint32_t limbs, uint32_t n,
+                                 uint64_t seed)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t q = mod[pm.p[limb]].q;
+    const uint64_t idx = (uint64_t)row * n + c;
+    out[idx] = mulhi64(rand64(seed, idx), q);
+}
+
+__global__ void k_sample_cbd(uint64_t *out, const ModConst *mod, PrimeMap pm, uint32_t limbs, uint32_t n,
+                             uint64_t seed)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint64_t r = rand64(seed, c);
+    const int e = __popcll(r & 0x1fffffull) - __popcll((r >> 21) & 0x1fffffull);
+    for (uint32_t i = 0; i < limbs; i++) {
+        const uint64_t q = mod[pm.p[i]].q;
+        out[(uint64_t)i * n + c] = e >= 0 ? (uint64_t)e : q - (uint64_t)(-e);
+    }
+}
+
+__global__ void k_small_to_rns(uint64_t *out, const int8_t *coef, const ModConst *mod, PrimeMap pm, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const uint64_t q = mod[pm.p[limb]].q;
+    const int v = coef[c];
+    out[(uint64_t)limb * n + c] = v >= 0 ? (uint64_t)v : q - (uint64_t)(-v);
+}
+
+__global__ void k_i64_to_rns(uint64_t *out, const int64_t *coef, const ModConst *mod, PrimeMap pm, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const ModConst m = mod[pm.p[limb]];
+    const int64_t v = coef[c];
+    const uint64_t a = barrett64(v >= 0 ? (uint64_t)v : (uint64_t)(-v), m);
+    out[(uint64_t)limb * n + c] = v >= 0 ? a : neg_mod(a, m.q);
+}
+
+__global__ void k_mul_add(uint64_t *out, const uint64_t *a, const uint64_t *b, const uint64_t *cc,
+                          const ModConst *mod, PrimeMap pm, uint32_t n, int negate_product)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const ModConst m = mod[pm.p[limb]];
+    const uint64_t idx = (uint64_t)limb * n + c;
+    uint64_t r = mul_mod(a[idx], b[idx], m);
+    if (negate_product) r = neg_mod(r, m.q);
+    if (cc) r = add_mod(r, cc[idx], m.q);
+    out[idx] = r;
+}
+
+PrimeMap make_map(const uint8_t *p, uint32_t count)
+{
+    PrimeMap m;
+    for (uint32_t i = 0; i < kMaxPrimes; i++) m.p[i] = i < count ? p[i] : 0;
+    return m;
+}
+
+}  // namespace
+
+void poly_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t polys,
+              uint32_t limbs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_add, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, b, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void poly_sub(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t polys,
+              uint32_t limbs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, b, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void poly_negate(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t polys, uint32_t limbs,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_negate, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void poly_mul_plain(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *pt, uint32_t polys,
+                    uint32_t limbs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mul_plain, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, pt, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void poly_tensor(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t limbs,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tensor, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, a, b, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,
+                     uint32_t polys, uint32_t limbs, hipStream_t s)
+{
+    LimbShoup w;
+    for (uint32_t i = 0; i < kMaxPrimes; i++) w.w[i] = i < limbs ? host_w[i] : make_ulonglong2(0, 0);
+    hipLaunchKernelGGL(k_mul_scalar, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, w, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void poly_permute(uint64_t *out, const uint64_t *a, const uint32_t *perm, uint32_t polys, uint32_t limbs, int logn,
+                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_permute, grid_for(logn, polys * limbs), dim3(kBlock), 0, s, out, a, perm, 1u << logn);
+}
+
+void poly_fill_limbs(uint64_t *out, const uint64_t *host_value, uint32_t polys, uint32_t limbs, int logn,
+                     hipStream_t s)
+{
+    LimbWords v;
+    for (uint32_t i = 0; i < kMaxPrimes; i++) v.v[i] = i < limbs ? host_value[i] : 0;
+    hipLaunchKernelGGL(k_fill_limbs, grid_for(logn, polys * limbs), dim3(kBlock), 0, s, out, v, limbs, 1u << logn);
+}
+
+void poly_add_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *host_v,
+                     uint32_t limbs, hipStream_t s)
+{
+    LimbWords v;
+    for (uint32_t i = 0; i < kMaxPrimes; i++) v.v[i] = i < limbs ? host_v[i] : 0;
+    hipLaunchKernelGGL(k_add_scalar, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, a, v, t.mod, 1u << t.logn);
+}
+
+void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last, uint64_t last_stride,
+                     uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_center_last, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, tmp, last, last_stride,
+                       t.mod, last_prime, limbs, 1u << t.logn);
+}
+
+void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
+                 const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
+                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub_mul, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, out_stride, x, x_stride,
+                       tmp, inv, t.mod, limbs, 1u << t.logn, accumulate);
+}
+
+void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const uint64_t *ntt_src, uint32_t l,
+                     const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+{
+    (void)ntt_src;
+    hipLaunchKernelGGL(k_modup_expand, grid_for(t.logn, nout * l), dim3(kBlock), 0, s, T, coef, t.mod,
+                       make_map(out_primes_host, nout), l, 1u << t.logn);
+}
+
+void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t l,
+                      uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, nout), dim3(kBlock), 0, s, acc, T, key, t.mod,
+                       make_map(out_primes_host, nout), l, key_limbs, nout, 1u << t.logn);
+}
+
+void sample_uniform(const DeviceTables &t, uint64_t *out, uint32_t polys, uint32_t limbs, const uint8_t *primes_host,
+                    uint64_t seed, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sample_uniform, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, t.mod,
+                       make_map(primes_host, limbs), limbs, 1u << t.logn, seed);
+}
+
+void sample_cbd(const DeviceTables &t, uint64_t *out, uint32_t limbs, const uint8_t *primes_host, uint64_t seed,
+                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sample_cbd, grid_for(t.logn, 1), dim3(kBlock), 0, s, out, t.mod,
+                       make_map(primes_host, limbs), limbs, 1u << t.logn, seed);
+}
+
+void small_to_rns(const DeviceTables &t, uint64_t *out, const int8_t *coef, uint32_t limbs,
+                  const uint8_t *primes_host, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_small_to_rns, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, coef, t.mod,
+                       make_map(primes_host, limbs), 1u << t.logn);
+}
+
+void i64_to_rns(const DeviceTables &t, uint64_t *out, const int64_t *coef, uint32_t limbs,
+                const uint8_t *primes_host, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_i64_to_rns, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, coef, t.mod,
+                       make_map(primes_host, limbs), 1u << t.logn);
+}
+
+void poly_mul_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, const uint64_t *c,
+                  uint32_t limbs, const uint8_t *primes_host, int negate_product, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mul_add, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, a, b, c, t.mod,
+                       make_map(primes_host, limbs), 1u << t.logn, negate_product);
+}
+
+}  // namespace nx
+
+namespace nx {
+
+namespace {
+
+__global__ void k_limb_add_scaled(uint64_t *out, const uint64_t *a, ulonglong2 w, const ModConst *mod, uint32_t prime,
+                                  uint32_t n)
+{
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const uint64_t q = mod[prime].q;
+    out[c] = add_mod(out[c], shoup(a[c], w.x, w.y, q), q);
+}
+
+// out[(i + k) mod n] = +-a[i]: coefficient i moves to i + k, wrapping with a
+// sign flip each time it passes x^n (util/polyarithsmallmod.h negacyclic_shift_poly_coeffmod).
+__global__ void k_negacyclic_shift(uint64_t *out, const uint64_t *a, const ModConst *mod, uint32_t k, uint32_t limbs,
+                                   uint32_t n)
+{
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t q = mod[limb].q;
+    const uint64_t v = a[(uint64_t)row * n + i];
+    const uint32_t pos = (i + k) & (2 * n - 1);
+    const bool flip = pos >= n;
+    out[(uint64_t)row * n + (pos & (n - 1))] = flip ? neg_mod(v, q) : v;
+}
+
+}  // namespace
+
+void limb_add_scaled(const DeviceTables &t, uint64_t *out, const uint64_t *a, ulonglong2 w, uint32_t prime,
+                     hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    hipLaunchKernelGGL(k_limb_add_scaled, dim3((n + 255) / 256), dim3(256), 0, s, out, a, w, t.mod, prime, n);
+}
+
+void poly_negacyclic_shift(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t k, uint32_t polys,
+                           uint32_t limbs, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    hipLaunchKernelGGL(k_negacyclic_shift, dim3((n + 255) / 256, polys * limbs), dim3(256), 0, s, out, a, t.mod, k,
+                       limbs, n);
+}
+
+}  // namespace nx

@@ -1,0 +1,517 @@
+// Ciphertext-level CKKS operations: SEAL-4.1-bs semantics (scale bookkeeping,
+// level rules, error messages) over the gfx950 kernels.  Every result is built
+// in a fresh buffer from the context's cache and then moved into the
+// destination, so any aliasing of destination and inputs is safe.
+#include <algorithm>
+#include <cmath>
+
+#include "context.h"
+
+namespace nx {
+
+namespace {
+
+size_t poly_words(const Context &c, size_t limbs) { return limbs * c.n; }
+
+void d2d(Context &c, uint64_t *dst, const uint64_t *src, size_t words)
+{
+    if (words) NX_HIP(hipMemcpyAsync(dst, src, words * 8, hipMemcpyDeviceToDevice, c.stream));
+}
+
+// Copy `limbs` limbs of each of `polys` polys between differently strided buffers.
+void copy_limbs(Context &c, uint64_t *dst, size_t dst_stride, const uint64_t *src, size_t src_stride, size_t polys,
+                size_t limbs)
+{
+    if (!polys || !limbs) return;
+    NX_HIP(hipMemcpy2DAsync(dst, dst_stride * 8, src, src_stride * 8, limbs * c.n * 8, polys,
+                            hipMemcpyDeviceToDevice, c.stream));
+}
+
+bool scale_within_bounds(const Context &c, double scale, size_t limbs)
+{
+    return !(scale <= 0 || (int)std::log2(scale) >= c.total_bits(limbs));
+}
+
+void same_params(const Ciphertext &a, const Ciphertext &b)
+{
+    require(a.ctx && a.ctx == b.ctx, "encrypted1 and encrypted2 parameter mismatch");
+    require(a.limbs == b.limbs, "encrypted1 and encrypted2 parameter mismatch");
+    require(a.ntt == b.ntt, "NTT form mismatch");
+}
+
+void valid(const Ciphertext &a)
+{
+    require(a.ctx != nullptr && a.size >= 2 && a.limbs >= 1, "encrypted is not valid for encryption parameters");
+}
+
+}  // namespace
+
+void ct_resize(Ciphertext &ct, Context *c, size_t size, size_t limbs)
+{
+    ct.ctx = c;
+    ct.size = size;
+    ct.limbs = limbs;
+    ct.data.reset(c, size * limbs * c->n);
+}
+
+void ct_copy(Ciphertext &dst, const Ciphertext &src)
+{
+    if (&dst == &src) return;
+    Context &c = *src.ctx;
+    DevBuf buf(&c, src.size * poly_words(c, src.limbs));
+    d2d(c, buf.get(), src.data.get(), buf.words());
+    dst.ctx = src.ctx;
+    dst.size = src.size;
+    dst.limbs = src.limbs;
+    dst.scale = src.scale;
+    dst.ntt = src.ntt;
+    dst.data = std::move(buf);
+}
+
+void pt_copy(Plaintext &dst, const Plaintext &src)
+{
+    if (&dst == &src) return;
+    Context &c = *src.ctx;
+    DevBuf buf(&c, poly_words(c, src.limbs));
+    d2d(c, buf.get(), src.data.get(), buf.words());
+    dst.ctx = src.ctx;
+    dst.limbs = src.limbs;
+    dst.scale = src.scale;
+    dst.ntt = src.ntt;
+    dst.data = std::move(buf);
+}
+
+static void add_sub(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst, bool sub)
+{
+    valid(a);
+    valid(b);
+    same_params(a, b);
+    require(are_close(a.scale, b.scale), "scale mismatch");
+    Context &c = *a.ctx;
+    const size_t pw = poly_words(c, a.limbs);
+    const size_t sz = std::max(a.size, b.size), mn = std::min(a.size, b.size);
+    DevBuf buf(&c, sz * pw);
+    if (sub)
+        poly_sub(c.tables, buf.get(), a.data.get(), b.data.get(), (uint32_t)mn, (uint32_t)a.limbs, c.stream);
+    else
+        poly_add(c.tables, buf.get(), a.data.get(), b.data.get(), (uint32_t)mn, (uint32_t)a.limbs, c.stream);
+    if (a.size > mn) d2d(c, buf.get() + mn * pw, a.data.get() + mn * pw, (a.size - mn) * pw);
+    if (b.size > mn) {
+        if (sub)
+            poly_negate(c.tables, buf.get() + mn * pw, b.data.get() + mn * pw, (uint32_t)(b.size - mn),
+                        (uint32_t)b.limbs, c.stream);
+        else
+            d2d(c, buf.get() + mn * pw, b.data.get() + mn * pw, (b.size - mn) * pw);
+    }
+    const double scale = a.scale;
+    const bool ntt = a.ntt;
+    const size_t limbs = a.limbs;
+    dst.ctx = &c;
+    dst.size = sz;
+    dst.limbs = limbs;
+    dst.scale = scale;
+    dst.ntt = ntt;
+    dst.data = std::move(buf);
+}
+
+void op_add(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst) { add_sub(a, b, dst, false); }
+void op_sub(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst) { add_sub(a, b, dst, true); }
+
+void op_negate(const Ciphertext &a, Ciphertext &dst)
+{
+    valid(a);
+    Context &c = *a.ctx;
+    DevBuf buf(&c, a.size * poly_words(c, a.limbs));
+    poly_negate(c.tables, buf.get(), a.data.get(), (uint32_t)a.size, (uint32_t)a.limbs, c.stream);
+    Ciphertext tmp;
+    tmp.ctx = &c;
+    tmp.size = a.size;
+    tmp.limbs = a.limbs;
+    tmp.scale = a.scale;
+    tmp.ntt = a.ntt;
+    tmp.data = std::move(buf);
+    dst = std::move(tmp);
+}
+
+static void plain_check(const Ciphertext &a, const Plaintext &p)
+{
+    valid(a);
+    require(p.ctx == a.ctx && p.limbs == a.limbs, "encrypted and plain parameter mismatch");
+    require(p.ntt == a.ntt, "NTT form mismatch");
+}
+
+static void add_sub_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst, bool sub)
+{
+    plain_check(a, p);
+    require(are_close(a.scale, p.scale), "scale mismatch");
+    Context &c = *a.ctx;
+    const size_t pw = poly_words(c, a.limbs);
+    DevBuf buf(&c, a.size * pw);
+    if (sub)
+        poly_sub(c.tables, buf.get(), a.data.get(), p.data.get(), 1, (uint32_t)a.limbs, c.stream);
+    else
+        poly_add(c.tables, buf.get(), a.data.get(), p.data.get(), 1, (uint32_t)a.limbs, c.stream);
+    d2d(c, buf.get() + pw, a.data.get() + pw, (a.size - 1) * pw);
+    Ciphertext tmp;
+    tmp.ctx = &c;
+    tmp.size = a.size;
+    tmp.limbs = a.limbs;
+    tmp.scale = a.scale;
+    tmp.ntt = a.ntt;
+    tmp.data = std::move(buf);
+    dst = std::move(tmp);
+}
+
+void op_add_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst) { add_sub_plain(a, p, dst, false); }
+void op_sub_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst) { add_sub_plain(a, p, dst, true); }
+
+void op_multiply_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst)
+{
+    plain_check(a, p);
+    require(a.ntt, "encrypted must be in NTT form");
+    Context &c = *a.ctx;
+    const double scale = a.scale * p.scale;
+    require(scale_within_bounds(c, scale, a.limbs), "scale out of bounds");
+    DevBuf buf(&c, a.size * poly_words(c, a.limbs));
+    poly_mul_plain(c.tables, buf.get(), a.data.get(), p.data.get(), (uint32_t)a.size, (uint32_t)a.limbs, c.stream);
+    Ciphertext tmp;
+    tmp.ctx = &c;
+    tmp.size = a.size;
+    tmp.limbs = a.limbs;
+    tmp.scale = scale;
+    tmp.ntt = true;
+    tmp.data = std::move(buf);
+    dst = std::move(tmp);
+}
+
+void op_multiply(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst)
+{
+    valid(a);
+    valid(b);
+    same_params(a, b);
+    require(a.ntt && b.ntt, "encrypted1 or encrypted2 must be in NTT form");
+    require(a.size == 2 && b.size == 2, "multiply expects two size-2 ciphertexts (relinearize first)");
+    Context &c = *a.ctx;
+    const double scale = a.scale * b.scale;
+    require(scale_within_bounds(c, scale, a.limbs), "scale out of bounds");
+    DevBuf buf(&c, 3 * poly_words(c, a.limbs));
+    poly_tensor(c.tables, buf.get(), a.data.get(), b.data.get(), (uint32_t)a.limbs, c.stream);
+    Ciphertext tmp;
+    tmp.ctx = &c;
+    tmp.size = 3;
+    tmp.limbs = a.limbs;
+    tmp.scale = scale;
+    tmp.ntt = true;
+    tmp.data = std::move(buf);
+    dst = std::move(tmp);
+}
+
+void op_square(const Ciphertext &a, Ciphertext &dst) { op_multiply(a, a, dst); }
+
+// ---------------------------------------------------------------- key switching
+// Adds key_switch(target) into (ct0, ct1); evaluator.cpp:2724-3021.
+// zero_c1: write instead of accumulate into ct1 (apply_galois clears c1 first).
+static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
+{
+    const size_t l = ct.limbs, n = c.n, L = c.L;
+    const size_t nout = l + 1;
+    uint8_t outp[kMaxPrimes];
+    for (size_t i = 0; i < l; i++) outp[i] = (uint8_t)i;
+    outp[l] = (uint8_t)L;
+
+    // ModUp: INTT the l digits, lift each to every output prime, NTT.
+    DevBuf coef(&c, l * n);
+    d2d(c, coef.get(), target, l * n);
+    ntt_inv(c, coef.get(), 1, l, l * n);
+    DevBuf T(&c, nout * l * n);
+    ks_modup_expand(c.tables, T.get(), coef.get(), target, (uint32_t)l, outp, (uint32_t)nout, c.stream);
+    {
+        NttBatch b;
+        b.data = T.get();
+        b.poly_stride = l * n;
+        b.polys = (uint32_t)nout;
+        b.limbs = (uint32_t)l;
+        b.prime_by_poly = 1;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = i < (int)nout ? outp[i] : 0;
+        ntt_forward(c.tables, b, c.stream);
+    }
+    // inner product with the device-resident key
+    DevBuf acc(&c, 2 * nout * n);
+    ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)l, (uint32_t)(L + 1), outp, (uint32_t)nout,
+                     c.stream);
+    // ModDown with rounding: INTT of the special-prime limb of both polys
+    {
+        NttBatch b;
+        b.data = acc.get() + l * n;
+        b.poly_stride = nout * n;
+        b.polys = 2;
+        b.limbs = 1;
+        b.prime_by_poly = 0;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)L;
+        ntt_inverse(c.tables, b, c.stream);
+    }
+    DevBuf tmp(&c, 2 * l * n);
+    rns_center_last(c.tables, tmp.get(), acc.get() + l * n, nout * n, (uint32_t)L, 2, (uint32_t)l, c.stream);
+    ntt_fwd(c, tmp.get(), 2, l, l * n);
+    rns_sub_mul(c.tables, ct.data.get(), l * n, acc.get(), nout * n, tmp.get(), c.d_p_inv, 1, (uint32_t)l, 1,
+                c.stream);
+    rns_sub_mul(c.tables, ct.data.get() + l * n, l * n, acc.get() + nout * n, nout * n, tmp.get() + l * n,
+                c.d_p_inv, 1, (uint32_t)l, zero_c1 ? 0 : 1, c.stream);
+}
+
+void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key)
+{
+    key_switch_impl(c, ct, target, key, false);
+}
+
+static const DevBuf &find_key(const KSwitchKeys &k, uint32_t elt, const char *msg)
+{
+    auto it = k.keys.find(elt);
+    require(it != k.keys.end(), msg);
+    return it->second;
+}
+
+void op_relinearize(const Ciphertext &a, const KSwitchKeys &rk, Ciphertext &dst)
+{
+    valid(a);
+    require(a.size == 3, "encrypted size must be 3 to relinearize");
+    require(rk.ctx == a.ctx, "relin_keys is not valid for encryption parameters");
+    require(a.ntt, "CKKS encrypted must be in NTT form");
+    Context &c = *a.ctx;
+    const DevBuf &key = find_key(rk, 0, "relinearization key not present");
+    const size_t pw = poly_words(c, a.limbs);
+    Ciphertext out;
+    out.ctx = &c;
+    out.size = 2;
+    out.limbs = a.limbs;
+    out.scale = a.scale;
+    out.ntt = true;
+    out.data.reset(&c, 2 * pw);
+    d2d(c, out.data.get(), a.data.get(), 2 * pw);
+    key_switch_impl(c, out, a.data.get() + 2 * pw, key, false);
+    dst = std::move(out);
+}
+
+void op_apply_galois(const Ciphertext &a, uint32_t elt, const KSwitchKeys &gk, Ciphertext &dst)
+{
+    valid(a);
+    require(a.size == 2, "encrypted size must be 2");
+    require(gk.ctx == a.ctx, "galois_keys is not valid for encryption parameters");
+    Context &c = *a.ctx;
+    require((elt & 1) && elt < 2 * c.n, "Galois element is not valid");
+    const DevBuf &key = find_key(gk, elt, "Galois key not present");
+    const uint32_t *perm = c.galois_table(elt);
+    const size_t pw = poly_words(c, a.limbs);
+    Ciphertext out;
+    out.ctx = &c;
+    out.size = 2;
+    out.limbs = a.limbs;
+    out.scale = a.scale;
+    out.ntt = true;
+    out.data.reset(&c, 2 * pw);
+    DevBuf t(&c, pw);
+    poly_permute(out.data.get(), a.data.get(), perm, 1, (uint32_t)a.limbs, c.logn, c.stream);
+    poly_permute(t.get(), a.data.get() + pw, perm, 1, (uint32_t)a.limbs, c.logn, c.stream);
+    key_switch_impl(c, out, t.get(), key, true);
+    dst = std::move(out);
+}
+
+static std::vector<int> naf(int value)
+{
+    // util/numth.h:22-42
+    std::vector<int> res;
+    const bool sign = value < 0;
+    value = std::abs(value);
+    for (int i = 0; value; i++) {
+        int zi = (value & 1) ? 2 - (value & 3) : 0;
+        value = (value - zi) >> 1;
+        if (zi) res.push_back((sign ? -zi : zi) * (1 << i));
+    }
+    return res;
+}
+
+void op_rotate(const Ciphertext &a, int steps, const KSwitchKeys &gk, Ciphertext &dst)
+{
+    // evaluator.cpp:2667-2722
+    Context &c = *a.ctx;
+    if (steps == 0) {
+        ct_copy(dst, a);
+        return;
+    }
+    const uint32_t elt = c.elt_from_step(steps);
+    if (gk.keys.count(elt)) {
+        op_apply_galois(a, elt, gk, dst);
+        return;
+    }
+    std::vector<int> terms = naf(steps);
+    require(terms.size() > 1, "Galois key not present");
+    Ciphertext cur;
+    ct_copy(cur, a);
+    for (int t : terms) {
+        if ((size_t)std::abs(t) == (c.n >> 1)) continue;
+        op_rotate(cur, t, gk, cur);
+    }
+    dst = std::move(cur);
+}
+
+void op_conjugate(const Ciphertext &a, const KSwitchKeys &gk, Ciphertext &dst)
+{
+    op_apply_galois(a, (uint32_t)(2 * a.ctx->n - 1), gk, dst);
+}
+
+// ---------------------------------------------------------------- rescale & level
+void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
+{
+    // evaluator.cpp:1402-1481 + util/rns.cpp:830-905
+    valid(a);
+    require(a.ntt, "CKKS encrypted must be in NTT form");
+    require(a.limbs >= 2, "end of modulus switching chain reached");
+    Context &c = *a.ctx;
+    const size_t l = a.limbs, n = c.n, P = a.size;
+    DevBuf last(&c, P * n);
+    copy_limbs(c, last.get(), n, a.data.get() + (l - 1) * n, l * n, P, 1);
+    {
+        NttBatch b;
+        b.data = last.get();
+        b.poly_stride = n;
+        b.polys = (uint32_t)P;
+        b.limbs = 1;
+        b.prime_by_poly = 0;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)(l - 1);
+        ntt_inverse(c.tables, b, c.stream);
+    }
+    DevBuf tmp(&c, P * (l - 1) * n);
+    rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1), c.stream);
+    ntt_fwd(c, tmp.get(), P, l - 1, (l - 1) * n);
+    DevBuf out(&c, P * (l - 1) * n);
+    rns_sub_mul(c.tables, out.get(), (l - 1) * n, a.data.get(), l * n, tmp.get(), c.rescale_inv(l), (uint32_t)P,
+                (uint32_t)(l - 1), 0, c.stream);
+    const double scale = a.scale / (double)c.moduli[l - 1];
+    Ciphertext r;
+    r.ctx = &c;
+    r.size = P;
+    r.limbs = l - 1;
+    r.scale = scale;
+    r.ntt = true;
+    r.data = std::move(out);
+    dst = std::move(r);
+}
+
+static void drop_to(const Ciphertext &a, size_t limbs, Ciphertext &dst)
+{
+    Context &c = *a.ctx;
+    if (limbs == a.limbs) {
+        ct_copy(dst, a);
+        return;
+    }
+    DevBuf out(&c, a.size * limbs * c.n);
+    copy_limbs(c, out.get(), limbs * c.n, a.data.get(), a.limbs * c.n, a.size, limbs);
+    Ciphertext r;
+    r.ctx = &c;
+    r.size = a.size;
+    r.limbs = limbs;
+    r.scale = a.scale;
+    r.ntt = a.ntt;
+    r.data = std::move(out);
+    dst = std::move(r);
+}
+
+void op_mod_switch_to_next(const Ciphertext &a, Ciphertext &dst)
+{
+    // evaluator.cpp:1483-1580 (CKKS: drop the last limb)
+    valid(a);
+    require(a.limbs >= 2, "end of modulus switching chain reached");
+    require(a.ntt, "CKKS encrypted must be in NTT form");
+    require(scale_within_bounds(*a.ctx, a.scale, a.limbs - 1), "scale out of bounds");
+    drop_to(a, a.limbs - 1, dst);
+}
+
+void op_mod_switch_to(const Ciphertext &a, size_t chain_index, Ciphertext &dst)
+{
+    // evaluator.cpp:1630-1680
+    valid(a);
+    Context &c = *a.ctx;
+    const size_t target = c.limbs_of(chain_index);
+    require(target <= a.limbs, "cannot switch to higher level modulus");
+    require(scale_within_bounds(c, a.scale, target), "scale out of bounds");
+    drop_to(a, target, dst);
+}
+
+void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &dst)
+{
+    require(a.ctx != nullptr, "plain is not valid for encryption parameters");
+    require(a.ntt, "plain is not in NTT form");
+    Context &c = *a.ctx;
+    const size_t target = c.limbs_of(chain_index);
+    require(target <= a.limbs, "cannot switch to higher level modulus");
+    if (target == a.limbs) {
+        pt_copy(dst, a);
+        return;
+    }
+    DevBuf out(&c, target * c.n);
+    d2d(c, out.get(), a.data.get(), target * c.n);
+    Plaintext r;
+    r.ctx = &c;
+    r.limbs = target;
+    r.scale = a.scale;
+    r.ntt = true;
+    r.data = std::move(out);
+    dst = std::move(r);
+}
+
+void op_to_ntt(const Ciphertext &a, Ciphertext &dst)
+{
+    valid(a);
+    require(!a.ntt, "encrypted is already in NTT form");
+    Ciphertext r;
+    ct_copy(r, a);
+    ntt_fwd(*a.ctx, r.data.get(), r.size, r.limbs, r.limbs * a.ctx->n);
+    r.ntt = true;
+    dst = std::move(r);
+}
+
+void op_from_ntt(const Ciphertext &a, Ciphertext &dst)
+{
+    valid(a);
+    require(a.ntt, "encrypted is not in NTT form");
+    Ciphertext r;
+    ct_copy(r, a);
+    ntt_inv(*a.ctx, r.data.get(), r.size, r.limbs, r.limbs * a.ctx->n);
+    r.ntt = false;
+    dst = std::move(r);
+}
+
+void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &dst)
+{
+    // x^(2n) = 1 in Z[x]/(x^n + 1): reduce the exponent into [0, 2n)
+    valid(a);
+    Context &c = *a.ctx;
+    const int64_t m = 2 * (int64_t)c.n;
+    const uint32_t k = (uint32_t)(((exponent % m) + m) % m);
+    const size_t pw = poly_words(c, a.limbs);
+    DevBuf coef(&c, a.size * pw);
+    d2d(c, coef.get(), a.data.get(), a.size * pw);
+    if (a.ntt) ntt_inv(c, coef.get(), a.size, a.limbs, pw);
+    DevBuf out(&c, a.size * pw);
+    poly_negacyclic_shift(c.tables, out.get(), coef.get(), k, (uint32_t)a.size, (uint32_t)a.limbs, c.stream);
+    if (a.ntt) ntt_fwd(c, out.get(), a.size, a.limbs, pw);
+    Ciphertext r;
+    r.ctx = &c;
+    r.size = a.size;
+    r.limbs = a.limbs;
+    r.scale = a.scale;
+    r.ntt = a.ntt;
+    r.data = std::move(out);
+    dst = std::move(r);
+}
+
+void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst)
+{
+    require(!in.empty(), "encrypteds cannot be empty");
+    Ciphertext acc;
+    ct_copy(acc, *in[0]);
+    for (size_t i = 1; i < in.size(); i++) op_add(acc, *in[i], acc);
+    dst = std::move(acc);
+}
+
+}  // namespace nx

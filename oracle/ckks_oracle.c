@@ -228,7 +228,9 @@ void or_ntt_inverse(uint64_t *a, int logn, uint64_t q, const uint64_t *irp)
 /* ------------------------------------------------------------------ */
 /* Galois automorphisms (util/galois.cpp:18-218)                      */
 /* ------------------------------------------------------------------ */
-/* galois.cpp:53-90: rotation step -> Galois element 3^step mod 2N (generator 3) */
+/* galois.cpp:53-90: rotation step -> Galois element 5^step mod 2N.  The bs fork
+ * sets GaloisTool::generator_ = 5 (util/galois.h:169), matching the encoder's
+ * slot order (ckks.cpp:36). */
 uint32_t or_elt_from_step(int logn, int step)
 {
     uint32_t n = 1u << logn, m = 2 * n;
@@ -239,7 +241,7 @@ uint32_t or_elt_from_step(int logn, int step)
     pos &= m - 1;
     int s = sign ? (int)(n >> 1) - (int)pos : (int)pos;
     uint64_t e = 1;
-    while (s--) e = (e * 3) & (m - 1);
+    while (s--) e = (e * 5) & (m - 1);
     return (uint32_t)e;
 }
 

@@ -124,11 +124,38 @@ def test_ntt_inverse_roundtrip_large():
 
 # ------------------------------------------------------------- Galois
 def test_galois_kat():
-    # tests/seal/util/galois.cpp:28-113
-    steps = [0, 1, -3, 2, -2, 3, -1]
-    assert [orc.elt_from_step(3, s) for s in steps] == [15, 3, 3, 9, 9, 11, 11]
+    # tests/seal/util/galois.cpp:95-113 (explicit element 3: generator independent)
     assert list(orc.apply_galois_ntt(np.arange(8), 3, 3)) == [4, 5, 7, 6, 1, 0, 2, 3]
     assert list(orc.apply_galois_coeff(np.arange(8), 3, 3, 17)) == [0, 14, 6, 1, 13, 7, 2, 12]
+    # The bs fork sets GaloisTool::generator_ = 5 (util/galois.h:169), so the
+    # upstream EltFromStep KAT (galois.cpp:36-48, generator 3) is stale; the same
+    # table with generator 5 (m = 16): step k -> 5^k, step -k -> 5^(n/2 - k).
+    steps = [0, 1, -3, 2, -2, 3, -1]
+    assert [orc.elt_from_step(3, s) for s in steps] == [15, 5, 5, 9, 9, 13, 13]
+
+
+@pytest.mark.parametrize("logn", [4, 8])
+def test_rotation_invariant_pins_generator(logn):
+    """decode(galois(encode(v), elt_from_step(k))) == v rotated left by k."""
+    n = 1 << logn
+    mods = orc.coeff_modulus_create(n, [60, 50])
+    rng = np.random.default_rng(11)
+    vals = rng.uniform(-1, 1, n // 2)
+    enc = orc.Encoder(logn)
+    scale = 2.0 ** 40
+    pt = enc.encode(vals, scale, mods)
+    for k in [1, 3, -2, n // 4 - 1]:
+        elt = orc.elt_from_step(logn, k)
+        rot = np.array([orc.apply_galois_ntt(pt[i], logn, elt) for i in range(len(mods))])
+        dec = enc.decode(rot, mods, scale)
+        assert np.max(np.abs(dec - np.roll(vals, -k))) < 1e-7
+    # conjugation element 2n-1
+    elt = orc.elt_from_step(logn, 0)
+    cv = rng.uniform(-1, 1, n // 2) + 1j * rng.uniform(-1, 1, n // 2)
+    pt = enc.encode(cv, scale, mods)
+    rot = np.array([orc.apply_galois_ntt(pt[i], logn, elt) for i in range(len(mods))])
+    dec = enc.decode(rot, mods, scale, complex_out=True)
+    assert np.max(np.abs(dec - np.conj(cv))) < 1e-7
 
 
 @pytest.mark.parametrize("logn", [4, 7, 10])
