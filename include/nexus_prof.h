@@ -1,0 +1,31 @@
+/*
+ * nexus_prof.h -- live kernel timing for benchmarks.  No counterpart in the
+ * reference (which only wall-clocks whole calls, cuda/src/utils.cuh:10-43).
+ *
+ * Every launch of a selected kernel class is bracketed by two HIP events on
+ * the stream it is launched on; the launch also records its algorithmic HBM
+ * bytes.  Kernel classes: "ntt_fwd_col", "ntt_fwd_row", "ntt_fwd_whole",
+ * "ntt_inv_row", "ntt_inv_col", "ntt_inv_whole", "ks_inner", "ks_modup",
+ * "tensor", "mul_plain", "rns_sub_mul".
+ */
+#ifndef NEXUS_PROF_H
+#define NEXUS_PROF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Comma-separated kernel classes to time, "*" for all, "" to stop.  Resets the statistics. */
+int nx_profile_select(const char *classes);
+/* Launches, summed event-measured milliseconds and summed algorithmic bytes of one class. */
+int nx_profile_read(const char *kernel_class, size_t *launches, double *total_ms, double *bytes);
+/* Comma-separated list of the classes with statistics. */
+int nx_profile_classes(char *buf, size_t len);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NEXUS_PROF_H */

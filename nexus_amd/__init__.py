@@ -453,3 +453,81 @@ def multiply_power_of_x(a, exponent, dest=None):
     d = _out(a.ctx, dest)
     _check(_lib.nx_multiply_power_of_x(a.h, exponent, d.h))
     return d
+
+
+# ---------------------------------------------------------------- evaluator programs (include/nexus_eval.h)
+_sig("nx_evaluator_create", [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(vp)])
+_sig("nx_evaluator_destroy", [vp])
+_sig("nx_eval_sgn", [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp])
+_sig("nx_eval_invert_sqrt", [vp, vp, ctypes.c_int, ctypes.c_int, vp])
+_sig("nx_eval_exp", [vp, vp, vp])
+_sig("nx_eval_inverse", [vp, vp, ctypes.c_int, vp])
+_sig("nx_eval_gelu", [vp, vp, vp])
+_sig("nx_eval_layer_norm", [vp, vp, ctypes.c_int, vp])
+_sig("nx_eval_softmax", [vp, vp, ctypes.c_int, vp])
+
+
+class CKKSEvaluator:
+    """NEXUS CKKSEvaluator programs (GELU, LayerNorm, Softmax, sign, inverse...)."""
+
+    def __init__(self, ctx, relin_keys, galois_keys=None, scale=2.0 ** 40, sk=None):
+        self.ctx = ctx
+        self._keep = (relin_keys, galois_keys, sk)
+        h = vp()
+        _check(_lib.nx_evaluator_create(ctx.h, sk.h if sk else None, relin_keys.h,
+                                        galois_keys.h if galois_keys else None, scale, ctypes.byref(h)))
+        self.h = h
+        self.scale = scale
+
+    def __del__(self, _fn=_lib.nx_evaluator_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def _run(self, fn, *args):
+        out = Ciphertext(self.ctx)
+        _check(fn(self.h, *args, out.h))
+        return out
+
+    def sgn_eval(self, x, d_g, d_f, sgn_factor=0.5):
+        return self._run(_lib.nx_eval_sgn, x.h, d_g, d_f, sgn_factor)
+
+    def invert_sqrt(self, x, d_newt=5, d_gold=2):
+        return self._run(_lib.nx_eval_invert_sqrt, x.h, d_newt, d_gold)
+
+    def exp(self, x):
+        return self._run(_lib.nx_eval_exp, x.h)
+
+    def inverse(self, x, iter=4):
+        return self._run(_lib.nx_eval_inverse, x.h, iter)
+
+    def gelu(self, x):
+        return self._run(_lib.nx_eval_gelu, x.h)
+
+    def layer_norm(self, x, length):
+        return self._run(_lib.nx_eval_layer_norm, x.h, length)
+
+    def softmax(self, x, length):
+        return self._run(_lib.nx_eval_softmax, x.h, length)
+
+
+# ---------------------------------------------------------------- live kernel timing (include/nexus_prof.h)
+_sig("nx_profile_select", [ctypes.c_char_p])
+_sig("nx_profile_read", [ctypes.c_char_p, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double)])
+_sig("nx_profile_classes", [ctypes.c_char_p, sz])
+
+
+def profile_select(classes):
+    """Time every launch of the given kernel classes ("*" = all) with HIP events."""
+    _check(_lib.nx_profile_select(classes.encode()))
+
+
+def profile_read(kernel_class):
+    n, ms, b = sz(), ctypes.c_double(), ctypes.c_double()
+    _check(_lib.nx_profile_read(kernel_class.encode(), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
+    return n.value, ms.value, b.value
+
+
+def profile_classes():
+    buf = ctypes.create_string_buffer(4096)
+    _check(_lib.nx_profile_classes(buf, 4096))
+    return [c for c in buf.value.decode().split(",") if c]

@@ -1,7 +1,8 @@
 """Build libnexus_ckks.so in-tree with hipcc for gfx950.
 
 Every source under nexus_amd/csrc is compiled (objects in nexus_amd/_build/) and
-linked into nexus_amd/libnexus_ckks.so.  Usage: ``python -m nexus_amd.build``.
+linked into nexus_amd/libnexus_ckks.so.  Usage: ``python nexus_amd/build.py``
+(run as a file: importing the package needs the library already built).
 """
 import concurrent.futures
 import os
@@ -15,6 +16,8 @@ LIB = os.path.join(HERE, "libnexus_ckks.so")
 INCLUDE = os.path.join(os.path.dirname(HERE), "include")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+ROCM_INCLUDE = "/opt/rocm/include"
 ARCH = "gfx950"
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + INCLUDE, "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
@@ -36,11 +39,11 @@ def sources():
 def compile_one(src):
     rel = os.path.relpath(src, CSRC).replace(os.sep, "__")
     obj = os.path.join(OBJ, rel + ".o")
-    cmd = [HIPCC] + COMMON
     if src.endswith(".hip"):
-        cmd += DEVICE
+        cmd = [HIPCC] + COMMON + DEVICE
     else:
-        cmd += ["-D__HIP_PLATFORM_AMD__"]
+        # host-only C++: no device pass
+        cmd = [CXX] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-I" + ROCM_INCLUDE]
     if os.path.basename(src) in NO_CONTRACT:
         cmd += ["-ffp-contract=off"]
     cmd += ["-c", src, "-o", obj]

@@ -24,14 +24,15 @@ struct nx_kswitch_keys {
     nx::KSwitchKeys impl;
 };
 
+namespace {
+thread_local std::string g_err;
+}
+
 namespace nx {
-// defined in encoder.cpp
-void constant_residues(Context &c, double value, size_t limbs, double scale, uint64_t *res);
+void set_last_error(const char *msg) { g_err = msg ? msg : ""; }
 }  // namespace nx
 
 namespace {
-
-thread_local std::string g_err;
 
 template <class F>
 int guard(F &&f)

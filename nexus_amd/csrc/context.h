@@ -34,6 +34,7 @@ inline void require(bool ok, const char *msg, int code = kInvalidArgument)
 }
 
 void hip_check(hipError_t e, const char *what);
+void set_last_error(const char *msg);  // thread-local message behind nx_last_error()
 #define NX_HIP(x) ::nx::hip_check((x), #x)
 
 // ---------------------------------------------------------------- memory

@@ -1,0 +1,553 @@
+// RAII value types and the Encoder / Encryptor / Decryptor / Evaluator classes
+// of the C++ drop-in (include/nexus/ckks.hpp), all over the C ABI.
+#include <cmath>
+#include <cstring>
+
+#include "../../../include/nexus/ckks.hpp"
+
+namespace nexus {
+
+void throw_status(int rc)
+{
+    const std::string msg = nx_last_error();
+    switch (rc) {
+    case NX_ERR_INVALID_ARGUMENT: throw std::invalid_argument(msg);
+    case NX_ERR_OUT_OF_RANGE: throw std::out_of_range(msg);
+    case NX_ERR_LOGIC: throw std::logic_error(msg);
+    default: throw std::runtime_error(msg);
+    }
+}
+
+// ---------------------------------------------------------------- context
+CKKSContext::CKKSContext(int log_n, const std::vector<int> &bits, uint32_t hw, size_t sparse) : log_n_(log_n)
+{
+    check(nx_context_create(log_n, bits.data(), bits.size(), hw, sparse, &h_));
+    load_info();
+}
+
+CKKSContext::CKKSContext(nx_context *borrowed) : h_(borrowed), owns_(false), log_n_(0) { load_info(); }
+
+void CKKSContext::load_info()
+{
+    size_t slots = 0;
+    check(nx_context_info(h_, &log_n_, &L_, &slots));
+    moduli_.resize(L_ + 1);
+    check(nx_context_moduli(h_, moduli_.data()));
+}
+
+CKKSContext::~CKKSContext()
+{
+    if (owns_) nx_context_destroy(h_);
+}
+
+uint32_t CKKSContext::galois_elt(int step) const
+{
+    uint32_t e = 0;
+    check(nx_galois_elt_from_step(h_, step, &e));
+    return e;
+}
+
+// ---------------------------------------------------------------- ciphertext
+Ciphertext::Ciphertext(const CKKSContext &ctx) { bind(ctx); }
+
+void Ciphertext::bind(const CKKSContext &ctx)
+{
+    if (h_) return;
+    ctx_ = &ctx;
+    check(nx_ciphertext_create(ctx.get(), &h_));
+}
+
+Ciphertext Ciphertext::copy_of(const CKKSContext &ctx, const nx_ciphertext *h)
+{
+    Ciphertext c(ctx);
+    check(nx_ciphertext_copy(c.h_, h));
+    return c;
+}
+
+Ciphertext::Ciphertext(const Ciphertext &o)
+{
+    if (o.h_) {
+        bind(*o.ctx_);
+        check(nx_ciphertext_copy(h_, o.h_));
+    }
+}
+
+Ciphertext::Ciphertext(Ciphertext &&o) noexcept : ctx_(o.ctx_), h_(o.h_) { o.h_ = nullptr; }
+
+Ciphertext &Ciphertext::operator=(const Ciphertext &o)
+{
+    if (this == &o) return *this;
+    if (!o.h_) {
+        nx_ciphertext_destroy(h_);
+        h_ = nullptr;
+        return *this;
+    }
+    bind(*o.ctx_);
+    check(nx_ciphertext_copy(h_, o.h_));
+    return *this;
+}
+
+Ciphertext &Ciphertext::operator=(Ciphertext &&o) noexcept
+{
+    if (this != &o) {
+        nx_ciphertext_destroy(h_);
+        ctx_ = o.ctx_;
+        h_ = o.h_;
+        o.h_ = nullptr;
+    }
+    return *this;
+}
+
+Ciphertext::~Ciphertext() { nx_ciphertext_destroy(h_); }
+
+size_t Ciphertext::size() const
+{
+    size_t s = 0;
+    check(nx_ciphertext_info(h_, &s, nullptr, nullptr, nullptr, nullptr));
+    return s;
+}
+
+size_t Ciphertext::params_id() const
+{
+    size_t c = 0;
+    check(nx_ciphertext_info(h_, nullptr, &c, nullptr, nullptr, nullptr));
+    return c;
+}
+
+size_t Ciphertext::coeff_modulus_size() const
+{
+    size_t c = 0;
+    check(nx_ciphertext_info(h_, nullptr, nullptr, &c, nullptr, nullptr));
+    return c;
+}
+
+double Ciphertext::scale() const
+{
+    double s = 0;
+    check(nx_ciphertext_info(h_, nullptr, nullptr, nullptr, &s, nullptr));
+    return s;
+}
+
+void Ciphertext::set_scale(double s) { check(nx_ciphertext_set_scale(h_, s)); }
+
+// ---------------------------------------------------------------- plaintext
+Plaintext::Plaintext(const CKKSContext &ctx) { bind(ctx); }
+
+void Plaintext::bind(const CKKSContext &ctx)
+{
+    if (h_) return;
+    ctx_ = &ctx;
+    check(nx_plaintext_create(ctx.get(), &h_));
+}
+
+Plaintext::Plaintext(const Plaintext &o)
+{
+    if (o.h_) {
+        bind(*o.ctx_);
+        check(nx_plaintext_copy(h_, o.h_));
+    }
+}
+
+Plaintext::Plaintext(Plaintext &&o) noexcept : ctx_(o.ctx_), h_(o.h_) { o.h_ = nullptr; }
+
+Plaintext &Plaintext::operator=(const Plaintext &o)
+{
+    if (this == &o) return *this;
+    if (!o.h_) {
+        nx_plaintext_destroy(h_);
+        h_ = nullptr;
+        return *this;
+    }
+    bind(*o.ctx_);
+    check(nx_plaintext_copy(h_, o.h_));
+    return *this;
+}
+
+Plaintext &Plaintext::operator=(Plaintext &&o) noexcept
+{
+    if (this != &o) {
+        nx_plaintext_destroy(h_);
+        ctx_ = o.ctx_;
+        h_ = o.h_;
+        o.h_ = nullptr;
+    }
+    return *this;
+}
+
+Plaintext::~Plaintext() { nx_plaintext_destroy(h_); }
+
+size_t Plaintext::params_id() const
+{
+    size_t c = 0;
+    check(nx_plaintext_info(h_, &c, nullptr, nullptr));
+    return c;
+}
+
+double Plaintext::scale() const
+{
+    double s = 0;
+    check(nx_plaintext_info(h_, nullptr, nullptr, &s));
+    return s;
+}
+
+void Plaintext::set_scale(double s) { check(nx_plaintext_set_scale(h_, s)); }
+
+// ---------------------------------------------------------------- keys
+SecretKey::SecretKey(const CKKSContext &ctx, uint64_t seed) { check(nx_secret_key_generate(ctx.get(), seed, &h_)); }
+SecretKey::~SecretKey()
+{
+    if (owns_) nx_secret_key_destroy(h_);
+}
+
+KSwitchKeys::KSwitchKeys(const CKKSContext &ctx) { check(nx_kswitch_keys_create(ctx.get(), &h_)); }
+KSwitchKeys::~KSwitchKeys()
+{
+    if (owns_) nx_kswitch_keys_destroy(h_);
+}
+
+bool KSwitchKeys::has(uint32_t elt) const
+{
+    int p = 0;
+    check(nx_kswitch_keys_has(h_, elt, &p));
+    return p != 0;
+}
+
+// ---------------------------------------------------------------- encoder
+void Encoder::encode(const std::vector<double> &values, size_t chain_index, double scale, Plaintext &plain) const
+{
+    plain.bind(*ctx_);
+    check(nx_encode(ctx_->get(), values.data(), nullptr, values.size(), chain_index, scale, plain.get()));
+}
+
+void Encoder::encode(const std::vector<double> &values, double scale, Plaintext &plain) const
+{
+    encode(values, 1, scale, plain);
+}
+
+void Encoder::encode(const std::vector<std::complex<double>> &values, size_t chain_index, double scale,
+                     Plaintext &plain) const
+{
+    std::vector<double> re(values.size()), im(values.size());
+    for (size_t i = 0; i < values.size(); i++) {
+        re[i] = values[i].real();
+        im[i] = values[i].imag();
+    }
+    plain.bind(*ctx_);
+    check(nx_encode(ctx_->get(), re.data(), im.data(), values.size(), chain_index, scale, plain.get()));
+}
+
+void Encoder::encode(const std::vector<std::complex<double>> &values, double scale, Plaintext &plain) const
+{
+    encode(values, 1, scale, plain);
+}
+
+void Encoder::encode(double value, size_t chain_index, double scale, Plaintext &plain) const
+{
+    plain.bind(*ctx_);
+    check(nx_encode_constant(ctx_->get(), value, chain_index, scale, plain.get()));
+}
+
+void Encoder::encode(double value, double scale, Plaintext &plain) const { encode(value, 1, scale, plain); }
+
+void Encoder::decode(const Plaintext &plain, std::vector<double> &values) const
+{
+    values.assign(slot_count(), 0.0);
+    check(nx_decode(ctx_->get(), plain.get(), values.data(), nullptr, values.size()));
+}
+
+void Encoder::decode(const Plaintext &plain, std::vector<std::complex<double>> &values) const
+{
+    std::vector<double> re(slot_count()), im(slot_count());
+    check(nx_decode(ctx_->get(), plain.get(), re.data(), im.data(), re.size()));
+    values.resize(re.size());
+    for (size_t i = 0; i < re.size(); i++) values[i] = {re[i], im[i]};
+}
+
+// ---------------------------------------------------------------- crypto
+void Encryptor::encrypt(const Plaintext &plain, Ciphertext &ct)
+{
+    ct.bind(*ctx_);
+    check(nx_encrypt_symmetric(ctx_->get(), sk_->get(), plain.get(), counter_++, ct.get()));
+}
+
+void Decryptor::decrypt(const Ciphertext &ct, Plaintext &plain) const
+{
+    plain.bind(*ctx_);
+    check(nx_decrypt(ctx_->get(), sk_->get(), ct.get(), plain.get()));
+}
+
+void Decryptor::create_galois_keys_from_steps(const std::vector<int> &steps, GaloisKeys &keys) const
+{
+    std::vector<uint32_t> elts;
+    for (int s : steps) elts.push_back(ctx_->galois_elt(s));
+    create_galois_keys_from_elts(elts, keys);
+}
+
+void Decryptor::create_galois_keys_from_elts(const std::vector<uint32_t> &elts, GaloisKeys &keys) const
+{
+    check(nx_galois_keys_generate(ctx_->get(), sk_->get(), elts.data(), elts.size(), 0x67616c6f, keys.get()));
+}
+
+void Decryptor::create_relin_keys(RelinKeys &keys) const
+{
+    check(nx_relin_key_generate(ctx_->get(), sk_->get(), 0x72656c69, keys.get()));
+}
+
+// ---------------------------------------------------------------- evaluator
+#define OUT(d) (d).bind(*ctx_)
+
+void Evaluator::mod_switch_to_next_inplace(Ciphertext &ct) const { check(nx_mod_switch_to_next(ct.get(), ct.get())); }
+void Evaluator::mod_switch_to_inplace(Ciphertext &ct, size_t chain_index) const
+{
+    check(nx_mod_switch_to(ct.get(), chain_index, ct.get()));
+}
+void Evaluator::mod_switch_to_inplace(Plaintext &pt, size_t chain_index) const
+{
+    check(nx_plain_mod_switch_to(pt.get(), chain_index, pt.get()));
+}
+void Evaluator::rescale_to_next_inplace(Ciphertext &ct) const { check(nx_rescale_to_next(ct.get(), ct.get())); }
+void Evaluator::relinearize_inplace(Ciphertext &ct, const RelinKeys &rk) const
+{
+    check(nx_relinearize(ct.get(), rk.get(), ct.get()));
+}
+void Evaluator::square(const Ciphertext &ct, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_square(ct.get(), dest.get()));
+}
+void Evaluator::square_inplace(Ciphertext &ct) const { check(nx_square(ct.get(), ct.get())); }
+void Evaluator::multiply(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_multiply(a.get(), b.get(), dest.get()));
+}
+void Evaluator::multiply_inplace(Ciphertext &a, const Ciphertext &b) const
+{
+    check(nx_multiply(a.get(), b.get(), a.get()));
+}
+void Evaluator::multiply_plain(const Ciphertext &ct, const Plaintext &p, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_multiply_plain(ct.get(), p.get(), dest.get()));
+}
+void Evaluator::multiply_plain_inplace(Ciphertext &ct, const Plaintext &p) const
+{
+    check(nx_multiply_plain(ct.get(), p.get(), ct.get()));
+}
+void Evaluator::add_plain(const Ciphertext &ct, const Plaintext &p, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_add_plain(ct.get(), p.get(), dest.get()));
+}
+void Evaluator::add_plain_inplace(Ciphertext &ct, const Plaintext &p) const
+{
+    check(nx_add_plain(ct.get(), p.get(), ct.get()));
+}
+void Evaluator::add(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_add(a.get(), b.get(), dest.get()));
+}
+void Evaluator::add_inplace(Ciphertext &a, const Ciphertext &b) const { check(nx_add(a.get(), b.get(), a.get())); }
+void Evaluator::add_many(const std::vector<Ciphertext> &cts, Ciphertext &dest) const
+{
+    if (cts.size() < 2) throw std::invalid_argument("add_many requires at least 2 ciphertexts");
+    std::vector<const nx_ciphertext *> hs;
+    for (auto &c : cts) hs.push_back(c.get());
+    OUT(dest);
+    check(nx_add_many(hs.data(), hs.size(), dest.get()));
+}
+void Evaluator::sub_plain(const Ciphertext &ct, const Plaintext &p, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_sub_plain(ct.get(), p.get(), dest.get()));
+}
+void Evaluator::sub_plain_inplace(Ciphertext &ct, const Plaintext &p) const
+{
+    check(nx_sub_plain(ct.get(), p.get(), ct.get()));
+}
+void Evaluator::sub(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_sub(a.get(), b.get(), dest.get()));
+}
+void Evaluator::sub_inplace(Ciphertext &a, const Ciphertext &b) const { check(nx_sub(a.get(), b.get(), a.get())); }
+void Evaluator::negate(const Ciphertext &ct, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_negate(ct.get(), dest.get()));
+}
+void Evaluator::negate_inplace(Ciphertext &ct) const { check(nx_negate(ct.get(), ct.get())); }
+void Evaluator::rotate_vector(const Ciphertext &ct, int steps, const GaloisKeys &k, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_rotate_vector(ct.get(), steps, k.get(), dest.get()));
+}
+void Evaluator::rotate_vector_inplace(Ciphertext &ct, int steps, const GaloisKeys &k) const
+{
+    check(nx_rotate_vector(ct.get(), steps, k.get(), ct.get()));
+}
+void Evaluator::apply_galois(const Ciphertext &ct, uint32_t elt, const GaloisKeys &k, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_apply_galois(ct.get(), elt, k.get(), dest.get()));
+}
+void Evaluator::apply_galois_inplace(Ciphertext &ct, uint32_t elt, const GaloisKeys &k) const
+{
+    check(nx_apply_galois(ct.get(), elt, k.get(), ct.get()));
+}
+void Evaluator::complex_conjugate(const Ciphertext &ct, const GaloisKeys &k, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_complex_conjugate(ct.get(), k.get(), dest.get()));
+}
+void Evaluator::complex_conjugate_inplace(Ciphertext &ct, const GaloisKeys &k) const
+{
+    check(nx_complex_conjugate(ct.get(), k.get(), ct.get()));
+}
+void Evaluator::transform_to_ntt(const Ciphertext &ct, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_transform_to_ntt(ct.get(), dest.get()));
+}
+void Evaluator::transform_to_ntt_inplace(Ciphertext &ct) const { check(nx_transform_to_ntt(ct.get(), ct.get())); }
+void Evaluator::transform_from_ntt(const Ciphertext &ct, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_transform_from_ntt(ct.get(), dest.get()));
+}
+void Evaluator::transform_from_ntt_inplace(Ciphertext &ct) const
+{
+    check(nx_transform_from_ntt(ct.get(), ct.get()));
+}
+void Evaluator::multiply_power_of_x(const Ciphertext &ct, int64_t e, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_multiply_power_of_x(ct.get(), e, dest.get()));
+}
+void Evaluator::multiply_const(const Ciphertext &ct, double v, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_multiply_const(ct.get(), v, dest.get()));
+}
+void Evaluator::multiply_const_inplace(Ciphertext &ct, double v) const
+{
+    check(nx_multiply_const(ct.get(), v, ct.get()));
+}
+void Evaluator::add_const(const Ciphertext &ct, double v, Ciphertext &dest) const
+{
+    OUT(dest);
+    check(nx_add_const(ct.get(), v, dest.get()));
+}
+void Evaluator::add_const_inplace(Ciphertext &ct, double v) const { check(nx_add_const(ct.get(), v, ct.get())); }
+void Evaluator::double_inplace(Ciphertext &ct) const { check(nx_add(ct.get(), ct.get(), ct.get())); }
+
+// ---- reduced-error alignment (SEAL-4.1-bs evaluator.cpp:420-594): bring the
+// higher operand down with multiply_const(scale_adjust) + rescale + mod_switch
+// so that the scales agree exactly.
+namespace {
+enum class RedOp { Add, Sub, Mul };
+}
+
+static void reduced_error(const Evaluator &ev, const CKKSContext &c, Ciphertext &e1, const Ciphertext &e2, RedOp op)
+{
+    const size_t n1 = e1.coeff_modulus_size(), n2 = e2.coeff_modulus_size();
+    auto apply = [&](Ciphertext &dst, const Ciphertext &a, const Ciphertext &b) {
+        if (op == RedOp::Add)
+            ev.add(a, b, dst);
+        else if (op == RedOp::Sub)
+            ev.sub(a, b, dst);
+        else
+            ev.multiply(a, b, dst);
+    };
+    if (n1 == n2) {
+        e1.set_scale(e2.scale());
+        apply(e1, e1, e2);
+        return;
+    }
+    if (n1 < n2) {
+        const double qlast = (double)c.moduli()[n2 - 1];
+        const double adj = e1.scale() * qlast / (e2.scale() * e2.scale());
+        Ciphertext e2a;
+        ev.multiply_const(e2, adj, e2a);
+        e2a.set_scale(e1.scale() * qlast);
+        ev.rescale_to_next_inplace(e2a);
+        ev.mod_switch_to_inplace(e2a, e1.params_id());
+        e1.set_scale(e2a.scale());
+        apply(e1, e1, e2a);
+    } else {
+        const double qlast = (double)c.moduli()[n1 - 1];
+        const double adj = e2.scale() * qlast / (e1.scale() * e1.scale());
+        Ciphertext e1a;
+        ev.multiply_const(e1, adj, e1a);
+        e1a.set_scale(e2.scale() * qlast);
+        ev.rescale_to_next_inplace(e1a);
+        ev.mod_switch_to_inplace(e1a, e2.params_id());
+        e1a.set_scale(e2.scale());
+        apply(e1, e1a, e2);
+    }
+}
+
+void Evaluator::add_inplace_reduced_error(Ciphertext &a, const Ciphertext &b) const
+{
+    reduced_error(*this, *ctx_, a, b, RedOp::Add);
+}
+void Evaluator::add_reduced_error(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
+{
+    if (&b == &dest) {
+        add_inplace_reduced_error(dest, a);
+    } else {
+        dest = a;
+        add_inplace_reduced_error(dest, b);
+    }
+}
+void Evaluator::sub_inplace_reduced_error(Ciphertext &a, const Ciphertext &b) const
+{
+    reduced_error(*this, *ctx_, a, b, RedOp::Sub);
+}
+void Evaluator::sub_reduced_error(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
+{
+    Ciphertext t = a;
+    sub_inplace_reduced_error(t, b);
+    dest = std::move(t);
+}
+void Evaluator::multiply_inplace_reduced_error(Ciphertext &a, const Ciphertext &b, const RelinKeys &rk) const
+{
+    reduced_error(*this, *ctx_, a, b, RedOp::Mul);
+    relinearize_inplace(a, rk);
+}
+void Evaluator::multiply_reduced_error(const Ciphertext &a, const Ciphertext &b, const RelinKeys &rk,
+                                       Ciphertext &dest) const
+{
+    if (&b == &dest) {
+        multiply_inplace_reduced_error(dest, a, rk);
+    } else {
+        dest = a;
+        multiply_inplace_reduced_error(dest, b, rk);
+    }
+}
+
+void Evaluator::multiply_vector_reduced_error(const Ciphertext &ct, const std::vector<double> &values,
+                                              Ciphertext &dest) const
+{
+    Plaintext p(*ctx_);
+    check(nx_encode(ctx_->get(), values.data(), nullptr, values.size(), ct.params_id(), ct.scale(), p.get()));
+    multiply_plain(ct, p, dest);
+}
+
+void Evaluator::multiply_vector_reduced_error(const Ciphertext &ct, const std::vector<std::complex<double>> &values,
+                                              Ciphertext &dest) const
+{
+    std::vector<double> re(values.size()), im(values.size());
+    for (size_t i = 0; i < values.size(); i++) {
+        re[i] = values[i].real();
+        im[i] = values[i].imag();
+    }
+    Plaintext p(*ctx_);
+    check(nx_encode(ctx_->get(), re.data(), im.data(), values.size(), ct.params_id(), ct.scale(), p.get()));
+    multiply_plain(ct, p, dest);
+}
+
+#undef OUT
+
+}  // namespace nexus

@@ -17,6 +17,7 @@
 // barriers, 256 threads (four wave64) per workgroup.  Row twiddles for row r at
 // local stage s are the contiguous block fwd[2^s (N1 + r) ...].
 #include "kernels.h"
+#include "prof.h"
 
 namespace nx {
 
@@ -262,6 +263,12 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     const uint32_t n = 1u << a.logn;
     const uint32_t tiles = ROLE == kWhole ? 1u : (n / G::TILE);
     dim3 grid(tiles, batch);
+    static const char *names[2][3] = {{"ntt_inv_whole", "ntt_inv_col", "ntt_inv_row"},
+                                      {"ntt_fwd_whole", "ntt_fwd_col", "ntt_fwd_row"}};
+    // algorithmic bytes: each limb read + written once (16 n) plus the pass's
+    // twiddle entries {w, w'} (16 B each)
+    const double tw = ROLE == kWhole ? (double)n : (ROLE == kCol ? (double)G::S : (double)(n - n / G::S));
+    ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)batch * (16.0 * n + 16.0 * tw));
     hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD>), grid, dim3(kThreads), 0, s, a);
 }
 

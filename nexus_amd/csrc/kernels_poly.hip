@@ -2,6 +2,7 @@
 // (poly, limb) pairs so the modulus is uniform per workgroup; grid x covers the
 // n coefficients of a limb, one coefficient per lane.
 #include "kernels.h"
+#include "prof.h"
 
 namespace nx {
 
@@ -308,6 +309,7 @@ void poly_negate(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32
 void poly_mul_plain(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *pt, uint32_t polys,
                     uint32_t limbs, hipStream_t s)
 {
+    ProfScope prof("mul_plain", s, 8.0 * (1u << t.logn) * ((double)polys * limbs * 2 + limbs));
     hipLaunchKernelGGL(k_mul_plain, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, pt, t.mod, limbs,
                        1u << t.logn);
 }
@@ -315,6 +317,7 @@ void poly_mul_plain(const DeviceTables &t, uint64_t *out, const uint64_t *a, con
 void poly_tensor(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t limbs,
                  hipStream_t s)
 {
+    ProfScope prof("tensor", s, 56.0 * (1u << t.logn) * limbs);
     hipLaunchKernelGGL(k_tensor, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, a, b, t.mod, limbs,
                        1u << t.logn);
 }
@@ -361,6 +364,7 @@ void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, cons
                  const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
                  hipStream_t s)
 {
+    ProfScope prof("rns_sub_mul", s, 8.0 * (1u << t.logn) * polys * limbs * (3 + (accumulate ? 1 : 0)));
     hipLaunchKernelGGL(k_sub_mul, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, out_stride, x, x_stride,
                        tmp, inv, t.mod, limbs, 1u << t.logn, accumulate);
 }
@@ -369,6 +373,7 @@ void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, c
                      const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
 {
     (void)ntt_src;
+    ProfScope prof("ks_modup", s, 8.0 * (1u << t.logn) * ((double)l + (double)nout * l));
     hipLaunchKernelGGL(k_modup_expand, grid_for(t.logn, nout * l), dim3(kBlock), 0, s, T, coef, t.mod,
                        make_map(out_primes_host, nout), l, 1u << t.logn);
 }
@@ -376,6 +381,8 @@ void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, c
 void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t l,
                       uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
 {
+    // T limbs (8 n) and two key limbs (16 n) per (output prime, digit), two outputs per prime
+    ProfScope prof("ks_inner", s, (double)(1u << t.logn) * nout * (24.0 * l + 16.0));
     hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, nout), dim3(kBlock), 0, s, acc, T, key, t.mod,
                        make_map(out_primes_host, nout), l, key_limbs, nout, 1u << t.logn);
 }

@@ -92,8 +92,12 @@ uint64_t minimal_primitive_root(uint64_t degree, uint64_t q)
 
 uint32_t reverse_bits(uint32_t x, int bits)
 {
-    if (bits == 0) return 0;
-    return __builtin_bitreverse32(x) >> (32 - bits);
+    uint32_t r = 0;
+    for (int i = 0; i < bits; i++) {
+        r = (r << 1) | (x & 1);
+        x >>= 1;
+    }
+    return r;
 }
 
 void ntt_root_tables(int logn, uint64_t q, std::vector<uint64_t> &fwd, std::vector<uint64_t> &inv)

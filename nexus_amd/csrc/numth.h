@@ -1,6 +1,7 @@
 // Host-side number theory for context set-up: NTT-friendly prime selection,
 // minimal primitive roots and the SEAL-ordered root tables.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #include <vector>

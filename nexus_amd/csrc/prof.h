@@ -1,0 +1,29 @@
+// Live per-kernel timing with HIP events.  When a kernel class is selected
+// (nx_profile_select), every launch of that class is bracketed by two events
+// recorded on the launch's own stream, together with the launch's work units
+// (RNS limbs processed), so average duration and algorithmic throughput can
+// be computed for the real workload.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace nx {
+
+bool prof_active(const char *kernel_class);
+void prof_record(const char *kernel_class, hipStream_t s, bool begin, double units);
+
+struct ProfScope {
+    const char *cls;
+    hipStream_t s;
+    double units;
+    bool on;
+    ProfScope(const char *c, hipStream_t st, double u) : cls(c), s(st), units(u), on(prof_active(c))
+    {
+        if (on) prof_record(cls, s, true, units);
+    }
+    ~ProfScope()
+    {
+        if (on) prof_record(cls, s, false, units);
+    }
+};
+
+}  // namespace nx
