@@ -5,24 +5,7 @@
 #include <cstring>
 #include <string>
 
-#include "context.h"
-
-struct nx_context {
-    nx::Context impl;
-    nx_context(int logn, const std::vector<int> &bits, uint32_t hw, size_t sparse) : impl(logn, bits, hw, sparse) {}
-};
-struct nx_ciphertext {
-    nx::Ciphertext impl;
-};
-struct nx_plaintext {
-    nx::Plaintext impl;
-};
-struct nx_secret_key {
-    nx::SecretKey impl;
-};
-struct nx_kswitch_keys {
-    nx::KSwitchKeys impl;
-};
+#include "handles.h"
 
 namespace {
 thread_local std::string g_err;

@@ -152,6 +152,9 @@ public:
 
     // host copies of SEAL-ordered root tables (encoder uses complex roots, not these)
     std::vector<std::vector<uint64_t>> fwd_host, inv_host;
+    // NTT-domain images of monomials x^k, keyed by (k, limbs)
+    std::map<std::pair<uint32_t, size_t>, DevBuf> xk_cache;
+    std::mutex xk_mu;
 
 private:
     std::map<uint32_t, uint32_t *> galois_;
@@ -190,6 +193,19 @@ void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &d
 
 // key switch of `target` (limbs x n, NTT) under key `key` added into ct (2 polys)
 void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key);
+// B key switches in one pass: targets [B][l][n] into cts [B][2][l][n]; with
+// zero_c1 the c1 of each result is written (not accumulated).
+void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
+                      bool zero_c1);
+
+// A batch of B ciphertexts of one shape (size 2) stored back to back: [B][2][limbs][n].
+struct CtBatch {
+    Context *ctx = nullptr;
+    size_t B = 0, limbs = 0;
+    double scale = 1.0;
+    DevBuf data;
+    uint64_t *ct(size_t i) const { return data.get() + i * 2 * limbs * ctx->n; }
+};
 
 // encoder (host FFT + device NTT) -- encoder.cpp
 void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,

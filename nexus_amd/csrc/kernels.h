@@ -22,13 +22,16 @@ struct DeviceTables {
 
 // A batch of in-place NTTs over polys x limbs RNS limbs of n words each.
 // Limb `i` of poly `p` lives at data + p * poly_stride + i * n.  Its prime is
-// prime_of[i] (prime_by_poly = 0) or prime_of[p] (prime_by_poly = 1).
+// prime_of[i] (prime_by_poly = 0) or prime_of[p'] (prime_by_poly = 1) with
+// p' = p, or p mod poly_period when poly_period > 0 (batches of ciphertexts
+// that each carry the same per-poly prime pattern).
 struct NttBatch {
     uint64_t *data;
     uint64_t poly_stride;
     uint32_t polys;
     uint32_t limbs;
     uint32_t prime_by_poly;
+    uint32_t poly_period = 0;
     uint8_t prime_of[kMaxPrimes];
 };
 
@@ -72,19 +75,21 @@ void poly_add_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, co
 // step 2 combines.  `acc_add` (may be null) receives out instead: acc[p][i] += result.
 void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last, uint64_t last_stride,
                      uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s);
+// accumulate: 0 = write, 1 = add into out, 2 = add into even polys / write odd polys
+// (a batch of key-switched ciphertexts whose c1 was cleared by the automorphism).
 void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
                  const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
                  hipStream_t s);
 
-// Key-switch ModUp: for every output prime I in out_primes (count = nout) and
-// digit j < l, T[I][j] = (coef[j] mod q_I) when I != j, or ntt_src[j] when the
-// output prime is the digit's own prime.  coef = INTT(c2) limbs, [l][n].
-void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const uint64_t *ntt_src, uint32_t l,
+// Key-switch ModUp over a batch of B targets: for ciphertext b, every output
+// prime I in out_primes (count = nout) and digit j < l,
+// T[b][I][j] = coef[b][j] mod q_I  (coef = INTT of the target, [B][l][n]).
+void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, uint32_t B, uint32_t l,
                      const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
-// Inner product: acc[k][I][c] = sum_j T[I][j][c] * key[j][k][kp(I)][c] mod q_kp(I)
-// key layout [digits][2][key_limbs][n].
-void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t l,
-                      uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+// Inner product: acc[b][k][I][c] = sum_j T[b][I][j][c] * key[j][k][kp(I)][c] mod q_kp(I)
+// key layout [digits][2][key_limbs][n], shared by the batch.
+void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t B,
+                      uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
 
 // ---------------------------------------------------------------- keys
 // Deterministic counter-based sampling (not SEAL's PRNG).
@@ -105,6 +110,13 @@ void poly_mul_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const
 // out[c] += a[c] * w (mod q_prime) for one limb; w = {value, Shoup quotient}
 void limb_add_scaled(const DeviceTables &t, uint64_t *out, const uint64_t *a, ulonglong2 w, uint32_t prime,
                      hipStream_t s);
+// out[p][i] = a[p][i] * P[i] + b[p][i] * Q[i]  (NTT-domain, limb i uses prime i)
+void poly_mul2_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *P, const uint64_t *b,
+                   const uint64_t *Q, uint32_t polys, uint32_t limbs, hipStream_t s);
+// out[i][p][l] = sum_{j < inner} E[i * inner + j][p][l] * A[j][l]  for i < cols, p < 2
+// (E: [cols*inner][2][limbs][n] expanded ciphertexts, A: [inner][limbs][n] plaintexts)
+void mm_accumulate(const DeviceTables &t, uint64_t *out, const uint64_t *E, const uint64_t *A, uint32_t cols,
+                   uint32_t inner, uint32_t limbs, hipStream_t s);
 // Negacyclic shift of coefficient-domain limbs: out = a * x^k mod (x^n + 1), 0 <= k < 2n.
 void poly_negacyclic_shift(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t k, uint32_t polys,
                            uint32_t limbs, hipStream_t s);

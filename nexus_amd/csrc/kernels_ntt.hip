@@ -31,6 +31,7 @@ struct NttKArgs {
     uint32_t limbs;
     uint32_t logn;
     uint32_t prime_by_poly;
+    uint32_t poly_period;
     const ulonglong2 *tw;
     const ModConst *mod;
     const ulonglong2 *ninv;
@@ -168,7 +169,8 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     __shared__ uint64_t lds[G::TILE];
     const uint32_t bidx = blockIdx.y;
     const uint32_t poly = bidx / a.limbs, limb = bidx - poly * a.limbs;
-    const uint32_t prime = a.prime_by_poly ? a.prime_of[poly] : a.prime_of[limb];
+    const uint32_t prime =
+        a.prime_by_poly ? a.prime_of[a.poly_period ? poly % a.poly_period : poly] : a.prime_of[limb];
     const uint32_t n = 1u << a.logn;
     uint64_t *data = a.data + poly * a.poly_stride + (uint64_t)limb * n;
     const ulonglong2 *tw = a.tw + (uint64_t)prime * n;
@@ -304,6 +306,7 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
     a.limbs = b.limbs;
     a.logn = (uint32_t)t.logn;
     a.prime_by_poly = b.prime_by_poly;
+    a.poly_period = b.poly_period;
     a.tw = fwd ? t.fwd : t.inv;
     a.mod = t.mod;
     a.ninv = t.ninv;

@@ -159,20 +159,21 @@ __global__ void k_sub_mul(uint64_t *out, uint64_t out_stride, const uint64_t *x,
     const uint64_t d = sub_mod(xv, tmp[(uint64_t)row * n + c], q);
     uint64_t r = shoup(d, w.x, w.y, q);
     uint64_t *o = out + poly * out_stride + (uint64_t)limb * n + c;
-    if (accumulate) r = add_mod(*o, r, q);
+    if (accumulate == 1 || (accumulate == 2 && (poly & 1) == 0)) r = add_mod(*o, r, q);
     *o = r;
 }
 
 // ---- key switching
 __global__ void k_modup_expand(uint64_t *T, const uint64_t *coef, const ModConst *mod, PrimeMap outp, uint32_t l,
-                               uint32_t n)
+                               uint32_t nout, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
-    const uint32_t row = blockIdx.y;  // I * l + j
-    const uint32_t I = row / l, j = row - I * l;
+    const uint32_t row = blockIdx.y;  // (b * nout + I) * l + j
+    const uint32_t bi = row / l, j = row - bi * l;
+    const uint32_t b = bi / nout, I = bi - b * nout;
     const ModConst m = mod[outp.p[I]];
-    const uint64_t v = coef[(uint64_t)j * n + c];
+    const uint64_t v = coef[((uint64_t)b * l + j) * n + c];
     T[(uint64_t)row * n + c] = v >= m.q ? barrett64(v, m) : v;
 }
 
@@ -181,11 +182,13 @@ __global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
-    const uint32_t I = blockIdx.y;
+    const uint32_t bi = blockIdx.y;  // b * nout + I
+    const uint32_t b = bi / nout, I = bi - b * nout;
     const uint32_t kp = outp.p[I];
     const ModConst m = mod[kp];
     uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
-    const uint64_t *t = T + (uint64_t)I * l * n + c;
+    const uint64_t *t = T + (uint64_t)bi * l * n + c;
+    acc += (uint64_t)b * 2 * nout * n;
     const uint64_t *k0 = key + (uint64_t)kp * n + c;
     const uint64_t kstride = (uint64_t)key_limbs * n;  // between poly 0 and 1 of a digit
     for (uint32_t j = 0; j < l; j++) {
@@ -369,21 +372,20 @@ void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, cons
                        tmp, inv, t.mod, limbs, 1u << t.logn, accumulate);
 }
 
-void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const uint64_t *ntt_src, uint32_t l,
+void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, uint32_t B, uint32_t l,
                      const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
 {
-    (void)ntt_src;
-    ProfScope prof("ks_modup", s, 8.0 * (1u << t.logn) * ((double)l + (double)nout * l));
-    hipLaunchKernelGGL(k_modup_expand, grid_for(t.logn, nout * l), dim3(kBlock), 0, s, T, coef, t.mod,
-                       make_map(out_primes_host, nout), l, 1u << t.logn);
+    ProfScope prof("ks_modup", s, 8.0 * (1u << t.logn) * B * ((double)l + (double)nout * l));
+    hipLaunchKernelGGL(k_modup_expand, grid_for(t.logn, B * nout * l), dim3(kBlock), 0, s, T, coef, t.mod,
+                       make_map(out_primes_host, nout), l, nout, 1u << t.logn);
 }
 
-void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t l,
-                      uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t B,
+                      uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
 {
     // T limbs (8 n) and two key limbs (16 n) per (output prime, digit), two outputs per prime
-    ProfScope prof("ks_inner", s, (double)(1u << t.logn) * nout * (24.0 * l + 16.0));
-    hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, nout), dim3(kBlock), 0, s, acc, T, key, t.mod,
+    ProfScope prof("ks_inner", s, (double)(1u << t.logn) * B * nout * (24.0 * l + 16.0));
+    hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, B * nout), dim3(kBlock), 0, s, acc, T, key, t.mod,
                        make_map(out_primes_host, nout), l, key_limbs, nout, 1u << t.logn);
 }
 
@@ -466,6 +468,78 @@ void poly_negacyclic_shift(const DeviceTables &t, uint64_t *out, const uint64_t 
 {
     const uint32_t n = 1u << t.logn;
     hipLaunchKernelGGL(k_negacyclic_shift, dim3((n + 255) / 256, polys * limbs), dim3(256), 0, s, out, a, t.mod, k,
+                       limbs, n);
+}
+
+}  // namespace nx
+
+namespace nx {
+
+namespace {
+
+__global__ void k_mul2_add(uint64_t *out, const uint64_t *a, const uint64_t *P, const uint64_t *b, const uint64_t *Q,
+                           const ModConst *mod, uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const ModConst m = mod[limb];
+    const uint64_t idx = (uint64_t)row * n + c, pidx = (uint64_t)limb * n + c;
+    const uint64_t x = a[idx], y = P[pidx], u = b[idx], v = Q[pidx];
+    uint64_t lo1 = x * y, hi1 = mulhi64(x, y);
+    uint64_t lo2 = u * v, hi2 = mulhi64(u, v);
+    uint64_t lo = lo1 + lo2;
+    uint64_t hi = hi1 + hi2 + (lo < lo1);
+    out[idx] = barrett128(hi, lo, m);
+}
+
+// One lane per (output column, poly, limb, coefficient); 128-bit accumulation,
+// folded back below q every 32 products (each product < 2^122, so 33 terms
+// stay below 2^128).
+__global__ void k_mm_acc(uint64_t *out, const uint64_t *E, const uint64_t *A, const ModConst *mod, uint32_t inner,
+                         uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * 256 + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // (i * 2 + p) * limbs + l
+    const uint32_t limb = row % limbs;
+    const uint32_t ip = row / limbs, i = ip >> 1, p = ip & 1;
+    const ModConst m = mod[limb];
+    const uint64_t ct_words = 2ull * limbs * n;
+    const uint64_t *e = E + (uint64_t)i * inner * ct_words + ((uint64_t)p * limbs + limb) * n + c;
+    const uint64_t *aa = A + (uint64_t)limb * n + c;
+    const uint64_t a_stride = (uint64_t)limbs * n;
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t j = 0; j < inner; j++) {
+        const uint64_t x = e[(uint64_t)j * ct_words], y = aa[(uint64_t)j * a_stride];
+        const uint64_t plo = x * y, phi = mulhi64(x, y);
+        lo += plo;
+        hi += phi + (lo < plo);
+        if ((j & 31) == 31) {
+            lo = barrett128(hi, lo, m);
+            hi = 0;
+        }
+    }
+    out[(uint64_t)row * n + c] = barrett128(hi, lo, m);
+}
+
+}  // namespace
+
+void poly_mul2_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *P, const uint64_t *b,
+                   const uint64_t *Q, uint32_t polys, uint32_t limbs, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    ProfScope prof("mul2_add", s, 8.0 * n * ((double)polys * limbs * 3 + 2.0 * limbs));
+    hipLaunchKernelGGL(k_mul2_add, dim3((n + 255) / 256, polys * limbs), dim3(256), 0, s, out, a, P, b, Q, t.mod,
+                       limbs, n);
+}
+
+void mm_accumulate(const DeviceTables &t, uint64_t *out, const uint64_t *E, const uint64_t *A, uint32_t cols,
+                   uint32_t inner, uint32_t limbs, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    ProfScope prof("mm_acc", s, 8.0 * n * limbs * ((double)cols * inner * 2 + inner + cols * 2.0));
+    hipLaunchKernelGGL(k_mm_acc, dim3((n + 255) / 256, cols * 2 * limbs), dim3(256), 0, s, out, E, A, t.mod, inner,
                        limbs, n);
 }
 

@@ -211,52 +211,60 @@ void op_square(const Ciphertext &a, Ciphertext &dst) { op_multiply(a, a, dst); }
 // ---------------------------------------------------------------- key switching
 // Adds key_switch(target) into (ct0, ct1); evaluator.cpp:2724-3021.
 // zero_c1: write instead of accumulate into ct1 (apply_galois clears c1 first).
-static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
+// Key switch of B targets ([B][l][n], NTT) added into B ciphertexts
+// ([B][2][l][n]); every stage runs as one launch over the whole batch.
+void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
+                      bool zero_c1)
 {
-    const size_t l = ct.limbs, n = c.n, L = c.L;
+    const size_t n = c.n, L = c.L;
     const size_t nout = l + 1;
     uint8_t outp[kMaxPrimes];
     for (size_t i = 0; i < l; i++) outp[i] = (uint8_t)i;
     outp[l] = (uint8_t)L;
 
     // ModUp: INTT the l digits, lift each to every output prime, NTT.
-    DevBuf coef(&c, l * n);
-    d2d(c, coef.get(), target, l * n);
-    ntt_inv(c, coef.get(), 1, l, l * n);
-    DevBuf T(&c, nout * l * n);
-    ks_modup_expand(c.tables, T.get(), coef.get(), target, (uint32_t)l, outp, (uint32_t)nout, c.stream);
+    DevBuf coef(&c, B * l * n);
+    d2d(c, coef.get(), targets, B * l * n);
+    ntt_inv(c, coef.get(), B, l, l * n);
+    DevBuf T(&c, B * nout * l * n);
+    ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
     {
         NttBatch b;
         b.data = T.get();
         b.poly_stride = l * n;
-        b.polys = (uint32_t)nout;
+        b.polys = (uint32_t)(B * nout);
         b.limbs = (uint32_t)l;
         b.prime_by_poly = 1;
+        b.poly_period = (uint32_t)nout;
         for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = i < (int)nout ? outp[i] : 0;
         ntt_forward(c.tables, b, c.stream);
     }
     // inner product with the device-resident key
-    DevBuf acc(&c, 2 * nout * n);
-    ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)l, (uint32_t)(L + 1), outp, (uint32_t)nout,
-                     c.stream);
-    // ModDown with rounding: INTT of the special-prime limb of both polys
+    DevBuf acc(&c, B * 2 * nout * n);
+    ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)B, (uint32_t)l, (uint32_t)(L + 1), outp,
+                     (uint32_t)nout, c.stream);
+    // ModDown with rounding: INTT of the special-prime limb of every poly
     {
         NttBatch b;
         b.data = acc.get() + l * n;
         b.poly_stride = nout * n;
-        b.polys = 2;
+        b.polys = (uint32_t)(2 * B);
         b.limbs = 1;
         b.prime_by_poly = 0;
         for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)L;
         ntt_inverse(c.tables, b, c.stream);
     }
-    DevBuf tmp(&c, 2 * l * n);
-    rns_center_last(c.tables, tmp.get(), acc.get() + l * n, nout * n, (uint32_t)L, 2, (uint32_t)l, c.stream);
-    ntt_fwd(c, tmp.get(), 2, l, l * n);
-    rns_sub_mul(c.tables, ct.data.get(), l * n, acc.get(), nout * n, tmp.get(), c.d_p_inv, 1, (uint32_t)l, 1,
-                c.stream);
-    rns_sub_mul(c.tables, ct.data.get() + l * n, l * n, acc.get() + nout * n, nout * n, tmp.get() + l * n,
-                c.d_p_inv, 1, (uint32_t)l, zero_c1 ? 0 : 1, c.stream);
+    DevBuf tmp(&c, B * 2 * l * n);
+    rns_center_last(c.tables, tmp.get(), acc.get() + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
+                    c.stream);
+    ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
+    rns_sub_mul(c.tables, cts, l * n, acc.get(), nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
+                zero_c1 ? 2 : 1, c.stream);
+}
+
+static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
+{
+    key_switch_batch(c, ct.data.get(), target, 1, ct.limbs, key, zero_c1);
 }
 
 void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key)
