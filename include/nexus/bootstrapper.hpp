@@ -1,0 +1,92 @@
+// CKKS bootstrapping over the C++ drop-in (reference Bootstrapper,
+// src/bootstrapping/Bootstrapper.{h,cpp}; cuda/src/bootstrapping/Bootstrapper.cu).
+//
+// Pipeline (as the reference's bootstrap_sparse_3 / bootstrap_full_3,
+// cuda/src/bootstrapping/Bootstrapper.cu:2987-3113):
+//   ModRaise -> SubSum (sparse packing) -> CoeffToSlot (3 BSGS levels) ->
+//   EvalMod (Chebyshev interpolant of cos(2 pi (K u - 1/4) / 2^r), then r
+//   double-angle steps) -> SlotToCoeff (3 BSGS levels).
+// The linear transforms are the special-FFT factorisation of the slot map
+// (butterfly stages with twiddles exp(2 pi i 5^j / 4 len)) grouped into three
+// levels in diagonal form; the bit-reversal permutations of CoeffToSlot and
+// SlotToCoeff cancel because EvalMod acts slot-wise.  Every level lands on an
+// exactly known scale: plaintext diagonals and constants are encoded at the
+// scale that compensates the prime dropped by the following rescale.
+#pragma once
+
+#include <complex>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "ckks.hpp"
+
+namespace nexus {
+
+// One level of a BSGS linear transform: y = sum_t D_t (.) rot(x, t * stride),
+// with t = g * j + i (baby i in [0, g), giant j).  Diagonals are length-N/2
+// slot vectors, already pre-rotated by -g*j*stride.
+struct LinearLevel {
+    int stride = 1;
+    int g = 1;
+    std::map<std::pair<int, int>, std::vector<std::complex<double>>> diag;  // key (j, i)
+};
+
+// Host-side plan of the three CoeffToSlot and three SlotToCoeff levels
+// (reference genfftcoeff_3 / geninvfftcoeff_3, Bootstrapper.cu:1137,1545).
+struct BootPlan {
+    int logn = 0, logNh = 0;
+    double K = 25;
+    std::vector<LinearLevel> cts, stc;  // in application order
+    std::vector<int> rotation_steps;    // sorted, unique; 0 = conjugation
+};
+BootPlan make_boot_plan(int logNh, int logn, double K);
+
+class Bootstrapper {
+public:
+    struct Params {
+        int logn = 0;            // log2 of the slot count (logn < logNh: sparse packing)
+        int K = 25;              // bound of the ModRaise overflow |I| (reference boundary_K)
+        int cheb_degree = 63;    // EvalMod interpolant degree (depth floor(log2 d) + 2)
+        int double_angles = 2;   // r (reference scale_factor = 2)
+        double final_scale = 0;  // output scale (0: the input's scale)
+    };
+
+    Bootstrapper(CKKSEvaluator &ckks, const Params &p);
+
+    // Rotation steps whose Galois keys bootstrapping needs (0 = conjugation),
+    // as the reference's addLeftRotKeys_Linear_to_vector_3 (Bootstrapper.cu:82-178).
+    const std::vector<int> &rotation_steps() const { return plan_.rotation_steps; }
+    // Levels bootstrapping consumes from the top data level.
+    int depth() const;
+    const BootPlan &plan() const { return plan_; }
+    const std::vector<double> &cheb_coeffs() const { return cheb_; }
+
+    void bootstrap(const Ciphertext &in, Ciphertext &out);
+
+    // stages (exposed for tests)
+    void mod_raise(const Ciphertext &in, Ciphertext &out) const;
+    void sub_sum(Ciphertext &ct) const;
+    // sparse: one output u (real, 2n values per period); dense: re and im parts
+    void coeff_to_slot(const Ciphertext &in, Ciphertext &u_re, Ciphertext &u_im);
+    void eval_mod(const Ciphertext &u, Ciphertext &out);
+    void slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out);
+
+private:
+    Ciphertext apply_level(const LinearLevel &lv, int tag, const Ciphertext &x, double out_scale, double value_factor);
+    Ciphertext cheb_eval(const std::vector<double> &c, size_t level, double scale);
+    const Ciphertext &basis(int k);
+    Ciphertext match(const Ciphertext &ct, size_t level, double scale);
+    Ciphertext mul_exact(const Ciphertext &a, const Ciphertext &b);
+    size_t chain_of(size_t level) const { return ck_.context->data_primes() + 1 - level; }
+    double prime(size_t level) const { return (double)ck_.context->moduli()[level - 1]; }
+
+    CKKSEvaluator &ck_;
+    Params p_;
+    BootPlan plan_;
+    std::vector<double> cheb_;
+    std::map<int, Ciphertext> basis_;  // Chebyshev T_k of the current EvalMod input
+    std::map<std::tuple<int, int, int, size_t, double>, Plaintext> ptcache_;
+};
+
+}  // namespace nexus

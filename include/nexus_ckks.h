@@ -139,6 +139,10 @@ int nx_multiply_const(const nx_ciphertext *a, double value, nx_ciphertext *dest)
 int nx_add_const(const nx_ciphertext *a, double value, nx_ciphertext *dest);
 /* multiply by x^exponent (coefficient shift; reference matrix_mul.cu:84-127) */
 int nx_multiply_power_of_x(const nx_ciphertext *a, int64_t exponent, nx_ciphertext *dest);
+/* ModRaise of a lowest-level ciphertext to the top data level (reference
+ * src/bootstrapping/Bootstrapper.cpp:2937-2991, cuda/src/bootstrapping/Bootstrapper.cu:2813-2860);
+ * the scale is left unchanged. */
+int nx_mod_raise(const nx_ciphertext *a, nx_ciphertext *dest);
 
 #ifdef __cplusplus
 }

@@ -38,6 +38,48 @@ int nx_eval_layer_norm(nx_evaluator *ev, const nx_ciphertext *x, int len, nx_cip
 /* SoftmaxEvaluator::softmax (src/softmax.cpp:6-42) */
 int nx_eval_softmax(nx_evaluator *ev, const nx_ciphertext *x, int len, nx_ciphertext *out);
 
+/* ---- bootstrapping (reference Bootstrapper, cuda/src/bootstrapping/Bootstrapper.cu;
+ * src/bootstrapping/Bootstrapper.cpp).  nx_bootstrapper_create plays the
+ * constructor + prepare_mod_polynomial + generate_LT_coefficient_3
+ * (cuda/src/bootstrapping.cu:78-115): log_slots (reference logn; 0 = N/2
+ * slots), K (boundary_K), degree of the EvalMod interpolant (sin_cos_deg),
+ * double_angles (scale_factor), final_scale (0 = the input's scale).  The
+ * evaluator must hold the relinearisation key and Galois keys for every step
+ * of nx_bootstrapper_rotation_steps. */
+typedef struct nx_bootstrapper nx_bootstrapper;
+int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, int double_angles, double final_scale,
+                           nx_bootstrapper **out);
+int nx_bootstrapper_destroy(nx_bootstrapper *bt);
+/* addLeftRotKeys_Linear_to_vector_3 (Bootstrapper.cu:82-178): rotation steps
+ * (0 = conjugation); *count receives the number, at most `capacity` written */
+int nx_bootstrapper_rotation_steps(const nx_bootstrapper *bt, int *steps, size_t capacity, size_t *count);
+/* levels consumed from the top data level */
+int nx_bootstrapper_depth(const nx_bootstrapper *bt, int *depth);
+/* bootstrap_3 (Bootstrapper.cu:3358-3370): a ciphertext at any level (it is
+ * switched to the lowest first) -> fresh ciphertext `depth` levels below the top */
+int nx_bootstrap(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
+/* stages of bootstrap_sparse_3 / bootstrap_full_3 (Bootstrapper.cu:2987-3113) */
+int nx_boot_mod_raise(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
+int nx_boot_sub_sum(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
+/* u_im is written for dense packing only (may be null for sparse) */
+int nx_boot_coeff_to_slot(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *u_re, nx_ciphertext *u_im);
+int nx_boot_eval_mod(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
+/* f_im: dense packing only (null for sparse); input_scale = scale of the bootstrapped input */
+int nx_boot_slot_to_coeff(nx_bootstrapper *bt, const nx_ciphertext *f_re, const nx_ciphertext *f_im,
+                          double input_scale, nx_ciphertext *out);
+
+/* Host-only view of the linear-transform plan (genfftcoeff_3 / geninvfftcoeff_3,
+ * Bootstrapper.cu:1137,1545); no device use.  which = 0: CoeffToSlot, 1:
+ * SlotToCoeff; level in application order.  A level computes
+ * y = sum_(j,i) rot(D_(j,i) (.) rot(x, i*stride), g*j*stride) over N/2 slots. */
+int nx_boot_plan_info(int log_half_n, int log_slots, double K, int which, int level, int *levels, int *stride, int *g,
+                      size_t *count);
+/* diagonals: js[count], is[count], re/im[count][N/2] */
+int nx_boot_plan_diagonals(int log_half_n, int log_slots, double K, int which, int level, int *js, int *is,
+                           double *re, double *im);
+/* rotation steps of a plan (0 = conjugation) */
+int nx_boot_plan_steps(int log_half_n, int log_slots, double K, int *steps, size_t capacity, size_t *count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -455,6 +455,16 @@ def multiply_power_of_x(a, exponent, dest=None):
     return d
 
 
+_sig("nx_mod_raise", [vp, vp])
+
+
+def mod_raise(a, dest=None):
+    """Lowest-level ciphertext -> top data level, scale unchanged (Bootstrapper.cpp:2937-2991)."""
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_mod_raise(a.h, d.h))
+    return d
+
+
 # ---------------------------------------------------------------- evaluator programs (include/nexus_eval.h)
 _sig("nx_evaluator_create", [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(vp)])
 _sig("nx_evaluator_destroy", [vp])
@@ -507,6 +517,111 @@ class CKKSEvaluator:
 
     def softmax(self, x, length):
         return self._run(_lib.nx_eval_softmax, x.h, length)
+
+
+# ---------------------------------------------------------------- bootstrapping (include/nexus_eval.h)
+_ip = ctypes.POINTER(ctypes.c_int)
+_sig("nx_bootstrapper_create", [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                ctypes.POINTER(vp)])
+_sig("nx_bootstrapper_destroy", [vp])
+_sig("nx_bootstrapper_rotation_steps", [vp, _ip, sz, ctypes.POINTER(sz)])
+_sig("nx_bootstrapper_depth", [vp, _ip])
+_sig("nx_bootstrap", [vp, vp, vp])
+_sig("nx_boot_mod_raise", [vp, vp, vp])
+_sig("nx_boot_sub_sum", [vp, vp, vp])
+_sig("nx_boot_coeff_to_slot", [vp, vp, vp, vp])
+_sig("nx_boot_eval_mod", [vp, vp, vp])
+_sig("nx_boot_slot_to_coeff", [vp, vp, vp, ctypes.c_double, vp])
+_sig("nx_boot_plan_info", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip,
+                           ctypes.POINTER(sz)])
+_sig("nx_boot_plan_diagonals", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _ip,
+                                f64p, f64p])
+_sig("nx_boot_plan_steps", [ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, sz, ctypes.POINTER(sz)])
+
+
+def boot_plan_steps(log_half_n, log_slots, K=25):
+    """Rotation steps of the bootstrapping linear transforms (0 = conjugation); host only."""
+    n = sz()
+    _check(_lib.nx_boot_plan_steps(log_half_n, log_slots, float(K), None, 0, ctypes.byref(n)))
+    buf = (ctypes.c_int * n.value)()
+    _check(_lib.nx_boot_plan_steps(log_half_n, log_slots, float(K), buf, n.value, ctypes.byref(n)))
+    return list(buf)
+
+
+def boot_plan_level(log_half_n, log_slots, which, level, K=25):
+    """One BSGS level of the plan: dict(stride, g, levels, diags={(j, i): complex[N/2]}); host only."""
+    levels, stride, g, cnt = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), sz()
+    _check(_lib.nx_boot_plan_info(log_half_n, log_slots, float(K), which, level, ctypes.byref(levels),
+                                  ctypes.byref(stride), ctypes.byref(g), ctypes.byref(cnt)))
+    nh = 1 << log_half_n
+    c = cnt.value
+    js = (ctypes.c_int * c)()
+    is_ = (ctypes.c_int * c)()
+    re = np.zeros(c * nh)
+    im = np.zeros(c * nh)
+    _check(_lib.nx_boot_plan_diagonals(log_half_n, log_slots, float(K), which, level, js, is_,
+                                       re.ctypes.data_as(f64p), im.ctypes.data_as(f64p)))
+    d = (re + 1j * im).reshape(c, nh)
+    return {"levels": levels.value, "stride": stride.value, "g": g.value,
+            "diags": {(js[k], is_[k]): d[k] for k in range(c)}}
+
+
+class Bootstrapper:
+    """CKKS bootstrapping (reference Bootstrapper::bootstrap_3): ModRaise, SubSum,
+    CoeffToSlot, EvalMod, SlotToCoeff.  `ev` must hold relinearisation keys and
+    Galois keys for every step of `rotation_steps()`."""
+
+    def __init__(self, ev, log_slots=0, K=25, degree=63, double_angles=2, final_scale=0.0):
+        self.ev = ev
+        self.ctx = ev.ctx
+        h = vp()
+        _check(_lib.nx_bootstrapper_create(ev.h, log_slots, K, degree, double_angles, final_scale, ctypes.byref(h)))
+        self.h = h
+        self.log_slots = log_slots or self.ctx.log_n - 1
+        self.dense = self.log_slots == self.ctx.log_n - 1
+
+    def __del__(self, _fn=_lib.nx_bootstrapper_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def rotation_steps(self):
+        n = sz()
+        _check(_lib.nx_bootstrapper_rotation_steps(self.h, None, 0, ctypes.byref(n)))
+        buf = (ctypes.c_int * n.value)()
+        _check(_lib.nx_bootstrapper_rotation_steps(self.h, buf, n.value, ctypes.byref(n)))
+        return list(buf)
+
+    def depth(self):
+        d = ctypes.c_int()
+        _check(_lib.nx_bootstrapper_depth(self.h, ctypes.byref(d)))
+        return d.value
+
+    def _run(self, fn, *args):
+        out = Ciphertext(self.ctx)
+        _check(fn(self.h, *args, out.h))
+        return out
+
+    def bootstrap(self, ct):
+        return self._run(_lib.nx_bootstrap, ct.h)
+
+    def mod_raise(self, ct):
+        return self._run(_lib.nx_boot_mod_raise, ct.h)
+
+    def sub_sum(self, ct):
+        return self._run(_lib.nx_boot_sub_sum, ct.h)
+
+    def coeff_to_slot(self, ct):
+        re = Ciphertext(self.ctx)
+        im = Ciphertext(self.ctx) if self.dense else None
+        _check(_lib.nx_boot_coeff_to_slot(self.h, ct.h, re.h, im.h if im is not None else None))
+        return re, im
+
+    def eval_mod(self, ct):
+        return self._run(_lib.nx_boot_eval_mod, ct.h)
+
+    def slot_to_coeff(self, f_re, f_im, input_scale):
+        out = Ciphertext(self.ctx)
+        _check(_lib.nx_boot_slot_to_coeff(self.h, f_re.h, f_im.h if f_im is not None else None, input_scale, out.h))
+        return out
 
 
 # ---------------------------------------------------------------- live kernel timing (include/nexus_prof.h)

@@ -111,3 +111,12 @@ int nx_mm_accumulate(const nx_plaintext *const *a_pts, size_t inner, const nx_ct
 }
 
 }  // extern "C"
+
+extern "C" int nx_mod_raise(const nx_ciphertext *a, nx_ciphertext *dest)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(a, "a");
+        nx::need_ptr(dest, "dest");
+        nx::op_mod_raise(a->impl, dest->impl);
+    });
+}

@@ -190,6 +190,7 @@ void op_to_ntt(const Ciphertext &a, Ciphertext &dst);
 void op_from_ntt(const Ciphertext &a, Ciphertext &dst);
 void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst);
 void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &dst);
+void op_mod_raise(const Ciphertext &a, Ciphertext &dst);
 
 // key switch of `target` (limbs x n, NTT) under key `key` added into ct (2 polys)
 void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key);

@@ -1,0 +1,575 @@
+// CKKS bootstrapping: host planning of the CoeffToSlot / SlotToCoeff levels
+// and the homomorphic pipeline over the C++ drop-in (include/nexus/bootstrapper.hpp).
+#include "../../../include/nexus/bootstrapper.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <set>
+#include <stdexcept>
+
+namespace nexus {
+
+namespace {
+
+typedef std::complex<double> cd;
+typedef std::vector<cd> cvec;
+typedef std::map<int, cvec> DiagMap;  // offset mod n -> n-periodic diagonal (length n)
+
+// Twiddle of butterfly stage i (block length len = 2^(i+1)) at in-block index j:
+// exp(2 pi i (5^j mod 4 len) / (4 len))  (reference genorigcoeff,
+// cuda/src/bootstrapping/Bootstrapper.cu:537-556: theta_0 * 2^(logn-1-i) * 5^j).
+std::vector<cd> stage_twiddles(int i)
+{
+    const uint64_t len = uint64_t(2) << i, m = 4 * len, h = len / 2;
+    std::vector<cd> z(h);
+    uint64_t e = 1;
+    for (uint64_t j = 0; j < h; j++) {
+        z[j] = std::polar(1.0, 2.0 * M_PI * (double)e / (double)m);
+        e = (e * 5) % m;
+    }
+    return z;
+}
+
+// Diagonal form of one butterfly stage on an n-vector in bit-reversed order:
+//   forward: y[s] = x[s] + z x[s+h],        y[s+h] = x[s] - z x[s+h]
+//   inverse: y[s] = (x[s] + x[s+h]) / 2,    y[s+h] = (x[s] - x[s+h]) / (2 z)
+// (reference genorigcoeff, Bootstrapper.cu:537-590; three diagonals at
+// offsets 0, +h, -h).
+DiagMap stage_diags(size_t n, int i, bool inverse)
+{
+    const size_t len = size_t(2) << i, h = len / 2;
+    const std::vector<cd> z = stage_twiddles(i);
+    DiagMap D;
+    cvec &d0 = D[0];
+    d0.assign(n, 0.0);
+    cvec &dp = D[(int)h];
+    dp.assign(n, 0.0);
+    cvec &dm = D[(int)((n - h) % n)];  // == dp when h == n/2
+    if (dm.empty()) dm.assign(n, 0.0);
+    for (size_t b = 0; b < n; b += len)
+        for (size_t j = 0; j < h; j++) {
+            const size_t s = b + j;
+            if (!inverse) {
+                d0[s] += 1.0;
+                D[(int)h][s] += z[j];
+                D[(int)((n - h) % n)][s + h] += 1.0;
+                d0[s + h] += -z[j];
+            } else {
+                const cd iz = std::conj(z[j]);
+                d0[s] += 0.5;
+                D[(int)h][s] += 0.5;
+                D[(int)((n - h) % n)][s + h] += 0.5 * iz;
+                d0[s + h] += -0.5 * iz;
+            }
+        }
+    return D;
+}
+
+// Diagonals of A*B (B applied first): C_{a+b} += A_a (.) rot(B_b, a).
+DiagMap compose(const DiagMap &A, const DiagMap &B, size_t n)
+{
+    DiagMap C;
+    for (const auto &a : A)
+        for (const auto &b : B) {
+            const int c = (int)(((size_t)a.first + (size_t)b.first) % n);
+            cvec &dst = C[c];
+            if (dst.empty()) dst.assign(n, 0.0);
+            for (size_t s = 0; s < n; s++) dst[s] += a.second[s] * b.second[(s + (size_t)a.first) % n];
+        }
+    // drop diagonals that cancel
+    for (auto it = C.begin(); it != C.end();) {
+        double mx = 0;
+        for (const cd &v : it->second) mx = std::max(mx, std::abs(v));
+        if (mx < 1e-12)
+            it = C.erase(it);
+        else
+            ++it;
+    }
+    return C;
+}
+
+// Turn the composed diagonals of stages [lo, hi) into a BSGS level on N/2 slots.
+// in_mask multiplies the element read at slot s + offset, out_mask the output
+// slot s (both length N/2 or empty).
+LinearLevel make_level(const DiagMap &D, size_t n, size_t Nh, int lo, const cvec &in_mask, const cvec &out_mask,
+                       double factor)
+{
+    LinearLevel lv;
+    lv.stride = 1 << lo;
+    const long M = (long)(n / (size_t)lv.stride);  // offsets are t * stride, t taken mod M
+    std::map<long, const cvec *> byt;
+    long tmin = 0, tmax = 0;
+    for (const auto &d : D) {
+        if (d.first % lv.stride) throw std::logic_error("bootstrap plan: offset is not a multiple of the stride");
+        long t = d.first / lv.stride;
+        if (t > M / 2) t -= M;
+        byt[t] = &d.second;
+        tmin = std::min(tmin, t);
+        tmax = std::max(tmax, t);
+    }
+    const long span = tmax - tmin + 1;
+    int lg = 0;
+    while ((1L << (2 * (lg + 1))) <= span) lg++;  // g = 2^floor(log4 span) .. keep g ~ sqrt(span)
+    if ((1L << (2 * lg + 1)) < span) lg++;
+    lv.g = 1 << lg;
+    for (const auto &e : byt) {
+        const long t = e.first;
+        const long j = (t >= 0) ? t / lv.g : -((-t + lv.g - 1) / lv.g);
+        const long i = t - j * lv.g;
+        const cvec &d = *e.second;
+        // tiled, masked diagonal over N/2 slots
+        cvec full(Nh);
+        const long off = t * lv.stride;
+        for (size_t s = 0; s < Nh; s++) {
+            cd v = d[s % n] * factor;
+            if (!in_mask.empty()) v *= in_mask[(size_t)(((long)s + off) % (long)Nh + (long)Nh) % Nh];
+            if (!out_mask.empty()) v *= out_mask[s];
+            full[s] = v;
+        }
+        // pre-rotate by -g*j*stride: Dt[s] = D[s - g j stride]
+        const long sh = j * lv.g * lv.stride;
+        cvec rot(Nh);
+        for (size_t s = 0; s < Nh; s++) rot[s] = full[(size_t)((((long)s - sh) % (long)Nh) + (long)Nh) % Nh];
+        lv.diag[{(int)j, (int)i}] = std::move(rot);
+    }
+    return lv;
+}
+
+// Split logn stages into three consecutive groups (fewer when logn < 3); the
+// group holding the top stage gets the extra stages: its offsets wrap mod n,
+// so it has half as many diagonals per stage count.
+std::vector<std::pair<int, int>> stage_groups(int logn)
+{
+    const int parts = std::min(3, logn);
+    std::vector<std::pair<int, int>> g;  // [lo, hi) in increasing stage order
+    int base = logn / parts, extra = logn % parts, lo = 0;
+    for (int p = 0; p < parts; p++) {
+        const int sz = base + (p >= parts - extra ? 1 : 0);
+        g.push_back({lo, lo + sz});
+        lo += sz;
+    }
+    return g;
+}
+
+DiagMap group_diags(size_t n, int lo, int hi, bool inverse)
+{
+    DiagMap G;
+    bool first = true;
+    if (!inverse) {
+        for (int i = lo; i < hi; i++) {
+            DiagMap S = stage_diags(n, i, false);
+            G = first ? S : compose(S, G, n);
+            first = false;
+        }
+    } else {
+        for (int i = hi - 1; i >= lo; i--) {
+            DiagMap S = stage_diags(n, i, true);
+            G = first ? S : compose(S, G, n);
+            first = false;
+        }
+    }
+    return G;
+}
+
+}  // namespace
+
+BootPlan make_boot_plan(int logNh, int logn, double K)
+{
+    if (logn < 1 || logn > logNh) throw std::invalid_argument("bootstrap plan: logn out of range");
+    BootPlan P;
+    P.logn = logn;
+    P.logNh = logNh;
+    P.K = K;
+    const size_t n = size_t(1) << logn, Nh = size_t(1) << logNh;
+    const bool sparse = logn < logNh;
+    const double c = (double)(Nh / n);
+    const auto groups = stage_groups(logn);
+
+    // CoeffToSlot: inverse stages logn-1 .. 0, i.e. groups from the top down.
+    // The last level carries 1/(c K) and the real/imaginary split mask
+    // (1/2 on the lower n slots of each 2n period, -i/2 on the upper n;
+    // dense packing: 1/2 everywhere, split by conjugation afterwards).
+    cvec A(Nh);
+    for (size_t s = 0; s < Nh; s++) A[s] = (!sparse || (s % (2 * n)) < n) ? cd(0.5, 0) : cd(0, -0.5);
+    for (int gi = (int)groups.size() - 1; gi >= 0; gi--) {
+        const auto &g = groups[gi];
+        DiagMap D = group_diags(n, g.first, g.second, true);
+        const bool last = gi == 0;
+        P.cts.push_back(make_level(D, n, Nh, g.first, {}, last ? A : cvec(), last ? 1.0 / (c * K) : 1.0));
+    }
+    // SlotToCoeff: forward stages 0 .. logn-1.  The first level carries
+    // 1/(2 pi) (EvalMod returns sin(2 pi x)) and, for sparse packing, the
+    // recombination mask (1 on the lower n slots of each 2n period, i on the
+    // upper n) applied to the input it reads.
+    cvec B1;
+    if (sparse) {
+        B1.resize(Nh);
+        for (size_t s = 0; s < Nh; s++) B1[s] = ((s % (2 * n)) < n) ? cd(1, 0) : cd(0, 1);
+    }
+    for (size_t gi = 0; gi < groups.size(); gi++) {
+        const auto &g = groups[gi];
+        DiagMap D = group_diags(n, g.first, g.second, false);
+        const bool first = gi == 0;
+        P.stc.push_back(make_level(D, n, Nh, g.first, first ? B1 : cvec(), {}, first ? 1.0 / (2.0 * M_PI) : 1.0));
+    }
+
+    std::set<int> steps;
+    steps.insert(0);  // conjugation
+    auto add_level_steps = [&](const LinearLevel &lv) {
+        for (const auto &d : lv.diag) {
+            const int j = d.first.first, i = d.first.second;
+            if (i) steps.insert(i * lv.stride);
+            if (j) steps.insert(j * lv.g * lv.stride);
+        }
+    };
+    for (const auto &lv : P.cts) add_level_steps(lv);
+    for (const auto &lv : P.stc) add_level_steps(lv);
+    if (sparse)
+        for (int i = logn; i < logNh; i++) steps.insert(1 << i);  // SubSum and the final rot(y, n)
+    P.rotation_steps.assign(steps.begin(), steps.end());
+    return P;
+}
+
+// ---------------------------------------------------------------- Bootstrapper
+
+Bootstrapper::Bootstrapper(CKKSEvaluator &ckks, const Params &p) : ck_(ckks), p_(p)
+{
+    const int logNh = ck_.context->log_n() - 1;
+    if (p_.logn == 0) p_.logn = logNh;
+    if (p_.cheb_degree < 2) throw std::invalid_argument("Bootstrapper: degree too small");
+    if (p_.double_angles < 0) throw std::invalid_argument("Bootstrapper: double_angles < 0");
+    plan_ = make_boot_plan(logNh, p_.logn, (double)p_.K);
+
+    // Chebyshev interpolant of cos(2 pi (K u - 1/4) / 2^r) on [-1, 1] at the
+    // d+1 Chebyshev nodes (stands in for the reference's minimax generator,
+    // RemezCos, cuda/src/bootstrapping/RemezCos.cuh:12-17).
+    const int d = p_.cheb_degree;
+    const double K = p_.K, R = std::ldexp(1.0, p_.double_angles);
+    std::vector<double> fx(d + 1);
+    for (int j = 0; j <= d; j++) {
+        const double x = std::cos(M_PI * (j + 0.5) / (d + 1));
+        fx[j] = std::cos(2.0 * M_PI * (K * x - 0.25) / R);
+    }
+    cheb_.assign(d + 1, 0.0);
+    for (int k = 0; k <= d; k++) {
+        double s = 0;
+        for (int j = 0; j <= d; j++) s += fx[j] * std::cos(M_PI * k * (j + 0.5) / (d + 1));
+        cheb_[k] = s * 2.0 / (d + 1);
+    }
+    cheb_[0] *= 0.5;
+}
+
+static int floor_log2(int v)
+{
+    int r = 0;
+    while ((2 << r) <= v) r++;
+    return r;
+}
+
+int Bootstrapper::depth() const
+{
+    const int d = p_.cheb_degree;
+    const int cheb_depth = d < 8 ? floor_log2(std::max(d, 1)) + 2 : floor_log2(d) + 2;
+    return (int)plan_.cts.size() + cheb_depth + p_.double_angles + (int)plan_.stc.size();
+}
+
+void Bootstrapper::mod_raise(const Ciphertext &in, Ciphertext &out) const
+{
+    Ciphertext low = in;
+    if (low.coeff_modulus_size() > 1) ck_.evaluator.mod_switch_to_inplace(low, ck_.context->data_primes());
+    Ciphertext r(*ck_.context);
+    check(nx_mod_raise(low.get(), r.get()));
+    // ModRaise leaves m + q_0 I: read it at scale q_0 (Bootstrapper.cu:2997-2998)
+    r.set_scale(prime(1));
+    out = std::move(r);
+}
+
+void Bootstrapper::sub_sum(Ciphertext &ct) const
+{
+    // Bootstrapper.cu:3004-3008
+    for (int i = plan_.logn; i < plan_.logNh; i++) {
+        Ciphertext rot;
+        ck_.evaluator.rotate_vector(ct, 1 << i, *ck_.galois_keys, rot);
+        ck_.evaluator.add_inplace(ct, rot);
+    }
+}
+
+Ciphertext Bootstrapper::match(const Ciphertext &ct, size_t level, double scale)
+{
+    // bring ct to `level` limbs at exactly `scale`: multiply by 1 encoded at
+    // the scale that the following rescale compensates
+    const size_t l = ct.coeff_modulus_size();
+    if (l <= level) throw std::logic_error("Bootstrapper: operand below the target level");
+    Ciphertext t = ct;
+    if (l > level + 1) ck_.evaluator.mod_switch_to_inplace(t, chain_of(level + 1));
+    const double sigma = scale * prime(level + 1) / t.scale();
+    Plaintext one(*ck_.context);
+    ck_.encoder.encode(1.0, chain_of(level + 1), sigma, one);
+    ck_.evaluator.multiply_plain_inplace(t, one);
+    ck_.evaluator.rescale_to_next_inplace(t);
+    t.set_scale(scale);
+    return t;
+}
+
+Ciphertext Bootstrapper::mul_exact(const Ciphertext &a, const Ciphertext &b)
+{
+    // product of two ciphertexts at the lower of their levels, relinearised and rescaled
+    const size_t la = a.coeff_modulus_size(), lb = b.coeff_modulus_size();
+    const size_t l = std::min(la, lb);
+    Ciphertext x = a, y = b;
+    if (la > l) ck_.evaluator.mod_switch_to_inplace(x, chain_of(l));
+    if (lb > l) ck_.evaluator.mod_switch_to_inplace(y, chain_of(l));
+    Ciphertext r;
+    if (&a == &b)
+        ck_.evaluator.square(x, r);
+    else
+        ck_.evaluator.multiply(x, y, r);
+    ck_.evaluator.relinearize_inplace(r, *ck_.relin_keys);
+    ck_.evaluator.rescale_to_next_inplace(r);
+    return r;
+}
+
+// Chebyshev basis T_k of the EvalMod input (basis_[1]); T_{a+b} = 2 T_a T_b - T_{a-b}
+// with a = 2^(ceil(log2 k) - 1), so T_k sits ceil(log2 k) levels below the input.
+// T_{a-b} is brought to the product's level and scale exactly (match), so no
+// scale mismatch enters the subtraction.
+const Ciphertext &Bootstrapper::basis(int k)
+{
+    auto it = basis_.find(k);
+    if (it != basis_.end()) return it->second;
+    int a = 1;
+    while (2 * a < k) a *= 2;
+    const int b = k - a;
+    const Ciphertext &Ta = basis(a);
+    const Ciphertext &Tb = basis(b);
+    Ciphertext prod = (a == b) ? mul_exact(Ta, Ta) : mul_exact(Ta, Tb);
+    ck_.evaluator.add_inplace(prod, prod);  // 2 T_a T_b
+    const size_t l = prod.coeff_modulus_size();
+    if (a == b) {
+        Plaintext one(*ck_.context);
+        ck_.encoder.encode(1.0, chain_of(l), prod.scale(), one);
+        ck_.evaluator.sub_plain_inplace(prod, one);
+    } else {
+        Ciphertext Td = match(basis(a - b), l, prod.scale());
+        ck_.evaluator.sub_inplace(prod, Td);
+    }
+    return basis_.emplace(k, std::move(prod)).first->second;
+}
+
+// p(u) = sum_k c_k T_k(u) at exactly (level, scale).  deg < 8: linear
+// combination of the baby steps; otherwise p = q T_{2^m} + r (Chebyshev
+// division by T_{2^m}, 2^m <= deg < 2^(m+1)).  The quotient is evaluated one
+// level higher at the scale that the product's rescale maps onto `scale`.
+Ciphertext Bootstrapper::cheb_eval(const std::vector<double> &c, size_t level, double scale)
+{
+    int d = (int)c.size() - 1;
+    while (d > 0 && c[d] == 0.0) d--;
+    if (d < 8) {
+        const size_t lam = level + 1;
+        const double q = prime(lam);
+        Ciphertext acc;
+        bool have = false;
+        for (int k = 1; k <= d; k++) {
+            if (c[k] == 0.0) continue;
+            Ciphertext t = basis(k);
+            if (t.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: basis below the leaf level");
+            if (t.coeff_modulus_size() > lam) ck_.evaluator.mod_switch_to_inplace(t, chain_of(lam));
+            Plaintext ck(*ck_.context);
+            ck_.encoder.encode(c[k], chain_of(lam), scale * q / t.scale(), ck);
+            ck_.evaluator.multiply_plain_inplace(t, ck);
+            if (!have) {
+                acc = std::move(t);
+                have = true;
+            } else {
+                t.set_scale(acc.scale());
+                ck_.evaluator.add_inplace(acc, t);
+            }
+        }
+        if (!have) {  // constant polynomial: 0 * T_1 carries the level and scale
+            acc = basis(1);
+            ck_.evaluator.mod_switch_to_inplace(acc, chain_of(lam));
+            Plaintext z(*ck_.context);
+            ck_.encoder.encode(0.0, chain_of(lam), scale * q / acc.scale(), z);
+            ck_.evaluator.multiply_plain_inplace(acc, z);
+        }
+        ck_.evaluator.rescale_to_next_inplace(acc);
+        acc.set_scale(scale);
+        if (c[0] != 0.0) {
+            Plaintext c0(*ck_.context);
+            ck_.encoder.encode(c[0], chain_of(level), scale, c0);
+            ck_.evaluator.add_plain_inplace(acc, c0);
+        }
+        return acc;
+    }
+    int P = 1;
+    while (2 * P <= d) P *= 2;
+    std::vector<double> q(d - P + 1, 0.0), r(P, 0.0);
+    for (int k = 0; k < P; k++) r[k] = c[k];
+    q[0] = c[P];
+    for (int k = P + 1; k <= d; k++) {
+        q[k - P] += 2.0 * c[k];
+        r[2 * P - k] -= c[k];
+    }
+    const Ciphertext &TP = basis(P);
+    const size_t lam = level + 1;
+    if (TP.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: giant step below the product level");
+    Ciphertext Q = cheb_eval(q, lam, scale * prime(lam) / TP.scale());
+    Ciphertext T = TP;
+    if (T.coeff_modulus_size() > lam) ck_.evaluator.mod_switch_to_inplace(T, chain_of(lam));
+    Ciphertext prod;
+    ck_.evaluator.multiply(Q, T, prod);
+    ck_.evaluator.relinearize_inplace(prod, *ck_.relin_keys);
+    ck_.evaluator.rescale_to_next_inplace(prod);
+    prod.set_scale(scale);
+    Ciphertext R = cheb_eval(r, level, scale);
+    R.set_scale(scale);
+    ck_.evaluator.add_inplace(prod, R);
+    return prod;
+}
+
+void Bootstrapper::eval_mod(const Ciphertext &u, Ciphertext &out)
+{
+    // ModularReducer::modular_reduction (cuda/src/bootstrapping/ModularReducer.cu:53-73):
+    // y = p(u) ~ cos(2 pi (K u - 1/4) / 2^r); r times y <- 2 y^2 - 1; y ~ sin(2 pi K u)
+    basis_.clear();
+    basis_.emplace(1, u);
+    const size_t l0 = u.coeff_modulus_size();
+    const int d = p_.cheb_degree;
+    const int cd_ = d < 8 ? floor_log2(std::max(d, 1)) + 2 : floor_log2(d) + 2;
+    if (l0 <= (size_t)cd_ + (size_t)p_.double_angles)
+        throw std::invalid_argument("Bootstrapper: not enough levels for EvalMod");
+    Ciphertext y = cheb_eval(cheb_, l0 - cd_, u.scale());
+    basis_.clear();
+    for (int i = 0; i < p_.double_angles; i++) {
+        ck_.evaluator.square_inplace(y);
+        ck_.evaluator.relinearize_inplace(y, *ck_.relin_keys);
+        ck_.evaluator.rescale_to_next_inplace(y);
+        ck_.evaluator.add_inplace(y, y);
+        Plaintext one(*ck_.context);
+        ck_.encoder.encode(1.0, y.chain_index(), y.scale(), one);
+        ck_.evaluator.sub_plain_inplace(y, one);
+    }
+    out = std::move(y);
+}
+
+Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphertext &x, double out_scale,
+                                     double value_factor)
+{
+    // bsgs_linear_transform (Bootstrapper.cu:1855-1919): baby rotations of x,
+    // per giant step a plaintext-weighted sum and one rotation, one rescale.
+    const size_t l = x.coeff_modulus_size();
+    const double sigma = out_scale * prime(l) / x.scale();
+    std::map<int, Ciphertext> baby;
+    for (const auto &d : lv.diag) {
+        const int i = d.first.second;
+        if (baby.count(i)) continue;
+        if (i == 0)
+            baby.emplace(0, x);
+        else {
+            Ciphertext r;
+            ck_.evaluator.rotate_vector(x, i * lv.stride, *ck_.galois_keys, r);
+            baby.emplace(i, std::move(r));
+        }
+    }
+    Ciphertext result;
+    bool have_result = false;
+    auto it = lv.diag.begin();
+    while (it != lv.diag.end()) {
+        const int j = it->first.first;
+        Ciphertext acc;
+        bool have = false;
+        for (; it != lv.diag.end() && it->first.first == j; ++it) {
+            const int i = it->first.second;
+            auto key = std::make_tuple(tag, j, i, l, sigma * value_factor);
+            auto pit = ptcache_.find(key);
+            if (pit == ptcache_.end()) {
+                Plaintext pt(*ck_.context);
+                ck_.encoder.encode(it->second, chain_of(l), sigma * value_factor, pt);
+                pit = ptcache_.emplace(key, std::move(pt)).first;
+            }
+            Ciphertext t;
+            ck_.evaluator.multiply_plain(baby.at(i), pit->second, t);
+            if (!have) {
+                acc = std::move(t);
+                have = true;
+            } else {
+                ck_.evaluator.add_inplace(acc, t);
+            }
+        }
+        if (j != 0) ck_.evaluator.rotate_vector_inplace(acc, j * lv.g * lv.stride, *ck_.galois_keys);
+        if (!have_result) {
+            result = std::move(acc);
+            have_result = true;
+        } else {
+            ck_.evaluator.add_inplace(result, acc);
+        }
+    }
+    ck_.evaluator.rescale_to_next_inplace(result);
+    result.set_scale(out_scale);
+    return result;
+}
+
+void Bootstrapper::coeff_to_slot(const Ciphertext &in, Ciphertext &u_re, Ciphertext &u_im)
+{
+    // coefftoslot_3 / sflinv_3 (Bootstrapper.cu:2435-2458, 2581-2586)
+    Ciphertext x = in;
+    for (size_t k = 0; k < plan_.cts.size(); k++) x = apply_level(plan_.cts[k], (int)k, x, x.scale(), 1.0);
+    Ciphertext xc;
+    ck_.evaluator.complex_conjugate(x, *ck_.galois_keys, xc);
+    if (plan_.logn < plan_.logNh) {
+        ck_.evaluator.add(x, xc, u_re);  // lower half: Re, upper half: Im (mask folded above)
+        u_im = Ciphertext();
+    } else {
+        // dense packing (coefftoslot_full_3): Re = x + conj x, Im = -i (x - conj x)
+        ck_.evaluator.add(x, xc, u_re);
+        Ciphertext d;
+        ck_.evaluator.sub(x, xc, d);
+        const int64_t N = (int64_t)ck_.context->poly_degree();
+        ck_.evaluator.multiply_power_of_x(d, 3 * N / 2, u_im);  // X^(3N/2): every slot times -i
+    }
+}
+
+void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out)
+{
+    // slottocoeff_3 / sfl_3 (Bootstrapper.cu:2275-2314, 2589-2594)
+    Ciphertext x = f_re;
+    if (f_im) {  // dense packing: f_re + i f_im, X^(N/2) multiplies every slot by i
+        Ciphertext t;
+        const int64_t N = (int64_t)ck_.context->poly_degree();
+        ck_.evaluator.multiply_power_of_x(*f_im, N / 2, t);
+        t.set_scale(x.scale());
+        ck_.evaluator.add_inplace(x, t);
+    }
+    const double final_scale = p_.final_scale > 0 ? p_.final_scale : input_scale;
+    const double q0 = prime(1);
+    for (size_t k = 0; k < plan_.stc.size(); k++) {
+        const bool last = k + 1 == plan_.stc.size();
+        // the last level maps the EvalMod output (m / q_0) back to the message at final_scale
+        x = apply_level(plan_.stc[k], 100 + (int)k, x, last ? final_scale : x.scale(), last ? q0 / input_scale : 1.0);
+    }
+    if (plan_.logn < plan_.logNh) {
+        Ciphertext r;
+        ck_.evaluator.rotate_vector(x, 1 << plan_.logn, *ck_.galois_keys, r);
+        ck_.evaluator.add_inplace(x, r);
+    }
+    x.set_scale(final_scale);
+    out = std::move(x);
+}
+
+void Bootstrapper::bootstrap(const Ciphertext &in, Ciphertext &out)
+{
+    // bootstrap_3 (Bootstrapper.cu:3358-3370) -> bootstrap_sparse_3 / bootstrap_full_3
+    if (!in.get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
+    const double input_scale = in.scale();
+    Ciphertext ct;
+    mod_raise(in, ct);
+    if (plan_.logn < plan_.logNh) sub_sum(ct);
+    Ciphertext u_re, u_im;
+    coeff_to_slot(ct, u_re, u_im);
+    Ciphertext f_re, f_im;
+    eval_mod(u_re, f_re);
+    if (u_im.get()) eval_mod(u_im, f_im);
+    slot_to_coeff(f_re, u_im.get() ? &f_im : nullptr, input_scale, out);
+}
+
+}  // namespace nexus

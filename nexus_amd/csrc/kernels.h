@@ -117,6 +117,9 @@ void poly_mul2_add(const DeviceTables &t, uint64_t *out, const uint64_t *a, cons
 // (E: [cols*inner][2][limbs][n] expanded ciphertexts, A: [inner][limbs][n] plaintexts)
 void mm_accumulate(const DeviceTables &t, uint64_t *out, const uint64_t *E, const uint64_t *A, uint32_t cols,
                    uint32_t inner, uint32_t limbs, hipStream_t s);
+// ModRaise: in0 = coefficient-domain q0 limb of each poly ([polys][n]) -> out [polys][limbs][n]
+void mod_raise(const DeviceTables &t, uint64_t *out, const uint64_t *in0, uint32_t polys, uint32_t limbs,
+               hipStream_t s);
 // Negacyclic shift of coefficient-domain limbs: out = a * x^k mod (x^n + 1), 0 <= k < 2n.
 void poly_negacyclic_shift(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t k, uint32_t polys,
                            uint32_t limbs, hipStream_t s);

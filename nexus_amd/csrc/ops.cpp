@@ -513,6 +513,30 @@ void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &d
     dst = std::move(r);
 }
 
+void op_mod_raise(const Ciphertext &a, Ciphertext &dst)
+{
+    // src/bootstrapping/Bootstrapper.cpp:2937-2991: lowest-level ciphertext -> top data level
+    valid(a);
+    require(a.size == 2, "Ciphertexts of size 2 are supported only!");
+    require(a.limbs == 1, "Ciphertexts in the lowest level are supported only!");
+    Context &c = *a.ctx;
+    const size_t n = c.n, L = c.L;
+    DevBuf low(&c, 2 * n);
+    d2d(c, low.get(), a.data.get(), 2 * n);
+    if (a.ntt) ntt_inv(c, low.get(), 2, 1, n);
+    DevBuf out(&c, 2 * L * n);
+    mod_raise(c.tables, out.get(), low.get(), 2, (uint32_t)L, c.stream);
+    ntt_fwd(c, out.get(), 2, L, L * n);
+    Ciphertext r;
+    r.ctx = &c;
+    r.size = 2;
+    r.limbs = L;
+    r.scale = a.scale;
+    r.ntt = true;
+    r.data = std::move(out);
+    dst = std::move(r);
+}
+
 void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst)
 {
     require(!in.empty(), "encrypteds cannot be empty");
