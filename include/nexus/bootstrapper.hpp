@@ -4,8 +4,9 @@
 // Pipeline (as the reference's bootstrap_sparse_3 / bootstrap_full_3,
 // cuda/src/bootstrapping/Bootstrapper.cu:2987-3113):
 //   ModRaise -> SubSum (sparse packing) -> CoeffToSlot (3 BSGS levels) ->
-//   EvalMod (Chebyshev interpolant of cos(2 pi (K u - 1/4) / 2^r), then r
-//   double-angle steps) -> SlotToCoeff (3 BSGS levels).
+//   EvalMod (Chebyshev-basis fit of cos(2 pi (K u - 1/4) / 2^r) on the
+//   windows |K u - i| <= 2^-log_width, then r double-angle steps) ->
+//   SlotToCoeff (3 BSGS levels).
 // The linear transforms are the special-FFT factorisation of the slot map
 // (butterfly stages with twiddles exp(2 pi i 5^j / 4 len)) grouped into three
 // levels in diagonal form; the bit-reversal permutations of CoeffToSlot and
@@ -42,13 +43,19 @@ struct BootPlan {
 };
 BootPlan make_boot_plan(int logNh, int logn, double K);
 
+// Chebyshev coefficients of the EvalMod polynomial and the levels its
+// evaluation consumes (host only).
+std::vector<double> eval_mod_coeffs(int K, int degree, int double_angles, int log_width);
+int chebyshev_depth(const std::vector<double> &c);
+
 class Bootstrapper {
 public:
     struct Params {
         int logn = 0;            // log2 of the slot count (logn < logNh: sparse packing)
         int K = 25;              // bound of the ModRaise overflow |I| (reference boundary_K)
-        int cheb_degree = 63;    // EvalMod interpolant degree (depth floor(log2 d) + 2)
+        int cheb_degree = 59;    // EvalMod polynomial degree (reference sin_cos_deg; 59 fits depth 6)
         int double_angles = 2;   // r (reference scale_factor = 2)
+        int log_width = 10;      // fit windows |K u - i| <= 2^-log_width (reference loge)
         double final_scale = 0;  // output scale (0: the input's scale)
     };
 
@@ -63,6 +70,8 @@ public:
     const std::vector<double> &cheb_coeffs() const { return cheb_; }
 
     void bootstrap(const Ciphertext &in, Ciphertext &out);
+    // Bootstrapper::set_final_scale (reference Bootstrapper.h)
+    void set_final_scale(double s) { p_.final_scale = s; }
 
     // stages (exposed for tests)
     void mod_raise(const Ciphertext &in, Ciphertext &out) const;
@@ -87,6 +96,20 @@ private:
     std::vector<double> cheb_;
     std::map<int, Ciphertext> basis_;  // Chebyshev T_k of the current EvalMod input
     std::map<std::tuple<int, int, int, size_t, double>, Plaintext> ptcache_;
+};
+
+// ArgmaxEvaluator (reference src/argmax.{h,cpp}, cuda/src/argmax.cu): log2(len)
+// rounds of max(a, b) = (a + b)/2 + (a - b) sgn(a - b)/2 with a bootstrap per
+// round, then the one-hot indicator sgn(x - max) + 1.
+class ArgmaxEvaluator {
+public:
+    ArgmaxEvaluator(CKKSEvaluator &ckks, Bootstrapper &bootstrapper) : ckks_(&ckks), bt_(&bootstrapper) {}
+    void argmax(Ciphertext &x, Ciphertext &res, int len);
+    void bootstrap(Ciphertext &x);
+
+private:
+    CKKSEvaluator *ckks_;
+    Bootstrapper *bt_;
 };
 
 }  // namespace nexus

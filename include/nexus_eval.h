@@ -47,8 +47,8 @@ int nx_eval_softmax(nx_evaluator *ev, const nx_ciphertext *x, int len, nx_cipher
  * evaluator must hold the relinearisation key and Galois keys for every step
  * of nx_bootstrapper_rotation_steps. */
 typedef struct nx_bootstrapper nx_bootstrapper;
-int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, int double_angles, double final_scale,
-                           nx_bootstrapper **out);
+int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, int double_angles, int log_width,
+                           double final_scale, nx_bootstrapper **out);
 int nx_bootstrapper_destroy(nx_bootstrapper *bt);
 /* addLeftRotKeys_Linear_to_vector_3 (Bootstrapper.cu:82-178): rotation steps
  * (0 = conjugation); *count receives the number, at most `capacity` written */
@@ -68,6 +68,11 @@ int nx_boot_eval_mod(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext
 int nx_boot_slot_to_coeff(nx_bootstrapper *bt, const nx_ciphertext *f_re, const nx_ciphertext *f_im,
                           double input_scale, nx_ciphertext *out);
 
+/* ArgmaxEvaluator::argmax (src/argmax.cpp:9-64): one-hot indicator of the
+ * maximum of the first len (power of two) slots, bootstrapping once per
+ * round; x is left unchanged */
+int nx_eval_argmax(nx_bootstrapper *bt, const nx_ciphertext *x, int len, nx_ciphertext *out);
+
 /* Host-only view of the linear-transform plan (genfftcoeff_3 / geninvfftcoeff_3,
  * Bootstrapper.cu:1137,1545); no device use.  which = 0: CoeffToSlot, 1:
  * SlotToCoeff; level in application order.  A level computes
@@ -77,6 +82,9 @@ int nx_boot_plan_info(int log_half_n, int log_slots, double K, int which, int le
 /* diagonals: js[count], is[count], re/im[count][N/2] */
 int nx_boot_plan_diagonals(int log_half_n, int log_slots, double K, int which, int level, int *js, int *is,
                            double *re, double *im);
+/* EvalMod polynomial (Chebyshev basis on [-1, 1], degree+1 coefficients) and
+ * the levels its evaluation consumes */
+int nx_boot_eval_mod_coeffs(int K, int degree, int double_angles, int log_width, double *coeffs, int *depth);
 /* rotation steps of a plan (0 = conjugation) */
 int nx_boot_plan_steps(int log_half_n, int log_slots, double K, int *steps, size_t capacity, size_t *count);
 

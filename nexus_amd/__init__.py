@@ -521,8 +521,8 @@ class CKKSEvaluator:
 
 # ---------------------------------------------------------------- bootstrapping (include/nexus_eval.h)
 _ip = ctypes.POINTER(ctypes.c_int)
-_sig("nx_bootstrapper_create", [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
-                                ctypes.POINTER(vp)])
+_sig("nx_bootstrapper_create", [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_double, ctypes.POINTER(vp)])
 _sig("nx_bootstrapper_destroy", [vp])
 _sig("nx_bootstrapper_rotation_steps", [vp, _ip, sz, ctypes.POINTER(sz)])
 _sig("nx_bootstrapper_depth", [vp, _ip])
@@ -532,10 +532,12 @@ _sig("nx_boot_sub_sum", [vp, vp, vp])
 _sig("nx_boot_coeff_to_slot", [vp, vp, vp, vp])
 _sig("nx_boot_eval_mod", [vp, vp, vp])
 _sig("nx_boot_slot_to_coeff", [vp, vp, vp, ctypes.c_double, vp])
+_sig("nx_eval_argmax", [vp, vp, ctypes.c_int, vp])
 _sig("nx_boot_plan_info", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _ip, _ip,
                            ctypes.POINTER(sz)])
 _sig("nx_boot_plan_diagonals", [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _ip, _ip,
                                 f64p, f64p])
+_sig("nx_boot_eval_mod_coeffs", [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, f64p, _ip])
 _sig("nx_boot_plan_steps", [ctypes.c_int, ctypes.c_int, ctypes.c_double, _ip, sz, ctypes.POINTER(sz)])
 
 
@@ -546,6 +548,14 @@ def boot_plan_steps(log_half_n, log_slots, K=25):
     buf = (ctypes.c_int * n.value)()
     _check(_lib.nx_boot_plan_steps(log_half_n, log_slots, float(K), buf, n.value, ctypes.byref(n)))
     return list(buf)
+
+
+def boot_eval_mod_coeffs(K=25, degree=59, double_angles=2, log_width=10):
+    """(Chebyshev coefficients of the EvalMod polynomial, levels it consumes); host only."""
+    c = np.zeros(degree + 1)
+    d = ctypes.c_int()
+    _check(_lib.nx_boot_eval_mod_coeffs(K, degree, double_angles, log_width, c.ctypes.data_as(f64p), ctypes.byref(d)))
+    return c, d.value
 
 
 def boot_plan_level(log_half_n, log_slots, which, level, K=25):
@@ -571,11 +581,12 @@ class Bootstrapper:
     CoeffToSlot, EvalMod, SlotToCoeff.  `ev` must hold relinearisation keys and
     Galois keys for every step of `rotation_steps()`."""
 
-    def __init__(self, ev, log_slots=0, K=25, degree=63, double_angles=2, final_scale=0.0):
+    def __init__(self, ev, log_slots=0, K=25, degree=59, double_angles=2, log_width=10, final_scale=0.0):
         self.ev = ev
         self.ctx = ev.ctx
         h = vp()
-        _check(_lib.nx_bootstrapper_create(ev.h, log_slots, K, degree, double_angles, final_scale, ctypes.byref(h)))
+        _check(_lib.nx_bootstrapper_create(ev.h, log_slots, K, degree, double_angles, log_width, final_scale,
+                                           ctypes.byref(h)))
         self.h = h
         self.log_slots = log_slots or self.ctx.log_n - 1
         self.dense = self.log_slots == self.ctx.log_n - 1
@@ -617,6 +628,12 @@ class Bootstrapper:
 
     def eval_mod(self, ct):
         return self._run(_lib.nx_boot_eval_mod, ct.h)
+
+    def argmax(self, x, length):
+        """ArgmaxEvaluator::argmax (src/argmax.cpp:9-64) with this bootstrapper."""
+        out = Ciphertext(self.ctx)
+        _check(_lib.nx_eval_argmax(self.h, x.h, length, out.h))
+        return out
 
     def slot_to_coeff(self, f_re, f_im, input_scale):
         out = Ciphertext(self.ctx)

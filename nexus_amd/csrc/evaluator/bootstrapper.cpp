@@ -171,7 +171,107 @@ DiagMap group_diags(size_t n, int lo, int hi, bool inverse)
     return G;
 }
 
+// Least squares min |A x - b| (A m x k, row-major, m >= k) by Householder QR.
+std::vector<double> lstsq(std::vector<double> A, std::vector<double> b, size_t m, size_t k)
+{
+    for (size_t j = 0; j < k; j++) {
+        double norm = 0;
+        for (size_t i = j; i < m; i++) norm += A[i * k + j] * A[i * k + j];
+        norm = std::sqrt(norm);
+        if (norm == 0) continue;
+        const double alpha = A[j * k + j] > 0 ? -norm : norm;
+        std::vector<double> v(m - j);
+        for (size_t i = j; i < m; i++) v[i - j] = A[i * k + j];
+        v[0] -= alpha;
+        double vn = 0;
+        for (double x : v) vn += x * x;
+        if (vn == 0) continue;
+        for (size_t c = j; c < k; c++) {
+            double sdot = 0;
+            for (size_t i = j; i < m; i++) sdot += v[i - j] * A[i * k + c];
+            const double f = 2 * sdot / vn;
+            for (size_t i = j; i < m; i++) A[i * k + c] -= f * v[i - j];
+        }
+        double sdot = 0;
+        for (size_t i = j; i < m; i++) sdot += v[i - j] * b[i];
+        const double f = 2 * sdot / vn;
+        for (size_t i = j; i < m; i++) b[i] -= f * v[i - j];
+    }
+    std::vector<double> x(k, 0.0);
+    for (size_t j = k; j-- > 0;) {
+        double s = b[j];
+        for (size_t c = j + 1; c < k; c++) s -= A[j * k + c] * x[c];
+        x[j] = s / A[j * k + j];
+    }
+    return x;
+}
+
+// Chebyshev coefficients of the EvalMod polynomial: fit of
+// cos(2 pi (K u - 1/4) / 2^r) at 40 Chebyshev-distributed points in each
+// window [(i - w) / K, (i + w) / K], w = 2^-log_width, |i| < K.
+std::vector<double> fit_eval_mod(int K, int degree, int r, int log_width)
+{
+    if (degree < 1) throw std::invalid_argument("Bootstrapper: degree too small");
+    const double w = std::ldexp(1.0, -log_width), R = std::ldexp(1.0, r);
+    const int per = 40;
+    std::vector<double> xs;
+    for (int i = -K + 1; i <= K - 1; i++)
+        for (int t = 0; t < per; t++) xs.push_back((i + w * std::cos(M_PI * (t + 0.5) / per)) / K);
+    const size_t m = xs.size(), k = (size_t)degree + 1;
+    if (m < k) throw std::invalid_argument("Bootstrapper: degree too large for the fit");
+    std::vector<double> A(m * k), b(m);
+    for (size_t i = 0; i < m; i++) {
+        const double x = xs[i];
+        double t0 = 1, t1 = x;
+        for (size_t j = 0; j < k; j++) {
+            A[i * k + j] = j == 0 ? 1.0 : (j == 1 ? x : 0.0);
+            if (j >= 2) {
+                const double t2 = 2 * x * t1 - t0;
+                A[i * k + j] = t2;
+                t0 = t1;
+                t1 = t2;
+            }
+        }
+        b[i] = std::cos(2.0 * M_PI * (K * x - 0.25) / R);
+    }
+    return lstsq(std::move(A), std::move(b), m, k);
+}
+
+// ceil(log2 k): levels below the input at which T_k is produced
+int tdepth(int k)
+{
+    int d = 0;
+    while ((1 << d) < k) d++;
+    return d;
+}
+
+// split of p = sum c_k T_k by T_P, 2^floor(log2 deg) = P: p = q T_P + r
+// (T_{P+j} = 2 T_P T_j - T_{P-j})
+void cheb_split(const std::vector<double> &c, int d, int P, std::vector<double> &q, std::vector<double> &r)
+{
+    q.assign(d - P + 1, 0.0);
+    r.assign(P, 0.0);
+    for (int k = 0; k < P; k++) r[k] = c[k];
+    q[0] = c[P];
+    for (int k = P + 1; k <= d; k++) {
+        q[k - P] += 2.0 * c[k];
+        r[2 * P - k] -= c[k];
+    }
+}
+
+int trimmed_degree(const std::vector<double> &c)
+{
+    int d = (int)c.size() - 1;
+    while (d > 0 && c[d] == 0.0) d--;
+    return d;
+}
+
 }  // namespace
+
+std::vector<double> eval_mod_coeffs(int K, int degree, int double_angles, int log_width)
+{
+    return fit_eval_mod(K, degree, double_angles, log_width);
+}
 
 BootPlan make_boot_plan(int logNh, int logn, double K)
 {
@@ -240,37 +340,38 @@ Bootstrapper::Bootstrapper(CKKSEvaluator &ckks, const Params &p) : ck_(ckks), p_
     if (p_.double_angles < 0) throw std::invalid_argument("Bootstrapper: double_angles < 0");
     plan_ = make_boot_plan(logNh, p_.logn, (double)p_.K);
 
-    // Chebyshev interpolant of cos(2 pi (K u - 1/4) / 2^r) on [-1, 1] at the
-    // d+1 Chebyshev nodes (stands in for the reference's minimax generator,
-    // RemezCos, cuda/src/bootstrapping/RemezCos.cuh:12-17).
-    const int d = p_.cheb_degree;
-    const double K = p_.K, R = std::ldexp(1.0, p_.double_angles);
-    std::vector<double> fx(d + 1);
-    for (int j = 0; j <= d; j++) {
-        const double x = std::cos(M_PI * (j + 0.5) / (d + 1));
-        fx[j] = std::cos(2.0 * M_PI * (K * x - 0.25) / R);
-    }
-    cheb_.assign(d + 1, 0.0);
-    for (int k = 0; k <= d; k++) {
-        double s = 0;
-        for (int j = 0; j <= d; j++) s += fx[j] * std::cos(M_PI * k * (j + 0.5) / (d + 1));
-        cheb_[k] = s * 2.0 / (d + 1);
-    }
-    cheb_[0] *= 0.5;
+    // EvalMod polynomial in the Chebyshev basis on [-1, 1]: least-squares fit
+    // of cos(2 pi (K u - 1/4) / 2^r) on the windows |K u - i| <= 2^-log_width,
+    // |i| < K, where the ModRaise overflow puts the inputs (stands in for the
+    // reference's minimax generator on the same windows, RemezCos,
+    // cuda/src/bootstrapping/RemezCos.cuh:12-17).
+    if (p_.K < 2 || p_.log_width < 1) throw std::invalid_argument("Bootstrapper: K >= 2 and log_width >= 1 required");
+    cheb_ = eval_mod_coeffs(p_.K, p_.cheb_degree, p_.double_angles, p_.log_width);
 }
 
-static int floor_log2(int v)
+// Levels consumed by cheb_eval for p: leaves (degree < 8) take the deepest
+// baby step plus one scalar level; a split p = q T_P + r takes one level over
+// the deeper of q and T_P, and r runs beside it.  Degree 59 fits in depth 6
+// (quotients of degree 27, 11, 3 ride on T_32, T_16, T_8).
+int chebyshev_depth(const std::vector<double> &c)
 {
-    int r = 0;
-    while ((2 << r) <= v) r++;
-    return r;
+    const int d = trimmed_degree(c);
+    if (d < 8) {
+        int m = 0;
+        for (int k = 1; k <= d; k++)
+            if (c[k] != 0.0) m = std::max(m, tdepth(k));
+        return m + 1;
+    }
+    int P = 1;
+    while (2 * P <= d) P *= 2;
+    std::vector<double> q, r;
+    cheb_split(c, d, P, q, r);
+    return std::max(std::max(chebyshev_depth(q), tdepth(P)) + 1, chebyshev_depth(r));
 }
 
 int Bootstrapper::depth() const
 {
-    const int d = p_.cheb_degree;
-    const int cheb_depth = d < 8 ? floor_log2(std::max(d, 1)) + 2 : floor_log2(d) + 2;
-    return (int)plan_.cts.size() + cheb_depth + p_.double_angles + (int)plan_.stc.size();
+    return (int)plan_.cts.size() + chebyshev_depth(cheb_) + p_.double_angles + (int)plan_.stc.size();
 }
 
 void Bootstrapper::mod_raise(const Ciphertext &in, Ciphertext &out) const
@@ -360,10 +461,10 @@ const Ciphertext &Bootstrapper::basis(int k)
 // combination of the baby steps; otherwise p = q T_{2^m} + r (Chebyshev
 // division by T_{2^m}, 2^m <= deg < 2^(m+1)).  The quotient is evaluated one
 // level higher at the scale that the product's rescale maps onto `scale`.
+// Feasible whenever cheb_depth(c) <= (input level - level).
 Ciphertext Bootstrapper::cheb_eval(const std::vector<double> &c, size_t level, double scale)
 {
-    int d = (int)c.size() - 1;
-    while (d > 0 && c[d] == 0.0) d--;
+    const int d = trimmed_degree(c);
     if (d < 8) {
         const size_t lam = level + 1;
         const double q = prime(lam);
@@ -403,13 +504,8 @@ Ciphertext Bootstrapper::cheb_eval(const std::vector<double> &c, size_t level, d
     }
     int P = 1;
     while (2 * P <= d) P *= 2;
-    std::vector<double> q(d - P + 1, 0.0), r(P, 0.0);
-    for (int k = 0; k < P; k++) r[k] = c[k];
-    q[0] = c[P];
-    for (int k = P + 1; k <= d; k++) {
-        q[k - P] += 2.0 * c[k];
-        r[2 * P - k] -= c[k];
-    }
+    std::vector<double> q, r;
+    cheb_split(c, d, P, q, r);
     const Ciphertext &TP = basis(P);
     const size_t lam = level + 1;
     if (TP.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: giant step below the product level");
@@ -434,8 +530,7 @@ void Bootstrapper::eval_mod(const Ciphertext &u, Ciphertext &out)
     basis_.clear();
     basis_.emplace(1, u);
     const size_t l0 = u.coeff_modulus_size();
-    const int d = p_.cheb_degree;
-    const int cd_ = d < 8 ? floor_log2(std::max(d, 1)) + 2 : floor_log2(d) + 2;
+    const int cd_ = chebyshev_depth(cheb_);
     if (l0 <= (size_t)cd_ + (size_t)p_.double_angles)
         throw std::invalid_argument("Bootstrapper: not enough levels for EvalMod");
     Ciphertext y = cheb_eval(cheb_, l0 - cd_, u.scale());
@@ -570,6 +665,72 @@ void Bootstrapper::bootstrap(const Ciphertext &in, Ciphertext &out)
     eval_mod(u_re, f_re);
     if (u_im.get()) eval_mod(u_im, f_im);
     slot_to_coeff(f_re, u_im.get() ? &f_im : nullptr, input_scale, out);
+}
+
+}  // namespace nexus
+
+namespace nexus {
+
+// src/argmax.cpp:9-64
+void ArgmaxEvaluator::argmax(Ciphertext &x, Ciphertext &res, int len)
+{
+    if (len < 1 || (len & (len - 1))) throw std::invalid_argument("argmax: len must be a power of two");
+    const Evaluator &ev = ckks_->evaluator;
+    int log_step = 0;
+    while ((1 << log_step) < len) log_step++;
+
+    // [a_0 .. a_len-1, 0 ..] -> [a_0 .. a_len-1, a_0 .. a_len-1, 0 ..]
+    Ciphertext tmp;
+    ev.rotate_vector(x, -len, *ckks_->galois_keys, tmp);
+    ev.add_inplace(x, tmp);
+
+    Ciphertext a = x;
+    for (int i = 0; i < log_step; ++i) {
+        Ciphertext b, a_plus_b, a_minus_b, a_minus_b_sgn;
+        ev.rotate_vector(a, 1 << i, *ckks_->galois_keys, b);
+        ev.add(a, b, a_plus_b);
+        ev.sub(a, b, a_minus_b);
+        Ciphertext sign = ckks_->sgn_eval(a_minus_b, 2, 2);
+
+        // (a - b) sgn(a - b) / 2 (the 1/2 is in sgn_eval's default factor)
+        ev.mod_switch_to_inplace(a_minus_b, sign.params_id());
+        ev.multiply(a_minus_b, sign, a_minus_b_sgn);
+        ev.relinearize_inplace(a_minus_b_sgn, *ckks_->relin_keys);
+        ev.rescale_to_next_inplace(a_minus_b_sgn);
+
+        // (a + b) / 2
+        Plaintext half(*ckks_->context);
+        ckks_->encoder.encode(0.5, a_plus_b.params_id(), a_plus_b.scale(), half);
+        ev.multiply_plain_inplace(a_plus_b, half);
+        ev.rescale_to_next_inplace(a_plus_b);
+
+        // a = max(a, b)
+        a_plus_b.set_scale(a_minus_b_sgn.scale());
+        ev.mod_switch_to_inplace(a_plus_b, a_minus_b_sgn.params_id());
+        ev.add(a_plus_b, a_minus_b_sgn, a);
+
+        bootstrap(a);
+    }
+
+    x.set_scale(a.scale());
+    ev.mod_switch_to_inplace(x, a.params_id());
+    ev.sub(x, a, res);
+
+    res = ckks_->sgn_eval(res, 2, 2, 1.0);
+
+    Plaintext one(*ckks_->context);
+    ckks_->encoder.encode(1.0, res.params_id(), res.scale(), one);
+    ev.add_plain_inplace(res, one);
+}
+
+// src/argmax.cpp:66-81: to the lowest level, bootstrap at the current scale
+void ArgmaxEvaluator::bootstrap(Ciphertext &x)
+{
+    const size_t L = ckks_->context->data_primes();
+    if (x.coeff_modulus_size() > 1) ckks_->evaluator.mod_switch_to_inplace(x, L);
+    Ciphertext x0 = x;
+    bt_->set_final_scale(x.scale());
+    bt_->bootstrap(x0, x);
 }
 
 }  // namespace nexus

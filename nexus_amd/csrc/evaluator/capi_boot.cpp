@@ -40,8 +40,8 @@ nexus::Ciphertext input(nx_bootstrapper *bt, const nx_ciphertext *x)
 
 extern "C" {
 
-int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, int double_angles, double final_scale,
-                           nx_bootstrapper **out)
+int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, int double_angles, int log_width,
+                           double final_scale, nx_bootstrapper **out)
 {
     return run([&] {
         need(ev);
@@ -52,6 +52,7 @@ int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, i
         p.K = K;
         p.cheb_degree = degree;
         p.double_angles = double_angles;
+        p.log_width = log_width;
         p.final_scale = final_scale;
         auto b = std::make_unique<nx_bootstrapper>();
         b->ev = ev;
@@ -155,6 +156,18 @@ int nx_boot_slot_to_coeff(nx_bootstrapper *bt, const nx_ciphertext *f_re, const 
     });
 }
 
+int nx_eval_argmax(nx_bootstrapper *bt, const nx_ciphertext *x, int len, nx_ciphertext *out)
+{
+    return run([&] {
+        need(bt);
+        need(out);
+        nexus::Ciphertext in = input(bt, x), res;
+        nexus::ArgmaxEvaluator am(*bt->ev->ckks, *bt->bt);
+        am.argmax(in, res, len);
+        store(res, out);
+    });
+}
+
 int nx_boot_plan_info(int log_half_n, int log_slots, double K, int which, int level, int *levels, int *stride, int *g,
                       size_t *count)
 {
@@ -189,6 +202,15 @@ int nx_boot_plan_diagonals(int log_half_n, int log_slots, double K, int which, i
             }
             k++;
         }
+    });
+}
+
+int nx_boot_eval_mod_coeffs(int K, int degree, int double_angles, int log_width, double *coeffs, int *depth)
+{
+    return run([&] {
+        const std::vector<double> c = nexus::eval_mod_coeffs(K, degree, double_angles, log_width);
+        if (coeffs) std::memcpy(coeffs, c.data(), c.size() * sizeof(double));
+        if (depth) *depth = nexus::chebyshev_depth(c);
     });
 }
 

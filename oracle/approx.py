@@ -77,6 +77,21 @@ def softmax_approx(x):
 
 
 # exact FP64 targets (data/data_generation.py:6-19)
+def argmax_approx(x, length):
+    """src/argmax.cpp:9-64 on a slot vector x (the first `length` slots hold the
+    inputs, the rest zero): duplicate, log2(length) rounds of
+    (a + b)/2 + (a - b) sgn(a - b)/2 with b = a rotated left by 2^i, then
+    sgn(x - max) + 1."""
+    x = np.asarray(x, dtype=np.float64)
+    x = x + np.roll(x, length)
+    a = x.copy()
+    steps = int(np.log2(length))
+    for i in range(steps):
+        b = np.roll(a, -(1 << i))
+        a = (a + b) * 0.5 + (a - b) * sgn_eval(a - b, 2, 2)
+    return sgn_eval(x - a, 2, 2, 1.0) + 1.0
+
+
 def gelu(x):
     return x * 0.5 * (1 + np.tanh(np.sqrt(2 / np.pi) * (x + 0.044715 * x ** 3)))
 

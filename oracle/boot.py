@@ -87,21 +87,23 @@ def mod_raise(c_q0, moduli):
     return np.array(out)
 
 
-def chebyshev_interpolant(K, degree, double_angles):
-    """Coefficients (Chebyshev basis on [-1, 1]) interpolating
-    cos(2 pi (K u - 1/4) / 2^r) at the degree+1 Chebyshev nodes."""
-    d = degree
-    j = np.arange(d + 1)
-    x = np.cos(np.pi * (j + 0.5) / (d + 1))
-    f = np.cos(2 * np.pi * (K * x - 0.25) / 2.0 ** double_angles)
-    c = np.array([2.0 / (d + 1) * np.sum(f * np.cos(np.pi * k * (j + 0.5) / (d + 1))) for k in range(d + 1)])
-    c[0] *= 0.5
-    return c
+def eval_mod_fit(K=25, degree=59, double_angles=2, log_width=10):
+    """Chebyshev coefficients (basis on [-1, 1]) of the least-squares fit of
+    cos(2 pi (K u - 1/4) / 2^r) at 40 Chebyshev-distributed points in each
+    window [(i - w) / K, (i + w) / K], w = 2^-log_width, |i| < K (the windows of
+    the reference's RemezCos, cuda/src/bootstrapping/RemezCos.cuh:12-17)."""
+    w = 2.0 ** -log_width
+    per = 40
+    t = np.cos(np.pi * (np.arange(per) + 0.5) / per)
+    xs = np.concatenate([(i + w * t) / K for i in range(-K + 1, K)])
+    V = np.polynomial.chebyshev.chebvander(xs, degree)
+    f = np.cos(2 * np.pi * (K * xs - 0.25) / 2.0 ** double_angles)
+    return np.linalg.lstsq(V, f, rcond=None)[0]
 
 
-def eval_mod_model(u, K, degree, double_angles):
+def eval_mod_model(u, K=25, degree=59, double_angles=2, log_width=10):
     """float64 EvalMod: p(u) then r double angles -> ~ sin(2 pi K u)."""
-    c = chebyshev_interpolant(K, degree, double_angles)
+    c = eval_mod_fit(K, degree, double_angles, log_width)
     y = np.polynomial.chebyshev.chebval(u, c)
     for _ in range(double_angles):
         y = 2 * y * y - 1

@@ -100,14 +100,25 @@ def test_plan_rotation_steps(logNh, logn):
     assert len(steps) <= 48
 
 
-def test_eval_mod_interpolant():
-    K, deg, r = 25, 63, 2
-    xs = np.concatenate([(i + np.linspace(-2.0 ** -8, 2.0 ** -8, 33)) / K for i in range(-K + 1, K)])
-    y = ob.eval_mod_model(xs, K, deg, r)
-    assert np.max(np.abs(y - np.sin(2 * np.pi * K * xs))) < 1e-7
-    # degree 59 (the reference's) at the same depth is visibly worse
-    y59 = ob.eval_mod_model(xs, K, 59, r)
-    assert np.max(np.abs(y59 - np.sin(2 * np.pi * K * xs))) > 1e-7
+def test_eval_mod_polynomial():
+    """the library's EvalMod polynomial: same fit as the oracle's, error
+    against sin(2 pi K u) on the windows, and depth 6 at degree 59 (the
+    reference's EvalMod budget: 9 levels with two double angles)"""
+    K = 25
+    c, depth = nx.boot_eval_mod_coeffs(K, 59, 2, 10)
+    assert depth == 6
+    assert nx.boot_eval_mod_coeffs(K, 63, 2, 10)[1] == 7
+    xs = np.concatenate([(i + np.linspace(-2.0 ** -10, 2.0 ** -10, 33)) / K for i in range(-K + 1, K)])
+    mine = np.polynomial.chebyshev.chebval(xs, c)
+    ref = np.polynomial.chebyshev.chebval(xs, ob.eval_mod_fit(K, 59, 2, 10))
+    assert np.max(np.abs(mine - ref)) < 1e-9
+    y = mine
+    for _ in range(2):
+        y = 2 * y * y - 1
+    assert np.max(np.abs(y - np.sin(2 * np.pi * K * xs))) < 1e-8
+    # the fit degrades gracefully outside the windows (inputs with |m / q0| up to 2^-7)
+    xw = np.concatenate([(i + np.linspace(-2.0 ** -7, 2.0 ** -7, 65)) / K for i in range(-K + 1, K)])
+    assert np.max(np.abs(ob.eval_mod_model(xw) - np.sin(2 * np.pi * K * xw))) < 1e-6
 
 
 def test_mod_raise_oracle_is_centred_lift():

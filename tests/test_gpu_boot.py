@@ -7,7 +7,7 @@ oracle/boot.py, and the whole pipeline against the ideal model
 sin(2 pi m / q0) / 2 pi mapped through the slot matrix (m = the decrypted
 lowest-level plaintext) and against the input message.  The EvalMod
 polynomial itself (the reference's NTL minimax generator is absent) is pinned
-by its error against sin on the near-integer inputs, not by coefficients.
+by its error against sin on the near-integer windows, not by coefficients.
 
 Sizes: N = 2^12 with 2^10 (sparse) and 2^11 (dense) slots on a 19-prime chain,
 and the reference's bootstrapping configuration (cuda/src/bootstrapping.cu:22-60:
@@ -74,7 +74,7 @@ def env(request):
 
 def test_rotation_steps_and_depth(env):
     assert sorted(env.bt.rotation_steps()) == sorted(env.steps)
-    assert env.bt.depth() == 3 + 7 + 2 + 3
+    assert env.bt.depth() == 3 + 6 + 2 + 3
 
 
 def test_mod_raise_bit_exact(env):
@@ -126,9 +126,9 @@ def test_eval_mod_stage(env):
     u = (rng.integers(-K + 1, K, Nh) + rng.uniform(-2.0 ** -8, 2.0 ** -8, Nh)) / K
     ct = env.enc(u, env.q0, chain_index=4)
     out = env.bt.eval_mod(ct)
-    assert out.chain_index == 4 + 9
+    assert out.chain_index == 4 + 8
     got = env.dec(out)
-    model = ob.eval_mod_model(u, K, 63, 2)
+    model = ob.eval_mod_model(u, K)
     err = np.max(np.abs(got - model))
     err_sin = np.max(np.abs(got - np.sin(2 * np.pi * K * u)))
     print("eval_mod max err vs model %.3g vs sin %.3g" % (err, err_sin))
@@ -142,7 +142,7 @@ def test_slot_to_coeff_stage(env):
     a = rng.uniform(-0.5, 0.5, n)
     b = rng.uniform(-0.5, 0.5, n)
     br = ob.brv_perm(env.logn)
-    ci = 4 + 9
+    ci = 4 + 8
     if n < Nh:
         f = np.tile(np.concatenate([a[br], b[br]]), Nh // (2 * n))
         out = env.bt.slot_to_coeff(env.enc(f, env.q0, ci), None, SCALE)
@@ -203,4 +203,42 @@ def test_bootstrap_reference_config():
     assert len(env.steps) <= 42
     _note("bootstrap reference config")
     out = _bootstrap_check(env, 7, 1e-4, 1e-3)
-    assert out.chain_index == 16  # 31 - 15 levels consumed: 16 limbs remain
+    assert out.chain_index == 15  # 14 levels consumed: 17 of 31 limbs remain
+
+
+def test_argmax_reference_config():
+    """ArgmaxEvaluator on the reference's argmax_test (src/main.cpp:181-303):
+    N = 2^15, 2^13 sparse slots, {51, 46 x 17, 51 x 14, 58}, h = 192,
+    scale 2^46, input data/input/argmax_input_8.txt at 14 levels below the top;
+    checked against the FP64 model of the same program and the reference's
+    calibration file."""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    from oracle import approx
+    from tests import fixtures
+
+    logN, logn = 15, 13
+    chain = [51] + [46] * 17 + [51] * 14 + [58]
+    _note("argmax keys logN=15 L=32")
+    ctx = nx.Context(logN, chain, hamming_weight=192)
+    sk = nx.SecretKey(ctx, seed=77)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=3)
+    steps = sorted(set(nx.boot_plan_steps(logN - 1, logn, K)) | {-8, 1, 2, 4})
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps], seed=4)
+    ev = nx.CKKSEvaluator(ctx, rk, gk, SCALE, sk)
+    bt = nx.Bootstrapper(ev, log_slots=logn, K=K)
+    assert bt.depth() == 14
+    x, calib = fixtures.argmax_vectors()
+    n = 1 << logn
+    period = np.zeros(n)
+    period[: len(x)] = x
+    ct = nx.encrypt(ctx, sk, nx.encode(ctx, np.tile(period, (1 << (logN - 1)) // n), SCALE), seed=5)
+    ct = nx.mod_switch_to(ct, 1 + 14)
+    _note("argmax len 8")
+    out = bt.argmax(ct, len(x))
+    got = nx.decode(ctx, nx.decrypt(ctx, sk, out))[: len(x)]
+    model = approx.argmax_approx(period, len(x))[: len(x)]
+    mae = np.mean(np.abs(got - calib))
+    print("argmax: out", np.round(got, 4), "max|out-model| %.3g MAE %.3g" % (np.max(np.abs(got - model)), mae))
+    assert np.max(np.abs(got - model)) < 1e-2
+    assert mae <= np.mean(np.abs(model - calib)) + 1e-3
