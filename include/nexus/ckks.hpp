@@ -212,6 +212,10 @@ public:
     void rotate_vector_inplace(Ciphertext &ct, int steps, const GaloisKeys &keys) const;
     void apply_galois(const Ciphertext &ct, uint32_t elt, const GaloisKeys &keys, Ciphertext &dest) const;
     void apply_galois_inplace(Ciphertext &ct, uint32_t elt, const GaloisKeys &keys) const;
+    // hoisted rotations (one ModUp shared by all steps; Phantom hoisting_inplace)
+    void rotate_vector_hoisted(const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &keys,
+                               std::vector<Ciphertext> &dest) const;
+    void hoisting_inplace(Ciphertext &ct, const GaloisKeys &keys, const std::vector<int> &steps) const;
     void complex_conjugate(const Ciphertext &ct, const GaloisKeys &keys, Ciphertext &dest) const;
     void complex_conjugate_inplace(Ciphertext &ct, const GaloisKeys &keys) const;
 

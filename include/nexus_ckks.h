@@ -143,6 +143,15 @@ int nx_multiply_power_of_x(const nx_ciphertext *a, int64_t exponent, nx_cipherte
  * src/bootstrapping/Bootstrapper.cpp:2937-2991, cuda/src/bootstrapping/Bootstrapper.cu:2813-2860);
  * the scale is left unchanged. */
 int nx_mod_raise(const nx_ciphertext *a, nx_ciphertext *dest);
+/* Hoisted rotations: one ModUp of c1 shared by every step (reference Phantom
+ * hoisting_inplace, cuda/thirdparty/phantom-fhe/src/evaluate.cu:1727-1920).
+ * nx_rotate_hoisted writes rot(a, steps[k]) to dests[k] (step 0 = copy);
+ * nx_hoisting writes sum_k rot(a, steps[k]) with a single ModDown.  Results
+ * decrypt like nx_rotate_vector's; residues differ within key-switching noise. */
+int nx_rotate_hoisted(const nx_ciphertext *a, const int *steps, size_t count, const nx_kswitch_keys *galois_keys,
+                      nx_ciphertext *const *dests);
+int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx_kswitch_keys *galois_keys,
+                nx_ciphertext *dest);
 
 #ifdef __cplusplus
 }

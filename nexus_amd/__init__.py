@@ -456,6 +456,25 @@ def multiply_power_of_x(a, exponent, dest=None):
 
 
 _sig("nx_mod_raise", [vp, vp])
+_sig("nx_rotate_hoisted", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, ctypes.POINTER(vp)])
+_sig("nx_hoisting", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, vp])
+
+
+def rotate_hoisted(a, steps, galois_keys):
+    """[rot(a, s) for s in steps] sharing one ModUp (Phantom hoisting)."""
+    outs = [Ciphertext(a.ctx) for _ in steps]
+    st = (ctypes.c_int * len(steps))(*steps)
+    hs = (vp * len(steps))(*[o.h for o in outs])
+    _check(_lib.nx_rotate_hoisted(a.h, st, len(steps), galois_keys.h, hs))
+    return outs
+
+
+def hoisting(a, steps, galois_keys, dest=None):
+    """sum_k rot(a, steps[k]) with one ModUp and one ModDown (Phantom hoisting_inplace)."""
+    d = _out(a.ctx, dest)
+    st = (ctypes.c_int * len(steps))(*steps)
+    _check(_lib.nx_hoisting(a.h, st, len(steps), galois_keys.h, d.h))
+    return d
 
 
 def mod_raise(a, dest=None):

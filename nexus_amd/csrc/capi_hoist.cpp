@@ -1,0 +1,37 @@
+// C ABI of the hoisted rotations (include/nexus_ckks.h).
+#include "../../include/nexus_ckks.h"
+
+#include "handles.h"
+
+extern "C" {
+
+int nx_rotate_hoisted(const nx_ciphertext *a, const int *steps, size_t count, const nx_kswitch_keys *galois_keys,
+                      nx_ciphertext *const *dests)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(a, "a");
+        nx::need_ptr(galois_keys, "galois_keys");
+        nx::need_ptr(dests, "dests");
+        if (count) nx::need_ptr(steps, "steps");
+        for (size_t k = 0; k < count; k++) nx::need_ptr(dests[k], "dests[k]");
+        std::vector<int> st(steps, steps + count);
+        std::vector<nx::Ciphertext> out;
+        nx::op_rotate_hoisted(a->impl, st, galois_keys->impl, out);
+        for (size_t k = 0; k < count; k++) dests[k]->impl = std::move(out[k]);
+    });
+}
+
+int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx_kswitch_keys *galois_keys,
+                nx_ciphertext *dest)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(a, "a");
+        nx::need_ptr(steps, "steps");
+        nx::need_ptr(galois_keys, "galois_keys");
+        nx::need_ptr(dest, "dest");
+        std::vector<int> st(steps, steps + count);
+        nx::op_hoisting(a->impl, st, galois_keys->impl, dest->impl);
+    });
+}
+
+}  // extern "C"

@@ -191,9 +191,19 @@ void op_from_ntt(const Ciphertext &a, Ciphertext &dst);
 void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst);
 void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &dst);
 void op_mod_raise(const Ciphertext &a, Ciphertext &dst);
+// hoisted rotations (one ModUp of c1): one output per step / their sum
+void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk,
+                       std::vector<Ciphertext> &dst);
+void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk, Ciphertext &dst);
 
 // key switch of `target` (limbs x n, NTT) under key `key` added into ct (2 polys)
 void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key);
+// Key-switch pieces.  ModUp of B targets [B][l][n] -> T [B][l+1][l][n] (NTT,
+// output primes q_0..q_{l-1}, p); ModDown of acc [B][2][l+1][n] into cts
+// [B][2][l][n]: accumulated (zero_c1 false) or c0 accumulated and c1 written.
+void ks_out_primes(const Context &c, size_t l, uint8_t *outp);
+DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l);
+void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1);
 // B key switches in one pass: targets [B][l][n] into cts [B][2][l][n]; with
 // zero_c1 the c1 of each result is written (not accumulated).
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,

@@ -554,18 +554,18 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
     // per giant step a plaintext-weighted sum and one rotation, one rescale.
     const size_t l = x.coeff_modulus_size();
     const double sigma = out_scale * prime(l) / x.scale();
-    std::map<int, Ciphertext> baby;
+    // baby steps share one ModUp of x (hoisted rotations)
+    std::vector<int> bi, bsteps;
     for (const auto &d : lv.diag) {
         const int i = d.first.second;
-        if (baby.count(i)) continue;
-        if (i == 0)
-            baby.emplace(0, x);
-        else {
-            Ciphertext r;
-            ck_.evaluator.rotate_vector(x, i * lv.stride, *ck_.galois_keys, r);
-            baby.emplace(i, std::move(r));
-        }
+        if (std::find(bi.begin(), bi.end(), i) != bi.end()) continue;
+        bi.push_back(i);
+        bsteps.push_back(i * lv.stride);
     }
+    std::vector<Ciphertext> rots;
+    ck_.evaluator.rotate_vector_hoisted(x, bsteps, *ck_.galois_keys, rots);
+    std::map<int, Ciphertext> baby;
+    for (size_t k = 0; k < bi.size(); k++) baby.emplace(bi[k], std::move(rots[k]));
     Ciphertext result;
     bool have_result = false;
     auto it = lv.diag.begin();

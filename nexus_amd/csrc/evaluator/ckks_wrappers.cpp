@@ -396,6 +396,22 @@ void Evaluator::apply_galois_inplace(Ciphertext &ct, uint32_t elt, const GaloisK
 {
     check(nx_apply_galois(ct.get(), elt, k.get(), ct.get()));
 }
+void Evaluator::rotate_vector_hoisted(const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &k,
+                                      std::vector<Ciphertext> &dest) const
+{
+    std::vector<Ciphertext> out(steps.size());
+    std::vector<nx_ciphertext *> h(steps.size());
+    for (size_t i = 0; i < steps.size(); i++) {
+        out[i].bind(*ctx_);
+        h[i] = out[i].get();
+    }
+    check(nx_rotate_hoisted(ct.get(), steps.data(), steps.size(), k.get(), h.data()));
+    dest = std::move(out);
+}
+void Evaluator::hoisting_inplace(Ciphertext &ct, const GaloisKeys &k, const std::vector<int> &steps) const
+{
+    check(nx_hoisting(ct.get(), steps.data(), steps.size(), k.get(), ct.get()));
+}
 void Evaluator::complex_conjugate(const Ciphertext &ct, const GaloisKeys &k, Ciphertext &dest) const
 {
     OUT(dest);

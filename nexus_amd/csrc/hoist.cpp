@@ -1,0 +1,134 @@
+// Hoisted rotations (reference Phantom hoisting_inplace,
+// cuda/thirdparty/phantom-fhe/src/evaluate.cu:1727-1920): the ModUp of c1 --
+// INTT, lift of every digit to every output prime, NTT, i.e. l(l+1) of the
+// l^2+3l+2 NTTs of a key switch -- is done once and shared by all steps.
+//
+// The automorphism is applied to the lifted digits (a permutation in the NTT
+// domain), so digit j of a hoisted rotation is sigma(lift(c1 mod q_j)) rather
+// than lift(sigma(c1) mod q_j): both are decompositions of sigma(c1), the
+// results decrypt to the same values, and the residues differ from the
+// one-at-a-time rotation by key-switching noise.
+#include "context.h"
+
+namespace nx {
+
+namespace {
+
+struct Prepared {
+    std::vector<const uint64_t *> keys;
+    std::vector<const uint32_t *> perms;
+    std::vector<size_t> index;  // position of each non-identity step in `steps`
+};
+
+Prepared prepare(Context &c, const std::vector<int> &steps, const KSwitchKeys &gk)
+{
+    Prepared p;
+    for (size_t k = 0; k < steps.size(); k++) {
+        if (steps[k] == 0) continue;
+        const uint32_t elt = c.elt_from_step(steps[k]);
+        auto it = gk.keys.find(elt);
+        require(it != gk.keys.end(), "Galois key not present in hoisting");
+        p.keys.push_back(it->second.get());
+        p.perms.push_back(c.galois_table(elt));
+        p.index.push_back(k);
+    }
+    return p;
+}
+
+void check_input(const Ciphertext &a, const KSwitchKeys &gk)
+{
+    require(a.ctx != nullptr && a.size == 2, "ciphertext size must be 2");
+    require(a.ntt, "CKKS encrypted must be in NTT form");
+    require(gk.ctx == a.ctx, "galois_keys is not valid for encryption parameters");
+}
+
+}  // namespace
+
+void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk,
+                       std::vector<Ciphertext> &dst)
+{
+    check_input(a, gk);
+    Context &c = *a.ctx;
+    const size_t n = c.n, l = a.limbs, nout = l + 1, pw = l * n;
+    Prepared p = prepare(c, steps, gk);
+    std::vector<Ciphertext> out(steps.size());
+    for (size_t k = 0; k < steps.size(); k++)
+        if (steps[k] == 0) ct_copy(out[k], a);
+    if (!p.keys.empty()) {
+        uint8_t outp[kMaxPrimes];
+        ks_out_primes(c, l, outp);
+        DevBuf T = ks_modup(c, a.data.get() + pw, 1, l);
+        for (size_t base = 0; base < p.keys.size(); base += kMaxHoist) {
+            const size_t cnt = std::min<size_t>(kMaxHoist, p.keys.size() - base);
+            DevBuf acc(&c, cnt * 2 * nout * n);
+            ks_inner_product_rot(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
+                                 (uint32_t)cnt, (uint32_t)l, (uint32_t)(c.L + 1), outp, (uint32_t)nout, c.stream);
+            // results [cnt][2][l][n]: c0 = sigma(c0) + ks0, c1 = ks1
+            DevBuf res(&c, cnt * 2 * pw);
+            for (size_t r = 0; r < cnt; r++)
+                poly_permute(res.get() + r * 2 * pw, a.data.get(), p.perms[base + r], 1, (uint32_t)l, c.logn,
+                             c.stream);
+            ks_moddown(c, res.get(), acc.get(), cnt, l, true);
+            for (size_t r = 0; r < cnt; r++) {
+                Ciphertext &o = out[p.index[base + r]];
+                o.ctx = &c;
+                o.size = 2;
+                o.limbs = l;
+                o.scale = a.scale;
+                o.ntt = true;
+                o.data.reset(&c, 2 * pw);
+                NX_HIP(hipMemcpyAsync(o.data.get(), res.get() + r * 2 * pw, 2 * pw * 8, hipMemcpyDeviceToDevice,
+                                      c.stream));
+            }
+        }
+    }
+    dst = std::move(out);
+}
+
+void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk, Ciphertext &dst)
+{
+    // hoisting_inplace: sum_k rot(a, steps[k]) with one ModUp and one ModDown
+    check_input(a, gk);
+    require(!steps.empty(), "steps cannot be empty");
+    Context &c = *a.ctx;
+    const size_t n = c.n, l = a.limbs, nout = l + 1, pw = l * n;
+    Prepared p = prepare(c, steps, gk);
+    Ciphertext out;
+    out.ctx = &c;
+    out.size = 2;
+    out.limbs = l;
+    out.scale = a.scale;
+    out.ntt = true;
+    out.data.reset(&c, 2 * pw);
+    // c0 = sum_k sigma_k(c0) and c1 = 0, plus (c0, c1) once per identity step;
+    // the ModDown then accumulates the switched sum into both
+    size_t identity = steps.size() - p.keys.size();
+    {
+        std::vector<uint64_t> zeros(l, 0);
+        poly_fill_limbs(out.data.get(), zeros.data(), 2, (uint32_t)l, c.logn, c.stream);
+    }
+    DevBuf t(&c, pw);
+    for (size_t r = 0; r < p.perms.size(); r++) {
+        poly_permute(t.get(), a.data.get(), p.perms[r], 1, (uint32_t)l, c.logn, c.stream);
+        poly_add(c.tables, out.data.get(), out.data.get(), t.get(), 1, (uint32_t)l, c.stream);
+    }
+    for (size_t r = 0; r < identity; r++) {
+        poly_add(c.tables, out.data.get(), out.data.get(), a.data.get(), 2, (uint32_t)l, c.stream);
+    }
+    if (!p.keys.empty()) {
+        uint8_t outp[kMaxPrimes];
+        ks_out_primes(c, l, outp);
+        DevBuf T = ks_modup(c, a.data.get() + pw, 1, l);
+        DevBuf acc(&c, 2 * nout * n);
+        for (size_t base = 0; base < p.keys.size(); base += kMaxHoist) {
+            const size_t cnt = std::min<size_t>(kMaxHoist, p.keys.size() - base);
+            ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
+                                     (uint32_t)cnt, (uint32_t)l, (uint32_t)(c.L + 1), outp, (uint32_t)nout,
+                                     base > 0, c.stream);
+        }
+        ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
+    }
+    dst = std::move(out);
+}
+
+}  // namespace nx

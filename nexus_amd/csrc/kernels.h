@@ -91,6 +91,19 @@ void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, u
 void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t B,
                       uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
 
+// Hoisted variants over one shared ModUp T[nout][l][n] (B = 1): rotation r
+// reads digit values through the NTT-domain Galois permutation perms[r]
+// (T[I][j][perms[r][c]]) and uses key keys[r].  _rot writes one result per
+// rotation, acc[r][2][nout][n]; _rot_sum writes (or adds to acc) their sum,
+// acc[2][nout][n].
+constexpr int kMaxHoist = 32;
+void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
+                          const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
+                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
+                              const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
+                              const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s);
+
 // ---------------------------------------------------------------- keys
 // Deterministic counter-based sampling (not SEAL's PRNG).
 void sample_uniform(const DeviceTables &t, uint64_t *out, uint32_t polys, uint32_t limbs, const uint8_t *primes_host,

@@ -1,0 +1,143 @@
+// Hoisted key switching: one ModUp of c1 shared by several Galois
+// automorphisms (reference Phantom hoisting_inplace,
+// cuda/thirdparty/phantom-fhe/src/evaluate.cu:1727-1920).  The automorphism is
+// applied to the ModUp digits inside the inner product: digit values are
+// gathered through the NTT-domain permutation of each element, so the
+// permuted digits are never written out.
+#include "kernels.h"
+#include "prof.h"
+
+namespace nx {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+struct OutPrimes {
+    uint8_t p[kMaxPrimes];
+};
+
+struct RotArgs {
+    const uint64_t *key[kMaxHoist];
+    const uint32_t *perm[kMaxHoist];
+};
+
+// acc[r][k][I][c] = sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]  (one rotation r per grid row group)
+__global__ void k_ks_inner_rot(uint64_t *acc, const uint64_t *T, RotArgs args, const ModConst *mod, OutPrimes outp,
+                               uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t ri = blockIdx.y;  // r * nout + I
+    const uint32_t r = ri / nout, I = ri - r * nout;
+    const uint32_t kp = outp.p[I];
+    const ModConst m = mod[kp];
+    const uint32_t src = args.perm[r][c];
+    const uint64_t *t = T + (uint64_t)I * l * n + src;
+    const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
+    const uint64_t kstride = (uint64_t)key_limbs * n;
+    uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+    for (uint32_t j = 0; j < l; j++) {
+        const uint64_t tv = t[(uint64_t)j * n];
+        const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
+        const uint64_t a = kj[0], b = kj[kstride];
+        uint64_t plo = tv * a, phi = mulhi64(tv, a);
+        lo0 += plo;
+        hi0 += phi + (lo0 < plo);
+        plo = tv * b;
+        phi = mulhi64(tv, b);
+        lo1 += plo;
+        hi1 += phi + (lo1 < plo);
+    }
+    uint64_t *out = acc + (uint64_t)r * 2 * nout * n;
+    out[(uint64_t)I * n + c] = barrett128(hi0, lo0, m);
+    out[((uint64_t)nout + I) * n + c] = barrett128(hi1, lo1, m);
+}
+
+// acc[k][I][c] (+)= sum_r sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]:
+// every product is below 2^120, so the 128-bit sums are folded every 128 terms.
+__global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs args, uint32_t count,
+                                   const ModConst *mod, OutPrimes outp, uint32_t l, uint32_t key_limbs,
+                                   uint32_t nout, uint32_t n, int accumulate)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t I = blockIdx.y;
+    const uint32_t kp = outp.p[I];
+    const ModConst m = mod[kp];
+    const uint64_t kstride = (uint64_t)key_limbs * n;
+    uint64_t r0 = 0, r1 = 0;  // folded partial sums (reduced)
+    if (accumulate) {
+        r0 = acc[(uint64_t)I * n + c];
+        r1 = acc[((uint64_t)nout + I) * n + c];
+    }
+    uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
+    uint32_t terms = 0;
+    for (uint32_t r = 0; r < count; r++) {
+        const uint32_t src = args.perm[r][c];
+        const uint64_t *t = T + (uint64_t)I * l * n + src;
+        const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
+        for (uint32_t j = 0; j < l; j++) {
+            const uint64_t tv = t[(uint64_t)j * n];
+            const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
+            const uint64_t a = kj[0], b = kj[kstride];
+            uint64_t plo = tv * a, phi = mulhi64(tv, a);
+            lo0 += plo;
+            hi0 += phi + (lo0 < plo);
+            plo = tv * b;
+            phi = mulhi64(tv, b);
+            lo1 += plo;
+            hi1 += phi + (lo1 < plo);
+            if (++terms == 128) {
+                r0 = add_mod(r0, barrett128(hi0, lo0, m), m.q);
+                r1 = add_mod(r1, barrett128(hi1, lo1, m), m.q);
+                lo0 = hi0 = lo1 = hi1 = 0;
+                terms = 0;
+            }
+        }
+    }
+    acc[(uint64_t)I * n + c] = add_mod(r0, barrett128(hi0, lo0, m), m.q);
+    acc[((uint64_t)nout + I) * n + c] = add_mod(r1, barrett128(hi1, lo1, m), m.q);
+}
+
+OutPrimes make_out(const uint8_t *p, uint32_t nout)
+{
+    OutPrimes o;
+    for (uint32_t i = 0; i < kMaxPrimes; i++) o.p[i] = i < nout ? p[i] : 0;
+    return o;
+}
+
+RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uint32_t count)
+{
+    RotArgs a;
+    for (uint32_t i = 0; i < kMaxHoist; i++) {
+        a.key[i] = i < count ? keys[i] : nullptr;
+        a.perm[i] = i < count ? perms[i] : nullptr;
+    }
+    return a;
+}
+
+}  // namespace
+
+void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
+                          const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
+                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    ProfScope prof("ks_inner_rot", s, (double)n * count * nout * (24.0 * l + 16.0 + 4.0));
+    hipLaunchKernelGGL(k_ks_inner_rot, dim3((n + kBlock - 1) / kBlock, count * nout), dim3(kBlock), 0, s, acc, T,
+                       make_args(keys, perms, count), t.mod, make_out(out_primes_host, nout), l, key_limbs, nout, n);
+}
+
+void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
+                              const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
+                              const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    ProfScope prof("ks_inner_rot_sum", s, (double)n * nout * (count * (24.0 * l + 4.0) + 16.0));
+    hipLaunchKernelGGL(k_ks_inner_rot_sum, dim3((n + kBlock - 1) / kBlock, nout), dim3(kBlock), 0, s, acc, T,
+                       make_args(keys, perms, count), count, t.mod, make_out(out_primes_host, nout), l, key_limbs,
+                       nout, n, accumulate ? 1 : 0);
+}
+
+}  // namespace nx

@@ -213,40 +213,59 @@ void op_square(const Ciphertext &a, Ciphertext &dst) { op_multiply(a, a, dst); }
 // zero_c1: write instead of accumulate into ct1 (apply_galois clears c1 first).
 // Key switch of B targets ([B][l][n], NTT) added into B ciphertexts
 // ([B][2][l][n]); every stage runs as one launch over the whole batch.
+void ks_out_primes(const Context &c, size_t l, uint8_t *outp)
+{
+    for (size_t i = 0; i < l; i++) outp[i] = (uint8_t)i;
+    outp[l] = (uint8_t)c.L;
+}
+
+DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
+{
+    // ModUp: INTT the l digits, lift each to every output prime, NTT.
+    const size_t n = c.n, nout = l + 1;
+    uint8_t outp[kMaxPrimes];
+    ks_out_primes(c, l, outp);
+    DevBuf coef(&c, B * l * n);
+    d2d(c, coef.get(), targets, B * l * n);
+    ntt_inv(c, coef.get(), B, l, l * n);
+    DevBuf T(&c, B * nout * l * n);
+    ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
+    NttBatch b;
+    b.data = T.get();
+    b.poly_stride = l * n;
+    b.polys = (uint32_t)(B * nout);
+    b.limbs = (uint32_t)l;
+    b.prime_by_poly = 1;
+    b.poly_period = (uint32_t)nout;
+    for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = i < (int)nout ? outp[i] : 0;
+    ntt_forward(c.tables, b, c.stream);
+    return T;
+}
+
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
                       bool zero_c1)
 {
     const size_t n = c.n, L = c.L;
     const size_t nout = l + 1;
     uint8_t outp[kMaxPrimes];
-    for (size_t i = 0; i < l; i++) outp[i] = (uint8_t)i;
-    outp[l] = (uint8_t)L;
-
-    // ModUp: INTT the l digits, lift each to every output prime, NTT.
-    DevBuf coef(&c, B * l * n);
-    d2d(c, coef.get(), targets, B * l * n);
-    ntt_inv(c, coef.get(), B, l, l * n);
-    DevBuf T(&c, B * nout * l * n);
-    ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
-    {
-        NttBatch b;
-        b.data = T.get();
-        b.poly_stride = l * n;
-        b.polys = (uint32_t)(B * nout);
-        b.limbs = (uint32_t)l;
-        b.prime_by_poly = 1;
-        b.poly_period = (uint32_t)nout;
-        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = i < (int)nout ? outp[i] : 0;
-        ntt_forward(c.tables, b, c.stream);
-    }
+    ks_out_primes(c, l, outp);
+    DevBuf T = ks_modup(c, targets, B, l);
     // inner product with the device-resident key
     DevBuf acc(&c, B * 2 * nout * n);
     ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)B, (uint32_t)l, (uint32_t)(L + 1), outp,
                      (uint32_t)nout, c.stream);
-    // ModDown with rounding: INTT of the special-prime limb of every poly
+    ks_moddown(c, cts, acc.get(), B, l, zero_c1);
+}
+
+void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1)
+{
+    const size_t n = c.n, L = c.L;
+    const size_t nout = l + 1;
+    // ModDown with rounding (the special-prime limbs of acc are transformed in
+    // place): INTT of the special-prime limb of every poly
     {
         NttBatch b;
-        b.data = acc.get() + l * n;
+        b.data = acc + l * n;
         b.poly_stride = nout * n;
         b.polys = (uint32_t)(2 * B);
         b.limbs = 1;
@@ -255,10 +274,10 @@ void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t
         ntt_inverse(c.tables, b, c.stream);
     }
     DevBuf tmp(&c, B * 2 * l * n);
-    rns_center_last(c.tables, tmp.get(), acc.get() + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
+    rns_center_last(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
                     c.stream);
     ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
-    rns_sub_mul(c.tables, cts, l * n, acc.get(), nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
+    rns_sub_mul(c.tables, cts, l * n, acc, nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
                 zero_c1 ? 2 : 1, c.stream);
 }
 
