@@ -29,7 +29,8 @@ inline void check(int rc)
 
 class CKKSContext {
 public:
-    CKKSContext(int log_n, const std::vector<int> &bit_sizes, uint32_t hamming_weight = 0, size_t sparse_slots = 0);
+    CKKSContext(int log_n, const std::vector<int> &bit_sizes, uint32_t hamming_weight = 0, size_t sparse_slots = 0,
+                size_t special_primes = 1);
     explicit CKKSContext(nx_context *borrowed);  // non-owning view of a C handle
     ~CKKSContext();
     CKKSContext(const CKKSContext &) = delete;
@@ -38,6 +39,7 @@ public:
     int log_n() const { return log_n_; }
     size_t poly_degree() const { return size_t(1) << log_n_; }
     size_t data_primes() const { return L_; }
+    size_t special_primes() const { return alpha_; }
     const std::vector<uint64_t> &moduli() const { return moduli_; }
     // modulus q_{k-1} counted from the top of a ciphertext at chain_index (k = 1: last limb)
     uint64_t modulus_from_top(size_t chain_index, int k) const { return moduli_[L_ + 1 - chain_index - k]; }
@@ -49,7 +51,7 @@ private:
     nx_context *h_ = nullptr;
     bool owns_ = true;
     int log_n_;
-    size_t L_;
+    size_t L_, alpha_ = 1;
     std::vector<uint64_t> moduli_;
 };
 

@@ -53,9 +53,18 @@ int nx_device_synchronize(void);
  * (SEAL-4.1-bs encryptionparams.h:188-231); sparse_slots 0 = N/2. */
 int nx_context_create(int log_n, const int *bit_sizes, size_t count, uint32_t hamming_weight, size_t sparse_slots,
                       nx_context **out);
+/* Hybrid key switching: the last special_primes of bit_sizes are special
+ * primes; key-switching digits are special_primes consecutive data primes
+ * (Phantom EncryptionParameters::set_special_modulus_size,
+ * cuda/thirdparty/phantom-fhe/include/context.cuh; ModUp/ModDown
+ * rns_bconv.cu:455-634,776-828).  special_primes = 1 is nx_context_create. */
+int nx_context_create_hybrid(int log_n, const int *bit_sizes, size_t count, size_t special_primes,
+                             uint32_t hamming_weight, size_t sparse_slots, nx_context **out);
+/* special primes, key-switching digits at the top level, words of one key */
+int nx_context_key_info(const nx_context *ctx, size_t *special_primes, size_t *digits, size_t *key_words);
 int nx_context_destroy(nx_context *ctx);
 int nx_context_info(const nx_context *ctx, int *log_n, size_t *data_primes, size_t *slot_count);
-/* moduli: L+1 words (q_0..q_{L-1}, p) */
+/* moduli: L + special words (q_0..q_{L-1}, p_0..) */
 int nx_context_moduli(const nx_context *ctx, uint64_t *moduli);
 int nx_context_synchronize(nx_context *ctx);
 /* Galois element of a rotation step (SEAL util/galois.cpp:53-90, generator 5). */
@@ -95,7 +104,7 @@ int nx_kswitch_keys_destroy(nx_kswitch_keys *keys);
 int nx_relin_key_generate(nx_context *ctx, const nx_secret_key *sk, uint64_t seed, nx_kswitch_keys *keys);
 int nx_galois_keys_generate(nx_context *ctx, const nx_secret_key *sk, const uint32_t *elts, size_t count,
                             uint64_t seed, nx_kswitch_keys *keys);
-/* key material [L digits][2][L+1][N] words */
+/* key material [digits][2][L + special][N] words (nx_context_key_info) */
 int nx_kswitch_key_upload(nx_kswitch_keys *keys, uint32_t elt, const uint64_t *data);
 int nx_kswitch_key_download(const nx_kswitch_keys *keys, uint32_t elt, uint64_t *data);
 int nx_kswitch_keys_has(const nx_kswitch_keys *keys, uint32_t elt, int *present);

@@ -51,6 +51,9 @@ _lib.nx_last_error.argtypes = []
 _sig("nx_device_count", [ctypes.POINTER(ctypes.c_int)])
 _sig("nx_device_synchronize", [])
 _sig("nx_context_create", [ctypes.c_int, ctypes.POINTER(ctypes.c_int), sz, ctypes.c_uint32, sz, ctypes.POINTER(vp)])
+_sig("nx_context_create_hybrid", [ctypes.c_int, ctypes.POINTER(ctypes.c_int), sz, sz, ctypes.c_uint32, sz,
+                                  ctypes.POINTER(vp)])
+_sig("nx_context_key_info", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)])
 _sig("nx_context_destroy", [vp])
 _sig("nx_context_info", [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(sz), ctypes.POINTER(sz)])
 _sig("nx_context_moduli", [vp, u64p])
@@ -134,17 +137,24 @@ def _u64(a):
 class Context:
     """CKKS parameters + device tables (SEALContext / PhantomContext)."""
 
-    def __init__(self, log_n, bit_sizes, hamming_weight=0, sparse_slots=0):
+    def __init__(self, log_n, bit_sizes, hamming_weight=0, sparse_slots=0, special_primes=1):
+        """bit_sizes: data primes then special primes; special_primes > 1 selects
+        hybrid key switching (digits of special_primes data primes)."""
         bits = (ctypes.c_int * len(bit_sizes))(*bit_sizes)
         h = vp()
-        _check(_lib.nx_context_create(log_n, bits, len(bit_sizes), hamming_weight, sparse_slots, ctypes.byref(h)))
+        _check(_lib.nx_context_create_hybrid(log_n, bits, len(bit_sizes), special_primes, hamming_weight,
+                                             sparse_slots, ctypes.byref(h)))
         self.h = h
         self.log_n = log_n
         self.n = 1 << log_n
         mods = np.zeros(len(bit_sizes), dtype=np.uint64)
         _check(_lib.nx_context_moduli(self.h, mods.ctypes.data_as(u64p)))
         self.moduli = [int(x) for x in mods]
-        self.L = len(bit_sizes) - 1
+        self.special_primes = special_primes
+        self.L = len(bit_sizes) - special_primes
+        d, kw = sz(), sz()
+        _check(_lib.nx_context_key_info(self.h, None, ctypes.byref(d), ctypes.byref(kw)))
+        self.digits, self.key_words = d.value, kw.value
         self.slot_count = self.n // 2
 
     def __del__(self, _fn=_lib.nx_context_destroy, _rel=_release):
@@ -299,8 +309,8 @@ class KSwitchKeys:
         return self
 
     def key_array(self, elt):
-        L = self.ctx.L
-        out = np.zeros((L, 2, L + 1, self.ctx.n), dtype=np.uint64)
+        c = self.ctx
+        out = np.zeros((c.digits, 2, c.L + c.special_primes, c.n), dtype=np.uint64)
         _check(_lib.nx_kswitch_key_download(self.h, elt, out.ctypes.data_as(u64p)))
         return out
 

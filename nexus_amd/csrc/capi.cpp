@@ -90,6 +90,27 @@ int nx_context_create(int log_n, const int *bit_sizes, size_t count, uint32_t ha
     });
 }
 
+int nx_context_create_hybrid(int log_n, const int *bit_sizes, size_t count, size_t special_primes,
+                             uint32_t hamming_weight, size_t sparse_slots, nx_context **out)
+{
+    return guard([&] {
+        need(bit_sizes, "bit_sizes");
+        need(out, "out");
+        std::vector<int> bits(bit_sizes, bit_sizes + count);
+        *out = new nx_context(log_n, bits, hamming_weight, sparse_slots, special_primes);
+    });
+}
+
+int nx_context_key_info(const nx_context *ctx, size_t *special_primes, size_t *digits, size_t *key_words)
+{
+    return guard([&] {
+        need(ctx, "ctx");
+        if (special_primes) *special_primes = ctx->impl.alpha;
+        if (digits) *digits = ctx->impl.dnum;
+        if (key_words) *key_words = ctx->impl.key_words();
+    });
+}
+
 int nx_context_destroy(nx_context *ctx)
 {
     return guard([&] { delete ctx; });
@@ -382,7 +403,7 @@ int nx_kswitch_key_upload(nx_kswitch_keys *keys, uint32_t elt, const uint64_t *d
         need(keys, "keys");
         need(data, "data");
         nx::Context &c = *keys->impl.ctx;
-        const size_t words = c.L * 2 * (c.L + 1) * c.n;
+        const size_t words = c.key_words();
         nx::DevBuf &b = keys->impl.keys[elt];
         b.reset(&c, words);
         NX_HIP(hipMemcpyAsync(b.get(), data, words * 8, hipMemcpyHostToDevice, c.stream));

@@ -97,14 +97,16 @@ void DevBuf::release()
 }
 
 // ---------------------------------------------------------------- context
-Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size_t sparse)
-    : logn(logn_), n(size_t(1) << logn_), hamming_weight(hw), sparse_slots(sparse)
+Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size_t sparse, size_t special)
+    : logn(logn_), n(size_t(1) << logn_), alpha(special), hamming_weight(hw), sparse_slots(sparse)
 {
     require(logn >= 1 && logn <= 17, "poly_modulus_degree is invalid");
-    require(bit_sizes.size() >= 2 && bit_sizes.size() <= (size_t)kMaxPrimes,
-            "coeff modulus needs at least one data prime and one special prime");
+    require(alpha >= 1 && bit_sizes.size() >= alpha + 1 && bit_sizes.size() <= (size_t)kMaxPrimes,
+            "coeff modulus needs at least one data prime and the special primes");
+    require(alpha <= 32, "at most 32 special primes");
     moduli = coeff_modulus_create(n, bit_sizes);
-    L = moduli.size() - 1;
+    L = moduli.size() - alpha;
+    dnum = (L + alpha - 1) / alpha;
     if (sparse_slots == 0) sparse_slots = n / 2;
     require(sparse_slots <= n / 2 && (sparse_slots & (sparse_slots - 1)) == 0, "sparse_slots is invalid");
     for (uint64_t q : moduli) mods.push_back(make_modconst(q));
@@ -140,9 +142,12 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     std::vector<ulonglong2> pinv(P, make_ulonglong2(0, 0));
     for (size_t i = 0; i < L; i++) {
         const uint64_t qi = moduli[i];
-        uint64_t v = host_inv_mod(moduli[L] % qi, qi);
+        uint64_t pm = 1;
+        for (size_t k = 0; k < alpha; k++) pm = host_mul_mod(pm, moduli[L + k] % qi, qi);
+        uint64_t v = host_inv_mod(pm, qi);
         pinv[i] = make_ulonglong2(v, shoup_quotient(v, qi));
     }
+    if (alpha > 1) build_hybrid_tables();
 
     NX_HIP(hipMalloc(&d_fwd, P * n * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_inv, P * n * sizeof(ulonglong2)));
@@ -165,9 +170,73 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     tables.mod = d_mod;
 }
 
+// Base-conversion constants of hybrid key switching (fast basis extension:
+// x -> sum_i [x_i Dhat_i^-1]_{q_i} Dhat_i mod t, Dhat_i = prod_{i' != i} q_i').
+void Context::build_hybrid_tables()
+{
+    const size_t K = key_limbs();
+    std::vector<ulonglong2> uw(dnum * alpha * alpha, make_ulonglong2(0, 0));
+    std::vector<uint64_t> uc(dnum * alpha * alpha * K, 0);
+    for (size_t j = 0; j < dnum; j++)
+        for (size_t s = 1; s <= alpha; s++) {
+            if (j * alpha + s > L) break;
+            for (size_t i = 0; i < s; i++) {
+                const size_t g = j * alpha + i, idx = (j * alpha + s - 1) * alpha + i;
+                const uint64_t qg = moduli[g];
+                uint64_t dh = 1;
+                for (size_t i2 = 0; i2 < s; i2++)
+                    if (i2 != i) dh = host_mul_mod(dh, moduli[j * alpha + i2] % qg, qg);
+                const uint64_t w = host_inv_mod(dh, qg);
+                uw[idx] = make_ulonglong2(w, shoup_quotient(w, qg));
+                for (size_t o = 0; o < K; o++) {
+                    const uint64_t t = moduli[o];
+                    uint64_t v = 1;
+                    for (size_t i2 = 0; i2 < s; i2++)
+                        if (i2 != i) v = host_mul_mod(v, moduli[j * alpha + i2] % t, t);
+                    uc[idx * K + o] = v;
+                }
+            }
+        }
+    std::vector<ulonglong2> dw(alpha);
+    std::vector<uint64_t> dc((alpha + 1) * L);  // row alpha: P mod q_i
+    for (size_t k = 0; k < alpha; k++) {
+        const uint64_t pk = moduli[L + k];
+        uint64_t ph = 1;
+        for (size_t k2 = 0; k2 < alpha; k2++)
+            if (k2 != k) ph = host_mul_mod(ph, moduli[L + k2] % pk, pk);
+        const uint64_t w = host_inv_mod(ph, pk);
+        dw[k] = make_ulonglong2(w, shoup_quotient(w, pk));
+        for (size_t i = 0; i < L; i++) {
+            const uint64_t qi = moduli[i];
+            uint64_t v = 1;
+            for (size_t k2 = 0; k2 < alpha; k2++)
+                if (k2 != k) v = host_mul_mod(v, moduli[L + k2] % qi, qi);
+            dc[k * L + i] = v;
+        }
+    }
+    for (size_t i = 0; i < L; i++) {
+        const uint64_t qi = moduli[i];
+        uint64_t v = 1;
+        for (size_t k = 0; k < alpha; k++) v = host_mul_mod(v, moduli[L + k] % qi, qi);
+        dc[alpha * L + i] = v;
+    }
+    NX_HIP(hipMalloc(&d_modup_w, uw.size() * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_modup_conv, uc.size() * sizeof(uint64_t)));
+    NX_HIP(hipMalloc(&d_moddown_w, dw.size() * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_moddown_conv, dc.size() * sizeof(uint64_t)));
+    NX_HIP(hipMemcpy(d_modup_w, uw.data(), uw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_modup_conv, uc.data(), uc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_moddown_w, dw.data(), dw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_moddown_conv, dc.data(), dc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+}
+
 Context::~Context()
 {
     if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipFree(d_modup_w);
+    (void)hipFree(d_modup_conv);
+    (void)hipFree(d_moddown_w);
+    (void)hipFree(d_moddown_conv);
     for (auto &kv : galois_) (void)hipFree(kv.second);
     (void)hipFree(d_fwd);
     (void)hipFree(d_inv);

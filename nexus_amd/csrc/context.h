@@ -114,13 +114,20 @@ struct KSwitchKeys {
 // ---------------------------------------------------------------- context
 class Context {
 public:
-    Context(int logn, const std::vector<int> &bit_sizes, uint32_t hamming_weight, size_t sparse_slots);
+    // The last `special` primes of bit_sizes are the key-switching special
+    // primes p_0..p_{alpha-1}; with one, key switching is SEAL's (one digit
+    // per data prime); with alpha > 1 it is hybrid: digit j is the alpha
+    // consecutive data primes q_{j alpha}.. (Phantom's special_modulus_size).
+    Context(int logn, const std::vector<int> &bit_sizes, uint32_t hamming_weight, size_t sparse_slots,
+            size_t special = 1);
     ~Context();
 
     int logn;
     size_t n;
     size_t L;                          // data primes at the top level
-    std::vector<uint64_t> moduli;      // q_0..q_{L-1}, p (special prime last)
+    size_t alpha = 1;                  // special primes
+    size_t dnum = 0;                   // key-switching digits at the top level, ceil(L / alpha)
+    std::vector<uint64_t> moduli;      // q_0..q_{L-1}, p_0..p_{alpha-1}
     std::vector<ModConst> mods;
     uint32_t hamming_weight;
     size_t sparse_slots;
@@ -133,7 +140,7 @@ public:
     ModConst *d_mod = nullptr;
     // rescale: d_rescale_inv[l] = {q_{l-1}^-1 mod q_i, Shoup} for i < l-1 (row l of a (L+1)x(L+1) table)
     ulonglong2 *d_rescale_inv = nullptr;
-    // ModDown: {p^-1 mod q_i, Shoup}
+    // ModDown: {P^-1 mod q_i, Shoup}, P = p_0 ... p_{alpha-1}
     ulonglong2 *d_p_inv = nullptr;
 
     // chain helpers
@@ -144,7 +151,17 @@ public:
         return L + 1 - chain_index;
     }
     int total_bits(size_t limbs) const;  // bit count of q_0 ... q_{limbs-1}
-    const ulonglong2 *rescale_inv(size_t limbs) const { return d_rescale_inv + limbs * (L + 1); }
+    size_t key_limbs() const { return L + alpha; }
+    size_t digits(size_t limbs) const { return (limbs + alpha - 1) / alpha; }
+    size_t key_words() const { return dnum * 2 * key_limbs() * n; }
+    // hybrid ModUp / ModDown base-conversion tables (alpha > 1)
+    //   modup_w[(j*alpha + s-1)*alpha + i]: [Dhat_i^-1]_{q_{j alpha + i}} (Shoup), digit j cut to s primes
+    //   modup_conv[((j*alpha + s-1)*alpha + i) * (L+alpha) + o]: Dhat_i mod (prime o)
+    //   moddown_w[k]: [Phat_k^-1]_{p_k} (Shoup); moddown_conv[k*L + i]: Phat_k mod q_i,
+    //   moddown_conv[alpha*L + i]: P mod q_i
+    ulonglong2 *d_modup_w = nullptr, *d_moddown_w = nullptr;
+    uint64_t *d_modup_conv = nullptr, *d_moddown_conv = nullptr;
+    const ulonglong2 *rescale_inv(size_t limbs) const { return d_rescale_inv + limbs * moduli.size(); }
 
     // Galois permutation tables (NTT domain), built on first use
     const uint32_t *galois_table(uint32_t elt);
@@ -157,6 +174,7 @@ public:
     std::mutex xk_mu;
 
 private:
+    void build_hybrid_tables();
     std::map<uint32_t, uint32_t *> galois_;
     std::mutex mu_;
 };

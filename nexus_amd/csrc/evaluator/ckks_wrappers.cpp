@@ -19,9 +19,10 @@ void throw_status(int rc)
 }
 
 // ---------------------------------------------------------------- context
-CKKSContext::CKKSContext(int log_n, const std::vector<int> &bits, uint32_t hw, size_t sparse) : log_n_(log_n)
+CKKSContext::CKKSContext(int log_n, const std::vector<int> &bits, uint32_t hw, size_t sparse, size_t special)
+    : log_n_(log_n)
 {
-    check(nx_context_create(log_n, bits.data(), bits.size(), hw, sparse, &h_));
+    check(nx_context_create_hybrid(log_n, bits.data(), bits.size(), special, hw, sparse, &h_));
     load_info();
 }
 
@@ -31,7 +32,8 @@ void CKKSContext::load_info()
 {
     size_t slots = 0;
     check(nx_context_info(h_, &log_n_, &L_, &slots));
-    moduli_.resize(L_ + 1);
+    check(nx_context_key_info(h_, &alpha_, nullptr, nullptr));
+    moduli_.resize(L_ + alpha_);
     check(nx_context_moduli(h_, moduli_.data()));
 }
 

@@ -5,7 +5,10 @@
 
 struct nx_context {
     nx::Context impl;
-    nx_context(int logn, const std::vector<int> &bits, uint32_t hw, size_t sparse) : impl(logn, bits, hw, sparse) {}
+    nx_context(int logn, const std::vector<int> &bits, uint32_t hw, size_t sparse, size_t special = 1)
+        : impl(logn, bits, hw, sparse, special)
+    {
+    }
 };
 struct nx_ciphertext {
     nx::Ciphertext impl;

@@ -49,7 +49,7 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
 {
     check_input(a, gk);
     Context &c = *a.ctx;
-    const size_t n = c.n, l = a.limbs, nout = l + 1, pw = l * n;
+    const size_t n = c.n, l = a.limbs, nout = l + c.alpha, pw = l * n;
     Prepared p = prepare(c, steps, gk);
     std::vector<Ciphertext> out(steps.size());
     for (size_t k = 0; k < steps.size(); k++)
@@ -62,7 +62,8 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
             const size_t cnt = std::min<size_t>(kMaxHoist, p.keys.size() - base);
             DevBuf acc(&c, cnt * 2 * nout * n);
             ks_inner_product_rot(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
-                                 (uint32_t)cnt, (uint32_t)l, (uint32_t)(c.L + 1), outp, (uint32_t)nout, c.stream);
+                                 (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
+                                 c.stream);
             // results [cnt][2][l][n]: c0 = sigma(c0) + ks0, c1 = ks1
             DevBuf res(&c, cnt * 2 * pw);
             for (size_t r = 0; r < cnt; r++)
@@ -91,7 +92,7 @@ void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwit
     check_input(a, gk);
     require(!steps.empty(), "steps cannot be empty");
     Context &c = *a.ctx;
-    const size_t n = c.n, l = a.limbs, nout = l + 1, pw = l * n;
+    const size_t n = c.n, l = a.limbs, nout = l + c.alpha, pw = l * n;
     Prepared p = prepare(c, steps, gk);
     Ciphertext out;
     out.ctx = &c;
@@ -123,8 +124,8 @@ void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwit
         for (size_t base = 0; base < p.keys.size(); base += kMaxHoist) {
             const size_t cnt = std::min<size_t>(kMaxHoist, p.keys.size() - base);
             ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
-                                     (uint32_t)cnt, (uint32_t)l, (uint32_t)(c.L + 1), outp, (uint32_t)nout,
-                                     base > 0, c.stream);
+                                     (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp,
+                                     (uint32_t)nout, base > 0, c.stream);
         }
         ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
     }

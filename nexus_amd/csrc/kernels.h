@@ -91,6 +91,18 @@ void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, u
 void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t B,
                       uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
 
+// Hybrid key switching (alpha special primes): fast basis extension of the
+// INTT'd limbs coef[B][l][n] digit by digit into T[B][I][j][n] for every output
+// prime I of (q_0..q_{l-1}, p_0..p_{alpha-1}); and of the special limbs of acc
+// (limbs l..l+alpha-1 of each poly, poly stride acc_stride) into z[polys][l][n].
+// Tables as Context::d_modup_* / d_moddown_*.
+void ks_modup_convert(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
+                      const uint64_t *conv, uint32_t B, uint32_t l, uint32_t alpha, uint32_t key_limbs,
+                      const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+void ks_moddown_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                        const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
+                        uint32_t L, hipStream_t s);
+
 // Hoisted variants over one shared ModUp T[nout][l][n] (B = 1): rotation r
 // reads digit values through the NTT-domain Galois permutation perms[r]
 // (T[I][j][perms[r][c]]) and uses key keys[r].  _rot writes one result per

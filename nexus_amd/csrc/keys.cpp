@@ -3,10 +3,11 @@
 // The key structure is SEAL's (keygenerator.cpp:56-336, rlwe.cpp): the secret
 // is ternary (optionally with a fixed Hamming weight, the bs fork's
 // set_secret_key_hamming_weight), and key-switching key digit j is
-//   ( -a_j s + e_j + [i == j] (P mod q_i) s' ,  a_j )   in every key-level limb i,
+//   ( -a_j s + e_j + [i in digit j] (P mod q_i) s' ,  a_j )   in every key-level limb i,
 // so the residues that key switching produces follow from the same algebra.
 // The randomness is this engine's own counter-based generator, not SEAL's
 // Blake2/SHAKE stream, so keys are not byte-identical to SEAL's keys.
+#include <algorithm>
 #include <random>
 
 #include "context.h"
@@ -51,18 +52,24 @@ void encrypt_zero_sk(Context &c, const SecretKey &sk, uint64_t *out, size_t coun
     poly_mul_add(c.tables, b, a, sk.ntt.get(), e.get(), (uint32_t)count, p, 1, c.stream);
 }
 
-// One key-switching key for new_key (NTT form, L+1 limbs) -> keygenerator.cpp:303-336.
+// One key-switching key for new_key (NTT form, key-level limbs) ->
+// keygenerator.cpp:303-336.  Digit j adds (P mod q_i) new_key to the limbs i of
+// its primes (one prime with one special prime; alpha consecutive primes in
+// hybrid mode), P the product of the special primes.
 void kswitch_key(Context &c, const SecretKey &sk, const uint64_t *new_key, uint64_t seed, DevBuf &out)
 {
-    const size_t L = c.L, K = L + 1, n = c.n;
-    out.reset(&c, L * 2 * K * n);
-    for (size_t j = 0; j < L; j++) {
+    const size_t L = c.L, K = c.key_limbs(), n = c.n;
+    out.reset(&c, c.key_words());
+    for (size_t j = 0; j < c.dnum; j++) {
         uint64_t *dj = out.get() + j * 2 * K * n;
         encrypt_zero_sk(c, sk, dj, K, K * n, subseed(seed, 100 + j));
-        const uint64_t qj = c.moduli[j];
-        const uint64_t factor = c.moduli[L] % qj;
-        limb_add_scaled(c.tables, dj + j * n, new_key + j * n, make_ulonglong2(factor, shoup_quotient(factor, qj)),
-                        (uint32_t)j, c.stream);
+        for (size_t i = j * c.alpha; i < std::min(L, (j + 1) * c.alpha); i++) {
+            const uint64_t qi = c.moduli[i];
+            uint64_t factor = 1;
+            for (size_t k = 0; k < c.alpha; k++) factor = host_mul_mod(factor, c.moduli[L + k] % qi, qi);
+            limb_add_scaled(c.tables, dj + i * n, new_key + i * n,
+                            make_ulonglong2(factor, shoup_quotient(factor, qi)), (uint32_t)i, c.stream);
+        }
     }
 }
 
@@ -70,7 +77,7 @@ void kswitch_key(Context &c, const SecretKey &sk, const uint64_t *new_key, uint6
 
 void secret_key_from_coeffs(Context &c, const int8_t *coeff, SecretKey &sk)
 {
-    const size_t K = c.L + 1, n = c.n;
+    const size_t K = c.key_limbs(), n = c.n;
     sk.ctx = &c;
     sk.coeff.assign(coeff, coeff + n);
     for (int8_t v : sk.coeff) require(v >= -1 && v <= 1, "secret key coefficients must be ternary");
@@ -107,7 +114,7 @@ void secret_key_generate(Context &c, uint64_t seed, SecretKey &sk)
 
 void relin_key_generate(Context &c, const SecretKey &sk, uint64_t seed, KSwitchKeys &out)
 {
-    const size_t K = c.L + 1, n = c.n;
+    const size_t K = c.key_limbs(), n = c.n;
     uint8_t p[kMaxPrimes];
     key_primes(c, K, p);
     DevBuf s2(&c, K * n);
@@ -120,7 +127,7 @@ void relin_key_generate(Context &c, const SecretKey &sk, uint64_t seed, KSwitchK
 void galois_keys_generate(Context &c, const SecretKey &sk, const uint32_t *elts, size_t count, uint64_t seed,
                           KSwitchKeys &out)
 {
-    const size_t K = c.L + 1, n = c.n;
+    const size_t K = c.key_limbs(), n = c.n;
     out.ctx = &c;
     DevBuf rot(&c, K * n);
     for (size_t i = 0; i < count; i++) {

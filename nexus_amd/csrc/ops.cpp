@@ -216,25 +216,30 @@ void op_square(const Ciphertext &a, Ciphertext &dst) { op_multiply(a, a, dst); }
 void ks_out_primes(const Context &c, size_t l, uint8_t *outp)
 {
     for (size_t i = 0; i < l; i++) outp[i] = (uint8_t)i;
-    outp[l] = (uint8_t)c.L;
+    for (size_t k = 0; k < c.alpha; k++) outp[l + k] = (uint8_t)(c.L + k);
 }
 
 DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
 {
-    // ModUp: INTT the l digits, lift each to every output prime, NTT.
-    const size_t n = c.n, nout = l + 1;
+    // ModUp: INTT the l limbs; each digit (one prime, or alpha consecutive
+    // primes in hybrid mode) extended to every output prime; NTT.
+    const size_t n = c.n, nout = l + c.alpha, digits = c.digits(l);
     uint8_t outp[kMaxPrimes];
     ks_out_primes(c, l, outp);
     DevBuf coef(&c, B * l * n);
     d2d(c, coef.get(), targets, B * l * n);
     ntt_inv(c, coef.get(), B, l, l * n);
-    DevBuf T(&c, B * nout * l * n);
-    ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
+    DevBuf T(&c, B * nout * digits * n);
+    if (c.alpha == 1)
+        ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
+    else
+        ks_modup_convert(c.tables, T.get(), coef.get(), c.d_modup_w, c.d_modup_conv, (uint32_t)B, (uint32_t)l,
+                         (uint32_t)c.alpha, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
     NttBatch b;
     b.data = T.get();
-    b.poly_stride = l * n;
+    b.poly_stride = digits * n;
     b.polys = (uint32_t)(B * nout);
-    b.limbs = (uint32_t)l;
+    b.limbs = (uint32_t)digits;
     b.prime_by_poly = 1;
     b.poly_period = (uint32_t)nout;
     for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = i < (int)nout ? outp[i] : 0;
@@ -245,37 +250,40 @@ DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
                       bool zero_c1)
 {
-    const size_t n = c.n, L = c.L;
-    const size_t nout = l + 1;
+    const size_t n = c.n, nout = l + c.alpha;
     uint8_t outp[kMaxPrimes];
     ks_out_primes(c, l, outp);
     DevBuf T = ks_modup(c, targets, B, l);
     // inner product with the device-resident key
     DevBuf acc(&c, B * 2 * nout * n);
-    ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)B, (uint32_t)l, (uint32_t)(L + 1), outp,
-                     (uint32_t)nout, c.stream);
+    ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)B, (uint32_t)c.digits(l),
+                     (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
     ks_moddown(c, cts, acc.get(), B, l, zero_c1);
 }
 
 void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1)
 {
-    const size_t n = c.n, L = c.L;
-    const size_t nout = l + 1;
-    // ModDown with rounding (the special-prime limbs of acc are transformed in
-    // place): INTT of the special-prime limb of every poly
+    const size_t n = c.n, L = c.L, alpha = c.alpha;
+    const size_t nout = l + alpha;
+    // The special-prime limbs of acc are transformed in place: INTT of the
+    // special limbs of every poly.
     {
         NttBatch b;
         b.data = acc + l * n;
         b.poly_stride = nout * n;
         b.polys = (uint32_t)(2 * B);
-        b.limbs = 1;
+        b.limbs = (uint32_t)alpha;
         b.prime_by_poly = 0;
-        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)L;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)(L + (i < (int)alpha ? i : 0));
         ntt_inverse(c.tables, b, c.stream);
     }
     DevBuf tmp(&c, B * 2 * l * n);
-    rns_center_last(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
-                    c.stream);
+    if (alpha == 1)  // SEAL: centred (rounded) division by p
+        rns_center_last(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
+                        c.stream);
+    else  // hybrid: fast conversion of the P part to every q_i
+        ks_moddown_convert(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv, (uint32_t)(2 * B),
+                           (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
     ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
     rns_sub_mul(c.tables, cts, l * n, acc, nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
                 zero_c1 ? 2 : 1, c.stream);
