@@ -11,7 +11,10 @@ per-call drop-in (one evaluator call per ciphertext):
 Inputs are synthetic: GELU ~N(0,1) clipped to [-5,5]; softmax scores ~N(0,3)
 shifted to max 0 and clipped to [-10,0]; LayerNorm ~N(0,1)*sqrt(30) (the range
 the reference's inverse-sqrt initial guess is tuned for).  Matrix products and
-bootstrapping of the layer are not part of this step.
+bootstrapping of the layer are not part of this step; they are timed after it
+and reported under "extra": one bootstrap at N = 2^16 with 2^15 slots (hybrid
+key switching) and at the reference's N = 2^15 / 2^13-slot configuration, and
+the reference's own MatrixMul (4096x768 x 768x64 at N = 2^13).
 
 Prints one JSON line.  The roofline entry is the dominant kernel class of the
 step, timed live with HIP events on its launch stream during the timed steps.
@@ -96,6 +99,66 @@ def cpu_baseline(gpu_ks_bytes, n, budget_s):
     }
 
 
+def bench_bootstrap(nx, logN, logn, chain, alpha, reps):
+    """Seconds per bootstrap of one ciphertext (ModRaise .. SlotToCoeff) and its
+    error against the input message; diagonals are encoded by an untimed first call."""
+    ctx = nx.Context(logN, chain, hamming_weight=192, special_primes=alpha)
+    sk = nx.SecretKey(ctx, seed=5)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=6)
+    steps = nx.boot_plan_steps(logN - 1, logn, 25)
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps], seed=7)
+    ev = nx.CKKSEvaluator(ctx, rk, gk, 2.0 ** 46, sk)
+    bt = nx.Bootstrapper(ev, log_slots=logn)
+    n = 1 << logn
+    v = np.random.default_rng(3).uniform(-1, 1, n)
+    ct = nx.encrypt(ctx, sk, nx.encode(ctx, np.tile(v, (1 << (logN - 1)) // n), 2.0 ** 46), seed=8)
+    ct = nx.mod_switch_to(ct, ctx.L)
+    out = bt.bootstrap(ct)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = bt.bootstrap(ct)
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    err = np.abs(nx.decode(ctx, nx.decrypt(ctx, sk, out))[:n] - v)
+    return {"ms_per_bootstrap": ms, "slots": n, "poly_degree": 1 << logN, "data_primes": ctx.L,
+            "special_primes": alpha, "key_switch_digits": ctx.digits, "depth": bt.depth(),
+            "rotation_keys": len(steps), "max_err": float(err.max()), "mean_err": float(err.mean())}
+
+
+def bench_matmul(nx, reps):
+    """The reference's MatrixMul (src/main.cpp:306-382): 4096x768 plaintext times
+    768x64 encrypted (coefficient-packed in 6 ciphertexts, expanded, accumulated);
+    N = 2^13, {60, 40, 60}; seconds per full product."""
+    from nexus_amd import mm
+    ctx = nx.Context(13, [60, 40, 60])
+    sk = nx.SecretKey(ctx, seed=4)
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, mm.reference_elts(ctx.n, 13), seed=2)
+    rng = np.random.default_rng(42)
+    x = np.round(rng.uniform(-1, 1, (4096, 768)), 4)
+    y = np.round(rng.uniform(-1, 1, (768, 64)), 4)
+    a_pts = [nx.encode(ctx, x.T[i], 2.0 ** 40) for i in range(768)]
+    flat = y.T.reshape(-1)
+    cts = [mm.compress(ctx, sk, flat[i * ctx.n:(i + 1) * ctx.n], 1e10, seed=10 + i) for i in range(6)]
+
+    def run():
+        return mm.accumulate(a_pts, [mm.expand(c, ctx.n, gk) for c in cts], 64)
+
+    res = run()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = run()
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    r = res[0]
+    r.scale = r.scale * 4096
+    d = nx.decode(ctx, nx.decrypt(ctx, sk, nx.rescale_to_next(r)))[:4096] / 2.0
+    err = float(np.mean(np.abs(d - (x @ y)[:, 0])))
+    return {"ms_per_matmul": ms, "shape": "4096x768 (plaintext) x 768x64 (encrypted)", "poly_degree": 1 << 13,
+            "key_switches": 6 * (ctx.n - 1), "mean_err_col0": err}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +166,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
+    ap.add_argument("--no-extra", action="store_true", help="skip the bootstrapping and MatMul measurements")
+    ap.add_argument("--only", choices=["bootstrap_dense", "bootstrap_sparse", "matmul"],
+                    help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: every box here has one GPU, so --gpus %d is not supported"
@@ -110,11 +176,21 @@ def main():
         sys.exit(2)
 
     import nexus_amd as nx
-    from oracle import approx
 
     if nx.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.only:
+        if args.only == "bootstrap_dense":
+            r = bench_bootstrap(nx, 16, 15, [51] + [46] * 16 + [51] * 14 + [58] * 3, 3, args.steps)
+        elif args.only == "bootstrap_sparse":
+            r = bench_bootstrap(nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, args.steps)
+        else:
+            r = bench_matmul(nx, args.steps)
+        print(json.dumps({"workload": args.only, **r}), flush=True)
+        return
+    from oracle import approx
+
     log("setup: context + keys (N=2^16, L=19)")
     ctx = nx.Context(16, CHAIN)
     sk = nx.SecretKey(ctx, seed=11)
@@ -225,9 +301,25 @@ def main():
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
         },
     }
+    n_main = ctx.n
+    if not args.no_extra:
+        del ev, gk, rk, sk, g_ct, s_ct, l_ct, outs
+        ctx = None
+        extra = {}
+        log("bootstrap N=2^16, 2^15 slots (dense), hybrid key switching (3 special primes)")
+        extra["bootstrap_dense_n65536"] = bench_bootstrap(nx, 16, 15, [51] + [46] * 16 + [51] * 14 + [58] * 3, 3, 3)
+        log("bootstrap N=2^15, 2^13 slots (sparse), the reference's chain (cuda/src/bootstrapping.cu:22-60)")
+        extra["bootstrap_sparse_n32768_reference_chain"] = bench_bootstrap(
+            nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, 3)
+        log("bootstrap N=2^15, 2^13 slots, same data chain with 4 special primes")
+        extra["bootstrap_sparse_n32768_hybrid"] = bench_bootstrap(
+            nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [58] * 4, 4, 3)
+        log("matmul 4096x768 x 768x64 (N=2^13)")
+        extra["matmul_reference_mm"] = bench_matmul(nx, 2)
+        res["extra"] = extra
     if not args.no_cpu_baseline:
         log("cpu baseline (oracle key switch)")
-        res["cpu_baseline"] = cpu_baseline(ks_bytes / args.steps, ctx.n, args.cpu_budget)
+        res["cpu_baseline"] = cpu_baseline(ks_bytes / args.steps, n_main, args.cpu_budget)
     print(json.dumps(res), flush=True)
 
 

@@ -198,6 +198,9 @@ public:
     void multiply_plain(const Ciphertext &ct, const Plaintext &plain, Ciphertext &dest) const;
     void multiply_plain_inplace(Ciphertext &ct, const Plaintext &plain) const;
 
+    // sum_r cts[r] * pts[r] in one pass
+    void multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
+                                   Ciphertext &dest) const;
     void add_plain(const Ciphertext &ct, const Plaintext &plain, Ciphertext &dest) const;
     void add_plain_inplace(Ciphertext &ct, const Plaintext &plain) const;
     void add(const Ciphertext &ct1, const Ciphertext &ct2, Ciphertext &dest) const;

@@ -161,6 +161,12 @@ int nx_rotate_hoisted(const nx_ciphertext *a, const int *steps, size_t count, co
                       nx_ciphertext *const *dests);
 int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx_kswitch_keys *galois_keys,
                 nx_ciphertext *dest);
+/* dest = sum_r cts[r] * pts[r] (size-2 ciphertexts of one level, NTT form) in one
+ * pass: the per-giant-step accumulation of the reference's BSGS transforms
+ * (cuda/src/bootstrapping/Bootstrapper.cu:1855-1919, multiply_vector_reduced_error
+ * + add per term) and of the MatrixMul row sums (cuda/src/matrix_mul.cu:202-275). */
+int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
+                                 nx_ciphertext *dest);
 
 #ifdef __cplusplus
 }

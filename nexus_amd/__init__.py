@@ -470,6 +470,18 @@ _sig("nx_rotate_hoisted", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, ctypes.POIN
 _sig("nx_hoisting", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, vp])
 
 
+_sig("nx_multiply_plain_accumulate", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp])
+
+
+def multiply_plain_accumulate(cts, pts, dest=None):
+    """sum_r cts[r] * pts[r] in one pass."""
+    d = _out(cts[0].ctx, dest)
+    c = (vp * len(cts))(*[x.h for x in cts])
+    p = (vp * len(pts))(*[x.h for x in pts])
+    _check(_lib.nx_multiply_plain_accumulate(c, p, len(cts), d.h))
+    return d
+
+
 def rotate_hoisted(a, steps, galois_keys):
     """[rot(a, s) for s in steps] sharing one ModUp (Phantom hoisting)."""
     outs = [Ciphertext(a.ctx) for _ in steps]

@@ -1,4 +1,5 @@
-// C ABI of the hoisted rotations (include/nexus_ckks.h).
+// C ABI of the hoisted rotations and the plaintext multiply-accumulate
+// (include/nexus_ckks.h).
 #include "../../include/nexus_ckks.h"
 
 #include "handles.h"
@@ -31,6 +32,25 @@ int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx
         nx::need_ptr(dest, "dest");
         std::vector<int> st(steps, steps + count);
         nx::op_hoisting(a->impl, st, galois_keys->impl, dest->impl);
+    });
+}
+
+int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
+                                 nx_ciphertext *dest)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(cts, "cts");
+        nx::need_ptr(pts, "pts");
+        nx::need_ptr(dest, "dest");
+        std::vector<const nx::Ciphertext *> c;
+        std::vector<const nx::Plaintext *> p;
+        for (size_t r = 0; r < count; r++) {
+            nx::need_ptr(cts[r], "cts[r]");
+            nx::need_ptr(pts[r], "pts[r]");
+            c.push_back(&cts[r]->impl);
+            p.push_back(&pts[r]->impl);
+        }
+        nx::op_dot_plain(c, p, dest->impl);
     });
 }
 

@@ -207,6 +207,9 @@ void op_conjugate(const Ciphertext &a, const KSwitchKeys &gk, Ciphertext &dst);
 void op_to_ntt(const Ciphertext &a, Ciphertext &dst);
 void op_from_ntt(const Ciphertext &a, Ciphertext &dst);
 void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst);
+// sum_r cts[r] * pts[r] in one pass (size-2 ciphertexts of one level)
+void op_dot_plain(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
+                  Ciphertext &dst);
 void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &dst);
 void op_mod_raise(const Ciphertext &a, Ciphertext &dst);
 // hoisted rotations (one ModUp of c1): one output per step / their sum

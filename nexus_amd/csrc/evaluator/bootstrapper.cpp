@@ -571,8 +571,9 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
     auto it = lv.diag.begin();
     while (it != lv.diag.end()) {
         const int j = it->first.first;
-        Ciphertext acc;
-        bool have = false;
+        // sum_i D~_(j,i) * rot(x, i stride) in one multiply-accumulate pass
+        std::vector<const Ciphertext *> terms;
+        std::vector<const Plaintext *> weights;
         for (; it != lv.diag.end() && it->first.first == j; ++it) {
             const int i = it->first.second;
             auto key = std::make_tuple(tag, j, i, l, sigma * value_factor);
@@ -582,15 +583,11 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
                 ck_.encoder.encode(it->second, chain_of(l), sigma * value_factor, pt);
                 pit = ptcache_.emplace(key, std::move(pt)).first;
             }
-            Ciphertext t;
-            ck_.evaluator.multiply_plain(baby.at(i), pit->second, t);
-            if (!have) {
-                acc = std::move(t);
-                have = true;
-            } else {
-                ck_.evaluator.add_inplace(acc, t);
-            }
+            terms.push_back(&baby.at(i));
+            weights.push_back(&pit->second);
         }
+        Ciphertext acc;
+        ck_.evaluator.multiply_plain_accumulate(terms, weights, acc);
         if (j != 0) ck_.evaluator.rotate_vector_inplace(acc, j * lv.g * lv.stride, *ck_.galois_keys);
         if (!have_result) {
             result = std::move(acc);

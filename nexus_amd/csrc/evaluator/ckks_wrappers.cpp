@@ -410,6 +410,17 @@ void Evaluator::rotate_vector_hoisted(const Ciphertext &ct, const std::vector<in
     check(nx_rotate_hoisted(ct.get(), steps.data(), steps.size(), k.get(), h.data()));
     dest = std::move(out);
 }
+void Evaluator::multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts,
+                                          const std::vector<const Plaintext *> &pts, Ciphertext &dest) const
+{
+    std::vector<const nx_ciphertext *> c;
+    std::vector<const nx_plaintext *> p;
+    for (const Ciphertext *x : cts) c.push_back(x->get());
+    for (const Plaintext *x : pts) p.push_back(x->get());
+    Ciphertext out(*ctx_);
+    check(nx_multiply_plain_accumulate(c.data(), p.data(), c.size(), out.get()));
+    dest = std::move(out);
+}
 void Evaluator::hoisting_inplace(Ciphertext &ct, const GaloisKeys &k, const std::vector<int> &steps) const
 {
     check(nx_hoisting(ct.get(), steps.data(), steps.size(), k.get(), ct.get()));

@@ -103,6 +103,21 @@ void ks_moddown_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc,
                         const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
                         uint32_t L, hipStream_t s);
 
+// Fused forms (one thread per input word, looping the outputs); usable while
+// alpha <= 16.
+bool ks_conv_fused_ok(uint32_t alpha);
+void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
+                            const uint64_t *conv, uint32_t B, uint32_t l, uint32_t alpha, uint32_t key_limbs,
+                            const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                              const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
+                              uint32_t L, hipStream_t s);
+// out[k][limb] = sum_r cts[r][k][limb] * pts[r][limb] for k < 2 (count <= kMaxDot):
+// the plaintext multiply-accumulate of a BSGS giant step in one pass.
+constexpr int kMaxDot = 64;
+void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const *cts, const uint64_t *const *pts,
+                    uint32_t count, uint32_t limbs, hipStream_t s);
+
 // Hoisted variants over one shared ModUp T[nout][l][n] (B = 1): rotation r
 // reads digit values through the NTT-domain Galois permutation perms[r]
 // (T[I][j][perms[r][c]]) and uses key keys[r].  _rot writes one result per

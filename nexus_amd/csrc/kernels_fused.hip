@@ -1,0 +1,180 @@
+// Fused forms of the hybrid base conversions and of the plaintext
+// multiply-accumulate of BSGS linear transforms.
+//
+// The conversions keep each input word in registers and loop over a chunk of
+// output primes: the per-input Shoup product [x Dhat^-1]_q is computed once
+// per chunk instead of once per output prime.  Conversion constants are
+// uniform across a wavefront (scalar loads).
+#include "kernels.h"
+#include "prof.h"
+
+namespace nx {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kRegAlpha = 16;  // digits / special primes held in registers
+constexpr uint32_t kOutChunk = 4;   // output primes per thread (more waves in flight)
+
+struct OutMap {
+    uint8_t p[kMaxPrimes];
+};
+
+// thread (b, j, chunk, c): T[b][I][j][c] for the output primes I of its chunk
+__global__ void k_modup_conv_fused(uint64_t *T, const uint64_t *coef, const ulonglong2 *w, const uint64_t *conv,
+                                   const ModConst *mod, OutMap outp, uint32_t l, uint32_t alpha, uint32_t digits,
+                                   uint32_t nout, uint32_t key_limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t chunks = (nout + kOutChunk - 1) / kOutChunk;
+    const uint32_t row = blockIdx.y;  // (b * digits + j) * chunks + chunk
+    const uint32_t bj = row / chunks, chunk = row - bj * chunks;
+    const uint32_t b = bj / digits, j = bj - b * digits;
+    const uint32_t first = j * alpha;
+    const uint32_t s = min(alpha, l - first);
+    const uint32_t base = (j * alpha + s - 1) * alpha;
+    const uint64_t *x = coef + (uint64_t)b * l * n + (uint64_t)first * n + c;
+    uint64_t y[kRegAlpha];
+#pragma unroll
+    for (uint32_t i = 0; i < kRegAlpha; i++) {
+        if (i < s) {
+            const ulonglong2 wi = w[base + i];
+            y[i] = shoup(x[(uint64_t)i * n], wi.x, wi.y, mod[first + i].q);
+        }
+    }
+    uint64_t *out = T + ((uint64_t)b * nout * digits + j) * n + c;
+    const uint32_t I_end = min(nout, (chunk + 1) * kOutChunk);
+    for (uint32_t I = chunk * kOutChunk; I < I_end; I++) {
+        const uint32_t kp = outp.p[I];
+        const uint64_t *f = conv + (uint64_t)base * key_limbs + kp;
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < kRegAlpha; i++) {
+            if (i < s) {
+                const uint64_t fi = f[(uint64_t)i * key_limbs];
+                const uint64_t plo = y[i] * fi, phi = mulhi64(y[i], fi);
+                lo += plo;
+                hi += phi + (lo < plo);
+            }
+        }
+        out[(uint64_t)I * digits * n] = barrett128(hi, lo, mod[kp]);
+    }
+}
+
+// thread (p, chunk, c): z[p][i][c] for the data primes i of its chunk
+__global__ void k_moddown_conv_fused(uint64_t *z, const uint64_t *acc, uint64_t acc_stride, const ulonglong2 *w,
+                                     const uint64_t *conv, const ModConst *mod, uint32_t l, uint32_t alpha,
+                                     uint32_t L, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
+    const uint32_t p = blockIdx.y / chunks, chunk = blockIdx.y - p * chunks;
+    const uint64_t *y = acc + (uint64_t)p * acc_stride + (uint64_t)l * n + c;
+    uint64_t v[kRegAlpha];
+    double frac = 0.0;
+#pragma unroll
+    for (uint32_t k = 0; k < kRegAlpha; k++) {
+        if (k < alpha) {
+            const ulonglong2 wk = w[k];
+            const uint64_t pk = mod[L + k].q;
+            v[k] = shoup(y[(uint64_t)k * n], wk.x, wk.y, pk);
+            frac += (double)v[k] / (double)pk;
+        }
+    }
+    const uint64_t r = (uint64_t)floor(frac + 0.5);
+    uint64_t *out = z + (uint64_t)p * l * n + c;
+    const uint32_t i_end = min(l, (chunk + 1) * kOutChunk);
+    for (uint32_t i = chunk * kOutChunk; i < i_end; i++) {
+        const uint64_t qi = mod[i].q;
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < kRegAlpha; k++) {
+            if (k < alpha) {
+                const uint64_t f = conv[(uint64_t)k * L + i];
+                const uint64_t plo = v[k] * f, phi = mulhi64(v[k], f);
+                lo += plo;
+                hi += phi + (lo < plo);
+            }
+        }
+        const uint64_t negP = qi - conv[(uint64_t)alpha * L + i];
+        const uint64_t plo = r * negP, phi = mulhi64(r, negP);
+        lo += plo;
+        hi += phi + (lo < plo);
+        out[(uint64_t)i * n] = barrett128(hi, lo, mod[i]);
+    }
+}
+
+struct DotArgs {
+    const uint64_t *ct[kMaxDot];
+    const uint64_t *pt[kMaxDot];
+};
+
+// out[k][limb][c] = sum_r ct_r[k][limb][c] * pt_r[limb][c]  (k < 2), folded every 32 products
+__global__ void k_dot_plain(uint64_t *out, DotArgs args, uint32_t count, const ModConst *mod, uint32_t limbs,
+                            uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // k * limbs + limb
+    const uint32_t limb = row % limbs;
+    const ModConst m = mod[limb];
+    const uint64_t off = (uint64_t)row * n + c, poff = (uint64_t)limb * n + c;
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t r = 0; r < count; r++) {
+        const uint64_t x = args.ct[r][off], y = args.pt[r][poff];
+        const uint64_t plo = x * y, phi = mulhi64(x, y);
+        lo += plo;
+        hi += phi + (lo < plo);
+        if ((r & 31) == 31) {
+            lo = barrett128(hi, lo, m);
+            hi = 0;
+        }
+    }
+    out[off] = barrett128(hi, lo, m);
+}
+
+}  // namespace
+
+bool ks_conv_fused_ok(uint32_t alpha) { return alpha <= kRegAlpha; }
+
+void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
+                            const uint64_t *conv, uint32_t B, uint32_t l, uint32_t alpha, uint32_t key_limbs,
+                            const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn, digits = (l + alpha - 1) / alpha;
+    OutMap o;
+    for (uint32_t i = 0; i < kMaxPrimes; i++) o.p[i] = i < nout ? out_primes_host[i] : 0;
+    ProfScope prof("ks_modup_conv", s, 8.0 * n * B * ((double)l + (double)nout * digits));
+    const uint32_t chunks = (nout + kOutChunk - 1) / kOutChunk;
+    hipLaunchKernelGGL(k_modup_conv_fused, dim3((n + kBlock - 1) / kBlock, B * digits * chunks), dim3(kBlock), 0, s,
+                       T, coef, w, conv, t.mod, o, l, alpha, digits, nout, key_limbs, n);
+}
+
+void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                              const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
+                              uint32_t L, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    ProfScope prof("ks_moddown_conv", s, 8.0 * n * polys * ((double)alpha + l));
+    const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
+    hipLaunchKernelGGL(k_moddown_conv_fused, dim3((n + kBlock - 1) / kBlock, polys * chunks), dim3(kBlock), 0, s, z,
+                       acc, acc_stride, w, conv, t.mod, l, alpha, L, n);
+}
+
+void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const *cts, const uint64_t *const *pts,
+                    uint32_t count, uint32_t limbs, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    DotArgs a;
+    for (uint32_t i = 0; i < kMaxDot; i++) {
+        a.ct[i] = i < count ? cts[i] : nullptr;
+        a.pt[i] = i < count ? pts[i] : nullptr;
+    }
+    ProfScope prof("dot_plain", s, 8.0 * n * limbs * (3.0 * count + 2.0));
+    hipLaunchKernelGGL(k_dot_plain, dim3((n + kBlock - 1) / kBlock, 2 * limbs), dim3(kBlock), 0, s, out, a, count,
+                       t.mod, limbs, n);
+}
+
+}  // namespace nx

@@ -232,6 +232,9 @@ DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
     DevBuf T(&c, B * nout * digits * n);
     if (c.alpha == 1)
         ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
+    else if (ks_conv_fused_ok((uint32_t)c.alpha))
+        ks_modup_convert_fused(c.tables, T.get(), coef.get(), c.d_modup_w, c.d_modup_conv, (uint32_t)B, (uint32_t)l,
+                               (uint32_t)c.alpha, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
     else
         ks_modup_convert(c.tables, T.get(), coef.get(), c.d_modup_w, c.d_modup_conv, (uint32_t)B, (uint32_t)l,
                          (uint32_t)c.alpha, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
@@ -281,7 +284,10 @@ void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bo
     if (alpha == 1)  // SEAL: centred (rounded) division by p
         rns_center_last(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
                         c.stream);
-    else  // hybrid: fast conversion of the P part to every q_i
+    else if (ks_conv_fused_ok((uint32_t)alpha))  // hybrid: exact centred conversion of the P part
+        ks_moddown_convert_fused(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv,
+                                 (uint32_t)(2 * B), (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
+    else
         ks_moddown_convert(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv, (uint32_t)(2 * B),
                            (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
     ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
@@ -562,6 +568,46 @@ void op_mod_raise(const Ciphertext &a, Ciphertext &dst)
     r.ntt = true;
     r.data = std::move(out);
     dst = std::move(r);
+}
+
+void op_dot_plain(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
+                  Ciphertext &dst)
+{
+    require(!cts.empty() && cts.size() == pts.size(), "ciphertext and plaintext counts must match");
+    const Ciphertext &a0 = *cts[0];
+    valid(a0);
+    require(a0.size == 2 && a0.ntt, "encrypted must be size 2 in NTT form");
+    Context &c = *a0.ctx;
+    const double scale = a0.scale * pts[0]->scale;
+    require(scale_within_bounds(c, scale, a0.limbs), "scale out of bounds");
+    std::vector<const uint64_t *> cp, pp;
+    for (size_t r = 0; r < cts.size(); r++) {
+        const Ciphertext &a = *cts[r];
+        const Plaintext &p = *pts[r];
+        valid(a);
+        require(a.ctx == &c && a.size == 2 && a.limbs == a0.limbs && a.ntt, "encrypted parameter mismatch");
+        plain_check(a, p);
+        require(are_close(a.scale * p.scale, scale), "scale mismatch");
+        cp.push_back(a.data.get());
+        pp.push_back(p.data.get());
+    }
+    DevBuf acc(&c, 2 * a0.limbs * c.n);
+    DevBuf part;
+    for (size_t base = 0; base < cp.size(); base += kMaxDot) {
+        const size_t cnt = std::min<size_t>(kMaxDot, cp.size() - base);
+        uint64_t *dst_words = base ? (part.reset(&c, acc.words()), part.get()) : acc.get();
+        poly_dot_plain(c.tables, dst_words, cp.data() + base, pp.data() + base, (uint32_t)cnt, (uint32_t)a0.limbs,
+                       c.stream);
+        if (base) poly_add(c.tables, acc.get(), acc.get(), part.get(), 2, (uint32_t)a0.limbs, c.stream);
+    }
+    Ciphertext out;
+    out.ctx = &c;
+    out.size = 2;
+    out.limbs = a0.limbs;
+    out.scale = scale;
+    out.ntt = true;
+    out.data = std::move(acc);
+    dst = std::move(out);
 }
 
 void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst)

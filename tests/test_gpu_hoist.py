@@ -90,3 +90,19 @@ def test_hoisting_sum(env, steps):
     exp = sum(np.roll(v, -s) for s in steps)
     got = _dec(ctx, sk, out).real
     assert np.max(np.abs(got - exp)) < 1e-5 * len(steps)
+
+
+@pytest.mark.parametrize("count", [1, 7, 70])
+def test_multiply_plain_accumulate_bit_exact(env, count):
+    """one-pass sum_r ct_r * pt_r equals the chain of multiply_plain + add residue for residue"""
+    ctx, sk, _, _ = env
+    _note("multiply_plain_accumulate count=%d" % count)
+    rng = np.random.default_rng(count)
+    cts = [_enc(ctx, sk, rng.uniform(-1, 1, ctx.slot_count)) for _ in range(count)]
+    pts = [nx.encode(ctx, rng.uniform(-1, 1, ctx.slot_count), SCALE) for _ in range(count)]
+    got = nx.multiply_plain_accumulate(cts, pts)
+    ref = nx.multiply_plain(cts[0], pts[0])
+    for c, p in zip(cts[1:], pts[1:]):
+        ref = nx.add(ref, nx.multiply_plain(c, p))
+    assert got.scale == ref.scale
+    assert np.array_equal(got.to_array(), ref.to_array())
