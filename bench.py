@@ -34,6 +34,10 @@ METRIC = "enc. BERT-base layers/sec + p50/p99 latency, batched vs plain GPU, ≤
 CHAIN = [58] + [40] * 18 + [58]
 SCALE = 2.0 ** 40
 HBM_PEAK_GBS = 8000.0
+# HBM bytes moved per algorithmic byte, from FETCH_SIZE / WRITE_SIZE counter runs
+# of `bench.py --only ntt` calibrated against a device copy of known size
+# (profiles/ntt_fwd_n65536_38limbs_traffic.md)
+TRAFFIC_PER_ALGORITHMIC_BYTE = {"ntt_fwd_col": 40.59 / 40.00, "ntt_fwd_row": 67.29 / 79.54}
 
 
 def log(msg):
@@ -159,6 +163,35 @@ def bench_matmul(nx, reps):
             "key_switches": 6 * (ctx.n - 1), "mean_err_col0": err}
 
 
+def bench_ntt(nx, reps):
+    """The dominant kernel class of the nonlinear step (forward NTT, column and
+    row passes) at its shape: one ciphertext of 2 x 19 limbs at N = 2^16, next to
+    a device-to-device copy of the same 19.9 MB (the copy calibrates HBM counters)."""
+    ctx = nx.Context(16, CHAIN)
+    rng = np.random.default_rng(1)
+    data = np.stack([np.stack([rng.integers(0, q, ctx.n, dtype=np.uint64) for q in ctx.moduli[:ctx.L]])
+                     for _ in range(2)])
+    ct = nx.Ciphertext.from_array(ctx, data, 1, SCALE, ntt_form=False)
+    nx.transform_to_ntt(ct)  # first launch loads the code object: untimed
+    ctx.synchronize()
+    nx.profile_select("ntt_fwd_col,ntt_fwd_row")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = nx.transform_to_ntt(ct)
+        cp = ct.copy()
+    ctx.synchronize()
+    wall = (time.perf_counter() - t0) / reps * 1e3
+    res = {"ms_per_rep": wall, "limbs_per_launch": 2 * ctx.L, "copy_bytes": int(data.nbytes)}
+    for cls in ("ntt_fwd_col", "ntt_fwd_row"):
+        launches, ms, b = nx.profile_read(cls)
+        res[cls] = {"launches": launches, "avg_us": ms / max(launches, 1) * 1e3,
+                    "algorithmic_bytes_per_launch": b / max(launches, 1),
+                    "achieved_gbs": (b / max(launches, 1)) / (ms / max(launches, 1) * 1e-3) / 1e9}
+    nx.profile_select("")
+    del out, cp
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,7 +200,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the bootstrapping and MatMul measurements")
-    ap.add_argument("--only", choices=["bootstrap_dense", "bootstrap_sparse", "matmul"],
+    ap.add_argument("--only", choices=["bootstrap_dense", "bootstrap_sparse", "matmul", "ntt"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -185,6 +218,8 @@ def main():
             r = bench_bootstrap(nx, 16, 15, [51] + [46] * 16 + [51] * 14 + [58] * 3, 3, args.steps)
         elif args.only == "bootstrap_sparse":
             r = bench_bootstrap(nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, args.steps)
+        elif args.only == "ntt":
+            r = bench_ntt(nx, args.steps)
         else:
             r = bench_matmul(nx, args.steps)
         print(json.dumps({"workload": args.only, **r}), flush=True)
@@ -295,7 +330,11 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": (TRAFFIC_PER_ALGORITHMIC_BYTE[dominant] * k_bytes / max(launches, 1)
+                        if dominant in TRAFFIC_PER_ALGORITHMIC_BYTE else None),
+            "traffic_source": "calibrated FETCH_SIZE+WRITE_SIZE per algorithmic byte of this kernel class "
+                              "(profiles/ntt_fwd_n65536_38limbs_traffic.md) times this step's average "
+                              "algorithmic bytes per launch",
             "launches": launches,
             "avg_launch_us": avg_launch_ms * 1e3,
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
