@@ -67,6 +67,7 @@ public:
     // Levels bootstrapping consumes from the top data level.
     int depth() const;
     const BootPlan &plan() const { return plan_; }
+    const Params &params() const { return p_; }
     const std::vector<double> &cheb_coeffs() const { return cheb_; }
 
     void bootstrap(const Ciphertext &in, Ciphertext &out);

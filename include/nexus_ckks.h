@@ -67,6 +67,14 @@ int nx_context_info(const nx_context *ctx, int *log_n, size_t *data_primes, size
 /* moduli: L + special words (q_0..q_{L-1}, p_0..) */
 int nx_context_moduli(const nx_context *ctx, uint64_t *moduli);
 int nx_context_synchronize(nx_context *ctx);
+/* Lanes: bind the calling thread to lane `lane` (1..64) of the context, a HIP
+ * stream with its own memory cache; 0 returns the thread to the default stream.
+ * Work of threads on different lanes overlaps on the device and shares the
+ * context's keys.  A ciphertext written on one lane and read on another must be
+ * synchronised first (nx_context_synchronize on the writing thread).  No
+ * counterpart in the reference, whose Phantom path runs one global stream
+ * (cuda/thirdparty/phantom-fhe/include/util/globals.h). */
+int nx_context_bind_lane(nx_context *ctx, size_t lane);
 /* Galois element of a rotation step (SEAL util/galois.cpp:53-90, generator 5). */
 int nx_galois_elt_from_step(const nx_context *ctx, int step, uint32_t *elt);
 /* Device memory held by the context's cache: in use / cached, bytes. */

@@ -73,6 +73,29 @@ int nx_boot_slot_to_coeff(nx_bootstrapper *bt, const nx_ciphertext *f_re, const 
  * round; x is left unchanged */
 int nx_eval_argmax(nx_bootstrapper *bt, const nx_ciphertext *x, int len, nx_ciphertext *out);
 
+/* ---- serving: a request loop over the evaluator programs.  Workers, one per
+ * lane of the context (nx_context_bind_lane), take requests in arrival order as
+ * soon as they are free and the request's footprint (6 x its input ciphertext
+ * bytes) fits in memory_budget (0 = no limit); requests on different lanes
+ * overlap on the device and share the evaluator's keys.  boot (optional) gives
+ * the parameters of bootstrap jobs.  No counterpart in the reference (its
+ * drivers run one call at a time, src/main.cpp). */
+typedef struct nx_server nx_server;
+enum { NX_JOB_GELU = 0, NX_JOB_SOFTMAX = 1, NX_JOB_LAYER_NORM = 2, NX_JOB_BOOTSTRAP = 3 };
+int nx_server_create(nx_evaluator *ev, int lanes, size_t memory_budget, const nx_bootstrapper *boot,
+                     nx_server **out);
+int nx_server_destroy(nx_server *s);
+/* one request = count jobs (kind, row length for softmax / LayerNorm, input);
+ * the inputs are copied */
+int nx_server_submit(nx_server *s, const int *kinds, const int *lens, const nx_ciphertext *const *inputs,
+                     size_t count, uint64_t *request_id);
+/* blocks until the request is done; count outputs, one per job */
+int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outputs, size_t count);
+int nx_server_drain(nx_server *s);
+/* latency from submission to completion on the device */
+int nx_server_stats(const nx_server *s, size_t *completed, double *p50_ms, double *p99_ms, double *mean_ms,
+                    size_t *peak_concurrent);
+
 /* Host-only view of the linear-transform plan (genfftcoeff_3 / geninvfftcoeff_3,
  * Bootstrapper.cu:1137,1545); no device use.  which = 0: CoeffToSlot, 1:
  * SlotToCoeff; level in application order.  A level computes

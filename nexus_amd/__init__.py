@@ -55,6 +55,7 @@ _sig("nx_context_create_hybrid", [ctypes.c_int, ctypes.POINTER(ctypes.c_int), sz
                                   ctypes.POINTER(vp)])
 _sig("nx_context_key_info", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)])
 _sig("nx_context_destroy", [vp])
+_sig("nx_context_bind_lane", [vp, sz])
 _sig("nx_context_info", [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(sz), ctypes.POINTER(sz)])
 _sig("nx_context_moduli", [vp, u64p])
 _sig("nx_context_synchronize", [vp])
@@ -165,6 +166,11 @@ class Context:
 
     def synchronize(self):
         _check(_lib.nx_context_synchronize(self.h))
+
+    def bind_lane(self, lane):
+        """Run this thread's operations on lane `lane` (1..64: a HIP stream with its
+        own memory cache) of this context; 0 returns to the default stream."""
+        _check(_lib.nx_context_bind_lane(self.h, lane))
 
     def elt_from_step(self, step):
         e = ctypes.c_uint32()
@@ -680,6 +686,65 @@ class Bootstrapper:
         out = Ciphertext(self.ctx)
         _check(_lib.nx_boot_slot_to_coeff(self.h, f_re.h, f_im.h if f_im is not None else None, input_scale, out.h))
         return out
+
+
+# ---------------------------------------------------------------- serving (include/nexus_eval.h)
+_sig("nx_server_create", [vp, ctypes.c_int, sz, vp, ctypes.POINTER(vp)])
+_sig("nx_server_destroy", [vp])
+_sig("nx_server_submit", [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(vp), sz,
+                          ctypes.POINTER(ctypes.c_uint64)])
+_sig("nx_server_wait", [vp, ctypes.c_uint64, ctypes.POINTER(vp), sz])
+_sig("nx_server_drain", [vp])
+_sig("nx_server_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                         ctypes.POINTER(ctypes.c_double), ctypes.POINTER(sz)])
+
+JOB_GELU, JOB_SOFTMAX, JOB_LAYER_NORM, JOB_BOOTSTRAP = 0, 1, 2, 3
+
+
+class Server:
+    """Request loop: requests (lists of evaluator calls) run on `lanes` concurrent
+    streams, admitted in arrival order within `memory_budget` bytes of footprint."""
+
+    def __init__(self, ev, lanes=2, memory_budget=0, bootstrapper=None):
+        self.ev = ev
+        self.ctx = ev.ctx
+        self._keep = (ev, bootstrapper)
+        h = vp()
+        _check(_lib.nx_server_create(ev.h, lanes, memory_budget, bootstrapper.h if bootstrapper else None,
+                                     ctypes.byref(h)))
+        self.h = h
+        self._counts = {}
+
+    def __del__(self, _fn=_lib.nx_server_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def submit(self, jobs):
+        """jobs: [(kind, row_length, ciphertext), ...] -> request id"""
+        k = (ctypes.c_int * len(jobs))(*[j[0] for j in jobs])
+        ln = (ctypes.c_int * len(jobs))(*[j[1] for j in jobs])
+        ins = (vp * len(jobs))(*[j[2].h for j in jobs])
+        rid = ctypes.c_uint64()
+        _check(_lib.nx_server_submit(self.h, k, ln, ins, len(jobs), ctypes.byref(rid)))
+        self._counts[rid.value] = len(jobs)
+        return rid.value
+
+    def wait(self, rid):
+        cnt = self._counts.pop(rid)
+        outs = [Ciphertext(self.ctx) for _ in range(cnt)]
+        arr = (vp * cnt)(*[o.h for o in outs])
+        _check(_lib.nx_server_wait(self.h, rid, arr, cnt))
+        return outs
+
+    def drain(self):
+        _check(_lib.nx_server_drain(self.h))
+
+    def stats(self):
+        c, pk = sz(), sz()
+        p50, p99, mean = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        _check(_lib.nx_server_stats(self.h, ctypes.byref(c), ctypes.byref(p50), ctypes.byref(p99), ctypes.byref(mean),
+                                    ctypes.byref(pk)))
+        return {"completed": c.value, "p50_ms": p50.value, "p99_ms": p99.value, "mean_ms": mean.value,
+                "peak_concurrent": pk.value}
 
 
 # ---------------------------------------------------------------- live kernel timing (include/nexus_prof.h)

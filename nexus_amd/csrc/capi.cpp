@@ -101,6 +101,14 @@ int nx_context_create_hybrid(int log_n, const int *bit_sizes, size_t count, size
     });
 }
 
+int nx_context_bind_lane(nx_context *ctx, size_t lane)
+{
+    return guard([&] {
+        need(ctx, "ctx");
+        ctx->impl.bind_lane(lane);
+    });
+}
+
 int nx_context_key_info(const nx_context *ctx, size_t *special_primes, size_t *digits, size_t *key_words)
 {
     return guard([&] {

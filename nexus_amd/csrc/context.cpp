@@ -81,9 +81,10 @@ DevBuf &DevBuf::operator=(DevBuf &&o) noexcept
 
 void DevBuf::reset(Context *ctx, size_t words)
 {
-    if (p_ && cache_ == ctx->cache && words == words_) return;
+    const std::shared_ptr<DeviceCache> &cur = ctx->current_cache();
+    if (p_ && cache_ == cur && words == words_) return;
     release();
-    cache_ = ctx->cache;
+    cache_ = cur;
     words_ = words;
     p_ = words ? (uint64_t *)cache_->get(words * 8) : nullptr;
 }
@@ -94,6 +95,53 @@ void DevBuf::release()
     p_ = nullptr;
     words_ = 0;
     cache_.reset();
+}
+
+// ---------------------------------------------------------------- lanes
+namespace {
+thread_local const Context *tl_ctx = nullptr;
+thread_local Lane *tl_lane = nullptr;
+}  // namespace
+
+StreamRef::operator hipStream_t() const
+{
+    return (tl_ctx == c_ && tl_lane) ? tl_lane->stream : def_;
+}
+
+Lane *Context::lane(size_t index)
+{
+    require(index >= 1 && index <= 64, "lane index out of range", kOutOfRange);
+    std::lock_guard<std::mutex> g(lane_mu_);
+    if (lanes_.size() < index) lanes_.resize(index);
+    std::unique_ptr<Lane> &l = lanes_[index - 1];
+    if (!l) {
+        l.reset(new Lane());
+        NX_HIP(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+        l->cache = std::make_shared<DeviceCache>();
+    }
+    return l.get();
+}
+
+void Context::bind_lane(size_t index)
+{
+    if (index == 0) {
+        if (tl_ctx == this) {
+            tl_ctx = nullptr;
+            tl_lane = nullptr;
+        }
+        return;
+    }
+    Lane *l = lane(index);
+    tl_ctx = this;
+    tl_lane = l;
+}
+
+Lane *Context::bound_lane() const { return tl_ctx == this ? tl_lane : nullptr; }
+
+const std::shared_ptr<DeviceCache> &Context::current_cache() const
+{
+    Lane *l = bound_lane();
+    return l ? l->cache : cache;
 }
 
 // ---------------------------------------------------------------- context
@@ -113,7 +161,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
 
     int dev = 0;
     NX_HIP(hipGetDevice(&dev));
-    NX_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    NX_HIP(hipStreamCreateWithFlags(&stream.def_, hipStreamNonBlocking));
 
     const size_t P = moduli.size();
     std::vector<ulonglong2> fwd(P * n), inv(P * n), ninv(P);
@@ -232,7 +280,13 @@ void Context::build_hybrid_tables()
 
 Context::~Context()
 {
-    if (stream) (void)hipStreamSynchronize(stream);
+    if (stream.def_) (void)hipStreamSynchronize(stream.def_);
+    for (auto &l : lanes_)
+        if (l && l->stream) (void)hipStreamSynchronize(l->stream);
+    if (tl_ctx == this) {
+        tl_ctx = nullptr;
+        tl_lane = nullptr;
+    }
     (void)hipFree(d_modup_w);
     (void)hipFree(d_modup_conv);
     (void)hipFree(d_moddown_w);
@@ -245,7 +299,12 @@ Context::~Context()
     (void)hipFree(d_rescale_inv);
     (void)hipFree(d_p_inv);
     cache->trim();
-    if (stream) (void)hipStreamDestroy(stream);
+    for (auto &l : lanes_)
+        if (l) {
+            l->cache->trim();
+            if (l->stream) (void)hipStreamDestroy(l->stream);
+        }
+    if (stream.def_) (void)hipStreamDestroy(stream.def_);
 }
 
 int Context::total_bits(size_t limbs) const

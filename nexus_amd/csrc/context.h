@@ -58,6 +58,32 @@ private:
 
 class Context;
 
+// A lane: one HIP stream (default priority) and the memory cache of the work
+// ordered on it.  A thread binds a lane of a context to run that context's
+// operations on the lane's stream; threads on different lanes overlap on the
+// device while sharing the context's keys and tables.  A buffer returns to the
+// cache of the lane that allocated it, so a ciphertext used on another lane
+// must be released only after that lane has synchronised.
+struct Lane {
+    hipStream_t stream = nullptr;
+    std::shared_ptr<DeviceCache> cache;
+};
+
+// Stream of the calling thread for one context: its bound lane's stream, else
+// the context's default stream.  Converts to hipStream_t wherever a stream is
+// passed.
+class StreamRef {
+public:
+    explicit StreamRef(const Context *c) : c_(c) {}
+    operator hipStream_t() const;
+    hipStream_t default_stream() const { return def_; }
+
+private:
+    friend class Context;
+    const Context *c_;
+    hipStream_t def_ = nullptr;
+};
+
 // Device buffer from a context's cache.  The buffer shares ownership of the
 // cache, so releasing it stays valid even after its context is gone.
 class DevBuf {
@@ -131,8 +157,15 @@ public:
     std::vector<ModConst> mods;
     uint32_t hamming_weight;
     size_t sparse_slots;
-    hipStream_t stream = nullptr;
-    std::shared_ptr<DeviceCache> cache = std::make_shared<DeviceCache>();
+    StreamRef stream{this};                                              // the calling thread's stream
+    std::shared_ptr<DeviceCache> cache = std::make_shared<DeviceCache>();  // default lane's cache
+
+    // lanes (created on demand, index >= 1; 0 is the default stream) and the
+    // calling thread's binding
+    Lane *lane(size_t index);
+    void bind_lane(size_t index);   // 0 unbinds
+    Lane *bound_lane() const;
+    const std::shared_ptr<DeviceCache> &current_cache() const;
 
     // device tables
     DeviceTables tables;
@@ -175,6 +208,8 @@ public:
 
 private:
     void build_hybrid_tables();
+    std::vector<std::unique_ptr<Lane>> lanes_;
+    std::mutex lane_mu_;
     std::map<uint32_t, uint32_t *> galois_;
     std::mutex mu_;
 };

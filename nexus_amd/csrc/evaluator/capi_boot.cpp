@@ -38,6 +38,10 @@ nexus::Ciphertext input(nx_bootstrapper *bt, const nx_ciphertext *x)
 
 }  // namespace
 
+namespace nexus_capi {
+const nexus::Bootstrapper::Params *bootstrapper_params(const nx_bootstrapper *bt) { return &bt->bt->params(); }
+}  // namespace nexus_capi
+
 extern "C" {
 
 int nx_bootstrapper_create(nx_evaluator *ev, int log_slots, int K, int degree, int double_angles, int log_width,

@@ -1,0 +1,101 @@
+// C ABI of the request loop (include/nexus_eval.h, serving section).
+#include "../../../include/nexus/server.hpp"
+#include "eval_handles.h"
+
+using nexus_capi::need;
+using nexus_capi::run;
+
+struct nx_bootstrapper;  // capi_boot.cpp
+namespace nexus_capi {
+const nexus::Bootstrapper::Params *bootstrapper_params(const nx_bootstrapper *bt);  // capi_boot.cpp
+}
+
+struct nx_server {
+    std::unique_ptr<nexus::Server> impl;
+    nx_evaluator *ev;
+};
+
+extern "C" {
+
+int nx_server_create(nx_evaluator *ev, int lanes, size_t memory_budget, const nx_bootstrapper *boot,
+                     nx_server **out)
+{
+    return run([&] {
+        need(ev);
+        need(out);
+        if (!ev->rk || !ev->gk) throw std::invalid_argument("the evaluator needs relinearisation and Galois keys");
+        nexus::Server::Options o;
+        o.lanes = lanes;
+        o.memory_budget = memory_budget;
+        auto s = std::make_unique<nx_server>();
+        s->ev = ev;
+        s->impl = std::make_unique<nexus::Server>(*ev->ctx, ev->sk.get(), *ev->rk, *ev->gk, ev->ckks->scale, o,
+                                                  boot ? nexus_capi::bootstrapper_params(boot) : nullptr);
+        *out = s.release();
+    });
+}
+
+int nx_server_destroy(nx_server *s)
+{
+    return run([&] { delete s; });
+}
+
+int nx_server_submit(nx_server *s, const int *kinds, const int *lens, const nx_ciphertext *const *inputs,
+                     size_t count, uint64_t *request_id)
+{
+    return run([&] {
+        need(s);
+        need(kinds);
+        need(inputs);
+        need(request_id);
+        std::vector<nexus::Job> jobs;
+        for (size_t k = 0; k < count; k++) {
+            need(inputs[k]);
+            if (kinds[k] < 0 || kinds[k] > 3) throw std::invalid_argument("unknown job kind");
+            nexus::Job j;
+            j.kind = (nexus::JobKind)kinds[k];
+            j.len = lens ? lens[k] : 0;
+            j.input = nexus::Ciphertext::copy_of(*s->ev->ctx, inputs[k]);
+            jobs.push_back(std::move(j));
+        }
+        *request_id = s->impl->submit(std::move(jobs));
+    });
+}
+
+int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outputs, size_t count)
+{
+    return run([&] {
+        need(s);
+        need(outputs);
+        std::vector<nexus::Ciphertext> res = s->impl->wait(request_id);
+        if (res.size() != count) throw std::invalid_argument("output count does not match the request's jobs");
+        for (size_t k = 0; k < count; k++) {
+            need(outputs[k]);
+            nexus::check(nx_ciphertext_copy(outputs[k], res[k].get()));
+        }
+    });
+}
+
+int nx_server_drain(nx_server *s)
+{
+    return run([&] {
+        need(s);
+        s->impl->drain();
+    });
+}
+
+int nx_server_stats(const nx_server *s, size_t *completed, double *p50_ms, double *p99_ms, double *mean_ms,
+                    size_t *peak_concurrent)
+{
+    return run([&] {
+        need(s);
+        const nexus::Server::Stats st = s->impl->stats();
+        if (completed) *completed = st.completed;
+        if (p50_ms) *p50_ms = st.p50_ms;
+        if (p99_ms) *p99_ms = st.p99_ms;
+        if (mean_ms) *mean_ms = st.mean_ms;
+        if (peak_concurrent) *peak_concurrent = st.peak_concurrent;
+    });
+}
+
+}  // extern "C"

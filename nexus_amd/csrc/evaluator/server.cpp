@@ -1,0 +1,216 @@
+// Request loop over the evaluator programs (include/nexus/server.hpp).
+#include "../../../include/nexus/server.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <stdexcept>
+
+namespace nexus {
+
+double Server::now_ms()
+{
+    using namespace std::chrono;
+    return duration<double, std::milli>(steady_clock::now().time_since_epoch()).count();
+}
+
+Server::Server(const CKKSContext &ctx, const SecretKey *sk, RelinKeys &rk, GaloisKeys &gk, double scale, Options opt,
+               const Bootstrapper::Params *boot)
+    : ctx_(ctx), sk_(sk), rk_(rk), gk_(gk), scale_(scale), opt_(opt)
+{
+    if (opt_.lanes < 1 || opt_.lanes > 16) throw std::invalid_argument("Server: lanes must be 1..16");
+    if (boot) {
+        have_boot_ = true;
+        boot_ = *boot;
+    }
+    for (int i = 0; i < opt_.lanes; i++) workers_.emplace_back(&Server::worker, this, i + 1);
+}
+
+Server::~Server()
+{
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        stop_ = true;
+    }
+    cv_work_.notify_all();
+    for (auto &t : workers_) t.join();
+}
+
+uint64_t Server::submit(std::vector<Job> jobs)
+{
+    if (jobs.empty()) throw std::invalid_argument("Server: empty request");
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        if (!lane_error_.empty()) throw std::runtime_error("Server: a lane failed to start: " + lane_error_);
+    }
+    auto r = std::make_shared<Request>();
+    size_t bytes = 0;
+    const size_t n = ctx_.poly_degree();
+    for (const Job &j : jobs) {
+        if (!j.input.get()) throw std::invalid_argument("Server: job without input");
+        bytes += j.input.size() * j.input.coeff_modulus_size() * n * 8;
+    }
+    // the inputs were produced on the caller's stream: complete them before a
+    // worker's stream reads them
+    ctx_.synchronize();
+    r->jobs = std::move(jobs);
+    r->footprint = (size_t)(opt_.footprint_factor * (double)bytes);
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        r->id = next_id_++;
+        r->t_submit = now_ms();
+        queue_.push_back(r);
+        all_[r->id] = r;
+        submitted_++;
+    }
+    cv_work_.notify_one();
+    return r->id;
+}
+
+std::vector<Ciphertext> Server::wait(uint64_t id)
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    auto it = all_.find(id);
+    if (it == all_.end()) throw std::invalid_argument("Server: unknown request id");
+    std::shared_ptr<Request> r = it->second;
+    cv_done_.wait(lk, [&] { return r->done; });
+    all_.erase(id);
+    lk.unlock();
+    if (!r->error.empty()) throw std::runtime_error("Server: request failed: " + r->error);
+    // copy the outputs onto the caller's stream and memory, and complete the
+    // copies before the worker lane's buffers go back to its cache
+    std::vector<Ciphertext> res;
+    for (const Ciphertext &o : r->outputs) res.push_back(Ciphertext(o));
+    ctx_.synchronize();
+    r->outputs.clear();
+    return res;
+}
+
+void Server::drain()
+{
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_done_.wait(lk, [&] { return queue_.empty() && in_flight_ == 0; });
+}
+
+Server::Stats Server::stats() const
+{
+    std::lock_guard<std::mutex> g(mu_);
+    Stats s;
+    s.submitted = submitted_;
+    s.completed = latencies_.size();
+    s.peak_concurrent = peak_concurrent_;
+    s.peak_footprint = peak_footprint_;
+    if (!latencies_.empty()) {
+        std::vector<double> v = latencies_;
+        std::sort(v.begin(), v.end());
+        auto pct = [&](double p) {
+            const double pos = p * (double)(v.size() - 1);
+            const size_t lo = (size_t)pos;
+            const size_t hi = std::min(lo + 1, v.size() - 1);
+            return v[lo] + (pos - (double)lo) * (v[hi] - v[lo]);
+        };
+        s.p50_ms = pct(0.50);
+        s.p99_ms = pct(0.99);
+        double sum = 0;
+        for (double x : v) sum += x;
+        s.mean_ms = sum / (double)v.size();
+        s.max_ms = v.back();
+    }
+    return s;
+}
+
+void Server::worker(int lane)
+{
+    try {
+        run_lane(lane);
+    } catch (const std::exception &e) {
+        // a lane that cannot start fails the requests it would have served
+        std::lock_guard<std::mutex> g(mu_);
+        lane_error_ = e.what();
+    }
+}
+
+void Server::run_lane(int lane)
+{
+    // this thread's work runs on its own lane (stream + memory cache)
+    check(nx_context_bind_lane(ctx_.get(), (size_t)lane));
+    CKKSEvaluator ck(&ctx_, sk_, &rk_, &gk_, scale_);
+    std::unique_ptr<Bootstrapper> boot;
+    if (have_boot_) boot.reset(new Bootstrapper(ck, boot_));
+    GELUEvaluator gelu(ck);
+    SoftmaxEvaluator softmax(ck);
+    LNEvaluator ln(ck);
+    for (;;) {
+        std::shared_ptr<Request> r;
+        {
+            std::unique_lock<std::mutex> lk(mu_);
+            for (;;) {
+                if (stop_) {
+                    check(nx_context_bind_lane(ctx_.get(), 0));
+                    return;
+                }
+                // admission: the oldest request whose footprint fits; with nothing
+                // in flight the oldest is always admitted
+                auto pick = queue_.end();
+                for (auto it = queue_.begin(); it != queue_.end(); ++it) {
+                    if (opt_.memory_budget == 0 || in_flight_ == 0 ||
+                        in_flight_bytes_ + (*it)->footprint <= opt_.memory_budget) {
+                        pick = it;
+                        break;
+                    }
+                }
+                if (pick != queue_.end()) {
+                    r = *pick;
+                    queue_.erase(pick);
+                    in_flight_++;
+                    in_flight_bytes_ += r->footprint;
+                    peak_concurrent_ = std::max(peak_concurrent_, in_flight_);
+                    peak_footprint_ = std::max(peak_footprint_, in_flight_bytes_);
+                    break;
+                }
+                cv_work_.wait(lk);
+            }
+        }
+        std::vector<Ciphertext> outs;
+        std::string err;
+        try {
+            for (Job &j : r->jobs) {
+                Ciphertext out;
+                switch (j.kind) {
+                case JobKind::Gelu: gelu.gelu(j.input, out); break;
+                case JobKind::Softmax: softmax.softmax(j.input, out, j.len); break;
+                case JobKind::LayerNorm: ln.layer_norm(j.input, out, j.len); break;
+                case JobKind::Bootstrap:
+                    if (!boot) throw std::invalid_argument("server has no bootstrapper");
+                    boot->bootstrap(j.input, out);
+                    break;
+                default: throw std::invalid_argument("unknown job kind");
+                }
+                outs.push_back(std::move(out));
+            }
+            // completion on this lane's stream before the outputs are handed over
+            ctx_.synchronize();
+        } catch (const std::exception &e) {
+            err = e.what();
+            try {
+                ctx_.synchronize();
+            } catch (...) {
+            }
+        }
+        const double t = now_ms();
+        r->jobs.clear();  // inputs: their last use on this lane is complete
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            r->outputs = std::move(outs);
+            r->error = err;
+            r->t_done = t;
+            r->done = true;
+            if (err.empty()) latencies_.push_back(t - r->t_submit);
+            in_flight_--;
+            in_flight_bytes_ -= r->footprint;
+        }
+        cv_done_.notify_all();
+        cv_work_.notify_all();
+    }
+}
+
+}  // namespace nexus

@@ -29,7 +29,7 @@ std::mutex g_mu;
 bool g_all = false;
 std::vector<std::string> g_sel;
 std::map<std::string, Stat> g_stats;
-std::map<std::string, hipEvent_t> g_open;
+std::map<std::pair<std::string, hipStream_t>, hipEvent_t> g_open;  // open launch per (class, stream)
 std::vector<Pending> g_pending;
 std::vector<hipEvent_t> g_pool;
 
@@ -84,9 +84,9 @@ void prof_record(const char *cls, hipStream_t s, bool begin, double bytes)
     if (begin) {
         hipEvent_t e = take();
         NX_HIP(hipEventRecord(e, s));
-        g_open[cls] = e;
+        g_open[{cls, s}] = e;
     } else {
-        auto it = g_open.find(cls);
+        auto it = g_open.find({cls, s});
         if (it == g_open.end()) return;
         hipEvent_t b = take();
         NX_HIP(hipEventRecord(b, s));
