@@ -103,6 +103,64 @@ def cpu_baseline(gpu_ks_bytes, n, budget_s):
     }
 
 
+def served_requests(nx, ctx, sk, rng, count):
+    """Requests of the serving workload: the nonlinear block of one BERT-base
+    layer for a sequence of length S ~ U{16..128}: GELU over S x 3072 values,
+    softmax over 12 heads x S rows (rows of 128, stride 256), LayerNorm over 2 x
+    S rows of 768 (16 rows per ciphertext).  Inputs are encrypted up front."""
+    slots = ctx.slot_count
+    reqs = []
+    for i in range(count):
+        S = int(rng.integers(16, 129))
+        jobs = []
+        for _ in range(-(-S * 3072 // slots)):
+            v = np.clip(rng.normal(0, 1, slots), -5, 5)
+            jobs.append((nx.JOB_GELU, 0, v))
+        for _ in range(-(-12 * S // 128)):
+            v = np.zeros(slots)
+            rows = np.clip(rng.normal(0, 3, (128, 128)), -10, 10)
+            rows = rows - rows.max(axis=1, keepdims=True)
+            for r in range(128):
+                v[256 * r: 256 * r + 128] = np.clip(rows[r], -10, 0)
+            jobs.append((nx.JOB_SOFTMAX, 128, v))
+        for _ in range(2 * (-(-S // 16))):
+            v = np.zeros(slots)
+            for r in range(16):
+                v[2048 * r: 2048 * r + 768] = rng.normal(0, 1, 768) * np.sqrt(30.0)
+            jobs.append((nx.JOB_LAYER_NORM, 1024, v))
+        enc = [(k, ln, nx.encrypt(ctx, sk, nx.encode(ctx, v, SCALE), seed=int(rng.integers(1 << 62))))
+               for k, ln, v in jobs]
+        reqs.append((S, enc))
+    return reqs
+
+
+def run_served(nx, ev, reqs, arrivals, lanes):
+    """Submit the requests at the given arrival times (s) and wait for all of them;
+    returns (makespan s, server stats)."""
+    import threading
+    srv = nx.Server(ev, lanes=lanes, memory_budget=64 << 30)
+    ids = [None] * len(reqs)
+    t0 = time.perf_counter()
+
+    def submitter():
+        for i, (S, jobs) in enumerate(reqs):
+            delay = arrivals[i] - (time.perf_counter() - t0)
+            if delay > 0:
+                time.sleep(delay)
+            ids[i] = srv.submit(jobs)
+
+    th = threading.Thread(target=submitter)
+    th.start()
+    th.join()
+    srv.drain()
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    for rid in ids:
+        srv.wait(rid)
+    del srv
+    return makespan, st
+
+
 def bench_bootstrap(nx, logN, logn, chain, alpha, reps):
     """Seconds per bootstrap of one ciphertext (ModRaise .. SlotToCoeff) and its
     error against the input message; diagonals are encoded by an untimed first call."""
@@ -200,6 +258,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the bootstrapping and MatMul measurements")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--requests", type=int, default=12)
+    ap.add_argument("--lanes", type=int, default=3)
     ap.add_argument("--only", choices=["bootstrap_dense", "bootstrap_sparse", "matmul", "ntt"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
@@ -340,6 +401,33 @@ def main():
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
         },
     }
+    if not args.no_served:
+        log("served: %d requests, lengths U{16..128}, Poisson arrivals" % args.requests)
+        srng = np.random.default_rng(77)
+        reqs = served_requests(nx, ctx, sk, srng, args.requests)
+        cts = sum(len(j) for _, j in reqs)
+        # arrival rate: 1.2x what the plain path sustains at the measured step cost per ciphertext
+        per_ct_s = ms_per_step / 1e3 / len(g_ct + s_ct + l_ct)
+        rate = 1.2 / (per_ct_s * cts / len(reqs))
+        arrivals = np.cumsum(srng.exponential(1.0 / rate, len(reqs)))
+        arrivals -= arrivals[0]
+        ctx.synchronize()
+        span1, st1 = run_served(nx, ev, reqs, arrivals, 1)
+        log("plain per-call (1 lane): %.2f s, p50 %.0f ms, p99 %.0f ms" % (span1, st1["p50_ms"], st1["p99_ms"]))
+        span3, st3 = run_served(nx, ev, reqs, arrivals, args.lanes)
+        log("served (%d lanes): %.2f s, p50 %.0f ms, p99 %.0f ms" % (args.lanes, span3, st3["p50_ms"], st3["p99_ms"]))
+        res["served"] = {
+            "workload": "nonlinear block of one BERT-base layer per request, S ~ U{16..128}, Poisson arrivals at "
+                        "%.2f requests/s (1.2x the plain path's capacity)" % rate,
+            "requests": len(reqs), "ciphertexts": cts, "mean_seq_len": float(np.mean([S for S, _ in reqs])),
+            "lanes": args.lanes,
+            "served": {"blocks_per_s": len(reqs) / span3, "p50_ms": st3["p50_ms"], "p99_ms": st3["p99_ms"],
+                       "mean_ms": st3["mean_ms"], "peak_concurrent": st3["peak_concurrent"], "makespan_s": span3},
+            "plain_per_call": {"blocks_per_s": len(reqs) / span1, "p50_ms": st1["p50_ms"], "p99_ms": st1["p99_ms"],
+                               "mean_ms": st1["mean_ms"], "makespan_s": span1},
+            "throughput_ratio": span1 / span3,
+        }
+        del reqs
     n_main = ctx.n
     if not args.no_extra:
         del ev, gk, rk, sk, g_ct, s_ct, l_ct, outs
