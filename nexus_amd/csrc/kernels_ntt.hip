@@ -40,10 +40,14 @@ struct NttKArgs {
 
 constexpr int kThreads = 256;
 
-template <int LOGS, int ROLE>
+// CW: sub-transforms per workgroup -- columns of the column pass (16 for large
+// batches, 8 for small ones: more workgroups when there are few limbs), and an
+// upper bound on the rows of the row pass.
+template <int LOGS, int ROLE, int CW>
 struct Geometry {
     static constexpr int S = 1 << LOGS;
-    static constexpr int C = ROLE == kCol ? 16 : (ROLE == kRow ? (S >= 2048 ? 1 : 2048 / S) : 1);
+    static constexpr int ROWS = S >= 2048 ? 1 : 2048 / S;
+    static constexpr int C = ROLE == kCol ? CW : (ROLE == kRow ? (ROWS < CW ? ROWS : CW) : 1);
     static constexpr int TILE = S * C;
 };
 
@@ -55,11 +59,11 @@ __device__ __forceinline__ int lds_addr(int cc, int pos)
 }
 
 // Forward radix-2^K step over local stages s .. s+K-1.
-template <int LOGS, int ROLE, int K>
+template <int LOGS, int ROLE, int K, int CW>
 __device__ __forceinline__ void fwd_step(uint64_t *lds, int s, const ulonglong2 *tw, uint64_t q, uint32_t m0,
                                          uint32_t row0)
 {
-    using G = Geometry<LOGS, ROLE>;
+    using G = Geometry<LOGS, ROLE, CW>;
     constexpr int R = 1 << K;
     constexpr int GROUPS_PER = G::S / R;
     constexpr int GROUPS = G::TILE / R;
@@ -109,11 +113,11 @@ __device__ __forceinline__ void fwd_step(uint64_t *lds, int s, const ulonglong2 
 }
 
 // Inverse radix-2^K step over local stages s .. s+K-1 (gaps 2^s .. 2^(s+K-1)).
-template <int LOGS, int ROLE, int K>
+template <int LOGS, int ROLE, int K, int CW>
 __device__ __forceinline__ void inv_step(uint64_t *lds, int s, const ulonglong2 *tw, uint64_t q, uint32_t n,
                                          uint32_t mx, uint32_t row0)
 {
-    using G = Geometry<LOGS, ROLE>;
+    using G = Geometry<LOGS, ROLE, CW>;
     constexpr int R = 1 << K;
     constexpr int GROUPS_PER = G::S / R;
     constexpr int GROUPS = G::TILE / R;
@@ -162,10 +166,10 @@ __device__ __forceinline__ void inv_step(uint64_t *lds, int s, const ulonglong2 
     }
 }
 
-template <int LOGS, int ROLE, bool FWD>
+template <int LOGS, int ROLE, bool FWD, int CW>
 __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
 {
-    using G = Geometry<LOGS, ROLE>;
+    using G = Geometry<LOGS, ROLE, CW>;
     __shared__ uint64_t lds[G::TILE];
     const uint32_t bidx = blockIdx.y;
     const uint32_t poly = bidx / a.limbs, limb = bidx - poly * a.limbs;
@@ -200,16 +204,16 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
         if constexpr (LOGS >= 3) {
 #pragma unroll
             for (int st = 0; st < LOGS / 3; st++) {
-                fwd_step<LOGS, ROLE, 3>(lds, s, tw, q, m0, first);
+                fwd_step<LOGS, ROLE, 3, CW>(lds, s, tw, q, m0, first);
                 s += 3;
                 __syncthreads();
             }
         }
         if constexpr (LOGS % 3 == 2) {
-            fwd_step<LOGS, ROLE, 2>(lds, s, tw, q, m0, first);
+            fwd_step<LOGS, ROLE, 2, CW>(lds, s, tw, q, m0, first);
             __syncthreads();
         } else if constexpr (LOGS % 3 == 1) {
-            fwd_step<LOGS, ROLE, 1>(lds, s, tw, q, m0, first);
+            fwd_step<LOGS, ROLE, 1, CW>(lds, s, tw, q, m0, first);
             __syncthreads();
         }
     } else {
@@ -218,16 +222,16 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
         if constexpr (LOGS >= 3) {
 #pragma unroll
             for (int st = 0; st < LOGS / 3; st++) {
-                inv_step<LOGS, ROLE, 3>(lds, s, tw, q, n, mx, first);
+                inv_step<LOGS, ROLE, 3, CW>(lds, s, tw, q, n, mx, first);
                 s += 3;
                 __syncthreads();
             }
         }
         if constexpr (LOGS % 3 == 2) {
-            inv_step<LOGS, ROLE, 2>(lds, s, tw, q, n, mx, first);
+            inv_step<LOGS, ROLE, 2, CW>(lds, s, tw, q, n, mx, first);
             __syncthreads();
         } else if constexpr (LOGS % 3 == 1) {
-            inv_step<LOGS, ROLE, 1>(lds, s, tw, q, n, mx, first);
+            inv_step<LOGS, ROLE, 1, CW>(lds, s, tw, q, n, mx, first);
             __syncthreads();
         }
     }
@@ -258,10 +262,10 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     }
 }
 
-template <int LOGS, int ROLE, bool FWD>
+template <int LOGS, int ROLE, bool FWD, int CW = 16>
 void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
 {
-    using G = Geometry<LOGS, ROLE>;
+    using G = Geometry<LOGS, ROLE, CW>;
     const uint32_t n = 1u << a.logn;
     const uint32_t tiles = ROLE == kWhole ? 1u : (n / G::TILE);
     dim3 grid(tiles, batch);
@@ -271,7 +275,7 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     // twiddle entries {w, w'} (16 B each)
     const double tw = ROLE == kWhole ? (double)n : (ROLE == kCol ? (double)G::S : (double)(n - n / G::S));
     ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)batch * (16.0 * n + 16.0 * tw));
-    hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD>), grid, dim3(kThreads), 0, s, a);
+    hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
 }
 
 template <bool FWD>
@@ -287,13 +291,24 @@ void launch_whole(const NttKArgs &a, uint32_t batch, hipStream_t s)
     }
 }
 
+// Column tiles of 8 below 128 limbs (twice the workgroups), 16 otherwise;
+// narrower tiles for still smaller batches measured no faster.
+template <int LOGS, int ROLE, bool FWD>
+void launch_sized(const NttKArgs &a, uint32_t batch, hipStream_t s)
+{
+    if (ROLE == kCol && batch < 128)
+        launch_one<LOGS, ROLE, FWD, 8>(a, batch, s);
+    else
+        launch_one<LOGS, ROLE, FWD, 16>(a, batch, s);
+}
+
 template <int ROLE, bool FWD>
 void launch_pass(int logs, const NttKArgs &a, uint32_t batch, hipStream_t s)
 {
     switch (logs) {
-    case 7: launch_one<7, ROLE, FWD>(a, batch, s); break;
-    case 8: launch_one<8, ROLE, FWD>(a, batch, s); break;
-    case 9: launch_one<9, ROLE, FWD>(a, batch, s); break;
+    case 7: launch_sized<7, ROLE, FWD>(a, batch, s); break;
+    case 8: launch_sized<8, ROLE, FWD>(a, batch, s); break;
+    case 9: launch_sized<9, ROLE, FWD>(a, batch, s); break;
     default: break;
     }
 }
