@@ -176,6 +176,26 @@ int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx
 int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
                                  nx_ciphertext *dest);
 
+/* ---- SEAL 4.1 serialization, compression mode "none" (reference
+ * thirdparty/SEAL-4.1-bs/native/src/seal/serialization.h:76-91 header,
+ * ciphertext.cpp / plaintext.cpp / kswitchkeys.cpp save_members and load).
+ * Save: buf == NULL returns the size in *size only.  Load validates the header,
+ * the parms_id (which level of this context), sizes and residue ranges.
+ * Key sets: galois = 0 saves the relinearisation key (one entry), 1 the Galois
+ * keys (N entries indexed by (elt - 1) / 2, empty where absent). */
+int nx_ciphertext_save(const nx_ciphertext *ct, uint8_t *buf, size_t capacity, size_t *size);
+int nx_ciphertext_load(nx_context *ctx, const uint8_t *buf, size_t size, nx_ciphertext *ct);
+int nx_plaintext_save(const nx_plaintext *pt, uint8_t *buf, size_t capacity, size_t *size);
+int nx_plaintext_load(nx_context *ctx, const uint8_t *buf, size_t size, nx_plaintext *pt);
+int nx_kswitch_keys_save(const nx_kswitch_keys *keys, int galois, uint8_t *buf, size_t capacity, size_t *size);
+int nx_kswitch_keys_load(nx_context *ctx, const uint8_t *buf, size_t size, int galois, nx_kswitch_keys *keys);
+/* SEAL's parms_id of a level (EncryptionParameters::compute_parms_id,
+ * encryptionparams.cpp): BLAKE2b-256 of {2 (CKKS), N, moduli of the level, 0};
+ * chain_index 0 = key level.  id: 4 words. */
+int nx_parms_id(const nx_context *ctx, size_t chain_index, uint64_t *id);
+/* BLAKE2b (RFC 7693, unkeyed), digest_size 1..64 (SEAL util/hash.h HashFunction) */
+int nx_hash_blake2b(const uint8_t *data, size_t size, uint8_t *digest, size_t digest_size);
+
 #ifdef __cplusplus
 }
 #endif

@@ -472,6 +472,60 @@ def multiply_power_of_x(a, exponent, dest=None):
 
 
 _sig("nx_mod_raise", [vp, vp])
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_sig("nx_hash_blake2b", [_u8p, sz, _u8p, sz])
+_sig("nx_parms_id", [vp, sz, u64p])
+for _n in ("ciphertext", "plaintext"):
+    _sig("nx_%s_save" % _n, [vp, _u8p, sz, ctypes.POINTER(sz)])
+    _sig("nx_%s_load" % _n, [vp, _u8p, sz, vp])
+_sig("nx_kswitch_keys_save", [vp, ctypes.c_int, _u8p, sz, ctypes.POINTER(sz)])
+_sig("nx_kswitch_keys_load", [vp, _u8p, sz, ctypes.c_int, vp])
+
+
+def blake2b(data, digest_size=32):
+    """BLAKE2b (RFC 7693) of bytes, as SEAL's HashFunction computes it."""
+    b = (ctypes.c_uint8 * max(len(data), 1)).from_buffer_copy(bytes(data) or b"\0")
+    out = (ctypes.c_uint8 * digest_size)()
+    _check(_lib.nx_hash_blake2b(b, len(data), out, digest_size))
+    return bytes(out)
+
+
+def parms_id(ctx, chain_index):
+    out = np.zeros(4, dtype=np.uint64)
+    _check(_lib.nx_parms_id(ctx.h, chain_index, out.ctypes.data_as(u64p)))
+    return [int(x) for x in out]
+
+
+def _save(fn, *args):
+    n = sz()
+    _check(fn(*args, None, 0, ctypes.byref(n)))
+    buf = (ctypes.c_uint8 * n.value)()
+    _check(fn(*args, buf, n.value, ctypes.byref(n)))
+    return bytes(buf)
+
+
+def save(obj, galois=False):
+    """SEAL 4.1 bytes of a Ciphertext, Plaintext or KSwitchKeys (compression none)."""
+    if isinstance(obj, Ciphertext):
+        return _save(_lib.nx_ciphertext_save, obj.h)
+    if isinstance(obj, Plaintext):
+        return _save(_lib.nx_plaintext_save, obj.h)
+    return _save(_lib.nx_kswitch_keys_save, obj.h, 1 if galois else 0)
+
+
+def load(ctx, data, kind, galois=False):
+    """kind: "ciphertext", "plaintext" or "keys"."""
+    buf = (ctypes.c_uint8 * len(data)).from_buffer_copy(data)
+    if kind == "ciphertext":
+        out = Ciphertext(ctx)
+        _check(_lib.nx_ciphertext_load(ctx.h, buf, len(data), out.h))
+    elif kind == "plaintext":
+        out = Plaintext(ctx)
+        _check(_lib.nx_plaintext_load(ctx.h, buf, len(data), out.h))
+    else:
+        out = KSwitchKeys(ctx)
+        _check(_lib.nx_kswitch_keys_load(ctx.h, buf, len(data), 1 if galois else 0, out.h))
+    return out
 _sig("nx_rotate_hoisted", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, ctypes.POINTER(vp)])
 _sig("nx_hoisting", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, vp])
 

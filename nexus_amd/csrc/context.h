@@ -274,6 +274,16 @@ struct CtBatch {
     uint64_t *ct(size_t i) const { return data.get() + i * 2 * limbs * ctx->n; }
 };
 
+// SEAL 4.1 serialization (compression "none") and parms_id -- serialize.cpp
+void blake2b(uint8_t *out, size_t outlen, const uint8_t *in, size_t inlen);
+void parms_id(const Context &c, size_t limbs, uint64_t out[4]);  // limbs 0 = key level
+std::vector<uint8_t> save_ciphertext(const Ciphertext &ct);
+void load_ciphertext(Context &c, const uint8_t *data, size_t n, Ciphertext &out);
+std::vector<uint8_t> save_plaintext(const Plaintext &pt);
+void load_plaintext(Context &c, const uint8_t *data, size_t n, Plaintext &out);
+std::vector<uint8_t> save_kswitch_keys(const KSwitchKeys &keys, bool galois);
+void load_kswitch_keys(Context &c, const uint8_t *data, size_t n, bool galois, KSwitchKeys &out);
+
 // encoder (host FFT + device NTT) -- encoder.cpp
 void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,
                    Plaintext &out);
