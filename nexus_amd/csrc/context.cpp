@@ -216,6 +216,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     tables.inv = d_inv;
     tables.ninv = d_ninv;
     tables.mod = d_mod;
+    upload_encoder_tables(*this);
 }
 
 // Base-conversion constants of hybrid key switching (fast basis extension:
@@ -298,6 +299,8 @@ Context::~Context()
     (void)hipFree(d_mod);
     (void)hipFree(d_rescale_inv);
     (void)hipFree(d_p_inv);
+    (void)hipFree(d_enc_index);
+    (void)hipFree(d_enc_roots);
     cache->trim();
     for (auto &l : lanes_)
         if (l) {

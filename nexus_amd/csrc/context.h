@@ -200,6 +200,11 @@ public:
     const uint32_t *galois_table(uint32_t elt);
     uint32_t elt_from_step(int step) const;
 
+    // encoder tables on the device: matrix_reps_index_map (n) and the inverse
+    // complex root powers (n) of the encoding FFT
+    uint32_t *d_enc_index = nullptr;
+    double2 *d_enc_roots = nullptr;
+
     // host copies of SEAL-ordered root tables (encoder uses complex roots, not these)
     std::vector<std::vector<uint64_t>> fwd_host, inv_host;
     // NTT-domain images of monomials x^k, keyed by (k, limbs)
@@ -284,7 +289,8 @@ void load_plaintext(Context &c, const uint8_t *data, size_t n, Plaintext &out);
 std::vector<uint8_t> save_kswitch_keys(const KSwitchKeys &keys, bool galois);
 void load_kswitch_keys(Context &c, const uint8_t *data, size_t n, bool galois, KSwitchKeys &out);
 
-// encoder (host FFT + device NTT) -- encoder.cpp
+// encoder (device FFT and NTT) -- encoder.cpp
+void upload_encoder_tables(Context &c);
 void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,
                    Plaintext &out);
 void encode_constant(Context &c, double value, size_t limbs, double scale, Plaintext &out);

@@ -1,7 +1,8 @@
-// CKKS encoder / decoder.  The complex FFT, rounding and RNS decomposition run
-// on the host in the order SEAL-4.1-bs uses (ckks.cpp:13-71, ckks.h:457-760,
-// util/croots.cpp, util/dwthandler.h), so the integer plaintext coefficients
-// are the ones SEAL produces; the NTT of the plaintext runs on the GPU.
+// CKKS encoder / decoder (SEAL-4.1-bs ckks.cpp:13-71, ckks.h:457-760,
+// util/croots.cpp, util/dwthandler.h).  Encoding runs on the device: the
+// complex FFT, rounding and RNS decomposition in kernels_encode.hip reproduce
+// SEAL's floating-point order bit for bit, then the NTT.  Decoding composes
+// the CRT value and runs the inverse FFT on the host in SEAL's order.
 // Compiled with -ffp-contract=off: no fused multiply-adds may change the
 // rounding of the double-precision butterflies.
 #include <cmath>
@@ -77,37 +78,6 @@ const EncoderTables &encoder_tables(int logn)
     return ref;
 }
 
-// Gentleman-Sande over doubles, bit-reversed in, natural out, scalar in the
-// last stage (dwthandler.h transform_from_rev).
-void fft_from_rev(cd *v, int logn, const cd *roots, double scalar)
-{
-    const size_t n = size_t(1) << logn;
-    size_t gap = 1, m = n >> 1;
-    const cd *r = roots;
-    for (; m > 1; m >>= 1) {
-        size_t off = 0;
-        for (size_t i = 0; i < m; i++) {
-            const cd w = *++r;
-            cd *x = v + off, *y = x + gap;
-            for (size_t j = 0; j < gap; j++) {
-                const cd u = x[j], t = y[j];
-                x[j] = u + t;
-                y[j] = (u - t) * w;
-            }
-            off += gap << 1;
-        }
-        gap <<= 1;
-    }
-    const cd w = *++r;
-    const cd sw = w * scalar;
-    cd *x = v, *y = v + gap;
-    for (size_t j = 0; j < gap; j++) {
-        const cd u = x[j], t = y[j];
-        x[j] = (u + t) * scalar;
-        y[j] = (u - t) * sw;
-    }
-}
-
 // Cooley-Tukey over doubles, natural in, bit-reversed out (transform_to_rev).
 void fft_to_rev(cd *v, int logn, const cd *roots)
 {
@@ -168,6 +138,21 @@ struct CoeffDecomposer {
 
 }  // namespace
 
+void upload_encoder_tables(Context &c)
+{
+    const EncoderTables &t = encoder_tables(c.logn);
+    std::vector<uint32_t> idx(c.n);
+    for (size_t i = 0; i < c.n; i++) idx[i] = (uint32_t)t.index_map[i];
+    static_assert(sizeof(cd) == sizeof(double2), "complex<double> is two doubles");
+    NX_HIP(hipMalloc(&c.d_enc_index, c.n * sizeof(uint32_t)));
+    NX_HIP(hipMalloc(&c.d_enc_roots, c.n * sizeof(double2)));
+    NX_HIP(hipMemcpy(c.d_enc_index, idx.data(), c.n * sizeof(uint32_t), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(c.d_enc_roots, t.iroots.data(), c.n * sizeof(double2), hipMemcpyHostToDevice));
+}
+
+// ckks.h:457-620 encode_internal: values -> index_map scatter with conjugates
+// -> transform_from_rev with scale / n -> round -> residues -> NTT.  The
+// largest |coefficient| comes back to the host for SEAL's size check.
 void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,
                    Plaintext &out)
 {
@@ -176,31 +161,16 @@ void encode_vector(Context &c, const double *re, const double *im, size_t count,
     require(limbs >= 1 && limbs <= c.L, "parms_id is not valid for encryption parameters");
     const int total_bits = c.total_bits(limbs);
     require(!(scale <= 0 || ((int)std::log2(scale) + 1 >= total_bits)), "scale out of bounds");
-    const EncoderTables &t = encoder_tables(c.logn);
-    std::vector<cd> v(n, cd(0, 0));
-    for (size_t i = 0; i < count; i++) {
-        const cd x(re[i], im ? im[i] : 0.0);
-        v[t.index_map[i]] = x;
-        v[t.index_map[i + slots]] = std::conj(x);
-    }
-    const double fix = scale / (double)n;
-    fft_from_rev(v.data(), c.logn, t.iroots.data(), fix);
-    double max_coeff = 0;
-    for (size_t i = 0; i < n; i++) max_coeff = std::max(max_coeff, std::fabs(v[i].real()));
-    const int bits = (int)std::ceil(std::log2(std::max(max_coeff, 1.0))) + 1;
-    require(bits < total_bits, "encoded values are too large");
-    std::vector<uint64_t> host(limbs * n);
-    CoeffDecomposer dec;
-    dec.bits = bits;
-    for (size_t i = 0; i < n; i++) {
-        double x = std::round(v[i].real());
-        const bool neg = std::signbit(x);
-        dec.load(std::fabs(x));
-        for (size_t j = 0; j < limbs; j++) {
-            const uint64_t q = c.moduli[j];
-            const uint64_t r = dec.mod(q);
-            host[j * n + i] = neg ? (r ? q - r : 0) : r;
-        }
+    const uint32_t blocks = encode_blocks(c.logn);
+    // work: [n] double2 | re[count] | im[count] | blockmax[blocks]
+    DevBuf work(&c, 2 * n + 2 * count + blocks);
+    double2 *v = reinterpret_cast<double2 *>(work.get());
+    double *d_re = reinterpret_cast<double *>(work.get() + 2 * n);
+    double *d_im = d_re + count;
+    uint64_t *blockmax = work.get() + 2 * n + 2 * count;
+    if (count) {
+        NX_HIP(hipMemcpyAsync(d_re, re, count * 8, hipMemcpyHostToDevice, c.stream));
+        if (im) NX_HIP(hipMemcpyAsync(d_im, im, count * 8, hipMemcpyHostToDevice, c.stream));
     }
     Plaintext p;
     p.ctx = &c;
@@ -208,10 +178,19 @@ void encode_vector(Context &c, const double *re, const double *im, size_t count,
     p.scale = scale;
     p.ntt = true;
     p.data.reset(&c, limbs * n);
-    NX_HIP(hipMemcpyAsync(p.data.get(), host.data(), limbs * n * 8, hipMemcpyHostToDevice, c.stream));
-    ntt_fwd(c, p.data.get(), 1, limbs, limbs * n);
-    // the host staging vector dies here: make the copy complete first
+    encode_fft(c.tables, p.data.get(), v, d_re, im ? d_im : nullptr, (uint32_t)count, c.d_enc_index, c.d_enc_roots,
+               scale / (double)n, (uint32_t)limbs, blockmax, c.stream);
+    std::vector<uint64_t> maxima(blocks);
+    NX_HIP(hipMemcpyAsync(maxima.data(), blockmax, blocks * 8, hipMemcpyDeviceToHost, c.stream));
     NX_HIP(hipStreamSynchronize(c.stream));
+    uint64_t mb = 0;
+    for (uint64_t x : maxima) mb = std::max(mb, x);
+    double max_coeff;
+    std::memcpy(&max_coeff, &mb, 8);
+    require(std::isfinite(max_coeff), "encoded values are too large");
+    const int bits = (int)std::ceil(std::log2(std::max(max_coeff, 1.0))) + 1;
+    require(bits < total_bits, "encoded values are too large");
+    ntt_fwd(c, p.data.get(), 1, limbs, limbs * n);
     out = std::move(p);
 }
 

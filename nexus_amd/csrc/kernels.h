@@ -164,4 +164,16 @@ void mod_raise(const DeviceTables &t, uint64_t *out, const uint64_t *in0, uint32
 void poly_negacyclic_shift(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t k, uint32_t polys,
                            uint32_t limbs, hipStream_t s);
 
+// ---------------------------------------------------------------- encode
+// CKKS encode of count <= n/2 complex values (re, im device arrays; im may be
+// null) to the rounded coefficients' residues out[limbs][n] (coefficient
+// domain), bit-identical to SEAL's host encoder.  work holds n double2;
+// index_map / roots are SEAL's matrix_reps_index_map and inverse root powers;
+// scalar = scale / n.  blockmax[encode_blocks(logn)] receives per-block maxima
+// of |coefficient| as IEEE bit patterns.
+uint32_t encode_blocks(int logn);
+void encode_fft(const DeviceTables &t, uint64_t *out, double2 *work, const double *re, const double *im,
+                uint32_t count, const uint32_t *index_map, const double2 *roots, double scalar, uint32_t limbs,
+                uint64_t *blockmax, hipStream_t s);
+
 }  // namespace nx

@@ -6,6 +6,8 @@ ctypes bindings over ``oracle/liboracle.so`` (built from ``ckks_oracle.c`` by
 may import this package; the product library never does.
 """
 import ctypes
+import fractions
+import math
 import os
 import subprocess
 
@@ -63,6 +65,8 @@ def lib():
         L.or_ckks_encode.restype = ctypes.c_int
         L.or_ckks_encode.argtypes = [ctypes.c_void_p, f64p, f64p, ctypes.c_size_t, ctypes.c_double, u64p,
                                      ctypes.c_size_t, ctypes.POINTER(u64p), u64p]
+        L.or_ckks_encode_coeffs.restype = None
+        L.or_ckks_encode_coeffs.argtypes = [ctypes.c_void_p, f64p, f64p, ctypes.c_size_t, ctypes.c_double, f64p]
         L.or_ckks_encode_const.restype = ctypes.c_int
         L.or_ckks_encode_const.argtypes = [ctypes.c_double, ctypes.c_double, u64p, ctypes.c_size_t, ctypes.c_int,
                                            u64p]
@@ -243,6 +247,27 @@ class Encoder:
         if r:
             raise ValueError("encoded values are too large")
         return out
+
+    def encode_coeffs(self, values, scale):
+        """Unrounded plaintext coefficients (float64, length n) in SEAL's order."""
+        v = np.asarray(values)
+        re = np.ascontiguousarray(np.real(v), dtype=np.float64)
+        im = np.ascontiguousarray(np.imag(v), dtype=np.float64) if np.iscomplexobj(v) else None
+        out = np.zeros(1 << self.logn)
+        lib().or_ckks_encode_coeffs(self.h, re.ctypes.data_as(f64p), im.ctypes.data_as(f64p) if im is not None else None,
+                                    len(re), scale, out.ctypes.data_as(f64p))
+        return out
+
+    @staticmethod
+    def residues_exact(coeffs, moduli):
+        """std::round(c) (halves away from zero) mod q for every coefficient, with
+        exact Python integers (any size)."""
+        ints = []
+        for c in coeffs:
+            x = fractions.Fraction(float(c))
+            m = math.floor(abs(x) + fractions.Fraction(1, 2))
+            ints.append(-m if x < 0 else m)
+        return np.array([[x % int(q) for x in ints] for q in moduli], dtype=np.uint64)
 
     def encode_const(self, value, scale, moduli):
         l = len(moduli)

@@ -565,6 +565,26 @@ int or_ckks_encode(void *h, const double *re, const double *im, size_t values_si
     return 0;
 }
 
+/* ckks.h:457-540 encode_internal up to the rounding: the real parts of the
+ * scaled inverse transform (the unrounded plaintext coefficients).  Checks of
+ * coefficients beyond 128 bits (SEAL's multi-word decomposition, ckks.h:
+ * 590-610) reduce these with exact big integers. */
+void or_ckks_encode_coeffs(void *h, const double *re, const double *im, size_t values_size, double scale,
+                           double *out)
+{
+    or_encoder *e = (or_encoder *)h;
+    size_t n = e->n, slots = e->slots;
+    double complex *cv = (double complex *)calloc(n, sizeof(double complex));
+    for (size_t i = 0; i < values_size; i++) {
+        double complex v = re[i] + I * (im ? im[i] : 0.0);
+        cv[e->index_map[i]] = v;
+        cv[e->index_map[i + slots]] = conj(v);
+    }
+    fft_from_rev(cv, e->logn, e->iroots, scale / (double)n);
+    for (size_t i = 0; i < n; i++) out[i] = creal(cv[i]);
+    free(cv);
+}
+
 /* ckks.cpp:73-...: constant encode; every coefficient-domain limb holds
  * round(value*scale) mod q_j in coefficient 0 only, i.e. the constant in every
  * NTT slot.  Returns -1 when |value*scale| needs more than 128 bits. */

@@ -177,6 +177,32 @@ def test_encode_parity(logn):
     assert np.array_equal(c.to_array(), enc.encode_const(-1.25, scale, ctx.moduli[:limbs]))
 
 
+def test_encode_wide_coefficients():
+    """Coefficients past 2^64 (SEAL's 128-bit decomposition) and past 2^128
+    (multi-word), a ragged count, an empty vector and the size check."""
+    logn = 12
+    _note("encode wide logn=%d" % logn)
+    ctx = nx.Context(logn, [60, 60, 60, 60, 60])
+    enc = orc.Encoder(logn)
+    rng = np.random.default_rng(31)
+    mods = ctx.moduli[: ctx.L]
+    v = rng.uniform(-(2.0 ** 20), 2.0 ** 20, ctx.slot_count) + 1j * rng.uniform(-1, 1, ctx.slot_count)
+    pt = nx.encode(ctx, v, 2.0 ** 60)
+    assert np.array_equal(pt.to_array(), enc.encode(v, 2.0 ** 60, mods))
+    # |coefficients| ~ 2^134: exact big-integer residues of SEAL's unrounded coefficients
+    w = rng.uniform(-(2.0 ** 30), 2.0 ** 30, 301)
+    c = enc.encode_coeffs(w, 2.0 ** 110)
+    assert np.max(np.abs(c)) > 2.0 ** 128
+    r = orc.Encoder.residues_exact(c, mods)
+    ref = np.stack([orc.ntt_forward(r[j], logn, mods[j]) for j in range(len(mods))])
+    assert np.array_equal(nx.encode(ctx, w, 2.0 ** 110).to_array(), ref)
+    # empty: the zero polynomial
+    assert not nx.encode(ctx, np.zeros(0), 2.0 ** 40).to_array().any()
+    # coefficients past the modulus: SEAL's "encoded values are too large"
+    with pytest.raises(Exception, match="too large"):
+        nx.encode(ctx, np.full(4, 2.0 ** 170), 2.0 ** 100)
+
+
 def test_encrypt_multiply_rotate_decrypt():
     _note("end-to-end logn=13")
     logn = 13

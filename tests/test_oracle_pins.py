@@ -394,6 +394,30 @@ def test_encode_matches_exact_dft(logn, scale):
     assert checked >= n - 2
 
 
+def test_encode_coeffs_and_exact_residues():
+    """The unrounded coefficients reduce (exact integers, halves away from zero)
+    to the oracle encoder's residues through the 64- and 128-bit decompositions,
+    and track the high-precision DFT to double accuracy."""
+    logn = 6
+    n = 1 << logn
+    enc = orc.Encoder(logn)
+    rng = np.random.default_rng(5)
+    vals = rng.uniform(-1, 1, n // 2) + 1j * rng.uniform(-1, 1, n // 2)
+    for scale, big in ((2.0 ** 40, 1.0), (2.0 ** 60, 2.0 ** 20)):
+        mods = orc.coeff_modulus_create(n, [60, 60, 60])
+        v = vals * big
+        c = enc.encode_coeffs(v, scale)
+        exact = exact_coeffs(v, logn, scale)
+        assert all(abs(float(e) - x) <= 1e-9 * max(1.0, abs(x)) for e, x in zip(exact, c))
+        r = orc.Encoder.residues_exact(c, mods)
+        got = np.stack([orc.ntt_forward(r[j], logn, mods[j]) for j in range(len(mods))])
+        assert np.array_equal(got, enc.encode(v, scale, mods))
+    # halves round away from zero, like std::round
+    q = mods[0]
+    r = orc.Encoder.residues_exact(np.array([2.5, -2.5, 0.5, -0.5, 2.0 ** 70 + 2.0 ** 18]), [q])[0]
+    assert [int(x) for x in r] == [3, q - 3, 1, q - 1, (2 ** 70 + 2 ** 18) % q]
+
+
 def test_encode_decode_roundtrip_and_sparse():
     logn = 10
     n = 1 << logn
