@@ -1,0 +1,102 @@
+// Encrypted BERT-base encoder layer over the evaluator programs (GELU,
+// softmax, LayerNorm helpers of src/ckks_evaluator.cpp, the Bootstrapper) and
+// plaintext-weight linear maps.  The reference ships these pieces but no
+// composed layer; the layer is:
+//   q, k, v = x Wq / 8, x Wk, x Wv        (12 heads of 64 features)
+//   o       = softmax(q_h k_h^T) v_h       per head
+//   z       = LN(x + o Wo)
+//   out     = LN(z + GELU(z W1) W2)
+// with LN(r) = r * invert_sqrt(mean(r^2)) on r * sqrt(30) (the range the
+// reference's inverse-sqrt initial guess is tuned for), softmax as
+// exp(x) ~ (1 + x/128)^128 and 1/sum from the reference's Newton product on
+// sum / s, GELU as src/gelu.cpp.  No biases; LayerNorm gamma = 1, beta = 0.
+//
+// Packing ("token groups"): a token matrix (tokens x features) is held in
+// features / 256 ciphertexts; token i owns slots [256 i, 256 i + 256) of every
+// ciphertext and ciphertext c holds features [256 c, 256 c + 256) there, so a
+// ciphertext carries 4 heads and slots / 256 tokens (128 at N = 2^16).
+// Rotating by 256 t moves every token by t (cyclically over all tokens).
+//
+// Levels: linear maps run at two limbs and end at one; their outputs are
+// bootstrapped (the range factor of each bootstrap is folded into the weights
+// and into the last SlotToCoeff level), so attention, LayerNorm and GELU each
+// start at the top compute level.  The chain needs 19 compute levels above q0
+// besides the bootstrapping levels, e.g. {51, 46 x 19, 51 x 14, 58 x 3}.
+#pragma once
+
+#include <map>
+#include <memory>
+#include <vector>
+
+#include "bootstrapper.hpp"
+#include "ckks.hpp"
+
+namespace nexus {
+
+struct BertShape {
+    int hidden = 768;  // multiple of 256
+    int heads = 12;    // hidden / heads = 64
+    int ffn = 3072;    // multiple of 256
+};
+
+class BertLayer {
+public:
+    static constexpr int kGroup = 256;   // slots per token
+    static constexpr int kHeadDim = 64;
+    enum Matrix { Wq = 0, Wk, Wv, Wo, W1, W2, kMatrices };
+
+    struct Options {
+        int baby = 32;                       // baby steps of the linear maps (power of two)
+        double ln_prescale = 5.477225575051661;  // sqrt(30)
+    };
+
+    // Weights are row-major [in][out] (x @ W), copied.  The context's chain
+    // must hold 19 compute levels below the bootstrapping levels.
+    BertLayer(CKKSEvaluator &ck, Bootstrapper &bt, const BertShape &shape, const std::vector<const double *> &w,
+              const Options &opt);
+
+    // Rotation steps the layer (without the bootstrapper) needs.
+    static std::vector<int> rotation_steps(size_t slots, const BertShape &shape, const Options &opt);
+    size_t tokens_per_ct() const { return T_; }
+
+    // x: hidden / 256 ciphertexts at >= 2 limbs, scale = the evaluator's scale;
+    // rows >= seq_len must be zero.  Returns the layer output in the same
+    // packing at 2 limbs.
+    std::vector<Ciphertext> forward(const std::vector<Ciphertext> &x, int seq_len);
+
+    // stages (exposed for tests)
+    // y = factor * x W at one limb (x taken to two limbs first)
+    std::vector<Ciphertext> linear(Matrix m, const std::vector<Ciphertext> &x, double factor);
+    // per-head softmax(q k^T) v of bootstrapped q (already / 8), k, v
+    std::vector<Ciphertext> attention(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
+                                      const std::vector<Ciphertext> &v, int seq_len);
+    // r * invert_sqrt(mean(r^2)) per token (r already pre-scaled)
+    std::vector<Ciphertext> layer_norm(const std::vector<Ciphertext> &r);
+
+private:
+    struct Diagonals {
+        int n_in = 0, n_out = 0, a_min = 0, a_max = 0;
+        std::vector<double> w;             // [in][out], as given
+        double factor = 0, scale = 0;      // what the plaintexts were encoded with
+        size_t chain = 0;
+        std::vector<Plaintext> pt;         // [((o * n_in + c) * n_a + a - a_min) * baby + b]
+        std::vector<char> present;
+    };
+    void encode_diagonals(Matrix m, double factor, size_t chain_index, double scale);
+    std::vector<Ciphertext> token_rotations(const Ciphertext &x, int t0, int count);
+    std::vector<Ciphertext> offset_rotations(const Ciphertext &x, int dmin, int dmax);
+    const Plaintext &offset_mask(int p, size_t limbs, double scale);
+    Ciphertext scaled(const Ciphertext &x, double value);  // x * value at the same scale, one level
+    size_t chain_of(size_t limbs) const { return ck_.context->data_primes() + 1 - limbs; }
+    uint64_t prime(size_t limbs) const { return ck_.context->moduli()[limbs - 1]; }
+
+    CKKSEvaluator &ck_;
+    Bootstrapper &bt_;
+    BertShape shape_;
+    Options opt_;
+    size_t slots_, T_;
+    Diagonals diag_[kMatrices];
+    std::map<std::tuple<int, size_t, double>, Plaintext> masks_;
+};
+
+}  // namespace nexus

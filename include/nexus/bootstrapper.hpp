@@ -70,7 +70,9 @@ public:
     const Params &params() const { return p_; }
     const std::vector<double> &cheb_coeffs() const { return cheb_; }
 
-    void bootstrap(const Ciphertext &in, Ciphertext &out);
+    // The output holds factor x the input message (factor folded into the
+    // last SlotToCoeff level: no extra level, error scaled by factor).
+    void bootstrap(const Ciphertext &in, Ciphertext &out, double factor = 1.0);
     // Bootstrapper::set_final_scale (reference Bootstrapper.h)
     void set_final_scale(double s) { p_.final_scale = s; }
 
@@ -80,7 +82,8 @@ public:
     // sparse: one output u (real, 2n values per period); dense: re and im parts
     void coeff_to_slot(const Ciphertext &in, Ciphertext &u_re, Ciphertext &u_im);
     void eval_mod(const Ciphertext &u, Ciphertext &out);
-    void slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out);
+    void slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out,
+                       double factor = 1.0);
 
 private:
     Ciphertext apply_level(const LinearLevel &lv, int tag, const Ciphertext &x, double out_scale, double value_factor);

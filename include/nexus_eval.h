@@ -58,6 +58,10 @@ int nx_bootstrapper_depth(const nx_bootstrapper *bt, int *depth);
 /* bootstrap_3 (Bootstrapper.cu:3358-3370): a ciphertext at any level (it is
  * switched to the lowest first) -> fresh ciphertext `depth` levels below the top */
 int nx_bootstrap(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
+/* the same with the output holding factor x the input message (the factor is
+ * folded into the last SlotToCoeff level: no extra level; the bootstrapping
+ * error is scaled by it).  No counterpart in the reference. */
+int nx_bootstrap_scaled(nx_bootstrapper *bt, const nx_ciphertext *in, double factor, nx_ciphertext *out);
 /* stages of bootstrap_sparse_3 / bootstrap_full_3 (Bootstrapper.cu:2987-3113) */
 int nx_boot_mod_raise(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
 int nx_boot_sub_sum(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
@@ -95,6 +99,37 @@ int nx_server_drain(nx_server *s);
 /* latency from submission to completion on the device */
 int nx_server_stats(const nx_server *s, size_t *completed, double *p50_ms, double *p99_ms, double *mean_ms,
                     size_t *peak_concurrent);
+
+/* ---- encrypted BERT-base encoder layer (include/nexus/bert.hpp).  NEXUS ships
+ * the pieces (MatrixMul with plaintext weights, GELU src/gelu.cpp, softmax
+ * src/softmax.cpp, LayerNorm src/layer_norm.cpp, Bootstrapper) but no
+ * composed layer.  Token matrices are packed in token groups: token i owns
+ * slots [256 i, 256 i + 256) of each of the features / 256 ciphertexts, and
+ * ciphertext c holds features [256 c, 256 c + 256).  Weights are row-major
+ * [in][out] float64, applied as x @ W: wq, wk, wv, wo hidden x hidden, w1
+ * hidden x ffn, w2 ffn x hidden; hidden = 64 heads; no biases.  The chain needs
+ * 19 compute levels below the bootstrapper's. */
+typedef struct nx_bert_layer nx_bert_layer;
+enum { NX_BERT_WQ = 0, NX_BERT_WK, NX_BERT_WV, NX_BERT_WO, NX_BERT_W1, NX_BERT_W2 };
+int nx_bert_layer_create(nx_bootstrapper *bt, int hidden, int heads, int ffn, const double *wq, const double *wk,
+                         const double *wv, const double *wo, const double *w1, const double *w2, int baby,
+                         nx_bert_layer **out);
+int nx_bert_layer_destroy(nx_bert_layer *layer);
+/* rotation steps the layer needs besides the bootstrapper's (host only) */
+int nx_bert_layer_rotation_steps(size_t slots, int hidden, int heads, int ffn, int baby, int *steps, size_t capacity,
+                                 size_t *count);
+/* x: hidden / 256 ciphertexts (>= 2 limbs, the evaluator's scale, rows >=
+ * seq_len zero) -> the layer output, hidden / 256 ciphertexts at 2 limbs */
+int nx_bert_layer_forward(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n, int seq_len,
+                          nx_ciphertext *const *out);
+/* stages: factor * x W (one limb out); attention of bootstrapped q (already
+ * scaled by 1/8), k, v; x * invert_sqrt(mean(x^2)) per token */
+int nx_bert_layer_linear(nx_bert_layer *layer, int matrix, const nx_ciphertext *const *x, size_t n_in, double factor,
+                         nx_ciphertext *const *out, size_t n_out);
+int nx_bert_layer_attention(nx_bert_layer *layer, const nx_ciphertext *const *q, const nx_ciphertext *const *k,
+                            const nx_ciphertext *const *v, size_t n, int seq_len, nx_ciphertext *const *out);
+int nx_bert_layer_layer_norm(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n,
+                             nx_ciphertext *const *out);
 
 /* Host-only view of the linear-transform plan (genfftcoeff_3 / geninvfftcoeff_3,
  * Bootstrapper.cu:1137,1545); no device use.  which = 0: CoeffToSlot, 1:

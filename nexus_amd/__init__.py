@@ -628,6 +628,7 @@ _sig("nx_bootstrapper_destroy", [vp])
 _sig("nx_bootstrapper_rotation_steps", [vp, _ip, sz, ctypes.POINTER(sz)])
 _sig("nx_bootstrapper_depth", [vp, _ip])
 _sig("nx_bootstrap", [vp, vp, vp])
+_sig("nx_bootstrap_scaled", [vp, vp, ctypes.c_double, vp])
 _sig("nx_boot_mod_raise", [vp, vp, vp])
 _sig("nx_boot_sub_sum", [vp, vp, vp])
 _sig("nx_boot_coeff_to_slot", [vp, vp, vp, vp])
@@ -712,8 +713,12 @@ class Bootstrapper:
         _check(fn(self.h, *args, out.h))
         return out
 
-    def bootstrap(self, ct):
-        return self._run(_lib.nx_bootstrap, ct.h)
+    def bootstrap(self, ct, factor=None):
+        """fresh ciphertext of the message (times `factor`, folded into the last
+        SlotToCoeff level, when given)"""
+        if factor is None:
+            return self._run(_lib.nx_bootstrap, ct.h)
+        return self._run(_lib.nx_bootstrap_scaled, ct.h, float(factor))
 
     def mod_raise(self, ct):
         return self._run(_lib.nx_boot_mod_raise, ct.h)
@@ -739,6 +744,102 @@ class Bootstrapper:
     def slot_to_coeff(self, f_re, f_im, input_scale):
         out = Ciphertext(self.ctx)
         _check(_lib.nx_boot_slot_to_coeff(self.h, f_re.h, f_im.h if f_im is not None else None, input_scale, out.h))
+        return out
+
+
+# ---------------------------------------------------------------- BERT layer (include/nexus_eval.h)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_sig("nx_bert_layer_create", [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int] + [_f64p] * 6
+     + [ctypes.c_int, ctypes.POINTER(vp)])
+_sig("nx_bert_layer_destroy", [vp])
+_sig("nx_bert_layer_rotation_steps", [sz, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.POINTER(ctypes.c_int), sz, ctypes.POINTER(sz)])
+_sig("nx_bert_layer_forward", [vp, ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.POINTER(vp)])
+_sig("nx_bert_layer_linear", [vp, ctypes.c_int, ctypes.POINTER(vp), sz, ctypes.c_double, ctypes.POINTER(vp), sz])
+_sig("nx_bert_layer_attention", [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), sz, ctypes.c_int,
+                                 ctypes.POINTER(vp)])
+_sig("nx_bert_layer_layer_norm", [vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
+
+BERT_GROUP = 256
+BERT_MATRICES = ("wq", "wk", "wv", "wo", "w1", "w2")
+
+
+def _handles(cts):
+    return (vp * len(cts))(*[c.h for c in cts])
+
+
+def bert_rotation_steps(slots, hidden=768, heads=12, ffn=3072, baby=32):
+    n = sz()
+    _check(_lib.nx_bert_layer_rotation_steps(slots, hidden, heads, ffn, baby, None, 0, ctypes.byref(n)))
+    buf = (ctypes.c_int * n.value)()
+    _check(_lib.nx_bert_layer_rotation_steps(slots, hidden, heads, ffn, baby, buf, n.value, ctypes.byref(n)))
+    return list(buf)
+
+
+def bert_pack(x, slots):
+    """token matrix (tokens x features) -> features / 256 slot vectors: token i
+    in slots [256 i, 256 i + 256) of ciphertext c = feature // 256"""
+    x = np.asarray(x, dtype=np.float64)
+    s, d = x.shape
+    if d % BERT_GROUP or s * BERT_GROUP > slots:
+        raise ValueError("features must be a multiple of 256 and tokens fit the slots")
+    out = []
+    for c in range(d // BERT_GROUP):
+        v = np.zeros(slots)
+        v[: s * BERT_GROUP] = x[:, c * BERT_GROUP:(c + 1) * BERT_GROUP].reshape(-1)
+        out.append(v)
+    return out
+
+
+def bert_unpack(vecs, tokens):
+    """inverse of bert_pack for the first `tokens` tokens"""
+    return np.concatenate([np.asarray(v)[: tokens * BERT_GROUP].reshape(tokens, BERT_GROUP) for v in vecs], axis=1)
+
+
+class BertLayer:
+    """Encrypted BERT-base encoder layer (include/nexus/bert.hpp) over a
+    Bootstrapper; weights: dict of float64 matrices wq, wk, wv, wo (hidden x
+    hidden), w1 (hidden x ffn), w2 (ffn x hidden), applied as x @ W."""
+
+    def __init__(self, bt, weights, heads=None, baby=32):
+        self.bt = bt
+        self.ctx = bt.ctx
+        ws = [np.ascontiguousarray(weights[k], dtype=np.float64) for k in BERT_MATRICES]
+        self.hidden = ws[0].shape[0]
+        self.ffn = ws[4].shape[1]
+        self.heads = heads or self.hidden // 64
+        h = vp()
+        _check(_lib.nx_bert_layer_create(bt.h, self.hidden, self.heads, self.ffn,
+                                         *[w.ctypes.data_as(_f64p) for w in ws], baby, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self, _fn=_lib.nx_bert_layer_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def _outs(self, n):
+        return [Ciphertext(self.ctx) for _ in range(n)]
+
+    def forward(self, x, seq_len):
+        out = self._outs(len(x))
+        _check(_lib.nx_bert_layer_forward(self.h, _handles(x), len(x), seq_len, _handles(out)))
+        return out
+
+    def linear(self, matrix, x, factor=1.0):
+        m = BERT_MATRICES.index(matrix)
+        n_out = (self.ffn if matrix == "w1" else self.hidden) // BERT_GROUP
+        out = self._outs(n_out)
+        _check(_lib.nx_bert_layer_linear(self.h, m, _handles(x), len(x), factor, _handles(out), n_out))
+        return out
+
+    def attention(self, q, k, v, seq_len):
+        out = self._outs(len(q))
+        _check(_lib.nx_bert_layer_attention(self.h, _handles(q), _handles(k), _handles(v), len(q), seq_len,
+                                            _handles(out)))
+        return out
+
+    def layer_norm(self, x):
+        out = self._outs(len(x))
+        _check(_lib.nx_bert_layer_layer_norm(self.h, _handles(x), len(x), _handles(out)))
         return out
 
 

@@ -621,7 +621,8 @@ void Bootstrapper::coeff_to_slot(const Ciphertext &in, Ciphertext &u_re, Ciphert
     }
 }
 
-void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out)
+void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out,
+                                 double factor)
 {
     // slottocoeff_3 / sfl_3 (Bootstrapper.cu:2275-2314, 2589-2594)
     Ciphertext x = f_re;
@@ -637,7 +638,8 @@ void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im,
     for (size_t k = 0; k < plan_.stc.size(); k++) {
         const bool last = k + 1 == plan_.stc.size();
         // the last level maps the EvalMod output (m / q_0) back to the message at final_scale
-        x = apply_level(plan_.stc[k], 100 + (int)k, x, last ? final_scale : x.scale(), last ? q0 / input_scale : 1.0);
+        x = apply_level(plan_.stc[k], 100 + (int)k, x, last ? final_scale : x.scale(),
+                        last ? q0 / input_scale * factor : 1.0);
     }
     if (plan_.logn < plan_.logNh) {
         Ciphertext r;
@@ -648,7 +650,7 @@ void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im,
     out = std::move(x);
 }
 
-void Bootstrapper::bootstrap(const Ciphertext &in, Ciphertext &out)
+void Bootstrapper::bootstrap(const Ciphertext &in, Ciphertext &out, double factor)
 {
     // bootstrap_3 (Bootstrapper.cu:3358-3370) -> bootstrap_sparse_3 / bootstrap_full_3
     if (!in.get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
@@ -661,7 +663,7 @@ void Bootstrapper::bootstrap(const Ciphertext &in, Ciphertext &out)
     Ciphertext f_re, f_im;
     eval_mod(u_re, f_re);
     if (u_im.get()) eval_mod(u_im, f_im);
-    slot_to_coeff(f_re, u_im.get() ? &f_im : nullptr, input_scale, out);
+    slot_to_coeff(f_re, u_im.get() ? &f_im : nullptr, input_scale, out, factor);
 }
 
 }  // namespace nexus

@@ -8,11 +8,6 @@ using nexus_capi::need;
 using nexus_capi::run;
 using nexus_capi::store;
 
-struct nx_bootstrapper {
-    nx_evaluator *ev;
-    std::unique_ptr<nexus::Bootstrapper> bt;
-};
-
 namespace {
 
 const nexus::LinearLevel &plan_level(const nexus::BootPlan &p, int which, int level)
@@ -94,6 +89,17 @@ int nx_bootstrap(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *ou
         need(out);
         nexus::Ciphertext res;
         bt->bt->bootstrap(input(bt, in), res);
+        store(res, out);
+    });
+}
+
+int nx_bootstrap_scaled(nx_bootstrapper *bt, const nx_ciphertext *in, double factor, nx_ciphertext *out)
+{
+    return run([&] {
+        need(bt);
+        need(out);
+        nexus::Ciphertext res;
+        bt->bt->bootstrap(input(bt, in), res, factor);
         store(res, out);
     });
 }

@@ -3,6 +3,7 @@
 #pragma once
 #include <memory>
 
+#include "../../../include/nexus/bootstrapper.hpp"
 #include "../../../include/nexus/ckks.hpp"
 #include "../../../include/nexus_eval.h"
 
@@ -11,6 +12,11 @@ struct nx_evaluator {
     std::unique_ptr<nexus::SecretKey> sk;
     std::unique_ptr<nexus::KSwitchKeys> rk, gk;
     std::unique_ptr<nexus::CKKSEvaluator> ckks;
+};
+
+struct nx_bootstrapper {
+    nx_evaluator *ev;
+    std::unique_ptr<nexus::Bootstrapper> bt;
 };
 
 namespace nx {

@@ -1,0 +1,138 @@
+"""Encrypted BERT layer on the GPU against its FP64 model (oracle/bert.py).
+
+Small ring: N = 2^12 (8 tokens per ciphertext in 256-slot token groups),
+hidden 256 (4 heads of 64), FFN 512, chain {51, 46 x 19, 51 x 14, 58 x 3}
+with three special primes, scale 2^46, dense bootstrapping.  Each stage
+(plaintext-weight linear map, attention with its internal bootstrap,
+LayerNorm) and the whole layer are compared with the float64 evaluation of
+the same approximations; the tolerance is 2^-10 absolute for the layer, far
+tighter for the linear maps (one rescale of CKKS noise).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+nx = pytest.importorskip("nexus_amd")
+from oracle import approx  # noqa: E402
+from oracle import bert as ob  # noqa: E402
+
+LOGN = 12
+CHAIN = [51] + [46] * 19 + [51] * 14 + [58] * 3
+SCALE = 2.0 ** 46
+HIDDEN, HEADS, FFN = 256, 4, 512
+
+
+def _note(msg):
+    print("[case]", msg, flush=True)
+    sys.stdout.flush()
+
+
+class Env:
+    def __init__(self):
+        _note("bert keys logN=%d L=%d" % (LOGN, len(CHAIN) - 3))
+        self.ctx = nx.Context(LOGN, CHAIN, hamming_weight=64, special_primes=3)
+        self.slots = self.ctx.slot_count
+        self.T = self.slots // nx.BERT_GROUP
+        self.sk = nx.SecretKey(self.ctx, seed=41)
+        self.rk = nx.KSwitchKeys(self.ctx).generate_relin(self.sk, seed=42)
+        steps = sorted(set(nx.bert_rotation_steps(self.slots, HIDDEN, HEADS, FFN))
+                       | set(nx.boot_plan_steps(LOGN - 1, LOGN - 1, 25)))
+        self.gk = nx.KSwitchKeys(self.ctx).generate_galois(self.sk, [self.ctx.elt_from_step(s) for s in steps],
+                                                            seed=43)
+        self.ev = nx.CKKSEvaluator(self.ctx, self.rk, self.gk, SCALE, self.sk)
+        self.bt = nx.Bootstrapper(self.ev)
+        self.W = ob.random_weights(HIDDEN, FFN, seed=44)
+        self.layer = nx.BertLayer(self.bt, self.W)
+
+    def enc(self, x, chain_index=1):
+        return [nx.encrypt(self.ctx, self.sk, nx.encode(self.ctx, v, SCALE, chain_index), seed=100 + i)
+                for i, v in enumerate(nx.bert_pack(x, self.slots))]
+
+    def dec(self, cts, tokens):
+        return nx.bert_unpack([nx.decode(self.ctx, nx.decrypt(self.ctx, self.sk, c)) for c in cts], tokens)
+
+
+@pytest.fixture(scope="module")
+def env():
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    return Env()
+
+
+def test_linear_maps(env):
+    rng = np.random.default_rng(1)
+    x = rng.normal(0, 1, (env.T, HIDDEN))
+    for name, factor in (("wq", 1.0), ("wo", 0.125), ("w1", 0.25)):
+        _note("bert linear %s" % name)
+        y = env.layer.linear(name, env.enc(x, chain_index=len(CHAIN) - 3 - 1), factor)
+        assert all(c.chain_index == len(CHAIN) - 3 for c in y)  # one limb left
+        got = env.dec(y, env.T)
+        err = np.max(np.abs(got - factor * x @ env.W[name]))
+        print("linear %s: max err %.3g" % (name, err))
+        assert err < 1e-5
+    _note("bert linear w2")
+    h = rng.normal(0, 1, (env.T, FFN))
+    got = env.dec(env.layer.linear("w2", env.enc(h), 1.0), env.T)
+    assert np.max(np.abs(got - h @ env.W["w2"])) < 1e-5
+
+
+def _attention_model(q, k, v, s):
+    o = np.zeros_like(q)
+    for h in range(HEADS):
+        sl = slice(64 * h, 64 * (h + 1))
+        o[:, sl] = ob._softmax_rows_approx(q[:, sl] @ k[:, sl].T, s) @ v[:, sl]
+    return o
+
+
+@pytest.mark.parametrize("s", [8, 5])
+def test_attention(env, s):
+    _note("bert attention s=%d" % s)
+    rng = np.random.default_rng(2 + s)
+    q = np.zeros((env.T, HIDDEN))
+    k = np.zeros((env.T, HIDDEN))
+    v = np.zeros((env.T, HIDDEN))
+    q[:s] = rng.normal(0, 0.2, (s, HIDDEN))
+    k[:s] = rng.normal(0, 0.2, (s, HIDDEN))
+    v[:s] = rng.normal(0, 1, (s, HIDDEN))
+    top = 1 + 14  # the bootstrapper's output level
+    o = env.layer.attention(env.enc(q, top), env.enc(k, top), env.enc(v, top), s)
+    got = env.dec(o, env.T)
+    want = _attention_model(q[:s], k[:s], v[:s], s)
+    err = np.max(np.abs(got[:s] - want))
+    print("attention s=%d: max err %.3g, padded rows %.3g" % (s, err, np.max(np.abs(got[s:])) if s < env.T else 0))
+    assert err < 2.0 ** -10
+    if s < env.T:
+        assert np.max(np.abs(got[s:])) < 2.0 ** -10
+
+
+def test_layer_norm(env):
+    _note("bert layer_norm")
+    rng = np.random.default_rng(3)
+    r = rng.normal(0, 1, (env.T, HIDDEN)) * np.sqrt(30.0)
+    got = env.dec(env.layer.layer_norm(env.enc(r, 1 + 14)), env.T)
+    m = np.sum(r * r, axis=1, keepdims=True) / HIDDEN
+    want = r * approx.invert_sqrt(m, 4, 2)
+    err = np.max(np.abs(got - want))
+    print("layer_norm: max err %.3g" % err)
+    assert err < 2.0 ** -10
+
+
+@pytest.mark.parametrize("s", [8, 6])
+def test_layer_forward(env, s):
+    _note("bert layer forward s=%d" % s)
+    rng = np.random.default_rng(10 + s)
+    x = np.zeros((env.T, HIDDEN))
+    x[:s] = rng.normal(0, 1, (s, HIDDEN))
+    # the layer's input arrives at two limbs, as a previous layer leaves it
+    out = env.layer.forward(env.enc(x, chain_index=len(CHAIN) - 3 - 1), s)
+    assert all(c.chain_index == len(CHAIN) - 3 - 1 for c in out)
+    got = env.dec(out, env.T)[:s]
+    want = ob.layer_approx(x[:s], env.W)
+    exact = ob.layer_exact(x[:s], env.W)
+    err = np.max(np.abs(got - want))
+    print("layer s=%d: max |enc - fp64 model| %.3g mean %.3g; max |enc - exact| %.3g"
+          % (s, err, np.mean(np.abs(got - want)), np.max(np.abs(got - exact))))
+    assert err < 2.0 ** -10
