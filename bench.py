@@ -1,20 +1,21 @@
 #!/usr/bin/env python3
 """Benchmark of the CKKS evaluation engine on one MI355X.
 
-Workload (one "step"): the nonlinear evaluator block of one encrypted
-BERT-base layer at the reference's full nonlinear chain -- N = 2^16, moduli
-{58, 40 x 18, 58} (src/main.cpp:27-30), scale 2^40 -- through the plain
-per-call drop-in (one evaluator call per ciphertext):
-  * GELU on a 128 x 3072 tensor            -> 12 ciphertexts of 32768 slots
-  * softmax over 12 x 128 x 128 scores     -> 12 ciphertexts (128 rows, stride 256)
-  * 2 x LayerNorm on 128 x 768             -> 16 ciphertexts (16 rows, stride 2048)
-Inputs are synthetic: GELU ~N(0,1) clipped to [-5,5]; softmax scores ~N(0,3)
-shifted to max 0 and clipped to [-10,0]; LayerNorm ~N(0,1)*sqrt(30) (the range
-the reference's inverse-sqrt initial guess is tuned for).  Matrix products and
-bootstrapping of the layer are not part of this step; they are timed after it
-and reported under "extra": one bootstrap at N = 2^16 with 2^15 slots (hybrid
-key switching) and at the reference's N = 2^15 / 2^13-slot configuration, and
-the reference's own MatrixMul (4096x768 x 768x64 at N = 2^13).
+Workload (one "step"): one encrypted BERT-base encoder layer (hidden 768, 12
+heads, FFN 3072, weights ~N(0, 0.02) random-init, no biases, LayerNorm gamma 1
+beta 0) over 8 sequences of uneven length ~U{32..128} with inputs ~N(0, 1), at
+N = 2^16 with bootstrapping between blocks (chain {51, 46 x 19, 51 x 14, 58 x 8},
+8 special primes (5 key-switching digits), Hamming weight 192, scale 2^46).  The batched path packs the
+sequences into sets of 128 token rows (first-fit decreasing; attention stays
+within each sequence) and runs one layer call per set; the plain path runs
+one call per sequence.  `value` is the batched path's sequence-layers per
+second; the plain path's rate, per-sequence p50/p99 latency and the decrypted
+error against the FP64 model of the same approximations (and against the
+exact FP64 layer) are reported beside it.
+
+Extras (after the timed steps): the nonlinear evaluators of one layer through
+the plain drop-in at the reference's chain {58, 40 x 18, 58}, their serving
+measurement, bootstrapping alone, and the reference's MatrixMul.
 
 Prints one JSON line.  The roofline entry is the dominant kernel class of the
 step, timed live with HIP events on its launch stream during the timed steps.
@@ -32,6 +33,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "enc. BERT-base layers/sec + p50/p99 latency, batched vs plain GPU, ≤2^-10 err"
 CHAIN = [58] + [40] * 18 + [58]
+BERT_CHAIN = [51] + [46] * 19 + [51] * 14 + [58] * 8
+BERT_SPECIAL = 8
+BERT_SCALE = 2.0 ** 46
+BERT_TOKENS = 128
 SCALE = 2.0 ** 40
 HBM_PEAK_GBS = 8000.0
 # HBM bytes moved per algorithmic byte, from FETCH_SIZE / WRITE_SIZE counter runs
@@ -64,7 +69,7 @@ def make_inputs(rng, slots):
     return gelu, sm, ln
 
 
-def cpu_baseline(gpu_ks_bytes, n, budget_s):
+def cpu_baseline(gpu_ks_bytes, n, budget_s, unit, units_per_step):
     """Time the CPU oracle's SEAL-restated key switch on the host (1 thread) and
     scale it to one step by the step's key-switch work measured on the GPU."""
     import oracle as orc
@@ -87,18 +92,18 @@ def cpu_baseline(gpu_ks_bytes, n, budget_s):
         if time.perf_counter() - t0 > budget_s or reps >= 3:
             break
     t_ks = (time.perf_counter() - t0) / reps
-    # ks_inner algorithmic bytes of one top-level key switch (l = 19, 20 output primes)
+    # ks_inner algorithmic bytes of one SEAL key switch at N = 2^16, l = 19 (20 output primes)
     top_bytes = n * (l + 1) * (24.0 * l + 16.0)
     ks_equiv = gpu_ks_bytes / top_bytes
     step_s = t_ks * ks_equiv
     return {
-        "value": 1.0 / step_s,
-        "unit": "layer nonlinear blocks/s",
+        "value": units_per_step / step_s,
+        "unit": unit,
         "cores": 1,
         "kind": "port",
         "sample": ("%d key switch(es) at N=2^16, l=19 on the CPU oracle (oracle/ckks_oracle.c, 1 thread): "
-                   "%.2f s each; scaled by the step's key-switch work (%.1f top-level key-switch "
-                   "equivalents, from ks_inner bytes); rescale/NTT-only and elementwise work not counted"
+                   "%.2f s each; scaled by the step's key-switch inner-product bytes (%.1f equivalents of that "
+                   "key switch); rescale/NTT-only, elementwise and encoding work not counted"
                    % (reps, t_ks, ks_equiv)),
     }
 
@@ -252,14 +257,60 @@ def bench_ntt(nx, reps):
     return res
 
 
+def bert_setup(nx):
+    """context, keys, bootstrapper and BERT-base layer at N = 2^16"""
+    from nexus_amd import bert_rotation_steps, boot_plan_steps
+    ctx = nx.Context(16, BERT_CHAIN, hamming_weight=192, special_primes=BERT_SPECIAL)
+    sk = nx.SecretKey(ctx, seed=21)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=22)
+    steps = sorted(set(bert_rotation_steps(ctx.slot_count)) | set(boot_plan_steps(15, 15, 25)))
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps], seed=23)
+    ev = nx.CKKSEvaluator(ctx, rk, gk, BERT_SCALE, sk)
+    bt = nx.Bootstrapper(ev)
+    rng = np.random.default_rng(42)
+    W = {k: rng.normal(0, 0.02, shape) for k, shape in
+         (("wq", (768, 768)), ("wk", (768, 768)), ("wv", (768, 768)), ("wo", (768, 768)),
+          ("w1", (768, 3072)), ("w2", (3072, 768)))}
+    layer = nx.BertLayer(bt, W)
+    return {"ctx": ctx, "sk": sk, "rk": rk, "gk": gk, "ev": ev, "bt": bt, "layer": layer, "W": W,
+            "keys": len(steps) + 1}
+
+
+def pack_sets(lens, capacity):
+    """first-fit decreasing: [[(sequence, start), ...], ...]"""
+    sets, fill = [], []
+    for k in sorted(range(len(lens)), key=lambda k: -lens[k]):
+        for j in range(len(sets)):
+            if fill[j] + lens[k] <= capacity:
+                sets[j].append((k, fill[j]))
+                fill[j] += lens[k]
+                break
+        else:
+            sets.append([(k, 0)])
+            fill.append(lens[k])
+    return sets
+
+
+def bert_encrypt(nx, B, rows):
+    ctx = B["ctx"]
+    return [nx.encrypt(ctx, B["sk"], nx.encode(ctx, v, BERT_SCALE, ctx.L - 1), seed=1000 + i)
+            for i, v in enumerate(nx.bert_pack(rows, ctx.slot_count))]
+
+
+def bert_decrypt(nx, B, cts):
+    ctx = B["ctx"]
+    return nx.bert_unpack([nx.decode(ctx, nx.decrypt(ctx, B["sk"], c)) for c in cts], BERT_TOKENS)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--sequences", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
-    ap.add_argument("--no-extra", action="store_true", help="skip the bootstrapping and MatMul measurements")
+    ap.add_argument("--no-extra", action="store_true", help="skip the nonlinear, bootstrapping and MatMul extras")
     ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--lanes", type=int, default=3)
@@ -287,104 +338,123 @@ def main():
             r = bench_matmul(nx, args.steps)
         print(json.dumps({"workload": args.only, **r}), flush=True)
         return
-    from oracle import approx
 
-    log("setup: context + keys (N=2^16, L=19)")
-    ctx = nx.Context(16, CHAIN)
-    sk = nx.SecretKey(ctx, seed=11)
-    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=12)
-    steps_needed = [-1024, -128] + [1 << i for i in range(10)]
-    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps_needed], seed=13)
-    ev = nx.CKKSEvaluator(ctx, rk, gk, SCALE, sk)
+    log("setup: BERT-base layer, N=2^16, L=%d, %d special primes" % (len(BERT_CHAIN) - BERT_SPECIAL, BERT_SPECIAL))
+    t0 = time.perf_counter()
+    B = bert_setup(nx)
+    ctx, layer = B["ctx"], B["layer"]
     rng = np.random.default_rng(2025)
-    g_in, s_in, l_in = make_inputs(rng, ctx.slot_count)
-    enc = lambda v: nx.encrypt(ctx, sk, nx.encode(ctx, v, SCALE), seed=int(rng.integers(1 << 62)))
-    g_ct = [enc(v) for v in g_in]
-    s_ct = [enc(v) for v in s_in]
-    l_ct = [enc(v) for v in l_in]
+    lens = [int(x) for x in rng.integers(32, BERT_TOKENS + 1, args.sequences)]
+    xs = [rng.normal(0, 1, (n, 768)) for n in lens]
+    sets = pack_sets(lens, BERT_TOKENS)
+    set_rows, set_segs = [], []
+    for members in sets:
+        rows = np.zeros((BERT_TOKENS, 768))
+        segs = []
+        for k, a in members:
+            rows[a:a + lens[k]] = xs[k]
+            segs.append((a, lens[k]))
+        set_rows.append(rows)
+        set_segs.append(segs)
+    single_rows = []
+    for k in range(len(lens)):
+        rows = np.zeros((BERT_TOKENS, 768))
+        rows[:lens[k]] = xs[k]
+        single_rows.append(rows)
+    enc_sets = [bert_encrypt(nx, B, r) for r in set_rows]
+    enc_single = [bert_encrypt(nx, B, r) for r in single_rows]
     ctx.synchronize()
+    log("setup %.1f s; %d sequences (lengths %s) -> %d packed sets" % (time.perf_counter() - t0, len(lens), lens,
+                                                                      len(sets)))
 
-    def step():
-        outs = []
-        for c in g_ct:
-            outs.append(ev.gelu(c))
-        for c in s_ct:
-            outs.append(ev.softmax(c, 128))
-        for c in l_ct:
-            outs.append(ev.layer_norm(c, 1024))
-        return outs
-
-    # warm-up; the first warm-up step ranks the kernel classes by event-timed duration
-    nx.profile_select("*")
+    # warm-up: the first call encodes the weight diagonals; one more call ranks the kernel classes
     for w in range(max(1, args.warmup)):
-        log("warmup step %d" % w)
-        step()
+        tw = time.perf_counter()
+        layer.forward(enc_sets[0], segments=set_segs[0])
         ctx.synchronize()
-        if w == 0:
-            ranking = sorted(((nx.profile_read(c)[1], c) for c in nx.profile_classes()), reverse=True)
-            nx.profile_select("")
+        log("warmup %d: %.1f s" % (w, time.perf_counter() - tw))
+    nx.profile_select("*")
+    layer.forward(enc_sets[0], segments=set_segs[0])
+    ctx.synchronize()
+    ranking = sorted(((nx.profile_read(c)[1], c) for c in nx.profile_classes()), reverse=True)
+    nx.profile_select("")
     dominant = ranking[0][1]
-    log("kernel time ranking (ms): " + ", ".join("%s=%.1f" % (c, t) for t, c in ranking))
+    log("kernel time ranking of one call (ms): " + ", ".join("%s=%.0f" % (c, t) for t, c in ranking[:10]))
 
-    # timed region: only the dominant class carries events
+    # plain per-call: one call per sequence
+    plain_lat = []
+    ctx.synchronize()
+    tp = time.perf_counter()
+    for k in range(len(lens)):
+        ts = time.perf_counter()
+        layer.forward(enc_single[k], lens[k])
+        ctx.synchronize()
+        plain_lat.append(time.perf_counter() - ts)
+    plain_s = time.perf_counter() - tp
+    log("plain per-call: %d sequences in %.2f s" % (len(lens), plain_s))
+
+    # batched: packed sets (the timed steps)
     nx.profile_select(dominant + ",ks_inner")
     ctx.synchronize()
-    t0 = time.perf_counter()
-    lat = []
+    tb = time.perf_counter()
+    set_lat, outs = [], None
     for k in range(args.steps):
-        ts = time.perf_counter()
-        outs = step()
-        ctx.synchronize()
-        lat.append(time.perf_counter() - ts)
-        log("step %d: %.1f ms" % (k, lat[-1] * 1e3))
+        outs = []
+        for j in range(len(sets)):
+            ts = time.perf_counter()
+            outs.append(layer.forward(enc_sets[j], segments=set_segs[j]))
+            ctx.synchronize()
+            set_lat.append(time.perf_counter() - ts)
+        log("step %d: %.2f s" % (k, sum(set_lat[-len(sets):])))
     ctx.synchronize()
-    elapsed = time.perf_counter() - t0
+    elapsed = time.perf_counter() - tb
     launches, k_ms, k_bytes = nx.profile_read(dominant)
     _, _, ks_bytes = nx.profile_read("ks_inner")
     nx.profile_select("")
     ms_per_step = elapsed / args.steps * 1e3
-
-    # accuracy of the last step vs the FP64 plaintext model of the same approximations
-    def decv(ct, count):
-        return nx.decode(ctx, nx.decrypt(ctx, sk, ct))[:count]
-    err_g = np.max(np.abs(decv(outs[0], ctx.slot_count) - approx.gelu_approx(g_in[0])))
-    sm_model = np.zeros(ctx.slot_count)
-    for r in range(128):
-        sm_model[256 * r: 256 * r + 128] = approx.softmax_approx(s_in[0][256 * r: 256 * r + 128])
-    d = decv(outs[12], ctx.slot_count)
-    err_s = max(np.max(np.abs(d[256 * r: 256 * r + 128] - sm_model[256 * r: 256 * r + 128])) for r in range(128))
-    d = decv(outs[24], ctx.slot_count)
-    err_l = max(np.max(np.abs(d[2048 * r: 2048 * r + 768] - approx.layer_norm_approx(l_in[0][2048 * r: 2048 * r + 768])))
-                for r in range(16))
-    max_err = float(max(err_g, err_s, err_l))
+    # per-sequence latency of the batched path = its set's call
+    seq_lat = []
+    for j, members in enumerate(sets):
+        seq_lat += [set_lat[-len(sets) + j]] * len(members)
+    got = {}
+    for j, members in enumerate(sets):
+        dec = bert_decrypt(nx, B, outs[j])
+        for k, a in members:
+            got[k] = dec[a:a + lens[k]]
 
     avg_launch_ms = k_ms / max(launches, 1)
     achieved = (k_bytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if launches else 0.0
     res = {
         "metric": METRIC,
-        "value": 1000.0 / ms_per_step,
-        "unit": "layer nonlinear blocks/s",
+        "value": len(lens) * 1000.0 / ms_per_step,
+        "unit": "BERT-base sequence-layers/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
-        "p50_ms": float(np.percentile(np.array(lat) * 1e3, 50)),
-        "p99_ms": float(np.percentile(np.array(lat) * 1e3, 99)),
+        "p50_ms": float(np.percentile(np.array(seq_lat) * 1e3, 50)),
+        "p99_ms": float(np.percentile(np.array(seq_lat) * 1e3, 99)),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u64 (exact RNS modular arithmetic)",
-        "data": "synthetic",
-        "max_err_vs_fp64_model": max_err,
-        "err_ok": bool(max_err <= 2.0 ** -10),
+        "data": "synthetic (random-init weights ~N(0,0.02), inputs ~N(0,1))",
+        "plain_per_call": {
+            "value": len(lens) / plain_s,
+            "seconds": plain_s,
+            "p50_ms": float(np.percentile(np.array(plain_lat) * 1e3, 50)),
+            "p99_ms": float(np.percentile(np.array(plain_lat) * 1e3, 99)),
+        },
+        "batched_over_plain": plain_s / (ms_per_step / 1e3),
         "config": {
-            "workload": "BERT-base layer nonlinear block, plain per-call drop-in: GELU 128x3072 (12 ct), "
-                        "softmax 12x128x128 (12 ct), 2x LayerNorm 128x768 (16 ct); matmuls and bootstrapping "
-                        "not included",
+            "workload": "BERT-base encoder layer (hidden 768, 12 heads, FFN 3072) over %d sequences of lengths %s, "
+                        "packed into %d sets of %d token rows, bootstrapping between blocks"
+                        % (len(lens), lens, len(sets), BERT_TOKENS),
             "poly_degree": 1 << 16,
-            "coeff_modulus_bits": CHAIN,
-            "scale": "2^40",
-            "ciphertexts_per_step": len(g_ct) + len(s_ct) + len(l_ct),
+            "coeff_modulus_bits": BERT_CHAIN,
+            "special_primes": BERT_SPECIAL,
+            "scale": "2^46",
+            "galois_keys": B["keys"],
         },
         "roofline": {
             "kernel": dominant,
@@ -403,14 +473,62 @@ def main():
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
         },
     }
+    W = B["W"]
+    del enc_sets, enc_single, outs, layer, B, ctx
+    if not args.no_extra:
+        res["extra"] = extras(nx, args)
+    if not args.no_cpu_baseline:
+        log("cpu baseline (oracle key switch) and the FP64 model of the layer")
+        res["cpu_baseline"] = cpu_baseline(ks_bytes / args.steps, 1 << 16, args.cpu_budget,
+                                           "BERT-base sequence-layers/s", len(lens))
+        from oracle import bert as obert
+        errs = [float(np.max(np.abs(got[k] - obert.layer_approx(xs[k], W)))) for k in range(len(lens))]
+        exact = [float(np.max(np.abs(got[k] - obert.layer_exact(xs[k], W)))) for k in range(len(lens))]
+        res["max_err_vs_fp64_model"] = max(errs)
+        res["max_err_vs_exact_fp64_layer"] = max(exact)
+        res["err_ok"] = bool(max(errs) <= 2.0 ** -10)
+    print(json.dumps(res), flush=True)
+
+
+def extras(nx, args):
+    """the nonlinear block at the reference's chain (plain and served),
+    bootstrapping alone, and the reference's MatrixMul"""
+    extra = {}
+    log("nonlinear block: context + keys (N=2^16, L=19)")
+    ctx = nx.Context(16, CHAIN)
+    sk = nx.SecretKey(ctx, seed=11)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=12)
+    steps_needed = [-1024, -128] + [1 << i for i in range(10)]
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps_needed], seed=13)
+    ev = nx.CKKSEvaluator(ctx, rk, gk, SCALE, sk)
+    rng = np.random.default_rng(2025)
+    g_in, s_in, l_in = make_inputs(rng, ctx.slot_count)
+    enc = lambda v: nx.encrypt(ctx, sk, nx.encode(ctx, v, SCALE), seed=int(rng.integers(1 << 62)))
+    cts = [(ev.gelu, enc(v), None) for v in g_in] + [(ev.softmax, enc(v), 128) for v in s_in] + \
+          [(ev.layer_norm, enc(v), 1024) for v in l_in]
+
+    def step():
+        return [f(c) if a is None else f(c, a) for f, c, a in cts]
+
+    step()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        step()
+    ctx.synchronize()
+    ms = (time.perf_counter() - t0) / 3 * 1e3
+    extra["nonlinear_block_plain"] = {
+        "workload": "GELU 128x3072 (12 ct), softmax 12x128x128 (12 ct), 2x LayerNorm 128x768 (16 ct) through "
+                    "the plain per-call drop-in, N=2^16, {58, 40 x 18, 58}, scale 2^40",
+        "ms_per_block": ms, "blocks_per_s": 1000.0 / ms}
+    log("nonlinear block: %.0f ms" % ms)
     if not args.no_served:
-        log("served: %d requests, lengths U{16..128}, Poisson arrivals" % args.requests)
+        log("served nonlinear blocks: %d requests, lengths U{16..128}, Poisson arrivals" % args.requests)
         srng = np.random.default_rng(77)
         reqs = served_requests(nx, ctx, sk, srng, args.requests)
-        cts = sum(len(j) for _, j in reqs)
-        # arrival rate: 1.2x what the plain path sustains at the measured step cost per ciphertext
-        per_ct_s = ms_per_step / 1e3 / len(g_ct + s_ct + l_ct)
-        rate = 1.2 / (per_ct_s * cts / len(reqs))
+        n_ct = sum(len(j) for _, j in reqs)
+        per_ct_s = ms / 1e3 / len(cts)
+        rate = 1.2 / (per_ct_s * n_ct / len(reqs))
         arrivals = np.cumsum(srng.exponential(1.0 / rate, len(reqs)))
         arrivals -= arrivals[0]
         ctx.synchronize()
@@ -418,10 +536,10 @@ def main():
         log("plain per-call (1 lane): %.2f s, p50 %.0f ms, p99 %.0f ms" % (span1, st1["p50_ms"], st1["p99_ms"]))
         span3, st3 = run_served(nx, ev, reqs, arrivals, args.lanes)
         log("served (%d lanes): %.2f s, p50 %.0f ms, p99 %.0f ms" % (args.lanes, span3, st3["p50_ms"], st3["p99_ms"]))
-        res["served"] = {
+        extra["nonlinear_served"] = {
             "workload": "nonlinear block of one BERT-base layer per request, S ~ U{16..128}, Poisson arrivals at "
                         "%.2f requests/s (1.2x the plain path's capacity)" % rate,
-            "requests": len(reqs), "ciphertexts": cts, "mean_seq_len": float(np.mean([S for S, _ in reqs])),
+            "requests": len(reqs), "ciphertexts": n_ct, "mean_seq_len": float(np.mean([S for S, _ in reqs])),
             "lanes": args.lanes,
             "served": {"blocks_per_s": len(reqs) / span3, "p50_ms": st3["p50_ms"], "p99_ms": st3["p99_ms"],
                        "mean_ms": st3["mean_ms"], "peak_concurrent": st3["peak_concurrent"], "makespan_s": span3},
@@ -430,26 +548,18 @@ def main():
             "throughput_ratio": span1 / span3,
         }
         del reqs
-    n_main = ctx.n
-    if not args.no_extra:
-        del ev, gk, rk, sk, g_ct, s_ct, l_ct, outs
-        ctx = None
-        extra = {}
-        log("bootstrap N=2^16, 2^15 slots (dense), hybrid key switching (3 special primes)")
-        extra["bootstrap_dense_n65536"] = bench_bootstrap(nx, 16, 15, [51] + [46] * 16 + [51] * 14 + [58] * 3, 3, 3)
-        log("bootstrap N=2^15, 2^13 slots (sparse), the reference's chain (cuda/src/bootstrapping.cu:22-60)")
-        extra["bootstrap_sparse_n32768_reference_chain"] = bench_bootstrap(
-            nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, 3)
-        log("bootstrap N=2^15, 2^13 slots, same data chain with 4 special primes")
-        extra["bootstrap_sparse_n32768_hybrid"] = bench_bootstrap(
-            nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [58] * 4, 4, 3)
-        log("matmul 4096x768 x 768x64 (N=2^13)")
-        extra["matmul_reference_mm"] = bench_matmul(nx, 2)
-        res["extra"] = extra
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle key switch)")
-        res["cpu_baseline"] = cpu_baseline(ks_bytes / args.steps, n_main, args.cpu_budget)
-    print(json.dumps(res), flush=True)
+    del cts, ev, gk, rk, sk, ctx
+    log("bootstrap N=2^16, 2^15 slots (dense), hybrid key switching (3 special primes)")
+    extra["bootstrap_dense_n65536"] = bench_bootstrap(nx, 16, 15, [51] + [46] * 16 + [51] * 14 + [58] * 3, 3, 3)
+    log("bootstrap N=2^15, 2^13 slots (sparse), the reference's chain (cuda/src/bootstrapping.cu:22-60)")
+    extra["bootstrap_sparse_n32768_reference_chain"] = bench_bootstrap(
+        nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, 3)
+    log("bootstrap N=2^15, 2^13 slots, same data chain with 4 special primes")
+    extra["bootstrap_sparse_n32768_hybrid"] = bench_bootstrap(
+        nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [58] * 4, 4, 3)
+    log("matmul 4096x768 x 768x64 (N=2^13)")
+    extra["matmul_reference_mm"] = bench_matmul(nx, 2)
+    return extra
 
 
 if __name__ == "__main__":

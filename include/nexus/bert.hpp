@@ -26,6 +26,9 @@
 
 #include <map>
 #include <memory>
+#include <mutex>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "bootstrapper.hpp"
@@ -48,44 +51,88 @@ public:
     struct Options {
         int baby = 32;                       // baby steps of the linear maps (power of two)
         double ln_prescale = 5.477225575051661;  // sqrt(30)
+        bool stage_timing = false;           // synchronise and time each stage of forward()
     };
 
     // Weights are row-major [in][out] (x @ W), copied.  The context's chain
     // must hold 19 compute levels below the bootstrapping levels.
     BertLayer(CKKSEvaluator &ck, Bootstrapper &bt, const BertShape &shape, const std::vector<const double *> &w,
               const Options &opt);
+    // A view for another lane (thread): shares the weights and their encoded
+    // diagonals (encoded once, under a lock, then read-only) and runs on its
+    // own evaluator and bootstrapper.
+    BertLayer(const BertLayer &shared, CKKSEvaluator &ck, Bootstrapper &bt);
 
     // Rotation steps the layer (without the bootstrapper) needs.
     static std::vector<int> rotation_steps(size_t slots, const BertShape &shape, const Options &opt);
     size_t tokens_per_ct() const { return T_; }
+    // stage name -> milliseconds of the last forward() (stage_timing only)
+    const std::vector<std::pair<std::string, double>> &stage_ms() const { return stage_ms_; }
+    void set_stage_timing(bool on) { opt_.stage_timing = on; }
 
-    // x: hidden / 256 ciphertexts at >= 2 limbs, scale = the evaluator's scale;
-    // rows >= seq_len must be zero.  Returns the layer output in the same
-    // packing at 2 limbs.
+    // A run of tokens forming one sequence: rows [start, start + len).
+    struct Segment {
+        int start, len;
+    };
+
+    // x: hidden / 256 ciphertexts at >= 2 limbs and one scale near the
+    // evaluator's; rows >= seq_len must be zero.  Returns the layer output in
+    // the same packing at 2 limbs (its scale is tracked exactly, near the
+    // evaluator's).
     std::vector<Ciphertext> forward(const std::vector<Ciphertext> &x, int seq_len);
+    // Several sequences packed in one set of ciphertexts: attention stays
+    // within each segment, so every sequence's rows are what it would get
+    // alone.  Rows outside the segments must be zero.
+    std::vector<Ciphertext> forward(const std::vector<Ciphertext> &x, const std::vector<Segment> &segments);
+
+    // Continuous batching: a sequence encrypted at rows [0, len) moved to rows
+    // [offset, offset + len) (token rotation, no level), and rows [offset,
+    // offset + len) of a packed result moved back to [0, len) with every other
+    // row cleared (one level).
+    std::vector<Ciphertext> shift_rows(const std::vector<Ciphertext> &x, int offset);
+    std::vector<Ciphertext> extract_rows(const std::vector<Ciphertext> &x, int offset, int len);
+    const BertShape &shape() const { return shape_; }
 
     // stages (exposed for tests)
-    // y = factor * x W at one limb (x taken to two limbs first)
+    // y = factor * x W at one limb and exactly the evaluator's scale (x taken
+    // to two limbs first)
     std::vector<Ciphertext> linear(Matrix m, const std::vector<Ciphertext> &x, double factor);
     // per-head softmax(q k^T) v of bootstrapped q (already / 8), k, v
     std::vector<Ciphertext> attention(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
                                       const std::vector<Ciphertext> &v, int seq_len);
+    std::vector<Ciphertext> attention(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
+                                      const std::vector<Ciphertext> &v, const std::vector<Segment> &segments);
     // r * invert_sqrt(mean(r^2)) per token (r already pre-scaled)
     std::vector<Ciphertext> layer_norm(const std::vector<Ciphertext> &r);
 
 private:
-    struct Diagonals {
-        int n_in = 0, n_out = 0, a_min = 0, a_max = 0;
-        std::vector<double> w;             // [in][out], as given
-        double factor = 0, scale = 0;      // what the plaintexts were encoded with
-        size_t chain = 0;
-        std::vector<Plaintext> pt;         // [((o * n_in + c) * n_a + a - a_min) * baby + b]
+    // plaintext diagonals of one matrix at one (factor, level, scale)
+    struct Encoded {
+        std::vector<Plaintext> pt;  // [((o * n_in + c) * n_a + a - a_min) * baby + b]
         std::vector<char> present;
     };
-    void encode_diagonals(Matrix m, double factor, size_t chain_index, double scale);
+    struct Weights {
+        int n_in = 0, n_out = 0, a_min = 0, a_max = 0;
+        std::vector<double> w;  // [in][out], as given
+        // at most two encodings are kept; callers hold theirs while in use
+        std::vector<std::pair<std::tuple<double, size_t, double>, std::shared_ptr<const Encoded>>> enc;
+    };
+    struct Shared {
+        Weights m[kMatrices];
+        std::mutex mu;
+    };
+    std::shared_ptr<const Encoded> diagonals(Matrix m, double factor, size_t chain_index, double scale);
+    Ciphertext rotate_tokens(const Ciphertext &x, int t);
     std::vector<Ciphertext> token_rotations(const Ciphertext &x, int t0, int count);
     std::vector<Ciphertext> offset_rotations(const Ciphertext &x, int dmin, int dmax);
     const Plaintext &offset_mask(int p, size_t limbs, double scale);
+    // per-token segment index (-1 outside every segment) and its 1 / length
+    std::vector<int> segment_of(const std::vector<Segment> &segments) const;
+    const Plaintext &key_mask(int eta, int tc, const std::vector<int> &seg, size_t limbs);
+    const Plaintext &row_mask(int tag, const std::vector<int> &seg, const std::vector<Segment> &segments, size_t limbs,
+                              double scale, int period);
+    // bootstrap real messages in place, two per bootstrap, times factor
+    void bootstrap_real(const std::vector<Ciphertext *> &cts, double factor);
     Ciphertext scaled(const Ciphertext &x, double value);  // x * value at the same scale, one level
     size_t chain_of(size_t limbs) const { return ck_.context->data_primes() + 1 - limbs; }
     uint64_t prime(size_t limbs) const { return ck_.context->moduli()[limbs - 1]; }
@@ -95,8 +142,9 @@ private:
     BertShape shape_;
     Options opt_;
     size_t slots_, T_;
-    Diagonals diag_[kMatrices];
-    std::map<std::tuple<int, size_t, double>, Plaintext> masks_;
+    std::shared_ptr<Shared> shared_;
+    std::map<std::tuple<int, size_t, double, std::vector<int>>, Plaintext> masks_;
+    std::vector<std::pair<std::string, double>> stage_ms_;
 };
 
 }  // namespace nexus

@@ -85,20 +85,39 @@ int nx_eval_argmax(nx_bootstrapper *bt, const nx_ciphertext *x, int len, nx_ciph
  * the parameters of bootstrap jobs.  No counterpart in the reference (its
  * drivers run one call at a time, src/main.cpp). */
 typedef struct nx_server nx_server;
-enum { NX_JOB_GELU = 0, NX_JOB_SOFTMAX = 1, NX_JOB_LAYER_NORM = 2, NX_JOB_BOOTSTRAP = 3 };
+enum { NX_JOB_GELU = 0, NX_JOB_SOFTMAX = 1, NX_JOB_LAYER_NORM = 2, NX_JOB_BOOTSTRAP = 3, NX_JOB_BERT = 4 };
 int nx_server_create(nx_evaluator *ev, int lanes, size_t memory_budget, const nx_bootstrapper *boot,
                      nx_server **out);
 int nx_server_destroy(nx_server *s);
+/* a server that also runs BERT-layer requests (nx_server_submit_bert) with
+ * the layer's weights shared by every lane.  pack != 0: continuous batching
+ * -- a lane that takes a BERT request packs the oldest queued BERT requests
+ * whose rows still fit into the same set of ciphertexts (token rotations),
+ * runs one layer call with attention kept inside each sequence and returns
+ * each request's rows moved back to [0, len) with the others cleared (one
+ * level); a request that runs alone returns the layer output as is.
+ * bert_footprint: admission bytes of one layer call (0 = 8 GiB). */
+typedef struct nx_bert_layer nx_bert_layer;
+int nx_server_create_bert(nx_evaluator *ev, int lanes, size_t memory_budget, const nx_bootstrapper *boot,
+                          const nx_bert_layer *layer, int pack, size_t bert_footprint, nx_server **out);
 /* one request = count jobs (kind, row length for softmax / LayerNorm, input);
  * the inputs are copied */
 int nx_server_submit(nx_server *s, const int *kinds, const int *lens, const nx_ciphertext *const *inputs,
                      size_t count, uint64_t *request_id);
+/* one BERT-layer request: a sequence encrypted at rows [0, seq_len) of n =
+ * hidden / 256 ciphertexts (the inputs are copied); its wait yields n outputs */
+int nx_server_submit_bert(nx_server *s, const nx_ciphertext *const *x, size_t n, int seq_len, uint64_t *request_id);
 /* blocks until the request is done; count outputs, one per job */
 int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outputs, size_t count);
 int nx_server_drain(nx_server *s);
+/* paused != 0: admit nothing until resumed (requests submitted meanwhile are
+ * batched together afterwards) */
+int nx_server_pause(nx_server *s, int paused);
 /* latency from submission to completion on the device */
 int nx_server_stats(const nx_server *s, size_t *completed, double *p50_ms, double *p99_ms, double *mean_ms,
                     size_t *peak_concurrent);
+/* layer calls that carried more than one BERT request, and those requests */
+int nx_server_batching_stats(const nx_server *s, size_t *packed_calls, size_t *packed_requests, size_t *peak_footprint);
 
 /* ---- encrypted BERT-base encoder layer (include/nexus/bert.hpp).  NEXUS ships
  * the pieces (MatrixMul with plaintext weights, GELU src/gelu.cpp, softmax
@@ -109,7 +128,6 @@ int nx_server_stats(const nx_server *s, size_t *completed, double *p50_ms, doubl
  * [in][out] float64, applied as x @ W: wq, wk, wv, wo hidden x hidden, w1
  * hidden x ffn, w2 ffn x hidden; hidden = 64 heads; no biases.  The chain needs
  * 19 compute levels below the bootstrapper's. */
-typedef struct nx_bert_layer nx_bert_layer;
 enum { NX_BERT_WQ = 0, NX_BERT_WK, NX_BERT_WV, NX_BERT_WO, NX_BERT_W1, NX_BERT_W2 };
 int nx_bert_layer_create(nx_bootstrapper *bt, int hidden, int heads, int ffn, const double *wq, const double *wk,
                          const double *wv, const double *wo, const double *w1, const double *w2, int baby,
@@ -122,6 +140,11 @@ int nx_bert_layer_rotation_steps(size_t slots, int hidden, int heads, int ffn, i
  * seq_len zero) -> the layer output, hidden / 256 ciphertexts at 2 limbs */
 int nx_bert_layer_forward(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n, int seq_len,
                           nx_ciphertext *const *out);
+/* several sequences packed in one set of ciphertexts: segment k is rows
+ * [starts[k], starts[k] + lens[k]); attention stays within each segment, so
+ * each sequence's rows equal what it would get alone; other rows must be zero */
+int nx_bert_layer_forward_packed(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n, const int *starts,
+                                 const int *lens, size_t segments, nx_ciphertext *const *out);
 /* stages: factor * x W (one limb out); attention of bootstrapped q (already
  * scaled by 1/8), k, v; x * invert_sqrt(mean(x^2)) per token */
 int nx_bert_layer_linear(nx_bert_layer *layer, int matrix, const nx_ciphertext *const *x, size_t n_in, double factor,
@@ -130,6 +153,11 @@ int nx_bert_layer_attention(nx_bert_layer *layer, const nx_ciphertext *const *q,
                             const nx_ciphertext *const *v, size_t n, int seq_len, nx_ciphertext *const *out);
 int nx_bert_layer_layer_norm(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n,
                              nx_ciphertext *const *out);
+/* stage timing: when on, forward() synchronises after each stage; the last
+ * forward's stages are read back as comma-separated names and milliseconds */
+int nx_bert_layer_set_stage_timing(nx_bert_layer *layer, int on);
+int nx_bert_layer_stage_ms(const nx_bert_layer *layer, char *names, size_t names_len, double *ms, size_t capacity,
+                           size_t *count);
 
 /* Host-only view of the linear-transform plan (genfftcoeff_3 / geninvfftcoeff_3,
  * Bootstrapper.cu:1137,1545); no device use.  which = 0: CoeffToSlot, 1:

@@ -755,10 +755,14 @@ _sig("nx_bert_layer_destroy", [vp])
 _sig("nx_bert_layer_rotation_steps", [sz, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.POINTER(ctypes.c_int), sz, ctypes.POINTER(sz)])
 _sig("nx_bert_layer_forward", [vp, ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.POINTER(vp)])
+_sig("nx_bert_layer_forward_packed", [vp, ctypes.POINTER(vp), sz, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), sz, ctypes.POINTER(vp)])
 _sig("nx_bert_layer_linear", [vp, ctypes.c_int, ctypes.POINTER(vp), sz, ctypes.c_double, ctypes.POINTER(vp), sz])
 _sig("nx_bert_layer_attention", [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), sz, ctypes.c_int,
                                  ctypes.POINTER(vp)])
 _sig("nx_bert_layer_layer_norm", [vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
+_sig("nx_bert_layer_set_stage_timing", [vp, ctypes.c_int])
+_sig("nx_bert_layer_stage_ms", [vp, ctypes.c_char_p, sz, _f64p, sz, ctypes.POINTER(sz)])
 
 BERT_GROUP = 256
 BERT_MATRICES = ("wq", "wk", "wv", "wo", "w1", "w2")
@@ -819,9 +823,17 @@ class BertLayer:
     def _outs(self, n):
         return [Ciphertext(self.ctx) for _ in range(n)]
 
-    def forward(self, x, seq_len):
+    def forward(self, x, seq_len=None, segments=None):
+        """one sequence of seq_len rows, or several packed sequences given as
+        segments [(start, len), ...] (attention stays within each)"""
         out = self._outs(len(x))
-        _check(_lib.nx_bert_layer_forward(self.h, _handles(x), len(x), seq_len, _handles(out)))
+        if segments is None:
+            _check(_lib.nx_bert_layer_forward(self.h, _handles(x), len(x), seq_len, _handles(out)))
+        else:
+            st = (ctypes.c_int * len(segments))(*[int(a) for a, _ in segments])
+            ln = (ctypes.c_int * len(segments))(*[int(b) for _, b in segments])
+            _check(_lib.nx_bert_layer_forward_packed(self.h, _handles(x), len(x), st, ln, len(segments),
+                                                     _handles(out)))
         return out
 
     def linear(self, matrix, x, factor=1.0):
@@ -842,6 +854,18 @@ class BertLayer:
         _check(_lib.nx_bert_layer_layer_norm(self.h, _handles(x), len(x), _handles(out)))
         return out
 
+    def set_stage_timing(self, on):
+        _check(_lib.nx_bert_layer_set_stage_timing(self.h, 1 if on else 0))
+
+    def stage_ms(self):
+        """[(stage, ms)] of the last forward() with stage timing on"""
+        n = sz()
+        names = ctypes.create_string_buffer(1024)
+        ms = (ctypes.c_double * 32)()
+        _check(_lib.nx_bert_layer_stage_ms(self.h, names, 1024, ms, 32, ctypes.byref(n)))
+        keys = names.value.decode().split(",") if n.value else []
+        return list(zip(keys, list(ms)[: n.value]))
+
 
 # ---------------------------------------------------------------- serving (include/nexus_eval.h)
 _sig("nx_server_create", [vp, ctypes.c_int, sz, vp, ctypes.POINTER(vp)])
@@ -853,20 +877,31 @@ _sig("nx_server_drain", [vp])
 _sig("nx_server_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(sz)])
 
-JOB_GELU, JOB_SOFTMAX, JOB_LAYER_NORM, JOB_BOOTSTRAP = 0, 1, 2, 3
+_sig("nx_server_create_bert", [vp, ctypes.c_int, sz, vp, vp, ctypes.c_int, sz, ctypes.POINTER(vp)])
+_sig("nx_server_submit_bert", [vp, ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)])
+_sig("nx_server_pause", [vp, ctypes.c_int])
+_sig("nx_server_batching_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)])
+
+JOB_GELU, JOB_SOFTMAX, JOB_LAYER_NORM, JOB_BOOTSTRAP, JOB_BERT = 0, 1, 2, 3, 4
 
 
 class Server:
     """Request loop: requests (lists of evaluator calls) run on `lanes` concurrent
     streams, admitted in arrival order within `memory_budget` bytes of footprint."""
 
-    def __init__(self, ev, lanes=2, memory_budget=0, bootstrapper=None):
+    def __init__(self, ev, lanes=2, memory_budget=0, bootstrapper=None, bert=None, pack=True, bert_footprint=0):
+        """bert: a BertLayer whose weights every lane shares (requests via
+        submit_bert); pack: continuous batching of BERT requests"""
         self.ev = ev
         self.ctx = ev.ctx
-        self._keep = (ev, bootstrapper)
+        self._keep = (ev, bootstrapper, bert)
         h = vp()
-        _check(_lib.nx_server_create(ev.h, lanes, memory_budget, bootstrapper.h if bootstrapper else None,
-                                     ctypes.byref(h)))
+        if bert is not None:
+            _check(_lib.nx_server_create_bert(ev.h, lanes, memory_budget, bert.bt.h, bert.h, 1 if pack else 0,
+                                              bert_footprint, ctypes.byref(h)))
+        else:
+            _check(_lib.nx_server_create(ev.h, lanes, memory_budget, bootstrapper.h if bootstrapper else None,
+                                         ctypes.byref(h)))
         self.h = h
         self._counts = {}
 
@@ -882,6 +917,21 @@ class Server:
         _check(_lib.nx_server_submit(self.h, k, ln, ins, len(jobs), ctypes.byref(rid)))
         self._counts[rid.value] = len(jobs)
         return rid.value
+
+    def submit_bert(self, cts, seq_len):
+        """one BERT-layer request: a sequence at rows [0, seq_len) of hidden/256 ciphertexts"""
+        rid = ctypes.c_uint64()
+        _check(_lib.nx_server_submit_bert(self.h, _handles(cts), len(cts), seq_len, ctypes.byref(rid)))
+        self._counts[rid.value] = len(cts)
+        return rid.value
+
+    def pause(self, paused=True):
+        _check(_lib.nx_server_pause(self.h, 1 if paused else 0))
+
+    def batching_stats(self):
+        a, b, c = sz(), sz(), sz()
+        _check(_lib.nx_server_batching_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return {"packed_calls": a.value, "packed_requests": b.value, "peak_footprint": c.value}
 
     def wait(self, rid):
         cnt = self._counts.pop(rid)

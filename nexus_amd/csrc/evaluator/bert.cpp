@@ -2,6 +2,7 @@
 #include "../../../include/nexus/bert.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <stdexcept>
 
@@ -36,17 +37,24 @@ BertLayer::BertLayer(CKKSEvaluator &ck, Bootstrapper &bt, const BertShape &shape
         throw std::invalid_argument("BertLayer: baby steps must be a power of two in [2, 256]");
     if (w.size() != (size_t)kMatrices) throw std::invalid_argument("BertLayer: six weight matrices expected");
     T_ = slots_ / kGroup;
+    shared_ = std::make_shared<Shared>();
     const int D = shape.hidden, F = shape.ffn;
     const int dims[kMatrices][2] = {{D, D}, {D, D}, {D, D}, {D, D}, {D, F}, {F, D}};
     for (int m = 0; m < kMatrices; m++) {
         if (!w[m]) throw std::invalid_argument("BertLayer: null weight matrix");
-        Diagonals &d = diag_[m];
+        Weights &d = shared_->m[m];
         d.n_in = dims[m][0] / kGroup;
         d.n_out = dims[m][1] / kGroup;
         d.a_min = floor_div(-(kGroup - 1), opt.baby);
         d.a_max = floor_div(kGroup - 1, opt.baby);
         d.w.assign(w[m], w[m] + (size_t)dims[m][0] * dims[m][1]);
     }
+}
+
+BertLayer::BertLayer(const BertLayer &o, CKKSEvaluator &ck, Bootstrapper &bt)
+    : ck_(ck), bt_(bt), shape_(o.shape_), opt_(o.opt_), slots_(o.slots_), T_(o.T_), shared_(o.shared_)
+{
+    if (ck.context->get() != o.ck_.context->get()) throw std::invalid_argument("BertLayer: view on another context");
 }
 
 std::vector<int> BertLayer::rotation_steps(size_t slots, const BertShape &shape, const Options &opt)
@@ -83,17 +91,19 @@ std::vector<int> BertLayer::rotation_steps(size_t slots, const BertShape &shape,
     return s;
 }
 
-void BertLayer::encode_diagonals(Matrix m, double factor, size_t chain_index, double scale)
+std::shared_ptr<const BertLayer::Encoded> BertLayer::diagonals(Matrix m, double factor, size_t chain_index,
+                                                                double scale)
 {
-    Diagonals &d = diag_[m];
+    std::lock_guard<std::mutex> g(shared_->mu);
+    Weights &d = shared_->m[m];
+    const auto key = std::make_tuple(factor, chain_index, scale);
+    for (auto &e : d.enc)
+        if (e.first == key) return e.second;
     const int B = opt_.baby, n_a = d.a_max - d.a_min + 1;
     const size_t per = (size_t)d.n_in * d.n_out * n_a * B;
-    if (d.pt.size() == per && d.factor == factor && d.chain == chain_index && d.scale == scale) return;
-    d.pt.assign(per, Plaintext());
-    d.present.assign(per, 0);
-    d.factor = factor;
-    d.chain = chain_index;
-    d.scale = scale;
+    auto enc = std::make_shared<Encoded>();
+    enc->pt.resize(per);
+    enc->present.assign(per, 0);
     const int out_dim = d.n_out * kGroup;
     std::vector<double> pattern(kGroup), v(slots_);
     for (int o = 0; o < d.n_out; o++)
@@ -117,33 +127,40 @@ void BertLayer::encode_diagonals(Matrix m, double factor, size_t chain_index, do
                     const int shift = ((B * a) % kGroup + kGroup) % kGroup;
                     for (size_t y = 0; y < slots_; y++) v[y] = pattern[((int)(y % kGroup) - shift + kGroup) % kGroup];
                     const size_t k = (((size_t)o * d.n_in + c) * n_a + (size_t)(a - d.a_min)) * B + b;
-                    d.pt[k] = Plaintext(*ck_.context);
-                    ck_.encoder.encode(v, chain_index, scale, d.pt[k]);
-                    d.present[k] = 1;
+                    enc->pt[k] = Plaintext(*ck_.context);
+                    ck_.encoder.encode(v, chain_index, scale, enc->pt[k]);
+                    enc->present[k] = 1;
                 }
+    // complete on this thread's stream before any lane reads them
+    ck_.context->synchronize();
+    if (d.enc.size() >= 2) d.enc.erase(d.enc.begin());
+    d.enc.emplace_back(key, enc);
+    return enc;
 }
 
 std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext> &x, double factor)
 {
     if (m < 0 || m >= kMatrices) throw std::invalid_argument("BertLayer: unknown matrix");
-    Diagonals &d = diag_[m];
+    const Weights &d = shared_->m[m];
     if (x.size() != (size_t)d.n_in) throw std::invalid_argument("BertLayer: wrong number of input ciphertexts");
     const Evaluator &ev = ck_.evaluator;
     const double delta = ck_.scale;
     const size_t two = chain_of(2);
-    // inputs at two limbs and exactly the evaluator's scale
+    // inputs at two limbs, all at one scale s_in; the plaintexts are encoded
+    // at delta q_1 / s_in so that the rescale (dropping q_1) lands exactly on
+    // delta -- no scale is overwritten
     std::vector<Ciphertext> in(x.size());
+    const double s_in = x[0].scale();
+    if (std::fabs(s_in / delta - 1.0) > 0.05)
+        throw std::invalid_argument("BertLayer: linear input is far from the evaluator's scale");
     for (size_t c = 0; c < x.size(); c++) {
         if (x[c].coeff_modulus_size() < 2) throw std::invalid_argument("BertLayer: linear input needs two limbs");
-        if (std::fabs(x[c].scale() / delta - 1.0) > 1e-3)
-            throw std::invalid_argument("BertLayer: linear input is not at the evaluator's scale");
+        if (x[c].scale() != s_in) throw std::invalid_argument("BertLayer: linear inputs at different scales");
         in[c] = x[c];
         if (in[c].coeff_modulus_size() > 2) ev.mod_switch_to_inplace(in[c], two);
-        in[c].set_scale(delta);
     }
-    // the plaintexts' scale is the prime the rescale drops: outputs land at delta
-    const double sigma = (double)prime(2);
-    encode_diagonals(m, factor, two, sigma);
+    const double sigma = delta * (double)prime(2) / s_in;
+    const std::shared_ptr<const Encoded> enc = diagonals(m, factor, two, sigma);
     const int B = opt_.baby, n_a = d.a_max - d.a_min + 1;
     std::vector<int> bsteps;
     for (int b = 1; b < B; b++) bsteps.push_back(b);
@@ -164,9 +181,9 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
             for (int c = 0; c < d.n_in; c++)
                 for (int b = 0; b < B; b++) {
                     const size_t k = (((size_t)o * d.n_in + c) * n_a + (size_t)(a - d.a_min)) * B + b;
-                    if (!d.present[k]) continue;
+                    if (!enc->present[k]) continue;
                     terms.push_back(&baby[c][b]);
-                    weights.push_back(&d.pt[k]);
+                    weights.push_back(&enc->pt[k]);
                 }
             if (terms.empty()) continue;
             Ciphertext acc;
@@ -217,6 +234,53 @@ std::vector<Ciphertext> BertLayer::token_rotations(const Ciphertext &x, int t0, 
     return out;
 }
 
+Ciphertext BertLayer::rotate_tokens(const Ciphertext &x, int t)
+{
+    // rot(x, 256 t) with the token-rotation keys: 256 tb a then 256 b
+    const Evaluator &ev = ck_.evaluator;
+    const int T = (int)T_;
+    t = ((t % T) + T) % T;
+    const int tb = sqrt_pow2(T_);
+    Ciphertext y = x;
+    if (t / tb) ev.rotate_vector_inplace(y, kGroup * tb * (t / tb), *ck_.galois_keys);
+    if (t % tb) ev.rotate_vector_inplace(y, kGroup * (t % tb), *ck_.galois_keys);
+    return y;
+}
+
+std::vector<Ciphertext> BertLayer::shift_rows(const std::vector<Ciphertext> &x, int offset)
+{
+    if (offset < 0 || (size_t)offset >= T_) throw std::invalid_argument("BertLayer: row offset out of range");
+    std::vector<Ciphertext> out;
+    for (const auto &c : x) out.push_back(offset ? rotate_tokens(c, -offset) : c);
+    return out;
+}
+
+std::vector<Ciphertext> BertLayer::extract_rows(const std::vector<Ciphertext> &x, int offset, int len)
+{
+    if (offset < 0 || len < 1 || (size_t)(offset + len) > T_)
+        throw std::invalid_argument("BertLayer: rows out of range");
+    const Evaluator &ev = ck_.evaluator;
+    std::vector<Ciphertext> out;
+    for (const auto &c : x) {
+        Ciphertext y = offset ? rotate_tokens(c, offset) : c;
+        const size_t l = y.coeff_modulus_size();
+        if (l < 2) throw std::invalid_argument("BertLayer: extract_rows needs two limbs");
+        auto key = std::make_tuple(-1000 - len, l, (double)prime(l), std::vector<int>());
+        auto it = masks_.find(key);
+        if (it == masks_.end()) {
+            std::vector<double> v(slots_, 0.0);
+            std::fill(v.begin(), v.begin() + (size_t)kGroup * len, 1.0);
+            Plaintext pt(*ck_.context);
+            ck_.encoder.encode(v, chain_of(l), (double)prime(l), pt);
+            it = masks_.emplace(key, std::move(pt)).first;
+        }
+        ev.multiply_plain_inplace(y, it->second);
+        ev.rescale_to_next_inplace(y);
+        out.push_back(std::move(y));
+    }
+    return out;
+}
+
 std::vector<Ciphertext> BertLayer::offset_rotations(const Ciphertext &x, int dmin, int dmax)
 {
     // rot(x, d) for d in [dmin, dmax], d = 8 a + b
@@ -243,7 +307,7 @@ std::vector<Ciphertext> BertLayer::offset_rotations(const Ciphertext &x, int dmi
 
 const Plaintext &BertLayer::offset_mask(int p, size_t limbs, double scale)
 {
-    auto key = std::make_tuple(p, limbs, scale);
+    auto key = std::make_tuple(p, limbs, scale, std::vector<int>());
     auto it = masks_.find(key);
     if (it != masks_.end()) return it->second;
     std::vector<double> v(slots_, 0.0);
@@ -253,14 +317,102 @@ const Plaintext &BertLayer::offset_mask(int p, size_t limbs, double scale)
     return masks_.emplace(key, std::move(pt)).first->second;
 }
 
+std::vector<int> BertLayer::segment_of(const std::vector<Segment> &segments) const
+{
+    std::vector<int> seg(T_, -1);
+    for (size_t k = 0; k < segments.size(); k++) {
+        const Segment &g = segments[k];
+        if (g.len < 1 || g.start < 0 || (size_t)g.start + (size_t)g.len > T_)
+            throw std::invalid_argument("BertLayer: segment out of range");
+        for (int i = g.start; i < g.start + g.len; i++) {
+            if (seg[i] >= 0) throw std::invalid_argument("BertLayer: overlapping segments");
+            seg[i] = (int)k;
+        }
+    }
+    if (segments.empty()) throw std::invalid_argument("BertLayer: no segment");
+    return seg;
+}
+
+const Plaintext &BertLayer::key_mask(int eta, int tc, const std::vector<int> &seg, size_t limbs)
+{
+    // 1 where query i and key (i + 64 eta + t') mod T lie in one segment, t' < tc;
+    // at the scale of the prime the next rescale drops
+    const double scale = (double)prime(limbs);
+    auto key = std::make_tuple(-1 - eta, limbs, scale, seg);
+    auto it = masks_.find(key);
+    if (it != masks_.end()) return it->second;
+    std::vector<double> km(slots_, 0.0);
+    const int T = (int)T_;
+    for (int i = 0; i < T; i++) {
+        if (seg[i] < 0) continue;
+        for (int h = 0; h < kGroup / kHeadDim; h++)
+            for (int tp = 0; tp < tc; tp++)
+                if (seg[(i + kHeadDim * eta + tp) % T] == seg[i]) km[(size_t)kGroup * i + kHeadDim * h + tp] = 1.0;
+    }
+    Plaintext pt(*ck_.context);
+    ck_.encoder.encode(km, chain_of(limbs), scale, pt);
+    return masks_.emplace(key, std::move(pt)).first->second;
+}
+
+const Plaintext &BertLayer::row_mask(int tag, const std::vector<int> &seg, const std::vector<Segment> &segments,
+                                     size_t limbs, double scale, int period)
+{
+    // 1 / (length of token i's segment) at slots 256 i + m period of token i's
+    // group (0 outside the segments)
+    auto key = std::make_tuple(-100 - tag, limbs, scale, seg);
+    auto it = masks_.find(key);
+    if (it != masks_.end()) return it->second;
+    std::vector<double> v(slots_, 0.0);
+    for (size_t i = 0; i < T_; i++) {
+        if (seg[i] < 0) continue;
+        const double w = 1.0 / segments[seg[i]].len;
+        for (int y = 0; y < kGroup; y += period) v[kGroup * i + y] = w;
+    }
+    Plaintext pt(*ck_.context);
+    ck_.encoder.encode(v, chain_of(limbs), scale, pt);
+    return masks_.emplace(key, std::move(pt)).first->second;
+}
+
+void BertLayer::bootstrap_real(const std::vector<Ciphertext *> &cts, double factor)
+{
+    // real messages two at a time: y = a + i b (X^(N/2) multiplies every slot
+    // by i), y' = bootstrap(y) factor / 2, then a' = y' + conj(y') and
+    // b' = -i (y' - conj(y')) -- one bootstrap and one conjugation per pair
+    const Evaluator &ev = ck_.evaluator;
+    const int64_t N = (int64_t)ck_.context->poly_degree();
+    size_t k = 0;
+    while (k < cts.size()) {
+        Ciphertext &a = *cts[k];
+        if (k + 1 < cts.size() && cts[k + 1]->scale() == a.scale() &&
+            cts[k + 1]->coeff_modulus_size() == a.coeff_modulus_size()) {
+            Ciphertext &b = *cts[k + 1];
+            Ciphertext y;
+            ev.multiply_power_of_x(b, N / 2, y);
+            ev.add_inplace(y, a);
+            Ciphertext yb, yc, d;
+            bt_.bootstrap(y, yb, factor / 2);
+            ev.complex_conjugate(yb, *ck_.galois_keys, yc);
+            ev.add(yb, yc, a);
+            ev.sub(yb, yc, d);
+            ev.multiply_power_of_x(d, 3 * N / 2, b);
+            k += 2;
+        } else {
+            Ciphertext o;
+            bt_.bootstrap(a, o, factor);
+            a = std::move(o);
+            k += 1;
+        }
+    }
+}
+
 Ciphertext BertLayer::scaled(const Ciphertext &x, double value)
 {
+    // x * value at two limbs -> one limb, landing exactly on the evaluator's scale
     const Evaluator &ev = ck_.evaluator;
     Ciphertext y = x;
     if (y.coeff_modulus_size() > 2) ev.mod_switch_to_inplace(y, chain_of(2));
-    y.set_scale(ck_.scale);
     Plaintext c(*ck_.context);
-    ck_.encoder.encode(value, y.params_id(), (double)prime(y.coeff_modulus_size()), c);
+    ck_.encoder.encode(value, y.params_id(), ck_.scale * (double)prime(y.coeff_modulus_size()) / y.scale(), c);
     ev.multiply_plain_inplace(y, c);
     ev.rescale_to_next_inplace(y);
     return y;
@@ -269,26 +421,34 @@ Ciphertext BertLayer::scaled(const Ciphertext &x, double value)
 std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
                                              const std::vector<Ciphertext> &v, int s)
 {
+    if (s < 1 || (size_t)s > T_) throw std::invalid_argument("BertLayer: sequence length out of range");
+    return attention(q, k, v, std::vector<Segment>{{0, s}});
+}
+
+std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
+                                             const std::vector<Ciphertext> &v, const std::vector<Segment> &segments)
+{
     const size_t n = (size_t)shape_.hidden / kGroup;
     if (q.size() != n || k.size() != n || v.size() != n)
         throw std::invalid_argument("BertLayer: attention needs hidden / 256 ciphertexts of q, k and v");
-    if (s < 1 || (size_t)s > T_) throw std::invalid_argument("BertLayer: sequence length out of range");
+    const std::vector<int> seg = segment_of(segments);
     const Evaluator &ev = ck_.evaluator;
     const RelinKeys &rk = *ck_.relin_keys;
     const GaloisKeys &gk = *ck_.galois_keys;
     const int T = (int)T_;
     const int halves = (T + kHeadDim - 1) / kHeadDim;
-    std::vector<Ciphertext> out(n);
+    // probabilities of every ciphertext and half, refreshed together
+    std::vector<std::vector<Ciphertext>> Pe(n, std::vector<Ciphertext>(halves));
+    std::vector<int> tcount(halves);
+    for (int eta = 0; eta < halves; eta++) tcount[eta] = std::min(kHeadDim, T - kHeadDim * eta);
     for (size_t c = 0; c < n; c++) {
         Ciphertext qc = q[c], kc = k[c];
         if (kc.coeff_modulus_size() > qc.coeff_modulus_size()) ev.mod_switch_to_inplace(kc, qc.params_id());
         if (qc.coeff_modulus_size() > kc.coeff_modulus_size()) ev.mod_switch_to_inplace(qc, kc.params_id());
         // ---- scores, compacted: S^eta[256 i + 64 h + t'] = q_i . k_(i + 64 eta + t') on head h
         std::vector<Ciphertext> S(halves);
-        std::vector<int> tcount(halves);
         for (int eta = 0; eta < halves; eta++) {
-            const int t0 = kHeadDim * eta, tc = std::min(kHeadDim, T - t0);
-            tcount[eta] = tc;
+            const int t0 = kHeadDim * eta, tc = tcount[eta];
             std::vector<Ciphertext> kr = token_rotations(kc, t0, tc);
             std::vector<Ciphertext> P(tc);
             for (int tau = 0; tau < tc; tau++) {
@@ -340,15 +500,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
         std::vector<Ciphertext> E(halves);
         for (int eta = 0; eta < halves; eta++) {
             E[eta] = ck_.exp(S[eta]);
-            const size_t le = E[eta].coeff_modulus_size();
-            std::vector<double> km(slots_, 0.0);
-            for (int i = 0; i < s; i++)
-                for (int h = 0; h < kGroup / kHeadDim; h++)
-                    for (int tp = 0; tp < tcount[eta]; tp++)
-                        if ((i + kHeadDim * eta + tp) % T < s) km[(size_t)kGroup * i + kHeadDim * h + tp] = 1.0;
-            Plaintext kmp(*ck_.context);
-            ck_.encoder.encode(km, E[eta].params_id(), (double)prime(le), kmp);
-            ev.multiply_plain_inplace(E[eta], kmp);
+            ev.multiply_plain_inplace(E[eta], key_mask(eta, tcount[eta], seg, E[eta].coeff_modulus_size()));
             ev.rescale_to_next_inplace(E[eta]);
         }
         Ciphertext tot = E[0];
@@ -358,45 +510,47 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             ev.rotate_vector(tot, st, gk, r);
             ev.add_inplace(tot, r);
         }
-        {
-            const size_t lt = tot.coeff_modulus_size();
-            std::vector<double> m0(slots_, 0.0);
-            for (size_t y = 0; y < slots_; y += kHeadDim) m0[y] = 1.0 / s;
-            Plaintext p0(*ck_.context);
-            ck_.encoder.encode(m0, tot.params_id(), (double)prime(lt), p0);
-            ev.multiply_plain_inplace(tot, p0);
-            ev.rescale_to_next_inplace(tot);
-        }
+        // sum / len at the first slot of every 64-slot block
+        ev.multiply_plain_inplace(tot, row_mask(0, seg, segments, tot.coeff_modulus_size(),
+                                                (double)prime(tot.coeff_modulus_size()), kHeadDim));
+        ev.rescale_to_next_inplace(tot);
         for (int st = 1; st < kHeadDim; st <<= 1) {
             Ciphertext r;
             ev.rotate_vector(tot, -st, gk, r);
             ev.add_inplace(tot, r);
         }
         Ciphertext inv = ck_.inverse(tot);
-        {
-            Plaintext cs(*ck_.context);
-            ck_.encoder.encode(1.0 / s, inv.params_id(), (double)prime(inv.coeff_modulus_size()), cs);
-            ev.multiply_plain_inplace(inv, cs);
-            ev.rescale_to_next_inplace(inv);
+        ev.multiply_plain_inplace(inv, row_mask(1, seg, segments, inv.coeff_modulus_size(),
+                                                (double)prime(inv.coeff_modulus_size()), 1));
+        ev.rescale_to_next_inplace(inv);
+        for (int eta = 0; eta < halves; eta++) {
+            Ciphertext &p = Pe[c][eta];
+            p = std::move(E[eta]);
+            ev.mod_switch_to_inplace(p, inv.params_id());
+            ev.multiply_inplace(p, inv);
+            ev.relinearize_inplace(p, rk);
+            ev.rescale_to_next_inplace(p);
         }
-        // ---- probabilities, refreshed; expansion over the 64 features; times v
-        Ciphertext vc = v[c];
+    }
+    {
+        std::vector<Ciphertext *> all;
+        for (auto &row : Pe)
+            for (auto &p : row) all.push_back(&p);
+        bootstrap_real(all, 1.0);
+    }
+    // ---- expansion over the 64 features of each head; times the rotated v
+    std::vector<Ciphertext> out(n);
+    for (size_t c = 0; c < n; c++) {
         Ciphertext acc;
         bool have_acc = false;
         for (int eta = 0; eta < halves; eta++) {
             const int t0 = kHeadDim * eta, tc = tcount[eta];
-            Ciphertext Pe = E[eta];
-            ev.mod_switch_to_inplace(Pe, inv.params_id());
-            ev.multiply_inplace(Pe, inv);
-            ev.relinearize_inplace(Pe, rk);
-            ev.rescale_to_next_inplace(Pe);
-            Ciphertext Pb;
-            bt_.bootstrap(Pe, Pb);
+            const Ciphertext &Pb = Pe[c][eta];
             // Pb_tau[256 i + 64 h + f] = P[256 i + 64 h + tau] = sum_d A_(tau - d) (.) rot(P, d)
             std::vector<Ciphertext> R = offset_rotations(Pb, -(kHeadDim - 1), tc - 1);
             const size_t lb = Pb.coeff_modulus_size();
             const double ms = (double)prime(lb);
-            Ciphertext vlow = vc;
+            Ciphertext vlow = v[c];
             if (vlow.coeff_modulus_size() > lb - 1) ev.mod_switch_to_inplace(vlow, chain_of(lb - 1));
             std::vector<Ciphertext> vr = token_rotations(vlow, t0, tc);
             for (int tau = 0; tau < tc; tau++) {
@@ -478,53 +632,80 @@ std::vector<Ciphertext> BertLayer::layer_norm(const std::vector<Ciphertext> &r)
 
 std::vector<Ciphertext> BertLayer::forward(const std::vector<Ciphertext> &x, int s)
 {
+    if (s < 1 || (size_t)s > T_) throw std::invalid_argument("BertLayer: sequence length out of range");
+    return forward(x, std::vector<Segment>{{0, s}});
+}
+
+std::vector<Ciphertext> BertLayer::forward(const std::vector<Ciphertext> &x, const std::vector<Segment> &segments)
+{
     const size_t n = (size_t)shape_.hidden / kGroup;
     if (x.size() != n) throw std::invalid_argument("BertLayer: input needs hidden / 256 ciphertexts");
-    if (s < 1 || (size_t)s > T_) throw std::invalid_argument("BertLayer: sequence length out of range");
+    segment_of(segments);  // validates
     const Evaluator &ev = ck_.evaluator;
     const double pre = opt_.ln_prescale;
     auto boot_all = [&](std::vector<Ciphertext> &v, double factor) {
-        for (auto &c : v) {
-            Ciphertext o;
-            bt_.bootstrap(c, o, factor);
-            c = std::move(o);
-        }
+        std::vector<Ciphertext *> p;
+        for (auto &c : v) p.push_back(&c);
+        bootstrap_real(p, factor);
     };
     auto add_scaled = [&](std::vector<Ciphertext> &y, const std::vector<Ciphertext> &res) {
         for (size_t c = 0; c < y.size(); c++) {
             Ciphertext r8 = scaled(res[c], 1.0 / 8);
-            r8.set_scale(y[c].scale());
+            r8.set_scale(y[c].scale());  // both are the evaluator's scale, by construction
             ev.add_inplace(y[c], r8);
         }
     };
+    stage_ms_.clear();
+    auto t_last = std::chrono::steady_clock::now();
+    auto mark = [&](const char *name) {
+        if (!opt_.stage_timing) return;
+        ck_.context->synchronize();
+        const auto now = std::chrono::steady_clock::now();
+        stage_ms_.emplace_back(name, std::chrono::duration<double, std::milli>(now - t_last).count());
+        t_last = now;
+    };
+    if (opt_.stage_timing) ck_.context->synchronize(), t_last = std::chrono::steady_clock::now();
     // attention block; the bootstrapping range factors are folded into the weights
     std::vector<Ciphertext> q = linear(Wq, x, 1.0 / 32), k = linear(Wk, x, 0.25), v = linear(Wv, x, 0.25);
-    boot_all(q, 4);
-    boot_all(k, 4);
-    boot_all(v, 4);
-    std::vector<Ciphertext> o = attention(q, k, v, s);
+    mark("qkv");
+    {
+        std::vector<Ciphertext *> p;
+        for (auto *set : {&q, &k, &v})
+            for (auto &c : *set) p.push_back(&c);
+        bootstrap_real(p, 4);
+    }
+    mark("bootstrap_qkv");
+    std::vector<Ciphertext> o = attention(q, k, v, segments);
     q.clear();
     k.clear();
     v.clear();
+    mark("attention");
     std::vector<Ciphertext> r1 = linear(Wo, o, 1.0 / 8);
     add_scaled(r1, x);
+    mark("wo_residual");
     boot_all(r1, 8 * pre);
+    mark("bootstrap_ln1");
     std::vector<Ciphertext> z = layer_norm(r1);
-    for (auto &c : z) c.set_scale(ck_.scale);
+    mark("ln1");
     // feed-forward block
     std::vector<Ciphertext> h = linear(W1, z, 0.25);
+    mark("ffn1");
     boot_all(h, 4);
+    mark("bootstrap_ffn");
     GELUEvaluator gelu(ck_);
     for (auto &c : h) {
         Ciphertext g;
         gelu.gelu(c, g);
         c = std::move(g);
     }
+    mark("gelu");
     std::vector<Ciphertext> r2 = linear(W2, h, 1.0 / 8);
     add_scaled(r2, z);
+    mark("ffn2_residual");
     boot_all(r2, 8 * pre);
+    mark("bootstrap_ln2");
     std::vector<Ciphertext> y = layer_norm(r2);
-    for (auto &c : y) c.set_scale(ck_.scale);
+    mark("ln2");
     return y;
 }
 

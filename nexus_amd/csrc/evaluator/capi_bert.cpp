@@ -8,11 +8,6 @@ using nexus_capi::need;
 using nexus_capi::run;
 using nexus_capi::store;
 
-struct nx_bert_layer {
-    nx_bootstrapper *bt;
-    std::unique_ptr<nexus::BertLayer> layer;
-};
-
 namespace {
 
 std::vector<nexus::Ciphertext> inputs(const nx_bert_layer *l, const nx_ciphertext *const *x, size_t n)
@@ -86,6 +81,19 @@ int nx_bert_layer_forward(nx_bert_layer *layer, const nx_ciphertext *const *x, s
     });
 }
 
+int nx_bert_layer_forward_packed(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n, const int *starts,
+                                 const int *lens, size_t segments, nx_ciphertext *const *out)
+{
+    return run([&] {
+        need(layer);
+        need(starts);
+        need(lens);
+        std::vector<nexus::BertLayer::Segment> seg(segments);
+        for (size_t k = 0; k < segments; k++) seg[k] = {starts[k], lens[k]};
+        outputs(layer->layer->forward(inputs(layer, x, n), seg), out, n);
+    });
+}
+
 int nx_bert_layer_linear(nx_bert_layer *layer, int matrix, const nx_ciphertext *const *x, size_t n_in, double factor,
                          nx_ciphertext *const *out, size_t n_out)
 {
@@ -112,6 +120,35 @@ int nx_bert_layer_layer_norm(nx_bert_layer *layer, const nx_ciphertext *const *x
     return run([&] {
         need(layer);
         outputs(layer->layer->layer_norm(inputs(layer, x, n)), out, n);
+    });
+}
+
+int nx_bert_layer_set_stage_timing(nx_bert_layer *layer, int on)
+{
+    return run([&] {
+        need(layer);
+        layer->layer->set_stage_timing(on != 0);
+    });
+}
+
+int nx_bert_layer_stage_ms(const nx_bert_layer *layer, char *names, size_t names_len, double *ms, size_t capacity,
+                           size_t *count)
+{
+    return run([&] {
+        need(layer);
+        need(count);
+        const auto &st = layer->layer->stage_ms();
+        *count = st.size();
+        std::string joined;
+        for (size_t i = 0; i < st.size(); i++) {
+            if (i) joined += ",";
+            joined += st[i].first;
+            if (ms && i < capacity) ms[i] = st[i].second;
+        }
+        if (names && names_len) {
+            std::strncpy(names, joined.c_str(), names_len - 1);
+            names[names_len - 1] = 0;
+        }
     });
 }
 

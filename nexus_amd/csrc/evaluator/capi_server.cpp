@@ -35,6 +35,28 @@ int nx_server_create(nx_evaluator *ev, int lanes, size_t memory_budget, const nx
     });
 }
 
+int nx_server_create_bert(nx_evaluator *ev, int lanes, size_t memory_budget, const nx_bootstrapper *boot,
+                          const nx_bert_layer *layer, int pack, size_t bert_footprint, nx_server **out)
+{
+    return run([&] {
+        need(ev);
+        need(boot);
+        need(layer);
+        need(out);
+        if (!ev->rk || !ev->gk) throw std::invalid_argument("the evaluator needs relinearisation and Galois keys");
+        nexus::Server::Options o;
+        o.lanes = lanes;
+        o.memory_budget = memory_budget;
+        o.pack = pack != 0;
+        if (bert_footprint) o.bert_footprint = bert_footprint;
+        auto s = std::make_unique<nx_server>();
+        s->ev = ev;
+        s->impl = std::make_unique<nexus::Server>(*ev->ctx, ev->sk.get(), *ev->rk, *ev->gk, ev->ckks->scale, o,
+                                                  nexus_capi::bootstrapper_params(boot), layer->layer.get());
+        *out = s.release();
+    });
+}
+
 int nx_server_destroy(nx_server *s)
 {
     return run([&] { delete s; });
@@ -62,6 +84,25 @@ int nx_server_submit(nx_server *s, const int *kinds, const int *lens, const nx_c
     });
 }
 
+int nx_server_submit_bert(nx_server *s, const nx_ciphertext *const *x, size_t n, int seq_len, uint64_t *request_id)
+{
+    return run([&] {
+        need(s);
+        need(x);
+        need(request_id);
+        nexus::Job j;
+        j.kind = nexus::JobKind::Bert;
+        j.len = seq_len;
+        for (size_t k = 0; k < n; k++) {
+            need(x[k]);
+            j.inputs.push_back(nexus::Ciphertext::copy_of(*s->ev->ctx, x[k]));
+        }
+        std::vector<nexus::Job> jobs;
+        jobs.push_back(std::move(j));
+        *request_id = s->impl->submit(std::move(jobs));
+    });
+}
+
 int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outputs, size_t count)
 {
     return run([&] {
@@ -73,6 +114,14 @@ int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outp
             need(outputs[k]);
             nexus::check(nx_ciphertext_copy(outputs[k], res[k].get()));
         }
+    });
+}
+
+int nx_server_pause(nx_server *s, int paused)
+{
+    return run([&] {
+        need(s);
+        s->impl->pause(paused != 0);
     });
 }
 
@@ -95,6 +144,17 @@ int nx_server_stats(const nx_server *s, size_t *completed, double *p50_ms, doubl
         if (p99_ms) *p99_ms = st.p99_ms;
         if (mean_ms) *mean_ms = st.mean_ms;
         if (peak_concurrent) *peak_concurrent = st.peak_concurrent;
+    });
+}
+
+int nx_server_batching_stats(const nx_server *s, size_t *packed_calls, size_t *packed_requests, size_t *peak_footprint)
+{
+    return run([&] {
+        need(s);
+        const nexus::Server::Stats st = s->impl->stats();
+        if (packed_calls) *packed_calls = st.packed_calls;
+        if (packed_requests) *packed_requests = st.packed_requests;
+        if (peak_footprint) *peak_footprint = st.peak_footprint;
     });
 }
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <memory>
 
+#include "../../../include/nexus/bert.hpp"
 #include "../../../include/nexus/bootstrapper.hpp"
 #include "../../../include/nexus/ckks.hpp"
 #include "../../../include/nexus_eval.h"
@@ -17,6 +18,11 @@ struct nx_evaluator {
 struct nx_bootstrapper {
     nx_evaluator *ev;
     std::unique_ptr<nexus::Bootstrapper> bt;
+};
+
+struct nx_bert_layer {
+    nx_bootstrapper *bt;
+    std::unique_ptr<nexus::BertLayer> layer;
 };
 
 namespace nx {

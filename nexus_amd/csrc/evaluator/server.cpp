@@ -14,13 +14,17 @@ double Server::now_ms()
 }
 
 Server::Server(const CKKSContext &ctx, const SecretKey *sk, RelinKeys &rk, GaloisKeys &gk, double scale, Options opt,
-               const Bootstrapper::Params *boot)
-    : ctx_(ctx), sk_(sk), rk_(rk), gk_(gk), scale_(scale), opt_(opt)
+               const Bootstrapper::Params *boot, const BertLayer *bert)
+    : ctx_(ctx), sk_(sk), rk_(rk), gk_(gk), scale_(scale), opt_(opt), bert_(bert)
 {
     if (opt_.lanes < 1 || opt_.lanes > 16) throw std::invalid_argument("Server: lanes must be 1..16");
     if (boot) {
         have_boot_ = true;
         boot_ = *boot;
+    }
+    if (bert_) {
+        if (!boot) throw std::invalid_argument("Server: a BERT layer needs the bootstrapper parameters");
+        tokens_ = bert_->tokens_per_ct();
     }
     for (int i = 0; i < opt_.lanes; i++) workers_.emplace_back(&Server::worker, this, i + 1);
 }
@@ -45,7 +49,18 @@ uint64_t Server::submit(std::vector<Job> jobs)
     auto r = std::make_shared<Request>();
     size_t bytes = 0;
     const size_t n = ctx_.poly_degree();
+    size_t extra = 0;
     for (const Job &j : jobs) {
+        if (j.kind == JobKind::Bert) {
+            if (!bert_) throw std::invalid_argument("Server: no BERT layer configured");
+            if (j.inputs.size() != (size_t)bert_->shape().hidden / BertLayer::kGroup)
+                throw std::invalid_argument("Server: a BERT job needs hidden / 256 input ciphertexts");
+            if (j.len < 1 || (size_t)j.len > tokens_) throw std::invalid_argument("Server: sequence length out of range");
+            for (const Ciphertext &c : j.inputs)
+                if (!c.get()) throw std::invalid_argument("Server: job without input");
+            extra += opt_.bert_footprint;
+            continue;
+        }
         if (!j.input.get()) throw std::invalid_argument("Server: job without input");
         bytes += j.input.size() * j.input.coeff_modulus_size() * n * 8;
     }
@@ -53,7 +68,7 @@ uint64_t Server::submit(std::vector<Job> jobs)
     // worker's stream reads them
     ctx_.synchronize();
     r->jobs = std::move(jobs);
-    r->footprint = (size_t)(opt_.footprint_factor * (double)bytes);
+    r->footprint = (size_t)(opt_.footprint_factor * (double)bytes) + extra;
     {
         std::lock_guard<std::mutex> g(mu_);
         r->id = next_id_++;
@@ -85,6 +100,15 @@ std::vector<Ciphertext> Server::wait(uint64_t id)
     return res;
 }
 
+void Server::pause(bool paused)
+{
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        paused_ = paused;
+    }
+    cv_work_.notify_all();
+}
+
 void Server::drain()
 {
     std::unique_lock<std::mutex> lk(mu_);
@@ -99,6 +123,8 @@ Server::Stats Server::stats() const
     s.completed = latencies_.size();
     s.peak_concurrent = peak_concurrent_;
     s.peak_footprint = peak_footprint_;
+    s.packed_calls = packed_calls_;
+    s.packed_requests = packed_requests_;
     if (!latencies_.empty()) {
         std::vector<double> v = latencies_;
         std::sort(v.begin(), v.end());
@@ -136,11 +162,14 @@ void Server::run_lane(int lane)
     CKKSEvaluator ck(&ctx_, sk_, &rk_, &gk_, scale_);
     std::unique_ptr<Bootstrapper> boot;
     if (have_boot_) boot.reset(new Bootstrapper(ck, boot_));
+    std::unique_ptr<BertLayer> bert;
+    if (bert_) bert.reset(new BertLayer(*bert_, ck, *boot));
     GELUEvaluator gelu(ck);
     SoftmaxEvaluator softmax(ck);
     LNEvaluator ln(ck);
     for (;;) {
-        std::shared_ptr<Request> r;
+        // a request, plus (continuous batching) queued BERT requests packed with it
+        std::vector<std::shared_ptr<Request>> group;
         {
             std::unique_lock<std::mutex> lk(mu_);
             for (;;) {
@@ -151,7 +180,7 @@ void Server::run_lane(int lane)
                 // admission: the oldest request whose footprint fits; with nothing
                 // in flight the oldest is always admitted
                 auto pick = queue_.end();
-                for (auto it = queue_.begin(); it != queue_.end(); ++it) {
+                for (auto it = queue_.begin(); !paused_ && it != queue_.end(); ++it) {
                     if (opt_.memory_budget == 0 || in_flight_ == 0 ||
                         in_flight_bytes_ + (*it)->footprint <= opt_.memory_budget) {
                         pick = it;
@@ -159,33 +188,83 @@ void Server::run_lane(int lane)
                     }
                 }
                 if (pick != queue_.end()) {
-                    r = *pick;
+                    std::shared_ptr<Request> r = *pick;
                     queue_.erase(pick);
-                    in_flight_++;
-                    in_flight_bytes_ += r->footprint;
+                    r->charged = r->footprint;
+                    group.push_back(r);
+                    if (opt_.pack && bert && packable(*r)) {
+                        // later BERT requests whose rows still fit ride in the same set
+                        const Ciphertext &c0 = r->jobs[0].inputs[0];
+                        size_t used = (size_t)r->jobs[0].len;
+                        for (auto it = queue_.begin(); it != queue_.end() && used < tokens_;) {
+                            const Request &q = **it;
+                            if (packable(q) && used + (size_t)q.jobs[0].len <= tokens_ &&
+                                q.jobs[0].inputs[0].scale() == c0.scale() &&
+                                q.jobs[0].inputs[0].coeff_modulus_size() == c0.coeff_modulus_size()) {
+                                used += (size_t)q.jobs[0].len;
+                                (*it)->charged = 0;  // shares the set's working memory
+                                group.push_back(*it);
+                                it = queue_.erase(it);
+                            } else {
+                                ++it;
+                            }
+                        }
+                    }
+                    for (auto &g : group) {
+                        in_flight_++;
+                        in_flight_bytes_ += g->charged;
+                    }
                     peak_concurrent_ = std::max(peak_concurrent_, in_flight_);
                     peak_footprint_ = std::max(peak_footprint_, in_flight_bytes_);
+                    if (group.size() > 1) {
+                        packed_calls_++;
+                        packed_requests_ += group.size();
+                    }
                     break;
                 }
                 cv_work_.wait(lk);
             }
         }
-        std::vector<Ciphertext> outs;
+        std::vector<std::vector<Ciphertext>> outs(group.size());
         std::string err;
         try {
-            for (Job &j : r->jobs) {
-                Ciphertext out;
-                switch (j.kind) {
-                case JobKind::Gelu: gelu.gelu(j.input, out); break;
-                case JobKind::Softmax: softmax.softmax(j.input, out, j.len); break;
-                case JobKind::LayerNorm: ln.layer_norm(j.input, out, j.len); break;
-                case JobKind::Bootstrap:
-                    if (!boot) throw std::invalid_argument("server has no bootstrapper");
-                    boot->bootstrap(j.input, out);
-                    break;
-                default: throw std::invalid_argument("unknown job kind");
+            if (group.size() > 1) {
+                // pack: member k at rows [off_k, off_k + len_k), one layer call, unpack
+                std::vector<Ciphertext> set;
+                std::vector<BertLayer::Segment> segs;
+                int off = 0;
+                for (auto &g : group) {
+                    const Job &j = g->jobs[0];
+                    std::vector<Ciphertext> sh = bert->shift_rows(j.inputs, off);
+                    if (set.empty())
+                        set = std::move(sh);
+                    else
+                        for (size_t c = 0; c < set.size(); c++) ck.evaluator.add_inplace(set[c], sh[c]);
+                    segs.push_back({off, j.len});
+                    off += j.len;
                 }
-                outs.push_back(std::move(out));
+                std::vector<Ciphertext> y = bert->forward(set, segs);
+                for (size_t k = 0; k < group.size(); k++) outs[k] = bert->extract_rows(y, segs[k].start, segs[k].len);
+            } else {
+                for (Job &j : group[0]->jobs) {
+                    if (j.kind == JobKind::Bert) {
+                        if (!bert) throw std::invalid_argument("server has no BERT layer");
+                        for (auto &c : bert->forward(j.inputs, j.len)) outs[0].push_back(std::move(c));
+                        continue;
+                    }
+                    Ciphertext out;
+                    switch (j.kind) {
+                    case JobKind::Gelu: gelu.gelu(j.input, out); break;
+                    case JobKind::Softmax: softmax.softmax(j.input, out, j.len); break;
+                    case JobKind::LayerNorm: ln.layer_norm(j.input, out, j.len); break;
+                    case JobKind::Bootstrap:
+                        if (!boot) throw std::invalid_argument("server has no bootstrapper");
+                        boot->bootstrap(j.input, out);
+                        break;
+                    default: throw std::invalid_argument("unknown job kind");
+                    }
+                    outs[0].push_back(std::move(out));
+                }
             }
             // completion on this lane's stream before the outputs are handed over
             ctx_.synchronize();
@@ -197,16 +276,19 @@ void Server::run_lane(int lane)
             }
         }
         const double t = now_ms();
-        r->jobs.clear();  // inputs: their last use on this lane is complete
+        for (auto &g : group) g->jobs.clear();  // inputs: their last use on this lane is complete
         {
-            std::lock_guard<std::mutex> g(mu_);
-            r->outputs = std::move(outs);
-            r->error = err;
-            r->t_done = t;
-            r->done = true;
-            if (err.empty()) latencies_.push_back(t - r->t_submit);
-            in_flight_--;
-            in_flight_bytes_ -= r->footprint;
+            std::lock_guard<std::mutex> lk(mu_);
+            for (size_t k = 0; k < group.size(); k++) {
+                Request &g = *group[k];
+                g.outputs = std::move(outs[k]);
+                g.error = err;
+                g.t_done = t;
+                g.done = true;
+                if (err.empty()) latencies_.push_back(t - g.t_submit);
+                in_flight_--;
+                in_flight_bytes_ -= g.charged;
+            }
         }
         cv_done_.notify_all();
         cv_work_.notify_all();

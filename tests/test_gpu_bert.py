@@ -136,3 +136,53 @@ def test_layer_forward(env, s):
     print("layer s=%d: max |enc - fp64 model| %.3g mean %.3g; max |enc - exact| %.3g"
           % (s, err, np.mean(np.abs(got - want)), np.max(np.abs(got - exact))))
     assert err < 2.0 ** -10
+
+
+def test_layer_packed_sequences(env):
+    """three sequences packed in one set: each equals its own layer"""
+    segs = [(0, 3), (3, 2), (5, 3)]
+    _note("bert layer packed %s" % segs)
+    rng = np.random.default_rng(30)
+    x = np.zeros((env.T, HIDDEN))
+    x[:8] = rng.normal(0, 1, (8, HIDDEN))
+    out = env.layer.forward(env.enc(x, chain_index=len(CHAIN) - 3 - 1), segments=segs)
+    got = env.dec(out, env.T)
+    for a, n in segs:
+        want = ob.layer_approx(x[a:a + n], env.W)
+        err = np.max(np.abs(got[a:a + n] - want))
+        print("packed segment (%d, %d): max err %.3g" % (a, n, err))
+        assert err < 2.0 ** -10
+
+
+def test_server_continuous_batching(env):
+    """BERT requests queued together are packed into shared layer calls; each
+    result matches its own solo layer call and the FP64 model"""
+    lens = [3, 2, 3, 4, 4]
+    _note("bert server lens=%s" % lens)
+    rng = np.random.default_rng(40)
+    xs = [rng.normal(0, 1, (n, HIDDEN)) for n in lens]
+    two = len(CHAIN) - 3 - 1
+    encs = []
+    for k, n in enumerate(lens):
+        rows = np.zeros((env.T, HIDDEN))
+        rows[:n] = xs[k]
+        encs.append(env.enc(rows, chain_index=two))
+    srv = nx.Server(env.ev, lanes=2, bert=env.layer, pack=True)
+    srv.pause(True)
+    ids = [srv.submit_bert(c, n) for c, n in zip(encs, lens)]
+    srv.pause(False)
+    outs = [srv.wait(i) for i in ids]
+    st = srv.batching_stats()
+    del srv
+    print("server: %s" % st)
+    assert st["packed_calls"] >= 1 and st["packed_requests"] >= 2
+    for k, n in enumerate(lens):
+        got = env.dec(outs[k], env.T)
+        solo = env.dec(env.layer.forward(encs[k], n), env.T)[:n]
+        want = ob.layer_approx(xs[k], env.W)
+        e_model = np.max(np.abs(got[:n] - want))
+        e_solo = np.max(np.abs(got[:n] - solo))
+        print("request %d (len %d): max err vs model %.3g, vs solo call %.3g, other rows %.3g"
+              % (k, n, e_model, e_solo, np.max(np.abs(got[n:]))))
+        assert e_model < 2.0 ** -10 and e_solo < 2.0 ** -10
+        assert np.max(np.abs(got[n:])) < 2.0 ** -10
