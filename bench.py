@@ -276,6 +276,24 @@ def bert_setup(nx):
             "keys": len(steps) + 1}
 
 
+def bench_bert_call(nx, reps):
+    """one BERT-base layer call on a 128-token sequence (after a warm-up call
+    that encodes the weight diagonals): seconds per call and its stages"""
+    B = bert_setup(nx)
+    rows = np.random.default_rng(5).normal(0, 1, (BERT_TOKENS, 768))
+    cts = bert_encrypt(nx, B, rows)
+    B["layer"].forward(cts, BERT_TOKENS)
+    B["ctx"].synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        B["layer"].forward(cts, BERT_TOKENS)
+    B["ctx"].synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    B["layer"].set_stage_timing(True)
+    B["layer"].forward(cts, BERT_TOKENS)
+    return {"ms_per_layer_call": ms, "tokens": BERT_TOKENS, "stages_ms": dict(B["layer"].stage_ms())}
+
+
 def pack_sets(lens, capacity):
     """first-fit decreasing: [[(sequence, start), ...], ...]"""
     sets, fill = [], []
@@ -314,7 +332,7 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--lanes", type=int, default=3)
-    ap.add_argument("--only", choices=["bootstrap_dense", "bootstrap_sparse", "matmul", "ntt"],
+    ap.add_argument("--only", choices=["bert", "bert_served", "bootstrap_dense", "bootstrap_sparse", "matmul", "ntt"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -328,7 +346,19 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.only:
-        if args.only == "bootstrap_dense":
+        if args.only == "bert":
+            r = bench_bert_call(nx, args.steps)
+        elif args.only == "bert_served":
+            B = bert_setup(nx)
+            rows = np.random.default_rng(5).normal(0, 1, (BERT_TOKENS, 768))
+            cts = bert_encrypt(nx, B, rows)
+            B["layer"].forward(cts, BERT_TOKENS)
+            B["ctx"].synchronize()
+            t0 = time.perf_counter()
+            B["layer"].forward(cts, BERT_TOKENS)
+            B["ctx"].synchronize()
+            r = served_bert(nx, B, args, time.perf_counter() - t0)
+        elif args.only == "bootstrap_dense":
             r = bench_bootstrap(nx, 16, 15, [51] + [46] * 16 + [51] * 14 + [58] * 3, 3, args.steps)
         elif args.only == "bootstrap_sparse":
             r = bench_bootstrap(nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, args.steps)
@@ -473,6 +503,8 @@ def main():
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
         },
     }
+    if not args.no_served:
+        res["served"] = served_bert(nx, B, args, np.mean(plain_lat))
     W = B["W"]
     del enc_sets, enc_single, outs, layer, B, ctx
     if not args.no_extra:
@@ -488,6 +520,68 @@ def main():
         res["max_err_vs_exact_fp64_layer"] = max(exact)
         res["err_ok"] = bool(max(errs) <= 2.0 ** -10)
     print(json.dumps(res), flush=True)
+
+
+def served_bert(nx, B, args, plain_call_s):
+    """Poisson arrivals of single-sequence BERT-layer requests, lengths
+    ~U{16..128}: the plain per-call path (one lane, one call per request) and
+    the served path (lanes + continuous batching of queued requests)."""
+    rng = np.random.default_rng(77)
+    n = args.requests
+    lens = [int(x) for x in rng.integers(16, BERT_TOKENS + 1, n)]
+    reqs = []
+    for L_ in lens:
+        rows = np.zeros((BERT_TOKENS, 768))
+        rows[:L_] = rng.normal(0, 1, (L_, 768))
+        reqs.append((L_, bert_encrypt(nx, B, rows)))
+    rate = 2.0 / plain_call_s  # twice what the plain path sustains: requests queue up
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    arrivals -= arrivals[0]
+    B["ctx"].synchronize()
+    import threading
+
+    def run(lanes, pack):
+        srv = nx.Server(B["ev"], lanes=lanes, bert=B["layer"], pack=pack, memory_budget=96 << 30)
+        ids = [None] * n
+        t0 = time.perf_counter()
+
+        def submitter():
+            for i, (L_, cts) in enumerate(reqs):
+                delay = arrivals[i] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+                ids[i] = srv.submit_bert(cts, L_)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        srv.drain()
+        span = time.perf_counter() - t0
+        st = srv.stats()
+        st.update(srv.batching_stats())
+        for rid in ids:
+            srv.wait(rid)
+        del srv
+        return span, st
+
+    log("served BERT: %d requests, lengths %s, Poisson arrivals at %.3f/s" % (n, lens, rate))
+    span1, st1 = run(1, False)
+    log("plain per-call (1 lane): %.1f s, p50 %.0f ms, p99 %.0f ms" % (span1, st1["p50_ms"], st1["p99_ms"]))
+    spanb, stb = run(args.lanes, True)
+    log("served (%d lanes, packing): %.1f s, p50 %.0f ms, p99 %.0f ms, %d packed calls carrying %d requests"
+        % (args.lanes, spanb, stb["p50_ms"], stb["p99_ms"], stb["packed_calls"], stb["packed_requests"]))
+    return {
+        "workload": "single-sequence BERT-base layer requests, lengths ~U{16..128} (%s), Poisson arrivals at "
+                    "%.3f requests/s (2x the plain path's capacity); served = %d lanes, one BERT call in flight, "
+                    "continuous batching of queued requests" % (lens, rate, args.lanes),
+        "requests": n, "lanes": args.lanes,
+        "served": {"layers_per_s": n / spanb, "p50_ms": stb["p50_ms"], "p99_ms": stb["p99_ms"],
+                   "mean_ms": stb["mean_ms"], "makespan_s": spanb, "packed_calls": stb["packed_calls"],
+                   "packed_requests": stb["packed_requests"], "peak_concurrent": stb["peak_concurrent"]},
+        "plain_per_call": {"layers_per_s": n / span1, "p50_ms": st1["p50_ms"], "p99_ms": st1["p99_ms"],
+                           "mean_ms": st1["mean_ms"], "makespan_s": span1},
+        "throughput_ratio": span1 / spanb,
+    }
 
 
 def extras(nx, args):

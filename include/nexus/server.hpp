@@ -14,6 +14,8 @@
 // ciphertexts (128 token rows at N = 2^16); each is moved to its rows by a
 // token rotation, the set runs one layer call with attention kept inside each
 // sequence, and each request's rows are moved back and the others cleared.
+// At most bert_concurrency BERT calls run at once (one call keeps the GPU
+// busy), so requests arriving meanwhile queue up and are packed together.
 #pragma once
 
 #include <condition_variable>
@@ -48,6 +50,7 @@ public:
         size_t memory_budget = 0;    // bytes of in-flight request footprint; 0 = no limit
         double footprint_factor = 6; // footprint = factor x input ciphertext bytes
         bool pack = true;            // continuous batching of BERT requests
+        int bert_concurrency = 1;    // BERT layer calls in flight at once (one call fills the GPU)
         size_t bert_footprint = size_t(8) << 30;  // working set of one BERT layer call
     };
     struct Stats {
@@ -111,6 +114,7 @@ private:
     std::map<uint64_t, std::shared_ptr<Request>> all_;
     std::vector<double> latencies_;
     size_t in_flight_ = 0, in_flight_bytes_ = 0, submitted_ = 0;
+    int bert_running_ = 0;
     size_t peak_concurrent_ = 0, peak_footprint_ = 0, packed_calls_ = 0, packed_requests_ = 0;
     uint64_t next_id_ = 1;
     bool stop_ = false, paused_ = false;

@@ -170,6 +170,7 @@ void Server::run_lane(int lane)
     for (;;) {
         // a request, plus (continuous batching) queued BERT requests packed with it
         std::vector<std::shared_ptr<Request>> group;
+        bool bert_call = false;
         {
             std::unique_lock<std::mutex> lk(mu_);
             for (;;) {
@@ -181,6 +182,9 @@ void Server::run_lane(int lane)
                 // in flight the oldest is always admitted
                 auto pick = queue_.end();
                 for (auto it = queue_.begin(); !paused_ && it != queue_.end(); ++it) {
+                    const bool has_bert = std::any_of((*it)->jobs.begin(), (*it)->jobs.end(),
+                                                      [](const Job &j) { return j.kind == JobKind::Bert; });
+                    if (has_bert && bert_running_ >= opt_.bert_concurrency) continue;
                     if (opt_.memory_budget == 0 || in_flight_ == 0 ||
                         in_flight_bytes_ + (*it)->footprint <= opt_.memory_budget) {
                         pick = it;
@@ -192,6 +196,12 @@ void Server::run_lane(int lane)
                     queue_.erase(pick);
                     r->charged = r->footprint;
                     group.push_back(r);
+                    for (const Job &j : r->jobs)
+                        if (j.kind == JobKind::Bert) {
+                            bert_call = true;
+                            break;
+                        }
+                    if (bert_call) bert_running_++;
                     if (opt_.pack && bert && packable(*r)) {
                         // later BERT requests whose rows still fit ride in the same set
                         const Ciphertext &c0 = r->jobs[0].inputs[0];
@@ -279,6 +289,7 @@ void Server::run_lane(int lane)
         for (auto &g : group) g->jobs.clear();  // inputs: their last use on this lane is complete
         {
             std::lock_guard<std::mutex> lk(mu_);
+            if (bert_call) bert_running_--;
             for (size_t k = 0; k < group.size(); k++) {
                 Request &g = *group[k];
                 g.outputs = std::move(outs[k]);
