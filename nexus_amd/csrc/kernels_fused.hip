@@ -111,28 +111,44 @@ struct DotArgs {
     const uint64_t *pt[kMaxDot];
 };
 
-// out[k][limb][c] = sum_r ct_r[k][limb][c] * pt_r[limb][c]  (k < 2), folded every 32 products
-__global__ void k_dot_plain(uint64_t *out, DotArgs args, uint32_t count, const ModConst *mod, uint32_t limbs,
-                            uint32_t n)
+// out[k][limb][c] = sum_r ct_r[k][limb][c] * pt_r[limb][c]  (k < 2), folded
+// every 32 products.  A thread owns two adjacent coefficients of one limb for
+// both polynomials: every plaintext word is read once, all loads are 16 bytes.
+__global__ __launch_bounds__(kBlock) void k_dot_plain(uint64_t *out, DotArgs args, uint32_t count,
+                                                      const ModConst *mod, uint32_t limbs, uint32_t n)
 {
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t row = blockIdx.y;  // k * limbs + limb
-    const uint32_t limb = row % limbs;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;  // coefficient pair
+    if (2 * i >= n) return;
+    const uint32_t limb = blockIdx.y;
     const ModConst m = mod[limb];
-    const uint64_t off = (uint64_t)row * n + c, poff = (uint64_t)limb * n + c;
-    uint64_t lo = 0, hi = 0;
+    const uint64_t poff = (uint64_t)limb * n + 2 * i, off1 = poff + (uint64_t)limbs * n;
+    uint64_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
     for (uint32_t r = 0; r < count; r++) {
-        const uint64_t x = args.ct[r][off], y = args.pt[r][poff];
-        const uint64_t plo = x * y, phi = mulhi64(x, y);
-        lo += plo;
-        hi += phi + (lo < plo);
+        const ulonglong2 y = *reinterpret_cast<const ulonglong2 *>(args.pt[r] + poff);
+        const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(args.ct[r] + poff);
+        const ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(args.ct[r] + off1);
+        const uint64_t xs[4] = {a.x, a.y, b.x, b.y}, ys[4] = {y.x, y.y, y.x, y.y};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const uint64_t plo = xs[k] * ys[k], phi = mulhi64(xs[k], ys[k]);
+            lo[k] += plo;
+            hi[k] += phi + (lo[k] < plo);
+        }
         if ((r & 31) == 31) {
-            lo = barrett128(hi, lo, m);
-            hi = 0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                lo[k] = barrett128(hi[k], lo[k], m);
+                hi[k] = 0;
+            }
         }
     }
-    out[off] = barrett128(hi, lo, m);
+    ulonglong2 o0, o1;
+    o0.x = barrett128(hi[0], lo[0], m);
+    o0.y = barrett128(hi[1], lo[1], m);
+    o1.x = barrett128(hi[2], lo[2], m);
+    o1.y = barrett128(hi[3], lo[3], m);
+    *reinterpret_cast<ulonglong2 *>(out + poff) = o0;
+    *reinterpret_cast<ulonglong2 *>(out + off1) = o1;
 }
 
 }  // namespace
@@ -173,7 +189,8 @@ void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const 
         a.pt[i] = i < count ? pts[i] : nullptr;
     }
     ProfScope prof("dot_plain", s, 8.0 * n * limbs * (3.0 * count + 2.0));
-    hipLaunchKernelGGL(k_dot_plain, dim3((n + kBlock - 1) / kBlock, 2 * limbs), dim3(kBlock), 0, s, out, a, count,
+    // two coefficients per thread (n is a power of two >= 2)
+    hipLaunchKernelGGL(k_dot_plain, dim3((n / 2 + kBlock - 1) / kBlock, limbs), dim3(kBlock), 0, s, out, a, count,
                        t.mod, limbs, n);
 }
 
