@@ -257,6 +257,39 @@ def bench_ntt(nx, reps):
     return res
 
 
+def bench_dot(nx, reps):
+    """The dominant kernel class of the BERT layer (dot_plain: sum of 64
+    ciphertext x plaintext products) at the attention's shape: 64 ciphertexts of
+    2 x 19 limbs and 64 plaintexts at N = 2^16, next to a device-to-device copy
+    of one ciphertext (19.9 MB; the copy calibrates HBM counters)."""
+    ctx = nx.Context(16, BERT_CHAIN, special_primes=BERT_SPECIAL)
+    rng = np.random.default_rng(1)
+    limbs = 19
+    mods = ctx.moduli[:limbs]
+    ci = ctx.L + 1 - limbs
+    cts = [nx.Ciphertext.from_array(
+        ctx, np.stack([np.stack([rng.integers(0, q, ctx.n, dtype=np.uint64) for q in mods]) for _ in range(2)]),
+        ci, BERT_SCALE) for _ in range(64)]
+    pts = [nx.Plaintext.from_array(ctx, np.stack([rng.integers(0, q, ctx.n, dtype=np.uint64) for q in mods]), ci,
+                                   BERT_SCALE) for _ in range(64)]
+    out = nx.multiply_plain_accumulate(cts, pts)  # first launch loads the code object: untimed
+    ctx.synchronize()
+    nx.profile_select("dot_plain")
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = nx.multiply_plain_accumulate(cts, pts)
+        cp = cts[0].copy()
+    ctx.synchronize()
+    wall = (time.perf_counter() - t0) / reps * 1e3
+    launches, ms, b = nx.profile_read("dot_plain")
+    nx.profile_select("")
+    del out, cp
+    return {"ms_per_rep": wall, "terms": 64, "limbs": limbs, "copy_bytes": 2 * limbs * ctx.n * 8,
+            "dot_plain": {"launches": launches, "avg_us": ms / max(launches, 1) * 1e3,
+                          "algorithmic_bytes_per_launch": b / max(launches, 1),
+                          "achieved_gbs": (b / max(launches, 1)) / (ms / max(launches, 1) * 1e-3) / 1e9}}
+
+
 def bert_setup(nx):
     """context, keys, bootstrapper and BERT-base layer at N = 2^16"""
     from nexus_amd import bert_rotation_steps, boot_plan_steps
@@ -332,7 +365,8 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     ap.add_argument("--requests", type=int, default=12)
     ap.add_argument("--lanes", type=int, default=3)
-    ap.add_argument("--only", choices=["bert", "bert_served", "bootstrap_dense", "bootstrap_sparse", "matmul", "ntt"],
+    ap.add_argument("--only", choices=["bert", "bert_served", "dot", "bootstrap_dense", "bootstrap_sparse", "matmul",
+                                       "ntt"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -364,6 +398,8 @@ def main():
             r = bench_bootstrap(nx, 15, 13, [51] + [46] * 16 + [51] * 14 + [51], 1, args.steps)
         elif args.only == "ntt":
             r = bench_ntt(nx, args.steps)
+        elif args.only == "dot":
+            r = bench_dot(nx, args.steps)
         else:
             r = bench_matmul(nx, args.steps)
         print(json.dumps({"workload": args.only, **r}), flush=True)
