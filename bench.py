@@ -40,9 +40,14 @@ BERT_TOKENS = 128
 SCALE = 2.0 ** 40
 HBM_PEAK_GBS = 8000.0
 # HBM bytes moved per algorithmic byte, from FETCH_SIZE / WRITE_SIZE counter runs
-# of `bench.py --only ntt` calibrated against a device copy of known size
-# (profiles/ntt_fwd_n65536_38limbs_traffic.md)
-TRAFFIC_PER_ALGORITHMIC_BYTE = {"ntt_fwd_col": 40.59 / 40.00, "ntt_fwd_row": 67.29 / 79.54}
+# of `bench.py --only ntt` / `--only dot` calibrated against a device copy of
+# known size (profiles/ntt_fwd_n65536_38limbs_traffic.md,
+# profiles/dot_plain_n65536_64terms_traffic.md)
+TRAFFIC_PER_ALGORITHMIC_BYTE = {"ntt_fwd_col": 40.59 / 40.00, "ntt_fwd_row": 67.29 / 79.54,
+                                "dot_plain": 1930.5 / 1932.5}
+TRAFFIC_PROFILE = {"ntt_fwd_col": "profiles/ntt_fwd_n65536_38limbs_traffic.md",
+                   "ntt_fwd_row": "profiles/ntt_fwd_n65536_38limbs_traffic.md",
+                   "dot_plain": "profiles/dot_plain_n65536_64terms_traffic.md"}
 
 
 def log(msg):
@@ -532,8 +537,8 @@ def main():
             "traffic": (TRAFFIC_PER_ALGORITHMIC_BYTE[dominant] * k_bytes / max(launches, 1)
                         if dominant in TRAFFIC_PER_ALGORITHMIC_BYTE else None),
             "traffic_source": "calibrated FETCH_SIZE+WRITE_SIZE per algorithmic byte of this kernel class "
-                              "(profiles/ntt_fwd_n65536_38limbs_traffic.md) times this step's average "
-                              "algorithmic bytes per launch",
+                              "(%s) times this step's average algorithmic bytes per launch"
+                              % TRAFFIC_PROFILE.get(dominant, "no counter run for this class"),
             "launches": launches,
             "avg_launch_us": avg_launch_ms * 1e3,
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
