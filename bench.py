@@ -303,6 +303,7 @@ def bert_setup(nx):
     rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=22)
     steps = sorted(set(bert_rotation_steps(ctx.slot_count)) | set(boot_plan_steps(15, 15, 25)))
     gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps], seed=23)
+    tune = nx.autotune(ctx, reps=5)  # launch shapes for this ring size (bit-exact candidates only)
     ev = nx.CKKSEvaluator(ctx, rk, gk, BERT_SCALE, sk)
     bt = nx.Bootstrapper(ev)
     rng = np.random.default_rng(42)
@@ -311,7 +312,7 @@ def bert_setup(nx):
           ("w1", (768, 3072)), ("w2", (3072, 768)))}
     layer = nx.BertLayer(bt, W)
     return {"ctx": ctx, "sk": sk, "rk": rk, "gk": gk, "ev": ev, "bt": bt, "layer": layer, "W": W,
-            "keys": len(steps) + 1}
+            "keys": len(steps) + 1, "autotune": tune}
 
 
 def bench_bert_call(nx, reps):
@@ -526,6 +527,12 @@ def main():
             "special_primes": BERT_SPECIAL,
             "scale": "2^46",
             "galois_keys": B["keys"],
+            "autotuned": {
+                "ntt_tiles": ["%s batch %d: col %d row %d (%.1f vs %.1f us)" % (e["dir"], e["batch"], e["col"], e["row"],
+                                                                         e["us"], e["default_us"])
+                              for e in B["autotune"]["ntt"] if e["col"] or e["row"]],
+                "dot_plain_layouts": ["%d limbs: %d" % (d["limbs"], d["layout"]) for d in B["autotune"]["dot_plain"]],
+            },
         },
         "roofline": {
             "kernel": dominant,

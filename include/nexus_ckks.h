@@ -67,6 +67,14 @@ int nx_context_info(const nx_context *ctx, int *log_n, size_t *data_primes, size
 /* moduli: L + special words (q_0..q_{L-1}, p_0..) */
 int nx_context_moduli(const nx_context *ctx, uint64_t *moduli);
 int nx_context_synchronize(nx_context *ctx);
+/* Launch-shape autotuner (no counterpart in the reference, which fixes its
+ * launch shapes): times the NTT tile widths per pass, direction and batch
+ * bucket, and the multiply-accumulate thread layout per limb bucket, at this
+ * context's ring size, `reps` launches each.  A candidate is chosen only if its
+ * output equals the default shape's word for word.  The choices are
+ * process-wide and apply to later launches at this ring size.  report (may be
+ * null) receives a JSON summary; *needed its size including the terminator. */
+int nx_autotune(nx_context *ctx, int reps, char *report, size_t report_len, size_t *needed);
 /* Lanes: bind the calling thread to lane `lane` (1..64) of the context, a HIP
  * stream with its own memory cache; 0 returns the thread to the default stream.
  * Work of threads on different lanes overlaps on the device and shares the

@@ -627,6 +627,7 @@ _sig("nx_bootstrapper_create", [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ct
 _sig("nx_bootstrapper_destroy", [vp])
 _sig("nx_bootstrapper_rotation_steps", [vp, _ip, sz, ctypes.POINTER(sz)])
 _sig("nx_bootstrapper_depth", [vp, _ip])
+_sig("nx_autotune", [vp, ctypes.c_int, ctypes.c_char_p, sz, ctypes.POINTER(sz)])
 _sig("nx_bootstrap", [vp, vp, vp])
 _sig("nx_bootstrap_scaled", [vp, vp, ctypes.c_double, vp])
 _sig("nx_boot_mod_raise", [vp, vp, vp])
@@ -676,6 +677,17 @@ def boot_plan_level(log_half_n, log_slots, which, level, K=25):
     d = (re + 1j * im).reshape(c, nh)
     return {"levels": levels.value, "stride": stride.value, "g": g.value,
             "diags": {(js[k], is_[k]): d[k] for k in range(c)}}
+
+
+def autotune(ctx, reps=10):
+    """Choose NTT tile widths and the multiply-accumulate layout for ctx's ring
+    size by timing them (each candidate checked bit-exact against the default);
+    returns the report as a dict."""
+    import json
+    need = sz()
+    buf = ctypes.create_string_buffer(1 << 16)
+    _check(_lib.nx_autotune(ctx.h, reps, buf, 1 << 16, ctypes.byref(need)))
+    return json.loads(buf.value.decode())
 
 
 class Bootstrapper:

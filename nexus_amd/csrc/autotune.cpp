@@ -1,0 +1,189 @@
+// Autotuner of launch shapes at a context's ring size: the tile widths of the
+// two-pass NTT (per pass, direction and power-of-two batch bucket) and the
+// thread layout of the fused multiply-accumulate (per limb bucket).  Every
+// candidate's output is compared word for word with the default shape's on
+// the same data before it may be chosen; the choices are process-wide.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "context.h"
+
+namespace nx {
+
+namespace {
+
+struct Events {
+    hipEvent_t a, b;
+    Events()
+    {
+        NX_HIP(hipEventCreate(&a));
+        NX_HIP(hipEventCreate(&b));
+    }
+    ~Events()
+    {
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+    }
+};
+
+// residues of `count` limbs, limb i below the prime (i mod period)
+void fill_random(Context &c, std::vector<uint64_t> &h, size_t count, uint64_t seed, size_t period)
+{
+    h.resize(count * c.n);
+    uint64_t x = seed * 0x9E3779B97F4A7C15ull + 1;
+    for (size_t i = 0; i < count; i++) {
+        const uint64_t q = c.moduli[i % period];
+        for (size_t k = 0; k < c.n; k++) {
+            x ^= x << 13;
+            x ^= x >> 7;
+            x ^= x << 17;
+            h[i * c.n + k] = x % q;
+        }
+    }
+}
+
+std::vector<uint64_t> download(Context &c, const uint64_t *d, size_t words)
+{
+    std::vector<uint64_t> h(words);
+    NX_HIP(hipMemcpyAsync(h.data(), d, words * 8, hipMemcpyDeviceToHost, c.stream));
+    NX_HIP(hipStreamSynchronize(c.stream));
+    return h;
+}
+
+}  // namespace
+
+std::string autotune(Context &c, int reps)
+{
+    require(reps >= 1 && reps <= 1000, "reps must be in 1..1000");
+    const size_t n = c.n, P = c.moduli.size();
+    Events ev;
+    std::string rep = "{\"logn\": " + std::to_string(c.logn) + ", \"ntt\": [";
+    bool first_entry = true;
+    auto time_ms = [&](auto &&fn) {
+        fn();  // warm
+        NX_HIP(hipEventRecord(ev.a, c.stream));
+        for (int r = 0; r < reps; r++) fn();
+        NX_HIP(hipEventRecord(ev.b, c.stream));
+        NX_HIP(hipEventSynchronize(ev.b));
+        float ms = 0;
+        NX_HIP(hipEventElapsedTime(&ms, ev.a, ev.b));
+        return (double)ms / reps;
+    };
+    if (c.logn >= 14) {
+        const uint32_t max_batch = 128;
+        // limbs per poly of a tuning batch: a power of two <= min(primes, 16)
+        uint32_t lp = 1;
+        while (lp * 2 <= std::min<size_t>(P, 16)) lp *= 2;
+        std::vector<uint64_t> h;
+        fill_random(c, h, max_batch, 7, lp);
+        DevBuf src(&c, max_batch * n), work(&c, max_batch * n);
+        NX_HIP(hipMemcpyAsync(src.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, c.stream));
+        NX_HIP(hipStreamSynchronize(c.stream));
+        const int cols[3] = {4, 8, 16}, rows[2] = {4, 8};
+        for (int fwd = 1; fwd >= 0; fwd--)
+            for (uint32_t batch = 1; batch <= max_batch; batch <<= 1) {
+                // batch = polys x limbs with limb i on prime i mod P
+                const uint32_t limbs = std::min(batch, lp);
+                NttBatch b;
+                b.data = work.get();
+                b.polys = batch / limbs;
+                b.limbs = limbs;
+                b.poly_stride = (uint64_t)limbs * n;
+                b.prime_by_poly = 0;
+                for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)(i % lp);
+                auto run = [&] {
+                    if (fwd)
+                        ntt_forward(c.tables, b, c.stream);
+                    else
+                        ntt_inverse(c.tables, b, c.stream);
+                };
+                auto reset = [&] {
+                    NX_HIP(hipMemcpyAsync(work.get(), src.get(), (size_t)batch * n * 8, hipMemcpyDeviceToDevice,
+                                          c.stream));
+                };
+                // reference output of the default shapes
+                ntt_set_tile(c.logn, 0, fwd != 0, batch, 0);
+                ntt_set_tile(c.logn, 1, fwd != 0, batch, 0);
+                reset();
+                run();
+                const std::vector<uint64_t> ref = download(c, work.get(), (size_t)batch * n);
+                const double t_def = time_ms(run);
+                double best = t_def;
+                int best_c = 0, best_r = 0;
+                for (int cw : cols)
+                    for (int rw : rows) {
+                        ntt_set_tile(c.logn, 0, fwd != 0, batch, cw);
+                        ntt_set_tile(c.logn, 1, fwd != 0, batch, rw);
+                        reset();
+                        run();
+                        if (download(c, work.get(), (size_t)batch * n) != ref) continue;  // never chosen
+                        const double t = time_ms(run);
+                        if (t < best * 0.98) {
+                            best = t;
+                            best_c = cw;
+                            best_r = rw;
+                        }
+                    }
+                ntt_set_tile(c.logn, 0, fwd != 0, batch, best_c);
+                ntt_set_tile(c.logn, 1, fwd != 0, batch, best_r);
+                char buf[200];
+                std::snprintf(buf, sizeof buf,
+                              "%s{\"dir\": \"%s\", \"batch\": %u, \"col\": %d, \"row\": %d, \"us\": %.2f, "
+                              "\"default_us\": %.2f}",
+                              first_entry ? "" : ", ", fwd ? "fwd" : "inv", batch, best_c, best_r, best * 1e3,
+                              t_def * 1e3);
+                rep += buf;
+                first_entry = false;
+            }
+    }
+    rep += "], \"dot_plain\": [";
+    first_entry = true;
+    {
+        const uint32_t terms = 16;
+        for (uint32_t limbs : {2u, 4u, 8u, 16u, 20u}) {
+            if (limbs > P) break;
+            std::vector<uint64_t> h;
+            std::vector<DevBuf> cts(terms), pts(terms);
+            std::vector<const uint64_t *> cp(terms), pp(terms);
+            for (uint32_t r = 0; r < terms; r++) {
+                cts[r].reset(&c, 2 * limbs * n);
+                pts[r].reset(&c, limbs * n);
+                fill_random(c, h, limbs, 100 + r, limbs);
+                // both polynomials of a ciphertext use the same limb primes
+                NX_HIP(hipMemcpyAsync(cts[r].get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, c.stream));
+                NX_HIP(hipMemcpyAsync(cts[r].get() + limbs * n, h.data(), h.size() * 8, hipMemcpyHostToDevice,
+                                      c.stream));
+                fill_random(c, h, limbs, 200 + r, limbs);
+                NX_HIP(hipMemcpyAsync(pts[r].get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, c.stream));
+                NX_HIP(hipStreamSynchronize(c.stream));
+                cp[r] = cts[r].get();
+                pp[r] = pts[r].get();
+            }
+            DevBuf out(&c, 2 * limbs * n);
+            auto run = [&] { poly_dot_plain(c.tables, out.get(), cp.data(), pp.data(), terms, limbs, c.stream); };
+            dot_plain_set_layout(limbs, 0);
+            run();
+            const std::vector<uint64_t> ref = download(c, out.get(), 2 * limbs * n);
+            const double t0 = time_ms(run);
+            dot_plain_set_layout(limbs, 1);
+            run();
+            const bool same = download(c, out.get(), 2 * limbs * n) == ref;
+            const double t1 = same ? time_ms(run) : 1e30;
+            const int choice = (same && t1 < t0 * 0.98) ? 1 : 0;
+            dot_plain_set_layout(limbs, choice);
+            char buf[200];
+            std::snprintf(buf, sizeof buf, "%s{\"limbs\": %u, \"layout\": %d, \"us\": [%.2f, %.2f]}",
+                          first_entry ? "" : ", ", limbs, choice, t0 * 1e3, same ? t1 * 1e3 : -1.0);
+            rep += buf;
+            first_entry = false;
+        }
+    }
+    rep += "]}";
+    NX_HIP(hipStreamSynchronize(c.stream));
+    return rep;
+}
+
+}  // namespace nx

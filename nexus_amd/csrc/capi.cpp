@@ -151,6 +151,19 @@ int nx_context_synchronize(nx_context *ctx)
     });
 }
 
+int nx_autotune(nx_context *ctx, int reps, char *report, size_t report_len, size_t *needed)
+{
+    return guard([&] {
+        need(ctx, "ctx");
+        const std::string r = nx::autotune(ctx->impl, reps);
+        if (needed) *needed = r.size() + 1;
+        if (report && report_len) {
+            std::strncpy(report, r.c_str(), report_len - 1);
+            report[report_len - 1] = 0;
+        }
+    });
+}
+
 int nx_galois_elt_from_step(const nx_context *ctx, int step, uint32_t *elt)
 {
     return guard([&] {

@@ -289,6 +289,9 @@ void load_plaintext(Context &c, const uint8_t *data, size_t n, Plaintext &out);
 std::vector<uint8_t> save_kswitch_keys(const KSwitchKeys &keys, bool galois);
 void load_kswitch_keys(Context &c, const uint8_t *data, size_t n, bool galois, KSwitchKeys &out);
 
+// launch-shape autotuner -- autotune.cpp; returns a JSON report
+std::string autotune(Context &c, int reps);
+
 // encoder (device FFT and NTT) -- encoder.cpp
 void upload_encoder_tables(Context &c);
 void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,

@@ -39,6 +39,11 @@ struct NttBatch {
 void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s);
 // Inverse: bit-reversed in, natural order out, multiplied by n^-1, [0, q).
 void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s);
+// Tile width of the two-pass NTT (n >= 2^14) for a pass (0 = column, 1 = row),
+// a direction and the power-of-two bucket of a batch size: 4, 8 or 16 (rows:
+// 4 or 8), 0 = the default rule.  Process-wide; set by the autotuner.
+void ntt_set_tile(int logn, int pass, bool fwd, uint32_t batch, int width);
+int ntt_get_tile(int logn, int pass, bool fwd, uint32_t batch);
 
 // ---------------------------------------------------------------- dyadic ops
 // All pointers address [polys][limbs][n] arrays; limb i uses prime i.
@@ -117,6 +122,11 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
 constexpr int kMaxDot = 64;
 void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const *cts, const uint64_t *const *pts,
                     uint32_t count, uint32_t limbs, hipStream_t s);
+// Layout of k_dot_plain for launches of at most `limbs` limbs: 0 = default
+// (two coefficients of both polynomials per thread), 1 = one coefficient of
+// one polynomial per thread (four times the threads).  Set by the autotuner.
+void dot_plain_set_layout(uint32_t limbs, int layout);
+int dot_plain_get_layout(uint32_t limbs);
 
 // Hoisted variants over one shared ModUp T[nout][l][n] (B = 1): rotation r
 // reads digit values through the NTT-domain Galois permutation perms[r]

@@ -5,6 +5,8 @@
 // output primes: the per-input Shoup product [x Dhat^-1]_q is computed once
 // per chunk instead of once per output prime.  Conversion constants are
 // uniform across a wavefront (scalar loads).
+#include <atomic>
+
 #include "kernels.h"
 #include "prof.h"
 
@@ -151,7 +153,47 @@ __global__ __launch_bounds__(kBlock) void k_dot_plain(uint64_t *out, DotArgs arg
     *reinterpret_cast<ulonglong2 *>(out + off1) = o1;
 }
 
+// One coefficient of one polynomial per thread (grid y over poly x limb): four
+// times the threads of k_dot_plain, for launches with few limbs.
+__global__ __launch_bounds__(kBlock) void k_dot_plain1(uint64_t *out, DotArgs args, uint32_t count,
+                                                       const ModConst *mod, uint32_t limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // k * limbs + limb
+    const uint32_t limb = row % limbs;
+    const ModConst m = mod[limb];
+    const uint64_t off = (uint64_t)row * n + c, poff = (uint64_t)limb * n + c;
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t r = 0; r < count; r++) {
+        const uint64_t x = args.ct[r][off], y = args.pt[r][poff];
+        const uint64_t plo = x * y, phi = mulhi64(x, y);
+        lo += plo;
+        hi += phi + (lo < plo);
+        if ((r & 31) == 31) {
+            lo = barrett128(hi, lo, m);
+            hi = 0;
+        }
+    }
+    out[off] = barrett128(hi, lo, m);
+}
+
+// layout per limb-count bucket (<= 2, <= 4, <= 8, <= 16, more), 0 = default
+std::atomic<int> g_dot_layout[5];
+
+int dot_bucket(uint32_t limbs)
+{
+    return limbs <= 2 ? 0 : limbs <= 4 ? 1 : limbs <= 8 ? 2 : limbs <= 16 ? 3 : 4;
+}
+
 }  // namespace
+
+void dot_plain_set_layout(uint32_t limbs, int layout)
+{
+    if (layout == 0 || layout == 1) g_dot_layout[dot_bucket(limbs)].store(layout);
+}
+
+int dot_plain_get_layout(uint32_t limbs) { return g_dot_layout[dot_bucket(limbs)].load(); }
 
 bool ks_conv_fused_ok(uint32_t alpha) { return alpha <= kRegAlpha; }
 
@@ -189,9 +231,12 @@ void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const 
         a.pt[i] = i < count ? pts[i] : nullptr;
     }
     ProfScope prof("dot_plain", s, 8.0 * n * limbs * (3.0 * count + 2.0));
-    // two coefficients per thread (n is a power of two >= 2)
-    hipLaunchKernelGGL(k_dot_plain, dim3((n / 2 + kBlock - 1) / kBlock, limbs), dim3(kBlock), 0, s, out, a, count,
-                       t.mod, limbs, n);
+    if (g_dot_layout[dot_bucket(limbs)].load() == 1)
+        hipLaunchKernelGGL(k_dot_plain1, dim3((n + kBlock - 1) / kBlock, 2 * limbs), dim3(kBlock), 0, s, out, a,
+                           count, t.mod, limbs, n);
+    else  // two coefficients per thread (n is a power of two >= 2)
+        hipLaunchKernelGGL(k_dot_plain, dim3((n / 2 + kBlock - 1) / kBlock, limbs), dim3(kBlock), 0, s, out, a,
+                           count, t.mod, limbs, n);
 }
 
 }  // namespace nx

@@ -16,6 +16,8 @@
 // rows) through LDS and runs radix-8/4/2 register butterflies between
 // barriers, 256 threads (four wave64) per workgroup.  Row twiddles for row r at
 // local stage s are the contiguous block fwd[2^s (N1 + r) ...].
+#include <atomic>
+
 #include "kernels.h"
 #include "prof.h"
 
@@ -39,6 +41,7 @@ struct NttKArgs {
 };
 
 constexpr int kThreads = 256;
+constexpr int kTuneLogNMin = 14, kTuneLogN = 4, kTuneBuckets = 9;
 
 // CW: sub-transforms per workgroup -- columns of the column pass (16 for large
 // batches, 8 for small ones: more workgroups when there are few limbs), and an
@@ -291,12 +294,28 @@ void launch_whole(const NttKArgs &a, uint32_t batch, hipStream_t s)
     }
 }
 
-// Column tiles of 8 below 128 limbs (twice the workgroups), 16 otherwise;
-// narrower tiles for still smaller batches measured no faster.
+// Tile widths chosen by the autotuner: [log n - 14][pass][direction][batch bucket],
+// 0 = the default rule (column tiles of 8 below 128 limbs, 16 otherwise; row
+// tiles of 8).  Written by ntt_set_tile, read at every launch.
+std::atomic<int> g_tile[kTuneLogN][2][2][kTuneBuckets];
+
+int tune_bucket(uint32_t batch)
+{
+    int b = 0;
+    while (b + 1 < kTuneBuckets && (2u << b) <= batch) b++;
+    return b;  // floor(log2(batch)), capped
+}
+
 template <int LOGS, int ROLE, bool FWD>
 void launch_sized(const NttKArgs &a, uint32_t batch, hipStream_t s)
 {
-    if (ROLE == kCol && batch < 128)
+    int cw = 0;
+    const int li = (int)a.logn - kTuneLogNMin;
+    if (li >= 0 && li < kTuneLogN) cw = g_tile[li][ROLE == kCol ? 0 : 1][FWD ? 1 : 0][tune_bucket(batch)].load();
+    if (!cw) cw = ROLE == kCol ? (batch < 128 ? 8 : 16) : 8;
+    if (cw == 4)
+        launch_one<LOGS, ROLE, FWD, 4>(a, batch, s);
+    else if (cw == 8)
         launch_one<LOGS, ROLE, FWD, 8>(a, batch, s);
     else
         launch_one<LOGS, ROLE, FWD, 16>(a, batch, s);
@@ -336,6 +355,21 @@ static inline void split_logn(int logn, int &l1, int &l2)
 {
     l1 = logn / 2;
     l2 = logn - l1;
+}
+
+void ntt_set_tile(int logn, int pass, bool fwd, uint32_t batch, int width)
+{
+    const int li = logn - kTuneLogNMin;
+    if (li < 0 || li >= kTuneLogN || pass < 0 || pass > 1) return;
+    if (width != 0 && width != 4 && width != 8 && width != 16) return;
+    g_tile[li][pass][fwd ? 1 : 0][tune_bucket(batch)].store(width);
+}
+
+int ntt_get_tile(int logn, int pass, bool fwd, uint32_t batch)
+{
+    const int li = logn - kTuneLogNMin;
+    if (li < 0 || li >= kTuneLogN || pass < 0 || pass > 1) return 0;
+    return g_tile[li][pass][fwd ? 1 : 0][tune_bucket(batch)].load();
 }
 
 void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
