@@ -221,6 +221,9 @@ public:
     void rotate_vector_hoisted(const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &keys,
                                std::vector<Ciphertext> &dest) const;
     void hoisting_inplace(Ciphertext &ct, const GaloisKeys &keys, const std::vector<int> &steps) const;
+    // sum_i rot(cts[i], steps[i]) with a single ModDown (step 0 = the input)
+    void rotate_sum(const std::vector<const Ciphertext *> &cts, const std::vector<int> &steps, const GaloisKeys &keys,
+                    Ciphertext &dest) const;
     void complex_conjugate(const Ciphertext &ct, const GaloisKeys &keys, Ciphertext &dest) const;
     void complex_conjugate_inplace(Ciphertext &ct, const GaloisKeys &keys) const;
 

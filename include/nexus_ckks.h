@@ -177,6 +177,13 @@ int nx_rotate_hoisted(const nx_ciphertext *a, const int *steps, size_t count, co
                       nx_ciphertext *const *dests);
 int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx_kswitch_keys *galois_keys,
                 nx_ciphertext *dest);
+/* dest = sum_i rot(in[i], steps[i]) over distinct inputs of one level and scale
+ * (step 0 = the input itself) with a single ModDown: each input's ModUp feeds
+ * one accumulator in the extended basis (the BSGS giant-step sum of the
+ * reference's linear transforms, Bootstrapper.cu:1855-1919, without a ModDown
+ * per term).  Decrypts like the sum of nx_rotate_vector results. */
+int nx_rotate_sum(const nx_ciphertext *const *in, const int *steps, size_t count,
+                  const nx_kswitch_keys *galois_keys, nx_ciphertext *dest);
 /* dest = sum_r cts[r] * pts[r] (size-2 ciphertexts of one level, NTT form) in one
  * pass: the per-giant-step accumulation of the reference's BSGS transforms
  * (cuda/src/bootstrapping/Bootstrapper.cu:1855-1919, multiply_vector_reduced_error

@@ -533,6 +533,17 @@ _sig("nx_hoisting", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, vp])
 _sig("nx_multiply_plain_accumulate", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp])
 
 
+_sig("nx_rotate_sum", [ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int), sz, vp, vp])
+
+
+def rotate_sum(cts, steps, gk):
+    """sum_i rot(cts[i], steps[i]) with a single ModDown"""
+    out = Ciphertext(cts[0].ctx)
+    st = (ctypes.c_int * len(steps))(*steps)
+    _check(_lib.nx_rotate_sum((vp * len(cts))(*[c.h for c in cts]), st, len(cts), gk.h, out.h))
+    return out
+
+
 def multiply_plain_accumulate(cts, pts, dest=None):
     """sum_r cts[r] * pts[r] in one pass."""
     d = _out(cts[0].ctx, dest)

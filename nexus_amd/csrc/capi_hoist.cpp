@@ -35,6 +35,26 @@ int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx
     });
 }
 
+int nx_rotate_sum(const nx_ciphertext *const *in, const int *steps, size_t count,
+                  const nx_kswitch_keys *galois_keys, nx_ciphertext *dest)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(in, "in");
+        nx::need_ptr(steps, "steps");
+        nx::need_ptr(galois_keys, "galois_keys");
+        nx::need_ptr(dest, "dest");
+        std::vector<const nx::Ciphertext *> v;
+        for (size_t i = 0; i < count; i++) {
+            nx::need_ptr(in[i], "in[i]");
+            v.push_back(&in[i]->impl);
+        }
+        std::vector<int> st(steps, steps + count);
+        nx::Ciphertext out;
+        nx::op_rotate_sum(v, st, galois_keys->impl, out);
+        dest->impl = std::move(out);
+    });
+}
+
 int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
                                  nx_ciphertext *dest)
 {

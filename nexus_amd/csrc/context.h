@@ -256,6 +256,8 @@ void op_mod_raise(const Ciphertext &a, Ciphertext &dst);
 void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk,
                        std::vector<Ciphertext> &dst);
 void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk, Ciphertext &dst);
+void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<int> &steps, const KSwitchKeys &gk,
+                   Ciphertext &dst);
 
 // key switch of `target` (limbs x n, NTT) under key `key` added into ct (2 polys)
 void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key);

@@ -174,7 +174,10 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
     }
     std::vector<Ciphertext> out(d.n_out);
     for (int o = 0; o < d.n_out; o++) {
-        bool have = false;
+        // per giant step a multiply-accumulate; the giant rotations summed
+        // with a single ModDown
+        std::vector<Ciphertext> accs;
+        std::vector<int> gsteps;
         for (int a = d.a_min; a <= d.a_max; a++) {
             std::vector<const Ciphertext *> terms;
             std::vector<const Plaintext *> weights;
@@ -188,15 +191,13 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
             if (terms.empty()) continue;
             Ciphertext acc;
             ev.multiply_plain_accumulate(terms, weights, acc);
-            if (a) ev.rotate_vector_inplace(acc, B * a, *ck_.galois_keys);
-            if (!have) {
-                out[o] = std::move(acc);
-                have = true;
-            } else {
-                ev.add_inplace(out[o], acc);
-            }
+            accs.push_back(std::move(acc));
+            gsteps.push_back(B * a);
         }
-        if (!have) throw std::logic_error("BertLayer: all-zero weight block");
+        if (accs.empty()) throw std::logic_error("BertLayer: all-zero weight block");
+        std::vector<const Ciphertext *> ap;
+        for (const auto &a : accs) ap.push_back(&a);
+        ev.rotate_sum(ap, gsteps, *ck_.galois_keys, out[o]);
         ev.rescale_to_next_inplace(out[o]);
     }
     return out;
@@ -459,12 +460,15 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             kr.clear();
             const size_t lp = P[0].coeff_modulus_size();
             const double ms = (double)prime(lp);
-            // S = sum_d rot(Z_d, d), Z_d = sum_tau A_(tau + d) (.) P_tau, d = 8 a + b
+            // S = sum_d rot(Z_d, d), Z_d = sum_tau A_(tau + d) (.) P_tau, d = 8 a + b:
+            // Y_a = sum_b rot(Z_(8a+b), b), S = sum_a rot(Y_a, 8 a), each sum of
+            // rotations with a single ModDown
             const int dmin = -(tc - 1), dmax = kHeadDim - 1;
-            bool have = false;
+            std::vector<Ciphertext> Ys;
+            std::vector<int> ysteps;
             for (int a = floor_div(dmin, kOffBaby); a <= floor_div(dmax, kOffBaby); a++) {
-                Ciphertext Y;
-                bool haveY = false;
+                std::vector<Ciphertext> Zs;
+                std::vector<int> zsteps;
                 for (int b = 0; b < kOffBaby; b++) {
                     const int d = kOffBaby * a + b;
                     if (d < dmin || d > dmax) continue;
@@ -477,23 +481,20 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
                     if (terms.empty()) continue;
                     Ciphertext Z;
                     ev.multiply_plain_accumulate(terms, weights, Z);
-                    if (b) ev.rotate_vector_inplace(Z, b, gk);
-                    if (!haveY) {
-                        Y = std::move(Z);
-                        haveY = true;
-                    } else {
-                        ev.add_inplace(Y, Z);
-                    }
+                    Zs.push_back(std::move(Z));
+                    zsteps.push_back(b);
                 }
-                if (!haveY) continue;
-                if (a) ev.rotate_vector_inplace(Y, kOffBaby * a, gk);
-                if (!have) {
-                    S[eta] = std::move(Y);
-                    have = true;
-                } else {
-                    ev.add_inplace(S[eta], Y);
-                }
+                if (Zs.empty()) continue;
+                std::vector<const Ciphertext *> zp;
+                for (const auto &z : Zs) zp.push_back(&z);
+                Ciphertext Y;
+                ev.rotate_sum(zp, zsteps, gk, Y);
+                Ys.push_back(std::move(Y));
+                ysteps.push_back(kOffBaby * a);
             }
+            std::vector<const Ciphertext *> yp;
+            for (const auto &y : Ys) yp.push_back(&y);
+            ev.rotate_sum(yp, ysteps, gk, S[eta]);
             ev.rescale_to_next_inplace(S[eta]);
         }
         // ---- softmax over t for every (query, head): exp, key mask, sum, 1 / sum

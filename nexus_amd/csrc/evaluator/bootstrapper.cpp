@@ -566,8 +566,10 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
     ck_.evaluator.rotate_vector_hoisted(x, bsteps, *ck_.galois_keys, rots);
     std::map<int, Ciphertext> baby;
     for (size_t k = 0; k < bi.size(); k++) baby.emplace(bi[k], std::move(rots[k]));
-    Ciphertext result;
-    bool have_result = false;
+    // giant steps: one multiply-accumulate each, then their rotations summed
+    // with a single ModDown
+    std::vector<Ciphertext> accs;
+    std::vector<int> gsteps;
     auto it = lv.diag.begin();
     while (it != lv.diag.end()) {
         const int j = it->first.first;
@@ -588,14 +590,13 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
         }
         Ciphertext acc;
         ck_.evaluator.multiply_plain_accumulate(terms, weights, acc);
-        if (j != 0) ck_.evaluator.rotate_vector_inplace(acc, j * lv.g * lv.stride, *ck_.galois_keys);
-        if (!have_result) {
-            result = std::move(acc);
-            have_result = true;
-        } else {
-            ck_.evaluator.add_inplace(result, acc);
-        }
+        accs.push_back(std::move(acc));
+        gsteps.push_back(j * lv.g * lv.stride);
     }
+    std::vector<const Ciphertext *> ap;
+    for (const auto &a : accs) ap.push_back(&a);
+    Ciphertext result;
+    ck_.evaluator.rotate_sum(ap, gsteps, *ck_.galois_keys, result);
     ck_.evaluator.rescale_to_next_inplace(result);
     result.set_scale(out_scale);
     return result;

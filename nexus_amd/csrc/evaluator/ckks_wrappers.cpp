@@ -425,6 +425,16 @@ void Evaluator::hoisting_inplace(Ciphertext &ct, const GaloisKeys &k, const std:
 {
     check(nx_hoisting(ct.get(), steps.data(), steps.size(), k.get(), ct.get()));
 }
+void Evaluator::rotate_sum(const std::vector<const Ciphertext *> &cts, const std::vector<int> &steps,
+                           const GaloisKeys &k, Ciphertext &dest) const
+{
+    if (cts.empty() || cts.size() != steps.size()) throw std::invalid_argument("rotate_sum: inputs and steps must match");
+    std::vector<const nx_ciphertext *> h;
+    for (const Ciphertext *c : cts) h.push_back(c->get());
+    Ciphertext out(*ctx_);
+    check(nx_rotate_sum(h.data(), steps.data(), h.size(), k.get(), out.get()));
+    dest = std::move(out);
+}
 void Evaluator::complex_conjugate(const Ciphertext &ct, const GaloisKeys &k, Ciphertext &dest) const
 {
     OUT(dest);

@@ -132,4 +132,58 @@ void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwit
     dst = std::move(out);
 }
 
+void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<int> &steps, const KSwitchKeys &gk,
+                   Ciphertext &dst)
+{
+    // sum_i rot(in_i, steps_i) with one ModDown: each input's ModUp feeds the
+    // inner product with its rotation key into one accumulator in the
+    // extended basis (the ModDown is linear), the sigma_i(c0_i) are summed in
+    // the data basis, and the accumulator is brought down once.  The result
+    // decrypts like the sum of the separate rotations with one rounding
+    // instead of one per term.
+    require(!in.empty() && in.size() == steps.size(), "inputs and steps must match");
+    const Ciphertext &a0 = *in[0];
+    check_input(a0, gk);
+    Context &c = *a0.ctx;
+    const size_t n = c.n, l = a0.limbs, nout = l + c.alpha, pw = l * n;
+    for (const Ciphertext *a : in) {
+        check_input(*a, gk);
+        require(a->ctx == &c && a->limbs == l, "inputs must share the level");
+        require(are_close(a->scale, a0.scale), "inputs must share the scale");
+    }
+    Ciphertext out;
+    out.ctx = &c;
+    out.size = 2;
+    out.limbs = l;
+    out.scale = a0.scale;
+    out.ntt = true;
+    out.data.reset(&c, 2 * pw);
+    {
+        std::vector<uint64_t> zeros(l, 0);
+        poly_fill_limbs(out.data.get(), zeros.data(), 2, (uint32_t)l, c.logn, c.stream);
+    }
+    uint8_t outp[kMaxPrimes];
+    ks_out_primes(c, l, outp);
+    DevBuf acc, t(&c, pw);
+    bool have_acc = false;
+    for (size_t i = 0; i < in.size(); i++) {
+        const Ciphertext &a = *in[i];
+        if (steps[i] == 0) {
+            poly_add(c.tables, out.data.get(), out.data.get(), a.data.get(), 2, (uint32_t)l, c.stream);
+            continue;
+        }
+        Prepared p = prepare(c, std::vector<int>{steps[i]}, gk);
+        poly_permute(t.get(), a.data.get(), p.perms[0], 1, (uint32_t)l, c.logn, c.stream);
+        poly_add(c.tables, out.data.get(), out.data.get(), t.get(), 1, (uint32_t)l, c.stream);
+        DevBuf T = ks_modup(c, a.data.get() + pw, 1, l);
+        if (!have_acc) acc.reset(&c, 2 * nout * n);
+        ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), p.keys.data(), p.perms.data(), 1,
+                                 (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout, have_acc,
+                                 c.stream);
+        have_acc = true;
+    }
+    if (have_acc) ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
+    dst = std::move(out);
+}
+
 }  // namespace nx

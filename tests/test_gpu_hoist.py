@@ -106,3 +106,29 @@ def test_multiply_plain_accumulate_bit_exact(env, count):
         ref = nx.add(ref, nx.multiply_plain(c, p))
     assert got.scale == ref.scale
     assert np.array_equal(got.to_array(), ref.to_array())
+
+
+def test_rotate_sum_matches_separate_rotations():
+    """sum_i rot(ct_i, s_i) with one ModDown decrypts like the sum of separate
+    rotations (identity steps and repeated steps included)"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    logn = 13
+    ctx = nx.Context(logn, [60, 40, 40, 60], special_primes=1)
+    sk = nx.SecretKey(ctx, seed=61)
+    steps = [0, 1, 3, -5, 1, 64]
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in set(steps) if s], seed=62)
+    rng = np.random.default_rng(63)
+    vals = [rng.uniform(-1, 1, ctx.slot_count) for _ in steps]
+    cts = [nx.encrypt(ctx, sk, nx.encode(ctx, v, 2.0 ** 40), seed=70 + i) for i, v in enumerate(vals)]
+    got = nx.decode(ctx, nx.decrypt(ctx, sk, nx.rotate_sum(cts, steps, gk)))
+    want = sum(np.roll(v, -s) for v, s in zip(vals, steps))
+    sep = cts[0] if steps[0] == 0 else nx.rotate_vector(cts[0], steps[0], gk)
+    for c, s in zip(cts[1:], steps[1:]):
+        sep = nx.add(sep, c if s == 0 else nx.rotate_vector(c, s, gk))
+    ref = nx.decode(ctx, nx.decrypt(ctx, sk, sep))
+    print("rotate_sum: max err %.3g, separate %.3g" % (np.max(np.abs(got - want)), np.max(np.abs(ref - want))))
+    # six fresh encryptions at scale 2^40 carry ~1e-6 of noise; one ModDown
+    # instead of five must not add to it
+    assert np.max(np.abs(got - want)) < 1e-5
+    assert np.max(np.abs(got - want)) <= 2 * np.max(np.abs(ref - want)) + 1e-9
