@@ -186,3 +186,42 @@ def test_server_continuous_batching(env):
               % (k, n, e_model, e_solo, np.max(np.abs(got[n:]))))
         assert e_model < 2.0 ** -10 and e_solo < 2.0 ** -10
         assert np.max(np.abs(got[n:])) < 2.0 ** -10
+
+
+def test_bert_base_full_size():
+    """BERT-base (hidden 768, 12 heads, FFN 3072) at N = 2^16 with 128-token
+    sets, chain {51, 46 x 19, 51 x 14, 58 x 8}: a 40-token sequence and two
+    packed sequences against the FP64 model within 2^-10"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    _note("bert-base full size keys")
+    chain = [51] + [46] * 19 + [51] * 14 + [58] * 8
+    ctx = nx.Context(16, chain, hamming_weight=192, special_primes=8)
+    sk = nx.SecretKey(ctx, seed=81)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=82)
+    steps = sorted(set(nx.bert_rotation_steps(ctx.slot_count)) | set(nx.boot_plan_steps(15, 15, 25)))
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps], seed=83)
+    ev = nx.CKKSEvaluator(ctx, rk, gk, SCALE, sk)
+    bt = nx.Bootstrapper(ev)
+    W = ob.random_weights(768, 3072, seed=84)
+    layer = nx.BertLayer(bt, W)
+    rng = np.random.default_rng(85)
+    x = np.zeros((128, 768))
+    x[:40] = rng.normal(0, 1, (40, 768))
+    x[40:100] = rng.normal(0, 1, (60, 768))
+    enc = [nx.encrypt(ctx, sk, nx.encode(ctx, v, SCALE, ctx.L - 1), seed=90 + i)
+           for i, v in enumerate(nx.bert_pack(x, ctx.slot_count))]
+    dec = lambda cts: nx.bert_unpack([nx.decode(ctx, nx.decrypt(ctx, sk, c)) for c in cts], 128)
+    _note("bert-base full size: one sequence of 40")
+    single = [nx.encrypt(ctx, sk, nx.encode(ctx, v, SCALE, ctx.L - 1), seed=95 + i)
+              for i, v in enumerate(nx.bert_pack(np.vstack([x[:40], np.zeros((88, 768))]), ctx.slot_count))]
+    got = dec(layer.forward(single, 40))[:40]
+    err = np.max(np.abs(got - ob.layer_approx(x[:40], W)))
+    print("bert-base s=40: max err vs model %.3g" % err)
+    assert err < 2.0 ** -10
+    _note("bert-base full size: packed 40 + 60")
+    got = dec(layer.forward(enc, segments=[(0, 40), (40, 60)]))
+    for a, n in ((0, 40), (40, 60)):
+        e = np.max(np.abs(got[a:a + n] - ob.layer_approx(x[a:a + n], W)))
+        print("bert-base packed (%d, %d): max err vs model %.3g" % (a, n, e))
+        assert e < 2.0 ** -10
