@@ -438,24 +438,39 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
     const GaloisKeys &gk = *ck_.galois_keys;
     const int T = (int)T_;
     const int halves = (T + kHeadDim - 1) / kHeadDim;
-    // probabilities of every ciphertext and half, refreshed together
+    // token offsets t = 64 eta + t' that can pair two rows of one segment:
+    // t < lmax or t > T - lmax (cyclically); half eta computes t' in [lo, hi)
+    int lmax = 0;
+    for (const Segment &g : segments) lmax = std::max(lmax, g.len);
+    std::vector<int> tlo(halves), thi(halves);
+    for (int eta = 0; eta < halves; eta++) {
+        const int t0 = kHeadDim * eta, full = std::min(kHeadDim, T - t0);
+        const int hi1 = std::max(0, std::min(full, lmax - t0));
+        const int lo2 = std::max(0, std::min(full, T - lmax + 1 - t0));
+        const bool a1 = hi1 > 0, a2 = lo2 < full;
+        tlo[eta] = a1 ? 0 : (a2 ? lo2 : 0);
+        thi[eta] = a2 ? full : (a1 ? hi1 : 0);
+    }
+    std::vector<int> active;
+    for (int eta = 0; eta < halves; eta++)
+        if (thi[eta] > tlo[eta]) active.push_back(eta);
+    // probabilities of every ciphertext and active half, refreshed together
     std::vector<std::vector<Ciphertext>> Pe(n, std::vector<Ciphertext>(halves));
-    std::vector<int> tcount(halves);
-    for (int eta = 0; eta < halves; eta++) tcount[eta] = std::min(kHeadDim, T - kHeadDim * eta);
     for (size_t c = 0; c < n; c++) {
         Ciphertext qc = q[c], kc = k[c];
         if (kc.coeff_modulus_size() > qc.coeff_modulus_size()) ev.mod_switch_to_inplace(kc, qc.params_id());
         if (qc.coeff_modulus_size() > kc.coeff_modulus_size()) ev.mod_switch_to_inplace(qc, kc.params_id());
         // ---- scores, compacted: S^eta[256 i + 64 h + t'] = q_i . k_(i + 64 eta + t') on head h
         std::vector<Ciphertext> S(halves);
-        for (int eta = 0; eta < halves; eta++) {
-            const int t0 = kHeadDim * eta, tc = tcount[eta];
-            std::vector<Ciphertext> kr = token_rotations(kc, t0, tc);
-            std::vector<Ciphertext> P(tc);
-            for (int tau = 0; tau < tc; tau++) {
-                ev.multiply(qc, kr[tau], P[tau]);
-                ev.relinearize_inplace(P[tau], rk);
-                ev.rescale_to_next_inplace(P[tau]);
+        for (int eta : active) {
+            const int t0 = kHeadDim * eta, lo = tlo[eta], hi = thi[eta];
+            std::vector<Ciphertext> kr = token_rotations(kc, t0 + lo, hi - lo);
+            std::vector<Ciphertext> P(hi - lo);  // P[tau - lo]
+            for (int tau = lo; tau < hi; tau++) {
+                Ciphertext &pt = P[tau - lo];
+                ev.multiply(qc, kr[tau - lo], pt);
+                ev.relinearize_inplace(pt, rk);
+                ev.rescale_to_next_inplace(pt);
             }
             kr.clear();
             const size_t lp = P[0].coeff_modulus_size();
@@ -463,7 +478,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             // S = sum_d rot(Z_d, d), Z_d = sum_tau A_(tau + d) (.) P_tau, d = 8 a + b:
             // Y_a = sum_b rot(Z_(8a+b), b), S = sum_a rot(Y_a, 8 a), each sum of
             // rotations with a single ModDown
-            const int dmin = -(tc - 1), dmax = kHeadDim - 1;
+            const int dmin = -(hi - 1), dmax = kHeadDim - 1 - lo;
             std::vector<Ciphertext> Ys;
             std::vector<int> ysteps;
             for (int a = floor_div(dmin, kOffBaby); a <= floor_div(dmax, kOffBaby); a++) {
@@ -474,8 +489,8 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
                     if (d < dmin || d > dmax) continue;
                     std::vector<const Ciphertext *> terms;
                     std::vector<const Plaintext *> weights;
-                    for (int tau = std::max(0, -d); tau < tc && tau + d < kHeadDim; tau++) {
-                        terms.push_back(&P[tau]);
+                    for (int tau = std::max(lo, -d); tau < hi && tau + d < kHeadDim; tau++) {
+                        terms.push_back(&P[tau - lo]);
                         weights.push_back(&offset_mask(tau + d, lp, ms));
                     }
                     if (terms.empty()) continue;
@@ -499,13 +514,20 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
         }
         // ---- softmax over t for every (query, head): exp, key mask, sum, 1 / sum
         std::vector<Ciphertext> E(halves);
-        for (int eta = 0; eta < halves; eta++) {
+        Ciphertext tot;
+        bool have_tot = false;
+        for (int eta : active) {
             E[eta] = ck_.exp(S[eta]);
-            ev.multiply_plain_inplace(E[eta], key_mask(eta, tcount[eta], seg, E[eta].coeff_modulus_size()));
+            const int full = std::min(kHeadDim, T - kHeadDim * eta);
+            ev.multiply_plain_inplace(E[eta], key_mask(eta, full, seg, E[eta].coeff_modulus_size()));
             ev.rescale_to_next_inplace(E[eta]);
+            if (!have_tot) {
+                tot = E[eta];
+                have_tot = true;
+            } else {
+                ev.add_inplace(tot, E[eta]);
+            }
         }
-        Ciphertext tot = E[0];
-        for (int eta = 1; eta < halves; eta++) ev.add_inplace(tot, E[eta]);
         for (int st = 1; st < kHeadDim; st <<= 1) {
             Ciphertext r;
             ev.rotate_vector(tot, st, gk, r);
@@ -524,7 +546,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
         ev.multiply_plain_inplace(inv, row_mask(1, seg, segments, inv.coeff_modulus_size(),
                                                 (double)prime(inv.coeff_modulus_size()), 1));
         ev.rescale_to_next_inplace(inv);
-        for (int eta = 0; eta < halves; eta++) {
+        for (int eta : active) {
             Ciphertext &p = Pe[c][eta];
             p = std::move(E[eta]);
             ev.mod_switch_to_inplace(p, inv.params_id());
@@ -536,7 +558,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
     {
         std::vector<Ciphertext *> all;
         for (auto &row : Pe)
-            for (auto &p : row) all.push_back(&p);
+            for (int eta : active) all.push_back(&row[eta]);
         bootstrap_real(all, 1.0);
     }
     // ---- expansion over the 64 features of each head; times the rotated v
@@ -544,28 +566,29 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
     for (size_t c = 0; c < n; c++) {
         Ciphertext acc;
         bool have_acc = false;
-        for (int eta = 0; eta < halves; eta++) {
-            const int t0 = kHeadDim * eta, tc = tcount[eta];
+        for (int eta : active) {
+            const int t0 = kHeadDim * eta, lo = tlo[eta], hi = thi[eta];
             const Ciphertext &Pb = Pe[c][eta];
             // Pb_tau[256 i + 64 h + f] = P[256 i + 64 h + tau] = sum_d A_(tau - d) (.) rot(P, d)
-            std::vector<Ciphertext> R = offset_rotations(Pb, -(kHeadDim - 1), tc - 1);
+            const int dlo = lo - (kHeadDim - 1);
+            std::vector<Ciphertext> R = offset_rotations(Pb, dlo, hi - 1);
             const size_t lb = Pb.coeff_modulus_size();
             const double ms = (double)prime(lb);
             Ciphertext vlow = v[c];
             if (vlow.coeff_modulus_size() > lb - 1) ev.mod_switch_to_inplace(vlow, chain_of(lb - 1));
-            std::vector<Ciphertext> vr = token_rotations(vlow, t0, tc);
-            for (int tau = 0; tau < tc; tau++) {
+            std::vector<Ciphertext> vr = token_rotations(vlow, t0 + lo, hi - lo);
+            for (int tau = lo; tau < hi; tau++) {
                 std::vector<const Ciphertext *> terms;
                 std::vector<const Plaintext *> weights;
                 for (int d = tau - (kHeadDim - 1); d <= tau; d++) {
-                    terms.push_back(&R[d + (kHeadDim - 1)]);
+                    terms.push_back(&R[d - dlo]);
                     weights.push_back(&offset_mask(tau - d, lb, ms));
                 }
                 Ciphertext pb;
                 ev.multiply_plain_accumulate(terms, weights, pb);
                 ev.rescale_to_next_inplace(pb);
                 Ciphertext prod;
-                ev.multiply(pb, vr[tau], prod);
+                ev.multiply(pb, vr[tau - lo], prod);
                 if (!have_acc) {
                     acc = std::move(prod);
                     have_acc = true;
