@@ -212,6 +212,12 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
 
     tables.logn = logn;
     tables.num_primes = (int)P;
+    tables.max_bits = 0;
+    for (uint64_t q : moduli) {
+        int b = 0;
+        while (b < 64 && (q >> b)) b++;
+        tables.max_bits = std::max(tables.max_bits, b);
+    }
     tables.fwd = d_fwd;
     tables.inv = d_inv;
     tables.ninv = d_ninv;

@@ -14,6 +14,7 @@ constexpr int kMaxPrimes = 64;
 struct DeviceTables {
     int logn;
     int num_primes;
+    int max_bits = 64;        // bit size of the largest prime of the context
     const ulonglong2 *fwd;   // [prime][n] {psi power, Shoup quotient}, SEAL bit-reversed order
     const ulonglong2 *inv;   // [prime][n] {inverse psi power, Shoup quotient}, SEAL scrambled order
     const ulonglong2 *ninv;  // [prime] {n^-1 mod q, Shoup quotient}

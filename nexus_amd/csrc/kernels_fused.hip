@@ -64,6 +64,117 @@ __global__ void k_modup_conv_fused(uint64_t *T, const uint64_t *coef, const ulon
     }
 }
 
+// Sum of up to 16 products of factors below 2^58 kept as three 64-bit partial
+// sums of 32x32-bit products (value = lo32 + mid 2^32 + hi 2^64): four
+// multiplies per product and no carry chain -- the generic form needs seven.
+// Bounds: lo32 < 16 2^32; mid < 16 (2^32 + 2^59) < 2^64; hi < 16 2^52.
+struct Sum3 {
+    uint64_t lo32 = 0, mid = 0, hi = 0;
+    __device__ __forceinline__ void mac(uint64_t a, uint64_t b)
+    {
+        const uint64_t a0 = (uint32_t)a, a1 = a >> 32, b0 = (uint32_t)b, b1 = b >> 32;
+        const uint64_t p00 = a0 * b0;
+        lo32 += (uint32_t)p00;
+        mid += (p00 >> 32) + a0 * b1 + a1 * b0;
+        hi += a1 * b1;
+    }
+    __device__ __forceinline__ void fold(uint64_t &h, uint64_t &l) const
+    {
+        const uint64_t ml = mid << 32;
+        l = lo32 + ml;
+        h = hi + (mid >> 32) + (l < ml);
+    }
+};
+
+// k_modup_conv_fused for alpha = A (compile time: no per-term branches) with
+// every prime below 2^58 (Sum3 products); a partial last digit zero-fills.
+template <int A>
+__global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ T, const uint64_t *__restrict__ coef,
+                                                        const ulonglong2 *__restrict__ w,
+                                                        const uint64_t *__restrict__ conv,
+                                                        const ModConst *__restrict__ mod, OutMap outp,
+                                                        uint32_t l, uint32_t digits, uint32_t nout,
+                                                        uint32_t key_limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t chunks = (nout + kOutChunk - 1) / kOutChunk;
+    const uint32_t row = blockIdx.y;  // (b * digits + j) * chunks + chunk
+    const uint32_t bj = row / chunks, chunk = row - bj * chunks;
+    const uint32_t b = bj / digits, j = bj - b * digits;
+    const uint32_t first = j * A;
+    const uint32_t s = min((uint32_t)A, l - first);
+    const uint32_t base = (j * A + s - 1) * A;
+    const uint64_t *x = coef + (uint64_t)b * l * n + (uint64_t)first * n + c;
+    uint64_t y[A];
+#pragma unroll
+    for (int i = 0; i < A; i++) {
+        y[i] = 0;
+        if ((uint32_t)i < s) {
+            const ulonglong2 wi = w[base + i];
+            y[i] = shoup(x[(uint64_t)i * n], wi.x, wi.y, mod[first + i].q);
+        }
+    }
+    uint64_t *out = T + ((uint64_t)b * nout * digits + j) * n + c;
+    const uint32_t I_end = min(nout, (chunk + 1) * kOutChunk);
+    for (uint32_t I = chunk * kOutChunk; I < I_end; I++) {
+        const uint32_t kp = outp.p[I];
+        const uint64_t *f = conv + (uint64_t)base * key_limbs + kp;
+        uint64_t fv[A];
+#pragma unroll
+        for (int i = 0; i < A; i++) fv[i] = (uint32_t)i < s ? f[(uint64_t)i * key_limbs] : 0;
+        Sum3 acc;
+#pragma unroll
+        for (int i = 0; i < A; i++) acc.mac(y[i], fv[i]);
+        uint64_t hi, lo;
+        acc.fold(hi, lo);
+        out[(uint64_t)I * digits * n] = barrett128(hi, lo, mod[kp]);
+    }
+}
+
+// k_moddown_conv_fused for alpha = A with every prime below 2^58 (Sum3 products)
+template <int A>
+__global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict__ z,
+                                                          const uint64_t *__restrict__ acc, uint64_t acc_stride,
+                                                          const ulonglong2 *__restrict__ w,
+                                                          const uint64_t *__restrict__ conv,
+                                                          const ModConst *__restrict__ mod, uint32_t l, uint32_t L,
+                                                          uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
+    const uint32_t p = blockIdx.y / chunks, chunk = blockIdx.y - p * chunks;
+    const uint64_t *y = acc + (uint64_t)p * acc_stride + (uint64_t)l * n + c;
+    uint64_t v[A];
+    double frac = 0.0;
+#pragma unroll
+    for (int k = 0; k < A; k++) {
+        const ulonglong2 wk = w[k];
+        const uint64_t pk = mod[L + k].q;
+        v[k] = shoup(y[(uint64_t)k * n], wk.x, wk.y, pk);
+        frac += (double)v[k] / (double)pk;
+    }
+    const uint64_t r = (uint64_t)floor(frac + 0.5);  // <= A
+    uint64_t *out = z + (uint64_t)p * l * n + c;
+    const uint32_t i_end = min(l, (chunk + 1) * kOutChunk);
+    for (uint32_t i = chunk * kOutChunk; i < i_end; i++) {
+        uint64_t fv[A];
+#pragma unroll
+        for (int k = 0; k < A; k++) fv[k] = conv[(uint64_t)k * L + i];
+        Sum3 s3;
+#pragma unroll
+        for (int k = 0; k < A; k++) s3.mac(v[k], fv[k]);
+        uint64_t hi, lo;
+        s3.fold(hi, lo);
+        // + r (P mod q_i)'s negative: r < 2^5, so the product fits one word
+        const uint64_t add = r * (mod[i].q - conv[(uint64_t)A * L + i]);
+        lo += add;
+        hi += lo < add;
+        out[(uint64_t)i * n] = barrett128(hi, lo, mod[i]);
+    }
+}
+
 // thread (p, chunk, c): z[p][i][c] for the data primes i of its chunk
 __global__ void k_moddown_conv_fused(uint64_t *z, const uint64_t *acc, uint64_t acc_stride, const ulonglong2 *w,
                                      const uint64_t *conv, const ModConst *mod, uint32_t l, uint32_t alpha,
@@ -206,8 +317,22 @@ void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *
     for (uint32_t i = 0; i < kMaxPrimes; i++) o.p[i] = i < nout ? out_primes_host[i] : 0;
     ProfScope prof("ks_modup_conv", s, 8.0 * n * B * ((double)l + (double)nout * digits));
     const uint32_t chunks = (nout + kOutChunk - 1) / kOutChunk;
-    hipLaunchKernelGGL(k_modup_conv_fused, dim3((n + kBlock - 1) / kBlock, B * digits * chunks), dim3(kBlock), 0, s,
-                       T, coef, w, conv, t.mod, o, l, alpha, digits, nout, key_limbs, n);
+    const dim3 grid((n + kBlock - 1) / kBlock, B * digits * chunks);
+    if (t.max_bits <= 58) {
+        switch (alpha) {
+#define NX_A(A)                                                                                                     \
+    case A:                                                                                                         \
+        hipLaunchKernelGGL(k_modup_conv_a<A>, grid, dim3(kBlock), 0, s, T, coef, w, conv, t.mod, o, l, digits, nout, \
+                           key_limbs, n);                                                                           \
+        return;
+            NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
+            NX_A(14) NX_A(15) NX_A(16)
+#undef NX_A
+        default: break;
+        }
+    }
+    hipLaunchKernelGGL(k_modup_conv_fused, grid, dim3(kBlock), 0, s, T, coef, w, conv, t.mod, o, l, alpha, digits,
+                       nout, key_limbs, n);
 }
 
 void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
@@ -217,8 +342,22 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
     const uint32_t n = 1u << t.logn;
     ProfScope prof("ks_moddown_conv", s, 8.0 * n * polys * ((double)alpha + l));
     const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
-    hipLaunchKernelGGL(k_moddown_conv_fused, dim3((n + kBlock - 1) / kBlock, polys * chunks), dim3(kBlock), 0, s, z,
-                       acc, acc_stride, w, conv, t.mod, l, alpha, L, n);
+    const dim3 grid((n + kBlock - 1) / kBlock, polys * chunks);
+    if (t.max_bits <= 58) {
+        switch (alpha) {
+#define NX_A(A)                                                                                                  \
+    case A:                                                                                                      \
+        hipLaunchKernelGGL(k_moddown_conv_a<A>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, l, L, \
+                           n);                                                                                   \
+        return;
+            NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
+            NX_A(14) NX_A(15) NX_A(16)
+#undef NX_A
+        default: break;
+        }
+    }
+    hipLaunchKernelGGL(k_moddown_conv_fused, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, l, alpha,
+                       L, n);
 }
 
 void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const *cts, const uint64_t *const *pts,
