@@ -23,6 +23,7 @@
 // start at the top compute level.  The chain needs 19 compute levels above q0
 // besides the bootstrapping levels, e.g. {51, 46 x 19, 51 x 14, 58 x 3}.
 #pragma once
+#include <chrono>
 
 #include <map>
 #include <memory>
@@ -125,7 +126,8 @@ private:
     Ciphertext rotate_tokens(const Ciphertext &x, int t);
     std::vector<Ciphertext> token_rotations(const Ciphertext &x, int t0, int count);
     std::vector<Ciphertext> offset_rotations(const Ciphertext &x, int dmin, int dmax);
-    const Plaintext &offset_mask(int p, size_t limbs, double scale);
+    // the 64 one-hot masks A_p (1 at slots = p mod 64), period 64, as periodic plaintexts
+    const PeriodicPlaintexts &offset_masks(size_t limbs, double scale);
     // per-token segment index (-1 outside every segment) and its 1 / length
     std::vector<int> segment_of(const std::vector<Segment> &segments) const;
     const Plaintext &key_mask(int eta, int tc, const std::vector<int> &seg, size_t limbs);
@@ -136,6 +138,8 @@ private:
     Ciphertext scaled(const Ciphertext &x, double value);  // x * value at the same scale, one level
     size_t chain_of(size_t limbs) const { return ck_.context->data_primes() + 1 - limbs; }
     uint64_t prime(size_t limbs) const { return ck_.context->moduli()[limbs - 1]; }
+    // stage_timing: synchronise and add the time since the last mark to `name`
+    void mark(const char *name);
 
     CKKSEvaluator &ck_;
     Bootstrapper &bt_;
@@ -144,7 +148,9 @@ private:
     size_t slots_, T_;
     std::shared_ptr<Shared> shared_;
     std::map<std::tuple<int, size_t, double, std::vector<int>>, Plaintext> masks_;
+    std::map<std::pair<size_t, double>, std::unique_ptr<PeriodicPlaintexts>> offset_masks_;
     std::vector<std::pair<std::string, double>> stage_ms_;
+    std::chrono::steady_clock::time_point t_mark_;
 };
 
 }  // namespace nexus

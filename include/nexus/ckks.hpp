@@ -132,6 +132,26 @@ private:
 typedef KSwitchKeys RelinKeys;
 typedef KSwitchKeys GaloisKeys;
 
+// `count` real slot vectors repeating every `period` slots (values
+// [count][period]), encoded exactly and held as 2 period NTT words per limb
+// (nx_periodic_plain_create).
+class PeriodicPlaintexts {
+public:
+    PeriodicPlaintexts(const CKKSContext &ctx, const std::vector<double> &values, size_t period, size_t chain_index,
+                       double scale);
+    ~PeriodicPlaintexts();
+    PeriodicPlaintexts(const PeriodicPlaintexts &) = delete;
+    PeriodicPlaintexts &operator=(const PeriodicPlaintexts &) = delete;
+    nx_periodic_plain *get() const { return h_; }
+    size_t count() const { return count_; }
+    void expand(size_t index, Plaintext &out) const;
+
+private:
+    const CKKSContext *ctx_;
+    nx_periodic_plain *h_ = nullptr;
+    size_t count_ = 0;
+};
+
 class Encoder {
 public:
     Encoder() = default;
@@ -201,6 +221,9 @@ public:
     // sum_r cts[r] * pts[r] in one pass
     void multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                                    Ciphertext &dest) const;
+    // out[m] = sum_k w[index[m K + k]] (.) cts[k] (index -1: no term), M = index.size() / K
+    void multiply_periodic_mix(const std::vector<const Ciphertext *> &cts, const PeriodicPlaintexts &w,
+                               const std::vector<int32_t> &index, std::vector<Ciphertext> &out) const;
     void add_plain(const Ciphertext &ct, const Plaintext &plain, Ciphertext &dest) const;
     void add_plain_inplace(Ciphertext &ct, const Plaintext &plain) const;
     void add(const Ciphertext &ct1, const Ciphertext &ct2, Ciphertext &dest) const;

@@ -27,6 +27,7 @@ extern "C" {
 typedef struct nx_context nx_context;
 typedef struct nx_ciphertext nx_ciphertext;
 typedef struct nx_plaintext nx_plaintext;
+typedef struct nx_periodic_plain nx_periodic_plain;
 typedef struct nx_secret_key nx_secret_key;
 typedef struct nx_kswitch_keys nx_kswitch_keys;
 
@@ -190,6 +191,27 @@ int nx_rotate_sum(const nx_ciphertext *const *in, const int *steps, size_t count
  * + add per term) and of the MatrixMul row sums (cuda/src/matrix_mul.cu:202-275). */
 int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
                                  nx_ciphertext *dest);
+
+/* ---- Periodic plaintexts and the block mix.  No counterpart in the
+ * reference, which multiplies such masks as full plaintexts term by term
+ * (multiply_plain + add per mask).  A real slot vector repeating every
+ * `period` slots (a power of two <= slots) is a polynomial in X^(N / (2 period));
+ * encoded exactly (coefficients round(scale c_k) of the real inverse
+ * embedding, not the FFT path of nx_encode) its NTT form is constant over
+ * blocks of N / (2 period) positions, so `count` of them are held as
+ * 2 period words per limb.  values: [count][period]. */
+int nx_periodic_plain_create(nx_context *ctx, const double *values, size_t period, size_t count, size_t chain_index,
+                             double scale, nx_periodic_plain **out);
+int nx_periodic_plain_destroy(nx_periodic_plain *p);
+/* The full NTT-form plaintext of entry `index` (decodes to its slot vector). */
+int nx_periodic_plain_expand(const nx_periodic_plain *p, size_t index, nx_plaintext *out);
+/* out[m] = sum_k W[index[m K + k]] (.) in[k] for m < M (index -1: no term):
+ * K <= 128 size-2 inputs of one level and scale, the level of W, each read
+ * once for all outputs; M <= 128 outputs at that level with scale
+ * in.scale * W.scale.  Equals nx_multiply_plain_accumulate over the expanded
+ * plaintexts word for word.  Outputs may alias inputs. */
+int nx_multiply_periodic_mix(const nx_ciphertext *const *in, size_t K, const nx_periodic_plain *w,
+                             const int32_t *index, size_t M, nx_ciphertext *const *out);
 
 /* ---- SEAL 4.1 serialization, compression mode "none" (reference
  * thirdparty/SEAL-4.1-bs/native/src/seal/serialization.h:76-91 header,

@@ -553,6 +553,50 @@ def multiply_plain_accumulate(cts, pts, dest=None):
     return d
 
 
+_sig("nx_periodic_plain_create", [vp, f64p, sz, sz, sz, ctypes.c_double, ctypes.POINTER(vp)])
+_sig("nx_periodic_plain_destroy", [vp])
+_sig("nx_periodic_plain_expand", [vp, sz, vp])
+_sig("nx_multiply_periodic_mix", [ctypes.POINTER(vp), sz, vp, ctypes.POINTER(ctypes.c_int32), sz,
+                                  ctypes.POINTER(vp)])
+
+
+class PeriodicPlaintexts:
+    """Real slot vectors repeating every `period` slots (values: [count][period]),
+    encoded exactly as polynomials in X^(N / (2 period)) and kept as 2 period
+    NTT words per limb."""
+
+    def __init__(self, ctx, values, scale, chain_index=1):
+        v = np.ascontiguousarray(np.atleast_2d(np.asarray(values, dtype=np.float64)))
+        self.ctx = ctx
+        self.count, self.period = v.shape
+        h = vp()
+        _check(_lib.nx_periodic_plain_create(ctx.h, v.ctypes.data_as(f64p), self.period, self.count, chain_index,
+                                             scale, ctypes.byref(h)))
+        self.h = h
+
+    def __del__(self, _fn=_lib.nx_periodic_plain_destroy, _rel=_release):
+        _rel(self, _fn)
+
+    def expand(self, index):
+        """the full NTT-form plaintext of entry `index`"""
+        pt = Plaintext(self.ctx)
+        _check(_lib.nx_periodic_plain_expand(self.h, index, pt.h))
+        return pt
+
+
+def multiply_periodic_mix(cts, w, index):
+    """[sum_k w[index[m, k]] (.) cts[k] for each row m of index] (-1: no term)"""
+    idx = np.ascontiguousarray(np.asarray(index, dtype=np.int32))
+    M, K = idx.shape
+    if K != len(cts):
+        raise ValueError("index needs one column per input")
+    outs = [Ciphertext(cts[0].ctx) for _ in range(M)]
+    c = (vp * K)(*[x.h for x in cts])
+    o = (vp * M)(*[x.h for x in outs])
+    _check(_lib.nx_multiply_periodic_mix(c, K, w.h, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), M, o))
+    return outs
+
+
 def rotate_hoisted(a, steps, galois_keys):
     """[rot(a, s) for s in steps] sharing one ModUp (Phantom hoisting)."""
     outs = [Ciphertext(a.ctx) for _ in steps]

@@ -291,6 +291,30 @@ void load_plaintext(Context &c, const uint8_t *data, size_t n, Plaintext &out);
 std::vector<uint8_t> save_kswitch_keys(const KSwitchKeys &keys, bool galois);
 void load_kswitch_keys(Context &c, const uint8_t *data, size_t n, bool galois, KSwitchKeys &out);
 
+// Periodic plaintexts -- periodic.cpp.  `count` real slot vectors repeating
+// every `period` slots, encoded exactly as polynomials in X^(n / (2 period))
+// (coefficients round(scale c_k) of the real inverse embedding) and held in
+// NTT form as `blocks` = 2 period words per limb: table [count][limbs][blocks].
+struct PeriodicPlain {
+    Context *ctx = nullptr;
+    size_t count = 0, period = 0, blocks = 0, limbs = 0;
+    double scale = 1.0;
+    DevBuf table;
+    // device term lists of block_mix, per index pattern (M, K, index)
+    struct Terms {
+        DevBuf words;  // first[M + 1] (u32) then (k, entry) int32 pairs, 8-byte aligned
+        uint32_t M = 0, nterms = 0;
+    };
+    std::map<std::vector<int32_t>, Terms> terms;
+    std::mutex mu;
+};
+void periodic_create(Context &c, const double *values, size_t period, size_t count, size_t limbs, double scale,
+                     PeriodicPlain &out);
+void periodic_expand_entry(const PeriodicPlain &p, size_t index, Plaintext &out);
+// out[m] = sum_k W[index[m K + k]] (.) in[k] (index < 0: no term)
+void op_periodic_mix(const std::vector<const Ciphertext *> &in, PeriodicPlain &w, const int32_t *index, size_t M,
+                     const std::vector<Ciphertext *> &out);
+
 // launch-shape autotuner -- autotune.cpp; returns a JSON report
 std::string autotune(Context &c, int reps);
 

@@ -306,16 +306,15 @@ std::vector<Ciphertext> BertLayer::offset_rotations(const Ciphertext &x, int dmi
     return out;
 }
 
-const Plaintext &BertLayer::offset_mask(int p, size_t limbs, double scale)
+const PeriodicPlaintexts &BertLayer::offset_masks(size_t limbs, double scale)
 {
-    auto key = std::make_tuple(p, limbs, scale, std::vector<int>());
-    auto it = masks_.find(key);
-    if (it != masks_.end()) return it->second;
-    std::vector<double> v(slots_, 0.0);
-    for (size_t y = (size_t)p; y < slots_; y += kHeadDim) v[y] = 1.0;
-    Plaintext pt(*ck_.context);
-    ck_.encoder.encode(v, chain_of(limbs), scale, pt);
-    return masks_.emplace(key, std::move(pt)).first->second;
+    const auto key = std::make_pair(limbs, scale);
+    auto it = offset_masks_.find(key);
+    if (it != offset_masks_.end()) return *it->second;
+    std::vector<double> eye(kHeadDim * kHeadDim, 0.0);
+    for (int p = 0; p < kHeadDim; p++) eye[p * kHeadDim + p] = 1.0;
+    auto w = std::make_unique<PeriodicPlaintexts>(*ck_.context, eye, kHeadDim, chain_of(limbs), scale);
+    return *offset_masks_.emplace(key, std::move(w)).first->second;
 }
 
 std::vector<int> BertLayer::segment_of(const std::vector<Segment> &segments) const
@@ -465,6 +464,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
         for (int eta : active) {
             const int t0 = kHeadDim * eta, lo = tlo[eta], hi = thi[eta];
             std::vector<Ciphertext> kr = token_rotations(kc, t0 + lo, hi - lo);
+            mark("attention.k_rotations");
             std::vector<Ciphertext> P(hi - lo);  // P[tau - lo]
             for (int tau = lo; tau < hi; tau++) {
                 Ciphertext &pt = P[tau - lo];
@@ -473,35 +473,40 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
                 ev.rescale_to_next_inplace(pt);
             }
             kr.clear();
+            mark("attention.qk_products");
             const size_t lp = P[0].coeff_modulus_size();
             const double ms = (double)prime(lp);
             // S = sum_d rot(Z_d, d), Z_d = sum_tau A_(tau + d) (.) P_tau, d = 8 a + b:
             // Y_a = sum_b rot(Z_(8a+b), b), S = sum_a rot(Y_a, 8 a), each sum of
             // rotations with a single ModDown
             const int dmin = -(hi - 1), dmax = kHeadDim - 1 - lo;
+            // every Z_d in one block mix: each P_tau is read once for all offsets
+            std::vector<Ciphertext> Zall;
+            {
+                std::vector<const Ciphertext *> in;
+                for (auto &p : P) in.push_back(&p);
+                const int K = hi - lo, M = dmax - dmin + 1;
+                std::vector<int32_t> index((size_t)M * K, -1);
+                for (int m = 0; m < M; m++)
+                    for (int k = 0; k < K; k++) {
+                        const int pidx = lo + k + dmin + m;  // tau + d
+                        if (pidx >= 0 && pidx < kHeadDim) index[(size_t)m * K + k] = pidx;
+                    }
+                ev.multiply_periodic_mix(in, offset_masks(lp, ms), index, Zall);
+            }
+            P.clear();
             std::vector<Ciphertext> Ys;
             std::vector<int> ysteps;
             for (int a = floor_div(dmin, kOffBaby); a <= floor_div(dmax, kOffBaby); a++) {
-                std::vector<Ciphertext> Zs;
+                std::vector<const Ciphertext *> zp;
                 std::vector<int> zsteps;
                 for (int b = 0; b < kOffBaby; b++) {
                     const int d = kOffBaby * a + b;
                     if (d < dmin || d > dmax) continue;
-                    std::vector<const Ciphertext *> terms;
-                    std::vector<const Plaintext *> weights;
-                    for (int tau = std::max(lo, -d); tau < hi && tau + d < kHeadDim; tau++) {
-                        terms.push_back(&P[tau - lo]);
-                        weights.push_back(&offset_mask(tau + d, lp, ms));
-                    }
-                    if (terms.empty()) continue;
-                    Ciphertext Z;
-                    ev.multiply_plain_accumulate(terms, weights, Z);
-                    Zs.push_back(std::move(Z));
+                    zp.push_back(&Zall[d - dmin]);
                     zsteps.push_back(b);
                 }
-                if (Zs.empty()) continue;
-                std::vector<const Ciphertext *> zp;
-                for (const auto &z : Zs) zp.push_back(&z);
+                if (zp.empty()) continue;
                 Ciphertext Y;
                 ev.rotate_sum(zp, zsteps, gk, Y);
                 Ys.push_back(std::move(Y));
@@ -511,6 +516,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             for (const auto &y : Ys) yp.push_back(&y);
             ev.rotate_sum(yp, ysteps, gk, S[eta]);
             ev.rescale_to_next_inplace(S[eta]);
+            mark("attention.compaction");
         }
         // ---- softmax over t for every (query, head): exp, key mask, sum, 1 / sum
         std::vector<Ciphertext> E(halves);
@@ -554,6 +560,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             ev.relinearize_inplace(p, rk);
             ev.rescale_to_next_inplace(p);
         }
+        mark("attention.softmax");
     }
     {
         std::vector<Ciphertext *> all;
@@ -561,6 +568,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             for (int eta : active) all.push_back(&row[eta]);
         bootstrap_real(all, 1.0);
     }
+    mark("attention.bootstrap");
     // ---- expansion over the 64 features of each head; times the rotated v
     std::vector<Ciphertext> out(n);
     for (size_t c = 0; c < n; c++) {
@@ -572,20 +580,30 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             // Pb_tau[256 i + 64 h + f] = P[256 i + 64 h + tau] = sum_d A_(tau - d) (.) rot(P, d)
             const int dlo = lo - (kHeadDim - 1);
             std::vector<Ciphertext> R = offset_rotations(Pb, dlo, hi - 1);
+            mark("attention.p_offsets");
             const size_t lb = Pb.coeff_modulus_size();
             const double ms = (double)prime(lb);
             Ciphertext vlow = v[c];
             if (vlow.coeff_modulus_size() > lb - 1) ev.mod_switch_to_inplace(vlow, chain_of(lb - 1));
             std::vector<Ciphertext> vr = token_rotations(vlow, t0 + lo, hi - lo);
+            mark("attention.v_rotations");
+            // every Pb_tau in one block mix over the rotations R_d
+            std::vector<Ciphertext> pbs;
+            {
+                std::vector<const Ciphertext *> in;
+                for (auto &r : R) in.push_back(&r);
+                const int K = (int)R.size(), M = hi - lo;
+                std::vector<int32_t> index((size_t)M * K, -1);
+                for (int m = 0; m < M; m++)
+                    for (int k = 0; k < K; k++) {
+                        const int pidx = (lo + m) - (dlo + k);  // tau - d
+                        if (pidx >= 0 && pidx < kHeadDim) index[(size_t)m * K + k] = pidx;
+                    }
+                ev.multiply_periodic_mix(in, offset_masks(lb, ms), index, pbs);
+            }
+            R.clear();
             for (int tau = lo; tau < hi; tau++) {
-                std::vector<const Ciphertext *> terms;
-                std::vector<const Plaintext *> weights;
-                for (int d = tau - (kHeadDim - 1); d <= tau; d++) {
-                    terms.push_back(&R[d - dlo]);
-                    weights.push_back(&offset_mask(tau - d, lb, ms));
-                }
-                Ciphertext pb;
-                ev.multiply_plain_accumulate(terms, weights, pb);
+                Ciphertext &pb = pbs[tau - lo];
                 ev.rescale_to_next_inplace(pb);
                 Ciphertext prod;
                 ev.multiply(pb, vr[tau - lo], prod);
@@ -596,6 +614,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
                     ev.add_inplace(acc, prod);
                 }
             }
+            mark("attention.expansion");
         }
         ev.relinearize_inplace(acc, rk);
         ev.rescale_to_next_inplace(acc);
@@ -654,6 +673,21 @@ std::vector<Ciphertext> BertLayer::layer_norm(const std::vector<Ciphertext> &r)
     return out;
 }
 
+void BertLayer::mark(const char *name)
+{
+    if (!opt_.stage_timing) return;
+    ck_.context->synchronize();
+    const auto now = std::chrono::steady_clock::now();
+    const double ms = std::chrono::duration<double, std::milli>(now - t_mark_).count();
+    t_mark_ = now;
+    for (auto &e : stage_ms_)
+        if (e.first == name) {
+            e.second += ms;
+            return;
+        }
+    stage_ms_.emplace_back(name, ms);
+}
+
 std::vector<Ciphertext> BertLayer::forward(const std::vector<Ciphertext> &x, int s)
 {
     if (s < 1 || (size_t)s > T_) throw std::invalid_argument("BertLayer: sequence length out of range");
@@ -680,15 +714,7 @@ std::vector<Ciphertext> BertLayer::forward(const std::vector<Ciphertext> &x, con
         }
     };
     stage_ms_.clear();
-    auto t_last = std::chrono::steady_clock::now();
-    auto mark = [&](const char *name) {
-        if (!opt_.stage_timing) return;
-        ck_.context->synchronize();
-        const auto now = std::chrono::steady_clock::now();
-        stage_ms_.emplace_back(name, std::chrono::duration<double, std::milli>(now - t_last).count());
-        t_last = now;
-    };
-    if (opt_.stage_timing) ck_.context->synchronize(), t_last = std::chrono::steady_clock::now();
+    if (opt_.stage_timing) ck_.context->synchronize(), t_mark_ = std::chrono::steady_clock::now();
     // attention block; the bootstrapping range factors are folded into the weights
     std::vector<Ciphertext> q = linear(Wq, x, 1.0 / 32), k = linear(Wk, x, 0.25), v = linear(Wv, x, 0.25);
     mark("qkv");

@@ -421,6 +421,43 @@ void Evaluator::multiply_plain_accumulate(const std::vector<const Ciphertext *> 
     check(nx_multiply_plain_accumulate(c.data(), p.data(), c.size(), out.get()));
     dest = std::move(out);
 }
+void Evaluator::multiply_periodic_mix(const std::vector<const Ciphertext *> &cts, const PeriodicPlaintexts &w,
+                                      const std::vector<int32_t> &index, std::vector<Ciphertext> &out) const
+{
+    const size_t K = cts.size();
+    if (!K || index.size() % K) throw std::invalid_argument("multiply_periodic_mix: index must be M x K");
+    const size_t M = index.size() / K;
+    std::vector<const nx_ciphertext *> c;
+    for (const Ciphertext *x : cts) c.push_back(x->get());
+    std::vector<Ciphertext> res;
+    res.reserve(M);
+    std::vector<nx_ciphertext *> o;
+    for (size_t m = 0; m < M; m++) {
+        res.emplace_back(*ctx_);
+        o.push_back(res.back().get());
+    }
+    check(nx_multiply_periodic_mix(c.data(), K, w.get(), index.data(), M, o.data()));
+    out = std::move(res);
+}
+
+PeriodicPlaintexts::PeriodicPlaintexts(const CKKSContext &ctx, const std::vector<double> &values, size_t period,
+                                       size_t chain_index, double scale)
+    : ctx_(&ctx)
+{
+    if (!period || values.size() % period) throw std::invalid_argument("PeriodicPlaintexts: values must be count x period");
+    count_ = values.size() / period;
+    check(nx_periodic_plain_create(ctx.get(), values.data(), period, count_, chain_index, scale, &h_));
+}
+
+PeriodicPlaintexts::~PeriodicPlaintexts() { nx_periodic_plain_destroy(h_); }
+
+void PeriodicPlaintexts::expand(size_t index, Plaintext &out) const
+{
+    Plaintext r(*ctx_);
+    check(nx_periodic_plain_expand(h_, index, r.get()));
+    out = std::move(r);
+}
+
 void Evaluator::hoisting_inplace(Ciphertext &ct, const GaloisKeys &k, const std::vector<int> &steps) const
 {
     check(nx_hoisting(ct.get(), steps.data(), steps.size(), k.get(), ct.get()));

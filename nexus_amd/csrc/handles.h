@@ -25,6 +25,9 @@ struct nx_kswitch_keys {
 struct nx_ct_batch {
     nx::CtBatch impl;
 };
+struct nx_periodic_plain {
+    nx::PeriodicPlain impl;
+};
 
 namespace nx {
 

@@ -129,6 +129,31 @@ void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const 
 void dot_plain_set_layout(uint32_t limbs, int layout);
 int dot_plain_get_layout(uint32_t limbs);
 
+// Periodic plaintexts (kernels_mix.hip): `rows` = entries x limbs polynomials
+// whose NTT values are constant over `blocks` blocks of n / blocks positions.
+// scatter: work[row][c] = coef[row][c / stride] at multiples of stride, else 0
+// (and flags[row][*] = 0); gather: table[row][b] = work[row][b stride], flags
+// set where a block is not constant; expand: one entry's full NTT form.
+void periodic_scatter(const DeviceTables &t, uint64_t *work, uint8_t *flags, const uint64_t *coef, uint32_t rows,
+                      uint32_t blocks, hipStream_t s);
+void periodic_gather(const DeviceTables &t, uint64_t *table, uint8_t *flags, const uint64_t *work, uint32_t rows,
+                     uint32_t blocks, hipStream_t s);
+void periodic_expand(const DeviceTables &t, uint64_t *out, const uint64_t *table, uint32_t limbs, uint32_t blocks,
+                     hipStream_t s);
+// out[m] = sum over terms (k, e) of output m of table[e] (.) in[k], both
+// polynomials, for K inputs and M outputs of `limbs` limbs (K, M <= kMixMax):
+// terms[first[m] .. first[m + 1]) as (k, e) int32 pairs.  Needs n >= 64 and
+// block_mix_lds_bytes(...) <= 64 KiB.
+constexpr int kMixMax = 128;
+struct MixPtrs {
+    const uint64_t *in[kMixMax];
+    uint64_t *out[kMixMax];
+};
+size_t block_mix_lds_bytes(uint32_t K, uint32_t count, uint32_t blocks, int logn);
+void block_mix(const DeviceTables &t, const MixPtrs &p, const uint64_t *table, const int32_t *terms,
+               const uint32_t *first, uint32_t K, uint32_t M, uint32_t nterms, uint32_t limbs, uint32_t count,
+               uint32_t blocks, hipStream_t s);
+
 // Hoisted variants over one shared ModUp T[nout][l][n] (B = 1): rotation r
 // reads digit values through the NTT-domain Galois permutation perms[r]
 // (T[I][j][perms[r][c]]) and uses key keys[r].  _rot writes one result per
