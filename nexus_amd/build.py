@@ -23,8 +23,9 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + INCLUDE, "-Wall", "-
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 # Packed FP32 instructions are kept out of every device code object.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-# The encoder's double-precision FFT must round exactly as written.
-NO_CONTRACT = {"encoder.cpp", "kernels_encode.hip"}
+# The encoder's double-precision FFT and the NTT's double-precision modular
+# arithmetic must round exactly as written.
+NO_CONTRACT = {"encoder.cpp", "kernels_encode.hip", "kernels_ntt.hip"}
 
 
 def sources():

@@ -165,6 +165,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
 
     const size_t P = moduli.size();
     std::vector<ulonglong2> fwd(P * n), inv(P * n), ninv(P);
+    std::vector<double> fwd_d(P * n), inv_d(P * n), ninv_d(P);  // exact for primes below 2^53
     fwd_host.resize(P);
     inv_host.resize(P);
     for (size_t i = 0; i < P; i++) {
@@ -173,9 +174,12 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
         for (size_t k = 0; k < n; k++) {
             fwd[i * n + k] = make_ulonglong2(fwd_host[i][k], shoup_quotient(fwd_host[i][k], q));
             inv[i * n + k] = make_ulonglong2(inv_host[i][k], shoup_quotient(inv_host[i][k], q));
+            fwd_d[i * n + k] = (double)fwd_host[i][k];
+            inv_d[i * n + k] = (double)inv_host[i][k];
         }
         uint64_t ni = host_inv_mod(n % q, q);
         ninv[i] = make_ulonglong2(ni, shoup_quotient(ni, q));
+        ninv_d[i] = (double)ni;
     }
     // rescale constants: row `l` (limbs before the rescale), column i < l-1
     std::vector<ulonglong2> resc(P * P, make_ulonglong2(0, 0));
@@ -200,12 +204,18 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     NX_HIP(hipMalloc(&d_fwd, P * n * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_inv, P * n * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_ninv, P * sizeof(ulonglong2)));
+    NX_HIP(hipMalloc(&d_fwd_d, P * n * sizeof(double)));
+    NX_HIP(hipMalloc(&d_inv_d, P * n * sizeof(double)));
+    NX_HIP(hipMalloc(&d_ninv_d, P * sizeof(double)));
     NX_HIP(hipMalloc(&d_mod, P * sizeof(ModConst)));
     NX_HIP(hipMalloc(&d_rescale_inv, P * P * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_p_inv, P * sizeof(ulonglong2)));
     NX_HIP(hipMemcpy(d_fwd, fwd.data(), P * n * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_inv, inv.data(), P * n * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_ninv, ninv.data(), P * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_fwd_d, fwd_d.data(), P * n * sizeof(double), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_inv_d, inv_d.data(), P * n * sizeof(double), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_ninv_d, ninv_d.data(), P * sizeof(double), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_mod, mods.data(), P * sizeof(ModConst), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_rescale_inv, resc.data(), P * P * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_p_inv, pinv.data(), P * sizeof(ulonglong2), hipMemcpyHostToDevice));
@@ -221,6 +231,9 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     tables.fwd = d_fwd;
     tables.inv = d_inv;
     tables.ninv = d_ninv;
+    tables.fwd_d = d_fwd_d;
+    tables.inv_d = d_inv_d;
+    tables.ninv_d = d_ninv_d;
     tables.mod = d_mod;
     upload_encoder_tables(*this);
 }
@@ -302,6 +315,9 @@ Context::~Context()
     (void)hipFree(d_fwd);
     (void)hipFree(d_inv);
     (void)hipFree(d_ninv);
+    (void)hipFree(d_fwd_d);
+    (void)hipFree(d_inv_d);
+    (void)hipFree(d_ninv_d);
     (void)hipFree(d_mod);
     (void)hipFree(d_rescale_inv);
     (void)hipFree(d_p_inv);

@@ -18,6 +18,9 @@ struct DeviceTables {
     const ulonglong2 *fwd;   // [prime][n] {psi power, Shoup quotient}, SEAL bit-reversed order
     const ulonglong2 *inv;   // [prime][n] {inverse psi power, Shoup quotient}, SEAL scrambled order
     const ulonglong2 *ninv;  // [prime] {n^-1 mod q, Shoup quotient}
+    const double *fwd_d;     // [prime][n] the fwd powers as doubles (exact below 2^53)
+    const double *inv_d;     // [prime][n] the inv powers as doubles
+    const double *ninv_d;    // [prime] n^-1 mod q as doubles
     const ModConst *mod;     // [prime]
 };
 

@@ -16,6 +16,17 @@
 // rows) through LDS and runs radix-8/4/2 register butterflies between
 // barriers, 256 threads (four wave64) per workgroup.  Row twiddles for row r at
 // local stage s are the contiguous block fwd[2^s (N1 + r) ...].
+//
+// Arithmetic.  Limbs of primes q < 2^51 run in double precision, where the
+// FMA is full rate: with |a| <= q (1 + eps) and 0 <= w < q,
+//     h = a w (rounded), e = fma(a, w, -h) = a w - h exactly,
+//     Q = rint(h / q) (|Q - a w / q| <= 1.25), r = fma(-Q, q, h) + e = a w - Q q
+// is exact (every intermediate is an integer below 2^53) with |r| <= 1.25 q;
+// reduce(z) = fma(-rint(z / q), q, z) brings |z| < 2^52 to |z| <= q/2 (1 + eps).
+// Butterfly outputs are reduced, loads are reduced, and stores are mapped to
+// [0, q), so the output words equal the integer path's.  Larger primes use
+// 64-bit Shoup butterflies (ten 32-bit multiplies each) on [0, 4q).  This
+// file is compiled without floating-point contraction.
 #include <atomic>
 
 #include "kernels.h"
@@ -37,6 +48,8 @@ struct NttKArgs {
     const ulonglong2 *tw;
     const ModConst *mod;
     const ulonglong2 *ninv;
+    const double *twd;    // the same twiddles as doubles
+    const double *ninvd;  // n^-1 mod q as doubles
     uint8_t prime_of[kMaxPrimes];
 };
 
@@ -169,6 +182,205 @@ __device__ __forceinline__ void inv_step(uint64_t *lds, int s, const ulonglong2 
     }
 }
 
+constexpr uint64_t kFpPrimeBound = 1ull << 51;
+
+__device__ __forceinline__ double mulmod_f(double a, double w, double q, double qinv)
+{
+    const double h = a * w;
+    const double e = __builtin_fma(a, w, -h);
+    const double Q = __builtin_rint(h * qinv);
+    return __builtin_fma(-Q, q, h) + e;
+}
+
+__device__ __forceinline__ double reduce_f(double z, double q, double qinv)
+{
+    return __builtin_fma(-__builtin_rint(z * qinv), q, z);
+}
+
+// Forward radix-2^K step in double precision (values |v| <= q/2 (1 + eps)).
+template <int LOGS, int ROLE, int K, int CW>
+__device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw, double q, double qinv, uint32_t m0,
+                                           uint32_t row0)
+{
+    using G = Geometry<LOGS, ROLE, CW>;
+    constexpr int R = 1 << K;
+    constexpr int GROUPS_PER = G::S / R;
+    constexpr int GROUPS = G::TILE / R;
+    const int B = G::S >> s;
+    const int D = B >> K;
+    for (int g = threadIdx.x; g < GROUPS; g += kThreads) {
+        int cc, gg;
+        if (ROLE == kCol) {
+            cc = g % G::C;
+            gg = g / G::C;
+        } else {
+            cc = g / GROUPS_PER;
+            gg = g % GROUPS_PER;
+        }
+        const int blockid = gg / D;
+        const int o = gg - blockid * D;
+        const uint32_t mrow = (ROLE == kRow) ? (m0 + row0 + cc) : m0;
+        double v[R];
+        int addr[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            addr[t] = lds_addr<ROLE, G::S, G::C>(cc, blockid * B + o + t * D);
+            v[t] = lds[addr[t]];
+        }
+#pragma unroll
+        for (int d = 0; d < K; d++) {
+            const int h = 1 << (K - 1 - d);
+#pragma unroll
+            for (int j = 0; j < (1 << d); j++) {
+                const uint32_t b = (uint32_t)blockid * (1u << d) + j;
+                const double w = tw[((uint32_t)1 << (s + d)) * mrow + b];
+#pragma unroll
+                for (int u = 0; u < h; u++) {
+                    const int t = j * 2 * h + u;
+                    const double x = v[t];
+                    const double y = mulmod_f(v[t + h], w, q, qinv);
+                    v[t] = reduce_f(x + y, q, qinv);
+                    v[t + h] = reduce_f(x - y, q, qinv);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < R; t++) lds[addr[t]] = v[t];
+    }
+}
+
+// Inverse radix-2^K step in double precision.
+template <int LOGS, int ROLE, int K, int CW>
+__device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw, double q, double qinv, uint32_t n,
+                                           uint32_t mx, uint32_t row0)
+{
+    using G = Geometry<LOGS, ROLE, CW>;
+    constexpr int R = 1 << K;
+    constexpr int GROUPS_PER = G::S / R;
+    constexpr int GROUPS = G::TILE / R;
+    const int so = 1 << s;
+    for (int g = threadIdx.x; g < GROUPS; g += kThreads) {
+        int cc, gg;
+        if (ROLE == kCol) {
+            cc = g % G::C;
+            gg = g / G::C;
+        } else {
+            cc = g / GROUPS_PER;
+            gg = g % GROUPS_PER;
+        }
+        const int o = gg & (so - 1);
+        const int blockid = gg >> s;
+        const int base = blockid * (so << K) + o;
+        const uint32_t r = row0 + cc;
+        double v[R];
+        int addr[R];
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            addr[t] = lds_addr<ROLE, G::S, G::C>(cc, base + t * so);
+            v[t] = lds[addr[t]];
+        }
+#pragma unroll
+        for (int d = 0; d < K; d++) {
+            const int sd = s + d;
+            const uint32_t first = n - (mx >> sd) + 1 + (ROLE == kRow ? r * ((uint32_t)G::S >> (sd + 1)) : 0u);
+#pragma unroll
+            for (int j = 0; j < (R >> (d + 1)); j++) {
+                const uint32_t b = (uint32_t)blockid * (uint32_t)(R >> (d + 1)) + j;
+                const double w = tw[first + b];
+#pragma unroll
+                for (int u = 0; u < (1 << d); u++) {
+                    const int t = j * (2 << d) + u;
+                    const double x = v[t], y = v[t + (1 << d)];
+                    v[t] = reduce_f(x + y, q, qinv);
+                    v[t + (1 << d)] = reduce_f(mulmod_f(x - y, w, q, qinv), q, qinv);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < R; t++) lds[addr[t]] = v[t];
+    }
+}
+
+// The whole transform body of one tile in double precision: load (reduced),
+// the radix steps, store in [0, q) (times n^-1 on the inverse's last pass).
+template <int LOGS, int ROLE, bool FWD, int CW>
+__device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_words, uint64_t *data, uint32_t prime,
+                                           uint32_t n, uint32_t N2, uint32_t N1, uint32_t first)
+{
+    using G = Geometry<LOGS, ROLE, CW>;
+    double *lds = reinterpret_cast<double *>(lds_words);
+    const double q = (double)a.mod[prime].q;
+    const double qinv = 1.0 / q;
+    const double *tw = a.twd + (uint64_t)prime * n;
+    for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
+        uint64_t gidx;
+        if (ROLE == kCol) {
+            const int k = e / G::C, cc = e % G::C;
+            gidx = (uint64_t)k * N2 + first + cc;
+        } else if (ROLE == kRow) {
+            gidx = (uint64_t)first * G::S + e;
+        } else {
+            gidx = e;
+        }
+        lds[e] = reduce_f((double)data[gidx], q, qinv);
+    }
+    __syncthreads();
+    if (FWD) {
+        const uint32_t m0 = ROLE == kRow ? N1 : 1u;
+        int s = 0;
+        if constexpr (LOGS >= 3) {
+#pragma unroll
+            for (int st = 0; st < LOGS / 3; st++) {
+                fwd_step_f<LOGS, ROLE, 3, CW>(lds, s, tw, q, qinv, m0, first);
+                s += 3;
+                __syncthreads();
+            }
+        }
+        if constexpr (LOGS % 3 == 2) {
+            fwd_step_f<LOGS, ROLE, 2, CW>(lds, s, tw, q, qinv, m0, first);
+            __syncthreads();
+        } else if constexpr (LOGS % 3 == 1) {
+            fwd_step_f<LOGS, ROLE, 1, CW>(lds, s, tw, q, qinv, m0, first);
+            __syncthreads();
+        }
+    } else {
+        const uint32_t mx = ROLE == kCol ? N1 : n;
+        int s = 0;
+        if constexpr (LOGS >= 3) {
+#pragma unroll
+            for (int st = 0; st < LOGS / 3; st++) {
+                inv_step_f<LOGS, ROLE, 3, CW>(lds, s, tw, q, qinv, n, mx, first);
+                s += 3;
+                __syncthreads();
+            }
+        }
+        if constexpr (LOGS % 3 == 2) {
+            inv_step_f<LOGS, ROLE, 2, CW>(lds, s, tw, q, qinv, n, mx, first);
+            __syncthreads();
+        } else if constexpr (LOGS % 3 == 1) {
+            inv_step_f<LOGS, ROLE, 1, CW>(lds, s, tw, q, qinv, n, mx, first);
+            __syncthreads();
+        }
+    }
+    const bool scale_out = !FWD && ROLE != kRow;  // the inverse's last pass
+    const double ni = a.ninvd[prime];
+    for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
+        uint64_t gidx;
+        if (ROLE == kCol) {
+            const int k = e / G::C, cc = e % G::C;
+            gidx = (uint64_t)k * N2 + first + cc;
+        } else if (ROLE == kRow) {
+            gidx = (uint64_t)first * G::S + e;
+        } else {
+            gidx = e;
+        }
+        double v = lds[e];
+        if (scale_out) v = reduce_f(mulmod_f(v, ni, q, qinv), q, qinv);
+        v = v < 0 ? v + q : v;
+        data[gidx] = (uint64_t)v;
+    }
+}
+
 template <int LOGS, int ROLE, bool FWD, int CW>
 __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
 {
@@ -185,6 +397,10 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     const uint32_t N2 = ROLE == kCol ? (n >> LOGS) : (ROLE == kRow ? (uint32_t)G::S : n);
     const uint32_t N1 = n / N2;
     const uint32_t first = blockIdx.x * G::C;  // first column / row of the tile
+    if (q < kFpPrimeBound) {
+        ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, prime, n, N2, N1, first);
+        return;
+    }
 
     // ---- load
     for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
@@ -344,6 +560,8 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
     a.tw = fwd ? t.fwd : t.inv;
     a.mod = t.mod;
     a.ninv = t.ninv;
+    a.twd = fwd ? t.fwd_d : t.inv_d;
+    a.ninvd = t.ninv_d;
     for (int i = 0; i < kMaxPrimes; i++) a.prime_of[i] = b.prime_of[i];
     return a;
 }

@@ -61,6 +61,33 @@ def test_ntt_roundtrip_parity(logn):
     assert np.array_equal(back, data)
 
 
+@pytest.mark.parametrize("logn", [12, 14, 16])
+def test_ntt_extreme_values(logn):
+    """all q - 1, alternating 0 / q - 1, and a single non-zero word, on primes
+    of the double-precision path (30..51 bits) and of the 64-bit path (58, 60)"""
+    _note("ntt extremes logn=%d" % logn)
+    ctx = nx.Context(logn, [51, 50, 46, 30, 58, 60])
+    n = ctx.n
+    limbs = ctx.limbs_of(1)
+    for name in ("max", "alt", "one"):
+        data = np.zeros((2, limbs, n), dtype=np.uint64)
+        for i in range(limbs):
+            q = ctx.moduli[i]
+            if name == "max":
+                data[:, i] = q - 1
+            elif name == "alt":
+                data[:, i, ::2] = q - 1
+            else:
+                data[:, i, n // 3] = q - 1
+        ct = nx.Ciphertext.from_array(ctx, data, 1, 1.0, ntt_form=False)
+        fwd = nx.transform_to_ntt(ct).to_array()
+        inv = nx.transform_from_ntt(nx.Ciphertext.from_array(ctx, data, 1, 1.0, ntt_form=True)).to_array()
+        for i in range(limbs):
+            q = ctx.moduli[i]
+            assert np.array_equal(fwd[0, i], orc.ntt_forward(data[0, i], logn, q)), (name, i)
+            assert np.array_equal(inv[0, i], orc.ntt_inverse(data[0, i], logn, q)), (name, i)
+
+
 @pytest.mark.parametrize("logn", [12, 16])
 def test_inverse_ntt_parity(logn):
     _note("intt logn=%d" % logn)
