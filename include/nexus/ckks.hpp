@@ -207,6 +207,7 @@ public:
 
     void mod_switch_to_next_inplace(Ciphertext &ct) const;
     void mod_switch_to_inplace(Ciphertext &ct, size_t chain_index) const;
+    void mod_switch_to(const Ciphertext &ct, size_t chain_index, Ciphertext &dest) const;
     void mod_switch_to_inplace(Plaintext &pt, size_t chain_index) const;
     void rescale_to_next_inplace(Ciphertext &ct) const;
     void relinearize_inplace(Ciphertext &ct, const RelinKeys &relin_keys) const;

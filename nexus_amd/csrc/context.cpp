@@ -408,4 +408,22 @@ void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly
     ntt_inverse(c.tables, make_batch(data, polys, limbs, poly_stride), c.stream);
 }
 
+void ntt_fwd_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, size_t limbs, size_t dst_stride,
+                  size_t src_stride)
+{
+    NttBatch b = make_batch(dst, polys, limbs, dst_stride);
+    b.src = src;
+    b.src_poly_stride = src_stride;
+    ntt_forward(c.tables, b, c.stream);
+}
+
+void ntt_inv_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, size_t limbs, size_t dst_stride,
+                  size_t src_stride)
+{
+    NttBatch b = make_batch(dst, polys, limbs, dst_stride);
+    b.src = src;
+    b.src_poly_stride = src_stride;
+    ntt_inverse(c.tables, b, c.stream);
+}
+
 }  // namespace nx

@@ -224,6 +224,11 @@ private:
 // NTT of a [polys][limbs][n] buffer whose limb i uses prime i.
 void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
 void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
+// out of place: src (poly stride src_stride) -> dst (poly stride dst_stride)
+void ntt_fwd_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, size_t limbs, size_t dst_stride,
+                  size_t src_stride);
+void ntt_inv_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, size_t limbs, size_t dst_stride,
+                  size_t src_stride);
 
 void ct_resize(Ciphertext &ct, Context *c, size_t size, size_t limbs);
 void ct_copy(Ciphertext &dst, const Ciphertext &src);
@@ -267,11 +272,14 @@ void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const D
 // [B][2][l][n]: accumulated (zero_c1 false) or c0 accumulated and c1 written.
 void ks_out_primes(const Context &c, size_t l, uint8_t *outp);
 DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l);
-void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1);
+// base (optional, [B][2][l][n]): the accumulated terms are read from base instead of cts.
+void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1,
+                const uint64_t *base = nullptr);
 // B key switches in one pass: targets [B][l][n] into cts [B][2][l][n]; with
-// zero_c1 the c1 of each result is written (not accumulated).
+// zero_c1 the c1 of each result is written (not accumulated); with base, cts =
+// base + key switch.
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
-                      bool zero_c1);
+                      bool zero_c1, const uint64_t *base = nullptr);
 
 // A batch of B ciphertexts of one shape (size 2) stored back to back: [B][2][limbs][n].
 struct CtBatch {

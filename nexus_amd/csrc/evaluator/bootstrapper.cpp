@@ -376,10 +376,14 @@ int Bootstrapper::depth() const
 
 void Bootstrapper::mod_raise(const Ciphertext &in, Ciphertext &out) const
 {
-    Ciphertext low = in;
-    if (low.coeff_modulus_size() > 1) ck_.evaluator.mod_switch_to_inplace(low, ck_.context->data_primes());
+    Ciphertext low;
+    const Ciphertext *src = &in;
+    if (in.coeff_modulus_size() > 1) {
+        ck_.evaluator.mod_switch_to(in, ck_.context->data_primes(), low);
+        src = &low;
+    }
     Ciphertext r(*ck_.context);
-    check(nx_mod_raise(low.get(), r.get()));
+    check(nx_mod_raise(src->get(), r.get()));
     // ModRaise leaves m + q_0 I: read it at scale q_0 (Bootstrapper.cu:2997-2998)
     r.set_scale(prime(1));
     out = std::move(r);
@@ -401,12 +405,16 @@ Ciphertext Bootstrapper::match(const Ciphertext &ct, size_t level, double scale)
     // the scale that the following rescale compensates
     const size_t l = ct.coeff_modulus_size();
     if (l <= level) throw std::logic_error("Bootstrapper: operand below the target level");
-    Ciphertext t = ct;
-    if (l > level + 1) ck_.evaluator.mod_switch_to_inplace(t, chain_of(level + 1));
-    const double sigma = scale * prime(level + 1) / t.scale();
+    Ciphertext t;
+    const Ciphertext *src = &ct;
+    if (l > level + 1) {
+        ck_.evaluator.mod_switch_to(ct, chain_of(level + 1), t);
+        src = &t;
+    }
+    const double sigma = scale * prime(level + 1) / src->scale();
     Plaintext one(*ck_.context);
     ck_.encoder.encode(1.0, chain_of(level + 1), sigma, one);
-    ck_.evaluator.multiply_plain_inplace(t, one);
+    ck_.evaluator.multiply_plain(*src, one, t);
     ck_.evaluator.rescale_to_next_inplace(t);
     t.set_scale(scale);
     return t;
@@ -417,14 +425,21 @@ Ciphertext Bootstrapper::mul_exact(const Ciphertext &a, const Ciphertext &b)
     // product of two ciphertexts at the lower of their levels, relinearised and rescaled
     const size_t la = a.coeff_modulus_size(), lb = b.coeff_modulus_size();
     const size_t l = std::min(la, lb);
-    Ciphertext x = a, y = b;
-    if (la > l) ck_.evaluator.mod_switch_to_inplace(x, chain_of(l));
-    if (lb > l) ck_.evaluator.mod_switch_to_inplace(y, chain_of(l));
+    Ciphertext xs, ys;
+    const Ciphertext *x = &a, *y = &b;
+    if (la > l) {
+        ck_.evaluator.mod_switch_to(a, chain_of(l), xs);
+        x = &xs;
+    }
+    if (lb > l) {
+        ck_.evaluator.mod_switch_to(b, chain_of(l), ys);
+        y = &ys;
+    }
     Ciphertext r;
     if (&a == &b)
-        ck_.evaluator.square(x, r);
+        ck_.evaluator.square(*x, r);
     else
-        ck_.evaluator.multiply(x, y, r);
+        ck_.evaluator.multiply(*x, *y, r);
     ck_.evaluator.relinearize_inplace(r, *ck_.relin_keys);
     ck_.evaluator.rescale_to_next_inplace(r);
     return r;
@@ -472,12 +487,17 @@ Ciphertext Bootstrapper::cheb_eval(const std::vector<double> &c, size_t level, d
         bool have = false;
         for (int k = 1; k <= d; k++) {
             if (c[k] == 0.0) continue;
-            Ciphertext t = basis(k);
-            if (t.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: basis below the leaf level");
-            if (t.coeff_modulus_size() > lam) ck_.evaluator.mod_switch_to_inplace(t, chain_of(lam));
+            const Ciphertext &bk = basis(k);
+            if (bk.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: basis below the leaf level");
+            Ciphertext t;
+            const Ciphertext *src = &bk;
+            if (bk.coeff_modulus_size() > lam) {
+                ck_.evaluator.mod_switch_to(bk, chain_of(lam), t);
+                src = &t;
+            }
             Plaintext ck(*ck_.context);
-            ck_.encoder.encode(c[k], chain_of(lam), scale * q / t.scale(), ck);
-            ck_.evaluator.multiply_plain_inplace(t, ck);
+            ck_.encoder.encode(c[k], chain_of(lam), scale * q / src->scale(), ck);
+            ck_.evaluator.multiply_plain(*src, ck, t);
             if (!have) {
                 acc = std::move(t);
                 have = true;
@@ -510,10 +530,14 @@ Ciphertext Bootstrapper::cheb_eval(const std::vector<double> &c, size_t level, d
     const size_t lam = level + 1;
     if (TP.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: giant step below the product level");
     Ciphertext Q = cheb_eval(q, lam, scale * prime(lam) / TP.scale());
-    Ciphertext T = TP;
-    if (T.coeff_modulus_size() > lam) ck_.evaluator.mod_switch_to_inplace(T, chain_of(lam));
+    Ciphertext Tlow;
+    const Ciphertext *T = &TP;
+    if (TP.coeff_modulus_size() > lam) {
+        ck_.evaluator.mod_switch_to(TP, chain_of(lam), Tlow);
+        T = &Tlow;
+    }
     Ciphertext prod;
-    ck_.evaluator.multiply(Q, T, prod);
+    ck_.evaluator.multiply(Q, *T, prod);
     ck_.evaluator.relinearize_inplace(prod, *ck_.relin_keys);
     ck_.evaluator.rescale_to_next_inplace(prod);
     prod.set_scale(scale);

@@ -303,6 +303,12 @@ void Evaluator::mod_switch_to_inplace(Ciphertext &ct, size_t chain_index) const
 {
     check(nx_mod_switch_to(ct.get(), chain_index, ct.get()));
 }
+void Evaluator::mod_switch_to(const Ciphertext &ct, size_t chain_index, Ciphertext &dest) const
+{
+    Ciphertext out(*ctx_);
+    check(nx_mod_switch_to(ct.get(), chain_index, out.get()));
+    dest = std::move(out);
+}
 void Evaluator::mod_switch_to_inplace(Plaintext &pt, size_t chain_index) const
 {
     check(nx_plain_mod_switch_to(pt.get(), chain_index, pt.get()));

@@ -29,6 +29,8 @@ struct DeviceTables {
 // prime_of[i] (prime_by_poly = 0) or prime_of[p'] (prime_by_poly = 1) with
 // p' = p, or p mod poly_period when poly_period > 0 (batches of ciphertexts
 // that each carry the same per-poly prime pattern).
+// With src set, the transform reads its input from src (limb i of poly p at
+// src + p * src_poly_stride + i * n) and writes data: out of place, src unchanged.
 struct NttBatch {
     uint64_t *data;
     uint64_t poly_stride;
@@ -37,6 +39,8 @@ struct NttBatch {
     uint32_t prime_by_poly;
     uint32_t poly_period = 0;
     uint8_t prime_of[kMaxPrimes];
+    const uint64_t *src = nullptr;
+    uint64_t src_poly_stride = 0;
 };
 
 // Forward negacyclic NTT: natural order in, SEAL bit-reversed order out, [0, q).
@@ -57,6 +61,9 @@ void poly_sub(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uin
               uint32_t limbs, hipStream_t s);
 void poly_negate(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t polys, uint32_t limbs,
                  hipStream_t s);
+// out = a (polys polynomials) with pt[limbs][n] added to / subtracted from the first
+void poly_add_plain_first(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *pt, uint32_t polys,
+                          uint32_t limbs, bool sub, hipStream_t s);
 // out[p][i] = a[p][i] * pt[i]
 void poly_mul_plain(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *pt, uint32_t polys,
                     uint32_t limbs, hipStream_t s);
@@ -86,9 +93,11 @@ void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last,
                      uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s);
 // accumulate: 0 = write, 1 = add into out, 2 = add into even polys / write odd polys
 // (a batch of key-switched ciphertexts whose c1 was cleared by the automorphism).
+// With base set, the added term is read from base (same layout as out, poly
+// stride base_stride) instead of out.
 void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
                  const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
-                 hipStream_t s);
+                 hipStream_t s, const uint64_t *base = nullptr, uint64_t base_stride = 0);
 
 // Key-switch ModUp over a batch of B targets: for ciphertext b, every output
 // prime I in out_primes (count = nout) and digit j < l,

@@ -41,6 +41,8 @@ enum Role { kWhole = 0, kCol = 1, kRow = 2 };
 struct NttKArgs {
     uint64_t *data;
     uint64_t poly_stride;
+    const uint64_t *src;  // input of this pass (null: in place)
+    uint64_t src_poly_stride;
     uint32_t limbs;
     uint32_t logn;
     uint32_t prime_by_poly;
@@ -304,8 +306,9 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
 // The whole transform body of one tile in double precision: load (reduced),
 // the radix steps, store in [0, q) (times n^-1 on the inverse's last pass).
 template <int LOGS, int ROLE, bool FWD, int CW>
-__device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_words, uint64_t *data, uint32_t prime,
-                                           uint32_t n, uint32_t N2, uint32_t N1, uint32_t first)
+__device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_words, uint64_t *data,
+                                           const uint64_t *src, uint32_t prime, uint32_t n, uint32_t N2, uint32_t N1,
+                                           uint32_t first)
 {
     using G = Geometry<LOGS, ROLE, CW>;
     double *lds = reinterpret_cast<double *>(lds_words);
@@ -322,7 +325,7 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
         } else {
             gidx = e;
         }
-        lds[e] = reduce_f((double)data[gidx], q, qinv);
+        lds[e] = reduce_f((double)src[gidx], q, qinv);
     }
     __syncthreads();
     if (FWD) {
@@ -392,13 +395,14 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
         a.prime_by_poly ? a.prime_of[a.poly_period ? poly % a.poly_period : poly] : a.prime_of[limb];
     const uint32_t n = 1u << a.logn;
     uint64_t *data = a.data + poly * a.poly_stride + (uint64_t)limb * n;
+    const uint64_t *src = a.src ? a.src + poly * a.src_poly_stride + (uint64_t)limb * n : data;
     const ulonglong2 *tw = a.tw + (uint64_t)prime * n;
     const uint64_t q = a.mod[prime].q;
     const uint32_t N2 = ROLE == kCol ? (n >> LOGS) : (ROLE == kRow ? (uint32_t)G::S : n);
     const uint32_t N1 = n / N2;
     const uint32_t first = blockIdx.x * G::C;  // first column / row of the tile
     if (q < kFpPrimeBound) {
-        ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, prime, n, N2, N1, first);
+        ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, src, prime, n, N2, N1, first);
         return;
     }
 
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
         } else {
             gidx = e;
         }
-        lds[e] = data[gidx];
+        lds[e] = src[gidx];
     }
     __syncthreads();
 
@@ -553,6 +557,8 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
     NttKArgs a;
     a.data = b.data;
     a.poly_stride = b.poly_stride;
+    a.src = b.src;
+    a.src_poly_stride = b.src_poly_stride;
     a.limbs = b.limbs;
     a.logn = (uint32_t)t.logn;
     a.prime_by_poly = b.prime_by_poly;
@@ -601,6 +607,7 @@ void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
         int l1, l2;
         split_logn(t.logn, l1, l2);
         launch_pass<kCol, true>(l1, a, batch, s);
+        a.src = nullptr;  // the second pass runs in place on the output
         launch_pass<kRow, true>(l2, a, batch, s);
     }
 }
@@ -616,6 +623,7 @@ void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s)
         int l1, l2;
         split_logn(t.logn, l1, l2);
         launch_pass<kRow, false>(l2, a, batch, s);
+        a.src = nullptr;
         launch_pass<kCol, false>(l1, a, batch, s);
     }
 }

@@ -39,6 +39,23 @@ __global__ void k_add(uint64_t *out, const uint64_t *a, const uint64_t *b, const
     out[idx] = add_mod(a[idx], b[idx], q);
 }
 
+// out = a with pt added to (or subtracted from) its first polynomial; the
+// other polynomials are copied in the same launch
+__global__ void k_add_plain_first(uint64_t *out, const uint64_t *a, const uint64_t *pt, const ModConst *mod,
+                                  uint32_t limbs, uint32_t n, int sub)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y, limb = row % limbs;
+    const uint64_t idx = (uint64_t)row * n + c;
+    uint64_t v = a[idx];
+    if (row < limbs) {
+        const uint64_t q = mod[limb].q, p = pt[idx];
+        v = sub ? sub_mod(v, p, q) : add_mod(v, p, q);
+    }
+    out[idx] = v;
+}
+
 __global__ void k_sub(uint64_t *out, const uint64_t *a, const uint64_t *b, const ModConst *mod, uint32_t limbs,
                       uint32_t n)
 {
@@ -147,7 +164,7 @@ __global__ void k_center_last(uint64_t *tmp, const uint64_t *last, uint64_t last
 
 __global__ void k_sub_mul(uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
                           const uint64_t *tmp, const ulonglong2 *inv, const ModConst *mod, uint32_t limbs, uint32_t n,
-                          int accumulate)
+                          int accumulate, const uint64_t *base, uint64_t base_stride)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -159,7 +176,8 @@ __global__ void k_sub_mul(uint64_t *out, uint64_t out_stride, const uint64_t *x,
     const uint64_t d = sub_mod(xv, tmp[(uint64_t)row * n + c], q);
     uint64_t r = shoup(d, w.x, w.y, q);
     uint64_t *o = out + poly * out_stride + (uint64_t)limb * n + c;
-    if (accumulate == 1 || (accumulate == 2 && (poly & 1) == 0)) r = add_mod(*o, r, q);
+    if (accumulate == 1 || (accumulate == 2 && (poly & 1) == 0))
+        r = add_mod(base ? base[poly * base_stride + (uint64_t)limb * n + c] : *o, r, q);
     *o = r;
 }
 
@@ -302,6 +320,14 @@ void poly_sub(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uin
                        1u << t.logn);
 }
 
+void poly_add_plain_first(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *pt, uint32_t polys,
+                          uint32_t limbs, bool sub, hipStream_t s)
+{
+    ProfScope prof("add", s, 8.0 * (1u << t.logn) * limbs * (2.0 * polys + 1.0));
+    hipLaunchKernelGGL(k_add_plain_first, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, a, pt, t.mod,
+                       limbs, 1u << t.logn, sub ? 1 : 0);
+}
+
 void poly_negate(const DeviceTables &t, uint64_t *out, const uint64_t *a, uint32_t polys, uint32_t limbs,
                  hipStream_t s)
 {
@@ -365,11 +391,11 @@ void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last,
 
 void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
                  const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
-                 hipStream_t s)
+                 hipStream_t s, const uint64_t *base, uint64_t base_stride)
 {
     ProfScope prof("rns_sub_mul", s, 8.0 * (1u << t.logn) * polys * limbs * (3 + (accumulate ? 1 : 0)));
     hipLaunchKernelGGL(k_sub_mul, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, out_stride, x, x_stride,
-                       tmp, inv, t.mod, limbs, 1u << t.logn, accumulate);
+                       tmp, inv, t.mod, limbs, 1u << t.logn, accumulate, base, base_stride);
 }
 
 void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, uint32_t B, uint32_t l,

@@ -147,11 +147,8 @@ static void add_sub_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &d
     Context &c = *a.ctx;
     const size_t pw = poly_words(c, a.limbs);
     DevBuf buf(&c, a.size * pw);
-    if (sub)
-        poly_sub(c.tables, buf.get(), a.data.get(), p.data.get(), 1, (uint32_t)a.limbs, c.stream);
-    else
-        poly_add(c.tables, buf.get(), a.data.get(), p.data.get(), 1, (uint32_t)a.limbs, c.stream);
-    d2d(c, buf.get() + pw, a.data.get() + pw, (a.size - 1) * pw);
+    poly_add_plain_first(c.tables, buf.get(), a.data.get(), p.data.get(), (uint32_t)a.size, (uint32_t)a.limbs, sub,
+                         c.stream);
     Ciphertext tmp;
     tmp.ctx = &c;
     tmp.size = a.size;
@@ -227,8 +224,7 @@ DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
     uint8_t outp[kMaxPrimes];
     ks_out_primes(c, l, outp);
     DevBuf coef(&c, B * l * n);
-    d2d(c, coef.get(), targets, B * l * n);
-    ntt_inv(c, coef.get(), B, l, l * n);
+    ntt_inv_from(c, coef.get(), targets, B, l, l * n, l * n);
     DevBuf T(&c, B * nout * digits * n);
     if (c.alpha == 1)
         ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
@@ -251,7 +247,7 @@ DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
 }
 
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
-                      bool zero_c1)
+                      bool zero_c1, const uint64_t *base)
 {
     const size_t n = c.n, nout = l + c.alpha;
     uint8_t outp[kMaxPrimes];
@@ -261,10 +257,10 @@ void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t
     DevBuf acc(&c, B * 2 * nout * n);
     ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)B, (uint32_t)c.digits(l),
                      (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
-    ks_moddown(c, cts, acc.get(), B, l, zero_c1);
+    ks_moddown(c, cts, acc.get(), B, l, zero_c1, base);
 }
 
-void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1)
+void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1, const uint64_t *base)
 {
     const size_t n = c.n, L = c.L, alpha = c.alpha;
     const size_t nout = l + alpha;
@@ -292,7 +288,7 @@ void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bo
                            (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
     ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
     rns_sub_mul(c.tables, cts, l * n, acc, nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
-                zero_c1 ? 2 : 1, c.stream);
+                zero_c1 ? 2 : 1, c.stream, base, l * n);
 }
 
 static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
@@ -328,8 +324,8 @@ void op_relinearize(const Ciphertext &a, const KSwitchKeys &rk, Ciphertext &dst)
     out.scale = a.scale;
     out.ntt = true;
     out.data.reset(&c, 2 * pw);
-    d2d(c, out.data.get(), a.data.get(), 2 * pw);
-    key_switch_impl(c, out, a.data.get() + 2 * pw, key, false);
+    // out = (c0, c1) + key_switch(c2): c0, c1 are read where they lie
+    key_switch_batch(c, out.data.get(), a.data.get() + 2 * pw, 1, a.limbs, key, false, a.data.get());
     dst = std::move(out);
 }
 
@@ -410,8 +406,8 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
     Context &c = *a.ctx;
     const size_t l = a.limbs, n = c.n, P = a.size;
     DevBuf last(&c, P * n);
-    copy_limbs(c, last.get(), n, a.data.get() + (l - 1) * n, l * n, P, 1);
     {
+        // INTT of the last limb of every poly, read in place
         NttBatch b;
         b.data = last.get();
         b.poly_stride = n;
@@ -419,6 +415,8 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
         b.limbs = 1;
         b.prime_by_poly = 0;
         for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)(l - 1);
+        b.src = a.data.get() + (l - 1) * n;
+        b.src_poly_stride = l * n;
         ntt_inverse(c.tables, b, c.stream);
     }
     DevBuf tmp(&c, P * (l - 1) * n);
@@ -504,9 +502,15 @@ void op_to_ntt(const Ciphertext &a, Ciphertext &dst)
 {
     valid(a);
     require(!a.ntt, "encrypted is already in NTT form");
+    Context &c = *a.ctx;
+    const size_t pw = poly_words(c, a.limbs);
     Ciphertext r;
-    ct_copy(r, a);
-    ntt_fwd(*a.ctx, r.data.get(), r.size, r.limbs, r.limbs * a.ctx->n);
+    r.ctx = &c;
+    r.size = a.size;
+    r.limbs = a.limbs;
+    r.scale = a.scale;
+    r.data.reset(&c, a.size * pw);
+    ntt_fwd_from(c, r.data.get(), a.data.get(), a.size, a.limbs, pw, pw);
     r.ntt = true;
     dst = std::move(r);
 }
@@ -515,9 +519,15 @@ void op_from_ntt(const Ciphertext &a, Ciphertext &dst)
 {
     valid(a);
     require(a.ntt, "encrypted is not in NTT form");
+    Context &c = *a.ctx;
+    const size_t pw = poly_words(c, a.limbs);
     Ciphertext r;
-    ct_copy(r, a);
-    ntt_inv(*a.ctx, r.data.get(), r.size, r.limbs, r.limbs * a.ctx->n);
+    r.ctx = &c;
+    r.size = a.size;
+    r.limbs = a.limbs;
+    r.scale = a.scale;
+    r.data.reset(&c, a.size * pw);
+    ntt_inv_from(c, r.data.get(), a.data.get(), a.size, a.limbs, pw, pw);
     r.ntt = false;
     dst = std::move(r);
 }
@@ -531,8 +541,10 @@ void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &d
     const uint32_t k = (uint32_t)(((exponent % m) + m) % m);
     const size_t pw = poly_words(c, a.limbs);
     DevBuf coef(&c, a.size * pw);
-    d2d(c, coef.get(), a.data.get(), a.size * pw);
-    if (a.ntt) ntt_inv(c, coef.get(), a.size, a.limbs, pw);
+    if (a.ntt)
+        ntt_inv_from(c, coef.get(), a.data.get(), a.size, a.limbs, pw, pw);
+    else
+        d2d(c, coef.get(), a.data.get(), a.size * pw);
     DevBuf out(&c, a.size * pw);
     poly_negacyclic_shift(c.tables, out.get(), coef.get(), k, (uint32_t)a.size, (uint32_t)a.limbs, c.stream);
     if (a.ntt) ntt_fwd(c, out.get(), a.size, a.limbs, pw);
@@ -555,8 +567,10 @@ void op_mod_raise(const Ciphertext &a, Ciphertext &dst)
     Context &c = *a.ctx;
     const size_t n = c.n, L = c.L;
     DevBuf low(&c, 2 * n);
-    d2d(c, low.get(), a.data.get(), 2 * n);
-    if (a.ntt) ntt_inv(c, low.get(), 2, 1, n);
+    if (a.ntt)
+        ntt_inv_from(c, low.get(), a.data.get(), 2, 1, n, n);
+    else
+        d2d(c, low.get(), a.data.get(), 2 * n);
     DevBuf out(&c, 2 * L * n);
     mod_raise(c.tables, out.get(), low.get(), 2, (uint32_t)L, c.stream);
     ntt_fwd(c, out.get(), 2, L, L * n);
@@ -614,8 +628,12 @@ void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst)
 {
     require(!in.empty(), "encrypteds cannot be empty");
     Ciphertext acc;
-    ct_copy(acc, *in[0]);
-    for (size_t i = 1; i < in.size(); i++) op_add(acc, *in[i], acc);
+    if (in.size() == 1) {
+        ct_copy(acc, *in[0]);
+    } else {
+        op_add(*in[0], *in[1], acc);
+        for (size_t i = 2; i < in.size(); i++) op_add(acc, *in[i], acc);
+    }
     dst = std::move(acc);
 }
 
