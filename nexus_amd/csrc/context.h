@@ -272,6 +272,12 @@ void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const D
 // [B][2][l][n]: accumulated (zero_c1 false) or c0 accumulated and c1 written.
 void ks_out_primes(const Context &c, size_t l, uint8_t *outp);
 DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l);
+// ModUp of B <= kNttSrcMax targets given by pointer ([l][n] each)
+DevBuf ks_modup_list(Context &c, const uint64_t *const *targets, size_t B, size_t l);
+// ModDown up to the final combination: INTT of the special limbs of acc
+// [B][2][l+alpha][n], conversion, NTT; returns tmp [B][2][l][n] for
+// out = (acc - tmp) P^-1.
+DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l);
 // base (optional, [B][2][l][n]): the accumulated terms are read from base instead of cts.
 void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1,
                 const uint64_t *base = nullptr);

@@ -64,12 +64,9 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
             ks_inner_product_rot(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
                                  (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
                                  c.stream);
-            // results [cnt][2][l][n]: c0 = sigma(c0) + ks0, c1 = ks1
-            DevBuf res(&c, cnt * 2 * pw);
-            for (size_t r = 0; r < cnt; r++)
-                poly_permute(res.get() + r * 2 * pw, a.data.get(), p.perms[base + r], 1, (uint32_t)l, c.logn,
-                             c.stream);
-            ks_moddown(c, res.get(), acc.get(), cnt, l, true);
+            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l);
+            // each result written in place: c0 = sigma(c0) + ks0, c1 = ks1
+            std::vector<uint64_t *> outs(cnt);
             for (size_t r = 0; r < cnt; r++) {
                 Ciphertext &o = out[p.index[base + r]];
                 o.ctx = &c;
@@ -78,9 +75,10 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
                 o.scale = a.scale;
                 o.ntt = true;
                 o.data.reset(&c, 2 * pw);
-                NX_HIP(hipMemcpyAsync(o.data.get(), res.get() + r * 2 * pw, 2 * pw * 8, hipMemcpyDeviceToDevice,
-                                      c.stream));
+                outs[r] = o.data.get();
             }
+            sub_mul_rot(c.tables, outs.data(), a.data.get(), p.perms.data() + base, acc.get(), nout * n, tmp.get(),
+                        c.d_p_inv, (uint32_t)cnt, (uint32_t)l, c.stream);
         }
     }
     dst = std::move(out);
@@ -135,17 +133,17 @@ void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwit
 void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<int> &steps, const KSwitchKeys &gk,
                    Ciphertext &dst)
 {
-    // sum_i rot(in_i, steps_i) with one ModDown: each input's ModUp feeds the
-    // inner product with its rotation key into one accumulator in the
-    // extended basis (the ModDown is linear), the sigma_i(c0_i) are summed in
-    // the data basis, and the accumulator is brought down once.  The result
-    // decrypts like the sum of the separate rotations with one rounding
-    // instead of one per term.
+    // sum_i rot(in_i, steps_i) with one ModDown: the ModUps of all rotated
+    // inputs run as one batch, one inner product reads each input's ModUp
+    // with its rotation key into one accumulator in the extended basis (the
+    // ModDown is linear), one kernel sums the sigma_i(c0_i) in the data basis,
+    // and the accumulator is brought down once.  The result decrypts like the
+    // sum of the separate rotations with one rounding instead of one per term.
     require(!in.empty() && in.size() == steps.size(), "inputs and steps must match");
     const Ciphertext &a0 = *in[0];
     check_input(a0, gk);
     Context &c = *a0.ctx;
-    const size_t n = c.n, l = a0.limbs, nout = l + c.alpha, pw = l * n;
+    const size_t n = c.n, l = a0.limbs, nout = l + c.alpha, pw = l * n, digits = c.digits(l);
     for (const Ciphertext *a : in) {
         check_input(*a, gk);
         require(a->ctx == &c && a->limbs == l, "inputs must share the level");
@@ -158,31 +156,43 @@ void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<
     out.scale = a0.scale;
     out.ntt = true;
     out.data.reset(&c, 2 * pw);
-    {
-        std::vector<uint64_t> zeros(l, 0);
-        poly_fill_limbs(out.data.get(), zeros.data(), 2, (uint32_t)l, c.logn, c.stream);
-    }
-    uint8_t outp[kMaxPrimes];
-    ks_out_primes(c, l, outp);
-    DevBuf acc, t(&c, pw);
-    bool have_acc = false;
+    // data-basis part: sigma_i(c0_i), and (c0, c1) of the identity terms
+    std::vector<const uint64_t *> ins, c1s, keys;
+    std::vector<const uint32_t *> perms, rperms;
     for (size_t i = 0; i < in.size(); i++) {
-        const Ciphertext &a = *in[i];
+        ins.push_back(in[i]->data.get());
         if (steps[i] == 0) {
-            poly_add(c.tables, out.data.get(), out.data.get(), a.data.get(), 2, (uint32_t)l, c.stream);
+            perms.push_back(nullptr);
             continue;
         }
         Prepared p = prepare(c, std::vector<int>{steps[i]}, gk);
-        poly_permute(t.get(), a.data.get(), p.perms[0], 1, (uint32_t)l, c.logn, c.stream);
-        poly_add(c.tables, out.data.get(), out.data.get(), t.get(), 1, (uint32_t)l, c.stream);
-        DevBuf T = ks_modup(c, a.data.get() + pw, 1, l);
-        if (!have_acc) acc.reset(&c, 2 * nout * n);
-        ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), p.keys.data(), p.perms.data(), 1,
-                                 (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout, have_acc,
-                                 c.stream);
-        have_acc = true;
+        perms.push_back(p.perms[0]);
+        rperms.push_back(p.perms[0]);
+        keys.push_back(p.keys[0]);
+        c1s.push_back(in[i]->data.get() + pw);
     }
-    if (have_acc) ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
+    {
+        DevBuf part;
+        for (size_t base = 0; base < ins.size(); base += kMaxHoist) {
+            const size_t cnt = std::min<size_t>(kMaxHoist, ins.size() - base);
+            uint64_t *o = base ? (part.reset(&c, 2 * pw), part.get()) : out.data.get();
+            rot_base(c.tables, o, ins.data() + base, perms.data() + base, (uint32_t)cnt, (uint32_t)l, c.stream);
+            if (base) poly_add(c.tables, out.data.get(), out.data.get(), o, 2, (uint32_t)l, c.stream);
+        }
+    }
+    if (!keys.empty()) {
+        uint8_t outp[kMaxPrimes];
+        ks_out_primes(c, l, outp);
+        DevBuf acc(&c, 2 * nout * n);
+        for (size_t base = 0; base < keys.size(); base += kMaxHoist) {
+            const size_t cnt = std::min<size_t>(kMaxHoist, keys.size() - base);
+            DevBuf T = ks_modup_list(c, c1s.data() + base, cnt, l);
+            ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), keys.data() + base, rperms.data() + base,
+                                     (uint32_t)cnt, (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
+                                     base > 0, c.stream, nout * digits * n);
+        }
+        ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
+    }
     dst = std::move(out);
 }
 

@@ -41,7 +41,11 @@ struct NttBatch {
     uint8_t prime_of[kMaxPrimes];
     const uint64_t *src = nullptr;
     uint64_t src_poly_stride = 0;
+    // or one source per poly (host array of polys <= kNttSrcMax device pointers;
+    // limb i of poly p at src_list[p] + i * n)
+    const uint64_t *const *src_list = nullptr;
 };
+constexpr int kNttSrcMax = 32;
 
 // Forward negacyclic NTT: natural order in, SEAL bit-reversed order out, [0, q).
 void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s);
@@ -175,9 +179,30 @@ constexpr int kMaxHoist = 32;
 void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                           const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
                           const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+// t_stride > 0: rotation r reads its own ModUp at T + r t_stride (a sum of
+// rotations of different inputs) instead of the shared T.
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                               const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
-                              const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s);
+                              const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s,
+                              uint64_t t_stride = 0);
+// out[2][l][n] = sum_i (sigma_i(in_i[0]), 0), or (in_i[0], in_i[1]) where
+// perms[i] is null (count <= kMaxHoist; in_i are [2][l][n] ciphertexts).
+struct RotBaseArgs {
+    const uint64_t *in[kMaxHoist];
+    const uint32_t *perm[kMaxHoist];
+};
+void rot_base(const DeviceTables &t, uint64_t *out, const uint64_t *const *in, const uint32_t *const *perms,
+              uint32_t count, uint32_t l, hipStream_t s);
+// ModDown tail of `count` hoisted rotations into separate results:
+// outs[r][0] = sigma_r(a0) + (x[r][0] - tmp[r][0]) inv, outs[r][1] = (x[r][1] -
+// tmp[r][1]) inv; x poly stride x_stride, tmp [count][2][l][n].
+struct RotOutArgs {
+    uint64_t *out[kMaxHoist];
+    const uint32_t *perm[kMaxHoist];
+};
+void sub_mul_rot(const DeviceTables &t, uint64_t *const *outs, const uint64_t *a0, const uint32_t *const *perms,
+                 const uint64_t *x, uint64_t x_stride, const uint64_t *tmp, const ulonglong2 *inv, uint32_t count,
+                 uint32_t l, hipStream_t s);
 
 // ---------------------------------------------------------------- keys
 // Deterministic counter-based sampling (not SEAL's PRNG).

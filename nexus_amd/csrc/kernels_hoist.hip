@@ -58,7 +58,7 @@ __global__ void k_ks_inner_rot(uint64_t *acc, const uint64_t *T, RotArgs args, c
 // every product is below 2^120, so the 128-bit sums are folded every 128 terms.
 __global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs args, uint32_t count,
                                    const ModConst *mod, OutPrimes outp, uint32_t l, uint32_t key_limbs,
-                                   uint32_t nout, uint32_t n, int accumulate)
+                                   uint32_t nout, uint32_t n, int accumulate, uint64_t t_stride)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -75,7 +75,7 @@ __global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs arg
     uint32_t terms = 0;
     for (uint32_t r = 0; r < count; r++) {
         const uint32_t src = args.perm[r][c];
-        const uint64_t *t = T + (uint64_t)I * l * n + src;
+        const uint64_t *t = T + r * t_stride + (uint64_t)I * l * n + src;
         const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
         for (uint32_t j = 0; j < l; j++) {
             const uint64_t tv = t[(uint64_t)j * n];
@@ -98,6 +98,49 @@ __global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs arg
     }
     acc[(uint64_t)I * n + c] = add_mod(r0, barrett128(hi0, lo0, m), m.q);
     acc[((uint64_t)nout + I) * n + c] = add_mod(r1, barrett128(hi1, lo1, m), m.q);
+}
+
+// out[0] = sum_i sigma_i(a_i[0]) (a_i[0] where perm_i is null), out[1] = sum of
+// a_i[1] over the inputs without a permutation: the data-basis part of a sum
+// of rotations before the switched c1 terms are added.
+__global__ void k_rot_base(uint64_t *out, RotBaseArgs args, uint32_t count, const ModConst *mod, uint32_t l,
+                           uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // poly * l + limb
+    const uint32_t poly = row / l, limb = row - poly * l;
+    const uint64_t q = mod[limb].q;
+    uint64_t sum = 0;
+    for (uint32_t i = 0; i < count; i++) {
+        const uint64_t *a = args.in[i] + (uint64_t)row * n;
+        const uint32_t *perm = args.perm[i];
+        if (perm) {
+            if (poly == 0) sum = add_mod(sum, a[perm[c]], q);
+        } else {
+            sum = add_mod(sum, a[c], q);
+        }
+    }
+    out[(uint64_t)row * n + c] = sum;
+}
+
+// ModDown tail of hoisted rotations written into each result: for rotation r,
+// out_r[0] = sigma_r(a0) + (x_r[0] - tmp_r[0]) P^-1 and out_r[1] = (x_r[1] -
+// tmp_r[1]) P^-1, with x = acc[r][k][limb] (stride x_stride per poly).
+__global__ void k_sub_mul_rot(RotOutArgs args, const uint64_t *a0, const uint64_t *x, uint64_t x_stride,
+                              const uint64_t *tmp, const ulonglong2 *inv, const ModConst *mod, uint32_t l, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // (r * 2 + k) * l + limb
+    const uint32_t poly = row / l, limb = row - poly * l;
+    const uint32_t r = poly >> 1, k = poly & 1;
+    const uint64_t q = mod[limb].q;
+    const ulonglong2 w = inv[limb];
+    const uint64_t d = sub_mod(x[poly * x_stride + (uint64_t)limb * n + c], tmp[(uint64_t)row * n + c], q);
+    uint64_t v = shoup(d, w.x, w.y, q);
+    if (k == 0) v = add_mod(v, a0[(uint64_t)limb * n + args.perm[r][c]], q);
+    args.out[r][((uint64_t)k * l + limb) * n + c] = v;
 }
 
 OutPrimes make_out(const uint8_t *p, uint32_t nout)
@@ -131,13 +174,43 @@ void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *
 
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                               const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
-                              const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s)
+                              const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s,
+                              uint64_t t_stride)
 {
     const uint32_t n = 1u << t.logn;
     ProfScope prof("ks_inner_rot_sum", s, (double)n * nout * (count * (24.0 * l + 4.0) + 16.0));
     hipLaunchKernelGGL(k_ks_inner_rot_sum, dim3((n + kBlock - 1) / kBlock, nout), dim3(kBlock), 0, s, acc, T,
                        make_args(keys, perms, count), count, t.mod, make_out(out_primes_host, nout), l, key_limbs,
-                       nout, n, accumulate ? 1 : 0);
+                       nout, n, accumulate ? 1 : 0, t_stride);
+}
+
+void rot_base(const DeviceTables &t, uint64_t *out, const uint64_t *const *in, const uint32_t *const *perms,
+              uint32_t count, uint32_t l, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    RotBaseArgs a;
+    for (uint32_t i = 0; i < kMaxHoist; i++) {
+        a.in[i] = i < count ? in[i] : nullptr;
+        a.perm[i] = i < count ? perms[i] : nullptr;
+    }
+    ProfScope prof("add", s, 8.0 * n * l * (2.0 + 2.0 * count));
+    hipLaunchKernelGGL(k_rot_base, dim3((n + kBlock - 1) / kBlock, 2 * l), dim3(kBlock), 0, s, out, a, count, t.mod,
+                       l, n);
+}
+
+void sub_mul_rot(const DeviceTables &t, uint64_t *const *outs, const uint64_t *a0, const uint32_t *const *perms,
+                 const uint64_t *x, uint64_t x_stride, const uint64_t *tmp, const ulonglong2 *inv, uint32_t count,
+                 uint32_t l, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    RotOutArgs a;
+    for (uint32_t i = 0; i < kMaxHoist; i++) {
+        a.out[i] = i < count ? outs[i] : nullptr;
+        a.perm[i] = i < count ? perms[i] : nullptr;
+    }
+    ProfScope prof("rns_sub_mul", s, 8.0 * n * l * count * 7.0);
+    hipLaunchKernelGGL(k_sub_mul_rot, dim3((n + kBlock - 1) / kBlock, 2 * count * l), dim3(kBlock), 0, s, a, a0, x,
+                       x_stride, tmp, inv, t.mod, l, n);
 }
 
 }  // namespace nx

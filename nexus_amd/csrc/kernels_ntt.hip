@@ -28,6 +28,7 @@
 // 64-bit Shoup butterflies (ten 32-bit multiplies each) on [0, 4q).  This
 // file is compiled without floating-point contraction.
 #include <atomic>
+#include <cstdlib>
 
 #include "kernels.h"
 #include "prof.h"
@@ -43,6 +44,8 @@ struct NttKArgs {
     uint64_t poly_stride;
     const uint64_t *src;  // input of this pass (null: in place)
     uint64_t src_poly_stride;
+    uint32_t nsrcs;       // > 0: poly p of this pass reads srcs[p]
+    const uint64_t *srcs[kNttSrcMax];
     uint32_t limbs;
     uint32_t logn;
     uint32_t prime_by_poly;
@@ -395,7 +398,8 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
         a.prime_by_poly ? a.prime_of[a.poly_period ? poly % a.poly_period : poly] : a.prime_of[limb];
     const uint32_t n = 1u << a.logn;
     uint64_t *data = a.data + poly * a.poly_stride + (uint64_t)limb * n;
-    const uint64_t *src = a.src ? a.src + poly * a.src_poly_stride + (uint64_t)limb * n : data;
+    const uint64_t *src = a.nsrcs ? a.srcs[poly] + (uint64_t)limb * n
+                                  : (a.src ? a.src + poly * a.src_poly_stride + (uint64_t)limb * n : data);
     const ulonglong2 *tw = a.tw + (uint64_t)prime * n;
     const uint64_t q = a.mod[prime].q;
     const uint32_t N2 = ROLE == kCol ? (n >> LOGS) : (ROLE == kRow ? (uint32_t)G::S : n);
@@ -559,6 +563,14 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
     a.poly_stride = b.poly_stride;
     a.src = b.src;
     a.src_poly_stride = b.src_poly_stride;
+    a.nsrcs = 0;
+    for (int i = 0; i < kNttSrcMax; i++) a.srcs[i] = nullptr;
+    if (b.src_list) {
+        // a source per poly: the batch must fit the argument array
+        if (b.polys > (uint32_t)kNttSrcMax) abort();
+        a.nsrcs = b.polys;
+        for (uint32_t i = 0; i < b.polys; i++) a.srcs[i] = b.src_list[i];
+    }
     a.limbs = b.limbs;
     a.logn = (uint32_t)t.logn;
     a.prime_by_poly = b.prime_by_poly;
@@ -608,6 +620,7 @@ void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
         split_logn(t.logn, l1, l2);
         launch_pass<kCol, true>(l1, a, batch, s);
         a.src = nullptr;  // the second pass runs in place on the output
+        a.nsrcs = 0;
         launch_pass<kRow, true>(l2, a, batch, s);
     }
 }
@@ -624,6 +637,7 @@ void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s)
         split_logn(t.logn, l1, l2);
         launch_pass<kRow, false>(l2, a, batch, s);
         a.src = nullptr;
+        a.nsrcs = 0;
         launch_pass<kCol, false>(l1, a, batch, s);
     }
 }

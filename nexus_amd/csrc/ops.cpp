@@ -216,7 +216,8 @@ void ks_out_primes(const Context &c, size_t l, uint8_t *outp)
     for (size_t k = 0; k < c.alpha; k++) outp[l + k] = (uint8_t)(c.L + k);
 }
 
-DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
+static DevBuf ks_modup_impl(Context &c, const uint64_t *targets, const uint64_t *const *target_list, size_t B,
+                            size_t l)
 {
     // ModUp: INTT the l limbs; each digit (one prime, or alpha consecutive
     // primes in hybrid mode) extended to every output prime; NTT.
@@ -224,7 +225,19 @@ DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
     uint8_t outp[kMaxPrimes];
     ks_out_primes(c, l, outp);
     DevBuf coef(&c, B * l * n);
-    ntt_inv_from(c, coef.get(), targets, B, l, l * n, l * n);
+    if (target_list) {
+        NttBatch b;
+        b.data = coef.get();
+        b.poly_stride = l * n;
+        b.polys = (uint32_t)B;
+        b.limbs = (uint32_t)l;
+        b.prime_by_poly = 0;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)i;
+        b.src_list = target_list;
+        ntt_inverse(c.tables, b, c.stream);
+    } else {
+        ntt_inv_from(c, coef.get(), targets, B, l, l * n, l * n);
+    }
     DevBuf T(&c, B * nout * digits * n);
     if (c.alpha == 1)
         ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
@@ -246,6 +259,17 @@ DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
     return T;
 }
 
+DevBuf ks_modup(Context &c, const uint64_t *targets, size_t B, size_t l)
+{
+    return ks_modup_impl(c, targets, nullptr, B, l);
+}
+
+DevBuf ks_modup_list(Context &c, const uint64_t *const *targets, size_t B, size_t l)
+{
+    require(B >= 1 && B <= (size_t)kNttSrcMax, "too many ModUp targets in one batch");
+    return ks_modup_impl(c, nullptr, targets, B, l);
+}
+
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
                       bool zero_c1, const uint64_t *base)
 {
@@ -260,7 +284,7 @@ void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t
     ks_moddown(c, cts, acc.get(), B, l, zero_c1, base);
 }
 
-void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1, const uint64_t *base)
+DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l)
 {
     const size_t n = c.n, L = c.L, alpha = c.alpha;
     const size_t nout = l + alpha;
@@ -287,6 +311,13 @@ void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bo
         ks_moddown_convert(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv, (uint32_t)(2 * B),
                            (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
     ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
+    return tmp;
+}
+
+void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1, const uint64_t *base)
+{
+    const size_t n = c.n, nout = l + c.alpha;
+    DevBuf tmp = ks_moddown_prepare(c, acc, B, l);
     rns_sub_mul(c.tables, cts, l * n, acc, nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
                 zero_c1 ? 2 : 1, c.stream, base, l * n);
 }
