@@ -73,6 +73,12 @@ public:
     // The output holds factor x the input message (factor folded into the
     // last SlotToCoeff level: no extra level, error scaled by factor).
     void bootstrap(const Ciphertext &in, Ciphertext &out, double factor = 1.0);
+    // bootstrap of several ciphertexts (out[i] = factors[i] x in[i]'s message):
+    // ModRaise and CoeffToSlot per input, one EvalMod over every input's halves
+    // in lockstep (each multiplication, key switch and rescale one batched
+    // launch), SlotToCoeff per input
+    void bootstrap_many(const std::vector<const Ciphertext *> &in, std::vector<Ciphertext> &out,
+                        const std::vector<double> &factors);
     // Bootstrapper::set_final_scale (reference Bootstrapper.h)
     void set_final_scale(double s) { p_.final_scale = s; }
 
@@ -82,6 +88,8 @@ public:
     // sparse: one output u (real, 2n values per period); dense: re and im parts
     void coeff_to_slot(const Ciphertext &in, Ciphertext &u_re, Ciphertext &u_im);
     void eval_mod(const Ciphertext &u, Ciphertext &out);
+    // EvalMod of inputs at one level and scale, in lockstep
+    void eval_mod_many(const std::vector<const Ciphertext *> &u, std::vector<Ciphertext> &out);
     void slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im, double input_scale, Ciphertext &out,
                        double factor = 1.0);
 
@@ -91,6 +99,12 @@ private:
     const Ciphertext &basis(int k);
     Ciphertext match(const Ciphertext &ct, size_t level, double scale);
     Ciphertext mul_exact(const Ciphertext &a, const Ciphertext &b);
+    // the same steps over a batch in lockstep (one scale and level per batch)
+    using Batch = std::vector<Ciphertext>;
+    Batch cheb_eval_many(const std::vector<double> &c, size_t level, double scale);
+    const Batch &basis_many(int k);
+    Batch match_many(const Batch &ct, size_t level, double scale);
+    Batch mul_exact_many(const Batch &a, const Batch &b);
     size_t chain_of(size_t level) const { return ck_.context->data_primes() + 1 - level; }
     double prime(size_t level) const { return (double)ck_.context->moduli()[level - 1]; }
 
@@ -99,6 +113,7 @@ private:
     BootPlan plan_;
     std::vector<double> cheb_;
     std::map<int, Ciphertext> basis_;  // Chebyshev T_k of the current EvalMod input
+    std::map<int, Batch> basis_many_;  // the same for a batch of inputs
     std::map<std::tuple<int, int, int, size_t, double>, Plaintext> ptcache_;
 };
 

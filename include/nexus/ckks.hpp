@@ -222,6 +222,11 @@ public:
     // sum_r cts[r] * pts[r] in one pass
     void multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                                    Ciphertext &dest) const;
+    // batched: dest[i] = rescale(cts[i]) / rescale(relinearize(a[i] * b[i])),
+    // word for word the per-ciphertext results (dest may alias the inputs)
+    void rescale_to_next_many(const std::vector<const Ciphertext *> &cts, std::vector<Ciphertext> &dest) const;
+    void multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                                     const RelinKeys &rk, std::vector<Ciphertext> &dest) const;
     // out[m] = sum_k w[index[m K + k]] (.) cts[k] (index -1: no term), M = index.size() / K
     void multiply_periodic_mix(const std::vector<const Ciphertext *> &cts, const PeriodicPlaintexts &w,
                                const std::vector<int32_t> &index, std::vector<Ciphertext> &out) const;

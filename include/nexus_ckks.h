@@ -150,6 +150,14 @@ int nx_multiply(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *d
 int nx_square(const nx_ciphertext *a, nx_ciphertext *dest);
 int nx_relinearize(const nx_ciphertext *a, const nx_kswitch_keys *relin_keys, nx_ciphertext *dest);
 int nx_rescale_to_next(const nx_ciphertext *a, nx_ciphertext *dest);
+/* Batched forms (no counterpart in the reference, which runs these per
+ * ciphertext): dests[i] = rescale_to_next(cts[i]), and dests[i] =
+ * rescale_to_next(relinearize(a[i] * b[i])), word for word the per-ciphertext
+ * results; every stage runs as one launch over the batch.  Inputs of one
+ * level (and size); dests may alias inputs. */
+int nx_rescale_to_next_many(const nx_ciphertext *const *cts, size_t count, nx_ciphertext *const *dests);
+int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
+                                   const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests);
 int nx_mod_switch_to_next(const nx_ciphertext *a, nx_ciphertext *dest);
 int nx_mod_switch_to(const nx_ciphertext *a, size_t chain_index, nx_ciphertext *dest);
 int nx_plain_mod_switch_to(const nx_plaintext *a, size_t chain_index, nx_plaintext *dest);

@@ -62,6 +62,13 @@ int nx_bootstrap(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *ou
  * folded into the last SlotToCoeff level: no extra level; the bootstrapping
  * error is scaled by it).  No counterpart in the reference. */
 int nx_bootstrap_scaled(nx_bootstrapper *bt, const nx_ciphertext *in, double factor, nx_ciphertext *out);
+/* out[i] = bootstrap of in[i] times factors[i] (factors may be NULL: all 1)
+ * for i < count: ModRaise and CoeffToSlot per input, one EvalMod over every
+ * input in lockstep (each multiplication, key switch and rescale one batched
+ * launch), SlotToCoeff per input.  Decrypts like count nx_bootstrap_scaled
+ * calls.  No counterpart in the reference, which bootstraps one at a time. */
+int nx_bootstrap_many(nx_bootstrapper *bt, const nx_ciphertext *const *in, size_t count, const double *factors,
+                      nx_ciphertext *const *out);
 /* stages of bootstrap_sparse_3 / bootstrap_full_3 (Bootstrapper.cu:2987-3113) */
 int nx_boot_mod_raise(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
 int nx_boot_sub_sum(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);

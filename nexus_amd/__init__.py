@@ -553,6 +553,26 @@ def multiply_plain_accumulate(cts, pts, dest=None):
     return d
 
 
+_sig("nx_rescale_to_next_many", [ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
+_sig("nx_multiply_relin_rescale_many", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp, ctypes.POINTER(vp)])
+
+
+def rescale_to_next_many(cts):
+    """[rescale_to_next(c) for c in cts] as one batched pass"""
+    outs = [Ciphertext(cts[0].ctx) for _ in cts]
+    _check(_lib.nx_rescale_to_next_many((vp * len(cts))(*[c.h for c in cts]), len(cts),
+                                        (vp * len(outs))(*[o.h for o in outs])))
+    return outs
+
+
+def multiply_relin_rescale_many(a, b, relin_keys):
+    """[rescale_to_next(relinearize(x * y)) for x, y in zip(a, b)] as one batched pass"""
+    outs = [Ciphertext(a[0].ctx) for _ in a]
+    _check(_lib.nx_multiply_relin_rescale_many((vp * len(a))(*[c.h for c in a]), (vp * len(b))(*[c.h for c in b]),
+                                               len(a), relin_keys.h, (vp * len(outs))(*[o.h for o in outs])))
+    return outs
+
+
 _sig("nx_periodic_plain_create", [vp, f64p, sz, sz, sz, ctypes.c_double, ctypes.POINTER(vp)])
 _sig("nx_periodic_plain_destroy", [vp])
 _sig("nx_periodic_plain_expand", [vp, sz, vp])
@@ -685,6 +705,7 @@ _sig("nx_bootstrapper_depth", [vp, _ip])
 _sig("nx_autotune", [vp, ctypes.c_int, ctypes.c_char_p, sz, ctypes.POINTER(sz)])
 _sig("nx_bootstrap", [vp, vp, vp])
 _sig("nx_bootstrap_scaled", [vp, vp, ctypes.c_double, vp])
+_sig("nx_bootstrap_many", [vp, ctypes.POINTER(vp), sz, f64p, ctypes.POINTER(vp)])
 _sig("nx_boot_mod_raise", [vp, vp, vp])
 _sig("nx_boot_sub_sum", [vp, vp, vp])
 _sig("nx_boot_coeff_to_slot", [vp, vp, vp, vp])
@@ -786,6 +807,14 @@ class Bootstrapper:
         if factor is None:
             return self._run(_lib.nx_bootstrap, ct.h)
         return self._run(_lib.nx_bootstrap_scaled, ct.h, float(factor))
+
+    def bootstrap_many(self, cts, factors=None):
+        """[bootstrap(c, f)] for all inputs, with one EvalMod over the whole set"""
+        outs = [Ciphertext(self.ctx) for _ in cts]
+        f = (ctypes.c_double * len(cts))(*(factors if factors is not None else [1.0] * len(cts)))
+        _check(_lib.nx_bootstrap_many(self.h, (vp * len(cts))(*[c.h for c in cts]), len(cts), f,
+                                      (vp * len(outs))(*[o.h for o in outs])))
+        return outs
 
     def mod_raise(self, ct):
         return self._run(_lib.nx_boot_mod_raise, ct.h)

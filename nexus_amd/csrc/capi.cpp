@@ -556,6 +556,45 @@ NX_UN(nx_transform_to_ntt, op_to_ntt)
 NX_UN(nx_transform_from_ntt, op_from_ntt)
 #undef NX_UN
 
+int nx_rescale_to_next_many(const nx_ciphertext *const *cts, size_t count, nx_ciphertext *const *dests)
+{
+    return guard([&] {
+        need(cts, "cts");
+        need(dests, "dests");
+        std::vector<const nx::Ciphertext *> in;
+        std::vector<nx::Ciphertext *> out;
+        for (size_t i = 0; i < count; i++) {
+            need(cts[i], "cts[i]");
+            need(dests[i], "dests[i]");
+            in.push_back(&cts[i]->impl);
+            out.push_back(&dests[i]->impl);
+        }
+        nx::op_rescale_many(in, out);
+    });
+}
+
+int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
+                                   const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests)
+{
+    return guard([&] {
+        need(a, "a");
+        need(b, "b");
+        need(relin_keys, "relin_keys");
+        need(dests, "dests");
+        std::vector<const nx::Ciphertext *> x, y;
+        std::vector<nx::Ciphertext *> out;
+        for (size_t i = 0; i < count; i++) {
+            need(a[i], "a[i]");
+            need(b[i], "b[i]");
+            need(dests[i], "dests[i]");
+            x.push_back(&a[i]->impl);
+            y.push_back(&b[i]->impl);
+            out.push_back(&dests[i]->impl);
+        }
+        nx::op_multiply_relin_rescale_many(x, y, relin_keys->impl, out);
+    });
+}
+
 int nx_relinearize(const nx_ciphertext *a, const nx_kswitch_keys *relin_keys, nx_ciphertext *dest)
 {
     return guard([&] {

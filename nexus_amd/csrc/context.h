@@ -244,6 +244,10 @@ void op_multiply(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst);
 void op_square(const Ciphertext &a, Ciphertext &dst);
 void op_relinearize(const Ciphertext &a, const KSwitchKeys &rk, Ciphertext &dst);
 void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst);
+// batched forms, word for word the per-ciphertext results
+void op_rescale_many(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out);
+void op_multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                                    const KSwitchKeys &rk, const std::vector<Ciphertext *> &out);
 void op_mod_switch_to_next(const Ciphertext &a, Ciphertext &dst);
 void op_mod_switch_to(const Ciphertext &a, size_t chain_index, Ciphertext &dst);
 void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &dst);

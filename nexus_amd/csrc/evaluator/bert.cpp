@@ -377,30 +377,48 @@ void BertLayer::bootstrap_real(const std::vector<Ciphertext *> &cts, double fact
 {
     // real messages two at a time: y = a + i b (X^(N/2) multiplies every slot
     // by i), y' = bootstrap(y) factor / 2, then a' = y' + conj(y') and
-    // b' = -i (y' - conj(y')) -- one bootstrap and one conjugation per pair
+    // b' = -i (y' - conj(y')) -- one bootstrap and one conjugation per pair;
+    // all of them bootstrapped together (one EvalMod over the whole set)
     const Evaluator &ev = ck_.evaluator;
     const int64_t N = (int64_t)ck_.context->poly_degree();
+    std::vector<std::pair<Ciphertext *, Ciphertext *>> items;
+    std::vector<Ciphertext> ys;
+    ys.reserve(cts.size());
+    std::vector<const Ciphertext *> in;
+    std::vector<double> factors;
     size_t k = 0;
     while (k < cts.size()) {
         Ciphertext &a = *cts[k];
         if (k + 1 < cts.size() && cts[k + 1]->scale() == a.scale() &&
             cts[k + 1]->coeff_modulus_size() == a.coeff_modulus_size()) {
             Ciphertext &b = *cts[k + 1];
-            Ciphertext y;
-            ev.multiply_power_of_x(b, N / 2, y);
-            ev.add_inplace(y, a);
-            Ciphertext yb, yc, d;
-            bt_.bootstrap(y, yb, factor / 2);
-            ev.complex_conjugate(yb, *ck_.galois_keys, yc);
-            ev.add(yb, yc, a);
-            ev.sub(yb, yc, d);
-            ev.multiply_power_of_x(d, 3 * N / 2, b);
+            ys.emplace_back();
+            ev.multiply_power_of_x(b, N / 2, ys.back());
+            ev.add_inplace(ys.back(), a);
+            in.push_back(&ys.back());
+            factors.push_back(factor / 2);
+            items.emplace_back(&a, &b);
             k += 2;
         } else {
-            Ciphertext o;
-            bt_.bootstrap(a, o, factor);
-            a = std::move(o);
+            in.push_back(&a);
+            factors.push_back(factor);
+            items.emplace_back(&a, nullptr);
             k += 1;
+        }
+    }
+    std::vector<Ciphertext> outs;
+    bt_.bootstrap_many(in, outs, factors);
+    ys.clear();
+    for (size_t i = 0; i < items.size(); i++) {
+        Ciphertext &yb = outs[i];
+        if (items[i].second) {
+            Ciphertext yc, d;
+            ev.complex_conjugate(yb, *ck_.galois_keys, yc);
+            ev.add(yb, yc, *items[i].first);
+            ev.sub(yb, yc, d);
+            ev.multiply_power_of_x(d, 3 * N / 2, *items[i].second);
+        } else {
+            *items[i].first = std::move(yb);
         }
     }
 }
@@ -465,12 +483,11 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             const int t0 = kHeadDim * eta, lo = tlo[eta], hi = thi[eta];
             std::vector<Ciphertext> kr = token_rotations(kc, t0 + lo, hi - lo);
             mark("attention.k_rotations");
-            std::vector<Ciphertext> P(hi - lo);  // P[tau - lo]
-            for (int tau = lo; tau < hi; tau++) {
-                Ciphertext &pt = P[tau - lo];
-                ev.multiply(qc, kr[tau - lo], pt);
-                ev.relinearize_inplace(pt, rk);
-                ev.rescale_to_next_inplace(pt);
+            std::vector<Ciphertext> P;  // P[tau - lo] = q (.) rot(k, 256 tau), relinearised and rescaled
+            {
+                std::vector<const Ciphertext *> qa(hi - lo, &qc), kb;
+                for (auto &x : kr) kb.push_back(&x);
+                ev.multiply_relin_rescale_many(qa, kb, rk, P);
             }
             kr.clear();
             mark("attention.qk_products");
@@ -602,9 +619,13 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
                 ev.multiply_periodic_mix(in, offset_masks(lb, ms), index, pbs);
             }
             R.clear();
+            {
+                std::vector<const Ciphertext *> pp;
+                for (auto &x : pbs) pp.push_back(&x);
+                ev.rescale_to_next_many(pp, pbs);
+            }
             for (int tau = lo; tau < hi; tau++) {
                 Ciphertext &pb = pbs[tau - lo];
-                ev.rescale_to_next_inplace(pb);
                 Ciphertext prod;
                 ev.multiply(pb, vr[tau - lo], prod);
                 if (!have_acc) {

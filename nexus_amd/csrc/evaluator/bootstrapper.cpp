@@ -571,6 +571,201 @@ void Bootstrapper::eval_mod(const Ciphertext &u, Ciphertext &out)
     out = std::move(y);
 }
 
+// ---- EvalMod over a batch: the single-input steps above, each multiplication
+// (with its relinearisation and rescale) and each rescale one batched call.
+namespace {
+
+std::vector<const Ciphertext *> ptrs(const std::vector<Ciphertext> &v)
+{
+    std::vector<const Ciphertext *> p;
+    for (const auto &x : v) p.push_back(&x);
+    return p;
+}
+
+}  // namespace
+
+Bootstrapper::Batch Bootstrapper::match_many(const Batch &ct, size_t level, double scale)
+{
+    Batch t(ct.size());
+    Plaintext one;
+    double one_sigma = 0;
+    for (size_t i = 0; i < ct.size(); i++) {
+        const size_t l = ct[i].coeff_modulus_size();
+        if (l <= level) throw std::logic_error("Bootstrapper: operand below the target level");
+        const Ciphertext *src = &ct[i];
+        if (l > level + 1) {
+            ck_.evaluator.mod_switch_to(ct[i], chain_of(level + 1), t[i]);
+            src = &t[i];
+        }
+        const double sigma = scale * prime(level + 1) / src->scale();
+        if (!one.get() || sigma != one_sigma) {
+            one = Plaintext(*ck_.context);
+            ck_.encoder.encode(1.0, chain_of(level + 1), sigma, one);
+            one_sigma = sigma;
+        }
+        ck_.evaluator.multiply_plain(*src, one, t[i]);
+    }
+    ck_.evaluator.rescale_to_next_many(ptrs(t), t);
+    for (auto &x : t) x.set_scale(scale);
+    return t;
+}
+
+Bootstrapper::Batch Bootstrapper::mul_exact_many(const Batch &a, const Batch &b)
+{
+    Batch xs(a.size()), ys(a.size());
+    std::vector<const Ciphertext *> x, y;
+    for (size_t i = 0; i < a.size(); i++) {
+        const size_t la = a[i].coeff_modulus_size(), lb = b[i].coeff_modulus_size();
+        const size_t l = std::min(la, lb);
+        const Ciphertext *pa = &a[i], *pb = &b[i];
+        if (la > l) {
+            ck_.evaluator.mod_switch_to(a[i], chain_of(l), xs[i]);
+            pa = &xs[i];
+        }
+        if (lb > l) {
+            ck_.evaluator.mod_switch_to(b[i], chain_of(l), ys[i]);
+            pb = &ys[i];
+        }
+        x.push_back(pa);
+        y.push_back(&a == &b ? pa : pb);
+    }
+    Batch r;
+    ck_.evaluator.multiply_relin_rescale_many(x, y, *ck_.relin_keys, r);
+    return r;
+}
+
+const Bootstrapper::Batch &Bootstrapper::basis_many(int k)
+{
+    auto it = basis_many_.find(k);
+    if (it != basis_many_.end()) return it->second;
+    int a = 1;
+    while (2 * a < k) a *= 2;
+    const int b = k - a;
+    const Batch &Ta = basis_many(a);
+    const Batch &Tb = basis_many(b);
+    Batch prod = (a == b) ? mul_exact_many(Ta, Ta) : mul_exact_many(Ta, Tb);
+    for (auto &x : prod) ck_.evaluator.add_inplace(x, x);  // 2 T_a T_b
+    const size_t l = prod[0].coeff_modulus_size();
+    if (a == b) {
+        Plaintext one(*ck_.context);
+        ck_.encoder.encode(1.0, chain_of(l), prod[0].scale(), one);
+        for (auto &x : prod) ck_.evaluator.sub_plain_inplace(x, one);
+    } else {
+        Batch Td = match_many(basis_many(a - b), l, prod[0].scale());
+        for (size_t i = 0; i < prod.size(); i++) ck_.evaluator.sub_inplace(prod[i], Td[i]);
+    }
+    return basis_many_.emplace(k, std::move(prod)).first->second;
+}
+
+Bootstrapper::Batch Bootstrapper::cheb_eval_many(const std::vector<double> &c, size_t level, double scale)
+{
+    const int d = trimmed_degree(c);
+    const size_t B = basis_many_.at(1).size();
+    if (d < 8) {
+        const size_t lam = level + 1;
+        const double q = prime(lam);
+        Batch acc(B);
+        bool have = false;
+        for (int k = 1; k <= d; k++) {
+            if (c[k] == 0.0) continue;
+            const Batch &bk = basis_many(k);
+            if (bk[0].coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: basis below the leaf level");
+            Plaintext ck(*ck_.context);
+            for (size_t i = 0; i < B; i++) {
+                Ciphertext t;
+                const Ciphertext *src = &bk[i];
+                if (bk[i].coeff_modulus_size() > lam) {
+                    ck_.evaluator.mod_switch_to(bk[i], chain_of(lam), t);
+                    src = &t;
+                }
+                if (i == 0) ck_.encoder.encode(c[k], chain_of(lam), scale * q / src->scale(), ck);
+                ck_.evaluator.multiply_plain(*src, ck, t);
+                if (!have) {
+                    acc[i] = std::move(t);
+                } else {
+                    t.set_scale(acc[i].scale());
+                    ck_.evaluator.add_inplace(acc[i], t);
+                }
+            }
+            have = true;
+        }
+        if (!have) {  // constant polynomial: 0 * T_1 carries the level and scale
+            const Batch &b1 = basis_many(1);
+            Plaintext z(*ck_.context);
+            for (size_t i = 0; i < B; i++) {
+                ck_.evaluator.mod_switch_to(b1[i], chain_of(lam), acc[i]);
+                if (i == 0) ck_.encoder.encode(0.0, chain_of(lam), scale * q / acc[i].scale(), z);
+                ck_.evaluator.multiply_plain_inplace(acc[i], z);
+            }
+        }
+        ck_.evaluator.rescale_to_next_many(ptrs(acc), acc);
+        for (auto &x : acc) x.set_scale(scale);
+        if (c[0] != 0.0) {
+            Plaintext c0(*ck_.context);
+            ck_.encoder.encode(c[0], chain_of(level), scale, c0);
+            for (auto &x : acc) ck_.evaluator.add_plain_inplace(x, c0);
+        }
+        return acc;
+    }
+    int P = 1;
+    while (2 * P <= d) P *= 2;
+    std::vector<double> q, r;
+    cheb_split(c, d, P, q, r);
+    const Batch &TP = basis_many(P);
+    const size_t lam = level + 1;
+    if (TP[0].coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: giant step below the product level");
+    Batch Q = cheb_eval_many(q, lam, scale * prime(lam) / TP[0].scale());
+    Batch Tlow(B);
+    std::vector<const Ciphertext *> T;
+    for (size_t i = 0; i < B; i++) {
+        if (TP[i].coeff_modulus_size() > lam) {
+            ck_.evaluator.mod_switch_to(TP[i], chain_of(lam), Tlow[i]);
+            T.push_back(&Tlow[i]);
+        } else {
+            T.push_back(&TP[i]);
+        }
+    }
+    Batch prod;
+    ck_.evaluator.multiply_relin_rescale_many(ptrs(Q), T, *ck_.relin_keys, prod);
+    Batch R = cheb_eval_many(r, level, scale);
+    for (size_t i = 0; i < B; i++) {
+        prod[i].set_scale(scale);
+        R[i].set_scale(scale);
+        ck_.evaluator.add_inplace(prod[i], R[i]);
+    }
+    return prod;
+}
+
+void Bootstrapper::eval_mod_many(const std::vector<const Ciphertext *> &u, std::vector<Ciphertext> &out)
+{
+    if (u.empty()) throw std::invalid_argument("Bootstrapper: empty EvalMod batch");
+    const size_t l0 = u[0]->coeff_modulus_size();
+    const double s0 = u[0]->scale();
+    for (const Ciphertext *x : u)
+        if (x->coeff_modulus_size() != l0 || x->scale() != s0)
+            throw std::invalid_argument("Bootstrapper: EvalMod batch needs one level and scale");
+    const int cd_ = chebyshev_depth(cheb_);
+    if (l0 <= (size_t)cd_ + (size_t)p_.double_angles)
+        throw std::invalid_argument("Bootstrapper: not enough levels for EvalMod");
+    basis_many_.clear();
+    {
+        Batch b1;
+        for (const Ciphertext *x : u) b1.push_back(*x);
+        basis_many_.emplace(1, std::move(b1));
+    }
+    Batch y = cheb_eval_many(cheb_, l0 - cd_, s0);
+    basis_many_.clear();
+    for (int i = 0; i < p_.double_angles; i++) {
+        const std::vector<const Ciphertext *> p = ptrs(y);
+        ck_.evaluator.multiply_relin_rescale_many(p, p, *ck_.relin_keys, y);
+        for (auto &x : y) ck_.evaluator.add_inplace(x, x);
+        Plaintext one(*ck_.context);
+        ck_.encoder.encode(1.0, y[0].chain_index(), y[0].scale(), one);
+        for (auto &x : y) ck_.evaluator.sub_plain_inplace(x, one);
+    }
+    out = std::move(y);
+}
+
 Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphertext &x, double out_scale,
                                      double value_factor)
 {
@@ -673,6 +868,39 @@ void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im,
     }
     x.set_scale(final_scale);
     out = std::move(x);
+}
+
+void Bootstrapper::bootstrap_many(const std::vector<const Ciphertext *> &in, std::vector<Ciphertext> &out,
+                                  const std::vector<double> &factors)
+{
+    if (in.size() != factors.size()) throw std::invalid_argument("Bootstrapper: one factor per input");
+    const size_t B = in.size();
+    std::vector<double> input_scale(B);
+    std::vector<Ciphertext> u_re(B), u_im(B);
+    for (size_t i = 0; i < B; i++) {
+        if (!in[i] || !in[i]->get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
+        input_scale[i] = in[i]->scale();
+        Ciphertext ct;
+        mod_raise(*in[i], ct);
+        if (plan_.logn < plan_.logNh) sub_sum(ct);
+        coeff_to_slot(ct, u_re[i], u_im[i]);
+    }
+    std::vector<const Ciphertext *> u;
+    for (size_t i = 0; i < B; i++) {
+        u.push_back(&u_re[i]);
+        if (u_im[i].get()) u.push_back(&u_im[i]);
+    }
+    std::vector<Ciphertext> f;
+    eval_mod_many(u, f);
+    u_re.clear();
+    std::vector<Ciphertext> res(B);
+    size_t k = 0;
+    for (size_t i = 0; i < B; i++) {
+        const Ciphertext &fr = f[k++];
+        const Ciphertext *fi = u_im[i].get() ? &f[k++] : nullptr;
+        slot_to_coeff(fr, fi, input_scale[i], res[i], factors[i]);
+    }
+    out = std::move(res);
 }
 
 void Bootstrapper::bootstrap(const Ciphertext &in, Ciphertext &out, double factor)

@@ -104,6 +104,29 @@ int nx_bootstrap_scaled(nx_bootstrapper *bt, const nx_ciphertext *in, double fac
     });
 }
 
+int nx_bootstrap_many(nx_bootstrapper *bt, const nx_ciphertext *const *in, size_t count, const double *factors,
+                      nx_ciphertext *const *out)
+{
+    return run([&] {
+        need(bt);
+        need(in);
+        need(out);
+        std::vector<nexus::Ciphertext> x;
+        x.reserve(count);
+        for (size_t i = 0; i < count; i++) {
+            need(out[i]);
+            x.push_back(input(bt, in[i]));
+        }
+        std::vector<const nexus::Ciphertext *> p;
+        for (auto &c : x) p.push_back(&c);
+        std::vector<double> f(count, 1.0);
+        if (factors) f.assign(factors, factors + count);
+        std::vector<nexus::Ciphertext> res;
+        bt->bt->bootstrap_many(p, res, f);
+        for (size_t i = 0; i < count; i++) store(res[i], out[i]);
+    });
+}
+
 int nx_boot_mod_raise(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out)
 {
     return run([&] {

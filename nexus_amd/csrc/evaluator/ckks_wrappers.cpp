@@ -427,6 +427,42 @@ void Evaluator::multiply_plain_accumulate(const std::vector<const Ciphertext *> 
     check(nx_multiply_plain_accumulate(c.data(), p.data(), c.size(), out.get()));
     dest = std::move(out);
 }
+void Evaluator::rescale_to_next_many(const std::vector<const Ciphertext *> &cts, std::vector<Ciphertext> &dest) const
+{
+    std::vector<const nx_ciphertext *> in;
+    for (const Ciphertext *x : cts) in.push_back(x->get());
+    std::vector<Ciphertext> res;
+    res.reserve(cts.size());
+    std::vector<nx_ciphertext *> o;
+    for (size_t i = 0; i < cts.size(); i++) {
+        res.emplace_back(*ctx_);
+        o.push_back(res.back().get());
+    }
+    check(nx_rescale_to_next_many(in.data(), in.size(), o.data()));
+    dest = std::move(res);
+}
+
+void Evaluator::multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a,
+                                            const std::vector<const Ciphertext *> &b, const RelinKeys &rk,
+                                            std::vector<Ciphertext> &dest) const
+{
+    if (a.size() != b.size()) throw std::invalid_argument("multiply_relin_rescale_many: operand counts differ");
+    std::vector<const nx_ciphertext *> x, y;
+    for (size_t i = 0; i < a.size(); i++) {
+        x.push_back(a[i]->get());
+        y.push_back(b[i]->get());
+    }
+    std::vector<Ciphertext> res;
+    res.reserve(a.size());
+    std::vector<nx_ciphertext *> o;
+    for (size_t i = 0; i < a.size(); i++) {
+        res.emplace_back(*ctx_);
+        o.push_back(res.back().get());
+    }
+    check(nx_multiply_relin_rescale_many(x.data(), y.data(), x.size(), rk.get(), o.data()));
+    dest = std::move(res);
+}
+
 void Evaluator::multiply_periodic_mix(const std::vector<const Ciphertext *> &cts, const PeriodicPlaintexts &w,
                                       const std::vector<int32_t> &index, std::vector<Ciphertext> &out) const
 {

@@ -74,6 +74,23 @@ void poly_mul_plain(const DeviceTables &t, uint64_t *out, const uint64_t *a, con
 // (a0,a1) x (b0,b1) -> (a0b0, a0b1+a1b0, a1b1); out may alias neither input.
 void poly_tensor(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *b, uint32_t limbs,
                  hipStream_t s);
+// Pointer lists of batched element-wise kernels (count <= kPtrMax).
+constexpr int kPtrMax = 32;
+struct PairPtrs {
+    const uint64_t *x[kPtrMax];
+    const uint64_t *y[kPtrMax];
+};
+struct OutPtrs {
+    uint64_t *out[kPtrMax];
+};
+// c01[r][2][l][n] = (a_r0 b_r0, a_r0 b_r1 + a_r1 b_r0), c2[r][l][n] = a_r1 b_r1
+void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const uint64_t *const *a,
+                      const uint64_t *const *b, uint32_t count, uint32_t limbs, hipStream_t s);
+// out_r[p][i] = (x_r[p][i] - tmp[r polys + p][i]) inv[i] for i < l_out (x_r has
+// x_limbs limbs per poly): the rescale / ModDown tail into separate results
+void rns_sub_mul_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *const *x, const uint64_t *tmp,
+                      const ulonglong2 *inv, uint32_t count, uint32_t polys, uint32_t l_out, uint32_t x_limbs,
+                      hipStream_t s);
 // out[p][i] = a[p][i] * w[i] with per-limb Shoup constants {w, w'} (host array, passed by value)
 void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,
                      uint32_t polys, uint32_t limbs, hipStream_t s);

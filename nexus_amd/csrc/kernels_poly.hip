@@ -108,6 +108,48 @@ __global__ void k_tensor(uint64_t *out, const uint64_t *a, const uint64_t *b, co
     out[2 * ps + idx] = mul_mod(a1, b1, m);
 }
 
+// Tensor products of `count` pairs: c01[r][2][l][n] = (a0 b0, a0 b1 + a1 b0),
+// c2[r][l][n] = a1 b1, from ciphertexts given by pointer.
+__global__ void k_tensor_many(uint64_t *c01, uint64_t *c2, PairPtrs p, const ModConst *mod, uint32_t limbs,
+                              uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // r * limbs + limb
+    const uint32_t r = row / limbs, limb = row - r * limbs;
+    const ModConst m = mod[limb];
+    const uint64_t ps = (uint64_t)limbs * n;
+    const uint64_t idx = (uint64_t)limb * n + c;
+    const uint64_t *a = p.x[r], *b = p.y[r];
+    const uint64_t a0 = a[idx], a1 = a[ps + idx], b0 = b[idx], b1 = b[ps + idx];
+    uint64_t *o = c01 + (uint64_t)r * 2 * ps;
+    o[idx] = mul_mod(a0, b0, m);
+    uint64_t lo1 = a0 * b1, hi1 = mulhi64(a0, b1);
+    uint64_t lo2 = a1 * b0, hi2 = mulhi64(a1, b0);
+    uint64_t lo = lo1 + lo2;
+    uint64_t hi = hi1 + hi2 + (lo < lo1);
+    o[ps + idx] = barrett128(hi, lo, m);
+    c2[(uint64_t)r * ps + idx] = mul_mod(a1, b1, m);
+}
+
+// out_r[p][limb] = (x_r[p][limb] - tmp[r * polys + p][limb]) inv[limb] for the
+// first l_out limbs; x_r has x_limbs limbs per poly (pointer lists p.y = x, p.x
+// unused, outs = out pointers)
+__global__ void k_sub_mul_ptrs(OutPtrs outs, PairPtrs xs, const uint64_t *tmp, const ulonglong2 *inv,
+                               const ModConst *mod, uint32_t polys, uint32_t l_out, uint32_t x_limbs, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // (r * polys + p) * l_out + limb
+    const uint32_t rp = row / l_out, limb = row - rp * l_out;
+    const uint32_t r = rp / polys, p = rp - r * polys;
+    const uint64_t q = mod[limb].q;
+    const ulonglong2 w = inv[limb];
+    const uint64_t xv = xs.y[r][((uint64_t)p * x_limbs + limb) * n + c];
+    const uint64_t d = sub_mod(xv, tmp[(uint64_t)row * n + c], q);
+    outs.out[r][((uint64_t)p * l_out + limb) * n + c] = shoup(d, w.x, w.y, q);
+}
+
 __global__ void k_mul_scalar(uint64_t *out, const uint64_t *a, LimbShoup scal, const ModConst *mod, uint32_t limbs,
                              uint32_t n)
 {
@@ -349,6 +391,35 @@ void poly_tensor(const DeviceTables &t, uint64_t *out, const uint64_t *a, const 
     ProfScope prof("tensor", s, 56.0 * (1u << t.logn) * limbs);
     hipLaunchKernelGGL(k_tensor, grid_for(t.logn, limbs), dim3(kBlock), 0, s, out, a, b, t.mod, limbs,
                        1u << t.logn);
+}
+
+void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const uint64_t *const *a,
+                      const uint64_t *const *b, uint32_t count, uint32_t limbs, hipStream_t s)
+{
+    PairPtrs p;
+    for (uint32_t i = 0; i < kPtrMax; i++) {
+        p.x[i] = i < count ? a[i] : nullptr;
+        p.y[i] = i < count ? b[i] : nullptr;
+    }
+    ProfScope prof("tensor", s, 56.0 * (1u << t.logn) * limbs * count);
+    hipLaunchKernelGGL(k_tensor_many, grid_for(t.logn, count * limbs), dim3(kBlock), 0, s, c01, c2, p, t.mod, limbs,
+                       1u << t.logn);
+}
+
+void rns_sub_mul_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *const *x, const uint64_t *tmp,
+                      const ulonglong2 *inv, uint32_t count, uint32_t polys, uint32_t l_out, uint32_t x_limbs,
+                      hipStream_t s)
+{
+    OutPtrs o;
+    PairPtrs xs;
+    for (uint32_t i = 0; i < kPtrMax; i++) {
+        o.out[i] = i < count ? out[i] : nullptr;
+        xs.x[i] = nullptr;
+        xs.y[i] = i < count ? x[i] : nullptr;
+    }
+    ProfScope prof("rns_sub_mul", s, 24.0 * (1u << t.logn) * count * polys * l_out);
+    hipLaunchKernelGGL(k_sub_mul_ptrs, grid_for(t.logn, count * polys * l_out), dim3(kBlock), 0, s, o, xs, tmp, inv,
+                       t.mod, polys, l_out, x_limbs, 1u << t.logn);
 }
 
 void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,

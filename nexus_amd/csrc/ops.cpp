@@ -467,6 +467,124 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
     dst = std::move(r);
 }
 
+// Rescale tail shared by the batched forms: `count` ciphertexts of `polys`
+// polys at l limbs, given by pointer, into fresh results at l - 1 limbs.  One
+// INTT of all last limbs, one centring, one NTT, one combination.
+static void rescale_batch(Context &c, const std::vector<const uint64_t *> &x, size_t polys, size_t l,
+                          const std::vector<uint64_t *> &outs)
+{
+    const size_t n = c.n, B = x.size(), P = B * polys;
+    require(P <= (size_t)kNttSrcMax, "rescale batch too large");
+    std::vector<const uint64_t *> lasts;
+    for (size_t r = 0; r < B; r++)
+        for (size_t p = 0; p < polys; p++) lasts.push_back(x[r] + (p * l + l - 1) * n);
+    DevBuf last(&c, P * n);
+    {
+        NttBatch b;
+        b.data = last.get();
+        b.poly_stride = n;
+        b.polys = (uint32_t)P;
+        b.limbs = 1;
+        b.prime_by_poly = 0;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = (uint8_t)(l - 1);
+        b.src_list = lasts.data();
+        ntt_inverse(c.tables, b, c.stream);
+    }
+    DevBuf tmp(&c, P * (l - 1) * n);
+    rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1), c.stream);
+    ntt_fwd(c, tmp.get(), P, l - 1, (l - 1) * n);
+    rns_sub_mul_ptrs(c.tables, outs.data(), x.data(), tmp.get(), c.rescale_inv(l), (uint32_t)B, (uint32_t)polys,
+                     (uint32_t)(l - 1), (uint32_t)l, c.stream);
+}
+
+void op_rescale_many(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out)
+{
+    require(!in.empty() && in.size() == out.size(), "inputs and outputs must match");
+    const Ciphertext &a0 = *in[0];
+    valid(a0);
+    Context &c = *a0.ctx;
+    const size_t l = a0.limbs, P = a0.size, n = c.n;
+    for (const Ciphertext *a : in) {
+        valid(*a);
+        require(a->ctx == &c && a->size == P && a->limbs == l, "rescale batch needs one shape");
+        require(a->ntt, "CKKS encrypted must be in NTT form");
+    }
+    require(l >= 2, "end of modulus switching chain reached");
+    std::vector<Ciphertext> res(in.size());
+    const size_t chunk = (size_t)kNttSrcMax / P;
+    for (size_t base = 0; base < in.size(); base += chunk) {
+        const size_t cnt = std::min(chunk, in.size() - base);
+        std::vector<const uint64_t *> x;
+        std::vector<uint64_t *> o;
+        for (size_t r = 0; r < cnt; r++) {
+            const Ciphertext &a = *in[base + r];
+            Ciphertext &t = res[base + r];
+            t.ctx = &c;
+            t.size = P;
+            t.limbs = l - 1;
+            t.scale = a.scale / (double)c.moduli[l - 1];
+            t.ntt = true;
+            t.data.reset(&c, P * (l - 1) * n);
+            x.push_back(a.data.get());
+            o.push_back(t.data.get());
+        }
+        rescale_batch(c, x, P, l, o);
+    }
+    for (size_t r = 0; r < in.size(); r++) *out[r] = std::move(res[r]);
+}
+
+void op_multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                                    const KSwitchKeys &rk, const std::vector<Ciphertext *> &out)
+{
+    // multiply (evaluator.cpp:1003-1040), relinearize (2724-3021) and
+    // rescale_to_next (1402-1481) of every pair, each stage one batched launch
+    require(!a.empty() && a.size() == b.size() && a.size() == out.size(), "operand counts must match");
+    const Ciphertext &a0 = *a[0];
+    valid(a0);
+    Context &c = *a0.ctx;
+    const size_t l = a0.limbs, n = c.n, pw = l * n;
+    require(rk.ctx == &c, "relin_keys is not valid for encryption parameters");
+    const DevBuf &key = find_key(rk, 0, "relinearization key not present");
+    require(l >= 2, "end of modulus switching chain reached");
+    for (size_t r = 0; r < a.size(); r++) {
+        valid(*a[r]);
+        valid(*b[r]);
+        same_params(*a[r], *b[r]);
+        require(a[r]->ctx == &c && a[r]->limbs == l, "batch operands must share the level");
+        require(a[r]->ntt && b[r]->ntt, "encrypted1 or encrypted2 must be in NTT form");
+        require(a[r]->size == 2 && b[r]->size == 2, "multiply expects two size-2 ciphertexts (relinearize first)");
+        require(scale_within_bounds(c, a[r]->scale * b[r]->scale, l), "scale out of bounds");
+    }
+    std::vector<Ciphertext> res(a.size());
+    const size_t chunk = (size_t)kNttSrcMax / 2;
+    for (size_t base = 0; base < a.size(); base += chunk) {
+        const size_t cnt = std::min(chunk, a.size() - base);
+        std::vector<const uint64_t *> pa, pb;
+        for (size_t r = 0; r < cnt; r++) {
+            pa.push_back(a[base + r]->data.get());
+            pb.push_back(b[base + r]->data.get());
+        }
+        DevBuf c01(&c, cnt * 2 * pw), c2(&c, cnt * pw), ks(&c, cnt * 2 * pw);
+        poly_tensor_many(c.tables, c01.get(), c2.get(), pa.data(), pb.data(), (uint32_t)cnt, (uint32_t)l, c.stream);
+        key_switch_batch(c, ks.get(), c2.get(), cnt, l, key, false, c01.get());
+        std::vector<const uint64_t *> x;
+        std::vector<uint64_t *> o;
+        for (size_t r = 0; r < cnt; r++) {
+            Ciphertext &t = res[base + r];
+            t.ctx = &c;
+            t.size = 2;
+            t.limbs = l - 1;
+            t.scale = a[base + r]->scale * b[base + r]->scale / (double)c.moduli[l - 1];
+            t.ntt = true;
+            t.data.reset(&c, 2 * (l - 1) * n);
+            x.push_back(ks.get() + r * 2 * pw);
+            o.push_back(t.data.get());
+        }
+        rescale_batch(c, x, 2, l, o);
+    }
+    for (size_t r = 0; r < a.size(); r++) *out[r] = std::move(res[r]);
+}
+
 static void drop_to(const Ciphertext &a, size_t limbs, Ciphertext &dst)
 {
     Context &c = *a.ctx;

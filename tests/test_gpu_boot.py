@@ -183,6 +183,21 @@ def test_bootstrap(env):
     _bootstrap_check(env, 5, 1e-4, 1e-3)
 
 
+def test_bootstrap_many_matches_single(env):
+    """three inputs with different factors through one lockstep EvalMod: each
+    output equals its own single bootstrap word for word"""
+    _note("bootstrap_many logN=%d logn=%d" % (env.logN, env.logn))
+    rng = np.random.default_rng(77)
+    Nh = env.ctx.slot_count
+    cts = [nx.mod_switch_to(env.enc(rng.uniform(-1, 1, Nh), SCALE), env.L) for _ in range(3)]
+    factors = [1.0, 0.5, 2.0]
+    many = env.bt.bootstrap_many(cts, factors)
+    for c, f, m in zip(cts, factors, many):
+        single = env.bt.bootstrap(c, f)
+        assert np.array_equal(m.to_array(), single.to_array())
+        assert m.scale == single.scale
+
+
 def test_bootstrap_output_computes(env):
     """the refreshed ciphertext supports further multiplications"""
     _note("bootstrap then square logN=%d" % env.logN)
