@@ -314,6 +314,12 @@ public:
     Ciphertext exp(const Ciphertext &x);
     Ciphertext inverse(const Ciphertext &x, int iter = 4);
     void eval_odd_deg9_poly(const std::vector<double> &a, const Ciphertext &x, Ciphertext &dest);
+    // the same steps over a batch of inputs of one level and scale in lockstep
+    // (every multiplication with its relinearisation and rescale one batched
+    // call); word for word the single-input results
+    std::vector<Ciphertext> sgn_eval_many(const std::vector<const Ciphertext *> &x, int d_g, int d_f,
+                                          double sgn_factor = 0.5);
+    void eval_odd_deg9_poly_many(const std::vector<double> &a, std::vector<Ciphertext> &x);
     double calculate_MAE(const std::vector<double> &y_true, const Ciphertext &ct, size_t n);
 
 private:
@@ -331,6 +337,8 @@ class GELUEvaluator {
 public:
     explicit GELUEvaluator(CKKSEvaluator &ckks) : ckks_(&ckks) {}
     void gelu(const Ciphertext &x, Ciphertext &res);
+    // gelu of every input (one level and scale) in lockstep; word for word gelu()
+    void gelu_many(const std::vector<const Ciphertext *> &x, std::vector<Ciphertext> &res);
 
 private:
     CKKSEvaluator *ckks_;

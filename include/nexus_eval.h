@@ -33,6 +33,10 @@ int nx_eval_exp(nx_evaluator *ev, const nx_ciphertext *x, nx_ciphertext *out);
 int nx_eval_inverse(nx_evaluator *ev, const nx_ciphertext *x, int iter, nx_ciphertext *out);
 /* GeLUEvaluator::gelu (src/gelu.cpp:5-111) */
 int nx_eval_gelu(nx_evaluator *ev, const nx_ciphertext *x, nx_ciphertext *out);
+/* gelu of count inputs of one level and scale in lockstep (each
+ * multiplication, key switch and rescale one batched launch); out[i] equals
+ * nx_eval_gelu(x[i]) word for word.  No counterpart in the reference. */
+int nx_eval_gelu_many(nx_evaluator *ev, const nx_ciphertext *const *x, size_t count, nx_ciphertext *const *out);
 /* LNEvaluator::layer_norm (src/layer_norm.cpp:6-33) */
 int nx_eval_layer_norm(nx_evaluator *ev, const nx_ciphertext *x, int len, nx_ciphertext *out);
 /* SoftmaxEvaluator::softmax (src/softmax.cpp:6-42) */

@@ -649,6 +649,7 @@ _sig("nx_eval_invert_sqrt", [vp, vp, ctypes.c_int, ctypes.c_int, vp])
 _sig("nx_eval_exp", [vp, vp, vp])
 _sig("nx_eval_inverse", [vp, vp, ctypes.c_int, vp])
 _sig("nx_eval_gelu", [vp, vp, vp])
+_sig("nx_eval_gelu_many", [vp, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
 _sig("nx_eval_layer_norm", [vp, vp, ctypes.c_int, vp])
 _sig("nx_eval_softmax", [vp, vp, ctypes.c_int, vp])
 
@@ -687,6 +688,13 @@ class CKKSEvaluator:
 
     def gelu(self, x):
         return self._run(_lib.nx_eval_gelu, x.h)
+
+    def gelu_many(self, xs):
+        """[gelu(x) for x in xs] in lockstep"""
+        outs = [Ciphertext(xs[0].ctx) for _ in xs]
+        _check(_lib.nx_eval_gelu_many(self.h, (vp * len(xs))(*[c.h for c in xs]), len(xs),
+                                      (vp * len(outs))(*[o.h for o in outs])))
+        return outs
 
     def layer_norm(self, x, length):
         return self._run(_lib.nx_eval_layer_norm, x.h, length)

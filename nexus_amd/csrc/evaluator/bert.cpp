@@ -763,11 +763,13 @@ std::vector<Ciphertext> BertLayer::forward(const std::vector<Ciphertext> &x, con
     mark("ffn1");
     boot_all(h, 4);
     mark("bootstrap_ffn");
-    GELUEvaluator gelu(ck_);
-    for (auto &c : h) {
-        Ciphertext g;
-        gelu.gelu(c, g);
-        c = std::move(g);
+    {
+        GELUEvaluator gelu(ck_);
+        std::vector<const Ciphertext *> hp;
+        for (auto &c : h) hp.push_back(&c);
+        std::vector<Ciphertext> g;
+        gelu.gelu_many(hp, g);
+        h = std::move(g);
     }
     mark("gelu");
     std::vector<Ciphertext> r2 = linear(W2, h, 1.0 / 8);

@@ -84,6 +84,28 @@ int nx_eval_gelu(nx_evaluator *ev, const nx_ciphertext *x, nx_ciphertext *out)
     });
 }
 
+int nx_eval_gelu_many(nx_evaluator *ev, const nx_ciphertext *const *x, size_t count, nx_ciphertext *const *out)
+{
+    return run([&] {
+        need(ev);
+        need(x);
+        need(out);
+        std::vector<nexus::Ciphertext> in;
+        in.reserve(count);
+        for (size_t i = 0; i < count; i++) {
+            need(x[i]);
+            need(out[i]);
+            in.push_back(NX_IN(x[i]));
+        }
+        std::vector<const nexus::Ciphertext *> p;
+        for (auto &c : in) p.push_back(&c);
+        nexus::GELUEvaluator g(*ev->ckks);
+        std::vector<nexus::Ciphertext> res;
+        g.gelu_many(p, res);
+        for (size_t i = 0; i < count; i++) store(res[i], out[i]);
+    });
+}
+
 int nx_eval_layer_norm(nx_evaluator *ev, const nx_ciphertext *x, int len, nx_ciphertext *out)
 {
     return run([&] {

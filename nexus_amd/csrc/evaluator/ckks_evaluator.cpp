@@ -130,6 +130,188 @@ void CKKSEvaluator::eval_odd_deg9_poly(const std::vector<double> &a, const Ciphe
     dest = std::move(out);
 }
 
+// ---- lockstep batches of the evaluators above
+namespace {
+
+std::vector<const Ciphertext *> ptrs_of(const std::vector<Ciphertext> &v)
+{
+    std::vector<const Ciphertext *> p;
+    for (const auto &x : v) p.push_back(&x);
+    return p;
+}
+
+void rescale_all(const Evaluator &ev, std::vector<Ciphertext> &v) { ev.rescale_to_next_many(ptrs_of(v), v); }
+
+}  // namespace
+
+void CKKSEvaluator::eval_odd_deg9_poly_many(const std::vector<double> &a, std::vector<Ciphertext> &X)
+{
+    const Evaluator &ev = evaluator;
+    const size_t B = X.size();
+    const double D = X[0].scale();
+    const uint64_t p = get_modulus(X[0], 2), q = get_modulus(X[0], 3), r = get_modulus(X[0], 4);
+    std::vector<Ciphertext> x2, x3, x6, xs(B);
+    ev.multiply_relin_rescale_many(ptrs_of(X), ptrs_of(X), *relin_keys, x2);
+    for (size_t i = 0; i < B; i++) ev.mod_switch_to(X[i], X[i].params_id() + 1, xs[i]);
+    ev.multiply_relin_rescale_many(ptrs_of(x2), ptrs_of(xs), *relin_keys, x3);
+    ev.multiply_relin_rescale_many(ptrs_of(x3), ptrs_of(x3), *relin_keys, x6);
+    Plaintext a1, a3, a5, a7, a9;
+    // T1
+    std::vector<Ciphertext> T1(B);
+    encoder.encode(a[5], x2[0].params_id(), D / x2[0].scale() * p / x3[0].scale() * q, a5);
+    for (size_t i = 0; i < B; i++) ev.multiply_plain(x2[i], a5, T1[i]);
+    rescale_all(ev, T1);
+    encoder.encode(a[3], T1[0].params_id(), T1[0].scale(), a3);
+    for (auto &t : T1) ev.add_plain_inplace(t, a3);
+    ev.multiply_relin_rescale_many(ptrs_of(T1), ptrs_of(x3), *relin_keys, T1);
+    // T2
+    std::vector<Ciphertext> T2(B), a7x(B);
+    encoder.encode(a[9], x3[0].params_id(), D / x3[0].scale() * r / x6[0].scale() * q, a9);
+    for (size_t i = 0; i < B; i++) ev.multiply_plain(x3[i], a9, T2[i]);
+    rescale_all(ev, T2);
+    encoder.encode(a[7], xs[0].params_id(), T2[0].scale() / xs[0].scale() * p, a7);
+    for (size_t i = 0; i < B; i++) ev.multiply_plain(xs[i], a7, a7x[i]);
+    rescale_all(ev, a7x);
+    for (size_t i = 0; i < B; i++) {
+        ev.mod_switch_to_inplace(a7x[i], T2[i].params_id());
+        const double mid = (T2[i].scale() + a7x[i].scale()) / 2;
+        T2[i].set_scale(mid);
+        a7x[i].set_scale(mid);
+        ev.add_inplace(T2[i], a7x[i]);
+    }
+    ev.multiply_relin_rescale_many(ptrs_of(T2), ptrs_of(x6), *relin_keys, T2);
+    // T3
+    std::vector<Ciphertext> T3(B);
+    encoder.encode(a[1], xs[0].params_id(), (double)p, a1);
+    for (size_t i = 0; i < B; i++) ev.multiply_plain(xs[i], a1, T3[i]);
+    rescale_all(ev, T3);
+    for (size_t i = 0; i < B; i++) {
+        const double mid3 = (T1[i].scale() + T2[i].scale() + T3[i].scale()) / 3;
+        T1[i].set_scale(mid3);
+        T2[i].set_scale(mid3);
+        T3[i].set_scale(mid3);
+        ev.mod_switch_to_inplace(T1[i], T2[i].params_id());
+        ev.add_inplace(T2[i], T1[i]);
+        ev.mod_switch_to_inplace(T3[i], T2[i].params_id());
+        ev.add_inplace(T2[i], T3[i]);
+    }
+    X = std::move(T2);
+}
+
+std::vector<Ciphertext> CKKSEvaluator::sgn_eval_many(const std::vector<const Ciphertext *> &x, int d_g, int d_f,
+                                                     double sgn_factor)
+{
+    std::vector<double> f4(10), g4(10), f4_last(10), g4_last(10);
+    for (int i = 0; i <= 9; i++) {
+        f4[i] = F4[i] / F4_SCALE;
+        f4_last[i] = f4[i] * sgn_factor;
+        g4[i] = G4[i] / G4_SCALE;
+        g4_last[i] = g4[i] * sgn_factor;
+    }
+    std::vector<Ciphertext> dest;
+    for (const Ciphertext *c : x) dest.push_back(*c);
+    for (int i = 0; i < d_g; i++) eval_odd_deg9_poly_many(i == d_g - 1 ? g4_last : g4, dest);
+    for (int i = 0; i < d_f; i++) eval_odd_deg9_poly_many(i == d_f - 1 ? f4_last : f4, dest);
+    return dest;
+}
+
+void GELUEvaluator::gelu_many(const std::vector<const Ciphertext *> &xin, std::vector<Ciphertext> &res)
+{
+    CKKSEvaluator &ck = *ckks_;
+    const Evaluator &ev = ck.evaluator;
+    const size_t B = xin.size();
+    if (!B) throw std::invalid_argument("gelu_many: no inputs");
+    for (const Ciphertext *c : xin)
+        if (c->params_id() != xin[0]->params_id() || c->scale() != xin[0]->scale())
+            throw std::invalid_argument("gelu_many: inputs must share the level and scale");
+    const Ciphertext &x0 = *xin[0];
+    const Plaintext &p0 = ck.vector_constant(-3.5, x0.params_id(), x0.scale());
+    const Plaintext &p1 = ck.vector_constant(3.5, x0.params_id(), x0.scale());
+    const Plaintext &delta = ck.vector_constant(1.0 / 8.5, x0.params_id(), x0.scale());
+    // both sign arguments of every input in one batch: b0 = [0, B), b1 = [B, 2B)
+    std::vector<Ciphertext> b(2 * B);
+    for (size_t i = 0; i < B; i++) {
+        ev.sub_plain(*xin[i], p0, b[i]);
+        ev.multiply_plain_inplace(b[i], delta);
+        ev.sub_plain(*xin[i], p1, b[B + i]);
+        ev.multiply_plain_inplace(b[B + i], delta);
+    }
+    rescale_all(ev, b);
+    b = ck.sgn_eval_many(ptrs_of(b), 2, 2);
+    const Plaintext &half = ck.vector_constant(0.5, b[B].params_id(), b[B].scale());
+    std::vector<Ciphertext> a1(B), a2(B);
+    for (size_t i = 0; i < B; i++) {
+        ev.sub(b[i], b[B + i], a1[i]);
+        ev.add_plain(b[B + i], half, a2[i]);
+    }
+    // powers: x2, x4, then (x6 = x2 x4, x8 = x4^2), then (x10 = x2 x8, x12 = x6^2),
+    // x2 switched down to each product's level as gelu() does
+    std::vector<Ciphertext> x2, x4, p68, p1012;
+    ev.multiply_relin_rescale_many(xin, xin, *ck.relin_keys, x2);
+    ev.multiply_relin_rescale_many(ptrs_of(x2), ptrs_of(x2), *ck.relin_keys, x4);
+    for (auto &c : x2) ev.mod_switch_to_inplace(c, x4[0].params_id());
+    {
+        std::vector<const Ciphertext *> l, r;
+        for (size_t i = 0; i < B; i++) l.push_back(&x2[i]), r.push_back(&x4[i]);
+        for (size_t i = 0; i < B; i++) l.push_back(&x4[i]), r.push_back(&x4[i]);
+        ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, p68);  // x6 = [0, B), x8 = [B, 2B)
+    }
+    for (auto &c : x2) ev.mod_switch_to_inplace(c, p68[B].params_id());
+    {
+        std::vector<const Ciphertext *> l, r;
+        for (size_t i = 0; i < B; i++) l.push_back(&x2[i]), r.push_back(&p68[B + i]);
+        for (size_t i = 0; i < B; i++) l.push_back(&p68[i]), r.push_back(&p68[i]);
+        ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, p1012);  // x10, x12
+    }
+    const double A[] = {2.25775755e-04, 0.5, 3.96880960e-01, -6.37042698e-02, 8.38841647e-03, -7.17830961e-04,
+                        3.49617829e-05, -7.26059653e-07};
+    std::vector<Plaintext> coeff_A(8);
+    for (size_t i = 0; i < 8; i++) ck.encoder.encode(A[i], ck.scale, coeff_A[i]);
+    // term j of input i: x^(2j - 2) for j >= 2, x for j = 1
+    std::vector<std::vector<Ciphertext>> terms(8, std::vector<Ciphertext>(B));
+    for (size_t i = 0; i < B; i++) {
+        terms[1][i] = *xin[i];
+        terms[2][i] = x2[i];
+        terms[3][i] = x4[i];
+        terms[4][i] = p68[i];
+        terms[5][i] = p68[B + i];
+        terms[6][i] = p1012[i];
+        terms[7][i] = p1012[B + i];
+    }
+    for (size_t j = 1; j < 8; j++) {
+        ev.mod_switch_to_inplace(coeff_A[j], terms[j][0].params_id());
+        for (auto &t : terms[j]) ev.multiply_plain_inplace(t, coeff_A[j]);
+        rescale_all(ev, terms[j]);
+        for (auto &t : terms[j]) t.set_scale(ck.scale);
+    }
+    std::vector<Ciphertext> Ax = std::move(terms[7]);
+    for (size_t i = 0; i < B; i++)
+        for (size_t j = 1; j < 7; j++) {
+            ev.mod_switch_to_inplace(terms[j][i], Ax[i].params_id());
+            ev.add_inplace(Ax[i], terms[j][i]);
+        }
+    ev.mod_switch_to_inplace(coeff_A[0], Ax[0].params_id());
+    for (auto &c : Ax) ev.add_plain_inplace(c, coeff_A[0]);
+    std::vector<Ciphertext> xs(B), s;
+    std::vector<const Ciphertext *> l, r;
+    for (size_t i = 0; i < B; i++) {
+        ev.mod_switch_to_inplace(Ax[i], a1[i].params_id());
+        ev.mod_switch_to(*xin[i], a2[i].params_id(), xs[i]);
+        l.push_back(&Ax[i]);
+        r.push_back(&a1[i]);
+    }
+    for (size_t i = 0; i < B; i++) l.push_back(&xs[i]), r.push_back(&a2[i]);
+    ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, s);  // s1 = [0, B), s2 = [B, 2B)
+    std::vector<Ciphertext> out(B);
+    for (size_t i = 0; i < B; i++) {
+        s[i].set_scale(ck.scale);
+        s[B + i].set_scale(ck.scale);
+        ev.mod_switch_to_inplace(s[B + i], s[i].params_id());
+        ev.add(s[i], s[B + i], out[i]);
+    }
+    res = std::move(out);
+}
+
 // src/ckks_evaluator.cpp:366-422
 Ciphertext CKKSEvaluator::newton_iter(Ciphertext x, Ciphertext res, int iter)
 {

@@ -62,6 +62,17 @@ def test_gelu(env):
     print("gelu: max|dec-fp64 model| %.3g  MAE vs calibration %.3g" % (np.max(np.abs(out - model)), mae))
 
 
+def test_gelu_many_matches_single(env):
+    """three inputs in lockstep: each output equals its own gelu() word for word"""
+    ctx, sk, ev = env
+    _note("gelu_many x3")
+    rng = np.random.default_rng(21)
+    xs = [enc(ctx, sk, rng.uniform(-6, 6, ctx.slot_count)) for _ in range(3)]
+    many = ev.gelu_many(xs)
+    for x, m in zip(xs, many):
+        assert np.array_equal(m.to_array(), ev.gelu(x).to_array())
+
+
 def test_layer_norm(env):
     ctx, sk, ev = env
     _note("layernorm 768 len 1024")
