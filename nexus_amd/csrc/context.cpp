@@ -18,9 +18,20 @@ void hip_check(hipError_t e, const char *what)
 // ---------------------------------------------------------------- memory
 DeviceCache::~DeviceCache() { trim(); }
 
-void *DeviceCache::get(size_t bytes)
+size_t DeviceCache::size_class(size_t bytes)
+{
+    // eight classes per power of two above 64 KiB (at most 12.5% slack), so
+    // that buffers of nearby sizes are reused instead of cached side by side
+    if (bytes <= (size_t)1 << 16) return (bytes + 255) & ~(size_t)255;
+    int e = 63 - __builtin_clzll((unsigned long long)bytes);
+    const size_t step = (size_t)1 << (e - 3);
+    return (bytes + step - 1) & ~(step - 1);
+}
+
+void *DeviceCache::try_get(size_t bytes)
 {
     if (!bytes) return nullptr;
+    bytes = size_class(bytes);
     {
         std::lock_guard<std::mutex> g(mu_);
         auto it = free_.find(bytes);
@@ -33,21 +44,31 @@ void *DeviceCache::get(size_t bytes)
         }
     }
     void *p = nullptr;
-    hipError_t e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) {
-        // release cached blocks and retry once
+    if (hipMalloc(&p, bytes) != hipSuccess) {
         (void)hipGetLastError();
-        trim();
-        NX_HIP(hipMalloc(&p, bytes));
+        return nullptr;
     }
     std::lock_guard<std::mutex> g(mu_);
     in_use_ += bytes;
     return p;
 }
 
+void *DeviceCache::get(size_t bytes)
+{
+    if (!bytes) return nullptr;
+    void *p = try_get(bytes);
+    if (p) return p;
+    // release this cache's blocks and retry once
+    trim();
+    p = try_get(bytes);
+    if (!p) NX_HIP(hipErrorOutOfMemory);
+    return p;
+}
+
 void DeviceCache::put(void *p, size_t bytes)
 {
     if (!p) return;
+    bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu_);
     free_[bytes].push_back(p);
     in_use_ -= bytes;
@@ -86,7 +107,14 @@ void DevBuf::reset(Context *ctx, size_t words)
     release();
     cache_ = cur;
     words_ = words;
-    p_ = words ? (uint64_t *)cache_->get(words * 8) : nullptr;
+    if (!words) return;
+    p_ = (uint64_t *)cache_->try_get(words * 8);
+    if (!p_) {
+        // out of device memory: release the cached blocks of every lane of the
+        // context (a device-wide synchronisation) and retry
+        ctx->trim_caches();
+        p_ = (uint64_t *)cache_->get(words * 8);
+    }
 }
 
 void DevBuf::release()
@@ -137,6 +165,14 @@ void Context::bind_lane(size_t index)
 }
 
 Lane *Context::bound_lane() const { return tl_ctx == this ? tl_lane : nullptr; }
+
+void Context::trim_caches()
+{
+    cache->trim();
+    std::lock_guard<std::mutex> g(lane_mu_);
+    for (auto &l : lanes_)
+        if (l) l->cache->trim();
+}
 
 const std::shared_ptr<DeviceCache> &Context::current_cache() const
 {

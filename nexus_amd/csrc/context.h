@@ -44,9 +44,11 @@ void set_last_error(const char *msg);  // thread-local message behind nx_last_er
 class DeviceCache {
 public:
     ~DeviceCache();
-    void *get(size_t bytes);
+    void *get(size_t bytes);      // throws when the device is out of memory
+    void *try_get(size_t bytes);  // null when the device is out of memory
     void put(void *p, size_t bytes);
     void trim();
+    static size_t size_class(size_t bytes);
     size_t bytes_in_use() const { return in_use_; }
     size_t bytes_cached() const { return cached_; }
 
@@ -165,6 +167,7 @@ public:
     Lane *lane(size_t index);
     void bind_lane(size_t index);   // 0 unbinds
     Lane *bound_lane() const;
+    void trim_caches();  // free the cached blocks of the default lane and every lane
     const std::shared_ptr<DeviceCache> &current_cache() const;
 
     // device tables
