@@ -23,9 +23,9 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + INCLUDE, "-Wall", "-
           "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
 # Packed FP32 instructions are kept out of every device code object.
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-# The encoder's double-precision FFT and the NTT's double-precision modular
-# arithmetic must round exactly as written.
-NO_CONTRACT = {"encoder.cpp", "kernels_encode.hip", "kernels_ntt.hip"}
+# The encoder's double-precision FFT and the double-precision modular
+# arithmetic of the NTT and the base conversions must round exactly as written.
+NO_CONTRACT = {"encoder.cpp", "kernels_encode.hip", "kernels_ntt.hip", "kernels_fused.hip"}
 
 
 def sources():

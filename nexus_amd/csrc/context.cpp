@@ -270,6 +270,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     tables.fwd_d = d_fwd_d;
     tables.inv_d = d_inv_d;
     tables.ninv_d = d_ninv_d;
+    tables.modup_conv_d = d_modup_conv_d;
     tables.mod = d_mod;
     upload_encoder_tables(*this);
 }
@@ -330,6 +331,12 @@ void Context::build_hybrid_tables()
     NX_HIP(hipMalloc(&d_moddown_conv, dc.size() * sizeof(uint64_t)));
     NX_HIP(hipMemcpy(d_modup_w, uw.data(), uw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_modup_conv, uc.data(), uc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    {
+        // the same factors as doubles (exact: every factor is below its prime, < 2^53 where used)
+        std::vector<double> ucd(uc.begin(), uc.end());
+        NX_HIP(hipMalloc(&d_modup_conv_d, ucd.size() * sizeof(double)));
+        NX_HIP(hipMemcpy(d_modup_conv_d, ucd.data(), ucd.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
     NX_HIP(hipMemcpy(d_moddown_w, dw.data(), dw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_moddown_conv, dc.data(), dc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
 }
@@ -345,6 +352,7 @@ Context::~Context()
     }
     (void)hipFree(d_modup_w);
     (void)hipFree(d_modup_conv);
+    (void)hipFree(d_modup_conv_d);
     (void)hipFree(d_moddown_w);
     (void)hipFree(d_moddown_conv);
     for (auto &kv : galois_) (void)hipFree(kv.second);

@@ -198,6 +198,7 @@ public:
     //   moddown_conv[alpha*L + i]: P mod q_i
     ulonglong2 *d_modup_w = nullptr, *d_moddown_w = nullptr;
     uint64_t *d_modup_conv = nullptr, *d_moddown_conv = nullptr;
+    double *d_modup_conv_d = nullptr;  // d_modup_conv as doubles
     const ulonglong2 *rescale_inv(size_t limbs) const { return d_rescale_inv + limbs * moduli.size(); }
 
     // Galois permutation tables (NTT domain), built on first use

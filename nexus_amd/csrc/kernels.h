@@ -21,6 +21,7 @@ struct DeviceTables {
     const double *fwd_d;     // [prime][n] the fwd powers as doubles (exact below 2^53)
     const double *inv_d;     // [prime][n] the inv powers as doubles
     const double *ninv_d;    // [prime] n^-1 mod q as doubles
+    const double *modup_conv_d = nullptr;  // hybrid ModUp factors as doubles (Context::d_modup_conv)
     const ModConst *mod;     // [prime]
 };
 

@@ -88,17 +88,38 @@ struct Sum3 {
 
 // k_modup_conv_fused for alpha = A (compile time: no per-term branches) with
 // every prime below 2^58 (Sum3 products); a partial last digit zero-fills.
+// Exact products in double precision for a < 2^51 and w < q < 2^51:
+// h = a w, e = fma(a, w, -h) = a w - h, Q = rint(h / q) within 1.25 of a w / q,
+// r = fma(-Q, q, h) + e = a w - Q q with |r| <= 1.25 q (every intermediate an
+// integer below 2^53); reduce_f maps |z| < 2^52 to |z| <= q/2 (1 + eps).
+__device__ __forceinline__ double mulmod_f(double a, double w, double q, double qinv)
+{
+    const double h = a * w;
+    const double e = __builtin_fma(a, w, -h);
+    const double Q = __builtin_rint(h * qinv);
+    return __builtin_fma(-Q, q, h) + e;
+}
+
+__device__ __forceinline__ double reduce_f(double z, double q, double qinv)
+{
+    return __builtin_fma(-__builtin_rint(z * qinv), q, z);
+}
+
+constexpr uint64_t kFpPrimeBound = 1ull << 51;
+constexpr uint32_t kConvChunk = 12;  // output primes per thread of k_modup_conv_a
+
 template <int A>
 __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ T, const uint64_t *__restrict__ coef,
                                                         const ulonglong2 *__restrict__ w,
                                                         const uint64_t *__restrict__ conv,
+                                                        const double *__restrict__ convd,
                                                         const ModConst *__restrict__ mod, OutMap outp,
                                                         uint32_t l, uint32_t digits, uint32_t nout,
                                                         uint32_t key_limbs, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
-    const uint32_t chunks = (nout + kOutChunk - 1) / kOutChunk;
+    const uint32_t chunks = (nout + kConvChunk - 1) / kConvChunk;
     const uint32_t row = blockIdx.y;  // (b * digits + j) * chunks + chunk
     const uint32_t bj = row / chunks, chunk = row - bj * chunks;
     const uint32_t b = bj / digits, j = bj - b * digits;
@@ -106,19 +127,53 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
     const uint32_t s = min((uint32_t)A, l - first);
     const uint32_t base = (j * A + s - 1) * A;
     const uint64_t *x = coef + (uint64_t)b * l * n + (uint64_t)first * n + c;
+    // the digit's residues are below 2^51 whenever its primes are
+    bool digit_fp = convd != nullptr;
+#pragma unroll
+    for (int i = 0; i < A; i++)
+        if ((uint32_t)i < s && mod[first + i].q >= kFpPrimeBound) digit_fp = false;
     uint64_t y[A];
+    double yd[A];
 #pragma unroll
     for (int i = 0; i < A; i++) {
         y[i] = 0;
+        yd[i] = 0.0;
         if ((uint32_t)i < s) {
             const ulonglong2 wi = w[base + i];
-            y[i] = shoup(x[(uint64_t)i * n], wi.x, wi.y, mod[first + i].q);
+            const uint64_t qi = mod[first + i].q;
+            if (digit_fp) {
+                // [x Dhat^-1]_q in double precision, mapped to [0, q)
+                const double q = (double)qi, qinv = 1.0 / q;
+                double v = reduce_f(mulmod_f((double)x[(uint64_t)i * n], (double)wi.x, q, qinv), q, qinv);
+                v = v < 0 ? v + q : v;
+                yd[i] = v;
+                y[i] = (uint64_t)v;
+            } else {
+                y[i] = shoup(x[(uint64_t)i * n], wi.x, wi.y, qi);
+                yd[i] = (double)y[i];
+            }
         }
     }
     uint64_t *out = T + ((uint64_t)b * nout * digits + j) * n + c;
-    const uint32_t I_end = min(nout, (chunk + 1) * kOutChunk);
-    for (uint32_t I = chunk * kOutChunk; I < I_end; I++) {
+    const uint32_t I_end = min(nout, (chunk + 1) * kConvChunk);
+    for (uint32_t I = chunk * kConvChunk; I < I_end; I++) {
         const uint32_t kp = outp.p[I];
+        const uint64_t qk = mod[kp].q;
+        if (digit_fp && qk < kFpPrimeBound) {
+            // sum of products in double precision, reduced every second term
+            const double *fd = convd + (uint64_t)base * key_limbs + kp;
+            const double q = (double)qk, qinv = 1.0 / q;
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < A; i++) {
+                if ((uint32_t)i < s) acc += mulmod_f(yd[i], fd[(uint64_t)i * key_limbs], q, qinv);
+                if (i & 1) acc = reduce_f(acc, q, qinv);
+            }
+            acc = reduce_f(acc, q, qinv);
+            acc = acc < 0 ? acc + q : acc;
+            out[(uint64_t)I * digits * n] = (uint64_t)acc;
+            continue;
+        }
         const uint64_t *f = conv + (uint64_t)base * key_limbs + kp;
         uint64_t fv[A];
 #pragma unroll
@@ -318,12 +373,13 @@ void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *
     ProfScope prof("ks_modup_conv", s, 8.0 * n * B * ((double)l + (double)nout * digits));
     const uint32_t chunks = (nout + kOutChunk - 1) / kOutChunk;
     const dim3 grid((n + kBlock - 1) / kBlock, B * digits * chunks);
+    const dim3 grid_a((n + kBlock - 1) / kBlock, B * digits * ((nout + kConvChunk - 1) / kConvChunk));
     if (t.max_bits <= 58) {
         switch (alpha) {
 #define NX_A(A)                                                                                                     \
     case A:                                                                                                         \
-        hipLaunchKernelGGL(k_modup_conv_a<A>, grid, dim3(kBlock), 0, s, T, coef, w, conv, t.mod, o, l, digits, nout, \
-                           key_limbs, n);                                                                           \
+        hipLaunchKernelGGL(k_modup_conv_a<A>, grid_a, dim3(kBlock), 0, s, T, coef, w, conv, t.modup_conv_d, t.mod, o, l, \
+                           digits, nout, key_limbs, n);                                                             \
         return;
             NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
             NX_A(14) NX_A(15) NX_A(16)
