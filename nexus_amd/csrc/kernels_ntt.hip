@@ -404,7 +404,12 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     const uint64_t q = a.mod[prime].q;
     const uint32_t N2 = ROLE == kCol ? (n >> LOGS) : (ROLE == kRow ? (uint32_t)G::S : n);
     const uint32_t N1 = n / N2;
-    const uint32_t first = blockIdx.x * G::C;  // first column / row of the tile
+    // Column tiles share 128-byte lines with their neighbours (CW x 8 bytes per
+    // row).  Workgroups go to the 8 XCDs round-robin, so consecutive tiles are
+    // given to workgroups on the same XCD (one L2) when the grid allows.
+    uint32_t tile = blockIdx.x;
+    if (ROLE == kCol && (gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t first = tile * G::C;  // first column / row of the tile
     if (q < kFpPrimeBound) {
         ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, src, prime, n, N2, N1, first);
         return;
