@@ -271,6 +271,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     tables.inv_d = d_inv_d;
     tables.ninv_d = d_ninv_d;
     tables.modup_conv_d = d_modup_conv_d;
+    tables.mod_host = moduli.data();
     tables.mod = d_mod;
     upload_encoder_tables(*this);
 }

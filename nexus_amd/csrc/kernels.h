@@ -23,6 +23,7 @@ struct DeviceTables {
     const double *ninv_d;    // [prime] n^-1 mod q as doubles
     const double *modup_conv_d = nullptr;  // hybrid ModUp factors as doubles (Context::d_modup_conv)
     const ModConst *mod;     // [prime]
+    const uint64_t *mod_host = nullptr;  // [prime] the primes, host copy (launch-time choices)
 };
 
 // A batch of in-place NTTs over polys x limbs RNS limbs of n words each.
@@ -176,8 +177,9 @@ void periodic_expand(const DeviceTables &t, uint64_t *out, const uint64_t *table
                      hipStream_t s);
 // out[m] = sum over terms (k, e) of output m of table[e] (.) in[k], both
 // polynomials, for K inputs and M outputs of `limbs` limbs (K, M <= kMixMax):
-// terms[first[m] .. first[m + 1]) as (k, e) int32 pairs.  Needs n >= 64 and
-// block_mix_lds_bytes(...) <= 64 KiB.
+// terms as runs[first[m] .. first[m + 1]) of int32 quads (k0, e0, len, de),
+// meaning (k0 + j, e0 + j de) for j < len (16-byte aligned).  Needs n >= 64
+// and block_mix_lds_bytes(...) <= 64 KiB.
 constexpr int kMixMax = 128;
 struct MixPtrs {
     const uint64_t *in[kMixMax];

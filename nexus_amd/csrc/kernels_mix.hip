@@ -83,14 +83,34 @@ struct Acc3 {
     }
 };
 
+// Exact products in double precision for 0 <= a, w < q < 2^51 (see the NTT):
+// r = a w - rint(a w / q) q with |r| <= 1.25 q; reduce_f maps |z| < 2^52 to
+// |z| <= q/2 (1 + eps).
+__device__ __forceinline__ double mulmod_f(double a, double w, double q, double qinv)
+{
+    const double h = a * w;
+    const double e = __builtin_fma(a, w, -h);
+    const double Q = __builtin_rint(h * qinv);
+    return __builtin_fma(-Q, q, h) + e;
+}
+
+__device__ __forceinline__ double reduce_f(double z, double q, double qinv)
+{
+    return __builtin_fma(-__builtin_rint(z * qinv), q, z);
+}
+
+constexpr uint64_t kFpPrimeBound = 1ull << 51;
+
 // One (poly, limb) row and a tile of 64 positions per workgroup; LDS holds the
-// tile of all K inputs and the weights of the tile's blocks.  terms[first[o] ..
-// first[o + 1]) lists (k, entry) of output o.  FAST: every prime below 2^58
+// tile of all K inputs and the weights of the tile's blocks.  runs[first[o] ..
+// first[o + 1]) list the terms of output o as runs (k0, entry0, len, dentry):
+// terms (k0 + j, entry0 + j dentry) for j < len, so the inner loop computes its
+// LDS addresses instead of loading them.  FAST: every prime below 2^58
 // (products folded every 16 terms through Acc3); otherwise 128-bit sums folded
 // every 32 terms.
 template <bool FAST>
 __global__ __launch_bounds__(kBlock) void k_block_mix(MixPtrs ptrs, const uint64_t *__restrict__ table,
-                                                      const int2 *__restrict__ terms,
+                                                      const int4 *__restrict__ runs,
                                                       const uint32_t *__restrict__ first, const ModConst *mod,
                                                       uint32_t K, uint32_t M, uint32_t limbs, uint32_t count,
                                                       uint32_t blocks, uint32_t log_stride, uint32_t n)
@@ -104,58 +124,112 @@ __global__ __launch_bounds__(kBlock) void k_block_mix(MixPtrs ptrs, const uint64
     uint64_t *xs = lds;               // [K][64]
     uint64_t *ws = lds + K * kTile;   // [count][bpt]
     const uint64_t roff = (uint64_t)row * n + pos0;
-    for (uint32_t e = threadIdx.x; e < K * kTile; e += kBlock) xs[e] = ptrs.in[e / kTile][roff + (e % kTile)];
-    for (uint32_t e = threadIdx.x; e < count * bpt; e += kBlock) {
-        const uint32_t p = e / bpt, lb = e - p * bpt;
-        ws[e] = table[((uint64_t)p * limbs + limb) * blocks + b0 + lb];
-    }
-    __syncthreads();
+    const ModConst m = mod[limb];
     const uint32_t lane = threadIdx.x % kTile;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kTile);
     const uint32_t lb = log_stride >= 6 ? 0u : lane >> log_stride;
-    const ModConst m = mod[limb];
-    for (uint32_t o = wave; o < M; o += kBlock / kTile) {
-        const uint32_t e0 = first[o], e1 = first[o + 1];
-        uint64_t r = 0;
-        if (FAST) {
-            uint32_t e = e0;
-            while (e < e1) {
-                const uint32_t stop = min(e1, e + 16);
-                Acc3 acc;
-                for (; e + 4 <= stop; e += 4) {
-                    const int2 t0 = terms[e], t1 = terms[e + 1], t2 = terms[e + 2], t3 = terms[e + 3];
-                    const uint64_t x0 = xs[t0.x * kTile + lane], x1 = xs[t1.x * kTile + lane];
-                    const uint64_t x2 = xs[t2.x * kTile + lane], x3 = xs[t3.x * kTile + lane];
-                    const uint64_t w0 = ws[t0.y * bpt + lb], w1 = ws[t1.y * bpt + lb];
-                    const uint64_t w2 = ws[t2.y * bpt + lb], w3 = ws[t3.y * bpt + lb];
-                    acc.mac(x0, w0);
-                    acc.mac(x1, w1);
-                    acc.mac(x2, w2);
-                    acc.mac(x3, w3);
-                }
-                for (; e < stop; e++) {
-                    const int2 t = terms[e];
-                    acc.mac(xs[t.x * kTile + lane], ws[t.y * bpt + lb]);
-                }
-                r = add_mod(r, acc.reduce(m), m.q);
+    if (m.q < kFpPrimeBound) {
+        // double precision: the tile and the weights as doubles, one exact
+        // product per term, the sum reduced every second term (|sum| < 3 q)
+        double *xd = reinterpret_cast<double *>(xs), *wd = reinterpret_cast<double *>(ws);
+        // wave w stages input rows w, w + 4, ...: the row (and its pointer) is
+        // uniform across the wave, eight loads in flight
+        {
+            const uint32_t w0 = wave;
+            uint32_t k = w0;
+            for (; k + 28 < K; k += 32) {
+                uint64_t v[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) v[u] = ptrs.in[k + 4 * u][roff + lane];
+#pragma unroll
+                for (int u = 0; u < 8; u++) xd[(k + 4 * u) * kTile + lane] = (double)v[u];
             }
-        } else {
-            uint64_t lo = 0, hi = 0;
-            uint32_t cnt = 0;
-            for (uint32_t e = e0; e < e1; e++) {
-                const int2 t = terms[e];
-                const uint64_t x = xs[t.x * kTile + lane], w = ws[t.y * bpt + lb];
-                const uint64_t plo = x * w, phi = mulhi64(x, w);
-                lo += plo;
-                hi += phi + (lo < plo);
-                if (++cnt == 32) {
-                    r = add_mod(r, barrett128(hi, lo, m), m.q);
-                    lo = hi = 0;
-                    cnt = 0;
-                }
-            }
-            r = add_mod(r, barrett128(hi, lo, m), m.q);
+            for (; k < K; k += 4) xd[k * kTile + lane] = (double)ptrs.in[k][roff + lane];
         }
+        for (uint32_t e = threadIdx.x; e < count * bpt; e += kBlock) {
+            const uint32_t p = e / bpt, b = e - p * bpt;
+            wd[e] = (double)table[((uint64_t)p * limbs + limb) * blocks + b0 + b];
+        }
+        __syncthreads();
+        const double q = (double)m.q, qinv = 1.0 / q;
+        for (uint32_t o = wave; o < M; o += kBlock / kTile) {
+            // two independent sums (each reduced after every second term of its
+            // own) so consecutive products do not wait on one another
+            double a0 = 0.0, a1 = 0.0;
+            for (uint32_t ri = first[o]; ri < first[o + 1]; ri++) {
+                // a run: k = k0 + j, entry = p0 + j dp for j < len (indices computed, not loaded)
+                const int4 rn = runs[ri];
+                const double *xp = xd + rn.x * kTile + lane;
+                const int xstep = kTile, wstep = rn.w * (int)bpt;
+                const double *wp = wd + rn.y * (int)bpt + lb;
+                int j = 0;
+                for (; j + 4 <= rn.z; j += 4) {
+                    a0 += mulmod_f(xp[0], wp[0], q, qinv);
+                    a1 += mulmod_f(xp[xstep], wp[wstep], q, qinv);
+                    a0 += mulmod_f(xp[2 * xstep], wp[2 * wstep], q, qinv);
+                    a1 += mulmod_f(xp[3 * xstep], wp[3 * wstep], q, qinv);
+                    a0 = reduce_f(a0, q, qinv);
+                    a1 = reduce_f(a1, q, qinv);
+                    xp += 4 * xstep;
+                    wp += 4 * wstep;
+                }
+                for (; j < rn.z; j++) {
+                    a0 = reduce_f(a0 + mulmod_f(xp[0], wp[0], q, qinv), q, qinv);
+                    xp += xstep;
+                    wp += wstep;
+                }
+            }
+            double acc = reduce_f(a0 + a1, q, qinv);
+            acc = acc < 0 ? acc + q : acc;
+            ptrs.out[o][roff + lane] = (uint64_t)acc;
+        }
+        return;
+    }
+    {
+        uint32_t k = wave;
+        for (; k + 28 < K; k += 32) {
+            uint64_t v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) v[u] = ptrs.in[k + 4 * u][roff + lane];
+#pragma unroll
+            for (int u = 0; u < 8; u++) xs[(k + 4 * u) * kTile + lane] = v[u];
+        }
+        for (; k < K; k += 4) xs[k * kTile + lane] = ptrs.in[k][roff + lane];
+    }
+    for (uint32_t e = threadIdx.x; e < count * bpt; e += kBlock) {
+        const uint32_t p = e / bpt, b = e - p * bpt;
+        ws[e] = table[((uint64_t)p * limbs + limb) * blocks + b0 + b];
+    }
+    __syncthreads();
+    for (uint32_t o = wave; o < M; o += kBlock / kTile) {
+        uint64_t r = 0;
+        Acc3 acc;
+        uint64_t lo = 0, hi = 0;
+        uint32_t cnt = 0;
+        for (uint32_t ri = first[o]; ri < first[o + 1]; ri++) {
+            const int4 rn = runs[ri];
+            for (int j = 0; j < rn.z; j++) {
+                const uint64_t x = xs[(rn.x + j) * kTile + lane], w = ws[(rn.y + j * rn.w) * (int)bpt + lb];
+                if (FAST) {
+                    acc.mac(x, w);
+                    if (++cnt == 16) {
+                        r = add_mod(r, acc.reduce(m), m.q);
+                        acc = Acc3();
+                        cnt = 0;
+                    }
+                } else {
+                    const uint64_t plo = x * w, phi = mulhi64(x, w);
+                    lo += plo;
+                    hi += phi + (lo < plo);
+                    if (++cnt == 32) {
+                        r = add_mod(r, barrett128(hi, lo, m), m.q);
+                        lo = hi = 0;
+                        cnt = 0;
+                    }
+                }
+            }
+        }
+        r = FAST ? add_mod(r, acc.reduce(m), m.q) : add_mod(r, barrett128(hi, lo, m), m.q);
         ptrs.out[o][roff + lane] = r;
     }
 }
@@ -209,11 +283,11 @@ void block_mix(const DeviceTables &t, const MixPtrs &p, const uint64_t *table, c
     const dim3 grid(n / kTile, 2 * limbs);
     if (t.max_bits <= 58)
         hipLaunchKernelGGL(k_block_mix<true>, grid, dim3(kBlock), lds, s, p, table,
-                           reinterpret_cast<const int2 *>(terms), first, t.mod, K, M, limbs, count, blocks,
+                           reinterpret_cast<const int4 *>(terms), first, t.mod, K, M, limbs, count, blocks,
                            log_stride, n);
     else
         hipLaunchKernelGGL(k_block_mix<false>, grid, dim3(kBlock), lds, s, p, table,
-                           reinterpret_cast<const int2 *>(terms), first, t.mod, K, M, limbs, count, blocks,
+                           reinterpret_cast<const int4 *>(terms), first, t.mod, K, M, limbs, count, blocks,
                            log_stride, n);
 }
 

@@ -46,6 +46,7 @@ struct NttKArgs {
     uint64_t src_poly_stride;
     uint32_t nsrcs;       // > 0: poly p of this pass reads srcs[p]
     const uint64_t *srcs[kNttSrcMax];
+    double tw_bytes;      // mean twiddle bytes per entry over the batch (profiler accounting)
     uint32_t limbs;
     uint32_t logn;
     uint32_t prime_by_poly;
@@ -504,9 +505,10 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     static const char *names[2][3] = {{"ntt_inv_whole", "ntt_inv_col", "ntt_inv_row"},
                                       {"ntt_fwd_whole", "ntt_fwd_col", "ntt_fwd_row"}};
     // algorithmic bytes: each limb read + written once (16 n) plus the pass's
-    // twiddle entries {w, w'} (16 B each)
+    // twiddle entries: {w, w'} (16 B) on the 64-bit path, w (8 B) in double
+    // precision
     const double tw = ROLE == kWhole ? (double)n : (ROLE == kCol ? (double)G::S : (double)(n - n / G::S));
-    ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)batch * (16.0 * n + 16.0 * tw));
+    ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)batch * (16.0 * n + a.tw_bytes * tw));
     hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
 }
 
@@ -570,6 +572,14 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
     a.src_poly_stride = b.src_poly_stride;
     a.nsrcs = 0;
     for (int i = 0; i < kNttSrcMax; i++) a.srcs[i] = nullptr;
+    {
+        // limbs whose prime takes the double-precision path read 8-byte twiddles
+        const uint32_t m = b.prime_by_poly ? (b.poly_period ? b.poly_period : b.polys) : b.limbs;
+        const uint32_t cnt = m < (uint32_t)kMaxPrimes ? m : (uint32_t)kMaxPrimes;
+        uint32_t fp = 0;
+        for (uint32_t i = 0; i < cnt; i++) fp += t.mod_host[b.prime_of[i]] < kFpPrimeBound ? 1u : 0u;
+        a.tw_bytes = cnt ? (8.0 * fp + 16.0 * (cnt - fp)) / cnt : 16.0;
+    }
     if (b.src_list) {
         // a source per poly: the batch must fit the argument array
         if (b.polys > (uint32_t)kNttSrcMax) abort();

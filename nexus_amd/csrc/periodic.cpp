@@ -133,21 +133,38 @@ void op_periodic_mix(const std::vector<const Ciphertext *> &in, PeriodicPlain &w
         std::lock_guard<std::mutex> lk(w.mu);
         auto it = w.terms.find(key);
         if (it == w.terms.end()) {
-            // first[M + 1] (u32, padded to an even count) then (k, entry) pairs
-            std::vector<int32_t> h(M + 1 + ((M + 1) & 1));
-            std::vector<int32_t> pairs;
+            // first[M + 1] (u32, padded to a multiple of four) then runs (k0,
+            // entry0, len, dentry) as int32 quads: consecutive terms of an output
+            // with k + 1 and a constant entry step share one run
+            const size_t head = (M + 1 + 3) & ~(size_t)3;
+            std::vector<int32_t> h(head, 0);
+            std::vector<int32_t> runs;
             for (size_t m = 0; m < M; m++) {
-                h[m] = (int32_t)(pairs.size() / 2);
-                for (size_t k = 0; k < K; k++) {
+                h[m] = (int32_t)(runs.size() / 4);
+                size_t k = 0;
+                while (k < K) {
                     const int32_t e = index[m * K + k];
-                    if (e < 0) continue;
-                    pairs.push_back((int32_t)k);
-                    pairs.push_back(e);
+                    if (e < 0) {
+                        k++;
+                        continue;
+                    }
+                    int32_t len = 1, de = 0;
+                    if (k + 1 < K && index[m * K + k + 1] >= 0) {
+                        de = index[m * K + k + 1] - e;
+                        while (k + len < K && index[m * K + k + len] >= 0 &&
+                               index[m * K + k + len] == e + (int32_t)len * de)
+                            len++;
+                    }
+                    runs.push_back((int32_t)k);
+                    runs.push_back(e);
+                    runs.push_back(len);
+                    runs.push_back(de);
+                    k += (size_t)len;
                 }
             }
-            h[M] = (int32_t)(pairs.size() / 2);
-            const uint32_t nterms = (uint32_t)(pairs.size() / 2);
-            h.insert(h.end(), pairs.begin(), pairs.end());
+            h[M] = (int32_t)(runs.size() / 4);
+            const uint32_t nterms = (uint32_t)(runs.size() / 4);
+            h.insert(h.end(), runs.begin(), runs.end());
             PeriodicPlain::Terms t;
             t.words.reset(&c, (h.size() + 1) / 2);
             // stream-ordered: the buffer may come from work still queued on the stream
@@ -174,8 +191,8 @@ void op_periodic_mix(const std::vector<const Ciphertext *> &in, PeriodicPlain &w
         p.out[m] = res[m].data.get();
     }
     const uint32_t *first = reinterpret_cast<const uint32_t *>(terms->words.get());
-    const int32_t *pairs = reinterpret_cast<const int32_t *>(first + M + 1 + ((M + 1) & 1));
-    block_mix(c.tables, p, w.table.get(), pairs, first, (uint32_t)K, (uint32_t)M, terms->nterms, (uint32_t)l,
+    const int32_t *runs = reinterpret_cast<const int32_t *>(first + ((M + 1 + 3) & ~(size_t)3));
+    block_mix(c.tables, p, w.table.get(), runs, first, (uint32_t)K, (uint32_t)M, terms->nterms, (uint32_t)l,
               (uint32_t)w.count, (uint32_t)w.blocks, c.stream);
     for (size_t m = 0; m < M; m++) *out[m] = std::move(res[m]);
 }
