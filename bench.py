@@ -370,7 +370,10 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the nonlinear, bootstrapping and MatMul extras")
     ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     ap.add_argument("--requests", type=int, default=12)
-    ap.add_argument("--lanes", type=int, default=3)
+    ap.add_argument("--lanes", type=int, default=3, help="lanes of the served nonlinear-block measurement")
+    ap.add_argument("--bert-lanes", type=int, default=1,
+                    help="lanes of the served BERT measurement (one BERT call is in flight at a time, and each "
+                         "lane caches its own call working set)")
     ap.add_argument("--only", choices=["bert", "bert_served", "dot", "bootstrap_dense", "bootstrap_sparse", "matmul",
                                        "ntt"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
@@ -615,14 +618,14 @@ def served_bert(nx, B, args, plain_call_s):
     log("served BERT: %d requests, lengths %s, Poisson arrivals at %.3f/s" % (n, lens, rate))
     span1, st1 = run(1, False)
     log("plain per-call (1 lane): %.1f s, p50 %.0f ms, p99 %.0f ms" % (span1, st1["p50_ms"], st1["p99_ms"]))
-    spanb, stb = run(args.lanes, True)
+    spanb, stb = run(args.bert_lanes, True)
     log("served (%d lanes, packing): %.1f s, p50 %.0f ms, p99 %.0f ms, %d packed calls carrying %d requests"
-        % (args.lanes, spanb, stb["p50_ms"], stb["p99_ms"], stb["packed_calls"], stb["packed_requests"]))
+        % (args.bert_lanes, spanb, stb["p50_ms"], stb["p99_ms"], stb["packed_calls"], stb["packed_requests"]))
     return {
         "workload": "single-sequence BERT-base layer requests, lengths ~U{16..128} (%s), Poisson arrivals at "
                     "%.3f requests/s (2x the plain path's capacity); served = %d lanes, one BERT call in flight, "
-                    "continuous batching of queued requests" % (lens, rate, args.lanes),
-        "requests": n, "lanes": args.lanes,
+                    "continuous batching of queued requests" % (lens, rate, args.bert_lanes),
+        "requests": n, "lanes": args.bert_lanes,
         "served": {"layers_per_s": n / spanb, "p50_ms": stb["p50_ms"], "p99_ms": stb["p99_ms"],
                    "mean_ms": stb["mean_ms"], "makespan_s": spanb, "packed_calls": stb["packed_calls"],
                    "packed_requests": stb["packed_requests"], "peak_concurrent": stb["peak_concurrent"]},
