@@ -223,10 +223,12 @@ public:
     void multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                                    Ciphertext &dest) const;
     // batched: dest[i] = rescale(cts[i]) / rescale(relinearize(a[i] * b[i])),
-    // word for word the per-ciphertext results (dest may alias the inputs)
+    // word for word the per-ciphertext results (dest may alias the inputs);
+    // fused: relinearisation and rescale in one rounding (hybrid key
+    // switching), each coefficient within 1 of the unfused result
     void rescale_to_next_many(const std::vector<const Ciphertext *> &cts, std::vector<Ciphertext> &dest) const;
     void multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
-                                     const RelinKeys &rk, std::vector<Ciphertext> &dest) const;
+                                     const RelinKeys &rk, std::vector<Ciphertext> &dest, bool fused = false) const;
     // out[m] = sum_k w[index[m K + k]] (.) cts[k] (index -1: no term), M = index.size() / K
     void multiply_periodic_mix(const std::vector<const Ciphertext *> &cts, const PeriodicPlaintexts &w,
                                const std::vector<int32_t> &index, std::vector<Ciphertext> &out) const;

@@ -555,6 +555,7 @@ def multiply_plain_accumulate(cts, pts, dest=None):
 
 _sig("nx_rescale_to_next_many", [ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
 _sig("nx_multiply_relin_rescale_many", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp, ctypes.POINTER(vp)])
+_sig("nx_multiply_relin_rescale_fused_many", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp, ctypes.POINTER(vp)])
 
 
 def rescale_to_next_many(cts):
@@ -565,11 +566,15 @@ def rescale_to_next_many(cts):
     return outs
 
 
-def multiply_relin_rescale_many(a, b, relin_keys):
-    """[rescale_to_next(relinearize(x * y)) for x, y in zip(a, b)] as one batched pass"""
+def multiply_relin_rescale_many(a, b, relin_keys, fused=False):
+    """[rescale_to_next(relinearize(x * y)) for x, y in zip(a, b)] as one batched pass.
+
+    fused=True relinearises and rescales in one rounding (hybrid key switching):
+    every coefficient is within 1 of the unfused result."""
     outs = [Ciphertext(a[0].ctx) for _ in a]
-    _check(_lib.nx_multiply_relin_rescale_many((vp * len(a))(*[c.h for c in a]), (vp * len(b))(*[c.h for c in b]),
-                                               len(a), relin_keys.h, (vp * len(outs))(*[o.h for o in outs])))
+    fn = _lib.nx_multiply_relin_rescale_fused_many if fused else _lib.nx_multiply_relin_rescale_many
+    _check(fn((vp * len(a))(*[c.h for c in a]), (vp * len(b))(*[c.h for c in b]), len(a), relin_keys.h,
+              (vp * len(outs))(*[o.h for o in outs])))
     return outs
 
 

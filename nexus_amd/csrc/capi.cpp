@@ -573,8 +573,8 @@ int nx_rescale_to_next_many(const nx_ciphertext *const *cts, size_t count, nx_ci
     });
 }
 
-int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
-                                   const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests)
+static int multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
+                                       const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests, bool fused)
 {
     return guard([&] {
         need(a, "a");
@@ -591,8 +591,23 @@ int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphe
             y.push_back(&b[i]->impl);
             out.push_back(&dests[i]->impl);
         }
-        nx::op_multiply_relin_rescale_many(x, y, relin_keys->impl, out);
+        if (fused)
+            nx::op_multiply_relin_rescale_fused_many(x, y, relin_keys->impl, out);
+        else
+            nx::op_multiply_relin_rescale_many(x, y, relin_keys->impl, out);
     });
+}
+
+int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
+                                   const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests)
+{
+    return multiply_relin_rescale_many(a, b, count, relin_keys, dests, false);
+}
+
+int nx_multiply_relin_rescale_fused_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
+                                         const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests)
+{
+    return multiply_relin_rescale_many(a, b, count, relin_keys, dests, true);
 }
 
 int nx_relinearize(const nx_ciphertext *a, const nx_kswitch_keys *relin_keys, nx_ciphertext *dest)

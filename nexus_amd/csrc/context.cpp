@@ -326,6 +326,51 @@ void Context::build_hybrid_tables()
         for (size_t k = 0; k < alpha; k++) v = host_mul_mod(v, moduli[L + k] % qi, qi);
         dc[alpha * L + i] = v;
     }
+    // ModDown by P' = P q_(l-1) (relinearisation and rescale in one step), for
+    // every level l >= 2: base b_0 = q_(l-1), b_k = p_(k-1)
+    {
+        const size_t A = alpha + 1, W = A + 1;
+        std::vector<ulonglong2> fw((L + 1) * A, make_ulonglong2(0, 0));
+        std::vector<uint64_t> fc((L + 1) * W * L, 0);
+        std::vector<ulonglong2> fi((L + 1) * L, make_ulonglong2(0, 0));
+        mdr_pmod.assign(L + 1, make_ulonglong2(0, 0));
+        for (size_t l = 2; l <= L; l++) {
+            const uint64_t ql = moduli[l - 1];
+            for (size_t k = 0; k < A; k++) {
+                const uint64_t bk = k ? moduli[L + k - 1] : ql;
+                // P' / b_k mod b_k
+                uint64_t h = k ? ql % bk : 1;
+                for (size_t k2 = 0; k2 < alpha; k2++)
+                    if (!k || k2 != k - 1) h = host_mul_mod(h, moduli[L + k2] % bk, bk);
+                const uint64_t w = host_inv_mod(h, bk);
+                fw[l * A + k] = make_ulonglong2(w, shoup_quotient(w, bk));
+                for (size_t i = 0; i + 1 < l; i++) {
+                    const uint64_t qi = moduli[i];
+                    uint64_t v = k ? ql % qi : 1;
+                    for (size_t k2 = 0; k2 < alpha; k2++)
+                        if (!k || k2 != k - 1) v = host_mul_mod(v, moduli[L + k2] % qi, qi);
+                    fc[(l * W + k) * L + i] = v;
+                }
+            }
+            for (size_t i = 0; i + 1 < l; i++) {
+                const uint64_t qi = moduli[i];
+                uint64_t v = ql % qi;
+                for (size_t k2 = 0; k2 < alpha; k2++) v = host_mul_mod(v, moduli[L + k2] % qi, qi);
+                fc[(l * W + A) * L + i] = v;  // P' mod q_i
+                const uint64_t iv = host_inv_mod(v, qi);
+                fi[l * L + i] = make_ulonglong2(iv, shoup_quotient(iv, qi));
+            }
+            uint64_t pm = 1;
+            for (size_t k2 = 0; k2 < alpha; k2++) pm = host_mul_mod(pm, moduli[L + k2] % ql, ql);
+            mdr_pmod[l] = make_ulonglong2(pm, shoup_quotient(pm, ql));  // P mod q_(l-1)
+        }
+        NX_HIP(hipMalloc(&d_mdr_w, fw.size() * sizeof(ulonglong2)));
+        NX_HIP(hipMalloc(&d_mdr_conv, fc.size() * sizeof(uint64_t)));
+        NX_HIP(hipMalloc(&d_mdr_inv, fi.size() * sizeof(ulonglong2)));
+        NX_HIP(hipMemcpy(d_mdr_w, fw.data(), fw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
+        NX_HIP(hipMemcpy(d_mdr_conv, fc.data(), fc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+        NX_HIP(hipMemcpy(d_mdr_inv, fi.data(), fi.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
+    }
     NX_HIP(hipMalloc(&d_modup_w, uw.size() * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_modup_conv, uc.size() * sizeof(uint64_t)));
     NX_HIP(hipMalloc(&d_moddown_w, dw.size() * sizeof(ulonglong2)));
@@ -354,6 +399,9 @@ Context::~Context()
     (void)hipFree(d_modup_w);
     (void)hipFree(d_modup_conv);
     (void)hipFree(d_modup_conv_d);
+    (void)hipFree(d_mdr_w);
+    (void)hipFree(d_mdr_conv);
+    (void)hipFree(d_mdr_inv);
     (void)hipFree(d_moddown_w);
     (void)hipFree(d_moddown_conv);
     for (auto &kv : galois_) (void)hipFree(kv.second);

@@ -199,6 +199,14 @@ public:
     ulonglong2 *d_modup_w = nullptr, *d_moddown_w = nullptr;
     uint64_t *d_modup_conv = nullptr, *d_moddown_conv = nullptr;
     double *d_modup_conv_d = nullptr;  // d_modup_conv as doubles
+    // ModDown by P' = P q_(l-1) at level l (hybrid relinearisation + rescale in one
+    // rounding), base (q_(l-1), p_0..p_(alpha-1)):
+    //   mdr_w[l (alpha+1) + k] = [(P'/b_k)^-1]_(b_k) (Shoup)
+    //   mdr_conv[(l (alpha+2) + k) L + i] = P'/b_k mod q_i, row alpha+1: P' mod q_i
+    //   mdr_inv[l L + i] = {P'^-1 mod q_i, Shoup};  mdr_pmod[l] = P mod q_(l-1) (host, Shoup)
+    ulonglong2 *d_mdr_w = nullptr, *d_mdr_inv = nullptr;
+    uint64_t *d_mdr_conv = nullptr;
+    std::vector<ulonglong2> mdr_pmod;
     const ulonglong2 *rescale_inv(size_t limbs) const { return d_rescale_inv + limbs * moduli.size(); }
 
     // Galois permutation tables (NTT domain), built on first use
@@ -252,6 +260,11 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst);
 void op_rescale_many(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out);
 void op_multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
                                     const KSwitchKeys &rk, const std::vector<Ciphertext *> &out);
+// the same products with relinearisation and rescale in one rounding (hybrid
+// key switching): each coefficient within 1 of the form above
+void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> &a,
+                                          const std::vector<const Ciphertext *> &b, const KSwitchKeys &rk,
+                                          const std::vector<Ciphertext *> &out);
 void op_mod_switch_to_next(const Ciphertext &a, Ciphertext &dst);
 void op_mod_switch_to(const Ciphertext &a, size_t chain_index, Ciphertext &dst);
 void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &dst);

@@ -487,7 +487,7 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
             {
                 std::vector<const Ciphertext *> qa(hi - lo, &qc), kb;
                 for (auto &x : kr) kb.push_back(&x);
-                ev.multiply_relin_rescale_many(qa, kb, rk, P);
+                ev.multiply_relin_rescale_many(qa, kb, rk, P, true);
             }
             kr.clear();
             mark("attention.qk_products");

@@ -630,7 +630,7 @@ Bootstrapper::Batch Bootstrapper::mul_exact_many(const Batch &a, const Batch &b)
         y.push_back(&a == &b ? pa : pb);
     }
     Batch r;
-    ck_.evaluator.multiply_relin_rescale_many(x, y, *ck_.relin_keys, r);
+    ck_.evaluator.multiply_relin_rescale_many(x, y, *ck_.relin_keys, r, true);
     return r;
 }
 
@@ -726,7 +726,7 @@ Bootstrapper::Batch Bootstrapper::cheb_eval_many(const std::vector<double> &c, s
         }
     }
     Batch prod;
-    ck_.evaluator.multiply_relin_rescale_many(ptrs(Q), T, *ck_.relin_keys, prod);
+    ck_.evaluator.multiply_relin_rescale_many(ptrs(Q), T, *ck_.relin_keys, prod, true);
     Batch R = cheb_eval_many(r, level, scale);
     for (size_t i = 0; i < B; i++) {
         prod[i].set_scale(scale);
@@ -757,7 +757,7 @@ void Bootstrapper::eval_mod_many(const std::vector<const Ciphertext *> &u, std::
     basis_many_.clear();
     for (int i = 0; i < p_.double_angles; i++) {
         const std::vector<const Ciphertext *> p = ptrs(y);
-        ck_.evaluator.multiply_relin_rescale_many(p, p, *ck_.relin_keys, y);
+        ck_.evaluator.multiply_relin_rescale_many(p, p, *ck_.relin_keys, y, true);
         for (auto &x : y) ck_.evaluator.add_inplace(x, x);
         Plaintext one(*ck_.context);
         ck_.encoder.encode(1.0, y[0].chain_index(), y[0].scale(), one);

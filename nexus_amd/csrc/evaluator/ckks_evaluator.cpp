@@ -151,10 +151,10 @@ void CKKSEvaluator::eval_odd_deg9_poly_many(const std::vector<double> &a, std::v
     const double D = X[0].scale();
     const uint64_t p = get_modulus(X[0], 2), q = get_modulus(X[0], 3), r = get_modulus(X[0], 4);
     std::vector<Ciphertext> x2, x3, x6, xs(B);
-    ev.multiply_relin_rescale_many(ptrs_of(X), ptrs_of(X), *relin_keys, x2);
+    ev.multiply_relin_rescale_many(ptrs_of(X), ptrs_of(X), *relin_keys, x2, true);
     for (size_t i = 0; i < B; i++) ev.mod_switch_to(X[i], X[i].params_id() + 1, xs[i]);
-    ev.multiply_relin_rescale_many(ptrs_of(x2), ptrs_of(xs), *relin_keys, x3);
-    ev.multiply_relin_rescale_many(ptrs_of(x3), ptrs_of(x3), *relin_keys, x6);
+    ev.multiply_relin_rescale_many(ptrs_of(x2), ptrs_of(xs), *relin_keys, x3, true);
+    ev.multiply_relin_rescale_many(ptrs_of(x3), ptrs_of(x3), *relin_keys, x6, true);
     Plaintext a1, a3, a5, a7, a9;
     // T1
     std::vector<Ciphertext> T1(B);
@@ -163,7 +163,7 @@ void CKKSEvaluator::eval_odd_deg9_poly_many(const std::vector<double> &a, std::v
     rescale_all(ev, T1);
     encoder.encode(a[3], T1[0].params_id(), T1[0].scale(), a3);
     for (auto &t : T1) ev.add_plain_inplace(t, a3);
-    ev.multiply_relin_rescale_many(ptrs_of(T1), ptrs_of(x3), *relin_keys, T1);
+    ev.multiply_relin_rescale_many(ptrs_of(T1), ptrs_of(x3), *relin_keys, T1, true);
     // T2
     std::vector<Ciphertext> T2(B), a7x(B);
     encoder.encode(a[9], x3[0].params_id(), D / x3[0].scale() * r / x6[0].scale() * q, a9);
@@ -179,7 +179,7 @@ void CKKSEvaluator::eval_odd_deg9_poly_many(const std::vector<double> &a, std::v
         a7x[i].set_scale(mid);
         ev.add_inplace(T2[i], a7x[i]);
     }
-    ev.multiply_relin_rescale_many(ptrs_of(T2), ptrs_of(x6), *relin_keys, T2);
+    ev.multiply_relin_rescale_many(ptrs_of(T2), ptrs_of(x6), *relin_keys, T2, true);
     // T3
     std::vector<Ciphertext> T3(B);
     encoder.encode(a[1], xs[0].params_id(), (double)p, a1);
@@ -247,21 +247,21 @@ void GELUEvaluator::gelu_many(const std::vector<const Ciphertext *> &xin, std::v
     // powers: x2, x4, then (x6 = x2 x4, x8 = x4^2), then (x10 = x2 x8, x12 = x6^2),
     // x2 switched down to each product's level as gelu() does
     std::vector<Ciphertext> x2, x4, p68, p1012;
-    ev.multiply_relin_rescale_many(xin, xin, *ck.relin_keys, x2);
-    ev.multiply_relin_rescale_many(ptrs_of(x2), ptrs_of(x2), *ck.relin_keys, x4);
+    ev.multiply_relin_rescale_many(xin, xin, *ck.relin_keys, x2, true);
+    ev.multiply_relin_rescale_many(ptrs_of(x2), ptrs_of(x2), *ck.relin_keys, x4, true);
     for (auto &c : x2) ev.mod_switch_to_inplace(c, x4[0].params_id());
     {
         std::vector<const Ciphertext *> l, r;
         for (size_t i = 0; i < B; i++) l.push_back(&x2[i]), r.push_back(&x4[i]);
         for (size_t i = 0; i < B; i++) l.push_back(&x4[i]), r.push_back(&x4[i]);
-        ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, p68);  // x6 = [0, B), x8 = [B, 2B)
+        ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, p68, true);  // x6 = [0, B), x8 = [B, 2B)
     }
     for (auto &c : x2) ev.mod_switch_to_inplace(c, p68[B].params_id());
     {
         std::vector<const Ciphertext *> l, r;
         for (size_t i = 0; i < B; i++) l.push_back(&x2[i]), r.push_back(&p68[B + i]);
         for (size_t i = 0; i < B; i++) l.push_back(&p68[i]), r.push_back(&p68[i]);
-        ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, p1012);  // x10, x12
+        ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, p1012, true);  // x10, x12
     }
     const double A[] = {2.25775755e-04, 0.5, 3.96880960e-01, -6.37042698e-02, 8.38841647e-03, -7.17830961e-04,
                         3.49617829e-05, -7.26059653e-07};
@@ -301,7 +301,7 @@ void GELUEvaluator::gelu_many(const std::vector<const Ciphertext *> &xin, std::v
         r.push_back(&a1[i]);
     }
     for (size_t i = 0; i < B; i++) l.push_back(&xs[i]), r.push_back(&a2[i]);
-    ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, s);  // s1 = [0, B), s2 = [B, 2B)
+    ev.multiply_relin_rescale_many(l, r, *ck.relin_keys, s, true);  // s1 = [0, B), s2 = [B, 2B)
     std::vector<Ciphertext> out(B);
     for (size_t i = 0; i < B; i++) {
         s[i].set_scale(ck.scale);

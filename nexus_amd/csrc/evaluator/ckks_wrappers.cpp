@@ -444,7 +444,7 @@ void Evaluator::rescale_to_next_many(const std::vector<const Ciphertext *> &cts,
 
 void Evaluator::multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a,
                                             const std::vector<const Ciphertext *> &b, const RelinKeys &rk,
-                                            std::vector<Ciphertext> &dest) const
+                                            std::vector<Ciphertext> &dest, bool fused) const
 {
     if (a.size() != b.size()) throw std::invalid_argument("multiply_relin_rescale_many: operand counts differ");
     std::vector<const nx_ciphertext *> x, y;
@@ -459,7 +459,10 @@ void Evaluator::multiply_relin_rescale_many(const std::vector<const Ciphertext *
         res.emplace_back(*ctx_);
         o.push_back(res.back().get());
     }
-    check(nx_multiply_relin_rescale_many(x.data(), y.data(), x.size(), rk.get(), o.data()));
+    if (fused)
+        check(nx_multiply_relin_rescale_fused_many(x.data(), y.data(), x.size(), rk.get(), o.data()));
+    else
+        check(nx_multiply_relin_rescale_many(x.data(), y.data(), x.size(), rk.get(), o.data()));
     dest = std::move(res);
 }
 

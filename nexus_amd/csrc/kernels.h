@@ -93,6 +93,15 @@ void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const 
 void rns_sub_mul_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *const *x, const uint64_t *tmp,
                       const ulonglong2 *inv, uint32_t count, uint32_t polys, uint32_t l_out, uint32_t x_limbs,
                       hipStream_t s);
+// acc[p][limb][c] += d[p][limb][c] * w (Shoup constant of prime `limb`) for polys p
+void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stride, const uint64_t *d,
+                         uint64_t d_stride, uint32_t limb, ulonglong2 w, uint32_t polys, hipStream_t s);
+// out_r[k][i] = (x[2r+k][i] - tmp[2r+k][i]) inv1[i] + d[2r+k][i] inv2[i] for i < l_out: the
+// final step of a ModDown by P q_(l-1) (x = the key-switch accumulator, d = the
+// data part (c0, c1) of the product)
+void rns_sub_mul2_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *x, uint64_t x_stride,
+                       const uint64_t *d, uint64_t d_stride, const uint64_t *tmp, const ulonglong2 *inv1,
+                       const ulonglong2 *inv2, uint32_t count, uint32_t l_out, hipStream_t s);
 // out[p][i] = a[p][i] * w[i] with per-limb Shoup constants {w, w'} (host array, passed by value)
 void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,
                      uint32_t polys, uint32_t limbs, hipStream_t s);
@@ -153,6 +162,15 @@ void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *
 void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
                               const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
                               uint32_t L, hipStream_t s);
+// ModDown by P' = P q_(l-1) in one rounding: the limbs (q_(l-1), p_0..p_(alpha-1))
+// of each poly of acc (starting at limb l_out = l - 1, poly stride acc_stride,
+// coefficient form) converted exactly (centred) to q_0..q_(l_out - 1) into
+// z[polys][l_out][n], with the per-level tables Context::d_mdr_*.  Needs
+// ks_moddown_rescale_ok (every prime below 2^58, alpha + 1 <= 16).
+bool ks_moddown_rescale_ok(const DeviceTables &t, uint32_t alpha);
+void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                                const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l_out,
+                                uint32_t alpha, uint32_t L, hipStream_t s);
 // out[k][limb] = sum_r cts[r][k][limb] * pts[r][limb] for k < 2 (count <= kMaxDot):
 // the plaintext multiply-accumulate of a BSGS giant step in one pass.
 constexpr int kMaxDot = 64;

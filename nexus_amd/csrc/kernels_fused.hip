@@ -18,6 +18,11 @@ constexpr int kBlock = 256;
 constexpr uint32_t kRegAlpha = 16;  // digits / special primes held in registers
 constexpr uint32_t kOutChunk = 4;   // output primes per thread (more waves in flight)
 
+// prime index of each input limb of a ModDown conversion
+struct InPrimes {
+    uint8_t p[kRegAlpha + 1];
+};
+
 struct OutMap {
     uint8_t p[kMaxPrimes];
 };
@@ -193,8 +198,8 @@ __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict_
                                                           const uint64_t *__restrict__ acc, uint64_t acc_stride,
                                                           const ulonglong2 *__restrict__ w,
                                                           const uint64_t *__restrict__ conv,
-                                                          const ModConst *__restrict__ mod, uint32_t l, uint32_t L,
-                                                          uint32_t n)
+                                                          const ModConst *__restrict__ mod, InPrimes inp, uint32_t l,
+                                                          uint32_t L, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -206,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict_
 #pragma unroll
     for (int k = 0; k < A; k++) {
         const ulonglong2 wk = w[k];
-        const uint64_t pk = mod[L + k].q;
+        const uint64_t pk = mod[inp.p[k]].q;
         v[k] = shoup(y[(uint64_t)k * n], wk.x, wk.y, pk);
         frac += (double)v[k] / (double)pk;
     }
@@ -397,14 +402,16 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
 {
     const uint32_t n = 1u << t.logn;
     ProfScope prof("ks_moddown_conv", s, 8.0 * n * polys * ((double)alpha + l));
+    InPrimes inp;
+    for (uint32_t k = 0; k <= kRegAlpha; k++) inp.p[k] = (uint8_t)(L + (k < alpha ? k : 0));
     const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
     const dim3 grid((n + kBlock - 1) / kBlock, polys * chunks);
     if (t.max_bits <= 58) {
         switch (alpha) {
 #define NX_A(A)                                                                                                  \
     case A:                                                                                                      \
-        hipLaunchKernelGGL(k_moddown_conv_a<A>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, l, L, \
-                           n);                                                                                   \
+        hipLaunchKernelGGL(k_moddown_conv_a<A>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, inp, \
+                           l, L, n);                                                                             \
         return;
             NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
             NX_A(14) NX_A(15) NX_A(16)
@@ -434,4 +441,29 @@ void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const 
                            count, t.mod, limbs, n);
 }
 
+
+bool ks_moddown_rescale_ok(const DeviceTables &t, uint32_t alpha) { return t.max_bits <= 58 && alpha + 1 <= 16; }
+
+void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                                const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l_out,
+                                uint32_t alpha, uint32_t L, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn, A = alpha + 1;
+    ProfScope prof("ks_moddown_conv", s, 8.0 * n * polys * ((double)A + l_out));
+    InPrimes inp;
+    for (uint32_t k = 0; k <= kRegAlpha; k++) inp.p[k] = (uint8_t)(k == 0 ? l_out : L + (k - 1 < alpha ? k - 1 : 0));
+    const uint32_t chunks = (l_out + kOutChunk - 1) / kOutChunk;
+    const dim3 grid((n + kBlock - 1) / kBlock, polys * chunks);
+    switch (A) {
+#define NX_A(AA)                                                                                                  \
+    case AA:                                                                                                      \
+        hipLaunchKernelGGL(k_moddown_conv_a<AA>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, inp, \
+                           l_out, L, n);                                                                          \
+        return;
+        NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13) NX_A(14)
+        NX_A(15) NX_A(16)
+#undef NX_A
+    default: break;
+    }
+}
 }  // namespace nx

@@ -585,6 +585,93 @@ void op_multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, co
     for (size_t r = 0; r < a.size(); r++) *out[r] = std::move(res[r]);
 }
 
+void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> &a,
+                                          const std::vector<const Ciphertext *> &b, const KSwitchKeys &rk,
+                                          const std::vector<Ciphertext *> &out)
+{
+    // The relinearised product (c0, c1) + ModDown_P(acc) is rescaled by q_(l-1)
+    // in the same rounding: Y = acc + P (c0, c1) over Q_l P is divided by
+    // P' = P q_(l-1) with one exact centred conversion of its limbs
+    // (q_(l-1), p_0..p_(alpha-1)).  out_i = (acc_i - [Y]_(P'),i) P'^-1
+    // + d_i q_(l-1)^-1, one rounding instead of two: each coefficient is within
+    // 1 of op_multiply_relin_rescale_many's and the same integer on every limb.
+    require(!a.empty() && a.size() == b.size() && a.size() == out.size(), "operand counts must match");
+    const Ciphertext &a0 = *a[0];
+    valid(a0);
+    Context &c = *a0.ctx;
+    if (c.alpha == 1 || !ks_moddown_rescale_ok(c.tables, (uint32_t)c.alpha)) {
+        op_multiply_relin_rescale_many(a, b, rk, out);
+        return;
+    }
+    const size_t l = a0.limbs, n = c.n, pw = l * n, alpha = c.alpha, nout = l + alpha;
+    require(rk.ctx == &c, "relin_keys is not valid for encryption parameters");
+    const DevBuf &key = find_key(rk, 0, "relinearization key not present");
+    require(l >= 2, "end of modulus switching chain reached");
+    for (size_t r = 0; r < a.size(); r++) {
+        valid(*a[r]);
+        valid(*b[r]);
+        same_params(*a[r], *b[r]);
+        require(a[r]->ctx == &c && a[r]->limbs == l, "batch operands must share the level");
+        require(a[r]->ntt && b[r]->ntt, "encrypted1 or encrypted2 must be in NTT form");
+        require(a[r]->size == 2 && b[r]->size == 2, "multiply expects two size-2 ciphertexts (relinearize first)");
+        require(scale_within_bounds(c, a[r]->scale * b[r]->scale, l), "scale out of bounds");
+    }
+    uint8_t outp[kMaxPrimes];
+    ks_out_primes(c, l, outp);
+    const size_t A = alpha + 1, lo = l - 1;
+    std::vector<Ciphertext> res(a.size());
+    const size_t chunk = (size_t)kNttSrcMax / 2;
+    for (size_t base = 0; base < a.size(); base += chunk) {
+        const size_t cnt = std::min(chunk, a.size() - base);
+        std::vector<const uint64_t *> pa, pb;
+        for (size_t r = 0; r < cnt; r++) {
+            pa.push_back(a[base + r]->data.get());
+            pb.push_back(b[base + r]->data.get());
+        }
+        DevBuf c01(&c, cnt * 2 * pw), c2(&c, cnt * pw);
+        poly_tensor_many(c.tables, c01.get(), c2.get(), pa.data(), pb.data(), (uint32_t)cnt, (uint32_t)l, c.stream);
+        DevBuf acc(&c, cnt * 2 * nout * n);
+        {
+            DevBuf T = ks_modup(c, c2.get(), cnt, l);
+            ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)cnt, (uint32_t)c.digits(l),
+                             (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
+        }
+        // limb q_(l-1) of Y = acc + P d, then (q_(l-1), p_0..p_(alpha-1)) to coefficients
+        rns_add_limb_scaled(c.tables, acc.get(), nout * n, c01.get(), pw, (uint32_t)lo, c.mdr_pmod[l],
+                            (uint32_t)(2 * cnt), c.stream);
+        {
+            NttBatch nb;
+            nb.data = acc.get() + lo * n;
+            nb.poly_stride = nout * n;
+            nb.polys = (uint32_t)(2 * cnt);
+            nb.limbs = (uint32_t)A;
+            nb.prime_by_poly = 0;
+            for (int i = 0; i < kMaxPrimes; i++)
+                nb.prime_of[i] = (uint8_t)(i == 0 ? lo : c.L + (i - 1 < (int)alpha ? i - 1 : 0));
+            ntt_inverse(c.tables, nb, c.stream);
+        }
+        DevBuf tmp(&c, cnt * 2 * lo * n);
+        ks_moddown_rescale_convert(c.tables, tmp.get(), acc.get(), nout * n, c.d_mdr_w + l * A,
+                                   c.d_mdr_conv + l * (A + 1) * c.L, (uint32_t)(2 * cnt), (uint32_t)lo,
+                                   (uint32_t)alpha, (uint32_t)c.L, c.stream);
+        ntt_fwd(c, tmp.get(), 2 * cnt, lo, lo * n);
+        std::vector<uint64_t *> o;
+        for (size_t r = 0; r < cnt; r++) {
+            Ciphertext &t = res[base + r];
+            t.ctx = &c;
+            t.size = 2;
+            t.limbs = lo;
+            t.scale = a[base + r]->scale * b[base + r]->scale / (double)c.moduli[l - 1];
+            t.ntt = true;
+            t.data.reset(&c, 2 * lo * n);
+            o.push_back(t.data.get());
+        }
+        rns_sub_mul2_ptrs(c.tables, o.data(), acc.get(), nout * n, c01.get(), pw, tmp.get(), c.d_mdr_inv + l * c.L,
+                          c.rescale_inv(l), (uint32_t)cnt, (uint32_t)lo, c.stream);
+    }
+    for (size_t r = 0; r < a.size(); r++) *out[r] = std::move(res[r]);
+}
+
 static void drop_to(const Ciphertext &a, size_t limbs, Ciphertext &dst)
 {
     Context &c = *a.ctx;

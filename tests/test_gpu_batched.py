@@ -1,7 +1,9 @@
 """Batched rescale and multiply -> relinearize -> rescale on the GPU: every
 result equals the per-ciphertext operations word for word (one-prime and
 hybrid key switching, batches crossing the internal chunk size, size-3
-ciphertexts), and invalid batches are rejected on the host."""
+ciphertexts); the fused form (relinearisation and rescale in one rounding)
+is within one of it on every coefficient; invalid batches are rejected on
+the host."""
 import sys
 
 import numpy as np
@@ -49,6 +51,54 @@ def test_multiply_relin_rescale_many_matches(env, count):
         assert got[i].info()["scale"] == want.info()["scale"]
 
 
+def _centred_diff(f, u, q):
+    """(f - u) mod q as -1, 0, 1 (2 marks any other difference)"""
+    d = np.where(f >= u, f - u, f + np.uint64(q) - u)
+    out = np.full(d.shape, 2, dtype=np.int8)
+    out[d == 0] = 0
+    out[d == 1] = 1
+    out[d == np.uint64(q - 1)] = -1
+    return out
+
+
+@pytest.mark.parametrize("count", [1, 17])
+def test_fused_relin_rescale_within_one(env, count):
+    """relinearisation and rescale in one rounding: every coefficient of the
+    fused result is the unfused one plus d, d in {-1, 0, 1} and the same on
+    every limb.  Rounding X / P and then by q differs from rounding X / (P q)
+    only where X / (P q) lies within 1 / (2 q) of a half-integer, so d != 0 on
+    at most a handful of coefficients (a one-prime key switch has nothing to
+    fuse: word for word)"""
+    ctx, sk, rk = env
+    _note("fused mul-relin-rescale logn=%d count=%d" % (ctx.log_n, count))
+    a = _cts(ctx, sk, count, 210)
+    b = _cts(ctx, sk, count, 250)
+    b[0] = a[0]
+    fused = nx.multiply_relin_rescale_many(a, b, rk, fused=True)
+    plain = nx.multiply_relin_rescale_many(a, b, rk)
+    changed = 0
+    for i in range(count):
+        assert fused[i].info()["scale"] == plain[i].info()["scale"]
+        assert fused[i].chain_index == plain[i].chain_index
+        if ctx.special_primes == 1:
+            assert np.array_equal(fused[i].to_array(), plain[i].to_array())
+            continue
+        f = nx.transform_from_ntt(fused[i]).to_array()
+        u = nx.transform_from_ntt(plain[i]).to_array()
+        d0 = None
+        for j in range(f.shape[1]):
+            d = _centred_diff(f[:, j], u[:, j], ctx.moduli[j])
+            assert not np.any(d == 2), (i, j)
+            if d0 is None:
+                d0 = d
+            assert np.array_equal(d, d0), (i, j)
+        changed += int(np.count_nonzero(d0))
+        x = nx.decode(ctx, nx.decrypt(ctx, sk, fused[i]))
+        y = nx.decode(ctx, nx.decrypt(ctx, sk, plain[i]))
+        assert np.max(np.abs(x - y)) < 1e-4
+    assert changed <= 8
+
+
 @pytest.mark.parametrize("count", [1, 20])
 def test_rescale_many_matches(env, count):
     ctx, sk, rk = env
@@ -69,3 +119,5 @@ def test_batched_rejects_mixed_levels(env):
         nx.rescale_to_next_many([a[0], low])
     with pytest.raises(nx.NexusError):
         nx.multiply_relin_rescale_many([a[0], low], [a[0], low], rk)
+    with pytest.raises(nx.NexusError):
+        nx.multiply_relin_rescale_many([a[0], low], [a[0], low], rk, fused=True)

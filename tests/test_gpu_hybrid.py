@@ -135,3 +135,15 @@ def test_hybrid_bootstrap():
     print("hybrid bootstrap: max %.3g mean %.3g" % (err.max(), err.mean()))
     assert err.max() < 1e-3 and err.mean() < 1e-4
     assert out.chain_index == 1 + 14
+    # lockstep EvalMod of two inputs: products relinearised and rescaled in one
+    # rounding, so the outputs match the single bootstrap to rounding noise
+    _note("hybrid bootstrap_many x2")
+    w = np.random.default_rng(10).uniform(-1, 1, n)
+    ct2 = nx.mod_switch_to(nx.encrypt(ctx, sk, nx.encode(ctx, np.tile(w, (1 << (logN - 1)) // n), 2.0 ** 46),
+                                      seed=6), ctx.L)
+    many = bt.bootstrap_many([ct, ct2])
+    got = [nx.decode(ctx, nx.decrypt(ctx, sk, m))[:n] for m in many]
+    assert np.max(np.abs(got[0] - after)) < 1e-5
+    err2 = np.abs(got[1] - w)
+    assert err2.max() < 1e-3 and err2.mean() < 1e-4
+    assert all(m.chain_index == out.chain_index and m.scale == out.scale for m in many)
