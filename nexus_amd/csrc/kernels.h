@@ -46,6 +46,9 @@ struct NttBatch {
     // or one source per poly (host array of polys <= kNttSrcMax device pointers;
     // limb i of poly p at src_list[p] + i * n)
     const uint64_t *const *src_list = nullptr;
+    // with prime_by_poly and poly_period: limb j of poly p is left as it is
+    // when I = p % poly_period < skip_l and I / skip_alpha == j (0: none)
+    uint32_t skip_l = 0, skip_alpha = 0;
 };
 constexpr int kNttSrcMax = 32;
 
@@ -168,6 +171,12 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
 // z[polys][l_out][n], with the per-level tables Context::d_mdr_*.  Needs
 // ks_moddown_rescale_ok (every prime below 2^58, alpha + 1 <= 16).
 bool ks_moddown_rescale_ok(const DeviceTables &t, uint32_t alpha);
+// T[b][I][I / alpha] = target_b[I] (NTT form) for I < l: the limbs of ModUp
+// digit j on its own primes, which the conversion reproduces exactly.
+// targets [B][l][n], or target_list (B <= kNttSrcMax pointers) when non-null.
+void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targets,
+                       const uint64_t *const *target_list, uint32_t B, uint32_t l, uint32_t alpha, uint32_t digits,
+                       uint32_t nout, hipStream_t s);
 void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
                                 const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l_out,
                                 uint32_t alpha, uint32_t L, hipStream_t s);

@@ -162,6 +162,7 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
     uint64_t *out = T + ((uint64_t)b * nout * digits + j) * n + c;
     const uint32_t I_end = min(nout, (chunk + 1) * kConvChunk);
     for (uint32_t I = chunk * kConvChunk; I < I_end; I++) {
+        if (I - first < s) continue;  // the digit's own primes: copied in NTT form
         const uint32_t kp = outp.p[I];
         const uint64_t qk = mod[kp].q;
         if (digit_fp && qk < kFpPrimeBound) {
@@ -441,6 +442,34 @@ void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const 
                            count, t.mod, limbs, n);
 }
 
+
+struct TargetPtrs {
+    const uint64_t *p[kNttSrcMax];
+};
+
+__global__ void k_modup_copy_own(uint64_t *T, const uint64_t *targets, TargetPtrs list, uint32_t l, uint32_t alpha,
+                                 uint32_t digits, uint32_t nout, uint32_t n)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t b = blockIdx.y / l, I = blockIdx.y - b * l;
+    const uint64_t *src = (targets ? targets + (uint64_t)b * l * n : list.p[b]) + (uint64_t)I * n;
+    T[(((uint64_t)b * nout + I) * digits + I / alpha) * n + c] = src[c];
+}
+
+void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targets,
+                       const uint64_t *const *target_list, uint32_t B, uint32_t l, uint32_t alpha, uint32_t digits,
+                       uint32_t nout, hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    TargetPtrs list;
+    for (uint32_t i = 0; i < (uint32_t)kNttSrcMax; i++)
+        list.p[i] = (!targets && i < B) ? target_list[i] : nullptr;
+    if (!targets && B > (uint32_t)kNttSrcMax) abort();
+    ProfScope prof("modup_copy", s, 16.0 * n * B * l);
+    hipLaunchKernelGGL(k_modup_copy_own, dim3((n + kBlock - 1) / kBlock, B * l), dim3(kBlock), 0, s, T, targets, list,
+                       l, alpha, digits, nout, n);
+}
 
 bool ks_moddown_rescale_ok(const DeviceTables &t, uint32_t alpha) { return t.max_bits <= 58 && alpha + 1 <= 16; }
 

@@ -56,6 +56,7 @@ struct NttKArgs {
     const ulonglong2 *ninv;
     const double *twd;    // the same twiddles as doubles
     const double *ninvd;  // n^-1 mod q as doubles
+    uint32_t skip_l, skip_alpha, skipped;  // NttBatch::skip_*; skipped: limbs left out
     uint8_t prime_of[kMaxPrimes];
 };
 
@@ -395,6 +396,10 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     __shared__ uint64_t lds[G::TILE];
     const uint32_t bidx = blockIdx.y;
     const uint32_t poly = bidx / a.limbs, limb = bidx - poly * a.limbs;
+    if (a.skip_alpha) {  // uniform per workgroup, before any barrier
+        const uint32_t I = poly % a.poly_period;
+        if (I < a.skip_l && I / a.skip_alpha == limb) return;
+    }
     const uint32_t prime =
         a.prime_by_poly ? a.prime_of[a.poly_period ? poly % a.poly_period : poly] : a.prime_of[limb];
     const uint32_t n = 1u << a.logn;
@@ -508,7 +513,7 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     // twiddle entries: {w, w'} (16 B) on the 64-bit path, w (8 B) in double
     // precision
     const double tw = ROLE == kWhole ? (double)n : (ROLE == kCol ? (double)G::S : (double)(n - n / G::S));
-    ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)batch * (16.0 * n + a.tw_bytes * tw));
+    ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)(batch - a.skipped) * (16.0 * n + a.tw_bytes * tw));
     hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
 }
 
@@ -585,6 +590,13 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
         if (b.polys > (uint32_t)kNttSrcMax) abort();
         a.nsrcs = b.polys;
         for (uint32_t i = 0; i < b.polys; i++) a.srcs[i] = b.src_list[i];
+    }
+    a.skip_l = a.skip_alpha = a.skipped = 0;
+    if (b.skip_alpha && b.prime_by_poly && b.poly_period) {
+        a.skip_l = b.skip_l;
+        a.skip_alpha = b.skip_alpha;
+        for (uint32_t I = 0; I < b.skip_l && I < b.poly_period; I++)
+            if (I / b.skip_alpha < b.limbs) a.skipped += b.polys / b.poly_period;
     }
     a.limbs = b.limbs;
     a.logn = (uint32_t)t.logn;

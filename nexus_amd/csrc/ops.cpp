@@ -248,6 +248,13 @@ static DevBuf ks_modup_impl(Context &c, const uint64_t *targets, const uint64_t 
         ks_modup_convert(c.tables, T.get(), coef.get(), c.d_modup_w, c.d_modup_conv, (uint32_t)B, (uint32_t)l,
                          (uint32_t)c.alpha, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
     NttBatch b;
+    if (c.alpha > 1) {
+        // digit j on its own primes is its input: copied in NTT form, not transformed
+        ks_modup_copy_own(c.tables, T.get(), targets, target_list, (uint32_t)B, (uint32_t)l, (uint32_t)c.alpha,
+                          (uint32_t)digits, (uint32_t)nout, c.stream);
+        b.skip_l = (uint32_t)l;
+        b.skip_alpha = (uint32_t)c.alpha;
+    }
     b.data = T.get();
     b.poly_stride = digits * n;
     b.polys = (uint32_t)(B * nout);
