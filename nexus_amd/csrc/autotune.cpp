@@ -73,7 +73,9 @@ std::string autotune(Context &c, int reps)
         return (double)ms / reps;
     };
     if (c.logn >= 14) {
-        const uint32_t max_batch = 128;
+        // up to 512 limbs (256 MB at n = 2^16): the largest bucket is timed on a
+        // working set beyond the last-level cache, like the batched ModUp NTTs
+        const uint32_t max_batch = 512;
         // limbs per poly of a tuning batch: a power of two <= min(primes, 16)
         uint32_t lp = 1;
         while (lp * 2 <= std::min<size_t>(P, 16)) lp *= 2;

@@ -63,9 +63,10 @@ struct NttKArgs {
 constexpr int kThreads = 256;
 constexpr int kTuneLogNMin = 14, kTuneLogN = 4, kTuneBuckets = 9;
 
-// CW: sub-transforms per workgroup -- columns of the column pass (16 for large
-// batches, 8 for small ones: more workgroups when there are few limbs), and an
-// upper bound on the rows of the row pass.
+// CW: sub-transforms per workgroup -- columns of the column pass (8 by
+// default: a 16 KB tile keeps twice the workgroups of a 16-wide one resident,
+// and once a batch outgrows the last-level cache the strided column reads of
+// 8-wide tiles stream faster), and an upper bound on the rows of the row pass.
 template <int LOGS, int ROLE, int CW>
 struct Geometry {
     static constexpr int S = 1 << LOGS;
@@ -390,7 +391,7 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
 }
 
 template <int LOGS, int ROLE, bool FWD, int CW>
-__global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
+__device__ __forceinline__ void ntt_pass(const NttKArgs &a)
 {
     using G = Geometry<LOGS, ROLE, CW>;
     __shared__ uint64_t lds[G::TILE];
@@ -500,6 +501,20 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     }
 }
 
+template <int LOGS, int ROLE, bool FWD, int CW>
+__global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
+{
+    ntt_pass<LOGS, ROLE, FWD, CW>(a);
+}
+
+// The forward column pass with 8-wide tiles held to 64 VGPRs: 8 waves per SIMD
+// instead of 7 (the other passes reach 8 as they are, or would spill).
+template <int LOGS, int ROLE, bool FWD, int CW>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_ntt_w8(NttKArgs a)
+{
+    ntt_pass<LOGS, ROLE, FWD, CW>(a);
+}
+
 template <int LOGS, int ROLE, bool FWD, int CW = 16>
 void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
 {
@@ -514,7 +529,10 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     // precision
     const double tw = ROLE == kWhole ? (double)n : (ROLE == kCol ? (double)G::S : (double)(n - n / G::S));
     ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)(batch - a.skipped) * (16.0 * n + a.tw_bytes * tw));
-    hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
+    if constexpr (ROLE == kCol && FWD && CW == 8)
+        hipLaunchKernelGGL((k_ntt_w8<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
 }
 
 template <bool FWD>
@@ -531,8 +549,7 @@ void launch_whole(const NttKArgs &a, uint32_t batch, hipStream_t s)
 }
 
 // Tile widths chosen by the autotuner: [log n - 14][pass][direction][batch bucket],
-// 0 = the default rule (column tiles of 8 below 128 limbs, 16 otherwise; row
-// tiles of 8).  Written by ntt_set_tile, read at every launch.
+// 0 = the default (tiles of 8).  Written by ntt_set_tile, read at every launch.
 std::atomic<int> g_tile[kTuneLogN][2][2][kTuneBuckets];
 
 int tune_bucket(uint32_t batch)
@@ -548,7 +565,7 @@ void launch_sized(const NttKArgs &a, uint32_t batch, hipStream_t s)
     int cw = 0;
     const int li = (int)a.logn - kTuneLogNMin;
     if (li >= 0 && li < kTuneLogN) cw = g_tile[li][ROLE == kCol ? 0 : 1][FWD ? 1 : 0][tune_bucket(batch)].load();
-    if (!cw) cw = ROLE == kCol ? (batch < 128 ? 8 : 16) : 8;
+    if (!cw) cw = 8;
     if (cw == 4)
         launch_one<LOGS, ROLE, FWD, 4>(a, batch, s);
     else if (cw == 8)

@@ -53,6 +53,31 @@ def test_hybrid_relin_bit_exact(logn, L, alpha, chain_index):
     assert np.array_equal(got[1], exp1)
 
 
+@pytest.mark.parametrize("chain,alpha,chain_index", [([46] * 6 + [58] * 2, 2, 1), ([51] * 7 + [58] * 3, 3, 2),
+                                                     ([58] + [46] * 5 + [58] * 2, 2, 1)],
+                         ids=["46-58", "51-58", "mixed"])
+def test_hybrid_relin_bit_exact_wide(chain, alpha, chain_index):
+    """58-bit special primes (ModDown inputs beyond 2^51: split double-precision
+    products) and a 58-bit data prime (the 64-bit integer path for that output)"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    logn, L = 8, len(chain) - alpha
+    _note("hybrid relin wide chain=%s alpha=%d chain_index=%d" % (chain, alpha, chain_index))
+    ctx = nx.Context(logn, chain, special_primes=alpha)
+    sk = nx.SecretKey(ctx, seed=15)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=19)
+    key = rk.key_array(0)
+    rng = np.random.default_rng(L + alpha + chain_index)
+    limbs = ctx.limbs_of(chain_index)
+    data = _rand_ct(rng, ctx, 3, limbs)
+    ct = nx.Ciphertext.from_array(ctx, data, chain_index, 2.0 ** 40)
+    got = nx.relinearize(ct, rk).to_array()
+    ks0, ks1 = hybrid.switch_key_hybrid(data[2], key, ctx.moduli, L, alpha, logn)
+    mods = np.array(ctx.moduli[:limbs], dtype=object)[:, None]
+    assert np.array_equal(got[0], ((data[0].astype(object) + ks0.astype(object)) % mods).astype(np.uint64))
+    assert np.array_equal(got[1], ((data[1].astype(object) + ks1.astype(object)) % mods).astype(np.uint64))
+
+
 def test_hybrid_galois_bit_exact():
     if nx.device_count() < 1:
         pytest.skip("no GPU visible")
