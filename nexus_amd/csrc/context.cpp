@@ -496,6 +496,13 @@ void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly
     ntt_forward(c.tables, make_batch(data, polys, limbs, poly_stride), c.stream);
 }
 
+void ntt_fwd_epi(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride, const NttEpilogue &epi)
+{
+    NttBatch b = make_batch(data, polys, limbs, poly_stride);
+    b.epi = epi;
+    ntt_forward(c.tables, b, c.stream);
+}
+
 void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride)
 {
     ntt_inverse(c.tables, make_batch(data, polys, limbs, poly_stride), c.stream);

@@ -235,6 +235,8 @@ private:
 // ---------------------------------------------------------------- operations
 // NTT of a [polys][limbs][n] buffer whose limb i uses prime i.
 void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
+// forward NTT whose last pass stores through epi instead of into data
+void ntt_fwd_epi(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride, const NttEpilogue &epi);
 void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
 // out of place: src (poly stride src_stride) -> dst (poly stride dst_stride)
 void ntt_fwd_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, size_t limbs, size_t dst_stride,
@@ -298,7 +300,7 @@ DevBuf ks_modup_list(Context &c, const uint64_t *const *targets, size_t B, size_
 // ModDown up to the final combination: INTT of the special limbs of acc
 // [B][2][l+alpha][n], conversion, NTT; returns tmp [B][2][l][n] for
 // out = (acc - tmp) P^-1.
-DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l);
+DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool transform = true);
 // base (optional, [B][2][l][n]): the accumulated terms are read from base instead of cts.
 void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1,
                 const uint64_t *base = nullptr);

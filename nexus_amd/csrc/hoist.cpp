@@ -64,9 +64,10 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
             ks_inner_product_rot(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
                                  (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
                                  c.stream);
-            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l);
-            // each result written in place: c0 = sigma(c0) + ks0, c1 = ks1
-            std::vector<uint64_t *> outs(cnt);
+            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false);
+            // each result written in place by the transform's last pass:
+            // c0 = sigma(c0) + ks0, c1 = ks1
+            NttEpilogue e;
             for (size_t r = 0; r < cnt; r++) {
                 Ciphertext &o = out[p.index[base + r]];
                 o.ctx = &c;
@@ -75,10 +76,14 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
                 o.scale = a.scale;
                 o.ntt = true;
                 o.data.reset(&c, 2 * pw);
-                outs[r] = o.data.get();
+                e.outs[r] = o.data.get();
+                e.perm[r] = p.perms[base + r];
             }
-            sub_mul_rot(c.tables, outs.data(), a.data.get(), p.perms.data() + base, acc.get(), nout * n, tmp.get(),
-                        c.d_p_inv, (uint32_t)cnt, (uint32_t)l, c.stream);
+            e.x = acc.get();
+            e.x_stride = nout * n;
+            e.inv = c.d_p_inv;
+            e.a0 = a.data.get();
+            ntt_fwd_epi(c, tmp.get(), 2 * cnt, l, l * n, e);
         }
     }
     dst = std::move(out);

@@ -33,6 +33,35 @@ struct DeviceTables {
 // that each carry the same per-poly prime pattern).
 // With src set, the transform reads its input from src (limb i of poly p at
 // src + p * src_poly_stride + i * n) and writes data: out of place, src unchanged.
+// Optional last step of a forward NTT (NttBatch::epi): instead of the
+// transform t of poly p, limb i (prime i), the last pass stores
+//   v = (x[p][i] - t) inv[i]
+// (strides per poly, limb i at i n) in one of two forms:
+// * out non-null: out[p][i] = v (+ base[p][i], or + out[p][i] without base,
+//   where accumulate asks: 0 = never, 1 = every poly, 2 = even polys);
+// * outs[0] non-null: poly p = 2 r + k of a batch of l limbs goes to
+//   outs[r] + (k l + i) n, plus a0[i n + perm[r][j]] at index j when k = 0
+//   and a0 is set, plus d[p][i] inv2[i] when d is set.
+// The batch's data buffer keeps only the first pass's intermediate.  These are
+// the ModDown tails (and rns_sub_mul) folded into the transform.
+constexpr int kEpiMax = 32;
+struct NttEpilogue {
+    uint64_t *out = nullptr;
+    uint64_t out_stride = 0;
+    const uint64_t *x = nullptr;
+    uint64_t x_stride = 0;
+    const ulonglong2 *inv = nullptr;
+    int accumulate = 0;
+    const uint64_t *base = nullptr;
+    uint64_t base_stride = 0;
+    uint64_t *outs[kEpiMax] = {};
+    const uint32_t *perm[kEpiMax] = {};
+    const uint64_t *a0 = nullptr;
+    const uint64_t *d = nullptr;
+    uint64_t d_stride = 0;
+    const ulonglong2 *inv2 = nullptr;
+};
+
 struct NttBatch {
     uint64_t *data;
     uint64_t poly_stride;
@@ -49,6 +78,7 @@ struct NttBatch {
     // with prime_by_poly and poly_period: limb j of poly p is left as it is
     // when I = p % poly_period < skip_l and I / skip_alpha == j (0: none)
     uint32_t skip_l = 0, skip_alpha = 0;
+    NttEpilogue epi;  // forward transforms only
 };
 constexpr int kNttSrcMax = 32;
 
@@ -99,12 +129,6 @@ void rns_sub_mul_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_
 // acc[p][limb][c] += d[p][limb][c] * w (Shoup constant of prime `limb`) for polys p
 void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stride, const uint64_t *d,
                          uint64_t d_stride, uint32_t limb, ulonglong2 w, uint32_t polys, hipStream_t s);
-// out_r[k][i] = (x[2r+k][i] - tmp[2r+k][i]) inv1[i] + d[2r+k][i] inv2[i] for i < l_out: the
-// final step of a ModDown by P q_(l-1) (x = the key-switch accumulator, d = the
-// data part (c0, c1) of the product)
-void rns_sub_mul2_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *x, uint64_t x_stride,
-                       const uint64_t *d, uint64_t d_stride, const uint64_t *tmp, const ulonglong2 *inv1,
-                       const ulonglong2 *inv2, uint32_t count, uint32_t l_out, hipStream_t s);
 // out[p][i] = a[p][i] * w[i] with per-limb Shoup constants {w, w'} (host array, passed by value)
 void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,
                      uint32_t polys, uint32_t limbs, hipStream_t s);
@@ -240,16 +264,6 @@ struct RotBaseArgs {
 };
 void rot_base(const DeviceTables &t, uint64_t *out, const uint64_t *const *in, const uint32_t *const *perms,
               uint32_t count, uint32_t l, hipStream_t s);
-// ModDown tail of `count` hoisted rotations into separate results:
-// outs[r][0] = sigma_r(a0) + (x[r][0] - tmp[r][0]) inv, outs[r][1] = (x[r][1] -
-// tmp[r][1]) inv; x poly stride x_stride, tmp [count][2][l][n].
-struct RotOutArgs {
-    uint64_t *out[kMaxHoist];
-    const uint32_t *perm[kMaxHoist];
-};
-void sub_mul_rot(const DeviceTables &t, uint64_t *const *outs, const uint64_t *a0, const uint32_t *const *perms,
-                 const uint64_t *x, uint64_t x_stride, const uint64_t *tmp, const ulonglong2 *inv, uint32_t count,
-                 uint32_t l, hipStream_t s);
 
 // ---------------------------------------------------------------- keys
 // Deterministic counter-based sampling (not SEAL's PRNG).

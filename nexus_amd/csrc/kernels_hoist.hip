@@ -124,25 +124,6 @@ __global__ void k_rot_base(uint64_t *out, RotBaseArgs args, uint32_t count, cons
     out[(uint64_t)row * n + c] = sum;
 }
 
-// ModDown tail of hoisted rotations written into each result: for rotation r,
-// out_r[0] = sigma_r(a0) + (x_r[0] - tmp_r[0]) P^-1 and out_r[1] = (x_r[1] -
-// tmp_r[1]) P^-1, with x = acc[r][k][limb] (stride x_stride per poly).
-__global__ void k_sub_mul_rot(RotOutArgs args, const uint64_t *a0, const uint64_t *x, uint64_t x_stride,
-                              const uint64_t *tmp, const ulonglong2 *inv, const ModConst *mod, uint32_t l, uint32_t n)
-{
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t row = blockIdx.y;  // (r * 2 + k) * l + limb
-    const uint32_t poly = row / l, limb = row - poly * l;
-    const uint32_t r = poly >> 1, k = poly & 1;
-    const uint64_t q = mod[limb].q;
-    const ulonglong2 w = inv[limb];
-    const uint64_t d = sub_mod(x[poly * x_stride + (uint64_t)limb * n + c], tmp[(uint64_t)row * n + c], q);
-    uint64_t v = shoup(d, w.x, w.y, q);
-    if (k == 0) v = add_mod(v, a0[(uint64_t)limb * n + args.perm[r][c]], q);
-    args.out[r][((uint64_t)k * l + limb) * n + c] = v;
-}
-
 OutPrimes make_out(const uint8_t *p, uint32_t nout)
 {
     OutPrimes o;
@@ -196,21 +177,6 @@ void rot_base(const DeviceTables &t, uint64_t *out, const uint64_t *const *in, c
     ProfScope prof("add", s, 8.0 * n * l * (2.0 + 2.0 * count));
     hipLaunchKernelGGL(k_rot_base, dim3((n + kBlock - 1) / kBlock, 2 * l), dim3(kBlock), 0, s, out, a, count, t.mod,
                        l, n);
-}
-
-void sub_mul_rot(const DeviceTables &t, uint64_t *const *outs, const uint64_t *a0, const uint32_t *const *perms,
-                 const uint64_t *x, uint64_t x_stride, const uint64_t *tmp, const ulonglong2 *inv, uint32_t count,
-                 uint32_t l, hipStream_t s)
-{
-    const uint32_t n = 1u << t.logn;
-    RotOutArgs a;
-    for (uint32_t i = 0; i < kMaxHoist; i++) {
-        a.out[i] = i < count ? outs[i] : nullptr;
-        a.perm[i] = i < count ? perms[i] : nullptr;
-    }
-    ProfScope prof("rns_sub_mul", s, 8.0 * n * l * count * 7.0);
-    hipLaunchKernelGGL(k_sub_mul_rot, dim3((n + kBlock - 1) / kBlock, 2 * count * l), dim3(kBlock), 0, s, a, a0, x,
-                       x_stride, tmp, inv, t.mod, l, n);
 }
 
 }  // namespace nx

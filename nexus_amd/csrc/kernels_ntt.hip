@@ -57,8 +57,35 @@ struct NttKArgs {
     const double *twd;    // the same twiddles as doubles
     const double *ninvd;  // n^-1 mod q as doubles
     uint32_t skip_l, skip_alpha, skipped;  // NttBatch::skip_*; skipped: limbs left out
+    NttEpilogue epi;                        // applied by the forward transform's last pass
     uint8_t prime_of[kMaxPrimes];
 };
+
+__host__ __device__ inline bool epi_on(const NttEpilogue &e) { return e.out != nullptr || e.outs[0] != nullptr; }
+
+// NttEpilogue's store of the transform value t (in [0, q)) of poly p, limb i
+// of a batch of l limbs
+__device__ __forceinline__ void epi_store(const NttEpilogue &e, uint32_t p, uint32_t i, uint64_t gidx, uint64_t t,
+                                          uint64_t q, uint32_t l, uint32_t n)
+{
+    const uint64_t off = (uint64_t)i * n + gidx;
+    const ulonglong2 w = e.inv[i];
+    uint64_t r = shoup(sub_mod(e.x[p * e.x_stride + off], t, q), w.x, w.y, q);
+    if (e.out) {
+        uint64_t *o = e.out + p * e.out_stride + off;
+        if (e.accumulate == 1 || (e.accumulate == 2 && (p & 1) == 0))
+            r = add_mod(e.base ? e.base[p * e.base_stride + off] : *o, r, q);
+        *o = r;
+        return;
+    }
+    const uint32_t ct = p >> 1, k = p & 1;
+    if (e.a0 && k == 0) r = add_mod(r, e.a0[(uint64_t)i * n + e.perm[ct][gidx]], q);
+    if (e.d) {
+        const ulonglong2 w2 = e.inv2[i];
+        r = add_mod(r, shoup(e.d[p * e.d_stride + off], w2.x, w2.y, q), q);
+    }
+    e.outs[ct][((uint64_t)k * l + i) * n + gidx] = r;
+}
 
 constexpr int kThreads = 256;
 constexpr int kTuneLogNMin = 14, kTuneLogN = 4, kTuneBuckets = 9;
@@ -314,9 +341,10 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
 template <int LOGS, int ROLE, bool FWD, int CW>
 __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_words, uint64_t *data,
                                            const uint64_t *src, uint32_t prime, uint32_t n, uint32_t N2, uint32_t N1,
-                                           uint32_t first)
+                                           uint32_t first, uint32_t poly, uint32_t limb)
 {
     using G = Geometry<LOGS, ROLE, CW>;
+    const bool epi = FWD && ROLE != kCol && epi_on(a.epi);
     double *lds = reinterpret_cast<double *>(lds_words);
     const double q = (double)a.mod[prime].q;
     const double qinv = 1.0 / q;
@@ -386,7 +414,10 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
         double v = lds[e];
         if (scale_out) v = reduce_f(mulmod_f(v, ni, q, qinv), q, qinv);
         v = v < 0 ? v + q : v;
-        data[gidx] = (uint64_t)v;
+        if (epi)
+            epi_store(a.epi, poly, limb, gidx, (uint64_t)v, (uint64_t)q, a.limbs, n);
+        else
+            data[gidx] = (uint64_t)v;
     }
 }
 
@@ -418,7 +449,7 @@ __device__ __forceinline__ void ntt_pass(const NttKArgs &a)
     if (ROLE == kCol && (gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const uint32_t first = tile * G::C;  // first column / row of the tile
     if (q < kFpPrimeBound) {
-        ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, src, prime, n, N2, N1, first);
+        ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, src, prime, n, N2, N1, first, poly, limb);
         return;
     }
 
@@ -477,6 +508,7 @@ __device__ __forceinline__ void ntt_pass(const NttKArgs &a)
 
     // ---- store (final reduction on the last pass)
     const bool last_pass = FWD ? (ROLE != kCol) : (ROLE != kRow);
+    const bool epi = FWD && ROLE != kCol && epi_on(a.epi);
     const ulonglong2 ni = a.ninv[prime];
     for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
         uint64_t gidx;
@@ -497,7 +529,10 @@ __device__ __forceinline__ void ntt_pass(const NttKArgs &a)
                 v = shoup(v, ni.x, ni.y, q);
             }
         }
-        data[gidx] = v;
+        if (epi)
+            epi_store(a.epi, poly, limb, gidx, v, q, a.limbs, n);
+        else
+            data[gidx] = v;
     }
 }
 
@@ -528,7 +563,13 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     // twiddle entries: {w, w'} (16 B) on the 64-bit path, w (8 B) in double
     // precision
     const double tw = ROLE == kWhole ? (double)n : (ROLE == kCol ? (double)G::S : (double)(n - n / G::S));
-    ProfScope prof(names[FWD ? 1 : 0][ROLE], s, (double)(batch - a.skipped) * (16.0 * n + a.tw_bytes * tw));
+    // with an epilogue the last pass also reads x (and base or out) per word
+    const bool epi = FWD && ROLE != kCol && epi_on(a.epi);
+    const double epi_reads =
+        a.epi.out ? (a.epi.accumulate ? 1.0 : 0.0) : (a.epi.a0 ? 0.5 : 0.0) + (a.epi.d ? 1.0 : 0.0);
+    const double epi_bytes = epi ? 8.0 * n * (1.0 + epi_reads) : 0.0;
+    ProfScope prof(epi ? "ntt_fwd_sub_mul" : names[FWD ? 1 : 0][ROLE], s,
+                   (double)(batch - a.skipped) * (16.0 * n + a.tw_bytes * tw + epi_bytes));
     if constexpr (ROLE == kCol && FWD && CW == 8)
         hipLaunchKernelGGL((k_ntt_w8<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
     else
@@ -607,6 +648,11 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
         if (b.polys > (uint32_t)kNttSrcMax) abort();
         a.nsrcs = b.polys;
         for (uint32_t i = 0; i < b.polys; i++) a.srcs[i] = b.src_list[i];
+    }
+    a.epi = b.epi;
+    if (!fwd) {
+        a.epi.out = nullptr;
+        a.epi.outs[0] = nullptr;
     }
     a.skip_l = a.skip_alpha = a.skipped = 0;
     if (b.skip_alpha && b.prime_by_poly && b.poly_period) {

@@ -161,22 +161,6 @@ __global__ void k_add_limb_scaled(uint64_t *acc, uint64_t acc_stride, const uint
     *a = add_mod(*a, shoup(d[p * d_stride + (uint64_t)limb * n + c], w.x, w.y, q), q);
 }
 
-__global__ void k_sub_mul2_ptrs(OutPtrs outs, const uint64_t *x, uint64_t x_stride, const uint64_t *d,
-                                uint64_t d_stride, const uint64_t *tmp, const ulonglong2 *inv1,
-                                const ulonglong2 *inv2, const ModConst *mod, uint32_t l_out, uint32_t n)
-{
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t row = blockIdx.y;  // (r * 2 + k) * l_out + i
-    const uint32_t pk = row / l_out, i = row - pk * l_out;
-    const uint32_t r = pk >> 1, k = pk & 1;
-    const uint64_t q = mod[i].q;
-    const ulonglong2 w1 = inv1[i], w2 = inv2[i];
-    const uint64_t dx = sub_mod(x[pk * x_stride + (uint64_t)i * n + c], tmp[(uint64_t)row * n + c], q);
-    const uint64_t v = add_mod(shoup(dx, w1.x, w1.y, q), shoup(d[pk * d_stride + (uint64_t)i * n + c], w2.x, w2.y, q), q);
-    outs.out[r][((uint64_t)k * l_out + i) * n + c] = v;
-}
-
 __global__ void k_mul_scalar(uint64_t *out, const uint64_t *a, LimbShoup scal, const ModConst *mod, uint32_t limbs,
                              uint32_t n)
 {
@@ -455,17 +439,6 @@ void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stri
     ProfScope prof("rns_sub_mul", s, 24.0 * (1u << t.logn) * polys);
     hipLaunchKernelGGL(k_add_limb_scaled, dim3(((1u << t.logn) + kBlock - 1) / kBlock, polys), dim3(kBlock), 0, s,
                        acc, acc_stride, d, d_stride, limb, w, t.mod, 1u << t.logn);
-}
-
-void rns_sub_mul2_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *x, uint64_t x_stride,
-                       const uint64_t *d, uint64_t d_stride, const uint64_t *tmp, const ulonglong2 *inv1,
-                       const ulonglong2 *inv2, uint32_t count, uint32_t l_out, hipStream_t s)
-{
-    OutPtrs o;
-    for (uint32_t i = 0; i < kPtrMax; i++) o.out[i] = i < count ? out[i] : nullptr;
-    ProfScope prof("rns_sub_mul", s, 32.0 * (1u << t.logn) * count * 2 * l_out);
-    hipLaunchKernelGGL(k_sub_mul2_ptrs, grid_for(t.logn, count * 2 * l_out), dim3(kBlock), 0, s, o, x, x_stride, d,
-                       d_stride, tmp, inv1, inv2, t.mod, l_out, 1u << t.logn);
 }
 
 void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,

@@ -291,7 +291,7 @@ void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t
     ks_moddown(c, cts, acc.get(), B, l, zero_c1, base);
 }
 
-DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l)
+DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool transform)
 {
     const size_t n = c.n, L = c.L, alpha = c.alpha;
     const size_t nout = l + alpha;
@@ -317,16 +317,26 @@ DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l)
     else
         ks_moddown_convert(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv, (uint32_t)(2 * B),
                            (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
-    ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
+    if (transform) ntt_fwd(c, tmp.get(), 2 * B, l, l * n);
     return tmp;
 }
 
 void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1, const uint64_t *base)
 {
+    // cts = (acc - NTT(conversion)) P^-1 (+ base or cts): the combination is the
+    // forward transform's last pass
     const size_t n = c.n, nout = l + c.alpha;
-    DevBuf tmp = ks_moddown_prepare(c, acc, B, l);
-    rns_sub_mul(c.tables, cts, l * n, acc, nout * n, tmp.get(), c.d_p_inv, (uint32_t)(2 * B), (uint32_t)l,
-                zero_c1 ? 2 : 1, c.stream, base, l * n);
+    DevBuf tmp = ks_moddown_prepare(c, acc, B, l, false);
+    NttEpilogue e;
+    e.out = cts;
+    e.out_stride = l * n;
+    e.x = acc;
+    e.x_stride = nout * n;
+    e.inv = c.d_p_inv;
+    e.accumulate = zero_c1 ? 2 : 1;
+    e.base = base;
+    e.base_stride = l * n;
+    ntt_fwd_epi(c, tmp.get(), 2 * B, l, l * n, e);
 }
 
 static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
@@ -657,12 +667,11 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
                 nb.prime_of[i] = (uint8_t)(i == 0 ? lo : c.L + (i - 1 < (int)alpha ? i - 1 : 0));
             ntt_inverse(c.tables, nb, c.stream);
         }
-        DevBuf tmp(&c, cnt * 2 * lo * n);
+        DevBuf tmp(&c, cnt * 2 * lo * n);  // conversion, then the transform's first pass
         ks_moddown_rescale_convert(c.tables, tmp.get(), acc.get(), nout * n, c.d_mdr_w + l * A,
                                    c.d_mdr_conv + l * (A + 1) * c.L, (uint32_t)(2 * cnt), (uint32_t)lo,
                                    (uint32_t)alpha, (uint32_t)c.L, c.stream);
-        ntt_fwd(c, tmp.get(), 2 * cnt, lo, lo * n);
-        std::vector<uint64_t *> o;
+        NttEpilogue e;  // out_i = (acc_i - NTT(tmp)_i) P'^-1 + d_i q_(l-1)^-1
         for (size_t r = 0; r < cnt; r++) {
             Ciphertext &t = res[base + r];
             t.ctx = &c;
@@ -671,10 +680,15 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
             t.scale = a[base + r]->scale * b[base + r]->scale / (double)c.moduli[l - 1];
             t.ntt = true;
             t.data.reset(&c, 2 * lo * n);
-            o.push_back(t.data.get());
+            e.outs[r] = t.data.get();
         }
-        rns_sub_mul2_ptrs(c.tables, o.data(), acc.get(), nout * n, c01.get(), pw, tmp.get(), c.d_mdr_inv + l * c.L,
-                          c.rescale_inv(l), (uint32_t)cnt, (uint32_t)lo, c.stream);
+        e.x = acc.get();
+        e.x_stride = nout * n;
+        e.inv = c.d_mdr_inv + l * c.L;
+        e.d = c01.get();
+        e.d_stride = pw;
+        e.inv2 = c.rescale_inv(l);
+        ntt_fwd_epi(c, tmp.get(), 2 * cnt, lo, lo * n, e);
     }
     for (size_t r = 0; r < a.size(); r++) *out[r] = std::move(res[r]);
 }
