@@ -39,11 +39,12 @@ struct DeviceTables {
 // (strides per poly, limb i at i n) in one of two forms:
 // * out non-null: out[p][i] = v (+ base[p][i], or + out[p][i] without base,
 //   where accumulate asks: 0 = never, 1 = every poly, 2 = even polys);
-// * outs[0] non-null: poly p = 2 r + k of a batch of l limbs goes to
-//   outs[r] + (k l + i) n, plus a0[i n + perm[r][j]] at index j when k = 0
-//   and a0 is set, plus d[p][i] inv2[i] when d is set.
+// * outs[0] non-null: poly p = per r + k (per = polys_per, 2 by default) of a
+//   batch of l limbs goes to outs[r] + (k l + i) n, plus a0[i n + perm[r][j]]
+//   at index j when k = 0 and a0 is set, plus d[p][i] inv2[i] when d is set;
+//   with xs[0] set, x[p][i] is read at xs[r] + (k x_limbs + i) n instead.
 // The batch's data buffer keeps only the first pass's intermediate.  These are
-// the ModDown tails (and rns_sub_mul) folded into the transform.
+// the rescale and ModDown tails folded into the transform.
 constexpr int kEpiMax = 32;
 struct NttEpilogue {
     uint64_t *out = nullptr;
@@ -60,6 +61,9 @@ struct NttEpilogue {
     const uint64_t *d = nullptr;
     uint64_t d_stride = 0;
     const ulonglong2 *inv2 = nullptr;
+    const uint64_t *xs[kEpiMax] = {};
+    uint32_t x_limbs = 0;
+    uint32_t polys_per = 2;
 };
 
 struct NttBatch {
@@ -115,17 +119,9 @@ struct PairPtrs {
     const uint64_t *x[kPtrMax];
     const uint64_t *y[kPtrMax];
 };
-struct OutPtrs {
-    uint64_t *out[kPtrMax];
-};
 // c01[r][2][l][n] = (a_r0 b_r0, a_r0 b_r1 + a_r1 b_r0), c2[r][l][n] = a_r1 b_r1
 void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const uint64_t *const *a,
                       const uint64_t *const *b, uint32_t count, uint32_t limbs, hipStream_t s);
-// out_r[p][i] = (x_r[p][i] - tmp[r polys + p][i]) inv[i] for i < l_out (x_r has
-// x_limbs limbs per poly): the rescale / ModDown tail into separate results
-void rns_sub_mul_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *const *x, const uint64_t *tmp,
-                      const ulonglong2 *inv, uint32_t count, uint32_t polys, uint32_t l_out, uint32_t x_limbs,
-                      hipStream_t s);
 // acc[p][limb][c] += d[p][limb][c] * w (Shoup constant of prime `limb`) for polys p
 void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stride, const uint64_t *d,
                          uint64_t d_stride, uint32_t limb, ulonglong2 w, uint32_t polys, hipStream_t s);
@@ -146,18 +142,10 @@ void poly_add_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, co
 // Rescale / ModDown tail, SEAL-exact rounding.
 //  last: coefficient-domain limb (already INTT'd) of prime `last_prime`, one per poly
 //  out[p][i] = (x[p][i] - NTT_i(centered(last[p]) mod q_i)) * inv[i]   for i < limbs
-// Step 1 writes the centered residues mod q_i into tmp[p][i]; the caller NTTs tmp;
-// step 2 combines.  `acc_add` (may be null) receives out instead: acc[p][i] += result.
+// This writes the centred residues mod q_i into tmp[p][i]; the caller's forward
+// NTT of tmp combines them with x through an NttEpilogue.
 void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last, uint64_t last_stride,
                      uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s);
-// accumulate: 0 = write, 1 = add into out, 2 = add into even polys / write odd polys
-// (a batch of key-switched ciphertexts whose c1 was cleared by the automorphism).
-// With base set, the added term is read from base (same layout as out, poly
-// stride base_stride) instead of out.
-void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
-                 const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
-                 hipStream_t s, const uint64_t *base = nullptr, uint64_t base_stride = 0);
-
 // Key-switch ModUp over a batch of B targets: for ciphertext b, every output
 // prime I in out_primes (count = nout) and digit j < l,
 // T[b][I][j] = coef[b][j] mod q_I  (coef = INTT of the target, [B][l][n]).

@@ -70,7 +70,9 @@ __device__ __forceinline__ void epi_store(const NttEpilogue &e, uint32_t p, uint
 {
     const uint64_t off = (uint64_t)i * n + gidx;
     const ulonglong2 w = e.inv[i];
-    uint64_t r = shoup(sub_mod(e.x[p * e.x_stride + off], t, q), w.x, w.y, q);
+    const uint32_t ct = p / e.polys_per, k = p - ct * e.polys_per;
+    const uint64_t xv = e.xs[0] ? e.xs[ct][((uint64_t)k * e.x_limbs + i) * n + gidx] : e.x[p * e.x_stride + off];
+    uint64_t r = shoup(sub_mod(xv, t, q), w.x, w.y, q);
     if (e.out) {
         uint64_t *o = e.out + p * e.out_stride + off;
         if (e.accumulate == 1 || (e.accumulate == 2 && (p & 1) == 0))
@@ -78,7 +80,6 @@ __device__ __forceinline__ void epi_store(const NttEpilogue &e, uint32_t p, uint
         *o = r;
         return;
     }
-    const uint32_t ct = p >> 1, k = p & 1;
     if (e.a0 && k == 0) r = add_mod(r, e.a0[(uint64_t)i * n + e.perm[ct][gidx]], q);
     if (e.d) {
         const ulonglong2 w2 = e.inv2[i];

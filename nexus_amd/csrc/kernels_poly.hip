@@ -135,21 +135,6 @@ __global__ void k_tensor_many(uint64_t *c01, uint64_t *c2, PairPtrs p, const Mod
 // out_r[p][limb] = (x_r[p][limb] - tmp[r * polys + p][limb]) inv[limb] for the
 // first l_out limbs; x_r has x_limbs limbs per poly (pointer lists p.y = x, p.x
 // unused, outs = out pointers)
-__global__ void k_sub_mul_ptrs(OutPtrs outs, PairPtrs xs, const uint64_t *tmp, const ulonglong2 *inv,
-                               const ModConst *mod, uint32_t polys, uint32_t l_out, uint32_t x_limbs, uint32_t n)
-{
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t row = blockIdx.y;  // (r * polys + p) * l_out + limb
-    const uint32_t rp = row / l_out, limb = row - rp * l_out;
-    const uint32_t r = rp / polys, p = rp - r * polys;
-    const uint64_t q = mod[limb].q;
-    const ulonglong2 w = inv[limb];
-    const uint64_t xv = xs.y[r][((uint64_t)p * x_limbs + limb) * n + c];
-    const uint64_t d = sub_mod(xv, tmp[(uint64_t)row * n + c], q);
-    outs.out[r][((uint64_t)p * l_out + limb) * n + c] = shoup(d, w.x, w.y, q);
-}
-
 __global__ void k_add_limb_scaled(uint64_t *acc, uint64_t acc_stride, const uint64_t *d, uint64_t d_stride,
                                   uint32_t limb, ulonglong2 w, const ModConst *mod, uint32_t n)
 {
@@ -213,25 +198,6 @@ __global__ void k_center_last(uint64_t *tmp, const uint64_t *last, uint64_t last
     const ModConst mi = mod[limb];
     const uint64_t r = sub_mod(barrett64(v, mi), barrett64(half, mi), mi.q);
     tmp[(uint64_t)row * n + c] = r;
-}
-
-__global__ void k_sub_mul(uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
-                          const uint64_t *tmp, const ulonglong2 *inv, const ModConst *mod, uint32_t limbs, uint32_t n,
-                          int accumulate, const uint64_t *base, uint64_t base_stride)
-{
-    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
-    if (c >= n) return;
-    const uint32_t row = blockIdx.y;
-    const uint32_t poly = row / limbs, limb = row - poly * limbs;
-    const uint64_t q = mod[limb].q;
-    const ulonglong2 w = inv[limb];
-    const uint64_t xv = x[poly * x_stride + (uint64_t)limb * n + c];
-    const uint64_t d = sub_mod(xv, tmp[(uint64_t)row * n + c], q);
-    uint64_t r = shoup(d, w.x, w.y, q);
-    uint64_t *o = out + poly * out_stride + (uint64_t)limb * n + c;
-    if (accumulate == 1 || (accumulate == 2 && (poly & 1) == 0))
-        r = add_mod(base ? base[poly * base_stride + (uint64_t)limb * n + c] : *o, r, q);
-    *o = r;
 }
 
 // ---- key switching
@@ -417,26 +383,10 @@ void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const 
                        1u << t.logn);
 }
 
-void rns_sub_mul_ptrs(const DeviceTables &t, uint64_t *const *out, const uint64_t *const *x, const uint64_t *tmp,
-                      const ulonglong2 *inv, uint32_t count, uint32_t polys, uint32_t l_out, uint32_t x_limbs,
-                      hipStream_t s)
-{
-    OutPtrs o;
-    PairPtrs xs;
-    for (uint32_t i = 0; i < kPtrMax; i++) {
-        o.out[i] = i < count ? out[i] : nullptr;
-        xs.x[i] = nullptr;
-        xs.y[i] = i < count ? x[i] : nullptr;
-    }
-    ProfScope prof("rns_sub_mul", s, 24.0 * (1u << t.logn) * count * polys * l_out);
-    hipLaunchKernelGGL(k_sub_mul_ptrs, grid_for(t.logn, count * polys * l_out), dim3(kBlock), 0, s, o, xs, tmp, inv,
-                       t.mod, polys, l_out, x_limbs, 1u << t.logn);
-}
-
 void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stride, const uint64_t *d,
                          uint64_t d_stride, uint32_t limb, ulonglong2 w, uint32_t polys, hipStream_t s)
 {
-    ProfScope prof("rns_sub_mul", s, 24.0 * (1u << t.logn) * polys);
+    ProfScope prof("add_limb_scaled", s, 24.0 * (1u << t.logn) * polys);
     hipLaunchKernelGGL(k_add_limb_scaled, dim3(((1u << t.logn) + kBlock - 1) / kBlock, polys), dim3(kBlock), 0, s,
                        acc, acc_stride, d, d_stride, limb, w, t.mod, 1u << t.logn);
 }
@@ -477,15 +427,6 @@ void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last,
 {
     hipLaunchKernelGGL(k_center_last, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, tmp, last, last_stride,
                        t.mod, last_prime, limbs, 1u << t.logn);
-}
-
-void rns_sub_mul(const DeviceTables &t, uint64_t *out, uint64_t out_stride, const uint64_t *x, uint64_t x_stride,
-                 const uint64_t *tmp, const ulonglong2 *inv, uint32_t polys, uint32_t limbs, int accumulate,
-                 hipStream_t s, const uint64_t *base, uint64_t base_stride)
-{
-    ProfScope prof("rns_sub_mul", s, 8.0 * (1u << t.logn) * polys * limbs * (3 + (accumulate ? 1 : 0)));
-    hipLaunchKernelGGL(k_sub_mul, grid_for(t.logn, polys * limbs), dim3(kBlock), 0, s, out, out_stride, x, x_stride,
-                       tmp, inv, t.mod, limbs, 1u << t.logn, accumulate, base, base_stride);
 }
 
 void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, uint32_t B, uint32_t l,

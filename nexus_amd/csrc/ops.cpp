@@ -469,10 +469,15 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
     }
     DevBuf tmp(&c, P * (l - 1) * n);
     rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1), c.stream);
-    ntt_fwd(c, tmp.get(), P, l - 1, (l - 1) * n);
+    // out = (a - NTT(centred last limb)) q_(l-1)^-1, stored by the transform's last pass
     DevBuf out(&c, P * (l - 1) * n);
-    rns_sub_mul(c.tables, out.get(), (l - 1) * n, a.data.get(), l * n, tmp.get(), c.rescale_inv(l), (uint32_t)P,
-                (uint32_t)(l - 1), 0, c.stream);
+    NttEpilogue e;
+    e.out = out.get();
+    e.out_stride = (l - 1) * n;
+    e.x = a.data.get();
+    e.x_stride = l * n;
+    e.inv = c.rescale_inv(l);
+    ntt_fwd_epi(c, tmp.get(), P, l - 1, (l - 1) * n, e);
     const double scale = a.scale / (double)c.moduli[l - 1];
     Ciphertext r;
     r.ctx = &c;
@@ -509,9 +514,15 @@ static void rescale_batch(Context &c, const std::vector<const uint64_t *> &x, si
     }
     DevBuf tmp(&c, P * (l - 1) * n);
     rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1), c.stream);
-    ntt_fwd(c, tmp.get(), P, l - 1, (l - 1) * n);
-    rns_sub_mul_ptrs(c.tables, outs.data(), x.data(), tmp.get(), c.rescale_inv(l), (uint32_t)B, (uint32_t)polys,
-                     (uint32_t)(l - 1), (uint32_t)l, c.stream);
+    NttEpilogue e;  // outs[r] = (x[r] - NTT(centred last limbs)) q_(l-1)^-1
+    for (size_t r = 0; r < B; r++) {
+        e.outs[r] = outs[r];
+        e.xs[r] = x[r];
+    }
+    e.x_limbs = (uint32_t)l;
+    e.polys_per = (uint32_t)polys;
+    e.inv = c.rescale_inv(l);
+    ntt_fwd_epi(c, tmp.get(), P, l - 1, (l - 1) * n, e);
 }
 
 void op_rescale_many(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out)

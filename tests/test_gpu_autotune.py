@@ -27,7 +27,7 @@ def test_autotune_then_parity(logn, chain):
     rep = nx.autotune(ctx, reps=3)
     print(rep)
     assert rep["logn"] == logn
-    assert len(rep["ntt"]) == 16  # 2 directions x batches 1..128
+    assert len(rep["ntt"]) == 20  # 2 directions x batches 1..512
     for e in rep["ntt"]:
         assert e["col"] in (0, 4, 8, 16) and e["row"] in (0, 4, 8)
     assert rep["dot_plain"] and all(d["layout"] in (0, 1) for d in rep["dot_plain"])
