@@ -134,6 +134,10 @@ void poly_permute(uint64_t *out, const uint64_t *a, const uint32_t *perm, uint32
 // Fill limb i of each poly with host_value[i] (host array, passed by value).
 void poly_fill_limbs(uint64_t *out, const uint64_t *host_value, uint32_t polys, uint32_t limbs, int logn,
                      hipStream_t s);
+// dst[p][0 .. limbs n) = src[p][0 .. limbs n) for p < polys (poly strides in
+// words, multiples of 2; n >= 2): the first limbs of each poly, e.g. a mod switch
+void poly_copy_limbs(const DeviceTables &t, uint64_t *dst, uint64_t dst_stride, const uint64_t *src,
+                     uint64_t src_stride, uint32_t polys, uint32_t limbs, hipStream_t s);
 // out[i][c] = a[i][c] + v[i] on one poly (host array of residues, passed by value)
 void poly_add_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const uint64_t *host_v,
                      uint32_t limbs, hipStream_t s);

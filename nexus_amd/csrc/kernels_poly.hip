@@ -146,6 +146,17 @@ __global__ void k_add_limb_scaled(uint64_t *acc, uint64_t acc_stride, const uint
     *a = add_mod(*a, shoup(d[p * d_stride + (uint64_t)limb * n + c], w.x, w.y, q), q);
 }
 
+// 16 bytes per thread: words 2i, 2i+1 of poly blockIdx.y (words = limbs n, even)
+__global__ void k_copy_polys(uint64_t *dst, uint64_t dst_stride, const uint64_t *src, uint64_t src_stride,
+                             uint64_t pairs)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i >= pairs) return;
+    const uint64_t p = blockIdx.y;
+    const ulonglong2 v = reinterpret_cast<const ulonglong2 *>(src + p * src_stride)[i];
+    reinterpret_cast<ulonglong2 *>(dst + p * dst_stride)[i] = v;
+}
+
 __global__ void k_mul_scalar(uint64_t *out, const uint64_t *a, LimbShoup scal, const ModConst *mod, uint32_t limbs,
                              uint32_t n)
 {
@@ -389,6 +400,16 @@ void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stri
     ProfScope prof("add_limb_scaled", s, 24.0 * (1u << t.logn) * polys);
     hipLaunchKernelGGL(k_add_limb_scaled, dim3(((1u << t.logn) + kBlock - 1) / kBlock, polys), dim3(kBlock), 0, s,
                        acc, acc_stride, d, d_stride, limb, w, t.mod, 1u << t.logn);
+}
+
+void poly_copy_limbs(const DeviceTables &t, uint64_t *dst, uint64_t dst_stride, const uint64_t *src,
+                     uint64_t src_stride, uint32_t polys, uint32_t limbs, hipStream_t s)
+{
+    const uint64_t pairs = ((uint64_t)limbs << t.logn) / 2;
+    if (!polys || !pairs) return;
+    ProfScope prof("copy", s, 16.0 * (double)pairs * 2.0 * polys);
+    hipLaunchKernelGGL(k_copy_polys, dim3((uint32_t)((pairs + kBlock - 1) / kBlock), polys), dim3(kBlock), 0, s, dst,
+                       dst_stride, src, src_stride, pairs);
 }
 
 void poly_mul_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, const ulonglong2 *host_w,

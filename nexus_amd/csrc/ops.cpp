@@ -23,8 +23,7 @@ void copy_limbs(Context &c, uint64_t *dst, size_t dst_stride, const uint64_t *sr
                 size_t limbs)
 {
     if (!polys || !limbs) return;
-    NX_HIP(hipMemcpy2DAsync(dst, dst_stride * 8, src, src_stride * 8, limbs * c.n * 8, polys,
-                            hipMemcpyDeviceToDevice, c.stream));
+    poly_copy_limbs(c.tables, dst, dst_stride, src, src_stride, (uint32_t)polys, (uint32_t)limbs, c.stream);
 }
 
 bool scale_within_bounds(const Context &c, double scale, size_t limbs)
