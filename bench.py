@@ -4,7 +4,7 @@
 Workload (one "step"): one encrypted BERT-base encoder layer (hidden 768, 12
 heads, FFN 3072, weights ~N(0, 0.02) random-init, no biases, LayerNorm gamma 1
 beta 0) over 8 sequences of uneven length ~U{32..128} with inputs ~N(0, 1), at
-N = 2^16 with bootstrapping between blocks (chain {51, 46 x 19, 51 x 14, 58 x 8},
+N = 2^16 with bootstrapping between blocks (chain {51, 46 x 19, 51 x 14, 51 x 8},
 8 special primes (5 key-switching digits), Hamming weight 192, scale 2^46).  The batched path packs the
 sequences into sets of 128 token rows (first-fit decreasing; attention stays
 within each sequence) and runs one layer call per set; the plain path runs
@@ -33,7 +33,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "enc. BERT-base layers/sec + p50/p99 latency, batched vs plain GPU, ≤2^-10 err"
 CHAIN = [58] + [40] * 18 + [58]
-BERT_CHAIN = [51] + [46] * 19 + [51] * 14 + [58] * 8
+BERT_CHAIN = [51] + [46] * 19 + [51] * 14 + [51] * 8  # special primes below 2^51: double-precision paths
 BERT_SPECIAL = 8
 BERT_SCALE = 2.0 ** 46
 BERT_TOKENS = 128

@@ -190,12 +190,12 @@ def test_server_continuous_batching(env):
 
 def test_bert_base_full_size():
     """BERT-base (hidden 768, 12 heads, FFN 3072) at N = 2^16 with 128-token
-    sets, chain {51, 46 x 19, 51 x 14, 58 x 8}: a 40-token sequence and two
+    sets, chain {51, 46 x 19, 51 x 14, 51 x 8}: a 40-token sequence and two
     packed sequences against the FP64 model within 2^-10"""
     if nx.device_count() < 1:
         pytest.skip("no GPU visible")
     _note("bert-base full size keys")
-    chain = [51] + [46] * 19 + [51] * 14 + [58] * 8
+    chain = [51] + [46] * 19 + [51] * 14 + [51] * 8
     ctx = nx.Context(16, chain, hamming_weight=192, special_primes=8)
     sk = nx.SecretKey(ctx, seed=81)
     rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=82)
