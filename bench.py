@@ -43,7 +43,7 @@ HBM_PEAK_GBS = 8000.0
 # of `bench.py --only ntt` / `--only dot` calibrated against a device copy of
 # known size (profiles/ntt_fwd_n65536_38limbs_traffic.md,
 # profiles/dot_plain_n65536_64terms_traffic.md)
-TRAFFIC_PER_ALGORITHMIC_BYTE = {"ntt_fwd_col": 40.37 / 39.93, "ntt_fwd_row": 50.54 / 60.73,
+TRAFFIC_PER_ALGORITHMIC_BYTE = {"ntt_fwd_col": 40.38 / 39.93, "ntt_fwd_row": 50.57 / 60.73,
                                 "dot_plain": 1930.5 / 1932.5}
 TRAFFIC_PROFILE = {"ntt_fwd_col": "profiles/ntt_fwd_n65536_38limbs_traffic.md",
                    "ntt_fwd_row": "profiles/ntt_fwd_n65536_38limbs_traffic.md",

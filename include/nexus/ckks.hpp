@@ -222,6 +222,11 @@ public:
     // sum_r cts[r] * pts[r] in one pass
     void multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                                    Ciphertext &dest) const;
+    // dest[g] = sum_r pts[g K + r] * cts[r] (K = cts.size(), null: no term): the
+    // giant steps of one BSGS output at once, each ciphertext read once per 4 rows
+    void multiply_plain_accumulate_multi(const std::vector<const Ciphertext *> &cts,
+                                         const std::vector<const Plaintext *> &pts,
+                                         std::vector<Ciphertext> &dest) const;
     // batched: dest[i] = rescale(cts[i]) / rescale(relinearize(a[i] * b[i])),
     // word for word the per-ciphertext results (dest may alias the inputs);
     // fused: relinearisation and rescale in one rounding (hybrid key

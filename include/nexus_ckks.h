@@ -209,6 +209,14 @@ int nx_rotate_sum(const nx_ciphertext *const *in, const int *steps, size_t count
  * + add per term) and of the MatrixMul row sums (cuda/src/matrix_mul.cu:202-275). */
 int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
                                  nx_ciphertext *dest);
+/* dests[g] = sum_r pts[g count + r] * cts[r] for g < rows (pts entries may be
+ * NULL: no term; every row needs one): all giant steps of one BSGS output at
+ * once (no counterpart in the reference, which runs one multiply-accumulate per
+ * giant step, cuda/src/bootstrapping/Bootstrapper.cu:1855-1919).  Each
+ * ciphertext is read once for every 4 rows instead of once per row; each row
+ * equals nx_multiply_plain_accumulate over its terms word for word. */
+int nx_multiply_plain_accumulate_multi(const nx_ciphertext *const *cts, size_t count, const nx_plaintext *const *pts,
+                                       size_t rows, nx_ciphertext *const *dests);
 
 /* ---- Periodic plaintexts and the block mix.  No counterpart in the
  * reference, which multiplies such masks as full plaintexts term by term

@@ -553,6 +553,24 @@ def multiply_plain_accumulate(cts, pts, dest=None):
     return d
 
 
+_sig("nx_multiply_plain_accumulate_multi", [ctypes.POINTER(vp), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
+
+
+def multiply_plain_accumulate_multi(cts, rows):
+    """[sum_r rows[g][r] * cts[r] for g] with each ciphertext read once per 4 rows
+    (rows[g][r] may be None: no term)."""
+    K, G = len(cts), len(rows)
+    flat = []
+    for row in rows:
+        if len(row) != K:
+            raise ValueError("every row needs one entry per ciphertext")
+        flat += [p.h if p is not None else None for p in row]
+    outs = [Ciphertext(cts[0].ctx) for _ in range(G)]
+    _check(_lib.nx_multiply_plain_accumulate_multi((vp * K)(*[c.h for c in cts]), K, (vp * (G * K))(*flat), G,
+                                                   (vp * G)(*[o.h for o in outs])))
+    return outs
+
+
 _sig("nx_rescale_to_next_many", [ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
 _sig("nx_multiply_relin_rescale_many", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp, ctypes.POINTER(vp)])
 _sig("nx_multiply_relin_rescale_fused_many", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp, ctypes.POINTER(vp)])

@@ -74,4 +74,27 @@ int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plain
     });
 }
 
+int nx_multiply_plain_accumulate_multi(const nx_ciphertext *const *cts, size_t count, const nx_plaintext *const *pts,
+                                       size_t rows, nx_ciphertext *const *dests)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(cts, "cts");
+        nx::need_ptr(pts, "pts");
+        nx::need_ptr(dests, "dests");
+        std::vector<const nx::Ciphertext *> c;
+        std::vector<const nx::Plaintext *> p;
+        std::vector<nx::Ciphertext *> o;
+        for (size_t r = 0; r < count; r++) {
+            nx::need_ptr(cts[r], "cts[r]");
+            c.push_back(&cts[r]->impl);
+        }
+        for (size_t i = 0; i < rows * count; i++) p.push_back(pts[i] ? &pts[i]->impl : nullptr);
+        for (size_t g = 0; g < rows; g++) {
+            nx::need_ptr(dests[g], "dests[g]");
+            o.push_back(&dests[g]->impl);
+        }
+        nx::op_dot_plain_multi(c, p, o);
+    });
+}
+
 }  // extern "C"

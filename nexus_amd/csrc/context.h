@@ -279,6 +279,10 @@ void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst);
 // sum_r cts[r] * pts[r] in one pass (size-2 ciphertexts of one level)
 void op_dot_plain(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                   Ciphertext &dst);
+// outs[g] = sum_r pts[g K + r] * cts[r] (K = cts.size(), null: no term), each
+// ciphertext read once for every few rows
+void op_dot_plain_multi(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
+                        const std::vector<Ciphertext *> &outs);
 void op_multiply_power_of_x(const Ciphertext &a, int64_t exponent, Ciphertext &dst);
 void op_mod_raise(const Ciphertext &a, Ciphertext &dst);
 // hoisted rotations (one ModUp of c1): one output per step / their sum

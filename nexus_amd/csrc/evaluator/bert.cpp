@@ -173,27 +173,30 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
         for (auto &t : r) baby[c].push_back(std::move(t));
     }
     std::vector<Ciphertext> out(d.n_out);
+    std::vector<const Ciphertext *> terms;  // every (input, baby) pair
+    for (int c = 0; c < d.n_in; c++)
+        for (int b = 0; b < B; b++) terms.push_back(&baby[c][b]);
     for (int o = 0; o < d.n_out; o++) {
-        // per giant step a multiply-accumulate; the giant rotations summed
-        // with a single ModDown
-        std::vector<Ciphertext> accs;
+        // every giant step's multiply-accumulate in one pass over the babies;
+        // the giant rotations summed with a single ModDown
+        std::vector<const Plaintext *> weights;
         std::vector<int> gsteps;
         for (int a = d.a_min; a <= d.a_max; a++) {
-            std::vector<const Ciphertext *> terms;
-            std::vector<const Plaintext *> weights;
+            std::vector<const Plaintext *> row(terms.size(), nullptr);
+            bool any = false;
             for (int c = 0; c < d.n_in; c++)
                 for (int b = 0; b < B; b++) {
                     const size_t k = (((size_t)o * d.n_in + c) * n_a + (size_t)(a - d.a_min)) * B + b;
                     if (!enc->present[k]) continue;
-                    terms.push_back(&baby[c][b]);
-                    weights.push_back(&enc->pt[k]);
+                    row[(size_t)c * B + b] = &enc->pt[k];
+                    any = true;
                 }
-            if (terms.empty()) continue;
-            Ciphertext acc;
-            ev.multiply_plain_accumulate(terms, weights, acc);
-            accs.push_back(std::move(acc));
+            if (!any) continue;
+            weights.insert(weights.end(), row.begin(), row.end());
             gsteps.push_back(B * a);
         }
+        std::vector<Ciphertext> accs;
+        if (!gsteps.empty()) ev.multiply_plain_accumulate_multi(terms, weights, accs);
         if (accs.empty()) throw std::logic_error("BertLayer: all-zero weight block");
         std::vector<const Ciphertext *> ap;
         for (const auto &a : accs) ap.push_back(&a);

@@ -785,16 +785,20 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
     ck_.evaluator.rotate_vector_hoisted(x, bsteps, *ck_.galois_keys, rots);
     std::map<int, Ciphertext> baby;
     for (size_t k = 0; k < bi.size(); k++) baby.emplace(bi[k], std::move(rots[k]));
-    // giant steps: one multiply-accumulate each, then their rotations summed
-    // with a single ModDown
-    std::vector<Ciphertext> accs;
+    // giant steps: sum_i D~_(j,i) * rot(x, i stride) for every j in one pass
+    // over the babies, then their rotations summed with a single ModDown
+    std::vector<const Ciphertext *> terms;
+    std::map<int, size_t> column;
+    for (auto &kv : baby) {
+        column[kv.first] = terms.size();
+        terms.push_back(&kv.second);
+    }
+    std::vector<const Plaintext *> weights;
     std::vector<int> gsteps;
     auto it = lv.diag.begin();
     while (it != lv.diag.end()) {
         const int j = it->first.first;
-        // sum_i D~_(j,i) * rot(x, i stride) in one multiply-accumulate pass
-        std::vector<const Ciphertext *> terms;
-        std::vector<const Plaintext *> weights;
+        std::vector<const Plaintext *> row(terms.size(), nullptr);
         for (; it != lv.diag.end() && it->first.first == j; ++it) {
             const int i = it->first.second;
             auto key = std::make_tuple(tag, j, i, l, sigma * value_factor);
@@ -804,14 +808,13 @@ Ciphertext Bootstrapper::apply_level(const LinearLevel &lv, int tag, const Ciphe
                 ck_.encoder.encode(it->second, chain_of(l), sigma * value_factor, pt);
                 pit = ptcache_.emplace(key, std::move(pt)).first;
             }
-            terms.push_back(&baby.at(i));
-            weights.push_back(&pit->second);
+            row[column.at(i)] = &pit->second;
         }
-        Ciphertext acc;
-        ck_.evaluator.multiply_plain_accumulate(terms, weights, acc);
-        accs.push_back(std::move(acc));
+        weights.insert(weights.end(), row.begin(), row.end());
         gsteps.push_back(j * lv.g * lv.stride);
     }
+    std::vector<Ciphertext> accs;
+    ck_.evaluator.multiply_plain_accumulate_multi(terms, weights, accs);
     std::vector<const Ciphertext *> ap;
     for (const auto &a : accs) ap.push_back(&a);
     Ciphertext result;

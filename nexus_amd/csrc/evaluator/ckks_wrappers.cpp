@@ -427,6 +427,28 @@ void Evaluator::multiply_plain_accumulate(const std::vector<const Ciphertext *> 
     check(nx_multiply_plain_accumulate(c.data(), p.data(), c.size(), out.get()));
     dest = std::move(out);
 }
+
+void Evaluator::multiply_plain_accumulate_multi(const std::vector<const Ciphertext *> &cts,
+                                                const std::vector<const Plaintext *> &pts,
+                                                std::vector<Ciphertext> &dest) const
+{
+    if (cts.empty() || pts.size() % cts.size())
+        throw std::invalid_argument("multiply_plain_accumulate_multi: plaintexts must be rows x ciphertexts");
+    const size_t rows = pts.size() / cts.size();
+    std::vector<const nx_ciphertext *> c;
+    std::vector<const nx_plaintext *> p;
+    for (const Ciphertext *x : cts) c.push_back(x->get());
+    for (const Plaintext *x : pts) p.push_back(x ? x->get() : nullptr);
+    std::vector<Ciphertext> res;
+    res.reserve(rows);
+    std::vector<nx_ciphertext *> o;
+    for (size_t g = 0; g < rows; g++) {
+        res.emplace_back(*ctx_);
+        o.push_back(res.back().get());
+    }
+    check(nx_multiply_plain_accumulate_multi(c.data(), c.size(), p.data(), rows, o.data()));
+    dest = std::move(res);
+}
 void Evaluator::rescale_to_next_many(const std::vector<const Ciphertext *> &cts, std::vector<Ciphertext> &dest) const
 {
     std::vector<const nx_ciphertext *> in;

@@ -199,6 +199,14 @@ void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64
 // out[k][limb] = sum_r cts[r][k][limb] * pts[r][limb] for k < 2 (count <= kMaxDot):
 // the plaintext multiply-accumulate of a BSGS giant step in one pass.
 constexpr int kMaxDot = 64;
+// out_g[k][limb] (+)= sum_r cts[r][k][limb] * pts[g count + r][limb] for rows
+// g < rows <= kMultiDotG, k < 2, count <= kMultiDotK (pts entries may be null:
+// no term); each ciphertext is read once for all rows.  accumulate adds into
+// the outputs instead of writing them.
+constexpr int kMultiDotG = 4, kMultiDotK = 32;
+void poly_dot_plain_multi(const DeviceTables &t, uint64_t *const *outs, const uint64_t *const *cts,
+                          const uint64_t *const *pts, uint32_t rows, uint32_t count, uint32_t limbs, bool accumulate,
+                          hipStream_t s);
 void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const *cts, const uint64_t *const *pts,
                     uint32_t count, uint32_t limbs, hipStream_t s);
 // Layout of k_dot_plain for launches of at most `limbs` limbs: 0 = default

@@ -325,6 +325,65 @@ __global__ __launch_bounds__(kBlock) void k_dot_plain(uint64_t *out, DotArgs arg
     *reinterpret_cast<ulonglong2 *>(out + off1) = o1;
 }
 
+struct MultiDotArgs {
+    const uint64_t *ct[kMultiDotK];
+    const uint64_t *pt[kMultiDotG][kMultiDotK];  // null: no term
+    uint64_t *out[kMultiDotG];                    // null: unused row
+};
+
+// out_g[k][limb][c] (+)= sum_r ct_r[k][limb][c] * pt_g,r[limb][c] for the rows g
+// of one launch (k < 2, count <= kMultiDotK, so no intermediate fold): every
+// ciphertext word is read once for all rows.  Two adjacent coefficients of one
+// limb per thread, 16-byte loads, as k_dot_plain.
+__global__ __launch_bounds__(kBlock) void k_dot_plain_multi(MultiDotArgs args, uint32_t count, const ModConst *mod,
+                                                            uint32_t limbs, uint32_t n, int accumulate)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;  // coefficient pair
+    if (2 * i >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const ModConst m = mod[limb];
+    const uint64_t poff = (uint64_t)limb * n + 2 * i, off1 = poff + (uint64_t)limbs * n;
+    uint64_t lo[kMultiDotG][4] = {}, hi[kMultiDotG][4] = {};
+    for (uint32_t r = 0; r < count; r++) {
+        const ulonglong2 a = *reinterpret_cast<const ulonglong2 *>(args.ct[r] + poff);
+        const ulonglong2 b = *reinterpret_cast<const ulonglong2 *>(args.ct[r] + off1);
+        const uint64_t xs[4] = {a.x, a.y, b.x, b.y};
+#pragma unroll
+        for (int g = 0; g < kMultiDotG; g++) {
+            const uint64_t *p = args.pt[g][r];
+            if (!p) continue;  // uniform across the launch
+            const ulonglong2 y = *reinterpret_cast<const ulonglong2 *>(p + poff);
+            const uint64_t ys[4] = {y.x, y.y, y.x, y.y};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const uint64_t plo = xs[k] * ys[k], phi = mulhi64(xs[k], ys[k]);
+                lo[g][k] += plo;
+                hi[g][k] += phi + (lo[g][k] < plo);
+            }
+        }
+    }
+#pragma unroll
+    for (int g = 0; g < kMultiDotG; g++) {
+        uint64_t *o = args.out[g];
+        if (!o) continue;
+        ulonglong2 o0, o1;
+        o0.x = barrett128(hi[g][0], lo[g][0], m);
+        o0.y = barrett128(hi[g][1], lo[g][1], m);
+        o1.x = barrett128(hi[g][2], lo[g][2], m);
+        o1.y = barrett128(hi[g][3], lo[g][3], m);
+        if (accumulate) {
+            const ulonglong2 p0 = *reinterpret_cast<const ulonglong2 *>(o + poff);
+            const ulonglong2 p1 = *reinterpret_cast<const ulonglong2 *>(o + off1);
+            o0.x = add_mod(o0.x, p0.x, m.q);
+            o0.y = add_mod(o0.y, p0.y, m.q);
+            o1.x = add_mod(o1.x, p1.x, m.q);
+            o1.y = add_mod(o1.y, p1.y, m.q);
+        }
+        *reinterpret_cast<ulonglong2 *>(o + poff) = o0;
+        *reinterpret_cast<ulonglong2 *>(o + off1) = o1;
+    }
+}
+
 // One coefficient of one polynomial per thread (grid y over poly x limb): four
 // times the threads of k_dot_plain, for launches with few limbs.
 __global__ __launch_bounds__(kBlock) void k_dot_plain1(uint64_t *out, DotArgs args, uint32_t count,
@@ -422,6 +481,28 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
     }
     hipLaunchKernelGGL(k_moddown_conv_fused, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, l, alpha,
                        L, n);
+}
+
+void poly_dot_plain_multi(const DeviceTables &t, uint64_t *const *outs, const uint64_t *const *cts,
+                          const uint64_t *const *pts, uint32_t rows, uint32_t count, uint32_t limbs, bool accumulate,
+                          hipStream_t s)
+{
+    const uint32_t n = 1u << t.logn;
+    if (rows == 0 || rows > (uint32_t)kMultiDotG || count > (uint32_t)kMultiDotK) abort();
+    MultiDotArgs a;
+    uint32_t terms = 0;
+    for (uint32_t r = 0; r < (uint32_t)kMultiDotK; r++) a.ct[r] = r < count ? cts[r] : nullptr;
+    for (uint32_t g = 0; g < (uint32_t)kMultiDotG; g++) {
+        a.out[g] = g < rows ? outs[g] : nullptr;
+        for (uint32_t r = 0; r < (uint32_t)kMultiDotK; r++) {
+            a.pt[g][r] = (g < rows && r < count) ? pts[(size_t)g * count + r] : nullptr;
+            terms += a.pt[g][r] ? 1u : 0u;
+        }
+    }
+    ProfScope prof("dot_plain", s,
+                   8.0 * n * limbs * (2.0 * count + terms + 2.0 * rows * (accumulate ? 2.0 : 1.0)));
+    hipLaunchKernelGGL(k_dot_plain_multi, dim3((n / 2 + kBlock - 1) / kBlock, limbs), dim3(kBlock), 0, s, a, count,
+                       t.mod, limbs, n, accumulate ? 1 : 0);
 }
 
 void poly_dot_plain(const DeviceTables &t, uint64_t *out, const uint64_t *const *cts, const uint64_t *const *pts,

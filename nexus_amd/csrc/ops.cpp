@@ -891,6 +891,67 @@ void op_dot_plain(const std::vector<const Ciphertext *> &cts, const std::vector<
     dst = std::move(out);
 }
 
+void op_dot_plain_multi(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
+                        const std::vector<Ciphertext *> &outs)
+{
+    // outs[g] = sum_r pts[g K + r] * cts[r] (null plaintexts: no term), the
+    // giant steps of one BSGS output at once; each row equals op_dot_plain over
+    // its terms word for word
+    const size_t K = cts.size(), G = outs.size();
+    require(K >= 1 && G >= 1 && pts.size() == G * K, "need rows x ciphertexts plaintexts");
+    const Ciphertext &a0 = *cts[0];
+    valid(a0);
+    require(a0.size == 2 && a0.ntt, "encrypted must be size 2 in NTT form");
+    Context &c = *a0.ctx;
+    const size_t l = a0.limbs;
+    for (const Ciphertext *a : cts) {
+        valid(*a);
+        require(a->ctx == &c && a->size == 2 && a->limbs == l && a->ntt, "encrypted parameter mismatch");
+    }
+    std::vector<double> scale(G, 0.0);
+    for (size_t g = 0; g < G; g++) {
+        for (size_t r = 0; r < K; r++) {
+            const Plaintext *p = pts[g * K + r];
+            if (!p) continue;
+            plain_check(*cts[r], *p);
+            const double sc = cts[r]->scale * p->scale;
+            if (scale[g] == 0.0) scale[g] = sc;
+            require(are_close(sc, scale[g]), "scale mismatch");
+        }
+        require(scale[g] != 0.0, "every output needs a term");
+        require(scale_within_bounds(c, scale[g], l), "scale out of bounds");
+    }
+    std::vector<Ciphertext> res(G);
+    std::vector<uint64_t *> op(G);
+    for (size_t g = 0; g < G; g++) {
+        Ciphertext &t = res[g];
+        t.ctx = &c;
+        t.size = 2;
+        t.limbs = l;
+        t.scale = scale[g];
+        t.ntt = true;
+        t.data.reset(&c, 2 * l * c.n);
+        op[g] = t.data.get();
+    }
+    std::vector<const uint64_t *> cp(K), pp;
+    for (size_t r = 0; r < K; r++) cp[r] = cts[r]->data.get();
+    for (size_t g0 = 0; g0 < G; g0 += kMultiDotG) {
+        const size_t rows = std::min<size_t>(kMultiDotG, G - g0);
+        for (size_t r0 = 0; r0 < K; r0 += kMultiDotK) {
+            const size_t cnt = std::min<size_t>(kMultiDotK, K - r0);
+            pp.assign(rows * cnt, nullptr);
+            for (size_t g = 0; g < rows; g++)
+                for (size_t r = 0; r < cnt; r++) {
+                    const Plaintext *p = pts[(g0 + g) * K + r0 + r];
+                    pp[g * cnt + r] = p ? p->data.get() : nullptr;
+                }
+            poly_dot_plain_multi(c.tables, op.data() + g0, cp.data() + r0, pp.data(), (uint32_t)rows, (uint32_t)cnt,
+                                 (uint32_t)l, r0 > 0, c.stream);
+        }
+    }
+    for (size_t g = 0; g < G; g++) *outs[g] = std::move(res[g]);
+}
+
 void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst)
 {
     require(!in.empty(), "encrypteds cannot be empty");

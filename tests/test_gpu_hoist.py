@@ -108,6 +108,31 @@ def test_multiply_plain_accumulate_bit_exact(env, count):
     assert np.array_equal(got.to_array(), ref.to_array())
 
 
+def test_multiply_plain_accumulate_multi_bit_exact(env):
+    """every giant step of a BSGS output in one pass (37 ciphertexts, 6 rows:
+    both chunk sizes crossed, holes in every row, one row with a single term)
+    equals multiply_plain_accumulate over each row's terms word for word"""
+    ctx, sk, _, _ = env
+    _note("multiply_plain_accumulate_multi 37 x 6")
+    rng = np.random.default_rng(77)
+    K, G = 37, 6
+    cts = [_enc(ctx, sk, rng.uniform(-1, 1, ctx.slot_count)) for _ in range(K)]
+    rows = []
+    for g in range(G):
+        keep = rng.random(K) < 0.7 if g != 3 else np.arange(K) == 33
+        rows.append([nx.encode(ctx, rng.uniform(-1, 1, ctx.slot_count), SCALE) if keep[r] else None
+                     for r in range(K)])
+    got = nx.multiply_plain_accumulate_multi(cts, rows)
+    assert len(got) == G
+    for g in range(G):
+        terms = [(c, p) for c, p in zip(cts, rows[g]) if p is not None]
+        ref = nx.multiply_plain_accumulate([c for c, _ in terms], [p for _, p in terms])
+        assert np.array_equal(got[g].to_array(), ref.to_array()), g
+        assert got[g].info()["scale"] == ref.info()["scale"]
+    with pytest.raises(nx.NexusError):  # a row without terms
+        nx.multiply_plain_accumulate_multi(cts[:2], [[None, None]])
+
+
 def test_rotate_sum_matches_separate_rotations():
     """sum_i rot(ct_i, s_i) with one ModDown decrypts like the sum of separate
     rotations (identity steps and repeated steps included)"""
