@@ -278,6 +278,21 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
 
 // Base-conversion constants of hybrid key switching (fast basis extension:
 // x -> sum_i [x_i Dhat_i^-1]_{q_i} Dhat_i mod t, Dhat_i = prod_{i' != i} q_i').
+// Double-precision copy of a ModDown conversion table over A input primes
+// (rows k < A: factors f_ki, row A: the base product mod q_i): rows k < A as
+// doubles, row A as q_i - (row A).  Used for output primes below 2^51 when every
+// input prime is below 2^51.
+static std::vector<double> moddown_conv_doubles(const uint64_t *conv, size_t A, size_t L,
+                                                const std::vector<uint64_t> &moduli)
+{
+    std::vector<double> d((A + 1) * L, 0.0);
+    for (size_t i = 0; i < L; i++) {
+        for (size_t k = 0; k < A; k++) d[k * L + i] = (double)conv[k * L + i];
+        d[A * L + i] = (double)(moduli[i] - conv[A * L + i] % moduli[i]);
+    }
+    return d;
+}
+
 void Context::build_hybrid_tables()
 {
     const size_t K = key_limbs();
@@ -364,6 +379,13 @@ void Context::build_hybrid_tables()
             for (size_t k2 = 0; k2 < alpha; k2++) pm = host_mul_mod(pm, moduli[L + k2] % ql, ql);
             mdr_pmod[l] = make_ulonglong2(pm, shoup_quotient(pm, ql));  // P mod q_(l-1)
         }
+        std::vector<double> fcd((L + 1) * W * L, 0.0);
+        for (size_t l = 2; l <= L; l++) {
+            const std::vector<double> d = moddown_conv_doubles(&fc[l * W * L], A, L, moduli);
+            std::copy(d.begin(), d.end(), fcd.begin() + l * W * L);
+        }
+        NX_HIP(hipMalloc(&d_mdr_conv_d, fcd.size() * sizeof(double)));
+        NX_HIP(hipMemcpy(d_mdr_conv_d, fcd.data(), fcd.size() * sizeof(double), hipMemcpyHostToDevice));
         NX_HIP(hipMalloc(&d_mdr_w, fw.size() * sizeof(ulonglong2)));
         NX_HIP(hipMalloc(&d_mdr_conv, fc.size() * sizeof(uint64_t)));
         NX_HIP(hipMalloc(&d_mdr_inv, fi.size() * sizeof(ulonglong2)));
@@ -385,6 +407,11 @@ void Context::build_hybrid_tables()
     }
     NX_HIP(hipMemcpy(d_moddown_w, dw.data(), dw.size() * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_moddown_conv, dc.data(), dc.size() * sizeof(uint64_t), hipMemcpyHostToDevice));
+    {
+        const std::vector<double> dcd = moddown_conv_doubles(dc.data(), alpha, L, moduli);
+        NX_HIP(hipMalloc(&d_moddown_conv_d, dcd.size() * sizeof(double)));
+        NX_HIP(hipMemcpy(d_moddown_conv_d, dcd.data(), dcd.size() * sizeof(double), hipMemcpyHostToDevice));
+    }
 }
 
 Context::~Context()
@@ -404,6 +431,8 @@ Context::~Context()
     (void)hipFree(d_mdr_inv);
     (void)hipFree(d_moddown_w);
     (void)hipFree(d_moddown_conv);
+    (void)hipFree(d_moddown_conv_d);
+    (void)hipFree(d_mdr_conv_d);
     for (auto &kv : galois_) (void)hipFree(kv.second);
     (void)hipFree(d_fwd);
     (void)hipFree(d_inv);

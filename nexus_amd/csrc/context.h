@@ -199,6 +199,9 @@ public:
     ulonglong2 *d_modup_w = nullptr, *d_moddown_w = nullptr;
     uint64_t *d_modup_conv = nullptr, *d_moddown_conv = nullptr;
     double *d_modup_conv_d = nullptr;  // d_modup_conv as doubles
+    // ModDown factors as doubles (row alpha: q_i - P mod q_i), and the same per
+    // level for the ModDown by P q_(l-1) ((alpha + 2) L doubles a level)
+    double *d_moddown_conv_d = nullptr, *d_mdr_conv_d = nullptr;
     // ModDown by P' = P q_(l-1) at level l (hybrid relinearisation + rescale in one
     // rounding), base (q_(l-1), p_0..p_(alpha-1)):
     //   mdr_w[l (alpha+1) + k] = [(P'/b_k)^-1]_(b_k) (Shoup)

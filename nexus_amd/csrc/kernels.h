@@ -178,9 +178,11 @@ bool ks_conv_fused_ok(uint32_t alpha);
 void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
                             const uint64_t *conv, uint32_t B, uint32_t l, uint32_t alpha, uint32_t key_limbs,
                             const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+// convd: the factors as doubles (Context::d_moddown_conv_d); used when every
+// special prime is below 2^51
 void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
-                              const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
-                              uint32_t L, hipStream_t s);
+                              const ulonglong2 *w, const uint64_t *conv, const double *convd, uint32_t polys,
+                              uint32_t l, uint32_t alpha, uint32_t L, hipStream_t s);
 // ModDown by P' = P q_(l-1) in one rounding: the limbs (q_(l-1), p_0..p_(alpha-1))
 // of each poly of acc (starting at limb l_out = l - 1, poly stride acc_stride,
 // coefficient form) converted exactly (centred) to q_0..q_(l_out - 1) into
@@ -194,8 +196,8 @@ void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targe
                        const uint64_t *const *target_list, uint32_t B, uint32_t l, uint32_t alpha, uint32_t digits,
                        uint32_t nout, hipStream_t s);
 void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
-                                const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l_out,
-                                uint32_t alpha, uint32_t L, hipStream_t s);
+                                const ulonglong2 *w, const uint64_t *conv, const double *convd, uint32_t polys,
+                                uint32_t l_out, uint32_t alpha, uint32_t L, hipStream_t s);
 // out[k][limb] = sum_r cts[r][k][limb] * pts[r][limb] for k < 2 (count <= kMaxDot):
 // the plaintext multiply-accumulate of a BSGS giant step in one pass.
 constexpr int kMaxDot = 64;

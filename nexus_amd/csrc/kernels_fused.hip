@@ -193,18 +193,24 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
     }
 }
 
-// k_moddown_conv_fused for alpha = A with every prime below 2^58 (Sum3 products)
+constexpr uint32_t kMdChunk = 8;  // output primes per thread of k_moddown_conv_a
+
+// k_moddown_conv_fused for A input primes, every prime below 2^58 (Sum3
+// products).  When convd is given (every input prime below 2^51), output
+// primes below 2^51 take exact double-precision products (mulmod_f), the sum
+// reduced every second term.
 template <int A>
 __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict__ z,
                                                           const uint64_t *__restrict__ acc, uint64_t acc_stride,
                                                           const ulonglong2 *__restrict__ w,
                                                           const uint64_t *__restrict__ conv,
+                                                          const double *__restrict__ convd,
                                                           const ModConst *__restrict__ mod, InPrimes inp, uint32_t l,
                                                           uint32_t L, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
-    const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
+    const uint32_t chunks = (l + kMdChunk - 1) / kMdChunk;
     const uint32_t p = blockIdx.y / chunks, chunk = blockIdx.y - p * chunks;
     const uint64_t *y = acc + (uint64_t)p * acc_stride + (uint64_t)l * n + c;
     uint64_t v[A];
@@ -218,8 +224,23 @@ __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict_
     }
     const uint64_t r = (uint64_t)floor(frac + 0.5);  // <= A
     uint64_t *out = z + (uint64_t)p * l * n + c;
-    const uint32_t i_end = min(l, (chunk + 1) * kOutChunk);
-    for (uint32_t i = chunk * kOutChunk; i < i_end; i++) {
+    const uint32_t i_end = min(l, (chunk + 1) * kMdChunk);
+    for (uint32_t i = chunk * kMdChunk; i < i_end; i++) {
+        const uint64_t qi = mod[i].q;
+        if (convd && qi < kFpPrimeBound) {
+            const double q = (double)qi, qinv = 1.0 / q;
+            // - r (P mod q_i) as r (q_i - P mod q_i)
+            double s = mulmod_f((double)(uint32_t)r, convd[(uint64_t)A * L + i], q, qinv);
+#pragma unroll
+            for (int k = 0; k < A; k++) {
+                s += mulmod_f((double)v[k], convd[(uint64_t)k * L + i], q, qinv);
+                if (k & 1) s = reduce_f(s, q, qinv);  // |s| stays below 4q < 2^53
+            }
+            s = reduce_f(s, q, qinv);
+            s = s < 0 ? s + q : s;
+            out[(uint64_t)i * n] = (uint64_t)s;
+            continue;
+        }
         uint64_t fv[A];
 #pragma unroll
         for (int k = 0; k < A; k++) fv[k] = conv[(uint64_t)k * L + i];
@@ -229,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict_
         uint64_t hi, lo;
         s3.fold(hi, lo);
         // + r (P mod q_i)'s negative: r < 2^5, so the product fits one word
-        const uint64_t add = r * (mod[i].q - conv[(uint64_t)A * L + i]);
+        const uint64_t add = r * (qi - conv[(uint64_t)A * L + i]);
         lo += add;
         hi += lo < add;
         out[(uint64_t)i * n] = barrett128(hi, lo, mod[i]);
@@ -456,9 +477,17 @@ void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *
                        nout, key_limbs, n);
 }
 
+// convd only when every input prime is below 2^51
+static const double *inputs_fp(const DeviceTables &t, const double *convd, const InPrimes &inp, uint32_t A)
+{
+    for (uint32_t k = 0; k < A; k++)
+        if (t.mod_host[inp.p[k]] >= kFpPrimeBound) return nullptr;
+    return convd;
+}
+
 void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
-                              const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l, uint32_t alpha,
-                              uint32_t L, hipStream_t s)
+                              const ulonglong2 *w, const uint64_t *conv, const double *convd, uint32_t polys,
+                              uint32_t l, uint32_t alpha, uint32_t L, hipStream_t s)
 {
     const uint32_t n = 1u << t.logn;
     ProfScope prof("ks_moddown_conv", s, 8.0 * n * polys * ((double)alpha + l));
@@ -466,12 +495,14 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
     for (uint32_t k = 0; k <= kRegAlpha; k++) inp.p[k] = (uint8_t)(L + (k < alpha ? k : 0));
     const uint32_t chunks = (l + kOutChunk - 1) / kOutChunk;
     const dim3 grid((n + kBlock - 1) / kBlock, polys * chunks);
+    const dim3 grid_a((n + kBlock - 1) / kBlock, polys * ((l + kMdChunk - 1) / kMdChunk));
+    const double *cd = inputs_fp(t, convd, inp, alpha);
     if (t.max_bits <= 58) {
         switch (alpha) {
 #define NX_A(A)                                                                                                  \
     case A:                                                                                                      \
-        hipLaunchKernelGGL(k_moddown_conv_a<A>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, inp, \
-                           l, L, n);                                                                             \
+        hipLaunchKernelGGL(k_moddown_conv_a<A>, grid_a, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, cd,     \
+                           t.mod, inp, l, L, n);                                                                 \
         return;
             NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
             NX_A(14) NX_A(15) NX_A(16)
@@ -555,20 +586,21 @@ void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targe
 bool ks_moddown_rescale_ok(const DeviceTables &t, uint32_t alpha) { return t.max_bits <= 58 && alpha + 1 <= 16; }
 
 void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
-                                const ulonglong2 *w, const uint64_t *conv, uint32_t polys, uint32_t l_out,
-                                uint32_t alpha, uint32_t L, hipStream_t s)
+                                const ulonglong2 *w, const uint64_t *conv, const double *convd, uint32_t polys,
+                                uint32_t l_out, uint32_t alpha, uint32_t L, hipStream_t s)
 {
     const uint32_t n = 1u << t.logn, A = alpha + 1;
     ProfScope prof("ks_moddown_conv", s, 8.0 * n * polys * ((double)A + l_out));
     InPrimes inp;
     for (uint32_t k = 0; k <= kRegAlpha; k++) inp.p[k] = (uint8_t)(k == 0 ? l_out : L + (k - 1 < alpha ? k - 1 : 0));
-    const uint32_t chunks = (l_out + kOutChunk - 1) / kOutChunk;
+    const uint32_t chunks = (l_out + kMdChunk - 1) / kMdChunk;
     const dim3 grid((n + kBlock - 1) / kBlock, polys * chunks);
+    const double *cd = inputs_fp(t, convd, inp, A);
     switch (A) {
 #define NX_A(AA)                                                                                                  \
     case AA:                                                                                                      \
-        hipLaunchKernelGGL(k_moddown_conv_a<AA>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, t.mod, inp, \
-                           l_out, L, n);                                                                          \
+        hipLaunchKernelGGL(k_moddown_conv_a<AA>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, cd,       \
+                           t.mod, inp, l_out, L, n);                                                              \
         return;
         NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13) NX_A(14)
         NX_A(15) NX_A(16)

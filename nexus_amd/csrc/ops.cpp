@@ -312,7 +312,8 @@ DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool tr
                         c.stream);
     else if (ks_conv_fused_ok((uint32_t)alpha))  // hybrid: exact centred conversion of the P part
         ks_moddown_convert_fused(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv,
-                                 (uint32_t)(2 * B), (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
+                                 c.d_moddown_conv_d, (uint32_t)(2 * B), (uint32_t)l, (uint32_t)alpha, (uint32_t)L,
+                                 c.stream);
     else
         ks_moddown_convert(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv, (uint32_t)(2 * B),
                            (uint32_t)l, (uint32_t)alpha, (uint32_t)L, c.stream);
@@ -679,7 +680,8 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
         }
         DevBuf tmp(&c, cnt * 2 * lo * n);  // conversion, then the transform's first pass
         ks_moddown_rescale_convert(c.tables, tmp.get(), acc.get(), nout * n, c.d_mdr_w + l * A,
-                                   c.d_mdr_conv + l * (A + 1) * c.L, (uint32_t)(2 * cnt), (uint32_t)lo,
+                                   c.d_mdr_conv + l * (A + 1) * c.L, c.d_mdr_conv_d + l * (A + 1) * c.L,
+                                   (uint32_t)(2 * cnt), (uint32_t)lo,
                                    (uint32_t)alpha, (uint32_t)c.L, c.stream);
         NttEpilogue e;  // out_i = (acc_i - NTT(tmp)_i) P'^-1 + d_i q_(l-1)^-1
         for (size_t r = 0; r < cnt; r++) {

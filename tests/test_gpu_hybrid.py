@@ -54,11 +54,15 @@ def test_hybrid_relin_bit_exact(logn, L, alpha, chain_index):
 
 
 @pytest.mark.parametrize("chain,alpha,chain_index", [([46] * 6 + [58] * 2, 2, 1), ([51] * 7 + [58] * 3, 3, 2),
-                                                     ([58] + [46] * 5 + [58] * 2, 2, 1)],
-                         ids=["46-58", "51-58", "mixed"])
+                                                     ([58] + [46] * 5 + [58] * 2, 2, 1),
+                                                     ([58] + [46] * 5 + [51] * 2, 2, 1),
+                                                     ([51] * 9 + [51] * 8, 8, 1)],
+                         ids=["46-58", "51-58", "mixed", "mixed-fp-inputs", "51-alpha8"])
 def test_hybrid_relin_bit_exact_wide(chain, alpha, chain_index):
-    """58-bit special primes (ModDown inputs beyond 2^51: split double-precision
-    products) and a 58-bit data prime (the 64-bit integer path for that output)"""
+    """58-bit special primes (ModDown inputs beyond 2^51: integer products), a
+    58-bit data prime (the integer path for that output) next to 51-bit special
+    primes (double-precision products for the other outputs), and 8 special
+    primes of 51 bits as in the BERT chain"""
     if nx.device_count() < 1:
         pytest.skip("no GPU visible")
     logn, L = 8, len(chain) - alpha
