@@ -222,6 +222,11 @@ public:
     // sum_r cts[r] * pts[r] in one pass
     void multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                                    Ciphertext &dest) const;
+    // dest = sum_k cts[k] * encode(values[k], chain_index, scales[k]) at chain_index,
+    // each input read at its first limbs: word for word mod_switch + multiply_plain
+    // + add per term, at scale cts[0].scale * scales[0]
+    void linear_combination(const std::vector<const Ciphertext *> &cts, const std::vector<double> &values,
+                            const std::vector<double> &scales, size_t chain_index, Ciphertext &dest) const;
     // dest[g] = sum_r pts[g K + r] * cts[r] (K = cts.size(), null: no term): the
     // giant steps of one BSGS output at once, each ciphertext read once per 4 rows
     void multiply_plain_accumulate_multi(const std::vector<const Ciphertext *> &cts,

@@ -215,6 +215,15 @@ int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plain
  * giant step, cuda/src/bootstrapping/Bootstrapper.cu:1855-1919).  Each
  * ciphertext is read once for every 4 rows instead of once per row; each row
  * equals nx_multiply_plain_accumulate over its terms word for word. */
+/* dest = sum_r cts[r] * encode_constant(values[r], scales[r]) at chain_index
+ * (each input read at its first limbs, no copy): word for word mod_switch_to,
+ * multiply_plain by the encoded constant and add per term (no counterpart in
+ * the reference, which runs a multiply_const and an add per term, e.g. the
+ * leaves of its Chebyshev evaluation, cuda/src/bootstrapping/common/
+ * Polynomial.cu:429-447).  The
+ * result carries scale cts[0].scale * scales[0]. */
+int nx_linear_combination(const nx_ciphertext *const *cts, const double *values, const double *scales, size_t count,
+                          size_t chain_index, nx_ciphertext *dest);
 int nx_multiply_plain_accumulate_multi(const nx_ciphertext *const *cts, size_t count, const nx_plaintext *const *pts,
                                        size_t rows, nx_ciphertext *const *dests);
 

@@ -553,6 +553,20 @@ def multiply_plain_accumulate(cts, pts, dest=None):
     return d
 
 
+_sig("nx_linear_combination", [ctypes.POINTER(vp), f64p, f64p, sz, sz, vp])
+
+
+def linear_combination(cts, values, scales, chain_index):
+    """sum_k cts[k] * encode_constant(values[k], scales[k]) at chain_index, each
+    input read at its first limbs (word for word mod_switch + multiply_plain + add)"""
+    K = len(cts)
+    out = Ciphertext(cts[0].ctx)
+    v = (ctypes.c_double * K)(*values)
+    sc = (ctypes.c_double * K)(*scales)
+    _check(_lib.nx_linear_combination((vp * K)(*[c.h for c in cts]), v, sc, K, chain_index, out.h))
+    return out
+
+
 _sig("nx_multiply_plain_accumulate_multi", [ctypes.POINTER(vp), sz, ctypes.POINTER(vp), sz, ctypes.POINTER(vp)])
 
 

@@ -74,6 +74,29 @@ int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plain
     });
 }
 
+int nx_linear_combination(const nx_ciphertext *const *cts, const double *values, const double *scales, size_t count,
+                          size_t chain_index, nx_ciphertext *dest)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(cts, "cts");
+        nx::need_ptr(values, "values");
+        nx::need_ptr(scales, "scales");
+        nx::need_ptr(dest, "dest");
+        nx::require(count >= 1, "count must be positive");
+        std::vector<const nx::Ciphertext *> c;
+        for (size_t r = 0; r < count; r++) {
+            nx::need_ptr(cts[r], "cts[r]");
+            c.push_back(&cts[r]->impl);
+        }
+        nx::Context &ctx = *c[0]->ctx;
+        nx::require(c[0]->ctx != nullptr && chain_index >= 1 && chain_index <= ctx.L, "chain_index out of range");
+        nx::Ciphertext out;
+        nx::op_linear_combination(c, std::vector<double>(values, values + count),
+                                  std::vector<double>(scales, scales + count), ctx.limbs_of(chain_index), out);
+        dest->impl = std::move(out);
+    });
+}
+
 int nx_multiply_plain_accumulate_multi(const nx_ciphertext *const *cts, size_t count, const nx_plaintext *const *pts,
                                        size_t rows, nx_ciphertext *const *dests)
 {

@@ -282,6 +282,10 @@ void op_add_many(const std::vector<const Ciphertext *> &in, Ciphertext &dst);
 // sum_r cts[r] * pts[r] in one pass (size-2 ciphertexts of one level)
 void op_dot_plain(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                   Ciphertext &dst);
+// sum_k xs[k] * encode_constant(values[k], scales[k]) at `limbs` limbs (each x_k
+// read at its first limbs), scale xs[0].scale * scales[0]
+void op_linear_combination(const std::vector<const Ciphertext *> &xs, const std::vector<double> &values,
+                           const std::vector<double> &scales, size_t limbs, Ciphertext &dst);
 // outs[g] = sum_r pts[g K + r] * cts[r] (K = cts.size(), null: no term), each
 // ciphertext read once for every few rows
 void op_dot_plain_multi(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
@@ -368,6 +372,8 @@ void upload_encoder_tables(Context &c);
 void encode_vector(Context &c, const double *re, const double *im, size_t count, size_t limbs, double scale,
                    Plaintext &out);
 void encode_constant(Context &c, double value, size_t limbs, double scale, Plaintext &out);
+// residues of round(value * scale) mod q_0 .. q_(limbs-1) (the constant encode_constant fills in)
+void constant_residues(Context &c, double value, size_t limbs, double scale, uint64_t *res);
 void decode(Context &c, const Plaintext &pt, double *re, double *im, size_t count);
 
 // keys -- keys.cpp

@@ -484,28 +484,19 @@ Ciphertext Bootstrapper::cheb_eval(const std::vector<double> &c, size_t level, d
         const size_t lam = level + 1;
         const double q = prime(lam);
         Ciphertext acc;
-        bool have = false;
+        // sum_k c_k T_k at level lam in one pass, the basis read in place
+        std::vector<const Ciphertext *> xs;
+        std::vector<double> vals, scs;
         for (int k = 1; k <= d; k++) {
             if (c[k] == 0.0) continue;
             const Ciphertext &bk = basis(k);
             if (bk.coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: basis below the leaf level");
-            Ciphertext t;
-            const Ciphertext *src = &bk;
-            if (bk.coeff_modulus_size() > lam) {
-                ck_.evaluator.mod_switch_to(bk, chain_of(lam), t);
-                src = &t;
-            }
-            Plaintext ck(*ck_.context);
-            ck_.encoder.encode(c[k], chain_of(lam), scale * q / src->scale(), ck);
-            ck_.evaluator.multiply_plain(*src, ck, t);
-            if (!have) {
-                acc = std::move(t);
-                have = true;
-            } else {
-                t.set_scale(acc.scale());
-                ck_.evaluator.add_inplace(acc, t);
-            }
+            xs.push_back(&bk);
+            vals.push_back(c[k]);
+            scs.push_back(scale * q / bk.scale());
         }
+        const bool have = !xs.empty();
+        if (have) ck_.evaluator.linear_combination(xs, vals, scs, chain_of(lam), acc);
         if (!have) {  // constant polynomial: 0 * T_1 carries the level and scale
             acc = basis(1);
             ck_.evaluator.mod_switch_to_inplace(acc, chain_of(lam));
@@ -665,29 +656,23 @@ Bootstrapper::Batch Bootstrapper::cheb_eval_many(const std::vector<double> &c, s
         const size_t lam = level + 1;
         const double q = prime(lam);
         Batch acc(B);
-        bool have = false;
+        // sum_k c_k T_k at level lam: the basis read in place, the constants
+        // encoded at scale * q / T_k's scale, every term in one pass per input
+        std::vector<const Batch *> bks;
+        std::vector<double> vals, scs;
         for (int k = 1; k <= d; k++) {
             if (c[k] == 0.0) continue;
             const Batch &bk = basis_many(k);
             if (bk[0].coeff_modulus_size() < lam) throw std::logic_error("Bootstrapper: basis below the leaf level");
-            Plaintext ck(*ck_.context);
-            for (size_t i = 0; i < B; i++) {
-                Ciphertext t;
-                const Ciphertext *src = &bk[i];
-                if (bk[i].coeff_modulus_size() > lam) {
-                    ck_.evaluator.mod_switch_to(bk[i], chain_of(lam), t);
-                    src = &t;
-                }
-                if (i == 0) ck_.encoder.encode(c[k], chain_of(lam), scale * q / src->scale(), ck);
-                ck_.evaluator.multiply_plain(*src, ck, t);
-                if (!have) {
-                    acc[i] = std::move(t);
-                } else {
-                    t.set_scale(acc[i].scale());
-                    ck_.evaluator.add_inplace(acc[i], t);
-                }
-            }
-            have = true;
+            bks.push_back(&bk);
+            vals.push_back(c[k]);
+            scs.push_back(scale * q / bk[0].scale());
+        }
+        const bool have = !bks.empty();
+        for (size_t i = 0; have && i < B; i++) {
+            std::vector<const Ciphertext *> xs;
+            for (const Batch *bk : bks) xs.push_back(&(*bk)[i]);
+            ck_.evaluator.linear_combination(xs, vals, scs, chain_of(lam), acc[i]);
         }
         if (!have) {  // constant polynomial: 0 * T_1 carries the level and scale
             const Batch &b1 = basis_many(1);

@@ -428,6 +428,18 @@ void Evaluator::multiply_plain_accumulate(const std::vector<const Ciphertext *> 
     dest = std::move(out);
 }
 
+void Evaluator::linear_combination(const std::vector<const Ciphertext *> &cts, const std::vector<double> &values,
+                                   const std::vector<double> &scales, size_t chain_index, Ciphertext &dest) const
+{
+    if (cts.size() != values.size() || cts.size() != scales.size())
+        throw std::invalid_argument("linear_combination: term counts differ");
+    std::vector<const nx_ciphertext *> c;
+    for (const Ciphertext *x : cts) c.push_back(x->get());
+    Ciphertext res(*ctx_);
+    check(nx_linear_combination(c.data(), values.data(), scales.data(), c.size(), chain_index, res.get()));
+    dest = std::move(res);
+}
+
 void Evaluator::multiply_plain_accumulate_multi(const std::vector<const Ciphertext *> &cts,
                                                 const std::vector<const Plaintext *> &pts,
                                                 std::vector<Ciphertext> &dest) const

@@ -134,6 +134,12 @@ void poly_permute(uint64_t *out, const uint64_t *a, const uint32_t *perm, uint32
 // Fill limb i of each poly with host_value[i] (host array, passed by value).
 void poly_fill_limbs(uint64_t *out, const uint64_t *host_value, uint32_t polys, uint32_t limbs, int logn,
                      hipStream_t s);
+// out[p][i] (+)= sum_k x[k][p][i] w[k][i] for polys p < 2, limbs i < limbs,
+// terms <= kLinMax (x[k] poly stride x_stride[k] words, at least `limbs` limbs;
+// w[k] host arrays of residues): constant multiples summed in one pass
+constexpr int kLinMax = 4;
+void poly_lincomb(const DeviceTables &t, uint64_t *out, const uint64_t *const *x, const uint64_t *x_stride,
+                  const uint64_t *const *w, uint32_t terms, uint32_t limbs, bool accumulate, hipStream_t s);
 // dst[p][0 .. limbs n) = src[p][0 .. limbs n) for p < polys (poly strides in
 // words, multiples of 2; n >= 2): the first limbs of each poly, e.g. a mod switch
 void poly_copy_limbs(const DeviceTables &t, uint64_t *dst, uint64_t dst_stride, const uint64_t *src,

@@ -146,6 +146,36 @@ __global__ void k_add_limb_scaled(uint64_t *acc, uint64_t acc_stride, const uint
     *a = add_mod(*a, shoup(d[p * d_stride + (uint64_t)limb * n + c], w.x, w.y, q), q);
 }
 
+struct LinArgs {
+    const uint64_t *x[kLinMax];
+    uint64_t x_stride[kLinMax];
+    uint64_t w[kLinMax][kMaxPrimes];
+};
+
+// out[p][i][c] (+)= sum_k x_k[p][i][c] w_k[i] mod q_i over the first `limbs`
+// limbs of each x_k (poly stride x_stride[k]): 128-bit sums, one reduction
+__global__ void k_lincomb(uint64_t *out, LinArgs a, uint32_t terms, const ModConst *mod, uint32_t limbs, uint32_t n,
+                          int accumulate)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t row = blockIdx.y;  // p * limbs + i
+    const uint32_t p = row / limbs, i = row - p * limbs;
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t k = 0; k < terms; k++) {
+        const uint64_t x = a.x[k][p * a.x_stride[k] + (uint64_t)i * n + c];
+        const uint64_t w = a.w[k][i];
+        const uint64_t plo = x * w, phi = mulhi64(x, w);
+        lo += plo;
+        hi += phi + (lo < plo);
+    }
+    const ModConst m = mod[i];
+    uint64_t r = barrett128(hi, lo, m);
+    uint64_t *o = out + (uint64_t)row * n + c;
+    if (accumulate) r = add_mod(r, *o, m.q);
+    *o = r;
+}
+
 // 16 bytes per thread: words 2i, 2i+1 of poly blockIdx.y (words = limbs n, even)
 __global__ void k_copy_polys(uint64_t *dst, uint64_t dst_stride, const uint64_t *src, uint64_t src_stride,
                              uint64_t pairs)
@@ -400,6 +430,21 @@ void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stri
     ProfScope prof("add_limb_scaled", s, 24.0 * (1u << t.logn) * polys);
     hipLaunchKernelGGL(k_add_limb_scaled, dim3(((1u << t.logn) + kBlock - 1) / kBlock, polys), dim3(kBlock), 0, s,
                        acc, acc_stride, d, d_stride, limb, w, t.mod, 1u << t.logn);
+}
+
+void poly_lincomb(const DeviceTables &t, uint64_t *out, const uint64_t *const *x, const uint64_t *x_stride,
+                  const uint64_t *const *w, uint32_t terms, uint32_t limbs, bool accumulate, hipStream_t s)
+{
+    if (terms == 0 || terms > (uint32_t)kLinMax || limbs > (uint32_t)kMaxPrimes) abort();
+    LinArgs a;
+    for (uint32_t k = 0; k < (uint32_t)kLinMax; k++) {
+        a.x[k] = k < terms ? x[k] : nullptr;
+        a.x_stride[k] = k < terms ? x_stride[k] : 0;
+        for (uint32_t i = 0; i < (uint32_t)kMaxPrimes; i++) a.w[k][i] = (k < terms && i < limbs) ? w[k][i] : 0;
+    }
+    ProfScope prof("lincomb", s, 8.0 * (1u << t.logn) * 2 * limbs * (terms + (accumulate ? 2.0 : 1.0)));
+    hipLaunchKernelGGL(k_lincomb, grid_for(t.logn, 2 * limbs), dim3(kBlock), 0, s, out, a, terms, t.mod, limbs,
+                       1u << t.logn, accumulate ? 1 : 0);
 }
 
 void poly_copy_limbs(const DeviceTables &t, uint64_t *dst, uint64_t dst_stride, const uint64_t *src,

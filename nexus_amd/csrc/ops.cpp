@@ -893,6 +893,47 @@ void op_dot_plain(const std::vector<const Ciphertext *> &cts, const std::vector<
     dst = std::move(out);
 }
 
+void op_linear_combination(const std::vector<const Ciphertext *> &xs, const std::vector<double> &values,
+                           const std::vector<double> &scales, size_t limbs, Ciphertext &dst)
+{
+    // sum_k x_k * encode_constant(values[k], scales[k]) over the first `limbs`
+    // limbs of each x_k: word for word mod_switch + multiply_plain + add of each
+    // term, with the sum at the first term's scale
+    require(!xs.empty() && xs.size() == values.size() && xs.size() == scales.size(), "term counts must match");
+    const Ciphertext &a0 = *xs[0];
+    valid(a0);
+    Context &c = *a0.ctx;
+    require(limbs >= 1 && limbs <= a0.limbs, "level out of range");
+    for (const Ciphertext *x : xs) {
+        valid(*x);
+        require(x->ctx == &c && x->size == 2 && x->ntt && x->limbs >= limbs, "encrypted parameter mismatch");
+    }
+    const double scale = a0.scale * scales[0];
+    require(scale_within_bounds(c, scale, limbs), "scale out of bounds");
+    const size_t K = xs.size();
+    std::vector<std::vector<uint64_t>> res(K, std::vector<uint64_t>(kMaxPrimes, 0));
+    for (size_t k = 0; k < K; k++) constant_residues(c, values[k], limbs, scales[k], res[k].data());
+    Ciphertext out;
+    out.ctx = &c;
+    out.size = 2;
+    out.limbs = limbs;
+    out.scale = scale;
+    out.ntt = true;
+    out.data.reset(&c, 2 * limbs * c.n);
+    for (size_t k0 = 0; k0 < K; k0 += kLinMax) {
+        const size_t cnt = std::min<size_t>(kLinMax, K - k0);
+        const uint64_t *x[kLinMax], *w[kLinMax];
+        uint64_t stride[kLinMax];
+        for (size_t k = 0; k < cnt; k++) {
+            x[k] = xs[k0 + k]->data.get();
+            stride[k] = xs[k0 + k]->limbs * c.n;
+            w[k] = res[k0 + k].data();
+        }
+        poly_lincomb(c.tables, out.data.get(), x, stride, w, (uint32_t)cnt, (uint32_t)limbs, k0 > 0, c.stream);
+    }
+    dst = std::move(out);
+}
+
 void op_dot_plain_multi(const std::vector<const Ciphertext *> &cts, const std::vector<const Plaintext *> &pts,
                         const std::vector<Ciphertext *> &outs)
 {

@@ -99,6 +99,33 @@ def test_fused_relin_rescale_within_one(env, count):
     assert changed <= 8
 
 
+def test_linear_combination_matches(env):
+    """sum_k x_k * encode_constant(v_k, s_k) at a lower level with inputs at
+    three levels (6 terms: two launches), negative and tiny constants: word for
+    word mod_switch + multiply_plain + add per term, at the first term's scale"""
+    ctx, sk, rk = env
+    _note("linear combination logn=%d" % ctx.log_n)
+    L = ctx.L
+    target = min(L, 3)
+    cts = _cts(ctx, sk, 6, 300)
+    cts = [nx.mod_switch_to(c, 1 + (i % target)) for i, c in enumerate(cts)]
+    # products stay below the target level's modulus (one 51-bit prime at n16)
+    values = [0.75, -1.25, 3e-3, -0.5, 1.0, 2.0 ** -6]
+    scales = [2.0 ** 8, 2.0 ** 8 * 1.5, 2.0 ** 7, 2.0 ** 9, 2.0 ** 8, 2.0 ** 8]
+    got = nx.linear_combination(cts, values, scales, target)
+    want = None
+    for c, v, sc in zip(cts, values, scales):
+        t = nx.multiply_plain(nx.mod_switch_to(c, target), nx.encode_constant(ctx, v, sc, target))
+        if want is None:
+            want = t
+        else:
+            t.scale = want.scale
+            want = nx.add(want, t)
+    assert got.chain_index == target
+    assert np.array_equal(got.to_array(), want.to_array())
+    assert got.info()["scale"] == want.info()["scale"]
+
+
 @pytest.mark.parametrize("count", [1, 20])
 def test_rescale_many_matches(env, count):
     ctx, sk, rk = env
