@@ -406,15 +406,8 @@ Ciphertext Bootstrapper::match(const Ciphertext &ct, size_t level, double scale)
     const size_t l = ct.coeff_modulus_size();
     if (l <= level) throw std::logic_error("Bootstrapper: operand below the target level");
     Ciphertext t;
-    const Ciphertext *src = &ct;
-    if (l > level + 1) {
-        ck_.evaluator.mod_switch_to(ct, chain_of(level + 1), t);
-        src = &t;
-    }
-    const double sigma = scale * prime(level + 1) / src->scale();
-    Plaintext one(*ck_.context);
-    ck_.encoder.encode(1.0, chain_of(level + 1), sigma, one);
-    ck_.evaluator.multiply_plain(*src, one, t);
+    const double sigma = scale * prime(level + 1) / ct.scale();
+    ck_.evaluator.linear_combination({&ct}, {1.0}, {sigma}, chain_of(level + 1), t);  // read in place
     ck_.evaluator.rescale_to_next_inplace(t);
     t.set_scale(scale);
     return t;
@@ -459,15 +452,18 @@ const Ciphertext &Bootstrapper::basis(int k)
     const Ciphertext &Ta = basis(a);
     const Ciphertext &Tb = basis(b);
     Ciphertext prod = (a == b) ? mul_exact(Ta, Ta) : mul_exact(Ta, Tb);
-    ck_.evaluator.add_inplace(prod, prod);  // 2 T_a T_b
     const size_t l = prod.coeff_modulus_size();
     if (a == b) {
+        ck_.evaluator.add_inplace(prod, prod);  // 2 T_a^2 - 1
         Plaintext one(*ck_.context);
         ck_.encoder.encode(1.0, chain_of(l), prod.scale(), one);
         ck_.evaluator.sub_plain_inplace(prod, one);
     } else {
+        // 2 T_a T_b - T_(a-b) in one pass (constants 2 and -1 at scale 1)
         Ciphertext Td = match(basis(a - b), l, prod.scale());
-        ck_.evaluator.sub_inplace(prod, Td);
+        Ciphertext r;
+        ck_.evaluator.linear_combination({&prod, &Td}, {2.0, -1.0}, {1.0, 1.0}, chain_of(l), r);
+        prod = std::move(r);
     }
     return basis_.emplace(k, std::move(prod)).first->second;
 }
@@ -578,23 +574,12 @@ std::vector<const Ciphertext *> ptrs(const std::vector<Ciphertext> &v)
 Bootstrapper::Batch Bootstrapper::match_many(const Batch &ct, size_t level, double scale)
 {
     Batch t(ct.size());
-    Plaintext one;
-    double one_sigma = 0;
     for (size_t i = 0; i < ct.size(); i++) {
         const size_t l = ct[i].coeff_modulus_size();
         if (l <= level) throw std::logic_error("Bootstrapper: operand below the target level");
-        const Ciphertext *src = &ct[i];
-        if (l > level + 1) {
-            ck_.evaluator.mod_switch_to(ct[i], chain_of(level + 1), t[i]);
-            src = &t[i];
-        }
-        const double sigma = scale * prime(level + 1) / src->scale();
-        if (!one.get() || sigma != one_sigma) {
-            one = Plaintext(*ck_.context);
-            ck_.encoder.encode(1.0, chain_of(level + 1), sigma, one);
-            one_sigma = sigma;
-        }
-        ck_.evaluator.multiply_plain(*src, one, t[i]);
+        // times 1 encoded at the compensating scale, read in place at level + 1
+        const double sigma = scale * prime(level + 1) / ct[i].scale();
+        ck_.evaluator.linear_combination({&ct[i]}, {1.0}, {sigma}, chain_of(level + 1), t[i]);
     }
     ck_.evaluator.rescale_to_next_many(ptrs(t), t);
     for (auto &x : t) x.set_scale(scale);
@@ -635,15 +620,20 @@ const Bootstrapper::Batch &Bootstrapper::basis_many(int k)
     const Batch &Ta = basis_many(a);
     const Batch &Tb = basis_many(b);
     Batch prod = (a == b) ? mul_exact_many(Ta, Ta) : mul_exact_many(Ta, Tb);
-    for (auto &x : prod) ck_.evaluator.add_inplace(x, x);  // 2 T_a T_b
     const size_t l = prod[0].coeff_modulus_size();
     if (a == b) {
+        for (auto &x : prod) ck_.evaluator.add_inplace(x, x);  // 2 T_a^2 - 1
         Plaintext one(*ck_.context);
         ck_.encoder.encode(1.0, chain_of(l), prod[0].scale(), one);
         for (auto &x : prod) ck_.evaluator.sub_plain_inplace(x, one);
     } else {
+        // 2 T_a T_b - T_(a-b) in one pass (constants 2 and -1 at scale 1)
         Batch Td = match_many(basis_many(a - b), l, prod[0].scale());
-        for (size_t i = 0; i < prod.size(); i++) ck_.evaluator.sub_inplace(prod[i], Td[i]);
+        for (size_t i = 0; i < prod.size(); i++) {
+            Ciphertext r;
+            ck_.evaluator.linear_combination({&prod[i], &Td[i]}, {2.0, -1.0}, {1.0, 1.0}, chain_of(l), r);
+            prod[i] = std::move(r);
+        }
     }
     return basis_many_.emplace(k, std::move(prod)).first->second;
 }
