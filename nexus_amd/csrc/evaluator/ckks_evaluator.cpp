@@ -173,11 +173,13 @@ void CKKSEvaluator::eval_odd_deg9_poly_many(const std::vector<double> &a, std::v
     for (size_t i = 0; i < B; i++) ev.multiply_plain(xs[i], a7, a7x[i]);
     rescale_all(ev, a7x);
     for (size_t i = 0; i < B; i++) {
-        ev.mod_switch_to_inplace(a7x[i], T2[i].params_id());
+        // T2 + a7x at T2's level and the mean scale: a7x read in place
         const double mid = (T2[i].scale() + a7x[i].scale()) / 2;
         T2[i].set_scale(mid);
         a7x[i].set_scale(mid);
-        ev.add_inplace(T2[i], a7x[i]);
+        Ciphertext sum;
+        ev.linear_combination({&T2[i], &a7x[i]}, {1.0, 1.0}, {1.0, 1.0}, T2[i].params_id(), sum);
+        T2[i] = std::move(sum);
     }
     ev.multiply_relin_rescale_many(ptrs_of(T2), ptrs_of(x6), *relin_keys, T2, true);
     // T3
@@ -186,14 +188,14 @@ void CKKSEvaluator::eval_odd_deg9_poly_many(const std::vector<double> &a, std::v
     for (size_t i = 0; i < B; i++) ev.multiply_plain(xs[i], a1, T3[i]);
     rescale_all(ev, T3);
     for (size_t i = 0; i < B; i++) {
+        // T2 + T1 + T3 at T2's level and the mean scale: T1, T3 read in place
         const double mid3 = (T1[i].scale() + T2[i].scale() + T3[i].scale()) / 3;
         T1[i].set_scale(mid3);
         T2[i].set_scale(mid3);
         T3[i].set_scale(mid3);
-        ev.mod_switch_to_inplace(T1[i], T2[i].params_id());
-        ev.add_inplace(T2[i], T1[i]);
-        ev.mod_switch_to_inplace(T3[i], T2[i].params_id());
-        ev.add_inplace(T2[i], T3[i]);
+        Ciphertext sum;
+        ev.linear_combination({&T2[i], &T1[i], &T3[i]}, {1.0, 1.0, 1.0}, {1.0, 1.0, 1.0}, T2[i].params_id(), sum);
+        T2[i] = std::move(sum);
     }
     X = std::move(T2);
 }
