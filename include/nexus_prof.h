@@ -4,9 +4,11 @@
  *
  * Every launch of a selected kernel class is bracketed by two HIP events on
  * the stream it is launched on; the launch also records its algorithmic HBM
- * bytes.  Kernel classes: "ntt_fwd_col", "ntt_fwd_row", "ntt_fwd_whole",
- * "ntt_inv_row", "ntt_inv_col", "ntt_inv_whole", "ks_inner", "ks_modup",
- * "tensor", "mul_plain", "rns_sub_mul".
+ * bytes.  Kernel classes include "ntt_fwd_col", "ntt_fwd_row",
+ * "ntt_fwd_sub_mul" (a forward row pass that stores a ModDown or rescale
+ * tail), "ntt_fwd_whole", "ntt_inv_row", "ntt_inv_col", "ntt_inv_whole",
+ * "ks_inner", "ks_modup_conv", "ks_moddown_conv", "dot_plain", "lincomb",
+ * "tensor", "mul_plain", "copy"; nx_profile_classes lists those seen.
  */
 #ifndef NEXUS_PROF_H
 #define NEXUS_PROF_H

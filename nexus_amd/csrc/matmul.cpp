@@ -117,10 +117,8 @@ void mm_expand(Context &c, const Ciphertext &ct, uint32_t m, const uint32_t *elt
         // row by row (rows of one ciphertext are contiguous)
         DevBuf perm_all(&c, S * ctw);
         poly_permute(perm_all.get(), cur.data.get(), perm, (uint32_t)(2 * S), (uint32_t)l, c.logn, c.stream);
-        NX_HIP(hipMemcpy2DAsync(rot.get(), ctw * 8, perm_all.get(), ctw * 8, l * n * 8, S, hipMemcpyDeviceToDevice,
-                                c.stream));
-        NX_HIP(hipMemcpy2DAsync(tgt.get(), l * n * 8, perm_all.get() + l * n, ctw * 8, l * n * 8, S,
-                                hipMemcpyDeviceToDevice, c.stream));
+        poly_copy_limbs(c.tables, rot.get(), ctw, perm_all.get(), ctw, (uint32_t)S, (uint32_t)l, c.stream);
+        poly_copy_limbs(c.tables, tgt.get(), l * n, perm_all.get() + l * n, ctw, (uint32_t)S, (uint32_t)l, c.stream);
         key_switch_batch(c, rot.get(), tgt.get(), S, l, it->second, true);
         // next round: [cur + rot | cur * x^(2n - 2^i) + rot * x^((2n - 2^i) * elt mod 2n)]
         CtBatch nxt;
