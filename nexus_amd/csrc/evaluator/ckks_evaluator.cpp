@@ -104,11 +104,15 @@ void CKKSEvaluator::eval_odd_deg9_poly(const std::vector<double> &a, const Ciphe
     encoder.encode(a[7], x.params_id(), a7_scale, a7);
     ev.multiply_plain(x, a7, a7x);
     ev.rescale_to_next_inplace(a7x);
-    ev.mod_switch_to_inplace(a7x, T2.params_id());
     const double mid_scale = (T2.scale() + a7x.scale()) / 2;
     T2.set_scale(mid_scale);
     a7x.set_scale(mid_scale);
-    ev.add_inplace(T2, a7x);
+    {
+        // T2 + a7x at T2's level: a7x read in place (mod switch + add in one pass)
+        Ciphertext sum;
+        ev.linear_combination({&T2, &a7x}, {1.0, 1.0}, {1.0, 1.0}, T2.params_id(), sum);
+        T2 = std::move(sum);
+    }
     ev.multiply_inplace(T2, x6);
     ev.relinearize_inplace(T2, *relin_keys);
     ev.rescale_to_next_inplace(T2);
@@ -122,11 +126,9 @@ void CKKSEvaluator::eval_odd_deg9_poly(const std::vector<double> &a, const Ciphe
     T1.set_scale(mid3);
     T2.set_scale(mid3);
     T3.set_scale(mid3);
-    Ciphertext out = T2;
-    ev.mod_switch_to_inplace(T1, out.params_id());
-    ev.add_inplace(out, T1);
-    ev.mod_switch_to_inplace(T3, out.params_id());
-    ev.add_inplace(out, T3);
+    // T2 + T1 + T3 at T2's level: T1 and T3 read in place
+    Ciphertext out;
+    ev.linear_combination({&T2, &T1, &T3}, {1.0, 1.0, 1.0}, {1.0, 1.0, 1.0}, T2.params_id(), out);
     dest = std::move(out);
 }
 
