@@ -269,31 +269,28 @@ void GELUEvaluator::gelu_many(const std::vector<const Ciphertext *> &xin, std::v
     }
     const double A[] = {2.25775755e-04, 0.5, 3.96880960e-01, -6.37042698e-02, 8.38841647e-03, -7.17830961e-04,
                         3.49617829e-05, -7.26059653e-07};
-    std::vector<Plaintext> coeff_A(8);
-    for (size_t i = 0; i < 8; i++) ck.encoder.encode(A[i], ck.scale, coeff_A[i]);
-    // term j of input i: x^(2j - 2) for j >= 2, x for j = 1
+    std::vector<Plaintext> coeff_A(1);
+    ck.encoder.encode(A[0], ck.scale, coeff_A[0]);
+    // term j of input i: A_j x^(2j - 2) for j >= 2, A_1 x for j = 1; each the
+    // power times the constant A_j encoded at the evaluator's scale, read in place
     std::vector<std::vector<Ciphertext>> terms(8, std::vector<Ciphertext>(B));
-    for (size_t i = 0; i < B; i++) {
-        terms[1][i] = *xin[i];
-        terms[2][i] = x2[i];
-        terms[3][i] = x4[i];
-        terms[4][i] = p68[i];
-        terms[5][i] = p68[B + i];
-        terms[6][i] = p1012[i];
-        terms[7][i] = p1012[B + i];
-    }
     for (size_t j = 1; j < 8; j++) {
-        ev.mod_switch_to_inplace(coeff_A[j], terms[j][0].params_id());
-        for (auto &t : terms[j]) ev.multiply_plain_inplace(t, coeff_A[j]);
+        for (size_t i = 0; i < B; i++) {
+            const Ciphertext *pw[8] = {nullptr, xin[i], &x2[i], &x4[i], &p68[i], &p68[B + i], &p1012[i],
+                                       &p1012[B + i]};
+            ev.linear_combination({pw[j]}, {A[j]}, {ck.scale}, pw[j]->params_id(), terms[j][i]);
+        }
         rescale_all(ev, terms[j]);
         for (auto &t : terms[j]) t.set_scale(ck.scale);
     }
-    std::vector<Ciphertext> Ax = std::move(terms[7]);
-    for (size_t i = 0; i < B; i++)
-        for (size_t j = 1; j < 7; j++) {
-            ev.mod_switch_to_inplace(terms[j][i], Ax[i].params_id());
-            ev.add_inplace(Ax[i], terms[j][i]);
-        }
+    // Ax = sum of the seven terms at the level of the last, each read in place
+    std::vector<Ciphertext> Ax(B);
+    for (size_t i = 0; i < B; i++) {
+        std::vector<const Ciphertext *> xs{&terms[7][i]};
+        for (size_t j = 1; j < 7; j++) xs.push_back(&terms[j][i]);
+        ev.linear_combination(xs, std::vector<double>(xs.size(), 1.0), std::vector<double>(xs.size(), 1.0),
+                              terms[7][i].params_id(), Ax[i]);
+    }
     ev.mod_switch_to_inplace(coeff_A[0], Ax[0].params_id());
     for (auto &c : Ax) ev.add_plain_inplace(c, coeff_A[0]);
     std::vector<Ciphertext> xs(B), s;
