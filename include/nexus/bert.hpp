@@ -93,6 +93,7 @@ public:
     std::vector<Ciphertext> shift_rows(const std::vector<Ciphertext> &x, int offset);
     std::vector<Ciphertext> extract_rows(const std::vector<Ciphertext> &x, int offset, int len);
     const BertShape &shape() const { return shape_; }
+    const CKKSContext *context() const { return ck_.context; }
 
     // stages (exposed for tests)
     // y = factor * x W at one limb and exactly the evaluator's scale (x taken

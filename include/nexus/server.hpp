@@ -115,6 +115,7 @@ private:
     std::vector<double> latencies_;
     size_t in_flight_ = 0, in_flight_bytes_ = 0, submitted_ = 0;
     int bert_running_ = 0;
+    int dead_lanes_ = 0;  // lanes that failed to start
     size_t peak_concurrent_ = 0, peak_footprint_ = 0, packed_calls_ = 0, packed_requests_ = 0;
     uint64_t next_id_ = 1;
     bool stop_ = false, paused_ = false;

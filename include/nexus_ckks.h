@@ -64,6 +64,11 @@ int nx_context_create_hybrid(int log_n, const int *bit_sizes, size_t count, size
 /* special primes, key-switching digits at the top level, words of one key */
 int nx_context_key_info(const nx_context *ctx, size_t *special_primes, size_t *digits, size_t *key_words);
 int nx_context_destroy(nx_context *ctx);
+/* CoeffModulus::Create (SEAL modulus.cpp:143-184, util/numth.cpp:278-310):
+ * for each bit size (2..60) the largest primes = 1 mod 2n below 2^bits, the
+ * first occurrence of a size taking the largest.  Host only (no GPU needed). */
+int nx_coeff_modulus_create(size_t n, const int *bit_sizes, size_t count, uint64_t *moduli);
+
 int nx_context_info(const nx_context *ctx, int *log_n, size_t *data_primes, size_t *slot_count);
 /* moduli: L + special words (q_0..q_{L-1}, p_0..) */
 int nx_context_moduli(const nx_context *ctx, uint64_t *moduli);
@@ -181,6 +186,25 @@ int nx_add_many(const nx_ciphertext *const *cts, size_t count, nx_ciphertext *de
  * encoded at the ciphertext's scale */
 int nx_multiply_const(const nx_ciphertext *a, double value, nx_ciphertext *dest);
 int nx_add_const(const nx_ciphertext *a, double value, nx_ciphertext *dest);
+/* SEAL-4.1-bs reduced-error operations (evaluator.cpp:420-594; CUDA mirror
+ * cuda/src/ckks_evaluator.cu:465-589, cuda/src/ckks_evaluator.cuh:338-393):
+ * dest = a + b / a - b / relinearize(a * b) after bringing the operand with
+ * more limbs down by multiply_const(s_other q_last / s^2), a scale override
+ * to s_other q_last, a rescale and a mod switch, so that both sides carry the
+ * same scale exactly.  At equal levels a takes b's scale.  dest may alias an
+ * input. */
+int nx_add_reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest);
+int nx_sub_reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest);
+int nx_multiply_reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, const nx_kswitch_keys *relin_keys,
+                              nx_ciphertext *dest);
+/* multiply_vector_reduced_error (cuda/src/ckks_evaluator.cuh:371-393; SEAL
+ * multiply_vector_inplace, evaluator.cpp:412-418): values (count <= slots,
+ * zero padded; im may be NULL) encoded at the ciphertext's level and scale,
+ * then multiply_plain. */
+int nx_multiply_vector_reduced_error(const nx_ciphertext *a, const double *re, const double *im, size_t count,
+                                     nx_ciphertext *dest);
+/* double_inplace (cuda/src/ckks_evaluator.cuh:367-369): dest = a + a */
+int nx_double(const nx_ciphertext *a, nx_ciphertext *dest);
 /* multiply by x^exponent (coefficient shift; reference matrix_mul.cu:84-127) */
 int nx_multiply_power_of_x(const nx_ciphertext *a, int64_t exponent, nx_ciphertext *dest);
 /* ModRaise of a lowest-level ciphertext to the top data level (reference

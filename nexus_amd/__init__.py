@@ -106,6 +106,14 @@ _sig("nx_add_const", [vp, ctypes.c_double, vp])
 _sig("nx_multiply_power_of_x", [vp, ctypes.c_int64, vp])
 
 
+_sig("nx_add_reduced_error", [vp, vp, vp])
+_sig("nx_sub_reduced_error", [vp, vp, vp])
+_sig("nx_multiply_reduced_error", [vp, vp, vp, vp])
+_sig("nx_multiply_vector_reduced_error", [vp, f64p, f64p, sz, vp])
+_sig("nx_double", [vp, vp])
+_sig("nx_coeff_modulus_create", [sz, ctypes.POINTER(ctypes.c_int), sz, u64p])
+
+
 def lib():
     return _lib
 
@@ -463,6 +471,53 @@ def add_const(a, value, dest=None):
     d = _out(a.ctx, dest)
     _check(_lib.nx_add_const(a.h, value, d.h))
     return d
+
+
+def add_reduced_error(a, b, dest=None):
+    """SEAL-4.1-bs add_reduced_error (evaluator.cpp:420-466)"""
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_add_reduced_error(a.h, b.h, d.h))
+    return d
+
+
+def sub_reduced_error(a, b, dest=None):
+    """SEAL-4.1-bs sub_reduced_error (evaluator.cpp:468-514)"""
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_sub_reduced_error(a.h, b.h, d.h))
+    return d
+
+
+def multiply_reduced_error(a, b, relin_keys, dest=None):
+    """SEAL-4.1-bs multiply_reduced_error (evaluator.cpp:516-594), relinearised"""
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_multiply_reduced_error(a.h, b.h, relin_keys.h, d.h))
+    return d
+
+
+def multiply_vector_reduced_error(a, values, dest=None):
+    """values encoded at a's level and scale, then multiply_plain
+    (cuda/src/ckks_evaluator.cuh:371-393)"""
+    v = np.asarray(values)
+    re = np.ascontiguousarray(np.real(v), dtype=np.float64)
+    im = np.ascontiguousarray(np.imag(v), dtype=np.float64) if np.iscomplexobj(v) else None
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_multiply_vector_reduced_error(a.h, re.ctypes.data_as(f64p),
+                                                 im.ctypes.data_as(f64p) if im is not None else None, len(re), d.h))
+    return d
+
+
+def double(a, dest=None):
+    d = _out(a.ctx, dest)
+    _check(_lib.nx_double(a.h, d.h))
+    return d
+
+
+def coeff_modulus_create(n, bit_sizes):
+    """SEAL CoeffModulus::Create on the host (no GPU needed)"""
+    bits = (ctypes.c_int * len(bit_sizes))(*bit_sizes)
+    out = np.zeros(len(bit_sizes), dtype=np.uint64)
+    _check(_lib.nx_coeff_modulus_create(n, bits, len(bit_sizes), out.ctypes.data_as(u64p)))
+    return [int(x) for x in out]
 
 
 def multiply_power_of_x(a, exponent, dest=None):

@@ -6,6 +6,7 @@
 #include <string>
 
 #include "handles.h"
+#include "numth.h"
 
 namespace {
 thread_local std::string g_err;
@@ -98,6 +99,16 @@ int nx_context_create_hybrid(int log_n, const int *bit_sizes, size_t count, size
         need(out, "out");
         std::vector<int> bits(bit_sizes, bit_sizes + count);
         *out = new nx_context(log_n, bits, hamming_weight, sparse_slots, special_primes);
+    });
+}
+
+int nx_coeff_modulus_create(size_t n, const int *bit_sizes, size_t count, uint64_t *moduli)
+{
+    return guard([&] {
+        need(bit_sizes, "bit_sizes");
+        need(moduli, "moduli");
+        const std::vector<uint64_t> q = nx::coeff_modulus_create(n, std::vector<int>(bit_sizes, bit_sizes + count));
+        for (size_t i = 0; i < q.size(); i++) moduli[i] = q[i];
     });
 }
 
@@ -703,6 +714,57 @@ int nx_add_const(const nx_ciphertext *a, double value, nx_ciphertext *dest)
         nx::Plaintext p;
         nx::encode_constant(c, value, a->impl.limbs, a->impl.scale, p);
         nx::op_add_plain(a->impl, p, dest->impl);
+    });
+}
+
+static int reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, nx::ReducedOp op,
+                         const nx_kswitch_keys *rk, nx_ciphertext *dest)
+{
+    return guard([&] {
+        need(a, "encrypted1");
+        need(b, "encrypted2");
+        need(dest, "destination");
+        if (op == nx::ReducedOp::Mul) need(rk, "relin_keys");
+        nx::op_reduced_error(a->impl, b->impl, op, rk ? &rk->impl : nullptr, dest->impl);
+    });
+}
+
+int nx_add_reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest)
+{
+    return reduced_error(a, b, nx::ReducedOp::Add, nullptr, dest);
+}
+
+int nx_sub_reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, nx_ciphertext *dest)
+{
+    return reduced_error(a, b, nx::ReducedOp::Sub, nullptr, dest);
+}
+
+int nx_multiply_reduced_error(const nx_ciphertext *a, const nx_ciphertext *b, const nx_kswitch_keys *relin_keys,
+                              nx_ciphertext *dest)
+{
+    return reduced_error(a, b, nx::ReducedOp::Mul, relin_keys, dest);
+}
+
+int nx_multiply_vector_reduced_error(const nx_ciphertext *a, const double *re, const double *im, size_t count,
+                                     nx_ciphertext *dest)
+{
+    return guard([&] {
+        need(a, "encrypted");
+        need(re, "values");
+        need(dest, "destination");
+        nx::Context &c = ctx_of(a->impl);
+        nx::Plaintext p;
+        nx::encode_vector(c, re, im, count, a->impl.limbs, a->impl.scale, p);
+        nx::op_multiply_plain(a->impl, p, dest->impl);
+    });
+}
+
+int nx_double(const nx_ciphertext *a, nx_ciphertext *dest)
+{
+    return guard([&] {
+        need(a, "encrypted");
+        need(dest, "destination");
+        nx::op_add(a->impl, a->impl, dest->impl);
     });
 }
 

@@ -88,8 +88,9 @@ int nx_linear_combination(const nx_ciphertext *const *cts, const double *values,
             nx::need_ptr(cts[r], "cts[r]");
             c.push_back(&cts[r]->impl);
         }
+        nx::require(c[0]->ctx != nullptr, "ciphertext is not initialised");
         nx::Context &ctx = *c[0]->ctx;
-        nx::require(c[0]->ctx != nullptr && chain_index >= 1 && chain_index <= ctx.L, "chain_index out of range");
+        nx::require(chain_index >= 1 && chain_index <= ctx.L, "chain_index out of range");
         nx::Ciphertext out;
         nx::op_linear_combination(c, std::vector<double>(values, values + count),
                                   std::vector<double>(scales, scales + count), ctx.limbs_of(chain_index), out);

@@ -270,6 +270,10 @@ void op_multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, co
 void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> &a,
                                           const std::vector<const Ciphertext *> &b, const KSwitchKeys &rk,
                                           const std::vector<Ciphertext *> &out);
+// SEAL-4.1-bs add/sub/multiply_inplace_reduced_error (evaluator.cpp:420-594):
+// dst = a (op) b after aligning the levels and scales; Mul relinearises
+enum class ReducedOp { Add, Sub, Mul };
+void op_reduced_error(const Ciphertext &a, const Ciphertext &b, ReducedOp op, const KSwitchKeys *rk, Ciphertext &dst);
 void op_mod_switch_to_next(const Ciphertext &a, Ciphertext &dst);
 void op_mod_switch_to(const Ciphertext &a, size_t chain_index, Ciphertext &dst);
 void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &dst);

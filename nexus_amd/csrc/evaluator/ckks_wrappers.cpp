@@ -595,99 +595,56 @@ void Evaluator::add_const(const Ciphertext &ct, double v, Ciphertext &dest) cons
     check(nx_add_const(ct.get(), v, dest.get()));
 }
 void Evaluator::add_const_inplace(Ciphertext &ct, double v) const { check(nx_add_const(ct.get(), v, ct.get())); }
-void Evaluator::double_inplace(Ciphertext &ct) const { check(nx_add(ct.get(), ct.get(), ct.get())); }
+void Evaluator::double_inplace(Ciphertext &ct) const { check(nx_double(ct.get(), ct.get())); }
 
-// ---- reduced-error alignment (SEAL-4.1-bs evaluator.cpp:420-594): bring the
-// higher operand down with multiply_const(scale_adjust) + rescale + mod_switch
-// so that the scales agree exactly.
-namespace {
-enum class RedOp { Add, Sub, Mul };
-}
-
-static void reduced_error(const Evaluator &ev, const CKKSContext &c, Ciphertext &e1, const Ciphertext &e2, RedOp op)
-{
-    const size_t n1 = e1.coeff_modulus_size(), n2 = e2.coeff_modulus_size();
-    auto apply = [&](Ciphertext &dst, const Ciphertext &a, const Ciphertext &b) {
-        if (op == RedOp::Add)
-            ev.add(a, b, dst);
-        else if (op == RedOp::Sub)
-            ev.sub(a, b, dst);
-        else
-            ev.multiply(a, b, dst);
-    };
-    if (n1 == n2) {
-        e1.set_scale(e2.scale());
-        apply(e1, e1, e2);
-        return;
-    }
-    if (n1 < n2) {
-        const double qlast = (double)c.moduli()[n2 - 1];
-        const double adj = e1.scale() * qlast / (e2.scale() * e2.scale());
-        Ciphertext e2a;
-        ev.multiply_const(e2, adj, e2a);
-        e2a.set_scale(e1.scale() * qlast);
-        ev.rescale_to_next_inplace(e2a);
-        ev.mod_switch_to_inplace(e2a, e1.params_id());
-        e1.set_scale(e2a.scale());
-        apply(e1, e1, e2a);
-    } else {
-        const double qlast = (double)c.moduli()[n1 - 1];
-        const double adj = e2.scale() * qlast / (e1.scale() * e1.scale());
-        Ciphertext e1a;
-        ev.multiply_const(e1, adj, e1a);
-        e1a.set_scale(e2.scale() * qlast);
-        ev.rescale_to_next_inplace(e1a);
-        ev.mod_switch_to_inplace(e1a, e2.params_id());
-        e1a.set_scale(e2.scale());
-        apply(e1, e1a, e2);
-    }
-}
-
+// ---- reduced-error operations (SEAL-4.1-bs evaluator.cpp:420-594), through
+// the C ABI; the _inplace forms write their first operand
 void Evaluator::add_inplace_reduced_error(Ciphertext &a, const Ciphertext &b) const
 {
-    reduced_error(*this, *ctx_, a, b, RedOp::Add);
+    check(nx_add_reduced_error(a.get(), b.get(), a.get()));
 }
 void Evaluator::add_reduced_error(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
 {
-    if (&b == &dest) {
-        add_inplace_reduced_error(dest, a);
-    } else {
-        dest = a;
-        add_inplace_reduced_error(dest, b);
-    }
+    // cuda/src/ckks_evaluator.cuh:338-345: dest = a, then dest += b (with dest
+    // aliasing b, b's operand is the one adjusted)
+    Ciphertext out(*ctx_);
+    if (&b == &dest)
+        check(nx_add_reduced_error(b.get(), a.get(), out.get()));
+    else
+        check(nx_add_reduced_error(a.get(), b.get(), out.get()));
+    dest = std::move(out);
 }
 void Evaluator::sub_inplace_reduced_error(Ciphertext &a, const Ciphertext &b) const
 {
-    reduced_error(*this, *ctx_, a, b, RedOp::Sub);
+    check(nx_sub_reduced_error(a.get(), b.get(), a.get()));
 }
 void Evaluator::sub_reduced_error(const Ciphertext &a, const Ciphertext &b, Ciphertext &dest) const
 {
-    Ciphertext t = a;
-    sub_inplace_reduced_error(t, b);
-    dest = std::move(t);
+    Ciphertext out(*ctx_);
+    check(nx_sub_reduced_error(a.get(), b.get(), out.get()));
+    dest = std::move(out);
 }
 void Evaluator::multiply_inplace_reduced_error(Ciphertext &a, const Ciphertext &b, const RelinKeys &rk) const
 {
-    reduced_error(*this, *ctx_, a, b, RedOp::Mul);
-    relinearize_inplace(a, rk);
+    check(nx_multiply_reduced_error(a.get(), b.get(), rk.get(), a.get()));
 }
 void Evaluator::multiply_reduced_error(const Ciphertext &a, const Ciphertext &b, const RelinKeys &rk,
                                        Ciphertext &dest) const
 {
-    if (&b == &dest) {
-        multiply_inplace_reduced_error(dest, a, rk);
-    } else {
-        dest = a;
-        multiply_inplace_reduced_error(dest, b, rk);
-    }
+    Ciphertext out(*ctx_);
+    if (&b == &dest)
+        check(nx_multiply_reduced_error(b.get(), a.get(), rk.get(), out.get()));
+    else
+        check(nx_multiply_reduced_error(a.get(), b.get(), rk.get(), out.get()));
+    dest = std::move(out);
 }
 
 void Evaluator::multiply_vector_reduced_error(const Ciphertext &ct, const std::vector<double> &values,
                                               Ciphertext &dest) const
 {
-    Plaintext p(*ctx_);
-    check(nx_encode(ctx_->get(), values.data(), nullptr, values.size(), ct.params_id(), ct.scale(), p.get()));
-    multiply_plain(ct, p, dest);
+    Ciphertext out(*ctx_);
+    check(nx_multiply_vector_reduced_error(ct.get(), values.data(), nullptr, values.size(), out.get()));
+    dest = std::move(out);
 }
 
 void Evaluator::multiply_vector_reduced_error(const Ciphertext &ct, const std::vector<std::complex<double>> &values,
@@ -698,9 +655,9 @@ void Evaluator::multiply_vector_reduced_error(const Ciphertext &ct, const std::v
         re[i] = values[i].real();
         im[i] = values[i].imag();
     }
-    Plaintext p(*ctx_);
-    check(nx_encode(ctx_->get(), re.data(), im.data(), values.size(), ct.params_id(), ct.scale(), p.get()));
-    multiply_plain(ct, p, dest);
+    Ciphertext out(*ctx_);
+    check(nx_multiply_vector_reduced_error(ct.get(), re.data(), im.data(), values.size(), out.get()));
+    dest = std::move(out);
 }
 
 #undef OUT

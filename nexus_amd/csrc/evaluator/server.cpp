@@ -24,6 +24,7 @@ Server::Server(const CKKSContext &ctx, const SecretKey *sk, RelinKeys &rk, Galoi
     }
     if (bert_) {
         if (!boot) throw std::invalid_argument("Server: a BERT layer needs the bootstrapper parameters");
+        if (bert_->context()->get() != ctx.get()) throw std::invalid_argument("Server: BERT layer on another context");
         tokens_ = bert_->tokens_per_ct();
     }
     for (int i = 0; i < opt_.lanes; i++) workers_.emplace_back(&Server::worker, this, i + 1);
@@ -149,9 +150,24 @@ void Server::worker(int lane)
     try {
         run_lane(lane);
     } catch (const std::exception &e) {
-        // a lane that cannot start fails the requests it would have served
-        std::lock_guard<std::mutex> g(mu_);
-        lane_error_ = e.what();
+        // a lane that cannot start takes no requests; once no lane is left, every
+        // queued request fails with its error instead of waiting forever
+        std::vector<std::shared_ptr<Request>> failed;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            lane_error_ = e.what();
+            if (++dead_lanes_ == opt_.lanes) {
+                const double t = now_ms();
+                for (auto &r : queue_) {
+                    r->error = std::string("no lane left to serve the request: ") + lane_error_;
+                    r->t_done = t;
+                    r->done = true;
+                }
+                queue_.clear();
+            }
+        }
+        cv_done_.notify_all();
+        cv_work_.notify_all();
     }
 }
 

@@ -519,7 +519,8 @@ void poly_dot_plain_multi(const DeviceTables &t, uint64_t *const *outs, const ui
                           hipStream_t s)
 {
     const uint32_t n = 1u << t.logn;
-    if (rows == 0 || rows > (uint32_t)kMultiDotG || count > (uint32_t)kMultiDotK) abort();
+    if (rows == 0 || rows > (uint32_t)kMultiDotG || count > (uint32_t)kMultiDotK)
+        throw std::invalid_argument("poly_dot_plain_multi: rows or terms out of range");
     MultiDotArgs a;
     uint32_t terms = 0;
     for (uint32_t r = 0; r < (uint32_t)kMultiDotK; r++) a.ct[r] = r < count ? cts[r] : nullptr;
@@ -577,7 +578,7 @@ void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targe
     TargetPtrs list;
     for (uint32_t i = 0; i < (uint32_t)kNttSrcMax; i++)
         list.p[i] = (!targets && i < B) ? target_list[i] : nullptr;
-    if (!targets && B > (uint32_t)kNttSrcMax) abort();
+    if (!targets && B > (uint32_t)kNttSrcMax) throw std::invalid_argument("modup_copy_own: too many targets");
     ProfScope prof("modup_copy", s, 16.0 * n * B * l);
     hipLaunchKernelGGL(k_modup_copy_own, dim3((n + kBlock - 1) / kBlock, B * l), dim3(kBlock), 0, s, T, targets, list,
                        l, alpha, digits, nout, n);

@@ -646,7 +646,7 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
     }
     if (b.src_list) {
         // a source per poly: the batch must fit the argument array
-        if (b.polys > (uint32_t)kNttSrcMax) abort();
+        if (b.polys > (uint32_t)kNttSrcMax) throw std::invalid_argument("ntt: too many sources in one batch");
         a.nsrcs = b.polys;
         for (uint32_t i = 0; i < b.polys; i++) a.srcs[i] = b.src_list[i];
     }

@@ -435,7 +435,8 @@ void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stri
 void poly_lincomb(const DeviceTables &t, uint64_t *out, const uint64_t *const *x, const uint64_t *x_stride,
                   const uint64_t *const *w, uint32_t terms, uint32_t limbs, bool accumulate, hipStream_t s)
 {
-    if (terms == 0 || terms > (uint32_t)kLinMax || limbs > (uint32_t)kMaxPrimes) abort();
+    if (terms == 0 || terms > (uint32_t)kLinMax || limbs > (uint32_t)kMaxPrimes)
+        throw std::invalid_argument("poly_lincomb: terms or limbs out of range");
     LinArgs a;
     for (uint32_t k = 0; k < (uint32_t)kLinMax; k++) {
         a.x[k] = k < terms ? x[k] : nullptr;

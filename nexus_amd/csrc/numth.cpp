@@ -58,7 +58,9 @@ std::vector<uint64_t> coeff_modulus_create(size_t n, const std::vector<int> &bit
     if (n < 2 || (n & (n - 1))) throw std::invalid_argument("poly_modulus_degree is invalid");
     std::unordered_map<int, size_t> count;
     for (int b : bit_sizes) {
-        if (b < 2 || b > 61) throw std::invalid_argument("bit_sizes is invalid");
+        // SEAL_USER_MOD_BIT_COUNT_MAX = 60 (util/defines.h:40): the lazy 128-bit
+        // accumulators sum up to 128 products of two residues below 2^60
+        if (b < 2 || b > 60) throw std::invalid_argument("bit_sizes is invalid");
         count[b]++;
     }
     std::unordered_map<int, std::vector<uint64_t>> pool;

@@ -767,6 +767,65 @@ void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &d
     dst = std::move(r);
 }
 
+// ---------------------------------------------------------------- reduced-error alignment
+void op_reduced_error(const Ciphertext &a, const Ciphertext &b, ReducedOp op, const KSwitchKeys *rk, Ciphertext &dst)
+{
+    // SEAL-4.1-bs evaluator.cpp:420-594 (CUDA mirror cuda/src/ckks_evaluator.cu:465-589):
+    // the operand with more limbs is multiplied by the constant
+    // s_other q_last / s^2 (encoded at its own scale), given the scale
+    // s_other q_last, rescaled and dropped to the other's level, so both
+    // operands carry the same scale exactly; the scale formulas and their
+    // double-precision order are SEAL's.  Equal levels take b's scale.
+    valid(a);
+    valid(b);
+    require(a.ctx == b.ctx, "encrypted1 and encrypted2 parameter mismatch");
+    require(op != ReducedOp::Mul || rk != nullptr, "relin_keys is null");
+    Context &c = *a.ctx;
+    auto apply = [&](const Ciphertext &x, const Ciphertext &y, Ciphertext &out) {
+        if (op == ReducedOp::Add)
+            op_add(x, y, out);
+        else if (op == ReducedOp::Sub)
+            op_sub(x, y, out);
+        else
+            op_multiply(x, y, out);
+    };
+    Ciphertext res;
+    if (a.limbs == b.limbs) {
+        Ciphertext e1;
+        ct_copy(e1, a);
+        e1.scale = b.scale;
+        apply(e1, b, res);
+    } else if (a.limbs < b.limbs) {
+        const double qlast = (double)c.moduli[b.limbs - 1];
+        const double adjust = a.scale * qlast / (b.scale * b.scale);
+        Plaintext k;
+        encode_constant(c, adjust, b.limbs, b.scale, k);
+        Ciphertext e2;
+        op_multiply_plain(b, k, e2);
+        e2.scale = a.scale * qlast;
+        op_rescale_to_next(e2, e2);
+        op_mod_switch_to(e2, c.chain_index_of(a.limbs), e2);
+        Ciphertext e1;
+        ct_copy(e1, a);
+        e1.scale = e2.scale;
+        apply(e1, e2, res);
+    } else {
+        const double qlast = (double)c.moduli[a.limbs - 1];
+        const double adjust = b.scale * qlast / (a.scale * a.scale);
+        Plaintext k;
+        encode_constant(c, adjust, a.limbs, a.scale, k);
+        Ciphertext e1;
+        op_multiply_plain(a, k, e1);
+        e1.scale = b.scale * qlast;
+        op_rescale_to_next(e1, e1);
+        op_mod_switch_to(e1, c.chain_index_of(b.limbs), e1);
+        e1.scale = b.scale;
+        apply(e1, b, res);
+    }
+    if (op == ReducedOp::Mul) op_relinearize(res, *rk, res);
+    dst = std::move(res);
+}
+
 void op_to_ntt(const Ciphertext &a, Ciphertext &dst)
 {
     valid(a);
@@ -910,6 +969,9 @@ void op_linear_combination(const std::vector<const Ciphertext *> &xs, const std:
     }
     const double scale = a0.scale * scales[0];
     require(scale_within_bounds(c, scale, limbs), "scale out of bounds");
+    // each term is an add of x_k * constant at scale x_k.scale * scales[k]:
+    // SEAL's add requires it to equal the sum's scale
+    for (size_t k = 0; k < xs.size(); k++) require(are_close(xs[k]->scale * scales[k], scale), "scale mismatch");
     const size_t K = xs.size();
     std::vector<std::vector<uint64_t>> res(K, std::vector<uint64_t>(kMaxPrimes, 0));
     for (size_t k = 0; k < K; k++) constant_residues(c, values[k], limbs, scales[k], res[k].data());

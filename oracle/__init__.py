@@ -57,6 +57,9 @@ def lib():
                                           ctypes.POINTER(u64p)]
         L.or_switch_key.argtypes = [u64p, u64p, ctypes.c_size_t, ctypes.c_int, u64p, ctypes.c_size_t, u64p,
                                     ctypes.POINTER(u64p), ctypes.POINTER(u64p)]
+        L.or_rotate_sum.argtypes = [u64p, ctypes.POINTER(u64p), u32p, ctypes.POINTER(u64p), ctypes.c_size_t,
+                                    ctypes.c_size_t, ctypes.c_int, u64p, ctypes.c_size_t, ctypes.POINTER(u64p),
+                                    ctypes.POINTER(u64p)]
         L.or_tensor.argtypes = [u64p, u64p, ctypes.c_size_t, ctypes.c_int, u64p, u64p]
         L.or_dyadic_mul.argtypes = [u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, u64p, u64p]
         L.or_encoder_create.restype = ctypes.c_void_p
@@ -195,6 +198,30 @@ def switch_key(ct, target, key_moduli, key, logn):
     irps, keep2 = _table_ptrs(logn, kmods, 1)
     lib().or_switch_key(_p(ct), _p(target), l, logn, _p(kmods), len(kmods), _p(key), rps, irps)
     return ct
+
+
+def rotate_sum(cts, elts, keys, key_moduli, logn):
+    """sum_r rot_r(cts[r]) with one ModDown, each rotation hoisted (its digits
+    are sigma(lift(c1 mod q_j))); elts[r] = 0 adds cts[r] itself.
+    cts: list of (2, l, n); keys: list of (L, 2, L+1, n) (None for identity terms)."""
+    cs = [_arr(c) for c in cts]
+    l = cs[0].shape[1]
+    n = 1 << logn
+    kmods = _arr(key_moduli)
+    ks = [_arr(k) if k is not None else cs[0] for k in keys]
+    out = np.zeros((2, l, n), dtype=np.uint64)
+    rps, keep1 = _table_ptrs(logn, kmods, 0)
+    irps, keep2 = _table_ptrs(logn, kmods, 1)
+    cp = (u64p * len(cs))(*[_p(c) for c in cs])
+    kp = (u64p * len(ks))(*[_p(k) for k in ks])
+    el = np.ascontiguousarray(np.asarray(elts, dtype=np.uint32))
+    lib().or_rotate_sum(_p(out), cp, el.ctypes.data_as(u32p), kp, len(cs), l, logn, _p(kmods), len(kmods), rps, irps)
+    return out
+
+
+def rotate_hoisted(ct, elt, key, key_moduli, logn):
+    """one hoisted rotation: sigma(c0) + ModDown(sum_j sigma(lift(c1 mod q_j)) key[j])"""
+    return rotate_sum([ct], [elt], [key], key_moduli, logn)
 
 
 def tensor(a, b, moduli, logn):

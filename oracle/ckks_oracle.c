@@ -314,18 +314,22 @@ void or_rescale_poly_ntt(uint64_t *poly, size_t l, int logn, const uint64_t *mod
  * key     : [L digits][2 polys][L+1 limbs][n]  (KSwitchKeys, keygenerator.cpp:303-336)
  * rp/irp  : NTT tables for the L+1 key moduli
  */
-void or_switch_key(uint64_t *ct, const uint64_t *target, size_t l, int logn, const uint64_t *key_mod, size_t L1,
-                   const uint64_t *key, const uint64_t *const *rp, const uint64_t *const *irp)
+/* ModUp of `target` and its inner product with `key` (evaluator.cpp:2812-2910),
+ * accumulated into acc [2][l+1][n] (output primes q_0..q_{l-1}, p).  With a
+ * Galois table `perm` (NTT domain, out[i] = in[perm[i]]) every lifted digit is
+ * permuted before the product: the digits of a hoisted rotation are
+ * sigma(lift(target_j)) (Phantom hoisting_inplace, evaluate.cu:1727-1920). */
+static void ks_accumulate(uint64_t *acc, const uint64_t *target, size_t l, int logn, const uint64_t *key_mod,
+                          size_t L1, const uint64_t *key, const uint32_t *perm, const uint64_t *const *rp,
+                          const uint64_t *const *irp)
 {
     size_t n = (size_t)1 << logn, L = L1 - 1;
-    uint64_t p = key_mod[L];
     /* t_target = INTT(target)   (evaluator.cpp:2812-2818) */
     uint64_t *tt = (uint64_t *)malloc(l * n * sizeof(uint64_t));
     memcpy(tt, target, l * n * sizeof(uint64_t));
     for (size_t j = 0; j < l; j++) or_ntt_inverse(tt + j * n, logn, key_mod[j], irp[j]);
-    /* accumulators for output primes q_0..q_{l-1}, p : [2][l+1][n] */
-    uint64_t *acc = (uint64_t *)calloc(2 * (l + 1) * n, sizeof(uint64_t));
     uint64_t *tn = (uint64_t *)malloc(n * sizeof(uint64_t));
+    uint64_t *tp = (uint64_t *)malloc(n * sizeof(uint64_t));
     for (size_t OI = 0; OI <= l; OI++) {
         size_t ki = (OI == l) ? L : OI;
         uint64_t qI = key_mod[ki];
@@ -338,6 +342,10 @@ void or_switch_key(uint64_t *ct, const uint64_t *target, size_t l, int logn, con
                 or_ntt_forward(tn, logn, qI, rp[ki]);
                 op = tn;
             }
+            if (perm) {
+                for (size_t c = 0; c < n; c++) tp[c] = op[perm[c]];
+                op = tp;
+            }
             for (size_t k = 0; k < 2; k++) {
                 const uint64_t *kp = key + ((DJ * 2 + k) * L1 + ki) * n;
                 uint64_t *ap = acc + (k * (l + 1) + OI) * n;
@@ -345,8 +353,20 @@ void or_switch_key(uint64_t *ct, const uint64_t *target, size_t l, int logn, con
             }
         }
     }
-    /* ModDown with rounding (:2953-3018) */
+    free(tp);
+    free(tn);
+    free(tt);
+}
+
+/* ModDown with rounding (evaluator.cpp:2953-3018): ct[k][i] += (acc_i -
+ * [acc_p + (p-1)/2]_qi + [(p-1)/2]_qi) p^-1 for both polys. */
+static void ks_moddown(uint64_t *ct, uint64_t *acc, size_t l, int logn, const uint64_t *key_mod, size_t L1,
+                       const uint64_t *const *rp, const uint64_t *const *irp)
+{
+    size_t n = (size_t)1 << logn, L = L1 - 1;
+    uint64_t p = key_mod[L];
     uint64_t half = p >> 1;
+    uint64_t *tn = (uint64_t *)malloc(n * sizeof(uint64_t));
     for (size_t k = 0; k < 2; k++) {
         uint64_t *last = acc + (k * (l + 1) + l) * n;
         or_ntt_inverse(last, logn, p, irp[L]);
@@ -363,8 +383,52 @@ void or_switch_key(uint64_t *ct, const uint64_t *target, size_t l, int logn, con
         }
     }
     free(tn);
+}
+
+void or_switch_key(uint64_t *ct, const uint64_t *target, size_t l, int logn, const uint64_t *key_mod, size_t L1,
+                   const uint64_t *key, const uint64_t *const *rp, const uint64_t *const *irp)
+{
+    size_t n = (size_t)1 << logn;
+    uint64_t *acc = (uint64_t *)calloc(2 * (l + 1) * n, sizeof(uint64_t));
+    ks_accumulate(acc, target, l, logn, key_mod, L1, key, NULL, rp, irp);
+    ks_moddown(ct, acc, l, logn, key_mod, L1, rp, irp);
     free(acc);
-    free(tt);
+}
+
+/* Sum of rotations with one ModDown (hoisted key switching; Phantom
+ * hoisting_inplace, evaluate.cu:1727-1920, and the BSGS giant-step sums of
+ * Bootstrapper.cu:1855-1919): out = sum_r rot_r(cts[r]) where rot_r is the
+ * automorphism elts[r] (0 = identity: the input itself) applied as
+ *   sigma_r(c0_r) + ModDown(sum_r sum_j sigma_r(lift(c1_r mod q_j)) key_r[j]).
+ * cts: count ciphertexts of 2 x l x n (NTT), keys: one key per term (unused
+ * for identity terms); out: 2 x l x n, overwritten.  With count = 1 it is
+ * one hoisted rotation; the same input repeated is Phantom's hoisting sum. */
+void or_rotate_sum(uint64_t *out, const uint64_t *const *cts, const uint32_t *elts, const uint64_t *const *keys,
+                   size_t count, size_t l, int logn, const uint64_t *key_mod, size_t L1, const uint64_t *const *rp,
+                   const uint64_t *const *irp)
+{
+    size_t n = (size_t)1 << logn;
+    uint64_t *acc = (uint64_t *)calloc(2 * (l + 1) * n, sizeof(uint64_t));
+    uint32_t *perm = (uint32_t *)malloc(n * sizeof(uint32_t));
+    memset(out, 0, 2 * l * n * sizeof(uint64_t));
+    int switched = 0;
+    for (size_t r = 0; r < count; r++) {
+        const uint64_t *c0 = cts[r], *c1 = cts[r] + l * n;
+        if (elts[r] == 0) {
+            for (size_t i = 0; i < 2 * l; i++)
+                for (size_t c = 0; c < n; c++)
+                    out[i * n + c] = addmod(out[i * n + c], cts[r][i * n + c], key_mod[i % l]);
+            continue;
+        }
+        or_galois_table_ntt(logn, elts[r], perm);
+        for (size_t i = 0; i < l; i++)
+            for (size_t c = 0; c < n; c++) out[i * n + c] = addmod(out[i * n + c], c0[i * n + perm[c]], key_mod[i]);
+        ks_accumulate(acc, c1, l, logn, key_mod, L1, keys[r], perm, rp, irp);
+        switched = 1;
+    }
+    if (switched) ks_moddown(out, acc, l, logn, key_mod, L1, rp, irp);
+    free(perm);
+    free(acc);
 }
 
 /* ------------------------------------------------------------------ */

@@ -52,3 +52,19 @@ def test_error_codes_and_messages():
         nx.Context(12, [58])  # needs a data prime and a special prime
     assert e.value.code == -1
     assert "special prime" in str(e.value)
+
+
+def test_coeff_modulus_create_host():
+    """CoeffModulus::Create runs on the host: equal to the oracle's restatement
+    (pinned by SEAL's KATs); bit sizes above SEAL's 60-bit cap
+    (util/defines.h:40) are rejected, since the lazy 128-bit accumulators of the
+    inner products assume residues below 2^60."""
+    import nexus_amd as nx
+    import oracle as orc
+
+    for n, bits in ((1 << 13, [60, 40, 40, 60]), (1 << 16, [58] + [40] * 18 + [58]), (1 << 12, [30, 45, 51, 2 + 58])):
+        assert nx.coeff_modulus_create(n, bits) == orc.coeff_modulus_create(n, bits)
+    for bad in ([61, 40, 61], [40, 1], [14, 40]):  # 14 bits: no prime = 1 mod 2^14 below 2^14
+        with pytest.raises(nx.NexusError) as e:
+            nx.coeff_modulus_create(1 << 13, bad)
+        assert e.value.code in (-1, -2)

@@ -91,3 +91,38 @@ def test_matrix_mul_reference_shape(env):
         err = np.mean(np.abs(d / 2.0 - ref[:, col]))
         print("column %d: mean |err| %.3g" % (col, err))
         assert err < 1e-3
+
+
+def test_matrix_mul_config1():
+    """The stated ct-pt configuration: N = 2^13, {60, 40, 40, 60}, an encrypted
+    128x768 input ~U[-1, 1] times a 768x768 plaintext ~N(0, 0.02), seed 42.
+    Through the MatrixMul pipeline as (X W) row i = sum_j X[i, j] W[j, :]: X is
+    coefficient-packed (12 ciphertexts), expanded to 128 x 768 scalars and
+    accumulated against the 768 weight rows, 64 rows of X at a time."""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    _note("matmul config 1: 128x768 (encrypted) x 768x768 (plaintext), {60, 40, 40, 60}")
+    ctx = nx.Context(LOGN, [60, 40, 40, 60])
+    sk = nx.SecretKey(ctx, seed=42)
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, mm.reference_elts(ctx.n, LOGN), seed=44)
+    rng = np.random.default_rng(42)
+    X = rng.uniform(-1, 1, (128, 768))
+    W = rng.normal(0, 0.02, (768, 768))
+    ref = X @ W
+    w_pts = [nx.encode(ctx, W[j], 2.0 ** 40) for j in range(768)]
+    rows = 64  # 64 x 768 = 6 packed ciphertexts of 8192 coefficients
+    errs = []
+    for r0 in range(0, 128, rows):
+        flat = X[r0:r0 + rows].reshape(-1)
+        packs = [flat[i * ctx.n:(i + 1) * ctx.n] for i in range(len(flat) // ctx.n)]
+        batches = [mm.expand(mm.compress(ctx, sk, p, 1e10, seed=200 + r0 + i), ctx.n, gk) for i, p in enumerate(packs)]
+        res = mm.accumulate(w_pts, batches, rows)
+        del batches
+        for i in range(rows):
+            r = res[i]
+            r.scale = r.scale * (ctx.n // 2)
+            d = nx.decode(ctx, nx.decrypt(ctx, sk, nx.rescale_to_next(r)))[:768] / 2.0
+            errs.append(np.abs(d - ref[r0 + i]))
+    errs = np.array(errs)
+    print("config 1: max |err| %.3g, mean %.3g" % (errs.max(), errs.mean()))
+    assert errs.max() < 1e-4
