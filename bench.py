@@ -295,6 +295,58 @@ def bench_dot(nx, reps):
                           "achieved_gbs": (b / max(launches, 1)) / (ms / max(launches, 1) * 1e-3) / 1e9}}
 
 
+KS_CLASSES = ("ks_modup_conv", "ks_inner_rot_sum", "ks_inner_rot", "ks_moddown_conv", "modup_copy", "ks_inner")
+
+
+def bench_ks(nx, reps):
+    """The key-switching kernel classes at the BERT layer's top compute level:
+    16 ciphertexts of 2 x 20 limbs at N = 2^16 on the BERT chain (8 special
+    primes: 3 key-switching digits at 20 limbs) -- one sum of 16 rotations with
+    one ModDown (batched ModUp conversion, ks_inner_rot_sum, ModDown
+    conversion), 16 hoisted rotations of one ciphertext (ks_inner_rot) and 16
+    fused multiply-relinearise-rescale products; next to a device-to-device
+    copy of one ciphertext (20.9 MB; the copy calibrates HBM counters)."""
+    ctx = nx.Context(16, BERT_CHAIN, hamming_weight=192, special_primes=BERT_SPECIAL)
+    sk = nx.SecretKey(ctx, seed=61)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=62)
+    steps = list(range(1, 17))
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps], seed=63)
+    rng = np.random.default_rng(64)
+    limbs = 20
+    ci = ctx.L + 1 - limbs
+    cts = [nx.Ciphertext.from_array(
+        ctx, np.stack([np.stack([rng.integers(0, q, ctx.n, dtype=np.uint64) for q in ctx.moduli[:limbs]])
+                       for _ in range(2)]), ci, BERT_SCALE) for _ in range(16)]
+
+    def run():
+        a = nx.rotate_sum(cts, steps, gk)
+        b = nx.rotate_hoisted(cts[0], steps, gk)
+        c = nx.multiply_relin_rescale_many(cts, cts[::-1], rk, fused=True)
+        return a, b, c
+
+    run()  # first launches load the code objects: untimed
+    ctx.synchronize()
+    nx.profile_select(",".join(KS_CLASSES))
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        out = run()
+        cp = cts[0].copy()
+    ctx.synchronize()
+    wall = (time.perf_counter() - t0) / reps * 1e3
+    res = {"ms_per_rep": wall, "limbs": limbs, "digits": ctx.digits, "copy_bytes": 2 * limbs * ctx.n * 8}
+    for cls in KS_CLASSES:
+        launches, ms, b = nx.profile_read(cls)
+        if launches:
+            res[cls] = {"launches": launches, "avg_us": ms / launches * 1e3,
+                        "algorithmic_bytes_per_launch": b / launches,
+                        "achieved_gbs": (b / launches) / (ms / launches * 1e-3) / 1e9}
+    nx.profile_select("")
+    res["valu_int64_products_per_s"] = nx.valu_rate(0)
+    res["valu_fp64_fma_per_s"] = nx.valu_rate(1)
+    del out, cp
+    return res
+
+
 def bert_setup(nx):
     """context, keys, bootstrapper and BERT-base layer at N = 2^16"""
     from nexus_amd import bert_rotation_steps, boot_plan_steps
@@ -375,7 +427,7 @@ def main():
                     help="lanes of the served BERT measurement (one BERT call is in flight at a time, and each "
                          "lane caches its own call working set)")
     ap.add_argument("--only", choices=["bert", "bert_served", "dot", "bootstrap_dense", "bootstrap_sparse", "matmul",
-                                       "ntt"],
+                                       "ntt", "ks", "valu"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -409,6 +461,10 @@ def main():
             r = bench_ntt(nx, args.steps)
         elif args.only == "dot":
             r = bench_dot(nx, args.steps)
+        elif args.only == "ks":
+            r = bench_ks(nx, args.steps)
+        elif args.only == "valu":
+            r = {"int64_products_per_s": nx.valu_rate(0), "fp64_fma_per_s": nx.valu_rate(1)}
         else:
             r = bench_matmul(nx, args.steps)
         print(json.dumps({"workload": args.only, **r}), flush=True)

@@ -25,6 +25,13 @@ int nx_profile_select(const char *classes);
 int nx_profile_read(const char *kernel_class, size_t *launches, double *total_ms, double *bytes);
 /* Comma-separated list of the classes with statistics. */
 int nx_profile_classes(char *buf, size_t len);
+/* Measured VALU issue rate of this device, the denominator of the compute
+ * roofline of the modular kernels: kind 0 = 64 x 64 -> 128-bit integer
+ * products per second (the low and high halves of the Shoup / Barrett
+ * multiplies), kind 1 = double-precision FMAs per second (the exact-product
+ * path of primes below 2^51).  Eight independent chains per thread, 8 waves
+ * per SIMD, no memory traffic. */
+int nx_profile_valu_rate(int kind, double *ops_per_s);
 
 #ifdef __cplusplus
 }

@@ -1170,3 +1170,14 @@ def profile_classes():
     buf = ctypes.create_string_buffer(4096)
     _check(_lib.nx_profile_classes(buf, 4096))
     return [c for c in buf.value.decode().split(",") if c]
+
+
+_sig("nx_profile_valu_rate", [ctypes.c_int, ctypes.POINTER(ctypes.c_double)])
+
+
+def valu_rate(kind):
+    """measured VALU rate of the device: kind 0 = 64x64->128-bit products/s,
+    kind 1 = double-precision FMAs/s"""
+    v = ctypes.c_double()
+    _check(_lib.nx_profile_valu_rate(kind, ctypes.byref(v)))
+    return v.value

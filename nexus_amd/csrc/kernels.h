@@ -320,4 +320,8 @@ void encode_fft(const DeviceTables &t, uint64_t *out, double2 *work, const doubl
                 uint32_t count, const uint32_t *index_map, const double2 *roots, double scalar, uint32_t limbs,
                 uint64_t *blockmax, hipStream_t s);
 
+// VALU issue-rate probe on the current device: kind 0 = 64 x 64 -> 128-bit
+// products per second, kind 1 = double-precision FMAs per second
+double probe_valu_rate(int kind, hipStream_t s);
+
 }  // namespace nx

@@ -9,6 +9,7 @@
 #include "../../include/nexus_ckks.h"
 #include "../../include/nexus_prof.h"
 #include "context.h"
+#include "handles.h"
 
 namespace nx {
 
@@ -164,6 +165,14 @@ int nx_profile_classes(char *buf, size_t len)
         nx::set_last_error(e.what());
         return e.code;
     }
+}
+
+int nx_profile_valu_rate(int kind, double *ops_per_s)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(ops_per_s, "ops_per_s");
+        *ops_per_s = nx::probe_valu_rate(kind, nullptr);
+    });
 }
 
 }  // extern "C"

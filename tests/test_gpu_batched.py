@@ -102,7 +102,8 @@ def test_fused_relin_rescale_within_one(env, count):
 def test_linear_combination_matches(env):
     """sum_k x_k * encode_constant(v_k, s_k) at a lower level with inputs at
     three levels (6 terms: two launches), negative and tiny constants: word for
-    word mod_switch + multiply_plain + add per term, at the first term's scale"""
+    word mod_switch + multiply_plain + add per term; every term's scale must
+    equal the sum's, as SEAL's add requires"""
     ctx, sk, rk = env
     _note("linear combination logn=%d" % ctx.log_n)
     L = ctx.L
@@ -111,19 +112,25 @@ def test_linear_combination_matches(env):
     cts = [nx.mod_switch_to(c, 1 + (i % target)) for i, c in enumerate(cts)]
     # products stay below the target level's modulus (one 51-bit prime at n16)
     values = [0.75, -1.25, 3e-3, -0.5, 1.0, 2.0 ** -6]
-    scales = [2.0 ** 8, 2.0 ** 8 * 1.5, 2.0 ** 7, 2.0 ** 9, 2.0 ** 8, 2.0 ** 8]
+    scales = [2.0 ** 8] * 6
     got = nx.linear_combination(cts, values, scales, target)
     want = None
     for c, v, sc in zip(cts, values, scales):
         t = nx.multiply_plain(nx.mod_switch_to(c, target), nx.encode_constant(ctx, v, sc, target))
-        if want is None:
-            want = t
-        else:
-            t.scale = want.scale
-            want = nx.add(want, t)
+        want = t if want is None else nx.add(want, t)
     assert got.chain_index == target
     assert np.array_equal(got.to_array(), want.to_array())
     assert got.info()["scale"] == want.info()["scale"]
+    # a term whose scale differs from the sum's is rejected, as by add
+    with pytest.raises(nx.NexusError, match="scale mismatch"):
+        nx.linear_combination(cts[:2], values[:2], [2.0 ** 8, 2.0 ** 8 * 1.5], target)
+    # a term at another scale brought to the sum's scale by its constant's scale
+    c2 = cts[1].copy()
+    c2.scale = cts[1].scale * 2.0
+    got = nx.linear_combination([cts[0], c2], values[:2], [2.0 ** 8, 2.0 ** 7], target)
+    t0 = nx.multiply_plain(nx.mod_switch_to(cts[0], target), nx.encode_constant(ctx, values[0], 2.0 ** 8, target))
+    t1 = nx.multiply_plain(nx.mod_switch_to(c2, target), nx.encode_constant(ctx, values[1], 2.0 ** 7, target))
+    assert np.array_equal(got.to_array(), nx.add(t0, t1).to_array())
 
 
 @pytest.mark.parametrize("count", [1, 20])
