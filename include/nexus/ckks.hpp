@@ -145,11 +145,14 @@ public:
     nx_periodic_plain *get() const { return h_; }
     size_t count() const { return count_; }
     void expand(size_t index, Plaintext &out) const;
+    // the full plaintext of entry `index`, expanded once and kept with this set
+    const Plaintext &expanded(size_t index) const;
 
 private:
     const CKKSContext *ctx_;
     nx_periodic_plain *h_ = nullptr;
     size_t count_ = 0;
+    mutable std::map<size_t, Plaintext> expanded_;
 };
 
 class Encoder {
@@ -204,6 +207,18 @@ public:
     Evaluator() = default;
     explicit Evaluator(const CKKSContext *ctx) : ctx_(ctx) {}
     const CKKSContext *context() const { return ctx_; }
+
+    // Per-op mode: every multi-ciphertext entry point below (hoisted
+    // rotations, sums of rotations, multiply-accumulates, linear combinations,
+    // block mixes, batched products and rescales) runs as the sequence of
+    // single-ciphertext calls the reference's evaluators issue to their
+    // library (rotate_vector, multiply_plain, add, multiply, relinearize,
+    // rescale_to_next per term), and the lockstep programs built on them
+    // (gelu_many, bootstrap_many) run input by input.  Results decrypt alike;
+    // this is the plain per-call drop-in the batched engine is measured
+    // against.
+    void set_per_op(bool on) { per_op_ = on; }
+    bool per_op() const { return per_op_; }
 
     void mod_switch_to_next_inplace(Ciphertext &ct) const;
     void mod_switch_to_inplace(Ciphertext &ct, size_t chain_index) const;
@@ -294,6 +309,7 @@ public:
 
 private:
     const CKKSContext *ctx_ = nullptr;
+    bool per_op_ = false;
 };
 
 // NEXUS's helper layer (src/ckks_evaluator.{h,cpp}): polynomial programs over

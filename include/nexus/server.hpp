@@ -49,6 +49,7 @@ public:
         int lanes = 2;               // concurrent requests (HIP streams)
         size_t memory_budget = 0;    // bytes of in-flight request footprint; 0 = no limit
         double footprint_factor = 6; // footprint = factor x input ciphertext bytes
+        bool per_op = false;         // the lanes' evaluators run in per-op mode (Evaluator::set_per_op)
         bool pack = true;            // continuous batching of BERT requests
         int bert_concurrency = 1;    // BERT layer calls in flight at once (one call fills the GPU)
         size_t bert_footprint = size_t(8) << 30;  // working set of one BERT layer call

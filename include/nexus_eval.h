@@ -22,6 +22,18 @@ typedef struct nx_evaluator nx_evaluator;
 int nx_evaluator_create(nx_context *ctx, nx_secret_key *sk, nx_kswitch_keys *relin_keys,
                         nx_kswitch_keys *galois_keys, double scale, nx_evaluator **out);
 int nx_evaluator_destroy(nx_evaluator *ev);
+/* Per-op mode (on = 1): the programs over this evaluator -- the evaluators,
+ * the bootstrapper, the BERT layer and the server lanes created while it is on
+ * -- issue only single-ciphertext operations, one library call per term, as
+ * the reference's evaluators call Phantom (e.g. cuda/src/gelu.cu:8-117,
+ * cuda/src/bootstrapping/Bootstrapper.cu:1855-1919: rotate_vector,
+ * multiply_plain, add, multiply, relinearize, rescale_to_next): no hoisting,
+ * no sums of rotations with one ModDown, no batched products, multiply-
+ * accumulates, linear combinations or block mixes, no lockstep GELU or
+ * EvalMod.  The results decrypt like the default (batched) mode's.  No
+ * counterpart in the reference: it is the plain per-call drop-in the batched
+ * engine is measured against. */
+int nx_evaluator_set_per_op(nx_evaluator *ev, int on);
 
 /* CKKSEvaluator::sgn_eval (src/ckks_evaluator.cpp:181-212) */
 int nx_eval_sgn(nx_evaluator *ev, const nx_ciphertext *x, int d_g, int d_f, double sgn_factor, nx_ciphertext *out);

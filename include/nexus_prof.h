@@ -25,6 +25,13 @@ int nx_profile_select(const char *classes);
 int nx_profile_read(const char *kernel_class, size_t *launches, double *total_ms, double *bytes);
 /* Comma-separated list of the classes with statistics. */
 int nx_profile_classes(char *buf, size_t len);
+/* Operation tally: start = 1 clears and starts counting the engine's
+ * operations by kind and level (key switches "ks", "rescale", ct x ct "tensor",
+ * ct x pt "mul_plain", elementwise "add", plaintext "encode"), start = 0 stops
+ * and writes {"kind": {"limbs": count, ...}, ...} to buf (*needed: its size
+ * with the terminator).  The CPU baseline composes its per-operation timings
+ * with this mix. */
+int nx_op_tally(int start, char *buf, size_t len, size_t *needed);
 /* Measured VALU issue rate of this device, the denominator of the compute
  * roofline of the modular kernels: kind 0 = 64 x 64 -> 128-bit integer
  * products per second (the low and high halves of the Shoup / Barrett

@@ -736,6 +736,7 @@ def mod_raise(a, dest=None):
 # ---------------------------------------------------------------- evaluator programs (include/nexus_eval.h)
 _sig("nx_evaluator_create", [vp, vp, vp, vp, ctypes.c_double, ctypes.POINTER(vp)])
 _sig("nx_evaluator_destroy", [vp])
+_sig("nx_evaluator_set_per_op", [vp, ctypes.c_int])
 _sig("nx_eval_sgn", [vp, vp, ctypes.c_int, ctypes.c_int, ctypes.c_double, vp])
 _sig("nx_eval_invert_sqrt", [vp, vp, ctypes.c_int, ctypes.c_int, vp])
 _sig("nx_eval_exp", [vp, vp, vp])
@@ -760,6 +761,11 @@ class CKKSEvaluator:
 
     def __del__(self, _fn=_lib.nx_evaluator_destroy, _rel=_release):
         _rel(self, _fn)
+
+    def set_per_op(self, on=True):
+        """per-op mode: the programs over this evaluator issue single-ciphertext
+        operations only (the plain per-call drop-in); see include/nexus_eval.h"""
+        _check(_lib.nx_evaluator_set_per_op(self.h, 1 if on else 0))
 
     def _run(self, fn, *args):
         out = Ciphertext(self.ctx)
@@ -1181,3 +1187,22 @@ def valu_rate(kind):
     v = ctypes.c_double()
     _check(_lib.nx_profile_valu_rate(kind, ctypes.byref(v)))
     return v.value
+
+
+_sig("nx_op_tally", [ctypes.c_int, ctypes.c_char_p, sz, ctypes.POINTER(sz)])
+
+
+def op_tally_start():
+    """clear and start the operation tally (include/nexus_prof.h)"""
+    _check(_lib.nx_op_tally(1, None, 0, None))
+
+
+def op_tally_stop():
+    """stop the tally: {kind: {limbs: count}}"""
+    import json
+    need = sz()
+    _check(_lib.nx_op_tally(0, None, 0, ctypes.byref(need)))
+    # the first call stopped the tally; read the kept counts with a buffer
+    buf = ctypes.create_string_buffer(max(need.value, 2))
+    _check(_lib.nx_op_tally(0, buf, len(buf), ctypes.byref(need)))
+    return {k: {int(l): c for l, c in v.items()} for k, v in json.loads(buf.value.decode()).items()}

@@ -12,6 +12,7 @@
 
 #include "context.h"
 #include "numth.h"
+#include "prof.h"
 
 namespace nx {
 
@@ -159,6 +160,7 @@ void encode_vector(Context &c, const double *re, const double *im, size_t count,
     const size_t n = c.n, slots = n / 2;
     require(count <= slots, "values_size is too large");
     require(limbs >= 1 && limbs <= c.L, "parms_id is not valid for encryption parameters");
+    tally("encode", limbs);
     const int total_bits = c.total_bits(limbs);
     require(!(scale <= 0 || ((int)std::log2(scale) + 1 >= total_bits)), "scale out of bounds");
     const uint32_t blocks = encode_blocks(c.logn);

@@ -853,6 +853,16 @@ void Bootstrapper::bootstrap_many(const std::vector<const Ciphertext *> &in, std
 {
     if (in.size() != factors.size()) throw std::invalid_argument("Bootstrapper: one factor per input");
     const size_t B = in.size();
+    if (ck_.evaluator.per_op()) {
+        // per-op mode: one bootstrap per input
+        std::vector<Ciphertext> res(B);
+        for (size_t i = 0; i < B; i++) {
+            if (!in[i] || !in[i]->get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
+            bootstrap(*in[i], res[i], factors[i]);
+        }
+        out = std::move(res);
+        return;
+    }
     std::vector<double> input_scale(B);
     std::vector<Ciphertext> u_re(B), u_im(B);
     for (size_t i = 0; i < B; i++) {

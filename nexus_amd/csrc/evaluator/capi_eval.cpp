@@ -29,6 +29,14 @@ int nx_evaluator_destroy(nx_evaluator *ev)
     return run([&] { delete ev; });
 }
 
+int nx_evaluator_set_per_op(nx_evaluator *ev, int on)
+{
+    return run([&] {
+        need(ev);
+        ev->ckks->evaluator.set_per_op(on != 0);
+    });
+}
+
 #define NX_IN(x) nexus::Ciphertext::copy_of(*ev->ctx, x)
 
 int nx_eval_sgn(nx_evaluator *ev, const nx_ciphertext *x, int d_g, int d_f, double sgn_factor, nx_ciphertext *out)

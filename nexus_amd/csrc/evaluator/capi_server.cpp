@@ -27,6 +27,7 @@ int nx_server_create(nx_evaluator *ev, int lanes, size_t memory_budget, const nx
         nexus::Server::Options o;
         o.lanes = lanes;
         o.memory_budget = memory_budget;
+        o.per_op = ev->ckks->evaluator.per_op();
         auto s = std::make_unique<nx_server>();
         s->ev = ev;
         s->impl = std::make_unique<nexus::Server>(*ev->ctx, ev->sk.get(), *ev->rk, *ev->gk, ev->ckks->scale, o,
@@ -48,6 +49,7 @@ int nx_server_create_bert(nx_evaluator *ev, int lanes, size_t memory_budget, con
         o.lanes = lanes;
         o.memory_budget = memory_budget;
         o.pack = pack != 0;
+        o.per_op = ev->ckks->evaluator.per_op();
         if (bert_footprint) o.bert_footprint = bert_footprint;
         auto s = std::make_unique<nx_server>();
         s->ev = ev;

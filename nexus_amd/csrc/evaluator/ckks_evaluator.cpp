@@ -225,6 +225,13 @@ void GELUEvaluator::gelu_many(const std::vector<const Ciphertext *> &xin, std::v
     const Evaluator &ev = ck.evaluator;
     const size_t B = xin.size();
     if (!B) throw std::invalid_argument("gelu_many: no inputs");
+    if (ev.per_op()) {
+        // per-op mode: the reference's GELU input by input
+        std::vector<Ciphertext> out(B);
+        for (size_t i = 0; i < B; i++) gelu(*xin[i], out[i]);
+        res = std::move(out);
+        return;
+    }
     for (const Ciphertext *c : xin)
         if (c->params_id() != xin[0]->params_id() || c->scale() != xin[0]->scale())
             throw std::invalid_argument("gelu_many: inputs must share the level and scale");

@@ -407,6 +407,17 @@ void Evaluator::apply_galois_inplace(Ciphertext &ct, uint32_t elt, const GaloisK
 void Evaluator::rotate_vector_hoisted(const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &k,
                                       std::vector<Ciphertext> &dest) const
 {
+    if (per_op_) {
+        std::vector<Ciphertext> out(steps.size());
+        for (size_t i = 0; i < steps.size(); i++) {
+            if (steps[i] == 0)
+                out[i] = ct;
+            else
+                rotate_vector(ct, steps[i], k, out[i]);
+        }
+        dest = std::move(out);
+        return;
+    }
     std::vector<Ciphertext> out(steps.size());
     std::vector<nx_ciphertext *> h(steps.size());
     for (size_t i = 0; i < steps.size(); i++) {
@@ -419,6 +430,17 @@ void Evaluator::rotate_vector_hoisted(const Ciphertext &ct, const std::vector<in
 void Evaluator::multiply_plain_accumulate(const std::vector<const Ciphertext *> &cts,
                                           const std::vector<const Plaintext *> &pts, Ciphertext &dest) const
 {
+    if (per_op_) {
+        if (cts.empty() || cts.size() != pts.size())
+            throw std::invalid_argument("multiply_plain_accumulate: one plaintext per ciphertext");
+        Ciphertext acc, t;
+        for (size_t r = 0; r < cts.size(); r++) {
+            multiply_plain(*cts[r], *pts[r], r ? t : acc);
+            if (r) add_inplace(acc, t);
+        }
+        dest = std::move(acc);
+        return;
+    }
     std::vector<const nx_ciphertext *> c;
     std::vector<const nx_plaintext *> p;
     for (const Ciphertext *x : cts) c.push_back(x->get());
@@ -433,6 +455,24 @@ void Evaluator::linear_combination(const std::vector<const Ciphertext *> &cts, c
 {
     if (cts.size() != values.size() || cts.size() != scales.size())
         throw std::invalid_argument("linear_combination: term counts differ");
+    if (per_op_) {
+        // mod_switch_to, multiply_plain by the encoded constant, add: per term
+        if (cts.empty()) throw std::invalid_argument("linear_combination: no terms");
+        Ciphertext acc;
+        for (size_t k = 0; k < cts.size(); k++) {
+            Ciphertext t;
+            mod_switch_to(*cts[k], chain_index, t);
+            Plaintext p(*ctx_);
+            check(nx_encode_constant(ctx_->get(), values[k], chain_index, scales[k], p.get()));
+            multiply_plain_inplace(t, p);
+            if (k)
+                add_inplace(acc, t);
+            else
+                acc = std::move(t);
+        }
+        dest = std::move(acc);
+        return;
+    }
     std::vector<const nx_ciphertext *> c;
     for (const Ciphertext *x : cts) c.push_back(x->get());
     Ciphertext res(*ctx_);
@@ -447,6 +487,23 @@ void Evaluator::multiply_plain_accumulate_multi(const std::vector<const Cipherte
     if (cts.empty() || pts.size() % cts.size())
         throw std::invalid_argument("multiply_plain_accumulate_multi: plaintexts must be rows x ciphertexts");
     const size_t rows = pts.size() / cts.size();
+    if (per_op_) {
+        const size_t K = cts.size();
+        std::vector<Ciphertext> res(rows);
+        for (size_t g = 0; g < rows; g++) {
+            std::vector<const Ciphertext *> c;
+            std::vector<const Plaintext *> p;
+            for (size_t r = 0; r < K; r++)
+                if (pts[g * K + r]) {
+                    c.push_back(cts[r]);
+                    p.push_back(pts[g * K + r]);
+                }
+            if (c.empty()) throw std::invalid_argument("multiply_plain_accumulate_multi: every row needs a term");
+            multiply_plain_accumulate(c, p, res[g]);
+        }
+        dest = std::move(res);
+        return;
+    }
     std::vector<const nx_ciphertext *> c;
     std::vector<const nx_plaintext *> p;
     for (const Ciphertext *x : cts) c.push_back(x->get());
@@ -463,6 +520,15 @@ void Evaluator::multiply_plain_accumulate_multi(const std::vector<const Cipherte
 }
 void Evaluator::rescale_to_next_many(const std::vector<const Ciphertext *> &cts, std::vector<Ciphertext> &dest) const
 {
+    if (per_op_) {
+        std::vector<Ciphertext> res(cts.size());
+        for (size_t i = 0; i < cts.size(); i++) {
+            res[i] = *cts[i];
+            rescale_to_next_inplace(res[i]);
+        }
+        dest = std::move(res);
+        return;
+    }
     std::vector<const nx_ciphertext *> in;
     for (const Ciphertext *x : cts) in.push_back(x->get());
     std::vector<Ciphertext> res;
@@ -481,6 +547,18 @@ void Evaluator::multiply_relin_rescale_many(const std::vector<const Ciphertext *
                                             std::vector<Ciphertext> &dest, bool fused) const
 {
     if (a.size() != b.size()) throw std::invalid_argument("multiply_relin_rescale_many: operand counts differ");
+    if (per_op_) {
+        // multiply, relinearize, rescale_to_next per pair (the fused single
+        // rounding is a batched form; per op it is the unfused sequence)
+        std::vector<Ciphertext> res(a.size());
+        for (size_t i = 0; i < a.size(); i++) {
+            multiply(*a[i], *b[i], res[i]);
+            relinearize_inplace(res[i], rk);
+            rescale_to_next_inplace(res[i]);
+        }
+        dest = std::move(res);
+        return;
+    }
     std::vector<const nx_ciphertext *> x, y;
     for (size_t i = 0; i < a.size(); i++) {
         x.push_back(a[i]->get());
@@ -506,6 +584,28 @@ void Evaluator::multiply_periodic_mix(const std::vector<const Ciphertext *> &cts
     const size_t K = cts.size();
     if (!K || index.size() % K) throw std::invalid_argument("multiply_periodic_mix: index must be M x K");
     const size_t M = index.size() / K;
+    if (per_op_) {
+        // the masks as full plaintexts, one multiply_plain and add per term
+        std::vector<Ciphertext> res(M);
+        for (size_t m = 0; m < M; m++) {
+            bool any = false;
+            Ciphertext t;
+            for (size_t k = 0; k < K; k++) {
+                const int32_t e = index[m * K + k];
+                if (e < 0) continue;
+                multiply_plain(*cts[k], w.expanded((size_t)e), any ? t : res[m]);
+                if (any) add_inplace(res[m], t);
+                any = true;
+            }
+            if (!any) {
+                // an output without terms is zero at the product's scale
+                multiply_plain(*cts[0], w.expanded(0), res[m]);
+                sub_inplace(res[m], res[m]);
+            }
+        }
+        out = std::move(res);
+        return;
+    }
     std::vector<const nx_ciphertext *> c;
     for (const Ciphertext *x : cts) c.push_back(x->get());
     std::vector<Ciphertext> res;
@@ -537,14 +637,43 @@ void PeriodicPlaintexts::expand(size_t index, Plaintext &out) const
     out = std::move(r);
 }
 
+const Plaintext &PeriodicPlaintexts::expanded(size_t index) const
+{
+    auto it = expanded_.find(index);
+    if (it != expanded_.end()) return it->second;
+    Plaintext r;
+    expand(index, r);
+    return expanded_.emplace(index, std::move(r)).first->second;
+}
+
 void Evaluator::hoisting_inplace(Ciphertext &ct, const GaloisKeys &k, const std::vector<int> &steps) const
 {
+    if (per_op_) {
+        std::vector<const Ciphertext *> c(steps.size(), &ct);
+        Ciphertext sum;
+        rotate_sum(c, steps, k, sum);
+        ct = std::move(sum);
+        return;
+    }
     check(nx_hoisting(ct.get(), steps.data(), steps.size(), k.get(), ct.get()));
 }
 void Evaluator::rotate_sum(const std::vector<const Ciphertext *> &cts, const std::vector<int> &steps,
                            const GaloisKeys &k, Ciphertext &dest) const
 {
     if (cts.empty() || cts.size() != steps.size()) throw std::invalid_argument("rotate_sum: inputs and steps must match");
+    if (per_op_) {
+        Ciphertext acc, t;
+        for (size_t i = 0; i < cts.size(); i++) {
+            Ciphertext &dst = i ? t : acc;
+            if (steps[i] == 0)
+                dst = *cts[i];
+            else
+                rotate_vector(*cts[i], steps[i], k, dst);
+            if (i) add_inplace(acc, t);
+        }
+        dest = std::move(acc);
+        return;
+    }
     std::vector<const nx_ciphertext *> h;
     for (const Ciphertext *c : cts) h.push_back(c->get());
     Ciphertext out(*ctx_);

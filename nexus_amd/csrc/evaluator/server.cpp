@@ -176,6 +176,7 @@ void Server::run_lane(int lane)
     // this thread's work runs on its own lane (stream + memory cache)
     check(nx_context_bind_lane(ctx_.get(), (size_t)lane));
     CKKSEvaluator ck(&ctx_, sk_, &rk_, &gk_, scale_);
+    ck.evaluator.set_per_op(opt_.per_op);
     std::unique_ptr<Bootstrapper> boot;
     if (have_boot_) boot.reset(new Bootstrapper(ck, boot_));
     std::unique_ptr<BertLayer> bert;

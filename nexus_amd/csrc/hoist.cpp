@@ -9,6 +9,7 @@
 // results decrypt to the same values, and the residues differ from the
 // one-at-a-time rotation by key-switching noise.
 #include "context.h"
+#include "prof.h"
 
 namespace nx {
 
@@ -51,6 +52,7 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
     Context &c = *a.ctx;
     const size_t n = c.n, l = a.limbs, nout = l + c.alpha, pw = l * n;
     Prepared p = prepare(c, steps, gk);
+    tally("ks_hoisted", l, (double)p.keys.size());
     std::vector<Ciphertext> out(steps.size());
     for (size_t k = 0; k < steps.size(); k++)
         if (steps[k] == 0) ct_copy(out[k], a);
@@ -97,6 +99,7 @@ void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwit
     Context &c = *a.ctx;
     const size_t n = c.n, l = a.limbs, nout = l + c.alpha, pw = l * n;
     Prepared p = prepare(c, steps, gk);
+    tally("ks_hoisted", l, (double)p.keys.size());
     Ciphertext out;
     out.ctx = &c;
     out.size = 2;
@@ -171,6 +174,7 @@ void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<
             continue;
         }
         Prepared p = prepare(c, std::vector<int>{steps[i]}, gk);
+        tally("ks_hoisted", l);
         perms.push_back(p.perms[0]);
         rperms.push_back(p.perms[0]);
         keys.push_back(p.keys[0]);

@@ -6,6 +6,7 @@
 #include <cmath>
 
 #include "context.h"
+#include "prof.h"
 
 namespace nx {
 
@@ -85,6 +86,7 @@ static void add_sub(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst, b
     valid(a);
     valid(b);
     same_params(a, b);
+    tally("add", a.limbs);
     require(are_close(a.scale, b.scale), "scale mismatch");
     Context &c = *a.ctx;
     const size_t pw = poly_words(c, a.limbs);
@@ -119,6 +121,7 @@ void op_sub(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst) { add_sub
 void op_negate(const Ciphertext &a, Ciphertext &dst)
 {
     valid(a);
+    tally("add", a.limbs);
     Context &c = *a.ctx;
     DevBuf buf(&c, a.size * poly_words(c, a.limbs));
     poly_negate(c.tables, buf.get(), a.data.get(), (uint32_t)a.size, (uint32_t)a.limbs, c.stream);
@@ -142,6 +145,7 @@ static void plain_check(const Ciphertext &a, const Plaintext &p)
 static void add_sub_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst, bool sub)
 {
     plain_check(a, p);
+    tally("add", a.limbs);
     require(are_close(a.scale, p.scale), "scale mismatch");
     Context &c = *a.ctx;
     const size_t pw = poly_words(c, a.limbs);
@@ -164,6 +168,7 @@ void op_sub_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst) { ad
 void op_multiply_plain(const Ciphertext &a, const Plaintext &p, Ciphertext &dst)
 {
     plain_check(a, p);
+    tally("mul_plain", a.limbs);
     require(a.ntt, "encrypted must be in NTT form");
     Context &c = *a.ctx;
     const double scale = a.scale * p.scale;
@@ -185,6 +190,7 @@ void op_multiply(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst)
     valid(a);
     valid(b);
     same_params(a, b);
+    tally("tensor", a.limbs);
     require(a.ntt && b.ntt, "encrypted1 or encrypted2 must be in NTT form");
     require(a.size == 2 && b.size == 2, "multiply expects two size-2 ciphertexts (relinearize first)");
     Context &c = *a.ctx;
@@ -279,6 +285,7 @@ DevBuf ks_modup_list(Context &c, const uint64_t *const *targets, size_t B, size_
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
                       bool zero_c1, const uint64_t *base)
 {
+    tally("ks", l, (double)B);
     const size_t n = c.n, nout = l + c.alpha;
     uint8_t outp[kMaxPrimes];
     ks_out_primes(c, l, outp);
@@ -449,6 +456,7 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
 {
     // evaluator.cpp:1402-1481 + util/rns.cpp:830-905
     valid(a);
+    tally("rescale", a.limbs, (double)a.size / 2);
     require(a.ntt, "CKKS encrypted must be in NTT form");
     require(a.limbs >= 2, "end of modulus switching chain reached");
     Context &c = *a.ctx;
@@ -495,6 +503,7 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
 static void rescale_batch(Context &c, const std::vector<const uint64_t *> &x, size_t polys, size_t l,
                           const std::vector<uint64_t *> &outs)
 {
+    tally("rescale", l, (double)x.size() * (double)polys / 2);
     const size_t n = c.n, B = x.size(), P = B * polys;
     require(P <= (size_t)kNttSrcMax, "rescale batch too large");
     std::vector<const uint64_t *> lasts;

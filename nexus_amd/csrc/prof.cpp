@@ -1,5 +1,6 @@
 #include "prof.h"
 
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -97,6 +98,20 @@ void prof_record(const char *cls, hipStream_t s, bool begin, double bytes)
     }
 }
 
+// ---------------------------------------------------------------- operation tally
+namespace {
+std::mutex g_tally_mu;
+bool g_tally_on = false;
+std::map<std::pair<std::string, size_t>, double> g_tally;
+}  // namespace
+
+void tally(const char *kind, size_t limbs, double count)
+{
+    if (!g_tally_on) return;
+    std::lock_guard<std::mutex> g(g_tally_mu);
+    if (g_tally_on) g_tally[{kind, limbs}] += count;
+}
+
 }  // namespace nx
 
 extern "C" {
@@ -165,6 +180,41 @@ int nx_profile_classes(char *buf, size_t len)
         nx::set_last_error(e.what());
         return e.code;
     }
+}
+
+int nx_op_tally(int start, char *buf, size_t len, size_t *needed)
+{
+    return nx::guard_call([&] {
+        std::lock_guard<std::mutex> g(nx::g_tally_mu);
+        if (start) {
+            nx::g_tally.clear();
+            nx::g_tally_on = true;
+            if (needed) *needed = 0;
+            return;
+        }
+        nx::g_tally_on = false;
+        std::string out = "{";
+        std::string kind;
+        for (auto &kv : nx::g_tally) {
+            if (kv.first.first != kind) {
+                if (!kind.empty()) out += "},";
+                kind = kv.first.first;
+                out += "\"" + kind + "\": {";
+            } else {
+                out += ", ";
+            }
+            char num[64];
+            std::snprintf(num, sizeof num, "\"%zu\": %.0f", kv.first.second, kv.second);
+            out += num;
+        }
+        if (!kind.empty()) out += "}";
+        out += "}";
+        if (needed) *needed = out.size() + 1;
+        if (buf && len) {
+            std::strncpy(buf, out.c_str(), len - 1);
+            buf[len - 1] = 0;
+        }
+    });
 }
 
 int nx_profile_valu_rate(int kind, double *ops_per_s)

@@ -9,6 +9,8 @@
 namespace nx {
 
 bool prof_active(const char *kernel_class);
+// Operation tally (nx_op_tally): `count` operations of `kind` on `limbs` limbs
+void tally(const char *kind, size_t limbs, double count = 1);
 void prof_record(const char *kernel_class, hipStream_t s, bool begin, double units);
 
 struct ProfScope {

@@ -76,6 +76,18 @@ def lib():
         L.or_crt_compose.argtypes = [u64p, ctypes.c_size_t, ctypes.c_size_t, u64p, u64p]
         L.or_ckks_decode.argtypes = [ctypes.c_void_p, u64p, ctypes.c_size_t, u64p, ctypes.POINTER(u64p),
                                      ctypes.c_double, ctypes.c_size_t, f64p, f64p]
+        vpp = ctypes.c_void_p
+        L.or_port_create.restype = vpp
+        L.or_port_create.argtypes = [ctypes.c_int, u64p, ctypes.c_size_t]
+        L.or_port_destroy.argtypes = [vpp]
+        L.or_port_ntt_forward.argtypes = [vpp, ctypes.c_size_t, u64p]
+        L.or_port_ntt_inverse.argtypes = [vpp, ctypes.c_size_t, u64p]
+        L.or_port_switch_key.argtypes = [vpp, u64p, u64p, ctypes.c_size_t, u64p]
+        L.or_port_rescale.argtypes = [vpp, u64p, ctypes.c_size_t]
+        L.or_port_multiply_plain.argtypes = [vpp, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, u64p]
+        L.or_port_tensor.argtypes = [vpp, u64p, u64p, ctypes.c_size_t, u64p]
+        L.or_port_add.argtypes = [vpp, u64p, u64p, ctypes.c_size_t, ctypes.c_size_t, u64p]
+        L.or_port_encode_residues.argtypes = [vpp, f64p, ctypes.c_size_t, u64p]
         _lib = L
     return _lib
 
@@ -315,3 +327,68 @@ class Encoder:
         lib().or_ckks_decode(self.h, _p(plain), l, _p(mods), irps, scale, sparse_slots, re.ctypes.data_as(f64p),
                              im.ctypes.data_as(f64p))
         return re + 1j * im if complex_out else re
+
+
+# ---------------------------------------------------------------- performance-faithful CPU port
+class Port:
+    """SEAL's CPU methods (lazy Harvey NTT, Shoup and Barrett products, SEAL-form
+    key switching) over `moduli` (L data primes then one special prime), for the
+    CPU baseline's timings; results equal the oracle's word for word.  ctypes
+    releases the GIL in every call, so threads run calls concurrently."""
+
+    def __init__(self, logn, moduli):
+        self.logn = logn
+        self.n = 1 << logn
+        self.moduli = [int(q) for q in moduli]
+        m = _arr(self.moduli)
+        self.h = lib().or_port_create(logn, _p(m), len(self.moduli))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_port_destroy(self.h)
+            self.h = None
+
+    def ntt_forward(self, a, prime):
+        a = _arr(a).copy()
+        lib().or_port_ntt_forward(self.h, prime, _p(a))
+        return a
+
+    def ntt_inverse(self, a, prime):
+        a = _arr(a).copy()
+        lib().or_port_ntt_inverse(self.h, prime, _p(a))
+        return a
+
+    def switch_key(self, ct, target, key):
+        ct = _arr(ct).copy()
+        target = _arr(target)
+        lib().or_port_switch_key(self.h, _p(ct), _p(target), target.shape[0], _p(_arr(key)))
+        return ct
+
+    def rescale(self, poly):
+        poly = _arr(poly).copy()
+        lib().or_port_rescale(self.h, _p(poly), poly.shape[0])
+        return poly[:-1].copy()
+
+    def multiply_plain(self, a, pt):
+        a = _arr(a)
+        out = np.zeros_like(a)
+        lib().or_port_multiply_plain(self.h, _p(a), _p(_arr(pt)), a.shape[0], a.shape[1], _p(out))
+        return out
+
+    def tensor(self, a, b):
+        a = _arr(a)
+        out = np.zeros((3,) + a.shape[1:], dtype=np.uint64)
+        lib().or_port_tensor(self.h, _p(a), _p(_arr(b)), a.shape[1], _p(out))
+        return out
+
+    def add(self, a, b):
+        a = _arr(a)
+        out = np.zeros_like(a)
+        lib().or_port_add(self.h, _p(a), _p(_arr(b)), a.shape[0], a.shape[1], _p(out))
+        return out
+
+    def encode_residues(self, coeffs, limbs):
+        c = np.ascontiguousarray(coeffs, dtype=np.float64)
+        out = np.zeros((limbs, self.n), dtype=np.uint64)
+        lib().or_port_encode_residues(self.h, c.ctypes.data_as(f64p), limbs, _p(out))
+        return out

@@ -138,6 +138,30 @@ def test_layer_forward(env, s):
     assert err < 2.0 ** -10
 
 
+def test_layer_forward_per_op(env):
+    """the plain per-call drop-in (per-op mode: single-ciphertext operations
+    only) computes the same layer: within 2^-10 of the FP64 model and within
+    noise of the batched program's output"""
+    s = 8
+    _note("bert layer forward per-op s=%d" % s)
+    rng = np.random.default_rng(10 + s)
+    x = np.zeros((env.T, HIDDEN))
+    x[:s] = rng.normal(0, 1, (s, HIDDEN))
+    enc = env.enc(x, chain_index=len(CHAIN) - 3 - 1)
+    batched = env.dec(env.layer.forward(enc, s), env.T)[:s]
+    env.ev.set_per_op(True)
+    try:
+        got = env.dec(env.layer.forward(enc, s), env.T)[:s]
+    finally:
+        env.ev.set_per_op(False)
+    want = ob.layer_approx(x[:s], env.W)
+    err = np.max(np.abs(got - want))
+    print("layer per-op: max |enc - fp64 model| %.3g, max |per-op - batched| %.3g"
+          % (err, np.max(np.abs(got - batched))))
+    assert err < 2.0 ** -10
+    assert np.max(np.abs(got - batched)) < 2.0 ** -12
+
+
 def test_layer_packed_sequences(env):
     """three sequences packed in one set: each equals its own layer"""
     segs = [(0, 3), (3, 2), (5, 3)]
