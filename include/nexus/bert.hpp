@@ -108,10 +108,16 @@ public:
     std::vector<Ciphertext> layer_norm(const std::vector<Ciphertext> &r);
 
 private:
-    // plaintext diagonals of one matrix at one (factor, level, scale)
+    // plaintext diagonals of one matrix at one (factor, level, scale): every
+    // diagonal is 256-periodic, so the diagonals of output ciphertext o over
+    // the inputs of group gi ([gi group, (gi + 1) group)) are one periodic set,
+    // entry ((c - gi group) n_a + a - a_min) baby + b, and index[o groups + gi]
+    // is its block-mix index (giant step a by (input, baby step); -1: no term)
     struct Encoded {
-        std::vector<Plaintext> pt;  // [((o * n_in + c) * n_a + a - a_min) * baby + b]
-        std::vector<char> present;
+        int group = 0, groups = 0;
+        std::vector<std::unique_ptr<PeriodicPlaintexts>> sets;  // [o * groups + gi]
+        std::vector<std::vector<int32_t>> index;
+        std::vector<char> giant_any;  // [o * n_a + a - a_min]: the giant step has a term
     };
     struct Weights {
         int n_in = 0, n_out = 0, a_min = 0, a_max = 0;

@@ -13,6 +13,7 @@
 #include <complex>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -153,6 +154,7 @@ private:
     nx_periodic_plain *h_ = nullptr;
     size_t count_ = 0;
     mutable std::map<size_t, Plaintext> expanded_;
+    mutable std::mutex mu_;  // guards expanded_ (sets are shared between lanes)
 };
 
 class Encoder {

@@ -100,37 +100,52 @@ std::shared_ptr<const BertLayer::Encoded> BertLayer::diagonals(Matrix m, double 
     for (auto &e : d.enc)
         if (e.first == key) return e.second;
     const int B = opt_.baby, n_a = d.a_max - d.a_min + 1;
-    const size_t per = (size_t)d.n_in * d.n_out * n_a * B;
     auto enc = std::make_shared<Encoded>();
-    enc->pt.resize(per);
-    enc->present.assign(per, 0);
+    // inputs per group: the block mix stages a 64-position tile of every input
+    // (512 bytes each) and the tile's weight words of every entry (one per
+    // 512-slot-period block the tile touches) in 64 KiB of LDS
+    const int stride = (int)(2 * slots_ / (2 * kGroup));  // NTT positions per block
+    const int bpt = stride >= 64 ? 1 : 64 / stride;
+    enc->group = std::max(1, std::min(d.n_in, 65536 / (B * (512 + 8 * n_a * bpt))));
+    enc->groups = (d.n_in + enc->group - 1) / enc->group;
+    enc->giant_any.assign((size_t)d.n_out * n_a, 0);
     const int out_dim = d.n_out * kGroup;
-    std::vector<double> pattern(kGroup), v(slots_);
+    std::vector<double> pattern(kGroup);
     for (int o = 0; o < d.n_out; o++)
-        for (int c = 0; c < d.n_in; c++)
-            for (int a = d.a_min; a <= d.a_max; a++)
-                for (int b = 0; b < B; b++) {
-                    const int e = B * a + b;
-                    if (e <= -kGroup || e >= kGroup) continue;
-                    // D_e[256 i + j] = W[256 c + j + e][256 o + j] for 0 <= j + e < 256
-                    bool any = false;
-                    for (int j = 0; j < kGroup; j++) {
-                        const int f = j + e;
-                        double x = 0;
-                        if (f >= 0 && f < kGroup)
-                            x = d.w[(size_t)(kGroup * c + f) * out_dim + (size_t)(kGroup * o + j)] * factor;
-                        pattern[j] = x;
-                        any |= x != 0;
+        for (int gi = 0; gi < enc->groups; gi++) {
+            const int c0 = gi * enc->group, c1 = std::min(d.n_in, c0 + enc->group);
+            const int K = (c1 - c0) * B;
+            const size_t count = (size_t)(c1 - c0) * n_a * B;
+            std::vector<double> vals(count * kGroup, 0.0);
+            std::vector<int32_t> index((size_t)n_a * K, -1);
+            for (int c = c0; c < c1; c++)
+                for (int a = d.a_min; a <= d.a_max; a++)
+                    for (int b = 0; b < B; b++) {
+                        const int e = B * a + b;
+                        if (e <= -kGroup || e >= kGroup) continue;
+                        // D_e[256 i + j] = W[256 c + j + e][256 o + j] for 0 <= j + e < 256
+                        bool any = false;
+                        for (int j = 0; j < kGroup; j++) {
+                            const int f = j + e;
+                            double x = 0;
+                            if (f >= 0 && f < kGroup)
+                                x = d.w[(size_t)(kGroup * c + f) * out_dim + (size_t)(kGroup * o + j)] * factor;
+                            pattern[j] = x;
+                            any |= x != 0;
+                        }
+                        if (!any) continue;
+                        // pre-rotated by the giant step: D'[y] = D_e[y - B a], still 256-periodic
+                        const int shift = ((B * a) % kGroup + kGroup) % kGroup;
+                        const size_t entry = ((size_t)(c - c0) * n_a + (size_t)(a - d.a_min)) * B + b;
+                        for (int y = 0; y < kGroup; y++)
+                            vals[entry * kGroup + y] = pattern[(y - shift + kGroup) % kGroup];
+                        index[(size_t)(a - d.a_min) * K + (size_t)(c - c0) * B + b] = (int32_t)entry;
+                        enc->giant_any[(size_t)o * n_a + (a - d.a_min)] = 1;
                     }
-                    if (!any) continue;
-                    // pre-rotated by the giant step: D'[y] = D_e[y - B a]
-                    const int shift = ((B * a) % kGroup + kGroup) % kGroup;
-                    for (size_t y = 0; y < slots_; y++) v[y] = pattern[((int)(y % kGroup) - shift + kGroup) % kGroup];
-                    const size_t k = (((size_t)o * d.n_in + c) * n_a + (size_t)(a - d.a_min)) * B + b;
-                    enc->pt[k] = Plaintext(*ck_.context);
-                    ck_.encoder.encode(v, chain_index, scale, enc->pt[k]);
-                    enc->present[k] = 1;
-                }
+            enc->sets.push_back(
+                std::make_unique<PeriodicPlaintexts>(*ck_.context, vals, (size_t)kGroup, chain_index, scale));
+            enc->index.push_back(std::move(index));
+        }
     // complete on this thread's stream before any lane reads them
     ck_.context->synchronize();
     if (d.enc.size() >= 2) d.enc.erase(d.enc.begin());
@@ -146,7 +161,7 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
     const Evaluator &ev = ck_.evaluator;
     const double delta = ck_.scale;
     const size_t two = chain_of(2);
-    // inputs at two limbs, all at one scale s_in; the plaintexts are encoded
+    // inputs at two limbs, all at one scale s_in; the diagonals are encoded
     // at delta q_1 / s_in so that the rescale (dropping q_1) lands exactly on
     // delta -- no scale is overwritten
     std::vector<Ciphertext> in(x.size());
@@ -173,33 +188,32 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
         for (auto &t : r) baby[c].push_back(std::move(t));
     }
     std::vector<Ciphertext> out(d.n_out);
-    std::vector<const Ciphertext *> terms;  // every (input, baby) pair
-    for (int c = 0; c < d.n_in; c++)
-        for (int b = 0; b < B; b++) terms.push_back(&baby[c][b]);
     for (int o = 0; o < d.n_out; o++) {
-        // every giant step's multiply-accumulate in one pass over the babies;
-        // the giant rotations summed with a single ModDown
-        std::vector<const Plaintext *> weights;
-        std::vector<int> gsteps;
-        for (int a = d.a_min; a <= d.a_max; a++) {
-            std::vector<const Plaintext *> row(terms.size(), nullptr);
-            bool any = false;
-            for (int c = 0; c < d.n_in; c++)
-                for (int b = 0; b < B; b++) {
-                    const size_t k = (((size_t)o * d.n_in + c) * n_a + (size_t)(a - d.a_min)) * B + b;
-                    if (!enc->present[k]) continue;
-                    row[(size_t)c * B + b] = &enc->pt[k];
-                    any = true;
-                }
-            if (!any) continue;
-            weights.insert(weights.end(), row.begin(), row.end());
-            gsteps.push_back(B * a);
-        }
+        // every giant step's multiply-accumulate as block mixes over the
+        // babies of each input group (the periodic diagonals are 512 words a
+        // limb); the giant rotations summed with a single ModDown
         std::vector<Ciphertext> accs;
-        if (!gsteps.empty()) ev.multiply_plain_accumulate_multi(terms, weights, accs);
-        if (accs.empty()) throw std::logic_error("BertLayer: all-zero weight block");
+        for (int gi = 0; gi < enc->groups; gi++) {
+            const int c0 = gi * enc->group, c1 = std::min(d.n_in, c0 + enc->group);
+            std::vector<const Ciphertext *> terms;
+            for (int c = c0; c < c1; c++)
+                for (int b = 0; b < B; b++) terms.push_back(&baby[c][b]);
+            std::vector<Ciphertext> part;
+            ev.multiply_periodic_mix(terms, *enc->sets[(size_t)o * enc->groups + gi],
+                                     enc->index[(size_t)o * enc->groups + gi], part);
+            if (accs.empty())
+                accs = std::move(part);
+            else
+                for (int a = 0; a < n_a; a++) ev.add_inplace(accs[a], part[a]);
+        }
         std::vector<const Ciphertext *> ap;
-        for (const auto &a : accs) ap.push_back(&a);
+        std::vector<int> gsteps;
+        for (int a = d.a_min; a <= d.a_max; a++)
+            if (enc->giant_any[(size_t)o * n_a + (a - d.a_min)]) {
+                ap.push_back(&accs[a - d.a_min]);
+                gsteps.push_back(B * a);
+            }
+        if (ap.empty()) throw std::logic_error("BertLayer: all-zero weight block");
         ev.rotate_sum(ap, gsteps, *ck_.galois_keys, out[o]);
         ev.rescale_to_next_inplace(out[o]);
     }

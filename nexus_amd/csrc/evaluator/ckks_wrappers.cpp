@@ -639,6 +639,7 @@ void PeriodicPlaintexts::expand(size_t index, Plaintext &out) const
 
 const Plaintext &PeriodicPlaintexts::expanded(size_t index) const
 {
+    std::lock_guard<std::mutex> g(mu_);
     auto it = expanded_.find(index);
     if (it != expanded_.end()) return it->second;
     Plaintext r;

@@ -9,12 +9,45 @@
 // evaluated here in long double and rounded half away from zero at `scale`.
 // No other coefficient is non-zero, so the NTT form is exactly constant over
 // blocks of n / B positions; creation checks that on the device.
+//
+// The sums are one length-2B transform per entry: with z_g = v_j at g = 5^j
+// mod 2B (zero elsewhere), sum_j v_j cos(pi g_j k / B) = Re sum_g z_g
+// exp(2 pi i g k / 2B), an unnormalised inverse DFT (radix-2, long double),
+// O(B log B) per entry instead of O(P B).
+#include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 
 #include "context.h"
+#include "numth.h"
 
 namespace nx {
+
+namespace {
+
+typedef std::complex<long double> cld;
+
+// a <- (sum_g a_g exp(2 pi i g k / M))_k for M = a.size() a power of two; w[t] = exp(2 pi i t / M), t < M/2
+void inverse_dft(std::vector<cld> &a, const std::vector<cld> &w, int logm)
+{
+    const size_t M = a.size();
+    for (size_t i = 0; i < M; i++) {
+        const size_t j = reverse_bits((uint32_t)i, logm);
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    for (size_t len = 2; len <= M; len <<= 1) {
+        const size_t half = len / 2, step = M / len;
+        for (size_t s0 = 0; s0 < M; s0 += len)
+            for (size_t j = 0; j < half; j++) {
+                const cld t = a[s0 + j + half] * w[j * step];
+                a[s0 + j + half] = a[s0 + j] - t;
+                a[s0 + j] += t;
+            }
+    }
+}
+
+}  // namespace
 
 void periodic_create(Context &c, const double *values, size_t period, size_t count, size_t limbs, double scale,
                      PeriodicPlain &out)
@@ -22,7 +55,7 @@ void periodic_create(Context &c, const double *values, size_t period, size_t cou
     require(values != nullptr, "values is null");
     require(period >= 1 && (period & (period - 1)) == 0 && period <= c.n / 2,
             "period must be a power of two between 1 and the slot count");
-    require(count >= 1 && count <= 256, "count must be in 1..256");
+    require(count >= 1 && count <= ((size_t)1 << 20), "count must be in 1..2^20");
     require(limbs >= 1 && limbs <= c.L, "chain index out of range");
     require(std::isfinite(scale) && scale > 0 && scale < std::ldexp(1.0, 62), "scale out of bounds");
     const size_t B = 2 * period, n = c.n;
@@ -35,16 +68,21 @@ void periodic_create(Context &c, const double *values, size_t period, size_t cou
     }
     std::vector<uint64_t> res(count * limbs * B);
     const long double pi = 3.141592653589793238462643383279502884L;
+    const size_t M = 2 * B;
+    const int logm = __builtin_ctzll((unsigned long long)M);
+    std::vector<cld> w(M / 2), z(M);
+    for (size_t t = 0; t < M / 2; t++) {
+        const long double ang = 2 * pi * (long double)t / (long double)M;
+        w[t] = cld(cosl(ang), sinl(ang));
+    }
     for (size_t e = 0; e < count; e++) {
         const double *v = values + e * period;
         for (size_t j = 0; j < period; j++) require(std::isfinite(v[j]), "values must be finite");
+        std::fill(z.begin(), z.end(), cld(0));
+        for (size_t j = 0; j < period; j++) z[g[j]] = (long double)v[j];
+        inverse_dft(z, w, logm);
         for (size_t k = 0; k < B; k++) {
-            long double ck = 0;
-            for (size_t j = 0; j < period; j++) {
-                if (v[j] == 0) continue;
-                const uint64_t ph = (g[j] * k) % (2 * B);  // cos(pi ph / B) has period 2B in ph
-                ck += (long double)v[j] * cosl(pi * (long double)ph / (long double)B);
-            }
+            long double ck = z[k].real();
             ck = ck / (long double)period * (long double)scale;
             require(fabsl(ck) < 4.611686018427387904e18L, "encoded value too large for the scale");
             const long long r = llroundl(ck);
