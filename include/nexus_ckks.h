@@ -105,6 +105,10 @@ int nx_ciphertext_download(const nx_ciphertext *ct, uint64_t *data);
 int nx_ciphertext_info(const nx_ciphertext *ct, size_t *size, size_t *chain_index, size_t *coeff_modulus_size,
                        double *scale, int *ntt_form);
 int nx_ciphertext_set_scale(nx_ciphertext *ct, double scale);
+/* Device address of the residues ([size][limbs][N] words; *words may be NULL),
+ * valid until the ciphertext is written or destroyed (reference
+ * PhantomCiphertext::data(), cuda/thirdparty/phantom-fhe/include/ciphertext.h:222). */
+int nx_ciphertext_device_data(const nx_ciphertext *ct, uint64_t **data, size_t *words);
 
 int nx_plaintext_create(nx_context *ctx, nx_plaintext **out);
 int nx_plaintext_destroy(nx_plaintext *pt);

@@ -68,6 +68,7 @@ for _n in ("ciphertext", "plaintext"):
     _sig("nx_%s_download" % _n, [vp, u64p])
     _sig("nx_%s_set_scale" % _n, [vp, ctypes.c_double])
 _sig("nx_ciphertext_upload", [vp, u64p, sz, sz, ctypes.c_double, ctypes.c_int])
+_sig("nx_ciphertext_device_data", [vp, ctypes.POINTER(u64p), ctypes.POINTER(sz)])
 _sig("nx_ciphertext_info", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz),
                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int)])
 _sig("nx_plaintext_upload", [vp, u64p, sz, ctypes.c_double, ctypes.c_int])
@@ -241,6 +242,12 @@ class Ciphertext:
         c = Ciphertext(self.ctx)
         _check(_lib.nx_ciphertext_copy(c.h, self.h))
         return c
+
+    def device_data(self):
+        """(device address, words) of the residues (PhantomCiphertext::data())"""
+        p, w = ctypes.POINTER(ctypes.c_uint64)(), sz()
+        _check(_lib.nx_ciphertext_device_data(self.h, ctypes.byref(p), ctypes.byref(w)))
+        return ctypes.cast(p, ctypes.c_void_p).value or 0, w.value
 
 
 class Plaintext:
@@ -1206,3 +1213,4 @@ def op_tally_stop():
     buf = ctypes.create_string_buffer(max(need.value, 2))
     _check(_lib.nx_op_tally(0, buf, len(buf), ctypes.byref(need)))
     return {k: {int(l): c for l, c in v.items()} for k, v in json.loads(buf.value.decode()).items()}
+

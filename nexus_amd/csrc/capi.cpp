@@ -262,6 +262,17 @@ int nx_ciphertext_info(const nx_ciphertext *ct, size_t *size, size_t *chain_inde
     });
 }
 
+int nx_ciphertext_device_data(const nx_ciphertext *ct, uint64_t **data, size_t *words)
+{
+    return guard([&] {
+        need(ct, "ct");
+        need(data, "data");
+        const nx::Ciphertext &c = ct->impl;
+        *data = c.data.get();
+        if (words) *words = c.size * c.limbs * (c.ctx ? c.ctx->n : 0);
+    });
+}
+
 int nx_ciphertext_set_scale(nx_ciphertext *ct, double scale)
 {
     return guard([&] {
