@@ -209,6 +209,19 @@ int nx_multiply_vector_reduced_error(const nx_ciphertext *a, const double *re, c
                                      nx_ciphertext *dest);
 /* double_inplace (cuda/src/ckks_evaluator.cuh:367-369): dest = a + a */
 int nx_double(const nx_ciphertext *a, nx_ciphertext *dest);
+/* Allocation-free rescale: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  The residues of nx_rescale_to_next
+ * (SEAL util/rns.cpp:830-905, q_last / 2 rounding) for `polys` (1..3)
+ * polynomials of `limbs` (2..L) NTT-form limbs at device pointer `in`
+ * ([polys][limbs][N] words, limb i on prime q_i) into device memory `out`
+ * ([polys][limbs - 1][N], not overlapping `in`); the scale is the caller's to
+ * divide by q_(limbs-1).  Calls no hipMalloc or hipFree: its scratch is a
+ * 32 MiB static device array (polys x limbs x N <= 4 Mi words, larger inputs
+ * are rejected), one per GPU, held until the process ends.  Stream-ordered on
+ * the calling thread's stream of ctx; calls that overlap on one GPU share the
+ * array and race. */
+int nx_rescale_to_next_noalloc(nx_context *ctx, const uint64_t *in, size_t polys, size_t limbs, uint64_t *out);
 /* multiply by x^exponent (coefficient shift; reference matrix_mul.cu:84-127) */
 int nx_multiply_power_of_x(const nx_ciphertext *a, int64_t exponent, nx_ciphertext *dest);
 /* ModRaise of a lowest-level ciphertext to the top data level (reference

@@ -1214,3 +1214,11 @@ def op_tally_stop():
     _check(_lib.nx_op_tally(0, buf, len(buf), ctypes.byref(need)))
     return {k: {int(l): c for l, c in v.items()} for k, v in json.loads(buf.value.decode()).items()}
 
+
+_sig("nx_rescale_to_next_noalloc", [vp, vp, sz, sz, vp])
+
+
+def rescale_to_next_noalloc(ctx, in_ptr, polys, limbs, out_ptr):
+    """allocation-free rescale of [polys][limbs][N] residues at device address
+    in_ptr into [polys][limbs-1][N] at out_ptr (include/nexus_ckks.h)"""
+    _check(_lib.nx_rescale_to_next_noalloc(ctx.h, vp(in_ptr), polys, limbs, vp(out_ptr)))

@@ -779,6 +779,14 @@ int nx_double(const nx_ciphertext *a, nx_ciphertext *dest)
     });
 }
 
+int nx_rescale_to_next_noalloc(nx_context *ctx, const uint64_t *in, size_t polys, size_t limbs, uint64_t *out)
+{
+    return guard([&] {
+        need(ctx, "ctx");
+        nx::rescale_noalloc(ctx->impl, in, polys, limbs, out);
+    });
+}
+
 int nx_multiply_power_of_x(const nx_ciphertext *a, int64_t exponent, nx_ciphertext *dest)
 {
     return guard([&] {

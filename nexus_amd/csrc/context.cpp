@@ -237,6 +237,8 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     }
     if (alpha > 1) build_hybrid_tables();
 
+    // the allocation-free entry points' scratch array: its address on this device
+    noalloc_init_current_device();
     NX_HIP(hipMalloc(&d_fwd, P * n * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_inv, P * n * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_ninv, P * sizeof(ulonglong2)));

@@ -274,6 +274,12 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
 // dst = a (op) b after aligning the levels and scales; Mul relinearises
 enum class ReducedOp { Add, Sub, Mul };
 void op_reduced_error(const Ciphertext &a, const Ciphertext &b, ReducedOp op, const KSwitchKeys *rk, Ciphertext &dst);
+// allocation-free rescale (noalloc.hip): [polys][limbs][n] NTT-form residues at
+// `in` -> [polys][limbs-1][n] at `out`, both caller device memory, scratch in a
+// static device array whose address is taken per device at context creation
+void noalloc_init_current_device();
+size_t noalloc_scratch_words();
+void rescale_noalloc(Context &c, const uint64_t *in, size_t polys, size_t limbs, uint64_t *out);
 void op_mod_switch_to_next(const Ciphertext &a, Ciphertext &dst);
 void op_mod_switch_to(const Ciphertext &a, size_t chain_index, Ciphertext &dst);
 void op_plain_mod_switch_to(const Plaintext &a, size_t chain_index, Plaintext &dst);
