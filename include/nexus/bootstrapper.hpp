@@ -74,11 +74,19 @@ public:
     // last SlotToCoeff level: no extra level, error scaled by factor).
     void bootstrap(const Ciphertext &in, Ciphertext &out, double factor = 1.0);
     // bootstrap of several ciphertexts (out[i] = factors[i] x in[i]'s message):
-    // ModRaise and CoeffToSlot per input, one EvalMod over every input's halves
-    // in lockstep (each multiplication, key switch and rescale one batched
-    // launch), SlotToCoeff per input
+    // bootstrap_mixed with this bootstrapper for every input
     void bootstrap_many(const std::vector<const Ciphertext *> &in, std::vector<Ciphertext> &out,
                         const std::vector<double> &factors);
+    // bootstrap of inputs with their own bootstrappers (slot counts) on one
+    // context, e.g. dense and sparse packings: out[i] = factors[i] x in[i]'s
+    // message through bts[i]'s transforms; inputs at any level (each is
+    // brought to q_0 by ModRaise's mod switch); every CoeffToSlot and
+    // SlotToCoeff level batched over the inputs of one bootstrapper, one
+    // EvalMod over every input's halves (the bootstrappers need the same
+    // EvalMod parameters).  Word for word each input's own bootstrap with one
+    // special prime.
+    static void bootstrap_mixed(const std::vector<Bootstrapper *> &bts, const std::vector<const Ciphertext *> &in,
+                                std::vector<Ciphertext> &out, const std::vector<double> &factors);
     // Bootstrapper::set_final_scale (reference Bootstrapper.h)
     void set_final_scale(double s) { p_.final_scale = s; }
 
@@ -95,6 +103,9 @@ public:
 
 private:
     Ciphertext apply_level(const LinearLevel &lv, int tag, const Ciphertext &x, double out_scale, double value_factor);
+    std::vector<Ciphertext> apply_level_many(const LinearLevel &lv, int tag, const std::vector<const Ciphertext *> &xs,
+                                             const std::vector<double> &out_scale,
+                                             const std::vector<double> &value_factor);
     Ciphertext cheb_eval(const std::vector<double> &c, size_t level, double scale);
     const Ciphertext &basis(int k);
     Ciphertext match(const Ciphertext &ct, size_t level, double scale);

@@ -279,6 +279,13 @@ public:
     void rotate_vector_hoisted(const Ciphertext &ct, const std::vector<int> &steps, const GaloisKeys &keys,
                                std::vector<Ciphertext> &dest) const;
     void hoisting_inplace(Ciphertext &ct, const GaloisKeys &keys, const std::vector<int> &steps) const;
+    // every input rotated by every step, batched across inputs: dest[b][k] = rot(cts[b], steps[k])
+    void rotate_vector_hoisted_many(const std::vector<const Ciphertext *> &cts, const std::vector<int> &steps,
+                                    const GaloisKeys &keys, std::vector<std::vector<Ciphertext>> &dest) const;
+    // dest[g] = sum_i rot(groups[g][i], steps[g][i]), the ModDowns of all groups batched
+    void rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &groups,
+                         const std::vector<std::vector<int>> &steps, const GaloisKeys &keys,
+                         std::vector<Ciphertext> &dest) const;
     // sum_i rot(cts[i], steps[i]) with a single ModDown (step 0 = the input)
     void rotate_sum(const std::vector<const Ciphertext *> &cts, const std::vector<int> &steps, const GaloisKeys &keys,
                     Ciphertext &dest) const;

@@ -82,6 +82,8 @@ public:
     // While paused, queued requests are not admitted (requests submitted
     // meanwhile are batched together once resumed).
     void pause(bool paused);
+    // BERT layer calls allowed in flight at once (1..lanes)
+    void set_bert_concurrency(int n);
 
 private:
     struct Request {

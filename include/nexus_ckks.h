@@ -244,6 +244,17 @@ int nx_hoisting(const nx_ciphertext *a, const int *steps, size_t count, const nx
  * per term).  Decrypts like the sum of nx_rotate_vector results. */
 int nx_rotate_sum(const nx_ciphertext *const *in, const int *steps, size_t count,
                   const nx_kswitch_keys *galois_keys, nx_ciphertext *dest);
+/* Batched forms of the two above (no counterpart in the reference): every
+ * input of one level rotated by every step (dests[b nsteps + k] =
+ * rot(in[b], steps[k]), word for word nx_rotate_hoisted of each input, the
+ * ModUps, inner products and ModDowns batched across inputs); and several
+ * sums of rotations at once (group i: the next group_sizes[i] entries of in /
+ * steps into dests[i], word for word nx_rotate_sum of each group, the ModDowns
+ * of all groups one batch).  The BSGS levels of a batch of bootstraps use them. */
+int nx_rotate_hoisted_many(const nx_ciphertext *const *in, size_t count, const int *steps, size_t nsteps,
+                           const nx_kswitch_keys *galois_keys, nx_ciphertext *const *dests);
+int nx_rotate_sum_many(const nx_ciphertext *const *in, const int *steps, const size_t *group_sizes, size_t groups,
+                       const nx_kswitch_keys *galois_keys, nx_ciphertext *const *dests);
 /* dest = sum_r cts[r] * pts[r] (size-2 ciphertexts of one level, NTT form) in one
  * pass: the per-giant-step accumulation of the reference's BSGS transforms
  * (cuda/src/bootstrapping/Bootstrapper.cu:1855-1919, multiply_vector_reduced_error

@@ -79,12 +79,24 @@ int nx_bootstrap(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *ou
  * error is scaled by it).  No counterpart in the reference. */
 int nx_bootstrap_scaled(nx_bootstrapper *bt, const nx_ciphertext *in, double factor, nx_ciphertext *out);
 /* out[i] = bootstrap of in[i] times factors[i] (factors may be NULL: all 1)
- * for i < count: ModRaise and CoeffToSlot per input, one EvalMod over every
- * input in lockstep (each multiplication, key switch and rescale one batched
- * launch), SlotToCoeff per input.  Decrypts like count nx_bootstrap_scaled
- * calls.  No counterpart in the reference, which bootstraps one at a time. */
+ * for i < count: every CoeffToSlot / SlotToCoeff level batched over the
+ * inputs, one EvalMod over every input in lockstep (each multiplication, key
+ * switch and rescale one batched launch): nx_bootstrap_many_mixed with bt for
+ * every input.  Decrypts like count nx_bootstrap_scaled calls.  No
+ * counterpart in the reference, which bootstraps one at a time. */
 int nx_bootstrap_many(nx_bootstrapper *bt, const nx_ciphertext *const *in, size_t count, const double *factors,
                       nx_ciphertext *const *out);
+/* Bootstrapping of a batch with one bootstrapper per input (of one context
+ * and EvalMod parameters, any slot counts: dense and sparse packings mixed,
+ * inputs at any level): out[i] = factors[i] (NULL: 1) x in[i]'s message
+ * through bts[i]'s transforms (bootstrap_full_3 / bootstrap_sparse_3,
+ * cuda/src/bootstrapping/Bootstrapper.cu:2987-3113).  Every CoeffToSlot and
+ * SlotToCoeff level runs once over the inputs of each bootstrapper (baby
+ * rotations, ModDowns and rescales batched), one EvalMod over every input's
+ * halves.  Word for word each input's nx_bootstrap_scaled with one special
+ * prime.  No counterpart in the reference, which bootstraps one at a time. */
+int nx_bootstrap_many_mixed(nx_bootstrapper *const *bts, const nx_ciphertext *const *in, size_t count,
+                            const double *factors, nx_ciphertext *const *out);
 /* stages of bootstrap_sparse_3 / bootstrap_full_3 (Bootstrapper.cu:2987-3113) */
 int nx_boot_mod_raise(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
 int nx_boot_sub_sum(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out);
@@ -111,6 +123,9 @@ typedef struct nx_server nx_server;
 enum { NX_JOB_GELU = 0, NX_JOB_SOFTMAX = 1, NX_JOB_LAYER_NORM = 2, NX_JOB_BOOTSTRAP = 3, NX_JOB_BERT = 4 };
 int nx_server_create(nx_evaluator *ev, int lanes, size_t memory_budget, const nx_bootstrapper *boot,
                      nx_server **out);
+/* BERT layer calls in flight at once, 1..lanes (default 1): each call's work
+ * overlaps on the device with the others'.  No counterpart in the reference. */
+int nx_server_set_bert_concurrency(nx_server *s, int n);
 int nx_server_destroy(nx_server *s);
 /* a server that also runs BERT-layer requests (nx_server_submit_bert) with
  * the layer's weights shared by every lane.  pack != 0: continuous batching

@@ -589,6 +589,9 @@ def load(ctx, data, kind, galois=False):
         _check(_lib.nx_kswitch_keys_load(ctx.h, buf, len(data), 1 if galois else 0, out.h))
     return out
 _sig("nx_rotate_hoisted", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, ctypes.POINTER(vp)])
+_sig("nx_rotate_hoisted_many", [ctypes.POINTER(vp), sz, ctypes.POINTER(ctypes.c_int), sz, vp, ctypes.POINTER(vp)])
+_sig("nx_rotate_sum_many", [ctypes.POINTER(vp), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(sz), sz, vp,
+                            ctypes.POINTER(vp)])
 _sig("nx_hoisting", [vp, ctypes.POINTER(ctypes.c_int), sz, vp, vp])
 
 
@@ -725,6 +728,27 @@ def rotate_hoisted(a, steps, galois_keys):
     return outs
 
 
+def rotate_hoisted_many(cts, steps, galois_keys):
+    """[[rot(c, s) for s in steps] for c in cts], hoisted per input, batched across inputs"""
+    B, S = len(cts), len(steps)
+    outs = [[Ciphertext(cts[0].ctx) for _ in steps] for _ in cts]
+    st = (ctypes.c_int * S)(*steps)
+    hs = (vp * (B * S))(*[o.h for row in outs for o in row])
+    _check(_lib.nx_rotate_hoisted_many((vp * B)(*[c.h for c in cts]), B, st, S, galois_keys.h, hs))
+    return outs
+
+
+def rotate_sum_many(groups, steps, galois_keys):
+    """[sum_i rot(g[i], s[i]) for g, s in zip(groups, steps)], the ModDowns batched"""
+    flat = [c for g in groups for c in g]
+    st = [s for ss in steps for s in ss]
+    sizes = (sz * len(groups))(*[len(g) for g in groups])
+    outs = [Ciphertext(flat[0].ctx) for _ in groups]
+    _check(_lib.nx_rotate_sum_many((vp * len(flat))(*[c.h for c in flat]), (ctypes.c_int * len(st))(*st), sizes,
+                                   len(groups), galois_keys.h, (vp * len(outs))(*[o.h for o in outs])))
+    return outs
+
+
 def hoisting(a, steps, galois_keys, dest=None):
     """sum_k rot(a, steps[k]) with one ModUp and one ModDown (Phantom hoisting_inplace)."""
     d = _out(a.ctx, dest)
@@ -820,6 +844,7 @@ _sig("nx_bootstrap", [vp, vp, vp])
 _sig("nx_bootstrap_scaled", [vp, vp, ctypes.c_double, vp])
 _sig("nx_bootstrap_many", [vp, ctypes.POINTER(vp), sz, f64p, ctypes.POINTER(vp)])
 _sig("nx_boot_mod_raise", [vp, vp, vp])
+_sig("nx_bootstrap_many_mixed", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, f64p, ctypes.POINTER(vp)])
 _sig("nx_boot_sub_sum", [vp, vp, vp])
 _sig("nx_boot_coeff_to_slot", [vp, vp, vp, vp])
 _sig("nx_boot_eval_mod", [vp, vp, vp])
@@ -927,6 +952,17 @@ class Bootstrapper:
         f = (ctypes.c_double * len(cts))(*(factors if factors is not None else [1.0] * len(cts)))
         _check(_lib.nx_bootstrap_many(self.h, (vp * len(cts))(*[c.h for c in cts]), len(cts), f,
                                       (vp * len(outs))(*[o.h for o in outs])))
+        return outs
+
+    @staticmethod
+    def bootstrap_mixed(bts, cts, factors=None):
+        """[bts[i].bootstrap(cts[i], factors[i])] as one batch: bootstrappers of one
+        context with different slot counts, inputs at any level"""
+        n = len(cts)
+        outs = [Ciphertext(cts[0].ctx) for _ in cts]
+        f = (ctypes.c_double * n)(*(factors if factors is not None else [1.0] * n))
+        _check(_lib.nx_bootstrap_many_mixed((vp * n)(*[b.h for b in bts]), (vp * n)(*[c.h for c in cts]), n, f,
+                                            (vp * n)(*[o.h for o in outs])))
         return outs
 
     def mod_raise(self, ct):
@@ -1089,6 +1125,7 @@ _sig("nx_server_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double)
 _sig("nx_server_create_bert", [vp, ctypes.c_int, sz, vp, vp, ctypes.c_int, sz, ctypes.POINTER(vp)])
 _sig("nx_server_submit_bert", [vp, ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)])
 _sig("nx_server_pause", [vp, ctypes.c_int])
+_sig("nx_server_set_bert_concurrency", [vp, ctypes.c_int])
 _sig("nx_server_batching_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)])
 
 JOB_GELU, JOB_SOFTMAX, JOB_LAYER_NORM, JOB_BOOTSTRAP, JOB_BERT = 0, 1, 2, 3, 4
@@ -1133,6 +1170,10 @@ class Server:
         _check(_lib.nx_server_submit_bert(self.h, _handles(cts), len(cts), seq_len, ctypes.byref(rid)))
         self._counts[rid.value] = len(cts)
         return rid.value
+
+    def set_bert_concurrency(self, n):
+        """BERT layer calls in flight at once (1..lanes)"""
+        _check(_lib.nx_server_set_bert_concurrency(self.h, n))
 
     def pause(self, paused=True):
         _check(_lib.nx_server_pause(self.h, 1 if paused else 0))

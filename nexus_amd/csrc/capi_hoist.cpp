@@ -55,6 +55,55 @@ int nx_rotate_sum(const nx_ciphertext *const *in, const int *steps, size_t count
     });
 }
 
+int nx_rotate_hoisted_many(const nx_ciphertext *const *in, size_t count, const int *steps, size_t nsteps,
+                           const nx_kswitch_keys *galois_keys, nx_ciphertext *const *dests)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(in, "in");
+        nx::need_ptr(galois_keys, "galois_keys");
+        nx::need_ptr(dests, "dests");
+        if (nsteps) nx::need_ptr(steps, "steps");
+        std::vector<const nx::Ciphertext *> v;
+        for (size_t b = 0; b < count; b++) {
+            nx::need_ptr(in[b], "in[b]");
+            v.push_back(&in[b]->impl);
+        }
+        for (size_t k = 0; k < count * nsteps; k++) nx::need_ptr(dests[k], "dests[k]");
+        std::vector<std::vector<nx::Ciphertext>> out;
+        nx::op_rotate_hoisted_many(v, std::vector<int>(steps, steps + nsteps), galois_keys->impl, out);
+        for (size_t b = 0; b < count; b++)
+            for (size_t k = 0; k < nsteps; k++) dests[b * nsteps + k]->impl = std::move(out[b][k]);
+    });
+}
+
+int nx_rotate_sum_many(const nx_ciphertext *const *in, const int *steps, const size_t *group_sizes, size_t groups,
+                       const nx_kswitch_keys *galois_keys, nx_ciphertext *const *dests)
+{
+    return nx::guard_call([&] {
+        nx::need_ptr(in, "in");
+        nx::need_ptr(steps, "steps");
+        nx::need_ptr(group_sizes, "group_sizes");
+        nx::need_ptr(galois_keys, "galois_keys");
+        nx::need_ptr(dests, "dests");
+        std::vector<std::vector<const nx::Ciphertext *>> g(groups);
+        std::vector<std::vector<int>> st(groups);
+        std::vector<nx::Ciphertext> res(groups);
+        std::vector<nx::Ciphertext *> outs;
+        size_t k = 0;
+        for (size_t i = 0; i < groups; i++) {
+            nx::need_ptr(dests[i], "dests[i]");
+            for (size_t j = 0; j < group_sizes[i]; j++, k++) {
+                nx::need_ptr(in[k], "in[k]");
+                g[i].push_back(&in[k]->impl);
+                st[i].push_back(steps[k]);
+            }
+            outs.push_back(&res[i]);
+        }
+        nx::op_rotate_sum_many(g, st, galois_keys->impl, outs);
+        for (size_t i = 0; i < groups; i++) dests[i]->impl = std::move(res[i]);
+    });
+}
+
 int nx_multiply_plain_accumulate(const nx_ciphertext *const *cts, const nx_plaintext *const *pts, size_t count,
                                  nx_ciphertext *dest)
 {

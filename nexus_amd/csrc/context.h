@@ -249,6 +249,9 @@ void ntt_inv_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, 
 
 void ct_resize(Ciphertext &ct, Context *c, size_t size, size_t limbs);
 void ct_copy(Ciphertext &dst, const Ciphertext &src);
+// copy `limbs` limbs of each of `polys` polys between strided buffers (stream-ordered)
+void copy_limbs(Context &c, uint64_t *dst, size_t dst_stride, const uint64_t *src, size_t src_stride, size_t polys,
+                size_t limbs);
 void pt_copy(Plaintext &dst, const Plaintext &src);
 
 void op_add(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst);
@@ -306,6 +309,14 @@ void op_mod_raise(const Ciphertext &a, Ciphertext &dst);
 void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk,
                        std::vector<Ciphertext> &dst);
 void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk, Ciphertext &dst);
+// every input rotated by every step, hoisted per input, batched across inputs:
+// dst[b][k] = op_rotate_hoisted(in[b], steps)[k] word for word
+void op_rotate_hoisted_many(const std::vector<const Ciphertext *> &in, const std::vector<int> &steps,
+                            const KSwitchKeys &gk, std::vector<std::vector<Ciphertext>> &dst);
+// *dst[g] = op_rotate_sum(groups[g], steps[g]) word for word, the ModDowns batched
+void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &groups,
+                        const std::vector<std::vector<int>> &steps, const KSwitchKeys &gk,
+                        const std::vector<Ciphertext *> &dst);
 void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<int> &steps, const KSwitchKeys &gk,
                    Ciphertext &dst);
 

@@ -848,45 +848,221 @@ void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im,
     out = std::move(x);
 }
 
+std::vector<Ciphertext> Bootstrapper::apply_level_many(const LinearLevel &lv, int tag,
+                                                       const std::vector<const Ciphertext *> &xs,
+                                                       const std::vector<double> &out_scale,
+                                                       const std::vector<double> &value_factor)
+{
+    // apply_level over a batch of inputs of one level: the baby rotations of
+    // all inputs hoisted together, each input's giant-step multiply-accumulates
+    // in one pass, every input's giant sum with one ModDown (the ModDowns of
+    // the batch together), one batched rescale -- word for word apply_level
+    // of each input
+    const size_t B = xs.size(), l = xs[0]->coeff_modulus_size();
+    for (const Ciphertext *x : xs)
+        if (x->coeff_modulus_size() != l) throw std::invalid_argument("Bootstrapper: batch needs one level");
+    std::vector<int> bi, bsteps;
+    for (const auto &d : lv.diag) {
+        const int i = d.first.second;
+        if (std::find(bi.begin(), bi.end(), i) != bi.end()) continue;
+        bi.push_back(i);
+        bsteps.push_back(i * lv.stride);
+    }
+    std::vector<std::vector<Ciphertext>> rots;
+    ck_.evaluator.rotate_vector_hoisted_many(xs, bsteps, *ck_.galois_keys, rots);
+    std::vector<std::vector<Ciphertext>> accs(B);
+    std::vector<std::vector<const Ciphertext *>> groups(B);
+    std::vector<std::vector<int>> gsteps(B);
+    for (size_t b = 0; b < B; b++) {
+        const double sigma = out_scale[b] * prime(l) / xs[b]->scale();
+        std::map<int, Ciphertext> baby;
+        for (size_t k = 0; k < bi.size(); k++) baby.emplace(bi[k], std::move(rots[b][k]));
+        std::vector<const Ciphertext *> terms;
+        std::map<int, size_t> column;
+        for (auto &kv : baby) {
+            column[kv.first] = terms.size();
+            terms.push_back(&kv.second);
+        }
+        std::vector<const Plaintext *> weights;
+        auto it = lv.diag.begin();
+        while (it != lv.diag.end()) {
+            const int j = it->first.first;
+            std::vector<const Plaintext *> row(terms.size(), nullptr);
+            for (; it != lv.diag.end() && it->first.first == j; ++it) {
+                const int i = it->first.second;
+                auto key = std::make_tuple(tag, j, i, l, sigma * value_factor[b]);
+                auto pit = ptcache_.find(key);
+                if (pit == ptcache_.end()) {
+                    Plaintext pt(*ck_.context);
+                    ck_.encoder.encode(it->second, chain_of(l), sigma * value_factor[b], pt);
+                    pit = ptcache_.emplace(key, std::move(pt)).first;
+                }
+                row[column.at(i)] = &pit->second;
+            }
+            weights.insert(weights.end(), row.begin(), row.end());
+            gsteps[b].push_back(j * lv.g * lv.stride);
+        }
+        ck_.evaluator.multiply_plain_accumulate_multi(terms, weights, accs[b]);
+        for (const auto &a : accs[b]) groups[b].push_back(&a);
+    }
+    rots.clear();
+    std::vector<Ciphertext> sums;
+    ck_.evaluator.rotate_sum_many(groups, gsteps, *ck_.galois_keys, sums);
+    accs.clear();
+    std::vector<Ciphertext> res;
+    ck_.evaluator.rescale_to_next_many(ptrs(sums), res);
+    for (size_t b = 0; b < B; b++) res[b].set_scale(out_scale[b]);
+    return res;
+}
+
 void Bootstrapper::bootstrap_many(const std::vector<const Ciphertext *> &in, std::vector<Ciphertext> &out,
                                   const std::vector<double> &factors)
 {
-    if (in.size() != factors.size()) throw std::invalid_argument("Bootstrapper: one factor per input");
+    bootstrap_mixed(std::vector<Bootstrapper *>(in.size(), this), in, out, factors);
+}
+
+void Bootstrapper::bootstrap_mixed(const std::vector<Bootstrapper *> &bts, const std::vector<const Ciphertext *> &in,
+                                   std::vector<Ciphertext> &out, const std::vector<double> &factors)
+{
     const size_t B = in.size();
-    if (ck_.evaluator.per_op()) {
+    if (bts.size() != B || factors.size() != B)
+        throw std::invalid_argument("Bootstrapper: one bootstrapper and one factor per input");
+    if (!B) {
+        out.clear();
+        return;
+    }
+    for (size_t i = 0; i < B; i++) {
+        if (!bts[i]) throw std::invalid_argument("Bootstrapper: null bootstrapper");
+        if (!in[i] || !in[i]->get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
+    }
+    Bootstrapper &lead = *bts[0];
+    if (lead.ck_.evaluator.per_op()) {
         // per-op mode: one bootstrap per input
         std::vector<Ciphertext> res(B);
-        for (size_t i = 0; i < B; i++) {
-            if (!in[i] || !in[i]->get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
-            bootstrap(*in[i], res[i], factors[i]);
-        }
+        for (size_t i = 0; i < B; i++) bts[i]->bootstrap(*in[i], res[i], factors[i]);
         out = std::move(res);
         return;
     }
-    std::vector<double> input_scale(B);
-    std::vector<Ciphertext> u_re(B), u_im(B);
+    // the inputs of each bootstrapper (slot count) in order of first appearance
+    std::vector<Bootstrapper *> plans;
+    std::vector<std::vector<size_t>> members;
     for (size_t i = 0; i < B; i++) {
-        if (!in[i] || !in[i]->get()) throw std::invalid_argument("Bootstrapper: empty ciphertext");
-        input_scale[i] = in[i]->scale();
-        Ciphertext ct;
-        mod_raise(*in[i], ct);
-        if (plan_.logn < plan_.logNh) sub_sum(ct);
-        coeff_to_slot(ct, u_re[i], u_im[i]);
+        Bootstrapper *b = bts[i];
+        if (b->ck_.context->get() != lead.ck_.context->get())
+            throw std::invalid_argument("Bootstrapper: inputs on different contexts");
+        if (b->p_.K != lead.p_.K || b->p_.cheb_degree != lead.p_.cheb_degree ||
+            b->p_.double_angles != lead.p_.double_angles || b->p_.log_width != lead.p_.log_width)
+            throw std::invalid_argument("Bootstrapper: one EvalMod needs the same K, degree, double angles and width");
+        auto it = std::find(plans.begin(), plans.end(), b);
+        if (it == plans.end()) {
+            plans.push_back(b);
+            members.emplace_back();
+            it = plans.end() - 1;
+        }
+        members[it - plans.begin()].push_back(i);
     }
+    // the linear transforms run over chunks of at most kLevelBatch inputs of one
+    // bootstrapper (bounding the baby rotations and giant sums held at once)
+    constexpr size_t kLevelBatch = 16;
+    {
+        std::vector<Bootstrapper *> p2;
+        std::vector<std::vector<size_t>> m2;
+        for (size_t g = 0; g < plans.size(); g++)
+            for (size_t k0 = 0; k0 < members[g].size(); k0 += kLevelBatch) {
+                p2.push_back(plans[g]);
+                m2.emplace_back(members[g].begin() + k0,
+                                members[g].begin() + std::min(members[g].size(), k0 + kLevelBatch));
+            }
+        plans = std::move(p2);
+        members = std::move(m2);
+    }
+    const int64_t N = (int64_t)lead.ck_.context->poly_degree();
+    std::vector<Ciphertext> u_re(B), u_im(B);
+    for (size_t g = 0; g < plans.size(); g++) {
+        Bootstrapper &bt = *plans[g];
+        const std::vector<size_t> &idx = members[g];
+        const Evaluator &ev = bt.ck_.evaluator;
+        std::vector<Ciphertext> x(idx.size());
+        for (size_t k = 0; k < idx.size(); k++) bt.mod_raise(*in[idx[k]], x[k]);
+        if (bt.plan_.logn < bt.plan_.logNh)  // SubSum (Bootstrapper.cu:3004-3008), batched per step
+            for (int i = bt.plan_.logn; i < bt.plan_.logNh; i++) {
+                std::vector<std::vector<Ciphertext>> r;
+                ev.rotate_vector_hoisted_many(ptrs(x), {1 << i}, *bt.ck_.galois_keys, r);
+                for (size_t k = 0; k < idx.size(); k++) ev.add_inplace(x[k], r[k][0]);
+            }
+        for (size_t lv = 0; lv < bt.plan_.cts.size(); lv++) {
+            std::vector<double> sc, vf(idx.size(), 1.0);
+            for (const auto &c : x) sc.push_back(c.scale());
+            x = bt.apply_level_many(bt.plan_.cts[lv], (int)lv, ptrs(x), sc, vf);
+        }
+        for (size_t k = 0; k < idx.size(); k++) {
+            Ciphertext xc;
+            ev.complex_conjugate(x[k], *bt.ck_.galois_keys, xc);
+            const size_t i = idx[k];
+            ev.add(x[k], xc, u_re[i]);
+            if (bt.plan_.logn == bt.plan_.logNh) {
+                // dense packing: Im = -i (x - conj x), X^(3N/2) multiplies every slot by -i
+                Ciphertext d;
+                ev.sub(x[k], xc, d);
+                ev.multiply_power_of_x(d, 3 * N / 2, u_im[i]);
+            }
+        }
+    }
+    // one EvalMod over every half of every input
     std::vector<const Ciphertext *> u;
     for (size_t i = 0; i < B; i++) {
         u.push_back(&u_re[i]);
         if (u_im[i].get()) u.push_back(&u_im[i]);
     }
     std::vector<Ciphertext> f;
-    eval_mod_many(u, f);
+    lead.eval_mod_many(u, f);
     u_re.clear();
+    std::vector<Ciphertext> fre(B), fim(B);
+    {
+        size_t k = 0;
+        for (size_t i = 0; i < B; i++) {
+            fre[i] = std::move(f[k++]);
+            if (u_im[i].get()) fim[i] = std::move(f[k++]);
+        }
+    }
+    u_im.clear();
     std::vector<Ciphertext> res(B);
-    size_t k = 0;
-    for (size_t i = 0; i < B; i++) {
-        const Ciphertext &fr = f[k++];
-        const Ciphertext *fi = u_im[i].get() ? &f[k++] : nullptr;
-        slot_to_coeff(fr, fi, input_scale[i], res[i], factors[i]);
+    for (size_t g = 0; g < plans.size(); g++) {
+        Bootstrapper &bt = *plans[g];
+        const std::vector<size_t> &idx = members[g];
+        const Evaluator &ev = bt.ck_.evaluator;
+        std::vector<Ciphertext> x(idx.size());
+        std::vector<double> final_scale(idx.size()), last_vf(idx.size());
+        const double q0 = bt.prime(1);
+        for (size_t k = 0; k < idx.size(); k++) {
+            const size_t i = idx[k];
+            x[k] = std::move(fre[i]);
+            if (fim[i].get()) {
+                Ciphertext t;
+                ev.multiply_power_of_x(fim[i], N / 2, t);
+                t.set_scale(x[k].scale());
+                ev.add_inplace(x[k], t);
+            }
+            const double input_scale = in[i]->scale();
+            final_scale[k] = bt.p_.final_scale > 0 ? bt.p_.final_scale : input_scale;
+            last_vf[k] = q0 / input_scale * factors[i];
+        }
+        for (size_t lv = 0; lv < bt.plan_.stc.size(); lv++) {
+            const bool last = lv + 1 == bt.plan_.stc.size();
+            std::vector<double> sc(idx.size()), vf(idx.size(), 1.0);
+            for (size_t k = 0; k < idx.size(); k++) sc[k] = last ? final_scale[k] : x[k].scale();
+            if (last) vf = last_vf;
+            x = bt.apply_level_many(bt.plan_.stc[lv], 100 + (int)lv, ptrs(x), sc, vf);
+        }
+        if (bt.plan_.logn < bt.plan_.logNh) {
+            std::vector<std::vector<Ciphertext>> r;
+            ev.rotate_vector_hoisted_many(ptrs(x), {1 << bt.plan_.logn}, *bt.ck_.galois_keys, r);
+            for (size_t k = 0; k < idx.size(); k++) ev.add_inplace(x[k], r[k][0]);
+        }
+        for (size_t k = 0; k < idx.size(); k++) {
+            x[k].set_scale(final_scale[k]);
+            res[idx[k]] = std::move(x[k]);
+        }
     }
     out = std::move(res);
 }

@@ -127,6 +127,34 @@ int nx_bootstrap_many(nx_bootstrapper *bt, const nx_ciphertext *const *in, size_
     });
 }
 
+int nx_bootstrap_many_mixed(nx_bootstrapper *const *bts, const nx_ciphertext *const *in, size_t count,
+                            const double *factors, nx_ciphertext *const *out)
+{
+    return run([&] {
+        need(bts);
+        need(in);
+        need(out);
+        std::vector<nexus::Ciphertext> x;
+        std::vector<nexus::Bootstrapper *> b;
+        x.reserve(count);
+        for (size_t i = 0; i < count; i++) {
+            need(bts[i]);
+            need(out[i]);
+            if (bts[i]->ev->ctx->get() != bts[0]->ev->ctx->get())
+                throw std::invalid_argument("bootstrappers on different contexts");
+            x.push_back(input(bts[i], in[i]));
+            b.push_back(bts[i]->bt.get());
+        }
+        std::vector<const nexus::Ciphertext *> p;
+        for (auto &c : x) p.push_back(&c);
+        std::vector<double> f(count, 1.0);
+        if (factors) f.assign(factors, factors + count);
+        std::vector<nexus::Ciphertext> res;
+        nexus::Bootstrapper::bootstrap_mixed(b, p, res, f);
+        for (size_t i = 0; i < count; i++) store(res[i], out[i]);
+    });
+}
+
 int nx_boot_mod_raise(nx_bootstrapper *bt, const nx_ciphertext *in, nx_ciphertext *out)
 {
     return run([&] {

@@ -59,6 +59,14 @@ int nx_server_create_bert(nx_evaluator *ev, int lanes, size_t memory_budget, con
     });
 }
 
+int nx_server_set_bert_concurrency(nx_server *s, int n)
+{
+    return run([&] {
+        need(s);
+        s->impl->set_bert_concurrency(n);
+    });
+}
+
 int nx_server_destroy(nx_server *s)
 {
     return run([&] { delete s; });

@@ -647,6 +647,58 @@ const Plaintext &PeriodicPlaintexts::expanded(size_t index) const
     return expanded_.emplace(index, std::move(r)).first->second;
 }
 
+void Evaluator::rotate_vector_hoisted_many(const std::vector<const Ciphertext *> &cts, const std::vector<int> &steps,
+                                           const GaloisKeys &k, std::vector<std::vector<Ciphertext>> &dest) const
+{
+    std::vector<std::vector<Ciphertext>> out(cts.size());
+    if (per_op_) {
+        for (size_t b = 0; b < cts.size(); b++) rotate_vector_hoisted(*cts[b], steps, k, out[b]);
+        dest = std::move(out);
+        return;
+    }
+    std::vector<const nx_ciphertext *> in;
+    std::vector<nx_ciphertext *> h;
+    for (size_t b = 0; b < cts.size(); b++) {
+        in.push_back(cts[b]->get());
+        out[b].resize(steps.size());
+        for (auto &o : out[b]) {
+            o.bind(*ctx_);
+            h.push_back(o.get());
+        }
+    }
+    check(nx_rotate_hoisted_many(in.data(), in.size(), steps.data(), steps.size(), k.get(), h.data()));
+    dest = std::move(out);
+}
+
+void Evaluator::rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &groups,
+                                const std::vector<std::vector<int>> &steps, const GaloisKeys &k,
+                                std::vector<Ciphertext> &dest) const
+{
+    if (groups.size() != steps.size()) throw std::invalid_argument("rotate_sum_many: groups and steps must match");
+    std::vector<Ciphertext> out(groups.size());
+    if (per_op_) {
+        for (size_t g = 0; g < groups.size(); g++) rotate_sum(groups[g], steps[g], k, out[g]);
+        dest = std::move(out);
+        return;
+    }
+    std::vector<const nx_ciphertext *> in;
+    std::vector<int> st;
+    std::vector<size_t> sizes;
+    std::vector<nx_ciphertext *> h;
+    for (size_t g = 0; g < groups.size(); g++) {
+        if (groups[g].size() != steps[g].size()) throw std::invalid_argument("rotate_sum_many: inputs and steps");
+        for (size_t i = 0; i < groups[g].size(); i++) {
+            in.push_back(groups[g][i]->get());
+            st.push_back(steps[g][i]);
+        }
+        sizes.push_back(groups[g].size());
+        out[g].bind(*ctx_);
+        h.push_back(out[g].get());
+    }
+    check(nx_rotate_sum_many(in.data(), st.data(), sizes.data(), sizes.size(), k.get(), h.data()));
+    dest = std::move(out);
+}
+
 void Evaluator::hoisting_inplace(Ciphertext &ct, const GaloisKeys &k, const std::vector<int> &steps) const
 {
     if (per_op_) {

@@ -110,6 +110,16 @@ void Server::pause(bool paused)
     cv_work_.notify_all();
 }
 
+void Server::set_bert_concurrency(int n)
+{
+    if (n < 1 || n > opt_.lanes) throw std::invalid_argument("Server: BERT concurrency must be 1..lanes");
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        opt_.bert_concurrency = n;
+    }
+    cv_work_.notify_all();
+}
+
 void Server::drain()
 {
     std::unique_lock<std::mutex> lk(mu_);

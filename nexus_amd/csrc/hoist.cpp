@@ -91,6 +91,160 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
     dst = std::move(out);
 }
 
+void op_rotate_hoisted_many(const std::vector<const Ciphertext *> &in, const std::vector<int> &steps,
+                            const KSwitchKeys &gk, std::vector<std::vector<Ciphertext>> &dst)
+{
+    // every input rotated by every step, the steps of each input hoisted: the
+    // ModUps of all inputs in one batch, the (input, rotation) inner products
+    // up to 32 per launch, their ModDowns batched likewise -- word for word
+    // op_rotate_hoisted of each input
+    require(!in.empty(), "no inputs");
+    const Ciphertext &a0 = *in[0];
+    check_input(a0, gk);
+    Context &c = *a0.ctx;
+    const size_t n = c.n, l = a0.limbs, nout = l + c.alpha, pw = l * n, digits = c.digits(l);
+    for (const Ciphertext *a : in) {
+        check_input(*a, gk);
+        require(a->ctx == &c && a->limbs == l, "inputs must share the level");
+    }
+    Prepared p = prepare(c, steps, gk);
+    tally("ks_hoisted", l, (double)(p.keys.size() * in.size()));
+    const size_t B = in.size(), R = p.keys.size();
+    std::vector<std::vector<Ciphertext>> out(B);
+    for (auto &o : out) o.resize(steps.size());
+    for (size_t b = 0; b < B; b++)
+        for (size_t k = 0; k < steps.size(); k++)
+            if (steps[k] == 0) ct_copy(out[b][k], *in[b]);
+    if (R) {
+        uint8_t outp[kMaxPrimes];
+        ks_out_primes(c, l, outp);
+        const size_t tstride = nout * digits * n;
+        // ModUp of every input, up to kNttSrcMax per batch
+        std::vector<DevBuf> Ts;
+        std::vector<const uint64_t *> tp(B);
+        for (size_t b0 = 0; b0 < B; b0 += (size_t)kNttSrcMax) {
+            const size_t cnt = std::min<size_t>((size_t)kNttSrcMax, B - b0);
+            std::vector<const uint64_t *> c1s(cnt);
+            for (size_t b = 0; b < cnt; b++) c1s[b] = in[b0 + b]->data.get() + pw;
+            Ts.push_back(ks_modup_list(c, c1s.data(), cnt, l));
+            for (size_t b = 0; b < cnt; b++) tp[b0 + b] = Ts.back().get() + b * tstride;
+        }
+        // (input, rotation) pairs in chunks of kMaxHoist
+        const size_t P = B * R;
+        for (size_t base = 0; base < P; base += (size_t)kMaxHoist) {
+            const size_t cnt = std::min<size_t>((size_t)kMaxHoist, P - base);
+            std::vector<const uint64_t *> keys(cnt), tps(cnt);
+            std::vector<const uint32_t *> perms(cnt);
+            for (size_t r = 0; r < cnt; r++) {
+                const size_t b = (base + r) / R, k = (base + r) % R;
+                keys[r] = p.keys[k];
+                perms[r] = p.perms[k];
+                tps[r] = tp[b];
+            }
+            DevBuf acc(&c, cnt * 2 * nout * n);
+            ks_inner_product_rot(c.tables, acc.get(), nullptr, keys.data(), perms.data(), (uint32_t)cnt,
+                                 (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream,
+                                 tps.data());
+            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false);
+            NttEpilogue e;
+            for (size_t r = 0; r < cnt; r++) {
+                const size_t b = (base + r) / R, k = (base + r) % R;
+                Ciphertext &o = out[b][p.index[k]];
+                o.ctx = &c;
+                o.size = 2;
+                o.limbs = l;
+                o.scale = in[b]->scale;
+                o.ntt = true;
+                o.data.reset(&c, 2 * pw);
+                e.outs[r] = o.data.get();
+                e.perm[r] = perms[r];
+                e.a0s[r] = in[b]->data.get();
+            }
+            e.x = acc.get();
+            e.x_stride = nout * n;
+            e.inv = c.d_p_inv;
+            ntt_fwd_epi(c, tmp.get(), 2 * cnt, l, l * n, e);
+        }
+    }
+    dst = std::move(out);
+}
+
+void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &groups,
+                        const std::vector<std::vector<int>> &steps, const KSwitchKeys &gk,
+                        const std::vector<Ciphertext *> &dst)
+{
+    // several independent sums of rotations (one output per group), each as
+    // op_rotate_sum, with the ModDowns of all groups as one batch
+    require(!groups.empty() && groups.size() == steps.size() && dst.size() == groups.size(),
+            "groups, steps and outputs must match");
+    const Ciphertext &a0 = *groups[0].at(0);
+    check_input(a0, gk);
+    Context &c = *a0.ctx;
+    const size_t n = c.n, l = a0.limbs, nout = l + c.alpha, pw = l * n, digits = c.digits(l);
+    const size_t G = groups.size();
+    DevBuf outs(&c, G * 2 * pw), acc(&c, G * 2 * nout * n);
+    uint8_t outp[kMaxPrimes];
+    ks_out_primes(c, l, outp);
+    std::vector<char> switched(G, 0);
+    for (size_t g = 0; g < G; g++) {
+        const auto &in = groups[g];
+        require(!in.empty() && in.size() == steps[g].size(), "inputs and steps must match");
+        std::vector<const uint64_t *> ins, c1s, keys;
+        std::vector<const uint32_t *> perms, rperms;
+        for (size_t i = 0; i < in.size(); i++) {
+            check_input(*in[i], gk);
+            require(in[i]->ctx == &c && in[i]->limbs == l, "inputs must share the level");
+            require(are_close(in[i]->scale, a0.scale), "inputs must share the scale");
+            ins.push_back(in[i]->data.get());
+            if (steps[g][i] == 0) {
+                perms.push_back(nullptr);
+                continue;
+            }
+            Prepared p = prepare(c, std::vector<int>{steps[g][i]}, gk);
+            tally("ks_hoisted", l);
+            perms.push_back(p.perms[0]);
+            rperms.push_back(p.perms[0]);
+            keys.push_back(p.keys[0]);
+            c1s.push_back(in[i]->data.get() + pw);
+        }
+        uint64_t *og = outs.get() + g * 2 * pw;
+        DevBuf part;
+        for (size_t base = 0; base < ins.size(); base += kMaxHoist) {
+            const size_t cnt = std::min<size_t>(kMaxHoist, ins.size() - base);
+            uint64_t *o = base ? (part.reset(&c, 2 * pw), part.get()) : og;
+            rot_base(c.tables, o, ins.data() + base, perms.data() + base, (uint32_t)cnt, (uint32_t)l, c.stream);
+            if (base) poly_add(c.tables, og, og, o, 2, (uint32_t)l, c.stream);
+        }
+        uint64_t *ag = acc.get() + g * 2 * nout * n;
+        if (keys.empty()) {
+            // no switched term: a zero accumulator leaves the base unchanged
+            std::vector<uint64_t> zeros(nout, 0);
+            poly_fill_limbs(ag, zeros.data(), 2, (uint32_t)nout, c.logn, c.stream);
+            continue;
+        }
+        switched[g] = 1;
+        for (size_t base = 0; base < keys.size(); base += kMaxHoist) {
+            const size_t cnt = std::min<size_t>(kMaxHoist, keys.size() - base);
+            DevBuf T = ks_modup_list(c, c1s.data() + base, cnt, l);
+            ks_inner_product_rot_sum(c.tables, ag, T.get(), keys.data() + base, rperms.data() + base,
+                                     (uint32_t)cnt, (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
+                                     base > 0, c.stream, nout * digits * n);
+        }
+    }
+    ks_moddown(c, outs.get(), acc.get(), G, l, false);
+    for (size_t g = 0; g < G; g++) {
+        Ciphertext r;
+        r.ctx = &c;
+        r.size = 2;
+        r.limbs = l;
+        r.scale = a0.scale;
+        r.ntt = true;
+        r.data.reset(&c, 2 * pw);
+        copy_limbs(c, r.data.get(), 2 * pw, outs.get() + g * 2 * pw, 2 * pw, 1, 2 * l);
+        *dst[g] = std::move(r);
+    }
+}
+
 void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwitchKeys &gk, Ciphertext &dst)
 {
     // hoisting_inplace: sum_k rot(a, steps[k]) with one ModUp and one ModDown

@@ -60,6 +60,7 @@ struct NttEpilogue {
     uint64_t *outs[kEpiMax] = {};
     const uint32_t *perm[kEpiMax] = {};
     const uint64_t *a0 = nullptr;
+    const uint64_t *a0s[kEpiMax] = {};  // per output ciphertext: its input's c0 (else a0)
     const uint64_t *d = nullptr;
     uint64_t d_stride = 0;
     const ulonglong2 *inv2 = nullptr;
@@ -259,7 +260,8 @@ void block_mix(const DeviceTables &t, const MixPtrs &p, const uint64_t *table, c
 constexpr int kMaxHoist = 32;
 void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                           const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
-                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s,
+                          const uint64_t *const *Ts = nullptr);
 // t_stride > 0: rotation r reads its own ModUp at T + r t_stride (a sum of
 // rotations of different inputs) instead of the shared T.
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,

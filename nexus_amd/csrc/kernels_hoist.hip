@@ -20,10 +20,11 @@ struct OutPrimes {
 struct RotArgs {
     const uint64_t *key[kMaxHoist];
     const uint32_t *perm[kMaxHoist];
+    const uint64_t *t[kMaxHoist];  // k_ks_inner_rot: the ModUp digits of rotation r's input
 };
 
 // acc[r][k][I][c] = sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]  (one rotation r per grid row group)
-__global__ void k_ks_inner_rot(uint64_t *acc, const uint64_t *T, RotArgs args, const ModConst *mod, OutPrimes outp,
+__global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod, OutPrimes outp,
                                uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
@@ -33,7 +34,7 @@ __global__ void k_ks_inner_rot(uint64_t *acc, const uint64_t *T, RotArgs args, c
     const uint32_t kp = outp.p[I];
     const ModConst m = mod[kp];
     const uint32_t src = args.perm[r][c];
-    const uint64_t *t = T + (uint64_t)I * l * n + src;
+    const uint64_t *t = args.t[r] + (uint64_t)I * l * n + src;
     const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
     const uint64_t kstride = (uint64_t)key_limbs * n;
     uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
@@ -131,12 +132,15 @@ OutPrimes make_out(const uint8_t *p, uint32_t nout)
     return o;
 }
 
-RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uint32_t count)
+RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uint32_t count,
+                  const uint64_t *T = nullptr, const uint64_t *const *Ts = nullptr)
 {
+    if (count > (uint32_t)kMaxHoist) throw std::invalid_argument("hoisted inner product: too many rotations");
     RotArgs a;
     for (uint32_t i = 0; i < kMaxHoist; i++) {
         a.key[i] = i < count ? keys[i] : nullptr;
         a.perm[i] = i < count ? perms[i] : nullptr;
+        a.t[i] = i < count ? (Ts ? Ts[i] : T) : nullptr;
     }
     return a;
 }
@@ -145,12 +149,13 @@ RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uin
 
 void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                           const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
-                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s, const uint64_t *const *Ts)
 {
     const uint32_t n = 1u << t.logn;
     ProfScope prof("ks_inner_rot", s, (double)n * count * nout * (24.0 * l + 16.0 + 4.0));
-    hipLaunchKernelGGL(k_ks_inner_rot, dim3((n + kBlock - 1) / kBlock, count * nout), dim3(kBlock), 0, s, acc, T,
-                       make_args(keys, perms, count), t.mod, make_out(out_primes_host, nout), l, key_limbs, nout, n);
+    hipLaunchKernelGGL(k_ks_inner_rot, dim3((n + kBlock - 1) / kBlock, count * nout), dim3(kBlock), 0, s, acc,
+                       make_args(keys, perms, count, T, Ts), t.mod, make_out(out_primes_host, nout), l, key_limbs,
+                       nout, n);
 }
 
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,

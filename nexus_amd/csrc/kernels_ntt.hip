@@ -80,7 +80,8 @@ __device__ __forceinline__ void epi_store(const NttEpilogue &e, uint32_t p, uint
         *o = r;
         return;
     }
-    if (e.a0 && k == 0) r = add_mod(r, e.a0[(uint64_t)i * n + e.perm[ct][gidx]], q);
+    const uint64_t *a0 = e.a0s[0] ? e.a0s[ct] : e.a0;
+    if (a0 && k == 0) r = add_mod(r, a0[(uint64_t)i * n + e.perm[ct][gidx]], q);
     if (e.d) {
         const ulonglong2 w2 = e.inv2[i];
         r = add_mod(r, shoup(e.d[p * e.d_stride + off], w2.x, w2.y, q), q);
@@ -567,7 +568,7 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     // with an epilogue the last pass also reads x (and base or out) per word
     const bool epi = FWD && ROLE != kCol && epi_on(a.epi);
     const double epi_reads =
-        a.epi.out ? (a.epi.accumulate ? 1.0 : 0.0) : (a.epi.a0 ? 0.5 : 0.0) + (a.epi.d ? 1.0 : 0.0);
+        a.epi.out ? (a.epi.accumulate ? 1.0 : 0.0) : ((a.epi.a0 || a.epi.a0s[0]) ? 0.5 : 0.0) + (a.epi.d ? 1.0 : 0.0);
     const double epi_bytes = epi ? 8.0 * n * (1.0 + epi_reads) : 0.0;
     ProfScope prof(epi ? "ntt_fwd_sub_mul" : names[FWD ? 1 : 0][ROLE], s,
                    (double)(batch - a.skipped) * (16.0 * n + a.tw_bytes * tw + epi_bytes));

@@ -19,6 +19,8 @@ void d2d(Context &c, uint64_t *dst, const uint64_t *src, size_t words)
     if (words) NX_HIP(hipMemcpyAsync(dst, src, words * 8, hipMemcpyDeviceToDevice, c.stream));
 }
 
+}  // namespace
+
 // Copy `limbs` limbs of each of `polys` polys between differently strided buffers.
 void copy_limbs(Context &c, uint64_t *dst, size_t dst_stride, const uint64_t *src, size_t src_stride, size_t polys,
                 size_t limbs)
@@ -26,6 +28,8 @@ void copy_limbs(Context &c, uint64_t *dst, size_t dst_stride, const uint64_t *sr
     if (!polys || !limbs) return;
     poly_copy_limbs(c.tables, dst, dst_stride, src, src_stride, (uint32_t)polys, (uint32_t)limbs, c.stream);
 }
+
+namespace {
 
 bool scale_within_bounds(const Context &c, double scale, size_t limbs)
 {

@@ -214,8 +214,10 @@ def test_server_continuous_batching(env):
 
 def test_bert_base_full_size():
     """BERT-base (hidden 768, 12 heads, FFN 3072) at N = 2^16 with 128-token
-    sets, chain {51, 46 x 19, 51 x 14, 51 x 8}: a 40-token sequence and two
-    packed sequences against the FP64 model within 2^-10"""
+    sets, chain {51, 46 x 19, 51 x 14, 51 x 8}: a 40-token sequence, two packed
+    sequences, and four uneven requests served with continuous batching (two
+    calls in flight), each against the FP64 model (and its solo call) within
+    2^-10"""
     if nx.device_count() < 1:
         pytest.skip("no GPU visible")
     _note("bert-base full size keys")
@@ -249,3 +251,32 @@ def test_bert_base_full_size():
         e = np.max(np.abs(got[a:a + n] - ob.layer_approx(x[a:a + n], W)))
         print("bert-base packed (%d, %d): max err vs model %.3g" % (a, n, e))
         assert e < 2.0 ** -10
+
+    # served (continuous batching, two BERT calls in flight on two lanes): four
+    # requests of uneven length, each within 2^-10 of its solo call and the model
+    lens = [17, 45, 30, 64]
+    _note("bert-base served: requests of lengths %s" % lens)
+    reqs, xs = [], []
+    for k, n in enumerate(lens):
+        rows = np.zeros((128, 768))
+        rows[:n] = rng.normal(0, 1, (n, 768))
+        xs.append(rows[:n])
+        reqs.append([nx.encrypt(ctx, sk, nx.encode(ctx, v, SCALE, ctx.L - 1), seed=200 + 10 * k + i)
+                     for i, v in enumerate(nx.bert_pack(rows, ctx.slot_count))])
+    srv = nx.Server(ev, lanes=2, bert=layer, pack=True)
+    srv.set_bert_concurrency(2)
+    srv.pause(True)
+    ids = [srv.submit_bert(c, n) for c, n in zip(reqs, lens)]
+    srv.pause(False)
+    outs = [srv.wait(i) for i in ids]
+    st = srv.batching_stats()
+    del srv
+    print("bert-base served: %s" % st)
+    for k, n in enumerate(lens):
+        got = dec(outs[k])
+        solo = dec(layer.forward(reqs[k], n))[:n]
+        e_model = np.max(np.abs(got[:n] - ob.layer_approx(xs[k], W)))
+        e_solo = np.max(np.abs(got[:n] - solo))
+        print("served request %d (len %d): max err vs model %.3g, vs solo %.3g" % (k, n, e_model, e_solo))
+        assert e_model < 2.0 ** -10 and e_solo < 2.0 ** -10
+        assert np.max(np.abs(got[n:])) < 2.0 ** -10

@@ -257,3 +257,36 @@ def test_argmax_reference_config():
     print("argmax: out", np.round(got, 4), "max|out-model| %.3g MAE %.3g" % (np.max(np.abs(got - model)), mae))
     assert np.max(np.abs(got - model)) < 1e-2
     assert mae <= np.mean(np.abs(model - calib)) + 1e-3
+
+
+def test_bootstrap_mixed_matches_single():
+    """dense (2^11 slots) and sparse (2^10 slots) inputs at three input levels in
+    one batch through two bootstrappers of one context: every CoeffToSlot /
+    SlotToCoeff level batched per bootstrapper, one EvalMod for all; each output
+    equals its own single bootstrap word for word (one special prime)"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    _note("bootstrap mixed dense+sparse logN=12")
+    env = Env(12, 11, SMALL_CHAIN, 64)
+    sparse_steps = nx.boot_plan_steps(11, 10, K)
+    env.gk.generate_galois(env.sk, [env.ctx.elt_from_step(s) for s in sparse_steps if s not in env.steps], seed=5)
+    bt_sparse = nx.Bootstrapper(env.ev, log_slots=10, K=K)
+    rng = np.random.default_rng(78)
+    Nh = env.ctx.slot_count
+    cts, bts, msgs = [], [], []
+    for i in range(7):
+        dense = i % 3 != 2
+        n = Nh if dense else Nh // 2
+        v = rng.uniform(-1, 1, n)
+        c = env.enc(np.tile(v, Nh // n), SCALE)
+        cts.append(nx.mod_switch_to(c, env.L - (i % 3)))  # input levels 0, 1, 2
+        bts.append(env.bt if dense else bt_sparse)
+        msgs.append(v)
+    factors = [1.0, 0.5, 2.0, 1.0, 1.0, 0.25, 1.0]
+    got = nx.Bootstrapper.bootstrap_mixed(bts, cts, factors)
+    for c, b, f, m, v in zip(cts, bts, factors, got, msgs):
+        single = b.bootstrap(c, f)
+        assert np.array_equal(m.to_array(), single.to_array())
+        assert m.scale == single.scale
+        err = np.max(np.abs(env.dec(m)[:len(v)].real - f * v))
+        assert err < 1e-3, err

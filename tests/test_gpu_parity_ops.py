@@ -225,3 +225,27 @@ def test_apply_galois_full_size(nl_env):
     ct = nx.Ciphertext.from_array(ctx, data, 1, 2.0 ** 40)
     got = nx.apply_galois(ct, elt, gk).to_array()
     assert np.array_equal(got, programs.apply_galois(data, elt, key, ctx.moduli, 16))
+
+
+@pytest.mark.parametrize("alpha", [1, 2])
+def test_rotations_many_match_single(alpha):
+    """the batched hoisted rotations and sums of rotations equal the per-input
+    calls word for word"""
+    logn = 12
+    _note("rotations many alpha=%d" % alpha)
+    ctx = nx.Context(logn, [58, 40, 40, 40, 40] + [58] * alpha, special_primes=alpha)
+    sk = nx.SecretKey(ctx, seed=61)
+    steps = [0, 1, -2, 5, 64]
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps if s], seed=62)
+    rng = np.random.default_rng(63 + alpha)
+    cts = [nx.Ciphertext.from_array(ctx, _rand(rng, ctx.moduli, 2, 4, ctx.n), 2, 2.0 ** 40) for _ in range(9)]
+    many = nx.rotate_hoisted_many(cts, steps, gk)
+    for c, row in zip(cts, many):
+        single = nx.rotate_hoisted(c, steps, gk)
+        for a, b in zip(row, single):
+            assert np.array_equal(a.to_array(), b.to_array())
+    groups = [cts[:3], cts[3:8], [cts[8]]]
+    gsteps = [[1, 0, 5], [0, 0, 64, 1, -2], [0]]
+    sums = nx.rotate_sum_many(groups, gsteps, gk)
+    for g, s, got in zip(groups, gsteps, sums):
+        assert np.array_equal(got.to_array(), nx.rotate_sum(g, s, gk).to_array())
