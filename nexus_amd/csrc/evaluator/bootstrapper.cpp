@@ -391,11 +391,12 @@ void Bootstrapper::mod_raise(const Ciphertext &in, Ciphertext &out) const
 
 void Bootstrapper::sub_sum(Ciphertext &ct) const
 {
-    // Bootstrapper.cu:3004-3008
+    // Bootstrapper.cu:3004-3008; each rotation in its hoisted form (the
+    // digits sigma(lift(c1))), as the batched bootstrap rotates
     for (int i = plan_.logn; i < plan_.logNh; i++) {
-        Ciphertext rot;
-        ck_.evaluator.rotate_vector(ct, 1 << i, *ck_.galois_keys, rot);
-        ck_.evaluator.add_inplace(ct, rot);
+        std::vector<Ciphertext> rot;
+        ck_.evaluator.rotate_vector_hoisted(ct, {1 << i}, *ck_.galois_keys, rot);
+        ck_.evaluator.add_inplace(ct, rot[0]);
     }
 }
 
@@ -840,9 +841,9 @@ void Bootstrapper::slot_to_coeff(const Ciphertext &f_re, const Ciphertext *f_im,
                         last ? q0 / input_scale * factor : 1.0);
     }
     if (plan_.logn < plan_.logNh) {
-        Ciphertext r;
-        ck_.evaluator.rotate_vector(x, 1 << plan_.logn, *ck_.galois_keys, r);
-        ck_.evaluator.add_inplace(x, r);
+        std::vector<Ciphertext> r;
+        ck_.evaluator.rotate_vector_hoisted(x, {1 << plan_.logn}, *ck_.galois_keys, r);
+        ck_.evaluator.add_inplace(x, r[0]);
     }
     x.set_scale(final_scale);
     out = std::move(x);

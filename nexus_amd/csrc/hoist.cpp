@@ -194,7 +194,7 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
         for (size_t i = 0; i < in.size(); i++) {
             check_input(*in[i], gk);
             require(in[i]->ctx == &c && in[i]->limbs == l, "inputs must share the level");
-            require(are_close(in[i]->scale, a0.scale), "inputs must share the scale");
+            require(are_close(in[i]->scale, in[0]->scale), "inputs of a sum must share the scale");
             ins.push_back(in[i]->data.get());
             if (steps[g][i] == 0) {
                 perms.push_back(nullptr);
@@ -237,7 +237,7 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
         r.ctx = &c;
         r.size = 2;
         r.limbs = l;
-        r.scale = a0.scale;
+        r.scale = groups[g][0]->scale;
         r.ntt = true;
         r.data.reset(&c, 2 * pw);
         copy_limbs(c, r.data.get(), 2 * pw, outs.get() + g * 2 * pw, 2 * pw, 1, 2 * l);

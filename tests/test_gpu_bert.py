@@ -279,4 +279,5 @@ def test_bert_base_full_size():
         e_solo = np.max(np.abs(got[:n] - solo))
         print("served request %d (len %d): max err vs model %.3g, vs solo %.3g" % (k, n, e_model, e_solo))
         assert e_model < 2.0 ** -10 and e_solo < 2.0 ** -10
-        assert np.max(np.abs(got[n:])) < 2.0 ** -10
+        if st["packed_requests"] == len(lens):  # rows past a packed request are cleared
+            assert np.max(np.abs(got[n:])) < 2.0 ** -10

@@ -238,7 +238,9 @@ def test_rotations_many_match_single(alpha):
     steps = [0, 1, -2, 5, 64]
     gk = nx.KSwitchKeys(ctx).generate_galois(sk, [ctx.elt_from_step(s) for s in steps if s], seed=62)
     rng = np.random.default_rng(63 + alpha)
-    cts = [nx.Ciphertext.from_array(ctx, _rand(rng, ctx.moduli, 2, 4, ctx.n), 2, 2.0 ** 40) for _ in range(9)]
+    # groups at different scales, as the giant sums of differently scaled inputs
+    cts = [nx.Ciphertext.from_array(ctx, _rand(rng, ctx.moduli, 2, 4, ctx.n), 2, 2.0 ** 40 * (1 + (k >= 3)))
+           for k in range(9)]
     many = nx.rotate_hoisted_many(cts, steps, gk)
     for c, row in zip(cts, many):
         single = nx.rotate_hoisted(c, steps, gk)
@@ -248,4 +250,5 @@ def test_rotations_many_match_single(alpha):
     gsteps = [[1, 0, 5], [0, 0, 64, 1, -2], [0]]
     sums = nx.rotate_sum_many(groups, gsteps, gk)
     for g, s, got in zip(groups, gsteps, sums):
-        assert np.array_equal(got.to_array(), nx.rotate_sum(g, s, gk).to_array())
+        want = nx.rotate_sum(g, s, gk)
+        assert np.array_equal(got.to_array(), want.to_array()) and got.scale == want.scale
