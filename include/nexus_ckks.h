@@ -169,7 +169,7 @@ int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphe
                                    const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests);
 /* The same products with relinearisation and rescale in one rounding (no
  * counterpart in the reference): with hybrid key switching the key-switch
- * accumulator plus P (c0, c1) is divided by P q_(l-1) in one exact centred
+ * accumulator plus P (c0, c1) is divided by P q_(l-1) in one centred
  * base conversion, saving one NTT and two passes per product.  Each result
  * coefficient is within 1 of nx_multiply_relin_rescale_many's (the same
  * integer on every limb); the two differ only where the exact quotient lies

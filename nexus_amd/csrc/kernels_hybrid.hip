@@ -6,10 +6,11 @@
 //   modup:   T[I][j] = sum_{i < s_j} [x_{j alpha + i} Dhat_i^-1]_{q_{j alpha + i}} * (Dhat_i mod t_I)  mod t_I
 //   moddown: z[p][i] = sum_{k < alpha} v_k * (Phat_k mod q_i) - r * (P mod q_i)  mod q_i,
 //            v_k = [y_k Phat_k^-1]_{p_k},  r = round(sum_k v_k / p_k)
-// The ModDown conversion is exact and centred: sum_k v_k Phat_k - r P is the
-// representative of y in (-P/2, P/2] (the rounding r is taken in float64,
-// wrong only within 2^-40 of a half-integer), so (x - z) / P = round(x / P) as
-// in the single-special-prime path.  Each term is below 2^122 and at most 33
+// The ModDown conversion is centred: sum_k v_k Phat_k - r P is the
+// representative of y in (-P/2, P/2] whenever the float64 rounding r is right,
+// i.e. unless sum_k v_k / p_k lies within about 2^-40 of a half-integer (then
+// the representative is off by P and the rounding of x / P by one), so
+// (x - z) / P = round(x / P) as in the single-special-prime path.  Each term is below 2^122 and at most 33
 // terms are summed, so one 128-bit accumulator and one Barrett reduction per
 // output word suffice.
 #include "kernels.h"

@@ -321,7 +321,7 @@ DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool tr
     if (alpha == 1)  // SEAL: centred (rounded) division by p
         rns_center_last(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
                         c.stream);
-    else if (ks_conv_fused_ok((uint32_t)alpha))  // hybrid: exact centred conversion of the P part
+    else if (ks_conv_fused_ok((uint32_t)alpha))  // hybrid: centred conversion of the P part
         ks_moddown_convert_fused(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv,
                                  c.d_moddown_conv_d, (uint32_t)(2 * B), (uint32_t)l, (uint32_t)alpha, (uint32_t)L,
                                  c.stream);
@@ -632,7 +632,7 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
 {
     // The relinearised product (c0, c1) + ModDown_P(acc) is rescaled by q_(l-1)
     // in the same rounding: Y = acc + P (c0, c1) over Q_l P is divided by
-    // P' = P q_(l-1) with one exact centred conversion of its limbs
+    // P' = P q_(l-1) with one centred conversion of its limbs
     // (q_(l-1), p_0..p_(alpha-1)).  out_i = (acc_i - [Y]_(P'),i) P'^-1
     // + d_i q_(l-1)^-1, one rounding instead of two: each coefficient is within
     // 1 of op_multiply_relin_rescale_many's and the same integer on every limb.

@@ -52,3 +52,30 @@ def test_hybrid_switch_identity(L, alpha, l):
         err = _centred(orc.ntt_inverse(diff, logn, q), q)
         # noise: rounding times s plus sum_j d_j e_j / P; the moduli are 2^40
         assert max(abs(int(v)) for v in err) < 2 ** 8, (i, max(abs(int(v)) for v in err))
+
+
+def test_centred_conversion_against_exact_integers():
+    """the ModDown conversion's float64 quotient r = round(sum_k v_k / p_k)
+    gives the representative of y in (-P/2, P/2] -- checked against exact CRT
+    composition with Python integers on random residues; within about 2^-40 P
+    of P/2 it may give the other representative (off by P)"""
+    import math
+    ps = [int(p) for p in orc.coeff_modulus_create(1 << 6, [51] * 8)]
+    P = math.prod(ps)
+    t = int(orc.coeff_modulus_create(1 << 6, [46])[0])
+    rng = np.random.default_rng(8)
+    ys = [int(rng.integers(0, 1 << 62)) * int(rng.integers(0, 1 << 62)) % P for _ in range(3000)]
+    ys += [0, 1, P - 1, P // 3, 2 * P // 3]
+    vals = [np.array([y % p for y in ys], dtype=np.uint64) for p in ps]
+    got = hybrid._conv_centred(vals, ps, t)
+    want = [(y - P if y > P // 2 else y) % t for y in ys]
+    assert [int(g) for g in got] == want
+    # within about 2^-40 P of P/2 the float64 quotient may round to the other
+    # representative, y - P instead of y: the stated window, and nothing else
+    edge = [P // 2 + d for d in (-(P >> 45), -2, -1, 0, 1, 2, P >> 45)]
+    vals = [np.array([y % p for y in edge], dtype=np.uint64) for p in ps]
+    got = hybrid._conv_centred(vals, ps, t)
+    for y, g in zip(edge, got):
+        assert int(g) in ((y - P) % t, y % t)
+        if abs(y - P // 2) >= P >> 45:
+            assert int(g) == ((y - P if y > P // 2 else y) % t)
