@@ -202,6 +202,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     const size_t P = moduli.size();
     std::vector<ulonglong2> fwd(P * n), inv(P * n), ninv(P);
     std::vector<double> fwd_d(P * n), inv_d(P * n), ninv_d(P);  // exact for primes below 2^53
+    std::vector<double2> qd(P);
     fwd_host.resize(P);
     inv_host.resize(P);
     for (size_t i = 0; i < P; i++) {
@@ -216,6 +217,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
         uint64_t ni = host_inv_mod(n % q, q);
         ninv[i] = make_ulonglong2(ni, shoup_quotient(ni, q));
         ninv_d[i] = (double)ni;
+        qd[i] = make_double2((double)q, 1.0 / (double)q);
     }
     // rescale constants: row `l` (limbs before the rescale), column i < l-1
     std::vector<ulonglong2> resc(P * P, make_ulonglong2(0, 0));
@@ -246,6 +248,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     NX_HIP(hipMalloc(&d_inv_d, P * n * sizeof(double)));
     NX_HIP(hipMalloc(&d_ninv_d, P * sizeof(double)));
     NX_HIP(hipMalloc(&d_mod, P * sizeof(ModConst)));
+    NX_HIP(hipMalloc(&d_qd, P * sizeof(double2)));
     NX_HIP(hipMalloc(&d_rescale_inv, P * P * sizeof(ulonglong2)));
     NX_HIP(hipMalloc(&d_p_inv, P * sizeof(ulonglong2)));
     NX_HIP(hipMemcpy(d_fwd, fwd.data(), P * n * sizeof(ulonglong2), hipMemcpyHostToDevice));
@@ -255,6 +258,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     NX_HIP(hipMemcpy(d_inv_d, inv_d.data(), P * n * sizeof(double), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_ninv_d, ninv_d.data(), P * sizeof(double), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_mod, mods.data(), P * sizeof(ModConst), hipMemcpyHostToDevice));
+    NX_HIP(hipMemcpy(d_qd, qd.data(), P * sizeof(double2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_rescale_inv, resc.data(), P * P * sizeof(ulonglong2), hipMemcpyHostToDevice));
     NX_HIP(hipMemcpy(d_p_inv, pinv.data(), P * sizeof(ulonglong2), hipMemcpyHostToDevice));
 
@@ -275,6 +279,7 @@ Context::Context(int logn_, const std::vector<int> &bit_sizes, uint32_t hw, size
     tables.modup_conv_d = d_modup_conv_d;
     tables.mod_host = moduli.data();
     tables.mod = d_mod;
+    tables.qd = d_qd;
     upload_encoder_tables(*this);
 }
 
@@ -443,6 +448,7 @@ Context::~Context()
     (void)hipFree(d_inv_d);
     (void)hipFree(d_ninv_d);
     (void)hipFree(d_mod);
+    (void)hipFree(d_qd);
     (void)hipFree(d_rescale_inv);
     (void)hipFree(d_p_inv);
     (void)hipFree(d_enc_index);

@@ -174,6 +174,7 @@ public:
     DeviceTables tables;
     ulonglong2 *d_fwd = nullptr, *d_inv = nullptr, *d_ninv = nullptr;
     double *d_fwd_d = nullptr, *d_inv_d = nullptr, *d_ninv_d = nullptr;
+    double2 *d_qd = nullptr;  // [prime] {q, 1 / q} in double precision
     ModConst *d_mod = nullptr;
     // rescale: d_rescale_inv[l] = {q_{l-1}^-1 mod q_i, Shoup} for i < l-1 (row l of a (L+1)x(L+1) table)
     ulonglong2 *d_rescale_inv = nullptr;

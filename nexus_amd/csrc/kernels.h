@@ -25,6 +25,7 @@ struct DeviceTables {
     const double *ninv_d;    // [prime] n^-1 mod q as doubles
     const double *modup_conv_d = nullptr;  // hybrid ModUp factors as doubles (Context::d_modup_conv)
     const ModConst *mod;     // [prime]
+    const double2 *qd = nullptr;  // [prime] {q, 1 / q} in double precision (1 / q correctly rounded)
     const uint64_t *mod_host = nullptr;  // [prime] the primes, host copy (launch-time choices)
 };
 

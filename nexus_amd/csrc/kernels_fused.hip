@@ -23,8 +23,10 @@ struct InPrimes {
     uint8_t p[kRegAlpha + 1];
 };
 
+// Words, not bytes: a uniform index into the kernel argument then takes a
+// scalar load, which does not wait on the thread's outstanding stores.
 struct OutMap {
-    uint8_t p[kMaxPrimes];
+    uint32_t p[kMaxPrimes];
 };
 
 // thread (b, j, chunk, c): T[b][I][j][c] for the output primes I of its chunk
@@ -118,7 +120,8 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
                                                         const ulonglong2 *__restrict__ w,
                                                         const uint64_t *__restrict__ conv,
                                                         const double *__restrict__ convd,
-                                                        const ModConst *__restrict__ mod, OutMap outp,
+                                                        const ModConst *__restrict__ mod,
+                                                        const double2 *__restrict__ qd, OutMap outp,
                                                         uint32_t l, uint32_t digits, uint32_t nout,
                                                         uint32_t key_limbs, uint32_t n)
 {
@@ -148,7 +151,7 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
             const uint64_t qi = mod[first + i].q;
             if (digit_fp) {
                 // [x Dhat^-1]_q in double precision, mapped to [0, q)
-                const double q = (double)qi, qinv = 1.0 / q;
+                const double q = qd[first + i].x, qinv = qd[first + i].y;
                 double v = reduce_f(mulmod_f((double)x[(uint64_t)i * n], (double)wi.x, q, qinv), q, qinv);
                 v = v < 0 ? v + q : v;
                 yd[i] = v;
@@ -165,14 +168,20 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
         if (I - first < s) continue;  // the digit's own primes: copied in NTT form
         const uint32_t kp = outp.p[I];
         const uint64_t qk = mod[kp].q;
+        // Rows i >= s of a digit's factor block are zero in the tables (and y, yd
+        // are zero there), so every term is formed without a branch and the
+        // factor loads issue together.
         if (digit_fp && qk < kFpPrimeBound) {
             // sum of products in double precision, reduced every second term
             const double *fd = convd + (uint64_t)base * key_limbs + kp;
-            const double q = (double)qk, qinv = 1.0 / q;
+            const double q = qd[kp].x, qinv = qd[kp].y;
+            double f[A];
+#pragma unroll
+            for (int i = 0; i < A; i++) f[i] = fd[(uint64_t)i * key_limbs];
             double acc = 0.0;
 #pragma unroll
             for (int i = 0; i < A; i++) {
-                if ((uint32_t)i < s) acc += mulmod_f(yd[i], fd[(uint64_t)i * key_limbs], q, qinv);
+                acc += mulmod_f(yd[i], f[i], q, qinv);
                 if (i & 1) acc = reduce_f(acc, q, qinv);
             }
             acc = reduce_f(acc, q, qinv);
@@ -183,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_modup_conv_a(uint64_t *__restrict__ 
         const uint64_t *f = conv + (uint64_t)base * key_limbs + kp;
         uint64_t fv[A];
 #pragma unroll
-        for (int i = 0; i < A; i++) fv[i] = (uint32_t)i < s ? f[(uint64_t)i * key_limbs] : 0;
+        for (int i = 0; i < A; i++) fv[i] = f[(uint64_t)i * key_limbs];
         Sum3 acc;
 #pragma unroll
         for (int i = 0; i < A; i++) acc.mac(y[i], fv[i]);
@@ -205,7 +214,8 @@ __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict_
                                                           const ulonglong2 *__restrict__ w,
                                                           const uint64_t *__restrict__ conv,
                                                           const double *__restrict__ convd,
-                                                          const ModConst *__restrict__ mod, InPrimes inp, uint32_t l,
+                                                          const ModConst *__restrict__ mod,
+                                                          const double2 *__restrict__ qd, InPrimes inp, uint32_t l,
                                                           uint32_t L, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
@@ -228,12 +238,15 @@ __global__ __launch_bounds__(kBlock) void k_moddown_conv_a(uint64_t *__restrict_
     for (uint32_t i = chunk * kMdChunk; i < i_end; i++) {
         const uint64_t qi = mod[i].q;
         if (convd && qi < kFpPrimeBound) {
-            const double q = (double)qi, qinv = 1.0 / q;
+            const double q = qd[i].x, qinv = qd[i].y;
+            double f[A + 1];  // loaded together, ahead of the products
+#pragma unroll
+            for (int k = 0; k <= A; k++) f[k] = convd[(uint64_t)k * L + i];
             // - r (P mod q_i) as r (q_i - P mod q_i)
-            double s = mulmod_f((double)(uint32_t)r, convd[(uint64_t)A * L + i], q, qinv);
+            double s = mulmod_f((double)(uint32_t)r, f[A], q, qinv);
 #pragma unroll
             for (int k = 0; k < A; k++) {
-                s += mulmod_f((double)v[k], convd[(uint64_t)k * L + i], q, qinv);
+                s += mulmod_f((double)v[k], f[k], q, qinv);
                 if (k & 1) s = reduce_f(s, q, qinv);  // |s| stays below 4q < 2^53
             }
             s = reduce_f(s, q, qinv);
@@ -464,7 +477,7 @@ void ks_modup_convert_fused(const DeviceTables &t, uint64_t *T, const uint64_t *
         switch (alpha) {
 #define NX_A(A)                                                                                                     \
     case A:                                                                                                         \
-        hipLaunchKernelGGL(k_modup_conv_a<A>, grid_a, dim3(kBlock), 0, s, T, coef, w, conv, t.modup_conv_d, t.mod, o, l, \
+        hipLaunchKernelGGL(k_modup_conv_a<A>, grid_a, dim3(kBlock), 0, s, T, coef, w, conv, t.modup_conv_d, t.mod, t.qd, o, l, \
                            digits, nout, key_limbs, n);                                                             \
         return;
             NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
@@ -502,7 +515,7 @@ void ks_moddown_convert_fused(const DeviceTables &t, uint64_t *z, const uint64_t
 #define NX_A(A)                                                                                                  \
     case A:                                                                                                      \
         hipLaunchKernelGGL(k_moddown_conv_a<A>, grid_a, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, cd,     \
-                           t.mod, inp, l, L, n);                                                                 \
+                           t.mod, t.qd, inp, l, L, n);                                                           \
         return;
             NX_A(2) NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13)
             NX_A(14) NX_A(15) NX_A(16)
@@ -601,7 +614,7 @@ void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64
 #define NX_A(AA)                                                                                                  \
     case AA:                                                                                                      \
         hipLaunchKernelGGL(k_moddown_conv_a<AA>, grid, dim3(kBlock), 0, s, z, acc, acc_stride, w, conv, cd,       \
-                           t.mod, inp, l_out, L, n);                                                              \
+                           t.mod, t.qd, inp, l_out, L, n);                                                        \
         return;
         NX_A(3) NX_A(4) NX_A(5) NX_A(6) NX_A(7) NX_A(8) NX_A(9) NX_A(10) NX_A(11) NX_A(12) NX_A(13) NX_A(14)
         NX_A(15) NX_A(16)
