@@ -311,6 +311,31 @@ def bench_ntt(nx, reps):
                     "achieved_gbs": (b / max(launches, 1)) / (ms / max(launches, 1) * 1e-3) / 1e9}
     nx.profile_select("")
     del out, cp
+    # the BERT chain at its top compute level (one 51-bit and nineteen 46-bit
+    # primes) and at the bootstrapping top (34 data primes), both directions
+    bctx = nx.Context(16, BERT_CHAIN, special_primes=BERT_SPECIAL)
+    for limbs in (20, 34):
+        mods = bctx.moduli[:limbs]
+        d = np.stack([np.stack([rng.integers(0, q, bctx.n, dtype=np.uint64) for q in mods]) for _ in range(2)])
+        c = nx.Ciphertext.from_array(bctx, d, bctx.L + 1 - limbs, BERT_SCALE, ntt_form=False)
+        f = nx.transform_to_ntt(c)
+        g = nx.transform_from_ntt(f)  # first launches load the code objects: untimed
+        bctx.synchronize()
+        classes = ("ntt_fwd_col", "ntt_fwd_row", "ntt_inv_col", "ntt_inv_row")
+        nx.profile_select(",".join(classes))
+        for _ in range(reps):
+            f = nx.transform_to_ntt(c)
+            g = nx.transform_from_ntt(f)
+        bctx.synchronize()
+        sub = {"limbs_per_launch": 2 * limbs}
+        for cls in classes:
+            launches, ms, b = nx.profile_read(cls)
+            sub[cls] = {"launches": launches, "avg_us": ms / max(launches, 1) * 1e3,
+                        "algorithmic_bytes_per_launch": b / max(launches, 1),
+                        "achieved_gbs": (b / max(launches, 1)) / (ms / max(launches, 1) * 1e-3) / 1e9}
+        nx.profile_select("")
+        res["bert_chain_%d_limbs" % limbs] = sub
+        del c, f, g
     return res
 
 

@@ -235,7 +235,8 @@ __device__ __forceinline__ double reduce_f(double z, double q, double qinv)
 }
 
 // Forward radix-2^K step in double precision (values |v| <= q/2 (1 + eps)).
-template <int LOGS, int ROLE, int K, int CW>
+// LAZY: the sums and differences are left unreduced (see ntt_tile_f).
+template <int LOGS, int ROLE, int K, int CW, bool LAZY>
 __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw, double q, double qinv, uint32_t m0,
                                            uint32_t row0)
 {
@@ -276,8 +277,13 @@ __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw,
                     const int t = j * 2 * h + u;
                     const double x = v[t];
                     const double y = mulmod_f(v[t + h], w, q, qinv);
-                    v[t] = reduce_f(x + y, q, qinv);
-                    v[t + h] = reduce_f(x - y, q, qinv);
+                    if (LAZY) {
+                        v[t] = x + y;
+                        v[t + h] = x - y;
+                    } else {
+                        v[t] = reduce_f(x + y, q, qinv);
+                        v[t + h] = reduce_f(x - y, q, qinv);
+                    }
                 }
             }
         }
@@ -286,8 +292,9 @@ __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw,
     }
 }
 
-// Inverse radix-2^K step in double precision.
-template <int LOGS, int ROLE, int K, int CW>
+// Inverse radix-2^K step in double precision.  LAZY: the products are left
+// unreduced (|v| <= 1.07 q; see ntt_tile_f).
+template <int LOGS, int ROLE, int K, int CW, bool LAZY>
 __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw, double q, double qinv, uint32_t n,
                                            uint32_t mx, uint32_t row0)
 {
@@ -329,7 +336,8 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
                     const int t = j * (2 << d) + u;
                     const double x = v[t], y = v[t + (1 << d)];
                     v[t] = reduce_f(x + y, q, qinv);
-                    v[t + (1 << d)] = reduce_f(mulmod_f(x - y, w, q, qinv), q, qinv);
+                    const double p = mulmod_f(x - y, w, q, qinv);
+                    v[t + (1 << d)] = LAZY ? p : reduce_f(p, q, qinv);
                 }
             }
         }
@@ -338,8 +346,62 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
     }
 }
 
+// The radix steps of one pass in double precision, barrier after each.
+template <int LOGS, int ROLE, bool FWD, int CW, bool LAZY>
+__device__ __forceinline__ void stages_f(double *lds, const double *tw, double q, double qinv, uint32_t n,
+                                         uint32_t N1, uint32_t first)
+{
+    int s = 0;
+    if (FWD) {
+        const uint32_t m0 = ROLE == kRow ? N1 : 1u;
+        if constexpr (LOGS >= 3) {
+#pragma unroll
+            for (int st = 0; st < LOGS / 3; st++) {
+                fwd_step_f<LOGS, ROLE, 3, CW, LAZY>(lds, s, tw, q, qinv, m0, first);
+                s += 3;
+                __syncthreads();
+            }
+        }
+        if constexpr (LOGS % 3 == 2) {
+            fwd_step_f<LOGS, ROLE, 2, CW, LAZY>(lds, s, tw, q, qinv, m0, first);
+            __syncthreads();
+        } else if constexpr (LOGS % 3 == 1) {
+            fwd_step_f<LOGS, ROLE, 1, CW, LAZY>(lds, s, tw, q, qinv, m0, first);
+            __syncthreads();
+        }
+    } else {
+        const uint32_t mx = ROLE == kCol ? N1 : n;
+        if constexpr (LOGS >= 3) {
+#pragma unroll
+            for (int st = 0; st < LOGS / 3; st++) {
+                inv_step_f<LOGS, ROLE, 3, CW, LAZY>(lds, s, tw, q, qinv, n, mx, first);
+                s += 3;
+                __syncthreads();
+            }
+        }
+        if constexpr (LOGS % 3 == 2) {
+            inv_step_f<LOGS, ROLE, 2, CW, LAZY>(lds, s, tw, q, qinv, n, mx, first);
+            __syncthreads();
+        } else if constexpr (LOGS % 3 == 1) {
+            inv_step_f<LOGS, ROLE, 1, CW, LAZY>(lds, s, tw, q, qinv, n, mx, first);
+            __syncthreads();
+        }
+    }
+}
+
 // The whole transform body of one tile in double precision: load (reduced),
 // the radix steps, store in [0, q) (times n^-1 on the inverse's last pass).
+//
+// Lazy reduction, for primes with q (1/2 + 5/4 LOGS) <= 2^50.  Let B bound |v|.
+// mulmod_f(a, w) for |a| <= 2^50, 0 <= w < q: |h / q - a w / q| <= 3.3 2^-53
+// |a w / q| <= 0.41, so |Q - a w / q| <= 0.91 and the result is below 0.91 q
+// (every intermediate stays an integer below 2^53, as for reduced inputs).
+// Forward (Cooley-Tukey): x +- mulmod_f(y, w) grows B by at most 0.91 q a
+// stage from q/2 after the load, so B <= q (1/2 + 0.91 LOGS) <= 2^50 throughout
+// and no sum needs reducing; the store reduces once.  Inverse
+// (Gentleman-Sande): the sums are reduced (|x + y| <= 1.82 q), the products
+// are not (<= 0.91 q); the store (or the n^-1 product) reduces once.  The
+// stored words are the fully reduced residues either way.
 template <int LOGS, int ROLE, bool FWD, int CW>
 __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_words, uint64_t *data,
                                            const uint64_t *src, uint32_t prime, uint32_t n, uint32_t N2, uint32_t N1,
@@ -364,43 +426,11 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
         lds[e] = reduce_f((double)src[gidx], q, qinv);
     }
     __syncthreads();
-    if (FWD) {
-        const uint32_t m0 = ROLE == kRow ? N1 : 1u;
-        int s = 0;
-        if constexpr (LOGS >= 3) {
-#pragma unroll
-            for (int st = 0; st < LOGS / 3; st++) {
-                fwd_step_f<LOGS, ROLE, 3, CW>(lds, s, tw, q, qinv, m0, first);
-                s += 3;
-                __syncthreads();
-            }
-        }
-        if constexpr (LOGS % 3 == 2) {
-            fwd_step_f<LOGS, ROLE, 2, CW>(lds, s, tw, q, qinv, m0, first);
-            __syncthreads();
-        } else if constexpr (LOGS % 3 == 1) {
-            fwd_step_f<LOGS, ROLE, 1, CW>(lds, s, tw, q, qinv, m0, first);
-            __syncthreads();
-        }
-    } else {
-        const uint32_t mx = ROLE == kCol ? N1 : n;
-        int s = 0;
-        if constexpr (LOGS >= 3) {
-#pragma unroll
-            for (int st = 0; st < LOGS / 3; st++) {
-                inv_step_f<LOGS, ROLE, 3, CW>(lds, s, tw, q, qinv, n, mx, first);
-                s += 3;
-                __syncthreads();
-            }
-        }
-        if constexpr (LOGS % 3 == 2) {
-            inv_step_f<LOGS, ROLE, 2, CW>(lds, s, tw, q, qinv, n, mx, first);
-            __syncthreads();
-        } else if constexpr (LOGS % 3 == 1) {
-            inv_step_f<LOGS, ROLE, 1, CW>(lds, s, tw, q, qinv, n, mx, first);
-            __syncthreads();
-        }
-    }
+    const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform: one prime per workgroup
+    if (lazy)
+        stages_f<LOGS, ROLE, FWD, CW, true>(lds, tw, q, qinv, n, N1, first);
+    else
+        stages_f<LOGS, ROLE, FWD, CW, false>(lds, tw, q, qinv, n, N1, first);
     const bool scale_out = !FWD && ROLE != kRow;  // the inverse's last pass
     const double ni = a.ninvd[prime];
     for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
@@ -414,7 +444,10 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
             gidx = e;
         }
         double v = lds[e];
-        if (scale_out) v = reduce_f(mulmod_f(v, ni, q, qinv), q, qinv);
+        if (scale_out)
+            v = reduce_f(mulmod_f(v, ni, q, qinv), q, qinv);
+        else if (lazy)
+            v = reduce_f(v, q, qinv);
         v = v < 0 ? v + q : v;
         if (epi)
             epi_store(a.epi, poly, limb, gidx, (uint64_t)v, (uint64_t)q, a.limbs, n);
