@@ -459,7 +459,8 @@ def bench_bert_call(nx, reps):
     ms = (time.perf_counter() - t0) / reps * 1e3
     B["layer"].set_stage_timing(True)
     B["layer"].forward(cts, BERT_TOKENS)
-    return {"ms_per_layer_call": ms, "tokens": BERT_TOKENS, "stages_ms": dict(B["layer"].stage_ms())}
+    return {"ms_per_layer_call": ms, "tokens": BERT_TOKENS, "stages_ms": dict(B["layer"].stage_ms()),
+            "autotune_modup": B["autotune"].get("modup", []), "autotune_moddown": B["autotune"].get("moddown", [])}
 
 
 def pack_sets(lens, capacity):
@@ -704,6 +705,16 @@ def main():
                                                                          e["us"], e["default_us"])
                               for e in B["autotune"]["ntt"] if e["col"] or e["row"]],
                 "dot_plain_layouts": ["%d limbs: %d" % (d["limbs"], d["layout"]) for d in B["autotune"]["dot_plain"]],
+                "modup": ["batch %d at %d limbs: %s (%.1f vs %.1f us unfused)"
+                          % (e["batch"], e["limbs"], "unfused" if e["mode"] < 0 else
+                             "conversion in the column pass, %s outputs per workgroup"
+                             % ("default" if e["mode"] == 0 else e["mode"]), e["us"], e["unfused_us"])
+                          for e in B["autotune"].get("modup", [])],
+                "moddown": ["batch %d at %d limbs: %s (%.1f vs %.1f us unfused)"
+                            % (e["batch"], e["limbs"], "unfused" if e["mode"] < 0 else
+                               "conversion in the column pass, %s outputs per workgroup"
+                               % ("default" if e["mode"] == 0 else e["mode"]), e["us"], e["unfused_us"])
+                            for e in B["autotune"].get("moddown", [])],
             },
         },
         "roofline": {

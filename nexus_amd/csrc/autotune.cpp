@@ -183,6 +183,113 @@ std::string autotune(Context &c, int reps)
             first_entry = false;
         }
     }
+    rep += "], \"modup\": [";
+    first_entry = true;
+    if (c.alpha > 1 && ks_modup_col_ok(c.tables, (uint32_t)c.alpha)) {
+        // hybrid ModUp at the top level: the conversion fused into the forward
+        // column pass (outputs per workgroup: default rule, 4, 8, 12, all of a
+        // digit) against separate conversion and column-pass launches
+        const size_t l = c.L;
+        for (uint32_t B : {1u, 2u, 4u, 8u, 16u, 32u}) {
+            std::vector<uint64_t> h;
+            fill_random(c, h, B * l, 300 + B, l);
+            DevBuf tg(&c, B * l * n);
+            NX_HIP(hipMemcpyAsync(tg.get(), h.data(), h.size() * 8, hipMemcpyHostToDevice, c.stream));
+            NX_HIP(hipStreamSynchronize(c.stream));
+            const size_t words = B * (l + c.alpha) * c.digits(l) * n;
+            auto run = [&] { (void)ks_modup(c, tg.get(), B, l); };
+            ks_modup_col_set(B, -1);
+            std::vector<uint64_t> ref;
+            {
+                DevBuf T = ks_modup(c, tg.get(), B, l);
+                ref = download(c, T.get(), words);
+            }
+            const double t_off = time_ms(run);
+            double best = t_off;
+            int best_m = -1;
+            std::string times;
+            char buf[64];
+            for (int m : {0, 4, 8, 12, 64}) {
+                ks_modup_col_set(B, m);
+                DevBuf T = ks_modup(c, tg.get(), B, l);
+                const bool same = download(c, T.get(), words) == ref;
+                const double t = same ? time_ms(run) : 1e30;
+                std::snprintf(buf, sizeof buf, ", %.2f", same ? t * 1e3 : -1.0);
+                times += buf;
+                if (t < best * 0.98) {
+                    best = t;
+                    best_m = m;
+                }
+            }
+            ks_modup_col_set(B, best_m);
+            char line[300];
+            std::snprintf(line, sizeof line,
+                          "%s{\"batch\": %u, \"limbs\": %zu, \"mode\": %d, \"us\": %.2f, \"unfused_us\": %.2f, "
+                          "\"fused_us\": [%s]}",
+                          first_entry ? "" : ", ", B, l, best_m, best * 1e3, t_off * 1e3, times.c_str() + 2);
+            rep += line;
+            first_entry = false;
+        }
+    }
+    rep += "], \"moddown\": [";
+    first_entry = true;
+    if (c.alpha > 1 && c.d_moddown_conv_d && ks_moddown_col_ok(c.tables, (uint32_t)c.alpha)) {
+        // hybrid ModDown at the top level (B ciphertexts, 2B polys): the centred
+        // conversion fused into the forward column pass against separate launches
+        const size_t l = c.L, nout = l + c.alpha;
+        std::vector<uint64_t> pr(nout);
+        for (size_t i = 0; i < nout; i++) pr[i] = i < l ? i : c.L + (i - l);
+        for (uint32_t B : {1u, 2u, 4u, 8u, 16u}) {
+            const size_t aw = (size_t)B * 2 * nout * n, ow = (size_t)B * 2 * l * n;
+            std::vector<uint64_t> h(aw);
+            uint64_t x = 0x2545F4914F6CDD1Dull + B;
+            for (size_t p = 0; p < (size_t)B * 2; p++)
+                for (size_t i = 0; i < nout; i++) {
+                    const uint64_t q = c.moduli[pr[i]];
+                    for (size_t k = 0; k < n; k++) {
+                        x ^= x << 13;
+                        x ^= x >> 7;
+                        x ^= x << 17;
+                        h[(p * nout + i) * n + k] = x % q;
+                    }
+                }
+            DevBuf src(&c, aw), acc(&c, aw), out(&c, ow);
+            NX_HIP(hipMemcpyAsync(src.get(), h.data(), aw * 8, hipMemcpyHostToDevice, c.stream));
+            auto run = [&] {
+                NX_HIP(hipMemcpyAsync(acc.get(), src.get(), aw * 8, hipMemcpyDeviceToDevice, c.stream));
+                NX_HIP(hipMemsetAsync(out.get(), 0, ow * 8, c.stream));
+                ks_moddown(c, out.get(), acc.get(), B, l, false, nullptr);
+            };
+            ks_moddown_col_set(2 * B, -1);
+            run();
+            const std::vector<uint64_t> ref = download(c, out.get(), ow);
+            const double t_off = time_ms(run);
+            double best = t_off;
+            int best_m = -1;
+            std::string times;
+            char buf[64];
+            for (int m : {0, 4, 8, 12, 64}) {
+                ks_moddown_col_set(2 * B, m);
+                run();
+                const bool same = download(c, out.get(), ow) == ref;
+                const double t = same ? time_ms(run) : 1e30;
+                std::snprintf(buf, sizeof buf, ", %.2f", same ? t * 1e3 : -1.0);
+                times += buf;
+                if (t < best * 0.98) {
+                    best = t;
+                    best_m = m;
+                }
+            }
+            ks_moddown_col_set(2 * B, best_m);
+            char line[300];
+            std::snprintf(line, sizeof line,
+                          "%s{\"batch\": %u, \"limbs\": %zu, \"mode\": %d, \"us\": %.2f, \"unfused_us\": %.2f, "
+                          "\"fused_us\": [%s]}",
+                          first_entry ? "" : ", ", B, l, best_m, best * 1e3, t_off * 1e3, times.c_str() + 2);
+            rep += line;
+            first_entry = false;
+        }
+    }
     rep += "]}";
     NX_HIP(hipStreamSynchronize(c.stream));
     return rep;

@@ -533,10 +533,12 @@ void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly
     ntt_forward(c.tables, make_batch(data, polys, limbs, poly_stride), c.stream);
 }
 
-void ntt_fwd_epi(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride, const NttEpilogue &epi)
+void ntt_fwd_epi(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride, const NttEpilogue &epi,
+                 bool rows_only)
 {
     NttBatch b = make_batch(data, polys, limbs, poly_stride);
     b.epi = epi;
+    b.rows_only = rows_only ? 1 : 0;
     ntt_forward(c.tables, b, c.stream);
 }
 

@@ -240,7 +240,9 @@ private:
 // NTT of a [polys][limbs][n] buffer whose limb i uses prime i.
 void ntt_fwd(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
 // forward NTT whose last pass stores through epi instead of into data
-void ntt_fwd_epi(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride, const NttEpilogue &epi);
+// rows_only: the column pass has already run on data (a fused conversion)
+void ntt_fwd_epi(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride, const NttEpilogue &epi,
+                 bool rows_only = false);
 void ntt_inv(Context &c, uint64_t *data, size_t polys, size_t limbs, size_t poly_stride);
 // out of place: src (poly stride src_stride) -> dst (poly stride dst_stride)
 void ntt_fwd_from(Context &c, uint64_t *dst, const uint64_t *src, size_t polys, size_t limbs, size_t dst_stride,
@@ -333,7 +335,12 @@ DevBuf ks_modup_list(Context &c, const uint64_t *const *targets, size_t B, size_
 // ModDown up to the final combination: INTT of the special limbs of acc
 // [B][2][l+alpha][n], conversion, NTT; returns tmp [B][2][l][n] for
 // out = (acc - tmp) P^-1.
-DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool transform = true);
+// col_done non-null: the conversion may run fused into the forward column
+// pass, and *col_done says whether it did (the caller's forward NTT of the
+// result then runs its row pass only).  transform: the whole forward NTT runs
+// here.
+DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool transform = true,
+                          bool *col_done = nullptr);
 // base (optional, [B][2][l][n]): the accumulated terms are read from base instead of cts.
 void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bool zero_c1,
                 const uint64_t *base = nullptr);

@@ -66,7 +66,8 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
             ks_inner_product_rot(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
                                  (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
                                  c.stream);
-            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false);
+            bool col_done = false;
+            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false, &col_done);
             // each result written in place by the transform's last pass:
             // c0 = sigma(c0) + ks0, c1 = ks1
             NttEpilogue e;
@@ -85,7 +86,7 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
             e.x_stride = nout * n;
             e.inv = c.d_p_inv;
             e.a0 = a.data.get();
-            ntt_fwd_epi(c, tmp.get(), 2 * cnt, l, l * n, e);
+            ntt_fwd_epi(c, tmp.get(), 2 * cnt, l, l * n, e, col_done);
         }
     }
     dst = std::move(out);
@@ -145,7 +146,8 @@ void op_rotate_hoisted_many(const std::vector<const Ciphertext *> &in, const std
             ks_inner_product_rot(c.tables, acc.get(), nullptr, keys.data(), perms.data(), (uint32_t)cnt,
                                  (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream,
                                  tps.data());
-            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false);
+            bool col_done = false;
+            DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false, &col_done);
             NttEpilogue e;
             for (size_t r = 0; r < cnt; r++) {
                 const size_t b = (base + r) / R, k = (base + r) % R;
@@ -163,7 +165,7 @@ void op_rotate_hoisted_many(const std::vector<const Ciphertext *> &in, const std
             e.x = acc.get();
             e.x_stride = nout * n;
             e.inv = c.d_p_inv;
-            ntt_fwd_epi(c, tmp.get(), 2 * cnt, l, l * n, e);
+            ntt_fwd_epi(c, tmp.get(), 2 * cnt, l, l * n, e, col_done);
         }
     }
     dst = std::move(out);

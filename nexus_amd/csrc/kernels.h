@@ -87,6 +87,9 @@ struct NttBatch {
     // when I = p % poly_period < skip_l and I / skip_alpha == j (0: none)
     uint32_t skip_l = 0, skip_alpha = 0;
     NttEpilogue epi;  // forward transforms only
+    // forward, n >= 2^14: the column pass has already run (ks_modup_convert_col);
+    // only the row pass is launched
+    uint32_t rows_only = 0;
 };
 constexpr int kNttSrcMax = 32;
 
@@ -160,6 +163,13 @@ void poly_add_scalar(const DeviceTables &t, uint64_t *out, const uint64_t *a, co
 // NTT of tmp combines them with x through an NttEpilogue.
 void rns_center_last(const DeviceTables &t, uint64_t *tmp, const uint64_t *last, uint64_t last_stride,
                      uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s);
+// rns_center_last fused into the forward NTT's column pass: tmp[p][i] holds the
+// column-transformed centred residues; the caller's forward NTT runs its row
+// pass only (NttBatch::rows_only).  Needs center_col_ok: every prime below
+// 2^51, 2^14 <= n <= 2^17.
+bool center_col_ok(const DeviceTables &t);
+void rns_center_last_col(const DeviceTables &t, uint64_t *tmp, const uint64_t *last, uint64_t last_stride,
+                         uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s);
 // Key-switch ModUp over a batch of B targets: for ciphertext b, every output
 // prime I in out_primes (count = nout) and digit j < l,
 // T[b][I][j] = coef[b][j] mod q_I  (coef = INTT of the target, [B][l][n]).
@@ -205,6 +215,33 @@ bool ks_moddown_rescale_ok(const DeviceTables &t, uint32_t alpha);
 void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targets,
                        const uint64_t *const *target_list, uint32_t B, uint32_t l, uint32_t alpha, uint32_t digits,
                        uint32_t nout, hipStream_t s);
+// ModUp conversion fused into the forward NTT's column pass: T[b][I][j] for
+// the output primes I outside digit j, column-transformed (the caller runs the
+// row pass with NttBatch::rows_only); the digit's own limbs are not written
+// (ks_modup_copy_own).  Needs ks_modup_col_ok: every prime below 2^51,
+// 2 <= alpha <= 8, 2^14 <= n <= 2^17, the double-precision factor table.
+bool ks_modup_col_ok(const DeviceTables &t, uint32_t alpha);
+// per power-of-two bucket of B: 0 = fused with the default outputs per
+// workgroup, -1 = unfused, k > 0 = fused with k outputs per workgroup
+void ks_modup_col_set(uint32_t B, int mode);
+int ks_modup_col_get(uint32_t B);
+void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
+                          uint32_t B, uint32_t l, uint32_t alpha, uint32_t key_limbs, const uint8_t *out_primes_host,
+                          uint32_t nout, hipStream_t s);
+// ModDown conversion fused into the forward column pass: the A coefficient-form
+// limbs lo .. lo + A - 1 of each poly of acc (primes in_primes_host, Shoup
+// constants w, double factor table convd with rows k <= A of stride L as in
+// Context::d_moddown_conv_d / d_mdr_conv_d) converted, centred, to the primes
+// q_0 .. q_(lo-1) and column-transformed into z[polys][lo][n]; the caller runs
+// the row pass (NttBatch::rows_only), with its epilogue.  Needs
+// ks_moddown_col_ok: every prime below 2^51, 2 <= A <= 9, 2^14 <= n <= 2^17.
+bool ks_moddown_col_ok(const DeviceTables &t, uint32_t A);
+void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                            const ulonglong2 *w, const double *convd, const uint8_t *in_primes_host, uint32_t A,
+                            uint32_t polys, uint32_t lo, uint32_t L, hipStream_t s);
+// per power-of-two bucket of the poly count: as ks_modup_col_set
+void ks_moddown_col_set(uint32_t polys, int mode);
+int ks_moddown_col_get(uint32_t polys);
 void ks_moddown_rescale_convert(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
                                 const ulonglong2 *w, const uint64_t *conv, const double *convd, uint32_t polys,
                                 uint32_t l_out, uint32_t alpha, uint32_t L, hipStream_t s);

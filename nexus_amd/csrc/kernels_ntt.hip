@@ -27,8 +27,10 @@
 // [0, q), so the output words equal the integer path's.  Larger primes use
 // 64-bit Shoup butterflies (ten 32-bit multiplies each) on [0, 4q).  This
 // file is compiled without floating-point contraction.
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
+#include <stdexcept>
 
 #include "kernels.h"
 #include "prof.h"
@@ -577,6 +579,272 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
     ntt_pass<LOGS, ROLE, FWD, CW>(a);
 }
 
+// ---- ModUp base conversion fused into the forward column pass
+//
+// Workgroup (tile, b, digit j, chunk): the digit's A input limbs are read once
+// over the tile (8 columns x 2^LOGS rows) and their products y_k = [x_k
+// Dhat_k^-1]_(q_k) kept in registers; then for each output prime I of the
+// chunk (the primes outside the digit) the converted values sum_k y_k f_(k,I)
+// are written to LDS and transformed by the column stages of prime I, and the
+// column-transformed limb is stored into T[b][I][j].  The conversion of
+// T's limb is never written and read back.  Every prime below 2^51: the
+// products are the exact double-precision ones of the unfused conversion
+// (k_modup_conv_a), and the column stages those of the double-precision pass,
+// so the stored words equal the unfused conversion followed by the column pass.
+struct ModUpColArgs {
+    uint64_t *T;
+    const uint64_t *coef;
+    const ulonglong2 *w;
+    const double *convd;
+    const double2 *qd;
+    const double *twd;
+    uint32_t l, digits, nout, key_limbs, logn, chunk, chunks;
+    uint32_t outp[kMaxPrimes];  // words: uniform indices take scalar loads
+};
+
+template <int LOGS, int A>
+__global__ __launch_bounds__(kThreads) void k_modup_col(ModUpColArgs a)
+{
+    using G = Geometry<LOGS, kCol, 8>;
+    constexpr int PER = G::TILE / kThreads;
+    __shared__ double lds[G::TILE];
+    const uint32_t n = 1u << a.logn;
+    const uint32_t N2 = n >> LOGS, N1 = 1u << LOGS;
+    const uint32_t row = blockIdx.y;  // (b * digits + j) * chunks + chunk
+    const uint32_t bj = row / a.chunks, chunk = row - bj * a.chunks;
+    const uint32_t b = bj / a.digits, j = bj - b * a.digits;
+    const uint32_t first = j * A;
+    const uint32_t s = min((uint32_t)A, a.l - first);
+    const uint32_t count = a.nout - s;  // output primes outside the digit
+    const uint32_t r0 = chunk * a.chunk;
+    if (r0 >= count) return;  // uniform, before any barrier
+    const uint32_t r1 = min(count, r0 + a.chunk);
+    uint32_t tile = blockIdx.x;
+    if ((gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t col0 = tile * G::C;
+    uint32_t gidx[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = threadIdx.x + t * kThreads;
+        gidx[t] = (uint32_t)(e / G::C) * N2 + col0 + (uint32_t)(e % G::C);
+    }
+    // the digit's products [x Dhat^-1]_q in [0, q) (rows i >= s stay zero)
+    const uint32_t base = (j * A + s - 1) * A;
+    double y[A][PER];
+#pragma unroll
+    for (int i = 0; i < A; i++) {
+        const bool on = (uint32_t)i < s;
+        const uint32_t p = on ? first + i : first;
+        const double q = a.qd[p].x, qinv = a.qd[p].y;
+        const double wi = (double)a.w[base + (on ? i : 0)].x;
+        const uint64_t *x = a.coef + ((uint64_t)b * a.l + p) * n;
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            double v = reduce_f(mulmod_f((double)x[gidx[t]], wi, q, qinv), q, qinv);
+            v = v < 0 ? v + q : v;
+            y[i][t] = on ? v : 0.0;
+        }
+    }
+    for (uint32_t r = r0; r < r1; r++) {
+        const uint32_t I = r < first ? r : r + s;
+        const uint32_t kp = a.outp[I];
+        const double q = a.qd[kp].x, qinv = a.qd[kp].y;
+        const double *fd = a.convd + (uint64_t)base * a.key_limbs + kp;
+        double f[A];
+#pragma unroll
+        for (int i = 0; i < A; i++) f[i] = fd[(uint64_t)i * a.key_limbs];
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < A; i++) {
+                acc += mulmod_f(y[i][t], f[i], q, qinv);
+                if (i & 1) acc = reduce_f(acc, q, qinv);
+            }
+            lds[threadIdx.x + t * kThreads] = reduce_f(acc, q, qinv);  // |v| <= q/2 (1 + eps)
+        }
+        __syncthreads();
+        const double *tw = a.twd + (uint64_t)kp * n;
+        const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
+        if (lazy)
+            stages_f<LOGS, kCol, true, 8, true>(lds, tw, q, qinv, n, N1, col0);
+        else
+            stages_f<LOGS, kCol, true, 8, false>(lds, tw, q, qinv, n, N1, col0);
+        uint64_t *out = a.T + (((uint64_t)b * a.nout + I) * a.digits + j) * n;
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            double v = lds[threadIdx.x + t * kThreads];
+            if (lazy) v = reduce_f(v, q, qinv);
+            v = v < 0 ? v + q : v;
+            out[gidx[t]] = (uint64_t)v;
+        }
+        // each thread rewrites only the LDS words it has just read: the next
+        // output's stages start after the barrier that follows those writes
+    }
+}
+
+// ---- ModDown base conversion fused into the forward column pass
+//
+// Workgroup (tile, poly p, chunk): the A input limbs at limbs lo .. lo + A - 1
+// of poly p of acc (coefficient form) are read once over the tile; per
+// position v_k = [y_k w_k]_(p_k) (Shoup products, as k_moddown_conv_a) and the
+// centring quotient r = floor(sum_k v_k / p_k + 1/2) stay in registers; each
+// output prime q_i (i < lo) of the chunk gets sum_k v_k f_(k,i) - r (P mod q_i)
+// in exact double-precision products, transformed by the column stages of q_i
+// and stored into z[p][i].  The words equal the unfused conversion followed
+// by the column pass.  Every prime below 2^51.
+// ---- Rescale centring fused into the forward column pass
+//
+// Workgroup (tile, poly p, chunk): the INTT'd last limb of poly p (prime p_l,
+// [0, p_l)) is read once over the tile; each output prime q_i of the chunk
+// gets (x + (p_l - 1)/2) mod p_l - (p_l - 1)/2 (SEAL's centred remainder,
+// an integer of magnitude below 2^51, exact in double precision) reduced mod
+// q_i, transformed by the column stages of q_i, stored into z[p][i].  The
+// words equal k_center_last followed by the column pass.  Every prime below 2^51.
+struct CenterColArgs {
+    uint64_t *z;
+    const uint64_t *last;
+    uint64_t last_stride;
+    const double2 *qd;
+    const double *twd;
+    uint64_t p, half;
+    uint32_t limbs, logn, chunk, chunks;
+};
+
+template <int LOGS>
+__global__ __launch_bounds__(kThreads) void k_center_col(CenterColArgs a)
+{
+    using G = Geometry<LOGS, kCol, 8>;
+    constexpr int PER = G::TILE / kThreads;
+    __shared__ double lds[G::TILE];
+    const uint32_t n = 1u << a.logn;
+    const uint32_t N2 = n >> LOGS, N1 = 1u << LOGS;
+    const uint32_t p = blockIdx.y / a.chunks, chunk = blockIdx.y - p * a.chunks;
+    const uint32_t i0 = chunk * a.chunk;
+    if (i0 >= a.limbs) return;  // uniform, before any barrier
+    const uint32_t i1 = min(a.limbs, i0 + a.chunk);
+    uint32_t tile = blockIdx.x;
+    if ((gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t col0 = tile * G::C;
+    uint32_t gidx[PER];
+    double v[PER];
+    const uint64_t *x = a.last + (uint64_t)p * a.last_stride;
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = threadIdx.x + t * kThreads;
+        gidx[t] = (uint32_t)(e / G::C) * N2 + col0 + (uint32_t)(e % G::C);
+        uint64_t u = x[gidx[t]] + a.half;
+        u = u >= a.p ? u - a.p : u;
+        v[t] = (double)u - (double)a.half;
+    }
+    for (uint32_t i = i0; i < i1; i++) {
+        const double q = a.qd[i].x, qinv = a.qd[i].y;
+#pragma unroll
+        for (int t = 0; t < PER; t++) lds[threadIdx.x + t * kThreads] = reduce_f(v[t], q, qinv);
+        __syncthreads();
+        const double *tw = a.twd + (uint64_t)i * n;
+        const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
+        if (lazy)
+            stages_f<LOGS, kCol, true, 8, true>(lds, tw, q, qinv, n, N1, col0);
+        else
+            stages_f<LOGS, kCol, true, 8, false>(lds, tw, q, qinv, n, N1, col0);
+        uint64_t *out = a.z + ((uint64_t)p * a.limbs + i) * n;
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            double y = lds[threadIdx.x + t * kThreads];
+            if (lazy) y = reduce_f(y, q, qinv);
+            y = y < 0 ? y + q : y;
+            out[gidx[t]] = (uint64_t)y;
+        }
+    }
+}
+
+struct ModDownColArgs {
+    uint64_t *z;
+    const uint64_t *acc;
+    uint64_t acc_stride;
+    const ulonglong2 *w;
+    const double *convd;
+    const double2 *qd;
+    const ModConst *mod;
+    const double *twd;
+    uint32_t lo, L, logn, chunk, chunks;
+    uint32_t inp[kMaxPrimes];
+};
+
+template <int LOGS, int A>
+__global__ __launch_bounds__(kThreads) void k_moddown_col(ModDownColArgs a)
+{
+    using G = Geometry<LOGS, kCol, 8>;
+    constexpr int PER = G::TILE / kThreads;
+    __shared__ double lds[G::TILE];
+    const uint32_t n = 1u << a.logn;
+    const uint32_t N2 = n >> LOGS, N1 = 1u << LOGS;
+    const uint32_t p = blockIdx.y / a.chunks, chunk = blockIdx.y - p * a.chunks;
+    const uint32_t i0 = chunk * a.chunk;
+    if (i0 >= a.lo) return;  // uniform, before any barrier
+    const uint32_t i1 = min(a.lo, i0 + a.chunk);
+    uint32_t tile = blockIdx.x;
+    if ((gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const uint32_t col0 = tile * G::C;
+    uint32_t gidx[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) {
+        const int e = threadIdx.x + t * kThreads;
+        gidx[t] = (uint32_t)(e / G::C) * N2 + col0 + (uint32_t)(e % G::C);
+    }
+    const uint64_t *y = a.acc + (uint64_t)p * a.acc_stride + (uint64_t)a.lo * n;
+    double v[A][PER], rr[PER];
+    double frac[PER];
+#pragma unroll
+    for (int t = 0; t < PER; t++) frac[t] = 0.0;
+#pragma unroll
+    for (int k = 0; k < A; k++) {
+        const ulonglong2 wk = a.w[k];
+        const uint64_t pk = a.mod[a.inp[k]].q;
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            const uint64_t x = shoup(y[(uint64_t)k * n + gidx[t]], wk.x, wk.y, pk);
+            frac[t] += (double)x / (double)pk;
+            v[k][t] = (double)x;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < PER; t++) rr[t] = (double)(uint32_t)(uint64_t)floor(frac[t] + 0.5);  // <= A
+    for (uint32_t i = i0; i < i1; i++) {
+        const double q = a.qd[i].x, qinv = a.qd[i].y;
+        double f[A + 1];
+#pragma unroll
+        for (int k = 0; k <= A; k++) f[k] = a.convd[(uint64_t)k * a.L + i];
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            // - r (P mod q_i) as r (q_i - P mod q_i)
+            double s = mulmod_f(rr[t], f[A], q, qinv);
+#pragma unroll
+            for (int k = 0; k < A; k++) {
+                s += mulmod_f(v[k][t], f[k], q, qinv);
+                if (k & 1) s = reduce_f(s, q, qinv);
+            }
+            lds[threadIdx.x + t * kThreads] = reduce_f(s, q, qinv);  // |v| <= q/2 (1 + eps)
+        }
+        __syncthreads();
+        const double *tw = a.twd + (uint64_t)i * n;
+        const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
+        if (lazy)
+            stages_f<LOGS, kCol, true, 8, true>(lds, tw, q, qinv, n, N1, col0);
+        else
+            stages_f<LOGS, kCol, true, 8, false>(lds, tw, q, qinv, n, N1, col0);
+        uint64_t *out = a.z + ((uint64_t)p * a.lo + i) * n;
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            double x = lds[threadIdx.x + t * kThreads];
+            if (lazy) x = reduce_f(x, q, qinv);
+            x = x < 0 ? x + q : x;
+            out[gidx[t]] = (uint64_t)x;
+        }
+    }
+}
+
 // The forward column pass with 8-wide tiles held to 64 VGPRs: 8 waves per SIMD
 // instead of 7 (the other passes reach 8 as they are, or would spill).
 template <int LOGS, int ROLE, bool FWD, int CW>
@@ -743,11 +1011,166 @@ void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
     } else {
         int l1, l2;
         split_logn(t.logn, l1, l2);
-        launch_pass<kCol, true>(l1, a, batch, s);
+        if (!b.rows_only) launch_pass<kCol, true>(l1, a, batch, s);
         a.src = nullptr;  // the second pass runs in place on the output
         a.nsrcs = 0;
         launch_pass<kRow, true>(l2, a, batch, s);
     }
+}
+
+// Fused ModUp column pass per power-of-two bucket of the batch size B:
+// 0 = on (outputs per workgroup by the default rule), -1 = off (conversion and
+// column pass as separate launches), k > 0 = on with k outputs per workgroup.
+// Process-wide; set by the autotuner.
+static std::atomic<int> g_modup_col[kTuneBuckets];
+
+void ks_modup_col_set(uint32_t B, int mode) { g_modup_col[tune_bucket(B)].store(mode); }
+
+int ks_modup_col_get(uint32_t B) { return g_modup_col[tune_bucket(B)].load(); }
+
+bool ks_modup_col_ok(const DeviceTables &t, uint32_t alpha)
+{
+    return t.max_bits <= 51 && alpha >= 2 && alpha <= 8 && t.logn >= 14 && t.logn <= 17 &&
+           t.modup_conv_d != nullptr;
+}
+
+void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
+                          uint32_t B, uint32_t l, uint32_t alpha, uint32_t key_limbs, const uint8_t *out_primes_host,
+                          uint32_t nout, hipStream_t s)
+{
+    if (!ks_modup_col_ok(t, alpha)) throw std::invalid_argument("ks_modup_convert_col: unsupported parameters");
+    if (nout > (uint32_t)kMaxPrimes || l < 1 || nout <= alpha) throw std::invalid_argument("ks_modup_convert_col: bad sizes");
+    ModUpColArgs a;
+    a.T = T;
+    a.coef = coef;
+    a.w = w;
+    a.convd = t.modup_conv_d;
+    a.qd = t.qd;
+    a.twd = t.fwd_d;
+    a.l = l;
+    a.digits = (l + alpha - 1) / alpha;
+    a.nout = nout;
+    a.key_limbs = key_limbs;
+    a.logn = (uint32_t)t.logn;
+    for (int i = 0; i < kMaxPrimes; i++) a.outp[i] = i < (int)nout ? out_primes_host[i] : 0;
+    int l1, l2;
+    split_logn(t.logn, l1, l2);
+    const uint32_t n = 1u << t.logn;
+    const uint32_t tiles = n / ((1u << l1) * 8u);
+    // outputs per workgroup: enough workgroups to fill the device, at least 4
+    // outputs per read of the digit's inputs
+    const uint32_t count = nout - std::min(alpha, l);  // most outputs of a digit (a partial digit has more)
+    const uint32_t want = (2048u + tiles * B * a.digits - 1) / (tiles * B * a.digits);
+    uint32_t chunk = (count + want - 1) / want;
+    chunk = std::max(chunk, std::min(4u, count));
+    const int mode = g_modup_col[tune_bucket(B)].load();
+    if (mode > 0) chunk = std::min((uint32_t)mode, count);
+    a.chunk = chunk;
+    a.chunks = (nout - 1 + chunk - 1) / chunk;  // covers a partial last digit (nout - s <= nout - 1 outputs)
+    uint32_t outs = 0;
+    for (uint32_t j = 0; j < a.digits; j++) outs += nout - std::min(alpha, l - j * alpha);
+    ProfScope prof("ks_modup_col", s,
+                   8.0 * n * ((double)B * l + (double)B * outs) + 8.0 * (double)(1u << l1) * B * outs);
+    const dim3 grid(tiles, B * a.digits * a.chunks);
+#define NX_MC(LOGS, A)                                                                          \
+    if (l1 == LOGS && alpha == A) {                                                           \
+        hipLaunchKernelGGL((k_modup_col<LOGS, A>), grid, dim3(kThreads), 0, s, a);             \
+        return;                                                                               \
+    }
+    NX_MC(7, 2) NX_MC(7, 3) NX_MC(7, 4) NX_MC(7, 5) NX_MC(7, 6) NX_MC(7, 7) NX_MC(7, 8)
+    NX_MC(8, 2) NX_MC(8, 3) NX_MC(8, 4) NX_MC(8, 5) NX_MC(8, 6) NX_MC(8, 7) NX_MC(8, 8)
+#undef NX_MC
+    throw std::invalid_argument("ks_modup_convert_col: unsupported split");
+}
+
+bool center_col_ok(const DeviceTables &t) { return t.max_bits <= 51 && t.logn >= 14 && t.logn <= 17; }
+
+void rns_center_last_col(const DeviceTables &t, uint64_t *z, const uint64_t *last, uint64_t last_stride,
+                         uint32_t last_prime, uint32_t polys, uint32_t limbs, hipStream_t s)
+{
+    if (!center_col_ok(t) || limbs < 1) throw std::invalid_argument("rns_center_last_col: unsupported parameters");
+    CenterColArgs a;
+    a.z = z;
+    a.last = last;
+    a.last_stride = last_stride;
+    a.qd = t.qd;
+    a.twd = t.fwd_d;
+    a.p = t.mod_host[last_prime];
+    a.half = a.p >> 1;
+    a.limbs = limbs;
+    a.logn = (uint32_t)t.logn;
+    int l1, l2;
+    split_logn(t.logn, l1, l2);
+    const uint32_t n = 1u << t.logn;
+    const uint32_t tiles = n / ((1u << l1) * 8u);
+    const uint32_t want = (2048u + tiles * polys - 1) / (tiles * polys);
+    uint32_t chunk = (limbs + want - 1) / want;
+    chunk = std::max(chunk, std::min(4u, limbs));
+    a.chunk = chunk;
+    a.chunks = (limbs + chunk - 1) / chunk;
+    ProfScope prof("rescale_col", s, 8.0 * n * polys * (1.0 + limbs) + 8.0 * (double)(1u << l1) * polys * limbs);
+    const dim3 grid(tiles, polys * a.chunks);
+    if (l1 == 7)
+        hipLaunchKernelGGL((k_center_col<7>), grid, dim3(kThreads), 0, s, a);
+    else if (l1 == 8)
+        hipLaunchKernelGGL((k_center_col<8>), grid, dim3(kThreads), 0, s, a);
+    else
+        throw std::invalid_argument("rns_center_last_col: unsupported split");
+}
+
+static std::atomic<int> g_moddown_col[kTuneBuckets];
+
+void ks_moddown_col_set(uint32_t polys, int mode) { g_moddown_col[tune_bucket(polys)].store(mode); }
+
+int ks_moddown_col_get(uint32_t polys) { return g_moddown_col[tune_bucket(polys)].load(); }
+
+bool ks_moddown_col_ok(const DeviceTables &t, uint32_t A)
+{
+    return t.max_bits <= 51 && A >= 2 && A <= 9 && t.logn >= 14 && t.logn <= 17;
+}
+
+void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
+                            const ulonglong2 *w, const double *convd, const uint8_t *in_primes_host, uint32_t A,
+                            uint32_t polys, uint32_t lo, uint32_t L, hipStream_t s)
+{
+    if (!ks_moddown_col_ok(t, A) || !convd) throw std::invalid_argument("ks_moddown_convert_col: unsupported parameters");
+    if (lo < 1 || lo + A > (uint32_t)kMaxPrimes) throw std::invalid_argument("ks_moddown_convert_col: bad sizes");
+    ModDownColArgs a;
+    a.z = z;
+    a.acc = acc;
+    a.acc_stride = acc_stride;
+    a.w = w;
+    a.convd = convd;
+    a.qd = t.qd;
+    a.mod = t.mod;
+    a.twd = t.fwd_d;
+    a.lo = lo;
+    a.L = L;
+    a.logn = (uint32_t)t.logn;
+    for (int i = 0; i < kMaxPrimes; i++) a.inp[i] = i < (int)A ? in_primes_host[i] : 0;
+    int l1, l2;
+    split_logn(t.logn, l1, l2);
+    const uint32_t n = 1u << t.logn;
+    const uint32_t tiles = n / ((1u << l1) * 8u);
+    const uint32_t want = (2048u + tiles * polys - 1) / (tiles * polys);
+    uint32_t chunk = (lo + want - 1) / want;
+    chunk = std::max(chunk, std::min(4u, lo));
+    const int mode = g_moddown_col[tune_bucket(polys)].load();
+    if (mode > 0) chunk = std::min((uint32_t)mode, lo);
+    a.chunk = chunk;
+    a.chunks = (lo + chunk - 1) / chunk;
+    ProfScope prof("ks_moddown_col", s,
+                   8.0 * n * polys * ((double)A + lo) + 8.0 * (double)(1u << l1) * polys * lo);
+    const dim3 grid(tiles, polys * a.chunks);
+#define NX_MD(LOGS, AA)                                                                         \
+    if (l1 == LOGS && A == AA) {                                                              \
+        hipLaunchKernelGGL((k_moddown_col<LOGS, AA>), grid, dim3(kThreads), 0, s, a);          \
+        return;                                                                               \
+    }
+    NX_MD(7, 2) NX_MD(7, 3) NX_MD(7, 4) NX_MD(7, 5) NX_MD(7, 6) NX_MD(7, 7) NX_MD(7, 8) NX_MD(7, 9)
+    NX_MD(8, 2) NX_MD(8, 3) NX_MD(8, 4) NX_MD(8, 5) NX_MD(8, 6) NX_MD(8, 7) NX_MD(8, 8) NX_MD(8, 9)
+#undef NX_MD
+    throw std::invalid_argument("ks_moddown_convert_col: unsupported split");
 }
 
 void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s)

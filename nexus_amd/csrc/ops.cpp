@@ -248,6 +248,27 @@ static DevBuf ks_modup_impl(Context &c, const uint64_t *targets, const uint64_t 
         ntt_inv_from(c, coef.get(), targets, B, l, l * n, l * n);
     }
     DevBuf T(&c, B * nout * digits * n);
+    if (c.alpha > 1 && ks_modup_col_ok(c.tables, (uint32_t)c.alpha) && ks_modup_col_get((uint32_t)B) >= 0) {
+        // the conversion feeds the forward column pass directly; the digits'
+        // own limbs are copied in NTT form; only the row pass remains
+        ks_modup_copy_own(c.tables, T.get(), targets, target_list, (uint32_t)B, (uint32_t)l, (uint32_t)c.alpha,
+                          (uint32_t)digits, (uint32_t)nout, c.stream);
+        ks_modup_convert_col(c.tables, T.get(), coef.get(), c.d_modup_w, (uint32_t)B, (uint32_t)l,
+                             (uint32_t)c.alpha, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
+        NttBatch b;
+        b.skip_l = (uint32_t)l;
+        b.skip_alpha = (uint32_t)c.alpha;
+        b.rows_only = 1;
+        b.data = T.get();
+        b.poly_stride = digits * n;
+        b.polys = (uint32_t)(B * nout);
+        b.limbs = (uint32_t)digits;
+        b.prime_by_poly = 1;
+        b.poly_period = (uint32_t)nout;
+        for (int i = 0; i < kMaxPrimes; i++) b.prime_of[i] = i < (int)nout ? outp[i] : 0;
+        ntt_forward(c.tables, b, c.stream);
+        return T;
+    }
     if (c.alpha == 1)
         ks_modup_expand(c.tables, T.get(), coef.get(), (uint32_t)B, (uint32_t)l, outp, (uint32_t)nout, c.stream);
     else if (ks_conv_fused_ok((uint32_t)c.alpha))
@@ -301,7 +322,7 @@ void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t
     ks_moddown(c, cts, acc.get(), B, l, zero_c1, base);
 }
 
-DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool transform)
+DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool transform, bool *col_done)
 {
     const size_t n = c.n, L = c.L, alpha = c.alpha;
     const size_t nout = l + alpha;
@@ -318,6 +339,26 @@ DevBuf ks_moddown_prepare(Context &c, uint64_t *acc, size_t B, size_t l, bool tr
         ntt_inverse(c.tables, b, c.stream);
     }
     DevBuf tmp(&c, B * 2 * l * n);
+    if (col_done) *col_done = false;
+    if (alpha > 1 && (col_done || transform) && c.d_moddown_conv_d && ks_moddown_col_ok(c.tables, (uint32_t)alpha) &&
+        ks_moddown_col_get((uint32_t)(2 * B)) >= 0) {
+        // the centred conversion feeds the forward column pass directly
+        uint8_t inp[kMaxPrimes];
+        for (size_t k = 0; k < alpha; k++) inp[k] = (uint8_t)(L + k);
+        ks_moddown_convert_col(c.tables, tmp.get(), acc, nout * n, c.d_moddown_w, c.d_moddown_conv_d, inp,
+                               (uint32_t)alpha, (uint32_t)(2 * B), (uint32_t)l, (uint32_t)L, c.stream);
+        if (transform) ntt_fwd_epi(c, tmp.get(), 2 * B, l, l * n, NttEpilogue(), true);
+        if (col_done) *col_done = !transform;
+        return tmp;
+    }
+    if (alpha == 1 && (col_done || transform) && center_col_ok(c.tables)) {
+        // SEAL's centring inside the forward column pass
+        rns_center_last_col(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
+                            c.stream);
+        if (transform) ntt_fwd_epi(c, tmp.get(), 2 * B, l, l * n, NttEpilogue(), true);
+        if (col_done) *col_done = !transform;
+        return tmp;
+    }
     if (alpha == 1)  // SEAL: centred (rounded) division by p
         rns_center_last(c.tables, tmp.get(), acc + l * n, nout * n, (uint32_t)L, (uint32_t)(2 * B), (uint32_t)l,
                         c.stream);
@@ -337,7 +378,8 @@ void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bo
     // cts = (acc - NTT(conversion)) P^-1 (+ base or cts): the combination is the
     // forward transform's last pass
     const size_t n = c.n, nout = l + c.alpha;
-    DevBuf tmp = ks_moddown_prepare(c, acc, B, l, false);
+    bool col_done = false;
+    DevBuf tmp = ks_moddown_prepare(c, acc, B, l, false, &col_done);
     NttEpilogue e;
     e.out = cts;
     e.out_stride = l * n;
@@ -347,7 +389,7 @@ void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bo
     e.accumulate = zero_c1 ? 2 : 1;
     e.base = base;
     e.base_stride = l * n;
-    ntt_fwd_epi(c, tmp.get(), 2 * B, l, l * n, e);
+    ntt_fwd_epi(c, tmp.get(), 2 * B, l, l * n, e, col_done);
 }
 
 static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
@@ -480,7 +522,13 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
         ntt_inverse(c.tables, b, c.stream);
     }
     DevBuf tmp(&c, P * (l - 1) * n);
-    rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1), c.stream);
+    const bool col = center_col_ok(c.tables);  // centring inside the forward column pass
+    if (col)
+        rns_center_last_col(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1),
+                            c.stream);
+    else
+        rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1),
+                        c.stream);
     // out = (a - NTT(centred last limb)) q_(l-1)^-1, stored by the transform's last pass
     DevBuf out(&c, P * (l - 1) * n);
     NttEpilogue e;
@@ -489,7 +537,7 @@ void op_rescale_to_next(const Ciphertext &a, Ciphertext &dst)
     e.x = a.data.get();
     e.x_stride = l * n;
     e.inv = c.rescale_inv(l);
-    ntt_fwd_epi(c, tmp.get(), P, l - 1, (l - 1) * n, e);
+    ntt_fwd_epi(c, tmp.get(), P, l - 1, (l - 1) * n, e, col);
     const double scale = a.scale / (double)c.moduli[l - 1];
     Ciphertext r;
     r.ctx = &c;
@@ -526,7 +574,13 @@ static void rescale_batch(Context &c, const std::vector<const uint64_t *> &x, si
         ntt_inverse(c.tables, b, c.stream);
     }
     DevBuf tmp(&c, P * (l - 1) * n);
-    rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1), c.stream);
+    const bool col = center_col_ok(c.tables);  // centring inside the forward column pass
+    if (col)
+        rns_center_last_col(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1),
+                            c.stream);
+    else
+        rns_center_last(c.tables, tmp.get(), last.get(), n, (uint32_t)(l - 1), (uint32_t)P, (uint32_t)(l - 1),
+                        c.stream);
     NttEpilogue e;  // outs[r] = (x[r] - NTT(centred last limbs)) q_(l-1)^-1
     for (size_t r = 0; r < B; r++) {
         e.outs[r] = outs[r];
@@ -535,7 +589,7 @@ static void rescale_batch(Context &c, const std::vector<const uint64_t *> &x, si
     e.x_limbs = (uint32_t)l;
     e.polys_per = (uint32_t)polys;
     e.inv = c.rescale_inv(l);
-    ntt_fwd_epi(c, tmp.get(), P, l - 1, (l - 1) * n, e);
+    ntt_fwd_epi(c, tmp.get(), P, l - 1, (l - 1) * n, e, col);
 }
 
 void op_rescale_many(const std::vector<const Ciphertext *> &in, const std::vector<Ciphertext *> &out)
@@ -692,10 +746,20 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
             ntt_inverse(c.tables, nb, c.stream);
         }
         DevBuf tmp(&c, cnt * 2 * lo * n);  // conversion, then the transform's first pass
-        ks_moddown_rescale_convert(c.tables, tmp.get(), acc.get(), nout * n, c.d_mdr_w + l * A,
-                                   c.d_mdr_conv + l * (A + 1) * c.L, c.d_mdr_conv_d + l * (A + 1) * c.L,
-                                   (uint32_t)(2 * cnt), (uint32_t)lo,
-                                   (uint32_t)alpha, (uint32_t)c.L, c.stream);
+        bool col_done = false;
+        if (ks_moddown_col_ok(c.tables, (uint32_t)A) && ks_moddown_col_get((uint32_t)(2 * cnt)) >= 0) {
+            // the conversion feeds the forward column pass directly
+            uint8_t inp[kMaxPrimes];
+            for (size_t k = 0; k < A; k++) inp[k] = (uint8_t)(k == 0 ? lo : c.L + k - 1);
+            ks_moddown_convert_col(c.tables, tmp.get(), acc.get(), nout * n, c.d_mdr_w + l * A,
+                                   c.d_mdr_conv_d + l * (A + 1) * c.L, inp, (uint32_t)A, (uint32_t)(2 * cnt),
+                                   (uint32_t)lo, (uint32_t)c.L, c.stream);
+            col_done = true;
+        } else {
+            ks_moddown_rescale_convert(c.tables, tmp.get(), acc.get(), nout * n, c.d_mdr_w + l * A,
+                                       c.d_mdr_conv + l * (A + 1) * c.L, c.d_mdr_conv_d + l * (A + 1) * c.L,
+                                       (uint32_t)(2 * cnt), (uint32_t)lo, (uint32_t)alpha, (uint32_t)c.L, c.stream);
+        }
         NttEpilogue e;  // out_i = (acc_i - NTT(tmp)_i) P'^-1 + d_i q_(l-1)^-1
         for (size_t r = 0; r < cnt; r++) {
             Ciphertext &t = res[base + r];
@@ -713,7 +777,7 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
         e.d = c01.get();
         e.d_stride = pw;
         e.inv2 = c.rescale_inv(l);
-        ntt_fwd_epi(c, tmp.get(), 2 * cnt, lo, lo * n, e);
+        ntt_fwd_epi(c, tmp.get(), 2 * cnt, lo, lo * n, e, col_done);
     }
     for (size_t r = 0; r < a.size(); r++) *out[r] = std::move(res[r]);
 }

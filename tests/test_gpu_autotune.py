@@ -56,3 +56,37 @@ def test_autotune_then_parity(logn, chain):
         for c, p in zip(cts[1:], pts[1:]):
             want = nx.add(want, nx.multiply_plain(c, p))
         assert np.array_equal(got, want.to_array())
+
+
+def test_autotune_modup_then_parity():
+    """hybrid context with every prime below 2^51: the tuner times the ModUp and
+    ModDown conversions fused into the column pass against the separate
+    launches (each candidate checked word for word), and relinearisation stays
+    bit-exact under its choices"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    from oracle import hybrid
+    logn, chain, alpha = 14, [51] + [46] * 9 + [51] * 4, 4
+    _note("autotune hybrid ModUp logn=%d alpha=%d" % (logn, alpha))
+    ctx = nx.Context(logn, chain, special_primes=alpha)
+    rep = nx.autotune(ctx, reps=3)
+    print(rep["modup"])
+    assert [e["batch"] for e in rep["modup"]] == [1, 2, 4, 8, 16, 32]
+    for e in rep["modup"]:
+        assert e["mode"] in (-1, 0, 4, 8, 12, 64)
+        assert all(t > 0 for t in e["fused_us"])  # every fused candidate equalled the unfused words
+    assert [e["batch"] for e in rep["moddown"]] == [1, 2, 4, 8, 16]
+    for e in rep["moddown"]:
+        assert e["mode"] in (-1, 0, 4, 8, 12, 64)
+        assert all(t > 0 for t in e["fused_us"])
+    L = ctx.L
+    sk = nx.SecretKey(ctx, seed=71)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=72)
+    rng = np.random.default_rng(73)
+    data = np.stack([np.stack([rng.integers(0, q, ctx.n, dtype=np.uint64) for q in ctx.moduli[:L]])
+                     for _ in range(3)])
+    got = nx.relinearize(nx.Ciphertext.from_array(ctx, data, 1, 2.0 ** 40), rk).to_array()
+    ks0, ks1 = hybrid.switch_key_hybrid(data[2], rk.key_array(0), ctx.moduli, L, alpha, logn)
+    mods = np.array(ctx.moduli[:L], dtype=object)[:, None]
+    assert np.array_equal(got[0], ((data[0].astype(object) + ks0.astype(object)) % mods).astype(np.uint64))
+    assert np.array_equal(got[1], ((data[1].astype(object) + ks1.astype(object)) % mods).astype(np.uint64))

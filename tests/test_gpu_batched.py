@@ -20,7 +20,8 @@ def _note(msg):
 
 
 @pytest.fixture(scope="module", params=[(12, [58, 40, 40, 40, 58], 1), (13, [51, 46, 46, 46, 51, 58, 58], 2),
-                                        (16, [51, 46, 46, 58, 58], 2)], ids=["seal", "hybrid", "n16"])
+                                        (16, [51, 46, 46, 58, 58], 2), (16, [51, 46, 46, 46, 51, 51], 2)],
+                ids=["seal", "hybrid", "n16", "n16-fp"])
 def env(request):
     if nx.device_count() < 1:
         pytest.skip("no GPU visible")

@@ -82,6 +82,60 @@ def test_hybrid_relin_bit_exact_wide(chain, alpha, chain_index):
     assert np.array_equal(got[1], ((data[1].astype(object) + ks1.astype(object)) % mods).astype(np.uint64))
 
 
+@pytest.mark.parametrize("logn,chain,alpha,chain_index",
+                         [(14, [45] * 6 + [50] * 2, 2, 1), (14, [51] * 9 + [51] * 8, 8, 1),
+                          (15, [46] * 7 + [51] * 3, 3, 2), (16, [51] + [46] * 11 + [51] * 8, 8, 1),
+                          (16, [51] + [46] * 11 + [51] * 8, 8, 4)],
+                         ids=["n14-a2", "n14-a8", "n15-a3-partial", "n16-a8-partial", "n16-a8-l9"])
+def test_hybrid_relin_bit_exact_two_pass(logn, chain, alpha, chain_index):
+    """ring sizes of the two-pass NTT with every prime below 2^51: the ModUp
+    conversion runs inside the forward column pass (full and partial last
+    digits, below the top level)"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    L = len(chain) - alpha
+    _note("hybrid relin two-pass logn=%d chain=%s alpha=%d chain_index=%d" % (logn, chain, alpha, chain_index))
+    ctx = nx.Context(logn, chain, special_primes=alpha)
+    sk = nx.SecretKey(ctx, seed=25)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=29)
+    key = rk.key_array(0)
+    rng = np.random.default_rng(logn + L + alpha + chain_index)
+    limbs = ctx.limbs_of(chain_index)
+    data = _rand_ct(rng, ctx, 3, limbs)
+    ct = nx.Ciphertext.from_array(ctx, data, chain_index, 2.0 ** 40)
+    got = nx.relinearize(ct, rk).to_array()
+    ks0, ks1 = hybrid.switch_key_hybrid(data[2], key, ctx.moduli, L, alpha, logn)
+    mods = np.array(ctx.moduli[:limbs], dtype=object)[:, None]
+    assert np.array_equal(got[0], ((data[0].astype(object) + ks0.astype(object)) % mods).astype(np.uint64))
+    assert np.array_equal(got[1], ((data[1].astype(object) + ks1.astype(object)) % mods).astype(np.uint64))
+
+
+def test_hybrid_rotations_two_pass():
+    """hoisted rotations (one shared ModUp) and a sum of rotations of four
+    inputs (one batched ModUp over a source list) at N = 2^14, bit-exact"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    logn, L, alpha = 14, 7, 3
+    _note("hybrid rotations two-pass logn=%d L=%d alpha=%d" % (logn, L, alpha))
+    ctx = nx.Context(logn, [46] * L + [51] * alpha, special_primes=alpha)
+    sk = nx.SecretKey(ctx, seed=31)
+    steps = [1, -2, 0, 77]
+    el = [ctx.elt_from_step(s) if s else 0 for s in steps]
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, [e for e in el if e], seed=32)
+    kk = [gk.key_array(e) if e else None for e in el]
+    rng = np.random.default_rng(33)
+    data = _rand_ct(rng, ctx, 2, L)
+    ct = nx.Ciphertext.from_array(ctx, data, 1, 2.0 ** 40)
+    outs = nx.rotate_hoisted(ct, steps, gk)
+    for k, s in enumerate(steps):
+        ref = data if s == 0 else hybrid.rotate_sum_hybrid([data], [el[k]], [kk[k]], ctx.moduli, L, alpha, logn)
+        assert np.array_equal(outs[k].to_array(), ref), s
+    others = [_rand_ct(rng, ctx, 2, L) for _ in steps]
+    cts = [nx.Ciphertext.from_array(ctx, d, 1, 2.0 ** 40) for d in others]
+    got = nx.rotate_sum(cts, steps, gk)
+    assert np.array_equal(got.to_array(), hybrid.rotate_sum_hybrid(others, el, kk, ctx.moduli, L, alpha, logn))
+
+
 def test_hybrid_galois_bit_exact():
     if nx.device_count() < 1:
         pytest.skip("no GPU visible")
