@@ -505,8 +505,8 @@ def main():
     ap.add_argument("--nonlinear-requests", type=int, default=12, help="requests of the served nonlinear extra")
     ap.add_argument("--lanes", type=int, default=3, help="lanes of the served nonlinear-block measurement")
     ap.add_argument("--bert-lanes", type=int, default=1,
-                    help="lanes of the served BERT measurement (one BERT call is in flight at a time, and each "
-                         "lane caches its own call working set)")
+                    help="lanes of the served BERT measurement, each with one BERT call in flight (each lane "
+                         "caches its own call working set)")
     ap.add_argument("--only", choices=["bert", "bert_served", "dot", "bootstrap_dense", "bootstrap_sparse", "matmul",
                                        "ntt", "ks", "valu"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
@@ -769,6 +769,7 @@ def served_bert(nx, B, args, plain_rate, batched_rate):
     def run(count, lanes, pack, per_op):
         B["ev"].set_per_op(per_op)
         srv = nx.Server(B["ev"], lanes=lanes, bert=B["layer"], pack=pack, memory_budget=96 << 30)
+        srv.set_bert_concurrency(lanes)
         B["ev"].set_per_op(False)
         ids = [None] * count
         t0 = time.perf_counter()

@@ -41,6 +41,9 @@ struct Job {
     int len = 0;  // row length of softmax / LayerNorm; sequence length of a BERT job
     Ciphertext input;
     std::vector<Ciphertext> inputs;  // BERT job: hidden / 256 ciphertexts, rows [0, len)
+    // BERT job over a set the caller packed: its sequences' rows (len unused);
+    // the set runs as one layer call and is never packed with other requests
+    std::vector<BertLayer::Segment> segments;
 };
 
 class Server {
@@ -97,7 +100,10 @@ private:
     };
     void worker(int lane);
     void run_lane(int lane);
-    static bool packable(const Request &r) { return r.jobs.size() == 1 && r.jobs[0].kind == JobKind::Bert; }
+    static bool packable(const Request &r)
+    {
+        return r.jobs.size() == 1 && r.jobs[0].kind == JobKind::Bert && r.jobs[0].segments.empty();
+    }
     static double now_ms();
 
     const CKKSContext &ctx_;

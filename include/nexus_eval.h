@@ -145,6 +145,12 @@ int nx_server_submit(nx_server *s, const int *kinds, const int *lens, const nx_c
 /* one BERT-layer request: a sequence encrypted at rows [0, seq_len) of n =
  * hidden / 256 ciphertexts (the inputs are copied); its wait yields n outputs */
 int nx_server_submit_bert(nx_server *s, const nx_ciphertext *const *x, size_t n, int seq_len, uint64_t *request_id);
+/* one BERT-layer request over a set the caller packed: `segments` sequences at
+ * rows [starts[k], starts[k] + lens[k]) of the n ciphertexts (attention kept
+ * inside each); one layer call, never packed with other requests; its wait
+ * yields the n layer outputs.  No counterpart in the reference. */
+int nx_server_submit_bert_set(nx_server *s, const nx_ciphertext *const *x, size_t n, const int *starts,
+                              const int *lens, size_t segments, uint64_t *request_id);
 /* blocks until the request is done; count outputs, one per job */
 int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outputs, size_t count);
 int nx_server_drain(nx_server *s);

@@ -1124,6 +1124,8 @@ _sig("nx_server_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double)
 
 _sig("nx_server_create_bert", [vp, ctypes.c_int, sz, vp, vp, ctypes.c_int, sz, ctypes.POINTER(vp)])
 _sig("nx_server_submit_bert", [vp, ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)])
+_sig("nx_server_submit_bert_set", [vp, ctypes.POINTER(vp), sz, ctypes.POINTER(ctypes.c_int),
+                                   ctypes.POINTER(ctypes.c_int), sz, ctypes.POINTER(ctypes.c_uint64)])
 _sig("nx_server_pause", [vp, ctypes.c_int])
 _sig("nx_server_set_bert_concurrency", [vp, ctypes.c_int])
 _sig("nx_server_batching_stats", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(sz)])
@@ -1168,6 +1170,16 @@ class Server:
         """one BERT-layer request: a sequence at rows [0, seq_len) of hidden/256 ciphertexts"""
         rid = ctypes.c_uint64()
         _check(_lib.nx_server_submit_bert(self.h, _handles(cts), len(cts), seq_len, ctypes.byref(rid)))
+        self._counts[rid.value] = len(cts)
+        return rid.value
+
+    def submit_bert_set(self, cts, segments):
+        """one BERT-layer request over a packed set: segments = [(start, length), ...]"""
+        st = (ctypes.c_int * len(segments))(*[int(a) for a, _ in segments])
+        ln = (ctypes.c_int * len(segments))(*[int(b) for _, b in segments])
+        rid = ctypes.c_uint64()
+        _check(_lib.nx_server_submit_bert_set(self.h, _handles(cts), len(cts), st, ln, len(segments),
+                                              ctypes.byref(rid)))
         self._counts[rid.value] = len(cts)
         return rid.value
 

@@ -113,6 +113,29 @@ int nx_server_submit_bert(nx_server *s, const nx_ciphertext *const *x, size_t n,
     });
 }
 
+int nx_server_submit_bert_set(nx_server *s, const nx_ciphertext *const *x, size_t n, const int *starts,
+                              const int *lens, size_t segments, uint64_t *request_id)
+{
+    return run([&] {
+        need(s);
+        need(x);
+        need(starts);
+        need(lens);
+        need(request_id);
+        if (!segments) throw std::invalid_argument("a packed set needs at least one segment");
+        nexus::Job j;
+        j.kind = nexus::JobKind::Bert;
+        for (size_t k = 0; k < segments; k++) j.segments.push_back({starts[k], lens[k]});
+        for (size_t k = 0; k < n; k++) {
+            need(x[k]);
+            j.inputs.push_back(nexus::Ciphertext::copy_of(*s->ev->ctx, x[k]));
+        }
+        std::vector<nexus::Job> jobs;
+        jobs.push_back(std::move(j));
+        *request_id = s->impl->submit(std::move(jobs));
+    });
+}
+
 int nx_server_wait(nx_server *s, uint64_t request_id, nx_ciphertext *const *outputs, size_t count)
 {
     return run([&] {

@@ -56,7 +56,19 @@ uint64_t Server::submit(std::vector<Job> jobs)
             if (!bert_) throw std::invalid_argument("Server: no BERT layer configured");
             if (j.inputs.size() != (size_t)bert_->shape().hidden / BertLayer::kGroup)
                 throw std::invalid_argument("Server: a BERT job needs hidden / 256 input ciphertexts");
-            if (j.len < 1 || (size_t)j.len > tokens_) throw std::invalid_argument("Server: sequence length out of range");
+            if (!j.segments.empty()) {
+                std::vector<bool> used(tokens_, false);
+                for (const BertLayer::Segment &g : j.segments) {
+                    if (g.len < 1 || g.start < 0 || (size_t)g.start + (size_t)g.len > tokens_)
+                        throw std::invalid_argument("Server: segment out of range");
+                    for (int i = g.start; i < g.start + g.len; i++) {
+                        if (used[i]) throw std::invalid_argument("Server: overlapping segments");
+                        used[i] = true;
+                    }
+                }
+            } else if (j.len < 1 || (size_t)j.len > tokens_) {
+                throw std::invalid_argument("Server: sequence length out of range");
+            }
             for (const Ciphertext &c : j.inputs)
                 if (!c.get()) throw std::invalid_argument("Server: job without input");
             extra += opt_.bert_footprint;
@@ -286,7 +298,9 @@ void Server::run_lane(int lane)
                 for (Job &j : group[0]->jobs) {
                     if (j.kind == JobKind::Bert) {
                         if (!bert) throw std::invalid_argument("server has no BERT layer");
-                        for (auto &c : bert->forward(j.inputs, j.len)) outs[0].push_back(std::move(c));
+                        std::vector<Ciphertext> y =
+                            j.segments.empty() ? bert->forward(j.inputs, j.len) : bert->forward(j.inputs, j.segments);
+                        for (auto &c : y) outs[0].push_back(std::move(c));
                         continue;
                     }
                     Ciphertext out;

@@ -212,6 +212,33 @@ def test_server_continuous_batching(env):
         assert np.max(np.abs(got[n:])) < 2.0 ** -10
 
 
+def test_server_packed_set(env):
+    """a set the caller packed, served as one request on a lane: word for word
+    the direct layer call over the same segments; two such sets run at once"""
+    segs = [(0, 3), (3, 4)]
+    _note("bert server packed set %s" % segs)
+    rng = np.random.default_rng(50)
+    two = len(CHAIN) - 3 - 1
+    sets = []
+    for _ in range(2):
+        x = np.zeros((env.T, HIDDEN))
+        x[:7] = rng.normal(0, 1, (7, HIDDEN))
+        sets.append(env.enc(x, chain_index=two))
+    srv = nx.Server(env.ev, lanes=2, bert=env.layer, pack=False)
+    srv.set_bert_concurrency(2)
+    ids = [srv.submit_bert_set(s, segs) for s in sets]
+    outs = [srv.wait(i) for i in ids]
+    del srv
+    for s, o in zip(sets, outs):
+        direct = env.layer.forward(s, segments=segs)
+        assert len(o) == len(direct)
+        for a, b in zip(o, direct):
+            assert np.array_equal(a.to_array(), b.to_array())
+    with pytest.raises(nx.NexusError):
+        srv = nx.Server(env.ev, lanes=1, bert=env.layer, pack=False)
+        srv.submit_bert_set(sets[0], [(0, 3), (2, 3)])  # overlapping rows
+
+
 def test_bert_base_full_size():
     """BERT-base (hidden 768, 12 heads, FFN 3072) at N = 2^16 with 128-token
     sets, chain {51, 46 x 19, 51 x 14, 51 x 8}: a 40-token sequence, two packed
