@@ -452,15 +452,26 @@ def bench_bert_call(nx, reps):
     cts = bert_encrypt(nx, B, rows)
     B["layer"].forward(cts, BERT_TOKENS)
     B["ctx"].synchronize()
+    w0 = time.clock_gettime_ns(time.CLOCK_MONOTONIC)
     t0 = time.perf_counter()
     for _ in range(reps):
         B["layer"].forward(cts, BERT_TOKENS)
     B["ctx"].synchronize()
     ms = (time.perf_counter() - t0) / reps * 1e3
+    # the timed calls' window, for nexus_amd/trace_busy.py on a kernel trace of this run
+    log("window monotonic_ns %d %d" % (w0, time.clock_gettime_ns(time.CLOCK_MONOTONIC)))
     B["layer"].set_stage_timing(True)
     B["layer"].forward(cts, BERT_TOKENS)
     return {"ms_per_layer_call": ms, "tokens": BERT_TOKENS, "stages_ms": dict(B["layer"].stage_ms()),
             "autotune_modup": B["autotune"].get("modup", []), "autotune_moddown": B["autotune"].get("moddown", [])}
+
+
+def fused_mode(mode):
+    """the autotuner's choice for a conversion fused into the NTT column pass"""
+    if mode < 0:
+        return "unfused"
+    return "conversion in the column pass, %s outputs per workgroup, %d-column tiles" % (
+        mode & 255 or "default", 4 if mode & 256 else 8)
 
 
 def pack_sets(lens, capacity):
@@ -706,14 +717,10 @@ def main():
                               for e in B["autotune"]["ntt"] if e["col"] or e["row"]],
                 "dot_plain_layouts": ["%d limbs: %d" % (d["limbs"], d["layout"]) for d in B["autotune"]["dot_plain"]],
                 "modup": ["batch %d at %d limbs: %s (%.1f vs %.1f us unfused)"
-                          % (e["batch"], e["limbs"], "unfused" if e["mode"] < 0 else
-                             "conversion in the column pass, %s outputs per workgroup"
-                             % ("default" if e["mode"] == 0 else e["mode"]), e["us"], e["unfused_us"])
+                          % (e["batch"], e["limbs"], fused_mode(e["mode"]), e["us"], e["unfused_us"])
                           for e in B["autotune"].get("modup", [])],
                 "moddown": ["batch %d at %d limbs: %s (%.1f vs %.1f us unfused)"
-                            % (e["batch"], e["limbs"], "unfused" if e["mode"] < 0 else
-                               "conversion in the column pass, %s outputs per workgroup"
-                               % ("default" if e["mode"] == 0 else e["mode"]), e["us"], e["unfused_us"])
+                            % (e["batch"], e["limbs"], fused_mode(e["mode"]), e["us"], e["unfused_us"])
                             for e in B["autotune"].get("moddown", [])],
             },
         },

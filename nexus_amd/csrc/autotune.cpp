@@ -188,7 +188,8 @@ std::string autotune(Context &c, int reps)
     if (c.alpha > 1 && ks_modup_col_ok(c.tables, (uint32_t)c.alpha)) {
         // hybrid ModUp at the top level: the conversion fused into the forward
         // column pass (outputs per workgroup: default rule, 4, 8, 12, all of a
-        // digit) against separate conversion and column-pass launches
+        // digit; 8- or 4-column tiles, +256) against separate conversion and
+        // column-pass launches
         const size_t l = c.L;
         for (uint32_t B : {1u, 2u, 4u, 8u, 16u, 32u}) {
             std::vector<uint64_t> h;
@@ -209,7 +210,7 @@ std::string autotune(Context &c, int reps)
             int best_m = -1;
             std::string times;
             char buf[64];
-            for (int m : {0, 4, 8, 12, 64}) {
+            for (int m : {0, 4, 8, 12, 64, 256, 256 + 8, 256 + 12}) {
                 ks_modup_col_set(B, m);
                 DevBuf T = ks_modup(c, tg.get(), B, l);
                 const bool same = download(c, T.get(), words) == ref;
@@ -268,7 +269,7 @@ std::string autotune(Context &c, int reps)
             int best_m = -1;
             std::string times;
             char buf[64];
-            for (int m : {0, 4, 8, 12, 64}) {
+            for (int m : {0, 4, 8, 12, 64, 256, 256 + 8, 256 + 12}) {
                 ks_moddown_col_set(2 * B, m);
                 run();
                 const bool same = download(c, out.get(), ow) == ref;

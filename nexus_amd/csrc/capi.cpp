@@ -448,6 +448,7 @@ int nx_kswitch_key_upload(nx_kswitch_keys *keys, uint32_t elt, const uint64_t *d
         nx::Context &c = *keys->impl.ctx;
         const size_t words = c.key_words();
         nx::DevBuf &b = keys->impl.keys[elt];
+        keys->impl.seeds.erase(elt);  // arbitrary words: the uniform half is read, not regenerated
         b.reset(&c, words);
         NX_HIP(hipMemcpyAsync(b.get(), data, words * 8, hipMemcpyHostToDevice, c.stream));
         NX_HIP(hipStreamSynchronize(c.stream));

@@ -17,6 +17,7 @@ namespace {
 
 struct Prepared {
     std::vector<const uint64_t *> keys;
+    std::vector<const uint64_t *> seeds;  // per key: its uniform half's digit seeds, or null
     std::vector<const uint32_t *> perms;
     std::vector<size_t> index;  // position of each non-identity step in `steps`
 };
@@ -30,6 +31,7 @@ Prepared prepare(Context &c, const std::vector<int> &steps, const KSwitchKeys &g
         auto it = gk.keys.find(elt);
         require(it != gk.keys.end(), "Galois key not present in hoisting");
         p.keys.push_back(it->second.get());
+        p.seeds.push_back(gk.seeds_of(elt));
         p.perms.push_back(c.galois_table(elt));
         p.index.push_back(k);
     }
@@ -65,7 +67,7 @@ void op_rotate_hoisted(const Ciphertext &a, const std::vector<int> &steps, const
             DevBuf acc(&c, cnt * 2 * nout * n);
             ks_inner_product_rot(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
                                  (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
-                                 c.stream);
+                                 c.stream, nullptr, p.seeds.data() + base);
             bool col_done = false;
             DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false, &col_done);
             // each result written in place by the transform's last pass:
@@ -134,18 +136,19 @@ void op_rotate_hoisted_many(const std::vector<const Ciphertext *> &in, const std
         const size_t P = B * R;
         for (size_t base = 0; base < P; base += (size_t)kMaxHoist) {
             const size_t cnt = std::min<size_t>((size_t)kMaxHoist, P - base);
-            std::vector<const uint64_t *> keys(cnt), tps(cnt);
+            std::vector<const uint64_t *> keys(cnt), tps(cnt), seeds(cnt);
             std::vector<const uint32_t *> perms(cnt);
             for (size_t r = 0; r < cnt; r++) {
                 const size_t b = (base + r) / R, k = (base + r) % R;
                 keys[r] = p.keys[k];
+                seeds[r] = p.seeds[k];
                 perms[r] = p.perms[k];
                 tps[r] = tp[b];
             }
             DevBuf acc(&c, cnt * 2 * nout * n);
             ks_inner_product_rot(c.tables, acc.get(), nullptr, keys.data(), perms.data(), (uint32_t)cnt,
                                  (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream,
-                                 tps.data());
+                                 tps.data(), seeds.data());
             bool col_done = false;
             DevBuf tmp = ks_moddown_prepare(c, acc.get(), cnt, l, false, &col_done);
             NttEpilogue e;
@@ -191,7 +194,7 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
     for (size_t g = 0; g < G; g++) {
         const auto &in = groups[g];
         require(!in.empty() && in.size() == steps[g].size(), "inputs and steps must match");
-        std::vector<const uint64_t *> ins, c1s, keys;
+        std::vector<const uint64_t *> ins, c1s, keys, seeds;
         std::vector<const uint32_t *> perms, rperms;
         for (size_t i = 0; i < in.size(); i++) {
             check_input(*in[i], gk);
@@ -207,6 +210,7 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
             perms.push_back(p.perms[0]);
             rperms.push_back(p.perms[0]);
             keys.push_back(p.keys[0]);
+            seeds.push_back(p.seeds[0]);
             c1s.push_back(in[i]->data.get() + pw);
         }
         uint64_t *og = outs.get() + g * 2 * pw;
@@ -230,7 +234,7 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
             DevBuf T = ks_modup_list(c, c1s.data() + base, cnt, l);
             ks_inner_product_rot_sum(c.tables, ag, T.get(), keys.data() + base, rperms.data() + base,
                                      (uint32_t)cnt, (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
-                                     base > 0, c.stream, nout * digits * n);
+                                     base > 0, c.stream, nout * digits * n, seeds.data() + base);
         }
     }
     ks_moddown(c, outs.get(), acc.get(), G, l, false);
@@ -287,7 +291,7 @@ void op_hoisting(const Ciphertext &a, const std::vector<int> &steps, const KSwit
             const size_t cnt = std::min<size_t>(kMaxHoist, p.keys.size() - base);
             ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), p.keys.data() + base, p.perms.data() + base,
                                      (uint32_t)cnt, (uint32_t)c.digits(l), (uint32_t)c.key_limbs(), outp,
-                                     (uint32_t)nout, base > 0, c.stream);
+                                     (uint32_t)nout, base > 0, c.stream, 0, p.seeds.data() + base);
         }
         ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
     }
@@ -321,7 +325,7 @@ void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<
     out.ntt = true;
     out.data.reset(&c, 2 * pw);
     // data-basis part: sigma_i(c0_i), and (c0, c1) of the identity terms
-    std::vector<const uint64_t *> ins, c1s, keys;
+    std::vector<const uint64_t *> ins, c1s, keys, seeds;
     std::vector<const uint32_t *> perms, rperms;
     for (size_t i = 0; i < in.size(); i++) {
         ins.push_back(in[i]->data.get());
@@ -334,6 +338,7 @@ void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<
         perms.push_back(p.perms[0]);
         rperms.push_back(p.perms[0]);
         keys.push_back(p.keys[0]);
+        seeds.push_back(p.seeds[0]);
         c1s.push_back(in[i]->data.get() + pw);
     }
     {
@@ -354,7 +359,7 @@ void op_rotate_sum(const std::vector<const Ciphertext *> &in, const std::vector<
             DevBuf T = ks_modup_list(c, c1s.data() + base, cnt, l);
             ks_inner_product_rot_sum(c.tables, acc.get(), T.get(), keys.data() + base, rperms.data() + base,
                                      (uint32_t)cnt, (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
-                                     base > 0, c.stream, nout * digits * n);
+                                     base > 0, c.stream, nout * digits * n, seeds.data() + base);
         }
         ks_moddown(c, out.data.get(), acc.get(), 1, l, false);
     }

@@ -177,8 +177,12 @@ void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, u
                      const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
 // Inner product: acc[b][k][I][c] = sum_j T[b][I][j][c] * key[j][k][kp(I)][c] mod q_kp(I)
 // key layout [digits][2][key_limbs][n], shared by the batch.
+// aseed (device array, one seed per digit) non-null: poly 1 of every key digit
+// is the sampler's uniform residues, ctr_uniform(aseed[j], kp n + c, q_kp),
+// regenerated in the kernel instead of read (keys generated by this engine).
 void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t B,
-                      uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s);
+                      uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s,
+                      const uint64_t *aseed = nullptr);
 
 // Hybrid key switching (alpha special primes): fast basis extension of the
 // INTT'd limbs coef[B][l][n] digit by digit into T[B][I][j][n] for every output
@@ -221,8 +225,9 @@ void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targe
 // (ks_modup_copy_own).  Needs ks_modup_col_ok: every prime below 2^51,
 // 2 <= alpha <= 8, 2^14 <= n <= 2^17, the double-precision factor table.
 bool ks_modup_col_ok(const DeviceTables &t, uint32_t alpha);
-// per power-of-two bucket of B: 0 = fused with the default outputs per
-// workgroup, -1 = unfused, k > 0 = fused with k outputs per workgroup
+// per power-of-two bucket of B: -1 = unfused; otherwise fused, with k =
+// mode & 255 outputs per workgroup (0: the default rule) and 4-column tiles
+// instead of 8 when mode & 256
 void ks_modup_col_set(uint32_t B, int mode);
 int ks_modup_col_get(uint32_t B);
 void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,
@@ -299,13 +304,13 @@ constexpr int kMaxHoist = 32;
 void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                           const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
                           const uint8_t *out_primes_host, uint32_t nout, hipStream_t s,
-                          const uint64_t *const *Ts = nullptr);
+                          const uint64_t *const *Ts = nullptr, const uint64_t *const *aseeds = nullptr);
 // t_stride > 0: rotation r reads its own ModUp at T + r t_stride (a sum of
 // rotations of different inputs) instead of the shared T.
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                               const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
                               const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s,
-                              uint64_t t_stride = 0);
+                              uint64_t t_stride = 0, const uint64_t *const *aseeds = nullptr);
 // out[2][l][n] = sum_i (sigma_i(in_i[0]), 0), or (in_i[0], in_i[1]) where
 // perms[i] is null (count <= kMaxHoist; in_i are [2][l][n] ciphertexts).
 struct RotBaseArgs {

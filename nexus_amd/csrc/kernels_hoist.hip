@@ -20,7 +20,8 @@ struct OutPrimes {
 struct RotArgs {
     const uint64_t *key[kMaxHoist];
     const uint32_t *perm[kMaxHoist];
-    const uint64_t *t[kMaxHoist];  // k_ks_inner_rot: the ModUp digits of rotation r's input
+    const uint64_t *t[kMaxHoist];      // k_ks_inner_rot: the ModUp digits of rotation r's input
+    const uint64_t *aseed[kMaxHoist];  // non-null: the key's uniform half regenerated (ctr_uniform)
 };
 
 // acc[r][k][I][c] = sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]  (one rotation r per grid row group)
@@ -37,11 +38,13 @@ __global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod,
     const uint64_t *t = args.t[r] + (uint64_t)I * l * n + src;
     const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
     const uint64_t kstride = (uint64_t)key_limbs * n;
+    const uint64_t *aseed = args.aseed[r];
+    const uint64_t ctr = (uint64_t)kp * n + c;
     uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
     for (uint32_t j = 0; j < l; j++) {
         const uint64_t tv = t[(uint64_t)j * n];
         const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
-        const uint64_t a = kj[0], b = kj[kstride];
+        const uint64_t a = kj[0], b = aseed ? ctr_uniform(aseed[j], ctr, m.q) : kj[kstride];
         uint64_t plo = tv * a, phi = mulhi64(tv, a);
         lo0 += plo;
         hi0 += phi + (lo0 < plo);
@@ -78,10 +81,12 @@ __global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs arg
         const uint32_t src = args.perm[r][c];
         const uint64_t *t = T + r * t_stride + (uint64_t)I * l * n + src;
         const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
+        const uint64_t *aseed = args.aseed[r];
+        const uint64_t ctr = (uint64_t)kp * n + c;
         for (uint32_t j = 0; j < l; j++) {
             const uint64_t tv = t[(uint64_t)j * n];
             const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
-            const uint64_t a = kj[0], b = kj[kstride];
+            const uint64_t a = kj[0], b = aseed ? ctr_uniform(aseed[j], ctr, m.q) : kj[kstride];
             uint64_t plo = tv * a, phi = mulhi64(tv, a);
             lo0 += plo;
             hi0 += phi + (lo0 < plo);
@@ -133,7 +138,7 @@ OutPrimes make_out(const uint8_t *p, uint32_t nout)
 }
 
 RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uint32_t count,
-                  const uint64_t *T = nullptr, const uint64_t *const *Ts = nullptr)
+                  const uint64_t *const *aseeds, const uint64_t *T = nullptr, const uint64_t *const *Ts = nullptr)
 {
     if (count > (uint32_t)kMaxHoist) throw std::invalid_argument("hoisted inner product: too many rotations");
     RotArgs a;
@@ -141,33 +146,47 @@ RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uin
         a.key[i] = i < count ? keys[i] : nullptr;
         a.perm[i] = i < count ? perms[i] : nullptr;
         a.t[i] = i < count ? (Ts ? Ts[i] : T) : nullptr;
+        a.aseed[i] = (i < count && aseeds) ? aseeds[i] : nullptr;
     }
     return a;
+}
+
+// key words read per (output prime, digit) of one rotation: 2, or 1 with the
+// uniform half regenerated
+double key_words_read(const uint64_t *const *aseeds, uint32_t count)
+{
+    if (!aseeds || !count) return 2.0;
+    uint32_t seeded = 0;
+    for (uint32_t i = 0; i < count; i++) seeded += aseeds[i] ? 1u : 0u;
+    return 2.0 - (double)seeded / count;
 }
 
 }  // namespace
 
 void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                           const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
-                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s, const uint64_t *const *Ts)
+                          const uint8_t *out_primes_host, uint32_t nout, hipStream_t s, const uint64_t *const *Ts,
+                          const uint64_t *const *aseeds)
 {
     const uint32_t n = 1u << t.logn;
-    ProfScope prof("ks_inner_rot", s, (double)n * count * nout * (24.0 * l + 16.0 + 4.0));
+    ProfScope prof("ks_inner_rot", s,
+                   (double)n * count * nout * ((8.0 + 8.0 * key_words_read(aseeds, count)) * l + 16.0 + 4.0));
     hipLaunchKernelGGL(k_ks_inner_rot, dim3((n + kBlock - 1) / kBlock, count * nout), dim3(kBlock), 0, s, acc,
-                       make_args(keys, perms, count, T, Ts), t.mod, make_out(out_primes_host, nout), l, key_limbs,
-                       nout, n);
+                       make_args(keys, perms, count, aseeds, T, Ts), t.mod, make_out(out_primes_host, nout), l,
+                       key_limbs, nout, n);
 }
 
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
                               const uint32_t *const *perms, uint32_t count, uint32_t l, uint32_t key_limbs,
                               const uint8_t *out_primes_host, uint32_t nout, bool accumulate, hipStream_t s,
-                              uint64_t t_stride)
+                              uint64_t t_stride, const uint64_t *const *aseeds)
 {
     const uint32_t n = 1u << t.logn;
-    ProfScope prof("ks_inner_rot_sum", s, (double)n * nout * (count * (24.0 * l + 4.0) + 16.0));
+    ProfScope prof("ks_inner_rot_sum", s,
+                   (double)n * nout * (count * ((8.0 + 8.0 * key_words_read(aseeds, count)) * l + 4.0) + 16.0));
     hipLaunchKernelGGL(k_ks_inner_rot_sum, dim3((n + kBlock - 1) / kBlock, nout), dim3(kBlock), 0, s, acc, T,
-                       make_args(keys, perms, count), count, t.mod, make_out(out_primes_host, nout), l, key_limbs,
-                       nout, n, accumulate ? 1 : 0, t_stride);
+                       make_args(keys, perms, count, aseeds), count, t.mod, make_out(out_primes_host, nout), l,
+                       key_limbs, nout, n, accumulate ? 1 : 0, t_stride);
 }
 
 void rot_base(const DeviceTables &t, uint64_t *out, const uint64_t *const *in, const uint32_t *const *perms,

@@ -602,10 +602,10 @@ struct ModUpColArgs {
     uint32_t outp[kMaxPrimes];  // words: uniform indices take scalar loads
 };
 
-template <int LOGS, int A>
+template <int LOGS, int A, int CW>
 __global__ __launch_bounds__(kThreads) void k_modup_col(ModUpColArgs a)
 {
-    using G = Geometry<LOGS, kCol, 8>;
+    using G = Geometry<LOGS, kCol, CW>;
     constexpr int PER = G::TILE / kThreads;
     __shared__ double lds[G::TILE];
     const uint32_t n = 1u << a.logn;
@@ -667,9 +667,9 @@ __global__ __launch_bounds__(kThreads) void k_modup_col(ModUpColArgs a)
         const double *tw = a.twd + (uint64_t)kp * n;
         const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
         if (lazy)
-            stages_f<LOGS, kCol, true, 8, true>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, true>(lds, tw, q, qinv, n, N1, col0);
         else
-            stages_f<LOGS, kCol, true, 8, false>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, false>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.T + (((uint64_t)b * a.nout + I) * a.digits + j) * n;
 #pragma unroll
         for (int t = 0; t < PER; t++) {
@@ -772,10 +772,10 @@ struct ModDownColArgs {
     uint32_t inp[kMaxPrimes];
 };
 
-template <int LOGS, int A>
+template <int LOGS, int A, int CW>
 __global__ __launch_bounds__(kThreads) void k_moddown_col(ModDownColArgs a)
 {
-    using G = Geometry<LOGS, kCol, 8>;
+    using G = Geometry<LOGS, kCol, CW>;
     constexpr int PER = G::TILE / kThreads;
     __shared__ double lds[G::TILE];
     const uint32_t n = 1u << a.logn;
@@ -831,9 +831,9 @@ __global__ __launch_bounds__(kThreads) void k_moddown_col(ModDownColArgs a)
         const double *tw = a.twd + (uint64_t)i * n;
         const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
         if (lazy)
-            stages_f<LOGS, kCol, true, 8, true>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, true>(lds, tw, q, qinv, n, N1, col0);
         else
-            stages_f<LOGS, kCol, true, 8, false>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, false>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.z + ((uint64_t)p * a.lo + i) * n;
 #pragma unroll
         for (int t = 0; t < PER; t++) {
@@ -1023,6 +1023,7 @@ void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
 // column pass as separate launches), k > 0 = on with k outputs per workgroup.
 // Process-wide; set by the autotuner.
 static std::atomic<int> g_modup_col[kTuneBuckets];
+constexpr int kColNarrow = 256;  // mode bit: 4-column tiles
 
 void ks_modup_col_set(uint32_t B, int mode) { g_modup_col[tune_bucket(B)].store(mode); }
 
@@ -1056,15 +1057,18 @@ void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *co
     int l1, l2;
     split_logn(t.logn, l1, l2);
     const uint32_t n = 1u << t.logn;
-    const uint32_t tiles = n / ((1u << l1) * 8u);
+    // mode: outputs per workgroup in its low byte (0: default rule), 4-column
+    // tiles (half the registers per thread) when bit 8 is set
+    const int mode = g_modup_col[tune_bucket(B)].load();
+    const uint32_t cw = (mode > 0 && (mode & kColNarrow)) ? 4u : 8u;
+    const uint32_t tiles = n / ((1u << l1) * cw);
     // outputs per workgroup: enough workgroups to fill the device, at least 4
     // outputs per read of the digit's inputs
     const uint32_t count = nout - std::min(alpha, l);  // most outputs of a digit (a partial digit has more)
     const uint32_t want = (2048u + tiles * B * a.digits - 1) / (tiles * B * a.digits);
     uint32_t chunk = (count + want - 1) / want;
     chunk = std::max(chunk, std::min(4u, count));
-    const int mode = g_modup_col[tune_bucket(B)].load();
-    if (mode > 0) chunk = std::min((uint32_t)mode, count);
+    if (mode > 0 && (mode & 255)) chunk = std::min((uint32_t)(mode & 255), count);
     a.chunk = chunk;
     a.chunks = (nout - 1 + chunk - 1) / chunk;  // covers a partial last digit (nout - s <= nout - 1 outputs)
     uint32_t outs = 0;
@@ -1074,7 +1078,10 @@ void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *co
     const dim3 grid(tiles, B * a.digits * a.chunks);
 #define NX_MC(LOGS, A)                                                                          \
     if (l1 == LOGS && alpha == A) {                                                           \
-        hipLaunchKernelGGL((k_modup_col<LOGS, A>), grid, dim3(kThreads), 0, s, a);             \
+        if (cw == 4)                                                                          \
+            hipLaunchKernelGGL((k_modup_col<LOGS, A, 4>), grid, dim3(kThreads), 0, s, a);      \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_modup_col<LOGS, A, 8>), grid, dim3(kThreads), 0, s, a);      \
         return;                                                                               \
     }
     NX_MC(7, 2) NX_MC(7, 3) NX_MC(7, 4) NX_MC(7, 5) NX_MC(7, 6) NX_MC(7, 7) NX_MC(7, 8)
@@ -1151,12 +1158,13 @@ void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *
     int l1, l2;
     split_logn(t.logn, l1, l2);
     const uint32_t n = 1u << t.logn;
-    const uint32_t tiles = n / ((1u << l1) * 8u);
+    const int mode = g_moddown_col[tune_bucket(polys)].load();
+    const uint32_t cw = (mode > 0 && (mode & kColNarrow)) ? 4u : 8u;
+    const uint32_t tiles = n / ((1u << l1) * cw);
     const uint32_t want = (2048u + tiles * polys - 1) / (tiles * polys);
     uint32_t chunk = (lo + want - 1) / want;
     chunk = std::max(chunk, std::min(4u, lo));
-    const int mode = g_moddown_col[tune_bucket(polys)].load();
-    if (mode > 0) chunk = std::min((uint32_t)mode, lo);
+    if (mode > 0 && (mode & 255)) chunk = std::min((uint32_t)(mode & 255), lo);
     a.chunk = chunk;
     a.chunks = (lo + chunk - 1) / chunk;
     ProfScope prof("ks_moddown_col", s,
@@ -1164,7 +1172,10 @@ void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *
     const dim3 grid(tiles, polys * a.chunks);
 #define NX_MD(LOGS, AA)                                                                         \
     if (l1 == LOGS && A == AA) {                                                              \
-        hipLaunchKernelGGL((k_moddown_col<LOGS, AA>), grid, dim3(kThreads), 0, s, a);          \
+        if (cw == 4)                                                                          \
+            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 4>), grid, dim3(kThreads), 0, s, a);   \
+        else                                                                                  \
+            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8>), grid, dim3(kThreads), 0, s, a);   \
         return;                                                                               \
     }
     NX_MD(7, 2) NX_MD(7, 3) NX_MD(7, 4) NX_MD(7, 5) NX_MD(7, 6) NX_MD(7, 7) NX_MD(7, 8) NX_MD(7, 9)

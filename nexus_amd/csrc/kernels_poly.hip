@@ -255,8 +255,10 @@ __global__ void k_modup_expand(uint64_t *T, const uint64_t *coef, const ModConst
     T[(uint64_t)row * n + c] = v >= m.q ? barrett64(v, m) : v;
 }
 
-__global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key, const ModConst *mod, PrimeMap outp,
-                           uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
+// aseed non-null: poly 1 of key digit j (uniform) is ctr_uniform(aseed[j], kp n + c, q_kp), regenerated
+// instead of read
+__global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key, const uint64_t *aseed,
+                           const ModConst *mod, PrimeMap outp, uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -269,10 +271,11 @@ __global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key
     acc += (uint64_t)b * 2 * nout * n;
     const uint64_t *k0 = key + (uint64_t)kp * n + c;
     const uint64_t kstride = (uint64_t)key_limbs * n;  // between poly 0 and 1 of a digit
+    const uint64_t ctr = (uint64_t)kp * n + c;
     for (uint32_t j = 0; j < l; j++) {
         const uint64_t tv = t[(uint64_t)j * n];
         const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
-        const uint64_t a = kj[0], b = kj[kstride];
+        const uint64_t a = kj[0], b = aseed ? ctr_uniform(aseed[j], ctr, m.q) : kj[kstride];
         uint64_t plo = tv * a, phi = mulhi64(tv, a);
         lo0 += plo;
         hi0 += phi + (lo0 < plo);
@@ -286,17 +289,7 @@ __global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key
 }
 
 // ---- sampling (counter-based; deterministic given the seed)
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
-{
-    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
-    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
-    return z ^ (z >> 31);
-}
-
-__device__ __forceinline__ uint64_t rand64(uint64_t seed, uint64_t ctr)
-{
-    return mix64(seed ^ mix64(ctr + 0x9e3779b97f4a7c15ull));
-}
+__device__ __forceinline__ uint64_t rand64(uint64_t seed, uint64_t ctr) { return ctr_rand64(seed, ctr); }
 
 __global__ void k_sample_uniform(uint64_t *out, const ModConst *mod, PrimeMap pm, uint32_t limbs, uint32_t n,
                                  uint64_t seed)
@@ -306,7 +299,7 @@ __global__ void k_sample_uniform(uint64_t *out, const ModConst *mod, PrimeMap pm
     const uint32_t row = blockIdx.y, limb = row % limbs;
     const uint64_t q = mod[pm.p[limb]].q;
     const uint64_t idx = (uint64_t)row * n + c;
-    out[idx] = mulhi64(rand64(seed, idx), q);
+    out[idx] = ctr_uniform(seed, idx, q);
 }
 
 __global__ void k_sample_cbd(uint64_t *out, const ModConst *mod, PrimeMap pm, uint32_t limbs, uint32_t n,
@@ -505,11 +498,13 @@ void ks_modup_expand(const DeviceTables &t, uint64_t *T, const uint64_t *coef, u
 }
 
 void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *key, uint32_t B,
-                      uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s)
+                      uint32_t l, uint32_t key_limbs, const uint8_t *out_primes_host, uint32_t nout, hipStream_t s,
+                      const uint64_t *aseed)
 {
-    // T limbs (8 n) and two key limbs (16 n) per (output prime, digit), two outputs per prime
-    ProfScope prof("ks_inner", s, (double)(1u << t.logn) * B * nout * (24.0 * l + 16.0));
-    hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, B * nout), dim3(kBlock), 0, s, acc, T, key, t.mod,
+    // T limbs (8 n) and the key limbs read (16 n, or 8 n with a regenerated
+    // uniform half) per (output prime, digit), two outputs per prime
+    ProfScope prof("ks_inner", s, (double)(1u << t.logn) * B * nout * ((aseed ? 16.0 : 24.0) * l + 16.0));
+    hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, B * nout), dim3(kBlock), 0, s, acc, T, key, aseed, t.mod,
                        make_map(out_primes_host, nout), l, key_limbs, nout, 1u << t.logn);
 }
 

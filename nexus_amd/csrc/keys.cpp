@@ -56,12 +56,17 @@ void encrypt_zero_sk(Context &c, const SecretKey &sk, uint64_t *out, size_t coun
 // keygenerator.cpp:303-336.  Digit j adds (P mod q_i) new_key to the limbs i of
 // its primes (one prime with one special prime; alpha consecutive primes in
 // hybrid mode), P the product of the special primes.
-void kswitch_key(Context &c, const SecretKey &sk, const uint64_t *new_key, uint64_t seed, DevBuf &out)
+// aseeds receives, per digit, the seed of the digit's uniform half (poly 1):
+// its residues are sample_uniform's words under that seed.
+void kswitch_key(Context &c, const SecretKey &sk, const uint64_t *new_key, uint64_t seed, DevBuf &out,
+                 DevBuf &aseeds)
 {
     const size_t L = c.L, K = c.key_limbs(), n = c.n;
     out.reset(&c, c.key_words());
+    std::vector<uint64_t> hs(c.dnum);
     for (size_t j = 0; j < c.dnum; j++) {
         uint64_t *dj = out.get() + j * 2 * K * n;
+        hs[j] = subseed(subseed(seed, 100 + j), 1);  // the uniform sample's seed inside encrypt_zero_sk
         encrypt_zero_sk(c, sk, dj, K, K * n, subseed(seed, 100 + j));
         for (size_t i = j * c.alpha; i < std::min(L, (j + 1) * c.alpha); i++) {
             const uint64_t qi = c.moduli[i];
@@ -71,6 +76,9 @@ void kswitch_key(Context &c, const SecretKey &sk, const uint64_t *new_key, uint6
                             make_ulonglong2(factor, shoup_quotient(factor, qi)), (uint32_t)i, c.stream);
         }
     }
+    aseeds.reset(&c, c.dnum);
+    NX_HIP(hipMemcpyAsync(aseeds.get(), hs.data(), hs.size() * 8, hipMemcpyHostToDevice, c.stream));
+    NX_HIP(hipStreamSynchronize(c.stream));  // hs is a local
 }
 
 }  // namespace
@@ -120,7 +128,7 @@ void relin_key_generate(Context &c, const SecretKey &sk, uint64_t seed, KSwitchK
     DevBuf s2(&c, K * n);
     poly_mul_add(c.tables, s2.get(), sk.ntt.get(), sk.ntt.get(), nullptr, (uint32_t)K, p, 0, c.stream);
     out.ctx = &c;
-    kswitch_key(c, sk, s2.get(), seed, out.keys[0]);
+    kswitch_key(c, sk, s2.get(), seed, out.keys[0], out.seeds[0]);
     NX_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -136,7 +144,7 @@ void galois_keys_generate(Context &c, const SecretKey &sk, const uint32_t *elts,
         if (out.keys.count(elt)) continue;
         const uint32_t *perm = c.galois_table(elt);
         poly_permute(rot.get(), sk.ntt.get(), perm, 1, (uint32_t)K, c.logn, c.stream);
-        kswitch_key(c, sk, rot.get(), subseed(seed, elt, 7), out.keys[elt]);
+        kswitch_key(c, sk, rot.get(), subseed(seed, elt, 7), out.keys[elt], out.seeds[elt]);
     }
     NX_HIP(hipStreamSynchronize(c.stream));
 }

@@ -119,7 +119,7 @@ void mm_expand(Context &c, const Ciphertext &ct, uint32_t m, const uint32_t *elt
         poly_permute(perm_all.get(), cur.data.get(), perm, (uint32_t)(2 * S), (uint32_t)l, c.logn, c.stream);
         poly_copy_limbs(c.tables, rot.get(), ctw, perm_all.get(), ctw, (uint32_t)S, (uint32_t)l, c.stream);
         poly_copy_limbs(c.tables, tgt.get(), l * n, perm_all.get() + l * n, ctw, (uint32_t)S, (uint32_t)l, c.stream);
-        key_switch_batch(c, rot.get(), tgt.get(), S, l, it->second, true);
+        key_switch_batch(c, rot.get(), tgt.get(), S, l, it->second, true, nullptr, gk.seeds_of(elt));
         // next round: [cur + rot | cur * x^(2n - 2^i) + rot * x^((2n - 2^i) * elt mod 2n)]
         CtBatch nxt;
         nxt.ctx = &c;

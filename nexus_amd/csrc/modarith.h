@@ -40,6 +40,20 @@ NX_HD uint64_t mulhi64(uint64_t a, uint64_t b)
 #endif
 }
 
+// Counter-based generator of the key sampler: the residue of sample_uniform's
+// word `ctr` under `seed` is ctr_uniform(seed, ctr, q).  Key-switching keys
+// generated here regenerate their uniform half from it instead of reading it.
+NX_HD uint64_t ctr_mix64(uint64_t z)
+{
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+NX_HD uint64_t ctr_rand64(uint64_t seed, uint64_t ctr) { return ctr_mix64(seed ^ ctr_mix64(ctr + 0x9e3779b97f4a7c15ull)); }
+
+NX_HD uint64_t ctr_uniform(uint64_t seed, uint64_t ctr, uint64_t q) { return mulhi64(ctr_rand64(seed, ctr), q); }
+
 // Shoup lazy: a * w mod q in [0, 2q) for any a < 2^64.
 NX_HD uint64_t shoup_lazy(uint64_t a, uint64_t w, uint64_t wp, uint64_t q)
 {

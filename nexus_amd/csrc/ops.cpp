@@ -308,7 +308,7 @@ DevBuf ks_modup_list(Context &c, const uint64_t *const *targets, size_t B, size_
 }
 
 void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t B, size_t l, const DevBuf &key,
-                      bool zero_c1, const uint64_t *base)
+                      bool zero_c1, const uint64_t *base, const uint64_t *aseed)
 {
     tally("ks", l, (double)B);
     const size_t n = c.n, nout = l + c.alpha;
@@ -318,7 +318,7 @@ void key_switch_batch(Context &c, uint64_t *cts, const uint64_t *targets, size_t
     // inner product with the device-resident key
     DevBuf acc(&c, B * 2 * nout * n);
     ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)B, (uint32_t)c.digits(l),
-                     (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
+                     (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream, aseed);
     ks_moddown(c, cts, acc.get(), B, l, zero_c1, base);
 }
 
@@ -392,14 +392,15 @@ void ks_moddown(Context &c, uint64_t *cts, uint64_t *acc, size_t B, size_t l, bo
     ntt_fwd_epi(c, tmp.get(), 2 * B, l, l * n, e, col_done);
 }
 
-static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1)
+static void key_switch_impl(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key, bool zero_c1,
+                            const uint64_t *aseed)
 {
-    key_switch_batch(c, ct.data.get(), target, 1, ct.limbs, key, zero_c1);
+    key_switch_batch(c, ct.data.get(), target, 1, ct.limbs, key, zero_c1, nullptr, aseed);
 }
 
 void key_switch_into(Context &c, Ciphertext &ct, const uint64_t *target, const DevBuf &key)
 {
-    key_switch_impl(c, ct, target, key, false);
+    key_switch_impl(c, ct, target, key, false, nullptr);
 }
 
 static const DevBuf &find_key(const KSwitchKeys &k, uint32_t elt, const char *msg)
@@ -426,7 +427,8 @@ void op_relinearize(const Ciphertext &a, const KSwitchKeys &rk, Ciphertext &dst)
     out.ntt = true;
     out.data.reset(&c, 2 * pw);
     // out = (c0, c1) + key_switch(c2): c0, c1 are read where they lie
-    key_switch_batch(c, out.data.get(), a.data.get() + 2 * pw, 1, a.limbs, key, false, a.data.get());
+    key_switch_batch(c, out.data.get(), a.data.get() + 2 * pw, 1, a.limbs, key, false, a.data.get(),
+                     rk.seeds_of(0));
     dst = std::move(out);
 }
 
@@ -450,7 +452,7 @@ void op_apply_galois(const Ciphertext &a, uint32_t elt, const KSwitchKeys &gk, C
     DevBuf t(&c, pw);
     poly_permute(out.data.get(), a.data.get(), perm, 1, (uint32_t)a.limbs, c.logn, c.stream);
     poly_permute(t.get(), a.data.get() + pw, perm, 1, (uint32_t)a.limbs, c.logn, c.stream);
-    key_switch_impl(c, out, t.get(), key, true);
+    key_switch_impl(c, out, t.get(), key, true, gk.seeds_of(elt));
     dst = std::move(out);
 }
 
@@ -661,7 +663,7 @@ void op_multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, co
         }
         DevBuf c01(&c, cnt * 2 * pw), c2(&c, cnt * pw), ks(&c, cnt * 2 * pw);
         poly_tensor_many(c.tables, c01.get(), c2.get(), pa.data(), pb.data(), (uint32_t)cnt, (uint32_t)l, c.stream);
-        key_switch_batch(c, ks.get(), c2.get(), cnt, l, key, false, c01.get());
+        key_switch_batch(c, ks.get(), c2.get(), cnt, l, key, false, c01.get(), rk.seeds_of(0));
         std::vector<const uint64_t *> x;
         std::vector<uint64_t *> o;
         for (size_t r = 0; r < cnt; r++) {
@@ -729,7 +731,7 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
         {
             DevBuf T = ks_modup(c, c2.get(), cnt, l);
             ks_inner_product(c.tables, acc.get(), T.get(), key.get(), (uint32_t)cnt, (uint32_t)c.digits(l),
-                             (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream);
+                             (uint32_t)c.key_limbs(), outp, (uint32_t)nout, c.stream, rk.seeds_of(0));
         }
         // limb q_(l-1) of Y = acc + P d, then (q_(l-1), p_0..p_(alpha-1)) to coefficients
         rns_add_limb_scaled(c.tables, acc.get(), nout * n, c01.get(), pw, (uint32_t)lo, c.mdr_pmod[l],

@@ -400,6 +400,7 @@ void load_kswitch_keys(Context &c, const uint8_t *data, size_t n, bool galois, K
         }
         const uint32_t elt = galois ? (uint32_t)(2 * index + 1) : 0u;
         DevBuf &b = out.keys[elt];
+        out.seeds.erase(elt);  // loaded words: the uniform half is read, not regenerated
         b.reset(&c, h.size());
         upload(c, b.get(), h);
     }

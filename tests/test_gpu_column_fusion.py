@@ -7,7 +7,9 @@ two-pass transform, every prime below 2^51), bit-exact against the oracle:
   the ModDown's centring of the special limb inside the column pass;
 * hybrid multiply-relinearise-rescale in one rounding: the conversion by
   P q_(l-1) inside the column pass equals the per-ciphertext fused products
-  word for word (the batched and single calls share no launch).
+  word for word (the batched and single calls share no launch);
+* key-switching keys generated here, whose uniform half the inner products
+  regenerate from its seed, against the same words uploaded as a plain key.
 """
 import sys
 
@@ -82,3 +84,38 @@ def test_fused_relin_rescale_batch_matches_single():
         dec = nx.decode(ctx, nx.decrypt(ctx, sk, many[i]))
         want = nx.decode(ctx, nx.decrypt(ctx, sk, a[i])) * nx.decode(ctx, nx.decrypt(ctx, sk, b[i]))
         assert np.max(np.abs(dec - want)) < 1e-5
+
+
+@pytest.mark.parametrize("special", [1, 3])
+def test_generated_keys_regenerate_uniform_half(special):
+    """keys generated here are used with their uniform half regenerated from
+    its seed inside the inner products; the same words uploaded as a plain key
+    (read in full) give identical relinearisation, rotation and rotation-sum
+    residues"""
+    if nx.device_count() < 1:
+        pytest.skip("no GPU visible")
+    logn = 14
+    chain = [51, 46, 46, 46, 46, 46, 51] + ([51, 51] if special == 3 else [])
+    _note("seeded keys vs uploaded keys logn=%d special=%d" % (logn, special))
+    ctx = nx.Context(logn, chain, special_primes=special)
+    sk = nx.SecretKey(ctx, seed=61)
+    rk = nx.KSwitchKeys(ctx).generate_relin(sk, seed=62)
+    steps = [1, -3, 17]
+    elts = [ctx.elt_from_step(s) for s in steps]
+    gk = nx.KSwitchKeys(ctx).generate_galois(sk, elts, seed=63)
+    rk2 = nx.KSwitchKeys(ctx)
+    rk2.upload(0, rk.key_array(0))
+    gk2 = nx.KSwitchKeys(ctx)
+    for e in elts:
+        gk2.upload(e, gk.key_array(e))
+    rng = np.random.default_rng(64)
+    L = ctx.L
+    d3 = _rand(rng, ctx.moduli, 3, L, ctx.n)
+    ct3 = nx.Ciphertext.from_array(ctx, d3, 1, 2.0 ** 40)
+    assert np.array_equal(nx.relinearize(ct3, rk).to_array(), nx.relinearize(ct3, rk2).to_array())
+    cts = [nx.Ciphertext.from_array(ctx, _rand(rng, ctx.moduli, 2, L, ctx.n), 1, 2.0 ** 40) for _ in steps]
+    for s in steps:
+        assert np.array_equal(nx.rotate_vector(cts[0], s, gk).to_array(), nx.rotate_vector(cts[0], s, gk2).to_array())
+    for a, b in zip(nx.rotate_hoisted(cts[0], steps, gk), nx.rotate_hoisted(cts[0], steps, gk2)):
+        assert np.array_equal(a.to_array(), b.to_array())
+    assert np.array_equal(nx.rotate_sum(cts, steps, gk).to_array(), nx.rotate_sum(cts, steps, gk2).to_array())
