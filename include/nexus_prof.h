@@ -23,6 +23,10 @@ extern "C" {
 int nx_profile_select(const char *classes);
 /* Launches, summed event-measured milliseconds and summed algorithmic bytes of one class. */
 int nx_profile_read(const char *kernel_class, size_t *launches, double *total_ms, double *bytes);
+/* Summed algorithmic double-precision operations (an FMA counts two) of one
+ * class; recorded by the compute-bound classes (the conversions fused into
+ * the NTT column pass), 0 for the others. */
+int nx_profile_read_flops(const char *kernel_class, double *flops);
 /* Comma-separated list of the classes with statistics. */
 int nx_profile_classes(char *buf, size_t len);
 /* Operation tally: start = 1 clears and starts counting the engine's

@@ -1218,6 +1218,7 @@ class Server:
 _sig("nx_profile_select", [ctypes.c_char_p])
 _sig("nx_profile_read", [ctypes.c_char_p, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_double),
                          ctypes.POINTER(ctypes.c_double)])
+_sig("nx_profile_read_flops", [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)])
 _sig("nx_profile_classes", [ctypes.c_char_p, sz])
 
 
@@ -1230,6 +1231,13 @@ def profile_read(kernel_class):
     n, ms, b = sz(), ctypes.c_double(), ctypes.c_double()
     _check(_lib.nx_profile_read(kernel_class.encode(), ctypes.byref(n), ctypes.byref(ms), ctypes.byref(b)))
     return n.value, ms.value, b.value
+
+
+def profile_read_flops(kernel_class):
+    """summed algorithmic double-precision operations of a class (0 unless it records them)"""
+    f = ctypes.c_double()
+    _check(_lib.nx_profile_read_flops(kernel_class.encode(), ctypes.byref(f)))
+    return f.value
 
 
 def profile_classes():

@@ -25,7 +25,8 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-I" + CSRC, "-I" + INCLUDE, "-Wall", "-
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 # The encoder's double-precision FFT and the double-precision modular
 # arithmetic of the NTT and the base conversions must round exactly as written.
-NO_CONTRACT = {"encoder.cpp", "kernels_encode.hip", "kernels_ntt.hip", "kernels_fused.hip", "kernels_mix.hip"}
+NO_CONTRACT = {"encoder.cpp", "kernels_encode.hip", "kernels_ntt.hip", "kernels_fused.hip", "kernels_mix.hip",
+               "kernels_hoist.hip", "kernels_poly.hip"}
 
 
 def sources():

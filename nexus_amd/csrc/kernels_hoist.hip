@@ -4,6 +4,7 @@
 // applied to the ModUp digits inside the inner product: digit values are
 // gathered through the NTT-domain permutation of each element, so the
 // permuted digits are never written out.
+#include "fpmod.h"
 #include "kernels.h"
 #include "prof.h"
 
@@ -24,8 +25,10 @@ struct RotArgs {
     const uint64_t *aseed[kMaxHoist];  // non-null: the key's uniform half regenerated (ctr_uniform)
 };
 
-// acc[r][k][I][c] = sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]  (one rotation r per grid row group)
-__global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod, OutPrimes outp,
+// acc[r][k][I][c] = sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]  (one rotation r per grid row group).
+// qd non-null (every prime below 2^51): exact double-precision products
+// (fp_mulmod), full-rate FMAs instead of 64-bit integer products; same words.
+__global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod, const double2 *qd, OutPrimes outp,
                                uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
@@ -40,6 +43,26 @@ __global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod,
     const uint64_t kstride = (uint64_t)key_limbs * n;
     const uint64_t *aseed = args.aseed[r];
     const uint64_t ctr = (uint64_t)kp * n + c;
+    uint64_t *out = acc + (uint64_t)r * 2 * nout * n;
+    if (qd) {
+        const double q = qd[kp].x, qinv = qd[kp].y;
+        double s0 = 0.0, s1 = 0.0;
+        for (uint32_t j = 0; j < l; j++) {
+            const double tv = (double)t[(uint64_t)j * n];
+            const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
+            const double a = (double)kj[0];
+            const double b = (double)(aseed ? ctr_uniform(aseed[j], ctr, m.q) : kj[kstride]);
+            s0 += fp_mulmod(tv, a, q, qinv);
+            s1 += fp_mulmod(tv, b, q, qinv);
+            if (j & 1) {  // |s| stays below 4 q < 2^53
+                s0 = fp_reduce(s0, q, qinv);
+                s1 = fp_reduce(s1, q, qinv);
+            }
+        }
+        out[(uint64_t)I * n + c] = fp_final(s0, q, qinv);
+        out[((uint64_t)nout + I) * n + c] = fp_final(s1, q, qinv);
+        return;
+    }
     uint64_t lo0 = 0, hi0 = 0, lo1 = 0, hi1 = 0;
     for (uint32_t j = 0; j < l; j++) {
         const uint64_t tv = t[(uint64_t)j * n];
@@ -53,7 +76,6 @@ __global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod,
         lo1 += plo;
         hi1 += phi + (lo1 < plo);
     }
-    uint64_t *out = acc + (uint64_t)r * 2 * nout * n;
     out[(uint64_t)I * n + c] = barrett128(hi0, lo0, m);
     out[((uint64_t)nout + I) * n + c] = barrett128(hi1, lo1, m);
 }
@@ -61,8 +83,8 @@ __global__ void k_ks_inner_rot(uint64_t *acc, RotArgs args, const ModConst *mod,
 // acc[k][I][c] (+)= sum_r sum_j T[I][j][perm_r[c]] * key_r[j][k][kp(I)][c]:
 // every product is below 2^120, so the 128-bit sums are folded every 128 terms.
 __global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs args, uint32_t count,
-                                   const ModConst *mod, OutPrimes outp, uint32_t l, uint32_t key_limbs,
-                                   uint32_t nout, uint32_t n, int accumulate, uint64_t t_stride)
+                                   const ModConst *mod, const double2 *qd, OutPrimes outp, uint32_t l,
+                                   uint32_t key_limbs, uint32_t nout, uint32_t n, int accumulate, uint64_t t_stride)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -70,6 +92,33 @@ __global__ void k_ks_inner_rot_sum(uint64_t *acc, const uint64_t *T, RotArgs arg
     const uint32_t kp = outp.p[I];
     const ModConst m = mod[kp];
     const uint64_t kstride = (uint64_t)key_limbs * n;
+    if (qd) {  // every prime below 2^51: exact double-precision products
+        const double q = qd[kp].x, qinv = qd[kp].y;
+        double s0 = 0.0, s1 = 0.0;
+        if (accumulate) {
+            s0 = (double)acc[(uint64_t)I * n + c];
+            s1 = (double)acc[((uint64_t)nout + I) * n + c];
+        }
+        const uint64_t ctr = (uint64_t)kp * n + c;
+        for (uint32_t r = 0; r < count; r++) {
+            const uint32_t src = args.perm[r][c];
+            const uint64_t *t = T + r * t_stride + (uint64_t)I * l * n + src;
+            const uint64_t *k0 = args.key[r] + (uint64_t)kp * n + c;
+            const uint64_t *aseed = args.aseed[r];
+            for (uint32_t j = 0; j < l; j++) {
+                const double tv = (double)t[(uint64_t)j * n];
+                const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
+                const double a = (double)kj[0];
+                const double b = (double)(aseed ? ctr_uniform(aseed[j], ctr, m.q) : kj[kstride]);
+                // two products a term: |s| <= q/2 (1 + eps) + 2.5 q after each pair, below 2^53
+                s0 = fp_reduce(s0 + fp_mulmod(tv, a, q, qinv), q, qinv);
+                s1 = fp_reduce(s1 + fp_mulmod(tv, b, q, qinv), q, qinv);
+            }
+        }
+        acc[(uint64_t)I * n + c] = fp_final(s0, q, qinv);
+        acc[((uint64_t)nout + I) * n + c] = fp_final(s1, q, qinv);
+        return;
+    }
     uint64_t r0 = 0, r1 = 0;  // folded partial sums (reduced)
     if (accumulate) {
         r0 = acc[(uint64_t)I * n + c];
@@ -151,6 +200,9 @@ RotArgs make_args(const uint64_t *const *keys, const uint32_t *const *perms, uin
     return a;
 }
 
+// {q, 1/q} when every prime is below 2^51 (the double-precision products), else null
+const double2 *fp_primes(const DeviceTables &t) { return t.max_bits <= 51 ? t.qd : nullptr; }
+
 // key words read per (output prime, digit) of one rotation: 2, or 1 with the
 // uniform half regenerated
 double key_words_read(const uint64_t *const *aseeds, uint32_t count)
@@ -172,8 +224,8 @@ void ks_inner_product_rot(const DeviceTables &t, uint64_t *acc, const uint64_t *
     ProfScope prof("ks_inner_rot", s,
                    (double)n * count * nout * ((8.0 + 8.0 * key_words_read(aseeds, count)) * l + 16.0 + 4.0));
     hipLaunchKernelGGL(k_ks_inner_rot, dim3((n + kBlock - 1) / kBlock, count * nout), dim3(kBlock), 0, s, acc,
-                       make_args(keys, perms, count, aseeds, T, Ts), t.mod, make_out(out_primes_host, nout), l,
-                       key_limbs, nout, n);
+                       make_args(keys, perms, count, aseeds, T, Ts), t.mod, fp_primes(t),
+                       make_out(out_primes_host, nout), l, key_limbs, nout, n);
 }
 
 void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64_t *T, const uint64_t *const *keys,
@@ -185,8 +237,8 @@ void ks_inner_product_rot_sum(const DeviceTables &t, uint64_t *acc, const uint64
     ProfScope prof("ks_inner_rot_sum", s,
                    (double)n * nout * (count * ((8.0 + 8.0 * key_words_read(aseeds, count)) * l + 4.0) + 16.0));
     hipLaunchKernelGGL(k_ks_inner_rot_sum, dim3((n + kBlock - 1) / kBlock, nout), dim3(kBlock), 0, s, acc, T,
-                       make_args(keys, perms, count, aseeds), count, t.mod, make_out(out_primes_host, nout), l,
-                       key_limbs, nout, n, accumulate ? 1 : 0, t_stride);
+                       make_args(keys, perms, count, aseeds), count, t.mod, fp_primes(t),
+                       make_out(out_primes_host, nout), l, key_limbs, nout, n, accumulate ? 1 : 0, t_stride);
 }
 
 void rot_base(const DeviceTables &t, uint64_t *out, const uint64_t *const *in, const uint32_t *const *perms,

@@ -1018,6 +1018,16 @@ void ntt_forward(const DeviceTables &t, const NttBatch &b, hipStream_t s)
     }
 }
 
+// Algorithmic double-precision operations (an FMA counts two, rint one) of the
+// fused column kernels: an exact product (mul, fma, mul, rint, fma, add) is 8,
+// a reduction (mul, rint, fma) 4; a conversion to one output word over A
+// inputs is A products, A additions, a reduction every second term and a
+// final one; a column pass is half a butterfly (a product, an addition and a
+// subtraction) per word and stage plus the store's reduction.
+constexpr double kFlopsProduct = 8.0, kFlopsReduce = 4.0;
+inline double conv_flops(uint32_t A) { return A * (kFlopsProduct + 1.0) + (A / 2) * kFlopsReduce + kFlopsReduce; }
+inline double col_flops(int logs) { return logs * (kFlopsProduct + 2.0) / 2.0 + kFlopsReduce; }
+
 // Fused ModUp column pass per power-of-two bucket of the batch size B:
 // 0 = on (outputs per workgroup by the default rule), -1 = off (conversion and
 // column pass as separate launches), k > 0 = on with k outputs per workgroup.
@@ -1074,7 +1084,8 @@ void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *co
     uint32_t outs = 0;
     for (uint32_t j = 0; j < a.digits; j++) outs += nout - std::min(alpha, l - j * alpha);
     ProfScope prof("ks_modup_col", s,
-                   8.0 * n * ((double)B * l + (double)B * outs) + 8.0 * (double)(1u << l1) * B * outs);
+                   8.0 * n * ((double)B * l + (double)B * outs) + 8.0 * (double)(1u << l1) * B * outs,
+                   (double)n * B * ((double)l * kFlopsProduct + outs * (conv_flops(alpha) + col_flops(l1))));
     const dim3 grid(tiles, B * a.digits * a.chunks);
 #define NX_MC(LOGS, A)                                                                          \
     if (l1 == LOGS && alpha == A) {                                                           \
@@ -1115,7 +1126,8 @@ void rns_center_last_col(const DeviceTables &t, uint64_t *z, const uint64_t *las
     chunk = std::max(chunk, std::min(4u, limbs));
     a.chunk = chunk;
     a.chunks = (limbs + chunk - 1) / chunk;
-    ProfScope prof("rescale_col", s, 8.0 * n * polys * (1.0 + limbs) + 8.0 * (double)(1u << l1) * polys * limbs);
+    ProfScope prof("rescale_col", s, 8.0 * n * polys * (1.0 + limbs) + 8.0 * (double)(1u << l1) * polys * limbs,
+                   (double)n * polys * limbs * (kFlopsReduce + col_flops(l1)));
     const dim3 grid(tiles, polys * a.chunks);
     if (l1 == 7)
         hipLaunchKernelGGL((k_center_col<7>), grid, dim3(kThreads), 0, s, a);
@@ -1168,7 +1180,8 @@ void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *
     a.chunk = chunk;
     a.chunks = (lo + chunk - 1) / chunk;
     ProfScope prof("ks_moddown_col", s,
-                   8.0 * n * polys * ((double)A + lo) + 8.0 * (double)(1u << l1) * polys * lo);
+                   8.0 * n * polys * ((double)A + lo) + 8.0 * (double)(1u << l1) * polys * lo,
+                   (double)n * polys * ((double)A * 2.0 + lo * (conv_flops(A + 1) + col_flops(l1))));
     const dim3 grid(tiles, polys * a.chunks);
 #define NX_MD(LOGS, AA)                                                                         \
     if (l1 == LOGS && A == AA) {                                                              \

@@ -1,6 +1,7 @@
 // Dyadic (coefficient-wise) RNS kernels: HBM-bound integer work.  Grid y walks
 // (poly, limb) pairs so the modulus is uniform per workgroup; grid x covers the
 // n coefficients of a limb, one coefficient per lane.
+#include "fpmod.h"
 #include "kernels.h"
 #include "prof.h"
 
@@ -257,8 +258,10 @@ __global__ void k_modup_expand(uint64_t *T, const uint64_t *coef, const ModConst
 
 // aseed non-null: poly 1 of key digit j (uniform) is ctr_uniform(aseed[j], kp n + c, q_kp), regenerated
 // instead of read
+// qd non-null (every prime below 2^51): exact double-precision products (fp_mulmod); same words
 __global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key, const uint64_t *aseed,
-                           const ModConst *mod, PrimeMap outp, uint32_t l, uint32_t key_limbs, uint32_t nout, uint32_t n)
+                           const ModConst *mod, const double2 *qd, PrimeMap outp, uint32_t l, uint32_t key_limbs,
+                           uint32_t nout, uint32_t n)
 {
     const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
     if (c >= n) return;
@@ -272,6 +275,25 @@ __global__ void k_ks_inner(uint64_t *acc, const uint64_t *T, const uint64_t *key
     const uint64_t *k0 = key + (uint64_t)kp * n + c;
     const uint64_t kstride = (uint64_t)key_limbs * n;  // between poly 0 and 1 of a digit
     const uint64_t ctr = (uint64_t)kp * n + c;
+    if (qd) {
+        const double q = qd[kp].x, qinv = qd[kp].y;
+        double s0 = 0.0, s1 = 0.0;
+        for (uint32_t j = 0; j < l; j++) {
+            const double tv = (double)t[(uint64_t)j * n];
+            const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
+            const double a = (double)kj[0];
+            const double b = (double)(aseed ? ctr_uniform(aseed[j], ctr, m.q) : kj[kstride]);
+            s0 += fp_mulmod(tv, a, q, qinv);
+            s1 += fp_mulmod(tv, b, q, qinv);
+            if (j & 1) {  // |s| stays below 4 q < 2^53
+                s0 = fp_reduce(s0, q, qinv);
+                s1 = fp_reduce(s1, q, qinv);
+            }
+        }
+        acc[(uint64_t)I * n + c] = fp_final(s0, q, qinv);
+        acc[((uint64_t)nout + I) * n + c] = fp_final(s1, q, qinv);
+        return;
+    }
     for (uint32_t j = 0; j < l; j++) {
         const uint64_t tv = t[(uint64_t)j * n];
         const uint64_t *kj = k0 + (uint64_t)j * 2 * kstride;
@@ -505,7 +527,8 @@ void ks_inner_product(const DeviceTables &t, uint64_t *acc, const uint64_t *T, c
     // uniform half) per (output prime, digit), two outputs per prime
     ProfScope prof("ks_inner", s, (double)(1u << t.logn) * B * nout * ((aseed ? 16.0 : 24.0) * l + 16.0));
     hipLaunchKernelGGL(k_ks_inner, grid_for(t.logn, B * nout), dim3(kBlock), 0, s, acc, T, key, aseed, t.mod,
-                       make_map(out_primes_host, nout), l, key_limbs, nout, 1u << t.logn);
+                       t.max_bits <= 51 ? t.qd : nullptr, make_map(out_primes_host, nout), l, key_limbs, nout,
+                       1u << t.logn);
 }
 
 void sample_uniform(const DeviceTables &t, uint64_t *out, uint32_t polys, uint32_t limbs, const uint8_t *primes_host,

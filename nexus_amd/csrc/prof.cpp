@@ -18,13 +18,13 @@ namespace {
 
 struct Stat {
     size_t launches = 0;
-    double ms = 0, bytes = 0;
+    double ms = 0, bytes = 0, flops = 0;
 };
 
 struct Pending {
     std::string cls;
     hipEvent_t a, b;
-    double bytes;
+    double bytes, flops;
 };
 
 std::mutex g_mu;
@@ -56,6 +56,7 @@ void drain()
         Stat &s = g_stats[p.cls];
         s.ms += ms;
         s.bytes += p.bytes;
+        s.flops += p.flops;
         s.launches++;
         g_pool.push_back(p.a);
         g_pool.push_back(p.b);
@@ -79,7 +80,7 @@ bool prof_active(const char *cls)
     return selected(cls);
 }
 
-void prof_record(const char *cls, hipStream_t s, bool begin, double bytes)
+void prof_record(const char *cls, hipStream_t s, bool begin, double bytes, double flops)
 {
     std::lock_guard<std::mutex> g(g_mu);
     if (!selected(cls)) return;
@@ -92,7 +93,7 @@ void prof_record(const char *cls, hipStream_t s, bool begin, double bytes)
         if (it == g_open.end()) return;
         hipEvent_t b = take();
         NX_HIP(hipEventRecord(b, s));
-        g_pending.push_back({cls, it->second, b, bytes});
+        g_pending.push_back({cls, it->second, b, bytes, flops});
         g_open.erase(it);
         if (g_pending.size() > 8192) drain();
     }
@@ -154,6 +155,20 @@ int nx_profile_read(const char *kernel_class, size_t *launches, double *total_ms
         if (launches) *launches = st.launches;
         if (total_ms) *total_ms = st.ms;
         if (bytes) *bytes = st.bytes;
+        return NX_OK;
+    } catch (const nx::Error &e) {
+        nx::set_last_error(e.what());
+        return e.code;
+    }
+}
+
+int nx_profile_read_flops(const char *kernel_class, double *flops)
+{
+    try {
+        std::lock_guard<std::mutex> g(nx::g_mu);
+        nx::drain();
+        auto it = nx::g_stats.find(kernel_class ? kernel_class : "");
+        if (flops) *flops = it == nx::g_stats.end() ? 0.0 : it->second.flops;
         return NX_OK;
     } catch (const nx::Error &e) {
         nx::set_last_error(e.what());

@@ -11,20 +11,23 @@ namespace nx {
 bool prof_active(const char *kernel_class);
 // Operation tally (nx_op_tally): `count` operations of `kind` on `limbs` limbs
 void tally(const char *kind, size_t limbs, double count = 1);
-void prof_record(const char *kernel_class, hipStream_t s, bool begin, double units);
+// units: algorithmic HBM bytes of the launch; flops: its algorithmic FP64
+// operations (an FMA counts two), for the compute-bound classes
+void prof_record(const char *kernel_class, hipStream_t s, bool begin, double units, double flops = 0);
 
 struct ProfScope {
     const char *cls;
     hipStream_t s;
-    double units;
+    double units, flops;
     bool on;
-    ProfScope(const char *c, hipStream_t st, double u) : cls(c), s(st), units(u), on(prof_active(c))
+    ProfScope(const char *c, hipStream_t st, double u, double f = 0)
+        : cls(c), s(st), units(u), flops(f), on(prof_active(c))
     {
-        if (on) prof_record(cls, s, true, units);
+        if (on) prof_record(cls, s, true, units, flops);
     }
     ~ProfScope()
     {
-        if (on) prof_record(cls, s, false, units);
+        if (on) prof_record(cls, s, false, units, flops);
     }
 };
 
