@@ -39,15 +39,17 @@ BERT_SCALE = 2.0 ** 46
 BERT_TOKENS = 128
 SCALE = 2.0 ** 40
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X published double-precision peak
 # HBM bytes moved per algorithmic byte, from FETCH_SIZE / WRITE_SIZE counter runs
 # of `bench.py --only ntt` / `--only dot` calibrated against a device copy of
 # known size (profiles/ntt_fwd_n65536_38limbs_traffic.md,
 # profiles/dot_plain_n65536_64terms_traffic.md)
 TRAFFIC_PER_ALGORITHMIC_BYTE = {"ntt_fwd_col": 40.38 / 39.93, "ntt_fwd_row": 50.57 / 60.73,
-                                "dot_plain": 1930.5 / 1932.5}
+                                "dot_plain": 1930.5 / 1932.5, "ks_modup_col": 1528.2 / 1457.6}
 TRAFFIC_PROFILE = {"ntt_fwd_col": "profiles/ntt_fwd_n65536_38limbs_traffic.md",
                    "ntt_fwd_row": "profiles/ntt_fwd_n65536_38limbs_traffic.md",
-                   "dot_plain": "profiles/dot_plain_n65536_64terms_traffic.md"}
+                   "dot_plain": "profiles/dot_plain_n65536_64terms_traffic.md",
+                   "ks_modup_col": "profiles/ks_kernels_n65536_l20_hybrid8.md"}
 
 
 def log(msg):
@@ -474,6 +476,31 @@ def fused_mode(mode):
         mode & 255 or "default", 4 if mode & 256 else 8)
 
 
+def bench_bert_sets(nx, reps, count=8):
+    """`count` full 128-token sets: one layer call per set against every set
+    in lockstep (forward_sets: the bootstrapping steps of all sets together)"""
+    B = bert_setup(nx)
+    rng = np.random.default_rng(6)
+    sets = [bert_encrypt(nx, B, rng.normal(0, 1, (BERT_TOKENS, 768))) for _ in range(count)]
+    segs = [[(0, BERT_TOKENS)]] * count
+    layer, ctx = B["layer"], B["ctx"]
+    layer.forward(sets[0], BERT_TOKENS)
+    layer.forward_sets(sets[:2], segs[:2])
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for x in sets:
+            layer.forward(x, BERT_TOKENS)
+    ctx.synchronize()
+    t_calls = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        layer.forward_sets(sets, segs)
+    ctx.synchronize()
+    t_lock = (time.perf_counter() - t0) / reps
+    return {"sets": count, "per_set_calls_s": t_calls, "lockstep_s": t_lock, "lockstep_speedup": t_calls / t_lock}
+
+
 def pack_sets(lens, capacity):
     """first-fit decreasing: [[(sequence, start), ...], ...]"""
     sets, fill = [], []
@@ -510,6 +537,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-extra", action="store_true", help="skip the nonlinear, bootstrapping and MatMul extras")
     ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--lockstep-steps", type=int, default=2,
+                    help="steps of the cross-set lockstep mode (every set of a step in one forward_sets call)")
     ap.add_argument("--requests", type=int, default=100, help="served BERT requests (p50/p99 samples)")
     ap.add_argument("--served-plain-requests", type=int, default=0,
                     help="also serve the first k requests through the plain per-call drop-in")
@@ -518,8 +547,8 @@ def main():
     ap.add_argument("--bert-lanes", type=int, default=1,
                     help="lanes of the served BERT measurement, each with one BERT call in flight (each lane "
                          "caches its own call working set)")
-    ap.add_argument("--only", choices=["bert", "bert_served", "dot", "bootstrap_dense", "bootstrap_sparse", "matmul",
-                                       "ntt", "ks", "valu"],
+    ap.add_argument("--only", choices=["bert", "bert_served", "bert_sets", "dot", "bootstrap_dense",
+                                       "bootstrap_sparse", "matmul", "ntt", "ks", "valu"],
                     help="run one of the extra workloads alone (for profiling) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -535,6 +564,8 @@ def main():
     if args.only:
         if args.only == "bert":
             r = bench_bert_call(nx, args.steps)
+        elif args.only == "bert_sets":
+            r = bench_bert_sets(nx, args.steps)
         elif args.only == "bert_served":
             B = bert_setup(nx)
             rows = np.random.default_rng(5).normal(0, 1, (BERT_TOKENS, 768))
@@ -640,9 +671,33 @@ def main():
     ctx.synchronize()
     elapsed = time.perf_counter() - tb
     launches, k_ms, k_bytes = nx.profile_read(dominant)
+    k_flops = nx.profile_read_flops(dominant)
     _, _, ks_bytes = nx.profile_read("ks_inner")
     nx.profile_select("")
     ms_per_step = elapsed / args.steps * 1e3
+    # cross-set lockstep: every set of the step in one call (each bootstrapping
+    # step over all sets, odd ciphertexts paired across sets); latency = the step
+    lock = None
+    if args.lockstep_steps > 0:
+        layer.forward_sets(enc_sets[:2], set_segs[:2])  # warm the lockstep working set
+        ctx.synchronize()
+        tl = time.perf_counter()
+        for _ in range(args.lockstep_steps):
+            lock_out = layer.forward_sets(enc_sets, set_segs)
+            ctx.synchronize()
+        lock_s = (time.perf_counter() - tl) / args.lockstep_steps
+        got_lock = {}
+        for j, members in enumerate(sets):
+            dec = bert_decrypt(nx, B, lock_out[j])
+            for k, a in members:
+                got_lock[k] = dec[a:a + lens[k]]
+        del lock_out
+        lock = {"value": len(lens) / lock_s, "seconds_per_step": lock_s, "latency_ms": lock_s * 1e3,
+                "speedup_over_per_set_calls": (ms_per_step / 1e3) / lock_s, "steps": args.lockstep_steps,
+                "definition": "BertLayer::forward_sets over the step's packed sets: every stage over all sets, "
+                              "each bootstrapping step refreshing every set's ciphertexts together (odd ones "
+                              "paired across sets); every sequence's result arrives at the end of the step"}
+        log("lockstep: %.2f s per step (%.3f sequence-layers/s)" % (lock_s, len(lens) / lock_s))
     # per-sequence latency of the batched path = its set's call
     seq_lat = []
     for j, members in enumerate(sets):
@@ -664,10 +719,16 @@ def main():
     err_b = max(float(np.max(np.abs(got[k] - model[k]))) for k in model)
     err_p = max(float(np.max(np.abs(got_plain[k] - model[k]))) for k in model)
     err_x = max(float(np.max(np.abs(got[k] - exact[k]))) for k in model)
+    if lock is not None:
+        lock["max_err_vs_fp64_model"] = max(float(np.max(np.abs(got_lock[k] - model[k]))) for k in model)
     model_x = max(float(np.max(np.abs(model[k] - exact[k]))) for k in model)
 
     avg_launch_ms = k_ms / max(launches, 1)
     achieved = (k_bytes / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e9 if launches else 0.0
+    # a class that records FP64 operations (the conversions fused into the NTT
+    # column pass) is bound by the double-precision VALU rate, not by HBM
+    achieved_tf = (k_flops / max(launches, 1)) / (avg_launch_ms * 1e-3) / 1e12 if launches else 0.0
+    valu = k_flops > 0
     res = {
         "metric": METRIC,
         "value": len(lens) * 1000.0 / ms_per_step,
@@ -690,13 +751,15 @@ def main():
             "p99_ms": float(np.percentile(np.array(plain_lat) * 1e3, 99)),
         },
         "batched_over_plain": plain_s / (ms_per_step / 1e3),
+        "lockstep": lock,
         "plain_definition": "the same layer program in per-op mode: every operation one single-ciphertext "
                             "call (rotate_vector, multiply_plain, add, multiply, relinearize, rescale_to_next per "
                             "term; one bootstrap and one GELU per ciphertext), one layer call per sequence",
         "max_err_vs_fp64_model": err_b,
         "max_err_vs_fp64_model_plain": err_p,
         "err_bound": 2.0 ** -10,
-        "err_ok": bool(err_b <= 2.0 ** -10 and err_p <= 2.0 ** -10),
+        "err_ok": bool(err_b <= 2.0 ** -10 and err_p <= 2.0 ** -10 and
+                       (lock is None or lock["max_err_vs_fp64_model"] <= 2.0 ** -10)),
         "err_reference": "FP64 model of the reference's own approximations (oracle/bert.py layer_approx); the "
                          "approximations themselves differ from the exact FP64 layer by model_vs_exact",
         "max_err_vs_exact_fp64_layer": err_x,
@@ -726,11 +789,15 @@ def main():
         },
         "roofline": {
             "kernel": dominant,
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
+            "bound": "valu-fp64" if valu else "hbm",
+            "achieved": achieved_tf if valu else achieved,
+            "peak": FP64_PEAK_TFLOPS if valu else HBM_PEAK_GBS,
+            "unit": "TFLOP/s" if valu else "GB/s",
+            "frac": achieved_tf / FP64_PEAK_TFLOPS if valu else achieved / HBM_PEAK_GBS,
+            "achieved_hbm_gbs": achieved,
+            "hbm_frac": achieved / HBM_PEAK_GBS,
+            "algorithmic_flops_per_launch": k_flops / max(launches, 1),
+            "algorithmic_bytes_per_launch": k_bytes / max(launches, 1),
             "traffic": (TRAFFIC_PER_ALGORITHMIC_BYTE[dominant] * k_bytes / max(launches, 1)
                         if dominant in TRAFFIC_PER_ALGORITHMIC_BYTE else None),
             "traffic_source": "calibrated FETCH_SIZE+WRITE_SIZE per algorithmic byte of this kernel class "

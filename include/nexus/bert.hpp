@@ -85,6 +85,13 @@ public:
     // within each segment, so every sequence's rows are what it would get
     // alone.  Rows outside the segments must be zero.
     std::vector<Ciphertext> forward(const std::vector<Ciphertext> &x, const std::vector<Segment> &segments);
+    // Several sets in lockstep: every stage runs over all sets before the next
+    // one, and each bootstrapping step refreshes the ciphertexts of every set
+    // together, so real messages pair up across sets (two per bootstrap) where
+    // one set alone leaves an odd one.  Each set's output equals forward() of
+    // that set up to the bootstrap pairing (the pairs differ, so the words do).
+    std::vector<std::vector<Ciphertext>> forward_sets(const std::vector<std::vector<Ciphertext>> &xs,
+                                                      const std::vector<std::vector<Segment>> &segments);
 
     // Continuous batching: a sequence encrypted at rows [0, len) moved to rows
     // [offset, offset + len) (token rotation, no level), and rows [offset,
@@ -108,6 +115,16 @@ public:
     std::vector<Ciphertext> layer_norm(const std::vector<Ciphertext> &r);
 
 private:
+    // attention up to the probabilities (before their bootstrap), then from
+    // the refreshed probabilities to the output
+    struct AttnState {
+        std::vector<int> seg, tlo, thi, active;
+        int halves = 0;
+        std::vector<std::vector<Ciphertext>> Pe;  // [ciphertext][half]
+    };
+    AttnState attention_probs(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
+                              const std::vector<Segment> &segments);
+    std::vector<Ciphertext> attention_values(AttnState &st, const std::vector<Ciphertext> &v);
     // plaintext diagonals of one matrix at one (factor, level, scale): every
     // diagonal is 256-periodic, so the diagonals of output ciphertext o over
     // the inputs of group gi ([gi group, (gi + 1) group)) are one periodic set,

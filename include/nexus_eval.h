@@ -189,6 +189,13 @@ int nx_bert_layer_forward(nx_bert_layer *layer, const nx_ciphertext *const *x, s
  * each sequence's rows equal what it would get alone; other rows must be zero */
 int nx_bert_layer_forward_packed(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t n, const int *starts,
                                  const int *lens, size_t segments, nx_ciphertext *const *out);
+/* `sets` packed sets in lockstep (x and out: sets * n handles, set s at
+ * [s n, s n + n)); set s has seg_counts[s] segments, their starts and lengths
+ * listed set after set.  Every stage runs over all sets, and each
+ * bootstrapping step refreshes every set's ciphertexts together (real
+ * messages paired across sets).  No counterpart in the reference. */
+int nx_bert_layer_forward_sets(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t sets, size_t n,
+                               const int *seg_counts, const int *starts, const int *lens, nx_ciphertext *const *out);
 /* stages: factor * x W (one limb out); attention of bootstrapped q (already
  * scaled by 1/8), k, v; x * invert_sqrt(mean(x^2)) per token */
 int nx_bert_layer_linear(nx_bert_layer *layer, int matrix, const nx_ciphertext *const *x, size_t n_in, double factor,

@@ -1002,6 +1002,8 @@ _sig("nx_bert_layer_rotation_steps", [sz, ctypes.c_int, ctypes.c_int, ctypes.c_i
 _sig("nx_bert_layer_forward", [vp, ctypes.POINTER(vp), sz, ctypes.c_int, ctypes.POINTER(vp)])
 _sig("nx_bert_layer_forward_packed", [vp, ctypes.POINTER(vp), sz, ctypes.POINTER(ctypes.c_int),
                                       ctypes.POINTER(ctypes.c_int), sz, ctypes.POINTER(vp)])
+_sig("nx_bert_layer_forward_sets", [vp, ctypes.POINTER(vp), sz, sz, ctypes.POINTER(ctypes.c_int),
+                                    ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(vp)])
 _sig("nx_bert_layer_linear", [vp, ctypes.c_int, ctypes.POINTER(vp), sz, ctypes.c_double, ctypes.POINTER(vp), sz])
 _sig("nx_bert_layer_attention", [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), sz, ctypes.c_int,
                                  ctypes.POINTER(vp)])
@@ -1080,6 +1082,19 @@ class BertLayer:
             _check(_lib.nx_bert_layer_forward_packed(self.h, _handles(x), len(x), st, ln, len(segments),
                                                      _handles(out)))
         return out
+
+    def forward_sets(self, xs, segments):
+        """several packed sets in lockstep: xs[s] the set's ciphertexts,
+        segments[s] its [(start, len), ...]; returns the outputs per set"""
+        n = len(xs[0])
+        flat = [c for x in xs for c in x]
+        out = self._outs(len(flat))
+        cnt = (ctypes.c_int * len(xs))(*[len(g) for g in segments])
+        st = [int(a) for g in segments for a, _ in g]
+        ln = [int(b) for g in segments for _, b in g]
+        _check(_lib.nx_bert_layer_forward_sets(self.h, _handles(flat), len(xs), n, cnt, (ctypes.c_int * len(st))(*st),
+                                               (ctypes.c_int * len(ln))(*ln), _handles(out)))
+        return [out[s * n:(s + 1) * n] for s in range(len(xs))]
 
     def linear(self, matrix, x, factor=1.0):
         m = BERT_MATRICES.index(matrix)

@@ -466,7 +466,26 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
     const size_t n = (size_t)shape_.hidden / kGroup;
     if (q.size() != n || k.size() != n || v.size() != n)
         throw std::invalid_argument("BertLayer: attention needs hidden / 256 ciphertexts of q, k and v");
-    const std::vector<int> seg = segment_of(segments);
+    AttnState st = attention_probs(q, k, segments);
+    {
+        std::vector<Ciphertext *> all;
+        for (auto &row : st.Pe)
+            for (int eta : st.active) all.push_back(&row[eta]);
+        bootstrap_real(all, 1.0);
+    }
+    mark("attention.bootstrap");
+    return attention_values(st, v);
+}
+
+BertLayer::AttnState BertLayer::attention_probs(const std::vector<Ciphertext> &q, const std::vector<Ciphertext> &k,
+                                                const std::vector<Segment> &segments)
+{
+    const size_t n = (size_t)shape_.hidden / kGroup;
+    if (q.size() != n || k.size() != n)
+        throw std::invalid_argument("BertLayer: attention needs hidden / 256 ciphertexts of q and k");
+    AttnState st;
+    st.seg = segment_of(segments);
+    const std::vector<int> &seg = st.seg;
     const Evaluator &ev = ck_.evaluator;
     const RelinKeys &rk = *ck_.relin_keys;
     const GaloisKeys &gk = *ck_.galois_keys;
@@ -476,7 +495,10 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
     // t < lmax or t > T - lmax (cyclically); half eta computes t' in [lo, hi)
     int lmax = 0;
     for (const Segment &g : segments) lmax = std::max(lmax, g.len);
-    std::vector<int> tlo(halves), thi(halves);
+    st.halves = halves;
+    std::vector<int> &tlo = st.tlo, &thi = st.thi;
+    tlo.assign(halves, 0);
+    thi.assign(halves, 0);
     for (int eta = 0; eta < halves; eta++) {
         const int t0 = kHeadDim * eta, full = std::min(kHeadDim, T - t0);
         const int hi1 = std::max(0, std::min(full, lmax - t0));
@@ -485,11 +507,12 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
         tlo[eta] = a1 ? 0 : (a2 ? lo2 : 0);
         thi[eta] = a2 ? full : (a1 ? hi1 : 0);
     }
-    std::vector<int> active;
+    std::vector<int> &active = st.active;
     for (int eta = 0; eta < halves; eta++)
         if (thi[eta] > tlo[eta]) active.push_back(eta);
     // probabilities of every ciphertext and active half, refreshed together
-    std::vector<std::vector<Ciphertext>> Pe(n, std::vector<Ciphertext>(halves));
+    std::vector<std::vector<Ciphertext>> &Pe = st.Pe;
+    Pe.assign(n, std::vector<Ciphertext>(halves));
     for (size_t c = 0; c < n; c++) {
         Ciphertext qc = q[c], kc = k[c];
         if (kc.coeff_modulus_size() > qc.coeff_modulus_size()) ev.mod_switch_to_inplace(kc, qc.params_id());
@@ -596,13 +619,18 @@ std::vector<Ciphertext> BertLayer::attention(const std::vector<Ciphertext> &q, c
         }
         mark("attention.softmax");
     }
-    {
-        std::vector<Ciphertext *> all;
-        for (auto &row : Pe)
-            for (int eta : active) all.push_back(&row[eta]);
-        bootstrap_real(all, 1.0);
-    }
-    mark("attention.bootstrap");
+    return st;
+}
+
+std::vector<Ciphertext> BertLayer::attention_values(AttnState &st, const std::vector<Ciphertext> &v)
+{
+    const size_t n = (size_t)shape_.hidden / kGroup;
+    if (v.size() != n || st.Pe.size() != n)
+        throw std::invalid_argument("BertLayer: attention needs hidden / 256 ciphertexts of v");
+    const Evaluator &ev = ck_.evaluator;
+    const RelinKeys &rk = *ck_.relin_keys;
+    const std::vector<int> &tlo = st.tlo, &thi = st.thi, &active = st.active;
+    std::vector<std::vector<Ciphertext>> &Pe = st.Pe;
     // ---- expansion over the 64 features of each head; times the rotated v
     std::vector<Ciphertext> out(n);
     for (size_t c = 0; c < n; c++) {
@@ -795,6 +823,111 @@ std::vector<Ciphertext> BertLayer::forward(const std::vector<Ciphertext> &x, con
     boot_all(r2, 8 * pre);
     mark("bootstrap_ln2");
     std::vector<Ciphertext> y = layer_norm(r2);
+    mark("ln2");
+    return y;
+}
+
+std::vector<std::vector<Ciphertext>> BertLayer::forward_sets(const std::vector<std::vector<Ciphertext>> &xs,
+                                                             const std::vector<std::vector<Segment>> &segments)
+{
+    const size_t n = (size_t)shape_.hidden / kGroup, S = xs.size();
+    if (segments.size() != S) throw std::invalid_argument("BertLayer: one segment list per set");
+    if (!S) return {};
+    for (size_t s = 0; s < S; s++) {
+        if (xs[s].size() != n) throw std::invalid_argument("BertLayer: input needs hidden / 256 ciphertexts");
+        segment_of(segments[s]);  // validates
+    }
+    const Evaluator &ev = ck_.evaluator;
+    const double pre = opt_.ln_prescale;
+    // every set's ciphertexts of one stage bootstrapped together (pairs across sets)
+    auto boot_sets = [&](std::vector<std::vector<Ciphertext>> &v, double factor) {
+        std::vector<Ciphertext *> p;
+        for (auto &set : v)
+            for (auto &c : set) p.push_back(&c);
+        bootstrap_real(p, factor);
+    };
+    auto add_scaled = [&](std::vector<Ciphertext> &y, const std::vector<Ciphertext> &res) {
+        for (size_t c = 0; c < y.size(); c++) {
+            Ciphertext r8 = scaled(res[c], 1.0 / 8);
+            r8.set_scale(y[c].scale());  // both are the evaluator's scale, by construction
+            ev.add_inplace(y[c], r8);
+        }
+    };
+    stage_ms_.clear();
+    if (opt_.stage_timing) ck_.context->synchronize(), t_mark_ = std::chrono::steady_clock::now();
+    std::vector<std::vector<Ciphertext>> q(S), k(S), v(S);
+    for (size_t s = 0; s < S; s++) {
+        q[s] = linear(Wq, xs[s], 1.0 / 32);
+        k[s] = linear(Wk, xs[s], 0.25);
+        v[s] = linear(Wv, xs[s], 0.25);
+    }
+    mark("qkv");
+    {
+        std::vector<Ciphertext *> p;
+        for (size_t s = 0; s < S; s++)
+            for (auto *set : {&q[s], &k[s], &v[s]})
+                for (auto &c : *set) p.push_back(&c);
+        bootstrap_real(p, 4);
+    }
+    mark("bootstrap_qkv");
+    std::vector<AttnState> st(S);
+    for (size_t s = 0; s < S; s++) {
+        st[s] = attention_probs(q[s], k[s], segments[s]);
+        q[s].clear();
+        k[s].clear();
+    }
+    {
+        std::vector<Ciphertext *> all;
+        for (auto &a : st)
+            for (auto &row : a.Pe)
+                for (int eta : a.active) all.push_back(&row[eta]);
+        bootstrap_real(all, 1.0);
+    }
+    mark("attention.bootstrap");
+    std::vector<std::vector<Ciphertext>> r1(S);
+    for (size_t s = 0; s < S; s++) {
+        std::vector<Ciphertext> o = attention_values(st[s], v[s]);
+        st[s] = AttnState();
+        v[s].clear();
+        r1[s] = linear(Wo, o, 1.0 / 8);
+        add_scaled(r1[s], xs[s]);
+    }
+    mark("attention_wo_residual");
+    boot_sets(r1, 8 * pre);
+    mark("bootstrap_ln1");
+    std::vector<std::vector<Ciphertext>> z(S), h(S);
+    for (size_t s = 0; s < S; s++) {
+        z[s] = layer_norm(r1[s]);
+        r1[s].clear();
+        h[s] = linear(W1, z[s], 0.25);
+    }
+    mark("ln1_ffn1");
+    boot_sets(h, 4);
+    mark("bootstrap_ffn");
+    {
+        GELUEvaluator gelu(ck_);
+        std::vector<const Ciphertext *> hp;
+        for (auto &set : h)
+            for (auto &c : set) hp.push_back(&c);
+        std::vector<Ciphertext> g;
+        gelu.gelu_many(hp, g);
+        size_t i = 0;
+        for (auto &set : h)
+            for (auto &c : set) c = std::move(g[i++]);
+    }
+    mark("gelu");
+    std::vector<std::vector<Ciphertext>> r2(S);
+    for (size_t s = 0; s < S; s++) {
+        r2[s] = linear(W2, h[s], 1.0 / 8);
+        h[s].clear();
+        add_scaled(r2[s], z[s]);
+        z[s].clear();
+    }
+    mark("ffn2_residual");
+    boot_sets(r2, 8 * pre);
+    mark("bootstrap_ln2");
+    std::vector<std::vector<Ciphertext>> y(S);
+    for (size_t s = 0; s < S; s++) y[s] = layer_norm(r2[s]);
     mark("ln2");
     return y;
 }

@@ -94,6 +94,29 @@ int nx_bert_layer_forward_packed(nx_bert_layer *layer, const nx_ciphertext *cons
     });
 }
 
+int nx_bert_layer_forward_sets(nx_bert_layer *layer, const nx_ciphertext *const *x, size_t sets, size_t n,
+                               const int *seg_counts, const int *starts, const int *lens, nx_ciphertext *const *out)
+{
+    return run([&] {
+        need(layer);
+        need(x);
+        need(seg_counts);
+        need(starts);
+        need(lens);
+        need(out);
+        std::vector<std::vector<nexus::Ciphertext>> xs(sets);
+        std::vector<std::vector<nexus::BertLayer::Segment>> segs(sets);
+        size_t k = 0;
+        for (size_t s = 0; s < sets; s++) {
+            xs[s] = inputs(layer, x + s * n, n);
+            if (seg_counts[s] < 1) throw std::invalid_argument("every set needs a segment");
+            for (int j = 0; j < seg_counts[s]; j++, k++) segs[s].push_back({starts[k], lens[k]});
+        }
+        std::vector<std::vector<nexus::Ciphertext>> ys = layer->layer->forward_sets(xs, segs);
+        for (size_t s = 0; s < sets; s++) outputs(ys[s], out + s * n, n);
+    });
+}
+
 int nx_bert_layer_linear(nx_bert_layer *layer, int matrix, const nx_ciphertext *const *x, size_t n_in, double factor,
                          nx_ciphertext *const *out, size_t n_out)
 {

@@ -212,6 +212,34 @@ def test_server_continuous_batching(env):
         assert np.max(np.abs(got[n:])) < 2.0 ** -10
 
 
+def test_forward_sets_lockstep(env):
+    """three packed sets in lockstep (every bootstrapping step over all sets,
+    real messages paired across sets): each sequence within 2^-10 of the FP64
+    model and of its set's own layer call"""
+    segs = [[(0, 3), (3, 4)], [(0, 8)], [(1, 2), (4, 3)]]
+    _note("bert forward_sets %s" % segs)
+    rng = np.random.default_rng(60)
+    two = len(CHAIN) - 3 - 1
+    xs, sets = [], []
+    for g in segs:
+        x = np.zeros((env.T, HIDDEN))
+        for a, n in g:
+            x[a:a + n] = rng.normal(0, 1, (n, HIDDEN))
+        xs.append(x)
+        sets.append(env.enc(x, chain_index=two))
+    outs = env.layer.forward_sets(sets, segs)
+    assert len(outs) == 3 and all(len(o) == len(s) for o, s in zip(outs, sets))
+    for x, g, s, o in zip(xs, segs, sets, outs):
+        got = env.dec(o, env.T)
+        alone = env.dec(env.layer.forward(s, segments=g), env.T)
+        for a, n in g:
+            want = ob.layer_approx(x[a:a + n], env.W)
+            e_model = np.max(np.abs(got[a:a + n] - want))
+            e_alone = np.max(np.abs(got[a:a + n] - alone[a:a + n]))
+            print("set segment (%d, %d): max err vs model %.3g, vs its own call %.3g" % (a, n, e_model, e_alone))
+            assert e_model < 2.0 ** -10 and e_alone < 2.0 ** -10
+
+
 def test_server_packed_set(env):
     """a set the caller packed, served as one request on a lane: word for word
     the direct layer call over the same segments; two such sets run at once"""
