@@ -465,15 +465,16 @@ def bench_bert_call(nx, reps):
     B["layer"].set_stage_timing(True)
     B["layer"].forward(cts, BERT_TOKENS)
     return {"ms_per_layer_call": ms, "tokens": BERT_TOKENS, "stages_ms": dict(B["layer"].stage_ms()),
-            "autotune_modup": B["autotune"].get("modup", []), "autotune_moddown": B["autotune"].get("moddown", [])}
+            "autotune_modup": B["autotune"].get("modup", []), "autotune_moddown": B["autotune"].get("moddown", []),
+            "autotune_epilogue": B["autotune"].get("epilogue")}
 
 
 def fused_mode(mode):
     """the autotuner's choice for a conversion fused into the NTT column pass"""
     if mode < 0:
         return "unfused"
-    return "conversion in the column pass, %s outputs per workgroup, %d-column tiles" % (
-        mode & 255 or "default", 4 if mode & 256 else 8)
+    return "conversion in the column pass, %s outputs per workgroup, %d-column tiles, %d threads" % (
+        mode & 255 or "default", 4 if mode & 256 else 8, 512 if mode & 512 else 256)
 
 
 def bench_bert_sets(nx, reps, count=8):
@@ -785,6 +786,7 @@ def main():
                 "moddown": ["batch %d at %d limbs: %s (%.1f vs %.1f us unfused)"
                             % (e["batch"], e["limbs"], fused_mode(e["mode"]), e["us"], e["unfused_us"])
                             for e in B["autotune"].get("moddown", [])],
+                "moddown_epilogue": B["autotune"].get("epilogue"),
             },
         },
         "roofline": {

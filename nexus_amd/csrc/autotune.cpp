@@ -188,8 +188,8 @@ std::string autotune(Context &c, int reps)
     if (c.alpha > 1 && ks_modup_col_ok(c.tables, (uint32_t)c.alpha)) {
         // hybrid ModUp at the top level: the conversion fused into the forward
         // column pass (outputs per workgroup: default rule, 4, 8, 12, all of a
-        // digit; 8- or 4-column tiles, +256) against separate conversion and
-        // column-pass launches
+        // digit; 8- or 4-column tiles (+256), 256 or 512 threads (+512)) against
+        // separate conversion and column-pass launches
         const size_t l = c.L;
         for (uint32_t B : {1u, 2u, 4u, 8u, 16u, 32u}) {
             std::vector<uint64_t> h;
@@ -210,7 +210,7 @@ std::string autotune(Context &c, int reps)
             int best_m = -1;
             std::string times;
             char buf[64];
-            for (int m : {0, 4, 8, 12, 64, 256, 256 + 8, 256 + 12}) {
+            for (int m : {0, 4, 8, 12, 64, 256, 256 + 8, 256 + 12, 512, 512 + 8, 512 + 12}) {
                 ks_modup_col_set(B, m);
                 DevBuf T = ks_modup(c, tg.get(), B, l);
                 const bool same = download(c, T.get(), words) == ref;
@@ -234,6 +234,8 @@ std::string autotune(Context &c, int reps)
     }
     rep += "], \"moddown\": [";
     first_entry = true;
+    bool epi_int = false;
+    std::string epi_rep = "null";
     if (c.alpha > 1 && c.d_moddown_conv_d && ks_moddown_col_ok(c.tables, (uint32_t)c.alpha)) {
         // hybrid ModDown at the top level (B ciphertexts, 2B polys): the centred
         // conversion fused into the forward column pass against separate launches
@@ -269,7 +271,7 @@ std::string autotune(Context &c, int reps)
             int best_m = -1;
             std::string times;
             char buf[64];
-            for (int m : {0, 4, 8, 12, 64, 256, 256 + 8, 256 + 12}) {
+            for (int m : {0, 4, 8, 12, 64, 256, 256 + 8, 256 + 12, 512, 512 + 8, 512 + 12}) {
                 ks_moddown_col_set(2 * B, m);
                 run();
                 const bool same = download(c, out.get(), ow) == ref;
@@ -282,6 +284,21 @@ std::string autotune(Context &c, int reps)
                 }
             }
             ks_moddown_col_set(2 * B, best_m);
+            if (B == 16) {
+                // the ModDown's combining epilogue in double-precision or integer products
+                ntt_set_epilogue_int(true);
+                run();
+                const bool same = download(c, out.get(), ow) == ref;
+                const double t_int = same ? time_ms(run) : 1e30;
+                ntt_set_epilogue_int(false);
+                const double t_fp = time_ms(run);
+                epi_int = same && t_int < t_fp * 0.98;
+                ntt_set_epilogue_int(epi_int);
+                char eb[120];
+                std::snprintf(eb, sizeof eb, "{\"integer\": %d, \"fp_us\": %.2f, \"int_us\": %.2f}", epi_int ? 1 : 0,
+                              t_fp * 1e3, same ? t_int * 1e3 : -1.0);
+                epi_rep = eb;
+            }
             char line[300];
             std::snprintf(line, sizeof line,
                           "%s{\"batch\": %u, \"limbs\": %zu, \"mode\": %d, \"us\": %.2f, \"unfused_us\": %.2f, "
@@ -291,7 +308,7 @@ std::string autotune(Context &c, int reps)
             first_entry = false;
         }
     }
-    rep += "]}";
+    rep += "], \"epilogue\": " + epi_rep + "}";
     NX_HIP(hipStreamSynchronize(c.stream));
     return rep;
 }

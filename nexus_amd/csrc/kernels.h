@@ -101,6 +101,11 @@ void ntt_inverse(const DeviceTables &t, const NttBatch &b, hipStream_t s);
 // a direction and the power-of-two bucket of a batch size: 4, 8 or 16 (rows:
 // 4 or 8), 0 = the default rule.  Process-wide; set by the autotuner.
 void ntt_set_tile(int logn, int pass, bool fwd, uint32_t batch, int width);
+// The epilogue of the double-precision path in 64-bit integer products (on) or
+// in exact double-precision products (off, the default); the same words.
+// Process-wide; set by the autotuner.
+void ntt_set_epilogue_int(bool on);
+bool ntt_get_epilogue_int();
 int ntt_get_tile(int logn, int pass, bool fwd, uint32_t batch);
 
 // ---------------------------------------------------------------- dyadic ops
@@ -226,8 +231,9 @@ void ks_modup_copy_own(const DeviceTables &t, uint64_t *T, const uint64_t *targe
 // 2 <= alpha <= 8, 2^14 <= n <= 2^17, the double-precision factor table.
 bool ks_modup_col_ok(const DeviceTables &t, uint32_t alpha);
 // per power-of-two bucket of B: -1 = unfused; otherwise fused, with k =
-// mode & 255 outputs per workgroup (0: the default rule) and 4-column tiles
-// instead of 8 when mode & 256
+// mode & 255 outputs per workgroup (0: the default rule), 4-column tiles
+// instead of 8 when mode & 256, 512 threads per workgroup (8-column tiles,
+// half the registers per thread) when mode & 512
 void ks_modup_col_set(uint32_t B, int mode);
 int ks_modup_col_get(uint32_t B);
 void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *coef, const ulonglong2 *w,

@@ -60,6 +60,7 @@ struct NttKArgs {
     const double *ninvd;  // n^-1 mod q as doubles
     uint32_t skip_l, skip_alpha, skipped;  // NttBatch::skip_*; skipped: limbs left out
     NttEpilogue epi;                        // applied by the forward transform's last pass
+    uint32_t epi_int;                       // double-precision path: the epilogue in 64-bit integers
     uint8_t prime_of[kMaxPrimes];
 };
 
@@ -236,9 +237,36 @@ __device__ __forceinline__ double reduce_f(double z, double q, double qinv)
     return __builtin_fma(-__builtin_rint(z * qinv), q, z);
 }
 
+// epi_store in double precision (every prime of the batch below 2^51): the
+// transform value t in [0, q) as a double, the Shoup constants' values (below
+// q) as factors of exact products.  |x - t| < q, each product is within 1.25 q
+// and a term in [0, q) is added at most twice, so the sum stays below 2^53 and
+// the stored residue equals epi_store's.
+__device__ __forceinline__ void epi_store_f(const NttEpilogue &e, uint32_t p, uint32_t i, uint64_t gidx, double t,
+                                            double q, double qinv, uint32_t l, uint32_t n)
+{
+    const uint64_t off = (uint64_t)i * n + gidx;
+    const uint32_t ct = p / e.polys_per, k = p - ct * e.polys_per;
+    const uint64_t xv = e.xs[0] ? e.xs[ct][((uint64_t)k * e.x_limbs + i) * n + gidx] : e.x[p * e.x_stride + off];
+    double r = mulmod_f((double)xv - t, (double)e.inv[i].x, q, qinv);
+    uint64_t *o;
+    if (e.out) {
+        o = e.out + p * e.out_stride + off;
+        if (e.accumulate == 1 || (e.accumulate == 2 && (p & 1) == 0))
+            r += (double)(e.base ? e.base[p * e.base_stride + off] : *o);
+    } else {
+        const uint64_t *a0 = e.a0s[0] ? e.a0s[ct] : e.a0;
+        if (a0 && k == 0) r += (double)a0[(uint64_t)i * n + e.perm[ct][gidx]];
+        if (e.d) r += mulmod_f((double)e.d[p * e.d_stride + off], (double)e.inv2[i].x, q, qinv);
+        o = e.outs[ct] + ((uint64_t)k * l + i) * n + gidx;
+    }
+    r = reduce_f(r, q, qinv);
+    *o = (uint64_t)(r < 0 ? r + q : r);
+}
+
 // Forward radix-2^K step in double precision (values |v| <= q/2 (1 + eps)).
 // LAZY: the sums and differences are left unreduced (see ntt_tile_f).
-template <int LOGS, int ROLE, int K, int CW, bool LAZY>
+template <int LOGS, int ROLE, int K, int CW, bool LAZY, int NT = kThreads>
 __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw, double q, double qinv, uint32_t m0,
                                            uint32_t row0)
 {
@@ -248,7 +276,7 @@ __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw,
     constexpr int GROUPS = G::TILE / R;
     const int B = G::S >> s;
     const int D = B >> K;
-    for (int g = threadIdx.x; g < GROUPS; g += kThreads) {
+    for (int g = threadIdx.x; g < GROUPS; g += NT) {
         int cc, gg;
         if (ROLE == kCol) {
             cc = g % G::C;
@@ -296,7 +324,7 @@ __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw,
 
 // Inverse radix-2^K step in double precision.  LAZY: the products are left
 // unreduced (|v| <= 1.07 q; see ntt_tile_f).
-template <int LOGS, int ROLE, int K, int CW, bool LAZY>
+template <int LOGS, int ROLE, int K, int CW, bool LAZY, int NT = kThreads>
 __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw, double q, double qinv, uint32_t n,
                                            uint32_t mx, uint32_t row0)
 {
@@ -305,7 +333,7 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
     constexpr int GROUPS_PER = G::S / R;
     constexpr int GROUPS = G::TILE / R;
     const int so = 1 << s;
-    for (int g = threadIdx.x; g < GROUPS; g += kThreads) {
+    for (int g = threadIdx.x; g < GROUPS; g += NT) {
         int cc, gg;
         if (ROLE == kCol) {
             cc = g % G::C;
@@ -349,7 +377,7 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
 }
 
 // The radix steps of one pass in double precision, barrier after each.
-template <int LOGS, int ROLE, bool FWD, int CW, bool LAZY>
+template <int LOGS, int ROLE, bool FWD, int CW, bool LAZY, int NT = kThreads>
 __device__ __forceinline__ void stages_f(double *lds, const double *tw, double q, double qinv, uint32_t n,
                                          uint32_t N1, uint32_t first)
 {
@@ -359,16 +387,16 @@ __device__ __forceinline__ void stages_f(double *lds, const double *tw, double q
         if constexpr (LOGS >= 3) {
 #pragma unroll
             for (int st = 0; st < LOGS / 3; st++) {
-                fwd_step_f<LOGS, ROLE, 3, CW, LAZY>(lds, s, tw, q, qinv, m0, first);
+                fwd_step_f<LOGS, ROLE, 3, CW, LAZY, NT>(lds, s, tw, q, qinv, m0, first);
                 s += 3;
                 __syncthreads();
             }
         }
         if constexpr (LOGS % 3 == 2) {
-            fwd_step_f<LOGS, ROLE, 2, CW, LAZY>(lds, s, tw, q, qinv, m0, first);
+            fwd_step_f<LOGS, ROLE, 2, CW, LAZY, NT>(lds, s, tw, q, qinv, m0, first);
             __syncthreads();
         } else if constexpr (LOGS % 3 == 1) {
-            fwd_step_f<LOGS, ROLE, 1, CW, LAZY>(lds, s, tw, q, qinv, m0, first);
+            fwd_step_f<LOGS, ROLE, 1, CW, LAZY, NT>(lds, s, tw, q, qinv, m0, first);
             __syncthreads();
         }
     } else {
@@ -376,16 +404,16 @@ __device__ __forceinline__ void stages_f(double *lds, const double *tw, double q
         if constexpr (LOGS >= 3) {
 #pragma unroll
             for (int st = 0; st < LOGS / 3; st++) {
-                inv_step_f<LOGS, ROLE, 3, CW, LAZY>(lds, s, tw, q, qinv, n, mx, first);
+                inv_step_f<LOGS, ROLE, 3, CW, LAZY, NT>(lds, s, tw, q, qinv, n, mx, first);
                 s += 3;
                 __syncthreads();
             }
         }
         if constexpr (LOGS % 3 == 2) {
-            inv_step_f<LOGS, ROLE, 2, CW, LAZY>(lds, s, tw, q, qinv, n, mx, first);
+            inv_step_f<LOGS, ROLE, 2, CW, LAZY, NT>(lds, s, tw, q, qinv, n, mx, first);
             __syncthreads();
         } else if constexpr (LOGS % 3 == 1) {
-            inv_step_f<LOGS, ROLE, 1, CW, LAZY>(lds, s, tw, q, qinv, n, mx, first);
+            inv_step_f<LOGS, ROLE, 1, CW, LAZY, NT>(lds, s, tw, q, qinv, n, mx, first);
             __syncthreads();
         }
     }
@@ -451,10 +479,14 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
         else if (lazy)
             v = reduce_f(v, q, qinv);
         v = v < 0 ? v + q : v;
-        if (epi)
-            epi_store(a.epi, poly, limb, gidx, (uint64_t)v, (uint64_t)q, a.limbs, n);
-        else
+        if (epi) {
+            if (a.epi_int)
+                epi_store(a.epi, poly, limb, gidx, (uint64_t)v, (uint64_t)q, a.limbs, n);
+            else
+                epi_store_f(a.epi, poly, limb, gidx, v, q, qinv, a.limbs, n);
+        } else {
             data[gidx] = (uint64_t)v;
+        }
     }
 }
 
@@ -602,11 +634,11 @@ struct ModUpColArgs {
     uint32_t outp[kMaxPrimes];  // words: uniform indices take scalar loads
 };
 
-template <int LOGS, int A, int CW>
-__global__ __launch_bounds__(kThreads) void k_modup_col(ModUpColArgs a)
+template <int LOGS, int A, int CW, int NT>
+__global__ __launch_bounds__(NT) void k_modup_col(ModUpColArgs a)
 {
     using G = Geometry<LOGS, kCol, CW>;
-    constexpr int PER = G::TILE / kThreads;
+    constexpr int PER = G::TILE / NT;
     __shared__ double lds[G::TILE];
     const uint32_t n = 1u << a.logn;
     const uint32_t N2 = n >> LOGS, N1 = 1u << LOGS;
@@ -625,7 +657,7 @@ __global__ __launch_bounds__(kThreads) void k_modup_col(ModUpColArgs a)
     uint32_t gidx[PER];
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-        const int e = threadIdx.x + t * kThreads;
+        const int e = threadIdx.x + t * NT;
         gidx[t] = (uint32_t)(e / G::C) * N2 + col0 + (uint32_t)(e % G::C);
     }
     // the digit's products [x Dhat^-1]_q in [0, q) (rows i >= s stay zero)
@@ -661,19 +693,19 @@ __global__ __launch_bounds__(kThreads) void k_modup_col(ModUpColArgs a)
                 acc += mulmod_f(y[i][t], f[i], q, qinv);
                 if (i & 1) acc = reduce_f(acc, q, qinv);
             }
-            lds[threadIdx.x + t * kThreads] = reduce_f(acc, q, qinv);  // |v| <= q/2 (1 + eps)
+            lds[threadIdx.x + t * NT] = reduce_f(acc, q, qinv);  // |v| <= q/2 (1 + eps)
         }
         __syncthreads();
         const double *tw = a.twd + (uint64_t)kp * n;
         const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
         if (lazy)
-            stages_f<LOGS, kCol, true, CW, true>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, true, NT>(lds, tw, q, qinv, n, N1, col0);
         else
-            stages_f<LOGS, kCol, true, CW, false>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, false, NT>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.T + (((uint64_t)b * a.nout + I) * a.digits + j) * n;
 #pragma unroll
         for (int t = 0; t < PER; t++) {
-            double v = lds[threadIdx.x + t * kThreads];
+            double v = lds[threadIdx.x + t * NT];
             if (lazy) v = reduce_f(v, q, qinv);
             v = v < 0 ? v + q : v;
             out[gidx[t]] = (uint64_t)v;
@@ -772,11 +804,11 @@ struct ModDownColArgs {
     uint32_t inp[kMaxPrimes];
 };
 
-template <int LOGS, int A, int CW>
-__global__ __launch_bounds__(kThreads) void k_moddown_col(ModDownColArgs a)
+template <int LOGS, int A, int CW, int NT>
+__global__ __launch_bounds__(NT) void k_moddown_col(ModDownColArgs a)
 {
     using G = Geometry<LOGS, kCol, CW>;
-    constexpr int PER = G::TILE / kThreads;
+    constexpr int PER = G::TILE / NT;
     __shared__ double lds[G::TILE];
     const uint32_t n = 1u << a.logn;
     const uint32_t N2 = n >> LOGS, N1 = 1u << LOGS;
@@ -790,7 +822,7 @@ __global__ __launch_bounds__(kThreads) void k_moddown_col(ModDownColArgs a)
     uint32_t gidx[PER];
 #pragma unroll
     for (int t = 0; t < PER; t++) {
-        const int e = threadIdx.x + t * kThreads;
+        const int e = threadIdx.x + t * NT;
         gidx[t] = (uint32_t)(e / G::C) * N2 + col0 + (uint32_t)(e % G::C);
     }
     const uint64_t *y = a.acc + (uint64_t)p * a.acc_stride + (uint64_t)a.lo * n;
@@ -825,19 +857,19 @@ __global__ __launch_bounds__(kThreads) void k_moddown_col(ModDownColArgs a)
                 s += mulmod_f(v[k][t], f[k], q, qinv);
                 if (k & 1) s = reduce_f(s, q, qinv);
             }
-            lds[threadIdx.x + t * kThreads] = reduce_f(s, q, qinv);  // |v| <= q/2 (1 + eps)
+            lds[threadIdx.x + t * NT] = reduce_f(s, q, qinv);  // |v| <= q/2 (1 + eps)
         }
         __syncthreads();
         const double *tw = a.twd + (uint64_t)i * n;
         const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
         if (lazy)
-            stages_f<LOGS, kCol, true, CW, true>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, true, NT>(lds, tw, q, qinv, n, N1, col0);
         else
-            stages_f<LOGS, kCol, true, CW, false>(lds, tw, q, qinv, n, N1, col0);
+            stages_f<LOGS, kCol, true, CW, false, NT>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.z + ((uint64_t)p * a.lo + i) * n;
 #pragma unroll
         for (int t = 0; t < PER; t++) {
-            double x = lds[threadIdx.x + t * kThreads];
+            double x = lds[threadIdx.x + t * NT];
             if (lazy) x = reduce_f(x, q, qinv);
             x = x < 0 ? x + q : x;
             out[gidx[t]] = (uint64_t)x;
@@ -891,6 +923,10 @@ void launch_whole(const NttKArgs &a, uint32_t batch, hipStream_t s)
     default: break;
     }
 }
+
+// Epilogue arithmetic of the double-precision path (0: exact double-precision
+// products, 1: 64-bit integer Shoup products; the same words).  Set by the autotuner.
+std::atomic<int> g_epi_int{0};
 
 // Tile widths chosen by the autotuner: [log n - 14][pass][direction][batch bucket],
 // 0 = the default (tiles of 8).  Written by ntt_set_tile, read at every launch.
@@ -953,6 +989,7 @@ NttKArgs make_args(const DeviceTables &t, const NttBatch &b, bool fwd)
         for (uint32_t i = 0; i < b.polys; i++) a.srcs[i] = b.src_list[i];
     }
     a.epi = b.epi;
+    a.epi_int = (uint32_t)g_epi_int.load();
     if (!fwd) {
         a.epi.out = nullptr;
         a.epi.outs[0] = nullptr;
@@ -985,6 +1022,9 @@ static inline void split_logn(int logn, int &l1, int &l2)
     l1 = logn / 2;
     l2 = logn - l1;
 }
+
+void ntt_set_epilogue_int(bool on) { g_epi_int.store(on ? 1 : 0); }
+bool ntt_get_epilogue_int() { return g_epi_int.load() != 0; }
 
 void ntt_set_tile(int logn, int pass, bool fwd, uint32_t batch, int width)
 {
@@ -1034,6 +1074,7 @@ inline double col_flops(int logs) { return logs * (kFlopsProduct + 2.0) / 2.0 + 
 // Process-wide; set by the autotuner.
 static std::atomic<int> g_modup_col[kTuneBuckets];
 constexpr int kColNarrow = 256;  // mode bit: 4-column tiles
+constexpr int kColWide = 512;    // mode bit: 512 threads per workgroup (8-column tiles, half the registers each)
 
 void ks_modup_col_set(uint32_t B, int mode) { g_modup_col[tune_bucket(B)].store(mode); }
 
@@ -1071,6 +1112,7 @@ void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *co
     // tiles (half the registers per thread) when bit 8 is set
     const int mode = g_modup_col[tune_bucket(B)].load();
     const uint32_t cw = (mode > 0 && (mode & kColNarrow)) ? 4u : 8u;
+    const uint32_t nt = (mode > 0 && (mode & kColWide)) ? 512u : 256u;
     const uint32_t tiles = n / ((1u << l1) * cw);
     // outputs per workgroup: enough workgroups to fill the device, at least 4
     // outputs per read of the digit's inputs
@@ -1090,9 +1132,11 @@ void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *co
 #define NX_MC(LOGS, A)                                                                          \
     if (l1 == LOGS && alpha == A) {                                                           \
         if (cw == 4)                                                                          \
-            hipLaunchKernelGGL((k_modup_col<LOGS, A, 4>), grid, dim3(kThreads), 0, s, a);      \
+            hipLaunchKernelGGL((k_modup_col<LOGS, A, 4, 256>), grid, dim3(256), 0, s, a);      \
+        else if (nt == 512)                                                                   \
+            hipLaunchKernelGGL((k_modup_col<LOGS, A, 8, 512>), grid, dim3(512), 0, s, a);      \
         else                                                                                  \
-            hipLaunchKernelGGL((k_modup_col<LOGS, A, 8>), grid, dim3(kThreads), 0, s, a);      \
+            hipLaunchKernelGGL((k_modup_col<LOGS, A, 8, 256>), grid, dim3(256), 0, s, a);      \
         return;                                                                               \
     }
     NX_MC(7, 2) NX_MC(7, 3) NX_MC(7, 4) NX_MC(7, 5) NX_MC(7, 6) NX_MC(7, 7) NX_MC(7, 8)
@@ -1172,6 +1216,7 @@ void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *
     const uint32_t n = 1u << t.logn;
     const int mode = g_moddown_col[tune_bucket(polys)].load();
     const uint32_t cw = (mode > 0 && (mode & kColNarrow)) ? 4u : 8u;
+    const uint32_t nt = (mode > 0 && (mode & kColWide)) ? 512u : 256u;
     const uint32_t tiles = n / ((1u << l1) * cw);
     const uint32_t want = (2048u + tiles * polys - 1) / (tiles * polys);
     uint32_t chunk = (lo + want - 1) / want;
@@ -1186,9 +1231,11 @@ void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *
 #define NX_MD(LOGS, AA)                                                                         \
     if (l1 == LOGS && A == AA) {                                                              \
         if (cw == 4)                                                                          \
-            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 4>), grid, dim3(kThreads), 0, s, a);   \
+            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 4, 256>), grid, dim3(256), 0, s, a);   \
+        else if (nt == 512)                                                                   \
+            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8, 512>), grid, dim3(512), 0, s, a);   \
         else                                                                                  \
-            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8>), grid, dim3(kThreads), 0, s, a);   \
+            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8, 256>), grid, dim3(256), 0, s, a);   \
         return;                                                                               \
     }
     NX_MD(7, 2) NX_MD(7, 3) NX_MD(7, 4) NX_MD(7, 5) NX_MD(7, 6) NX_MD(7, 7) NX_MD(7, 8) NX_MD(7, 9)
