@@ -76,5 +76,20 @@ def build(verbose=False, jobs=None):
     return LIB
 
 
+EXAMPLE_SRC = os.path.join(os.path.dirname(HERE), "tests", "cpp", "dropin_example.cpp")
+
+
+def build_example(out):
+    """the compiled C++ caller of the drop-in (tests/cpp/dropin_example.cpp),
+    linked against the built library (found next to the binary's ../../nexus_amd)"""
+    rel = os.path.relpath(HERE, os.path.dirname(os.path.abspath(out)))
+    cmd = [CXX, "-O1", "-std=c++17", "-I" + INCLUDE, EXAMPLE_SRC, "-o", out, "-L" + HERE, "-lnexus_ckks",
+           "-Wl,-rpath,$ORIGIN/" + rel, "-Wl,-rpath-link,/opt/rocm/lib"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("example build failed: %s\n%s" % (" ".join(cmd), r.stdout + r.stderr))
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -1074,6 +1074,7 @@ inline double col_flops(int logs) { return logs * (kFlopsProduct + 2.0) / 2.0 + 
 // Process-wide; set by the autotuner.
 static std::atomic<int> g_modup_col[kTuneBuckets];
 constexpr int kColNarrow = 256;  // mode bit: 4-column tiles
+constexpr int kColMaxA = 12;   // largest digit / special-prime count of the fused conversions
 constexpr int kColWide = 512;    // mode bit: 512 threads per workgroup (8-column tiles, half the registers each)
 
 void ks_modup_col_set(uint32_t B, int mode) { g_modup_col[tune_bucket(B)].store(mode); }
@@ -1082,7 +1083,7 @@ int ks_modup_col_get(uint32_t B) { return g_modup_col[tune_bucket(B)].load(); }
 
 bool ks_modup_col_ok(const DeviceTables &t, uint32_t alpha)
 {
-    return t.max_bits <= 51 && alpha >= 2 && alpha <= 8 && t.logn >= 14 && t.logn <= 17 &&
+    return t.max_bits <= 51 && alpha >= 2 && alpha <= kColMaxA && t.logn >= 14 && t.logn <= 17 &&
            t.modup_conv_d != nullptr;
 }
 
@@ -1133,14 +1134,16 @@ void ks_modup_convert_col(const DeviceTables &t, uint64_t *T, const uint64_t *co
     if (l1 == LOGS && alpha == A) {                                                           \
         if (cw == 4)                                                                          \
             hipLaunchKernelGGL((k_modup_col<LOGS, A, 4, 256>), grid, dim3(256), 0, s, a);      \
-        else if (nt == 512)                                                                   \
+        else if (nt == 512 || A > 8)                                                          \
             hipLaunchKernelGGL((k_modup_col<LOGS, A, 8, 512>), grid, dim3(512), 0, s, a);      \
         else                                                                                  \
-            hipLaunchKernelGGL((k_modup_col<LOGS, A, 8, 256>), grid, dim3(256), 0, s, a);      \
+            hipLaunchKernelGGL((k_modup_col<LOGS, A, 8, (A > 8 ? 512 : 256)>), grid, dim3(A > 8 ? 512 : 256), 0, s, a); \
         return;                                                                               \
     }
-    NX_MC(7, 2) NX_MC(7, 3) NX_MC(7, 4) NX_MC(7, 5) NX_MC(7, 6) NX_MC(7, 7) NX_MC(7, 8)
-    NX_MC(8, 2) NX_MC(8, 3) NX_MC(8, 4) NX_MC(8, 5) NX_MC(8, 6) NX_MC(8, 7) NX_MC(8, 8)
+    NX_MC(7, 2) NX_MC(7, 3) NX_MC(7, 4) NX_MC(7, 5) NX_MC(7, 6) NX_MC(7, 7) NX_MC(7, 8) NX_MC(7, 9) NX_MC(7, 10)
+    NX_MC(7, 11) NX_MC(7, 12)
+    NX_MC(8, 2) NX_MC(8, 3) NX_MC(8, 4) NX_MC(8, 5) NX_MC(8, 6) NX_MC(8, 7) NX_MC(8, 8) NX_MC(8, 9) NX_MC(8, 10)
+    NX_MC(8, 11) NX_MC(8, 12)
 #undef NX_MC
     throw std::invalid_argument("ks_modup_convert_col: unsupported split");
 }
@@ -1189,7 +1192,7 @@ int ks_moddown_col_get(uint32_t polys) { return g_moddown_col[tune_bucket(polys)
 
 bool ks_moddown_col_ok(const DeviceTables &t, uint32_t A)
 {
-    return t.max_bits <= 51 && A >= 2 && A <= 9 && t.logn >= 14 && t.logn <= 17;
+    return t.max_bits <= 51 && A >= 2 && A <= kColMaxA + 1 && t.logn >= 14 && t.logn <= 17;
 }
 
 void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *acc, uint64_t acc_stride,
@@ -1232,14 +1235,16 @@ void ks_moddown_convert_col(const DeviceTables &t, uint64_t *z, const uint64_t *
     if (l1 == LOGS && A == AA) {                                                              \
         if (cw == 4)                                                                          \
             hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 4, 256>), grid, dim3(256), 0, s, a);   \
-        else if (nt == 512)                                                                   \
+        else if (nt == 512 || AA > 9)                                                         \
             hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8, 512>), grid, dim3(512), 0, s, a);   \
         else                                                                                  \
-            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8, 256>), grid, dim3(256), 0, s, a);   \
+            hipLaunchKernelGGL((k_moddown_col<LOGS, AA, 8, (AA > 9 ? 512 : 256)>), grid, dim3(AA > 9 ? 512 : 256), 0, s, a); \
         return;                                                                               \
     }
-    NX_MD(7, 2) NX_MD(7, 3) NX_MD(7, 4) NX_MD(7, 5) NX_MD(7, 6) NX_MD(7, 7) NX_MD(7, 8) NX_MD(7, 9)
-    NX_MD(8, 2) NX_MD(8, 3) NX_MD(8, 4) NX_MD(8, 5) NX_MD(8, 6) NX_MD(8, 7) NX_MD(8, 8) NX_MD(8, 9)
+    NX_MD(7, 2) NX_MD(7, 3) NX_MD(7, 4) NX_MD(7, 5) NX_MD(7, 6) NX_MD(7, 7) NX_MD(7, 8) NX_MD(7, 9) NX_MD(7, 10)
+    NX_MD(7, 11) NX_MD(7, 12) NX_MD(7, 13)
+    NX_MD(8, 2) NX_MD(8, 3) NX_MD(8, 4) NX_MD(8, 5) NX_MD(8, 6) NX_MD(8, 7) NX_MD(8, 8) NX_MD(8, 9) NX_MD(8, 10)
+    NX_MD(8, 11) NX_MD(8, 12) NX_MD(8, 13)
 #undef NX_MD
     throw std::invalid_argument("ks_moddown_convert_col: unsupported split");
 }
