@@ -465,6 +465,7 @@ def bench_bert_call(nx, reps):
     B["layer"].set_stage_timing(True)
     B["layer"].forward(cts, BERT_TOKENS)
     return {"ms_per_layer_call": ms, "tokens": BERT_TOKENS, "stages_ms": dict(B["layer"].stage_ms()),
+            "autotune_ntt": B["autotune"]["ntt"],
             "autotune_modup": B["autotune"].get("modup", []), "autotune_moddown": B["autotune"].get("moddown", []),
             "autotune_epilogue": B["autotune"].get("epilogue")}
 
@@ -776,9 +777,9 @@ def main():
             "scale": "2^46",
             "galois_keys": B["keys"],
             "autotuned": {
-                "ntt_tiles": ["%s batch %d: col %d row %d (%.1f vs %.1f us)" % (e["dir"], e["batch"], e["col"], e["row"],
-                                                                         e["us"], e["default_us"])
-                              for e in B["autotune"]["ntt"] if e["col"] or e["row"]],
+                "ntt_tiles": ["%s batch %d: col %d row %d radix16 %s (%.1f vs %.1f us)"
+                              % (e["dir"], e["batch"], e["col"], e["row"], e.get("radix16"), e["us"], e["default_us"])
+                              for e in B["autotune"]["ntt"] if e["col"] or e["row"] or any(e.get("radix16", []))],
                 "dot_plain_layouts": ["%d limbs: %d" % (d["limbs"], d["layout"]) for d in B["autotune"]["dot_plain"]],
                 "modup": ["batch %d at %d limbs: %s (%.1f vs %.1f us unfused)"
                           % (e["batch"], e["limbs"], fused_mode(e["mode"]), e["us"], e["unfused_us"])

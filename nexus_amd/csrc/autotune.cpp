@@ -1,6 +1,9 @@
-// Autotuner of launch shapes at a context's ring size: the tile widths of the
-// two-pass NTT (per pass, direction and power-of-two batch bucket) and the
-// thread layout of the fused multiply-accumulate (per limb bucket).  Every
+// Autotuner of launch shapes at a context's ring size: the tile widths and the
+// radix (8 or 16) of the two-pass NTT (per pass, direction and power-of-two
+// batch bucket), the thread layout of the fused multiply-accumulate (per limb
+// bucket), the base conversions fused into the NTT column pass or not (and
+// their outputs per workgroup, tile width and threads), and the arithmetic of
+// the ModDown epilogue.  Every
 // candidate's output is compared word for word with the default shape's on
 // the same data before it may be chosen; the choices are process-wide.
 #include <algorithm>
@@ -109,6 +112,8 @@ std::string autotune(Context &c, int reps)
                 // reference output of the default shapes
                 ntt_set_tile(c.logn, 0, fwd != 0, batch, 0);
                 ntt_set_tile(c.logn, 1, fwd != 0, batch, 0);
+                ntt_set_radix16(c.logn, 0, fwd != 0, batch, false);
+                ntt_set_radix16(c.logn, 1, fwd != 0, batch, false);
                 reset();
                 run();
                 const std::vector<uint64_t> ref = download(c, work.get(), (size_t)batch * n);
@@ -131,12 +136,28 @@ std::string autotune(Context &c, int reps)
                     }
                 ntt_set_tile(c.logn, 0, fwd != 0, batch, best_c);
                 ntt_set_tile(c.logn, 1, fwd != 0, batch, best_r);
-                char buf[200];
+                // radix-16 steps per pass (double-precision path) with the chosen tiles
+                int r16[2] = {0, 0}, matched[2] = {0, 0};
+                for (int pass = 0; pass < 2; pass++) {
+                    ntt_set_radix16(c.logn, pass, fwd != 0, batch, true);
+                    reset();
+                    run();
+                    const bool same = download(c, work.get(), (size_t)batch * n) == ref;
+                    matched[pass] = same ? 1 : 0;
+                    const double t = same ? time_ms(run) : 1e30;
+                    if (t < best * 0.98) {
+                        best = t;
+                        r16[pass] = 1;
+                    } else {
+                        ntt_set_radix16(c.logn, pass, fwd != 0, batch, false);
+                    }
+                }
+                char buf[280];
                 std::snprintf(buf, sizeof buf,
-                              "%s{\"dir\": \"%s\", \"batch\": %u, \"col\": %d, \"row\": %d, \"us\": %.2f, "
-                              "\"default_us\": %.2f}",
-                              first_entry ? "" : ", ", fwd ? "fwd" : "inv", batch, best_c, best_r, best * 1e3,
-                              t_def * 1e3);
+                              "%s{\"dir\": \"%s\", \"batch\": %u, \"col\": %d, \"row\": %d, \"radix16\": [%d, %d], "
+                              "\"radix16_matched\": [%d, %d], \"us\": %.2f, \"default_us\": %.2f}",
+                              first_entry ? "" : ", ", fwd ? "fwd" : "inv", batch, best_c, best_r, r16[0], r16[1],
+                              matched[0], matched[1], best * 1e3, t_def * 1e3);
                 rep += buf;
                 first_entry = false;
             }

@@ -107,6 +107,11 @@ void ntt_set_tile(int logn, int pass, bool fwd, uint32_t batch, int width);
 void ntt_set_epilogue_int(bool on);
 bool ntt_get_epilogue_int();
 int ntt_get_tile(int logn, int pass, bool fwd, uint32_t batch);
+// Radix of the double-precision passes (n >= 2^14) for a pass, direction and
+// batch bucket: radix-16 steps (on) or radix-8 (off, the default).  Process-wide;
+// set by the autotuner.
+void ntt_set_radix16(int logn, int pass, bool fwd, uint32_t batch, bool on);
+bool ntt_get_radix16(int logn, int pass, bool fwd, uint32_t batch);
 
 // ---------------------------------------------------------------- dyadic ops
 // All pointers address [polys][limbs][n] arrays; limb i uses prime i.

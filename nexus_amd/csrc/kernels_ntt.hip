@@ -377,10 +377,41 @@ __device__ __forceinline__ void inv_step_f(double *lds, int s, const double *tw,
 }
 
 // The radix steps of one pass in double precision, barrier after each.
-template <int LOGS, int ROLE, bool FWD, int CW, bool LAZY, int NT = kThreads>
+template <int LOGS, int ROLE, bool FWD, int CW, bool LAZY, int NT = kThreads, bool R16 = false>
 __device__ __forceinline__ void stages_f(double *lds, const double *tw, double q, double qinv, uint32_t n,
                                          uint32_t N1, uint32_t first)
 {
+    if constexpr (R16) {
+        // radix-16 steps (half the LDS passes and barriers of radix-8), then
+        // the remaining 1..3 stages in one step
+        int s = 0;
+        if (FWD) {
+            const uint32_t m0 = ROLE == kRow ? N1 : 1u;
+#pragma unroll
+            for (int st = 0; st < LOGS / 4; st++) {
+                fwd_step_f<LOGS, ROLE, 4, CW, LAZY, NT>(lds, s, tw, q, qinv, m0, first);
+                s += 4;
+                __syncthreads();
+            }
+            if constexpr (LOGS % 4 != 0) {
+                fwd_step_f<LOGS, ROLE, LOGS % 4, CW, LAZY, NT>(lds, s, tw, q, qinv, m0, first);
+                __syncthreads();
+            }
+        } else {
+            const uint32_t mx = ROLE == kCol ? N1 : n;
+#pragma unroll
+            for (int st = 0; st < LOGS / 4; st++) {
+                inv_step_f<LOGS, ROLE, 4, CW, LAZY, NT>(lds, s, tw, q, qinv, n, mx, first);
+                s += 4;
+                __syncthreads();
+            }
+            if constexpr (LOGS % 4 != 0) {
+                inv_step_f<LOGS, ROLE, LOGS % 4, CW, LAZY, NT>(lds, s, tw, q, qinv, n, mx, first);
+                __syncthreads();
+            }
+        }
+        return;
+    }
     int s = 0;
     if (FWD) {
         const uint32_t m0 = ROLE == kRow ? N1 : 1u;
@@ -432,7 +463,7 @@ __device__ __forceinline__ void stages_f(double *lds, const double *tw, double q
 // (Gentleman-Sande): the sums are reduced (|x + y| <= 1.82 q), the products
 // are not (<= 0.91 q); the store (or the n^-1 product) reduces once.  The
 // stored words are the fully reduced residues either way.
-template <int LOGS, int ROLE, bool FWD, int CW>
+template <int LOGS, int ROLE, bool FWD, int CW, bool R16>
 __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_words, uint64_t *data,
                                            const uint64_t *src, uint32_t prime, uint32_t n, uint32_t N2, uint32_t N1,
                                            uint32_t first, uint32_t poly, uint32_t limb)
@@ -458,9 +489,9 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
     __syncthreads();
     const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform: one prime per workgroup
     if (lazy)
-        stages_f<LOGS, ROLE, FWD, CW, true>(lds, tw, q, qinv, n, N1, first);
+        stages_f<LOGS, ROLE, FWD, CW, true, kThreads, R16>(lds, tw, q, qinv, n, N1, first);
     else
-        stages_f<LOGS, ROLE, FWD, CW, false>(lds, tw, q, qinv, n, N1, first);
+        stages_f<LOGS, ROLE, FWD, CW, false, kThreads, R16>(lds, tw, q, qinv, n, N1, first);
     const bool scale_out = !FWD && ROLE != kRow;  // the inverse's last pass
     const double ni = a.ninvd[prime];
     for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
@@ -490,7 +521,7 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
     }
 }
 
-template <int LOGS, int ROLE, bool FWD, int CW>
+template <int LOGS, int ROLE, bool FWD, int CW, bool R16>
 __device__ __forceinline__ void ntt_pass(const NttKArgs &a)
 {
     using G = Geometry<LOGS, ROLE, CW>;
@@ -518,7 +549,7 @@ __device__ __forceinline__ void ntt_pass(const NttKArgs &a)
     if (ROLE == kCol && (gridDim.x & 7) == 0) tile = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const uint32_t first = tile * G::C;  // first column / row of the tile
     if (q < kFpPrimeBound) {
-        ntt_tile_f<LOGS, ROLE, FWD, CW>(a, lds, data, src, prime, n, N2, N1, first, poly, limb);
+        ntt_tile_f<LOGS, ROLE, FWD, CW, R16>(a, lds, data, src, prime, n, N2, N1, first, poly, limb);
         return;
     }
 
@@ -605,10 +636,10 @@ __device__ __forceinline__ void ntt_pass(const NttKArgs &a)
     }
 }
 
-template <int LOGS, int ROLE, bool FWD, int CW>
+template <int LOGS, int ROLE, bool FWD, int CW, bool R16>
 __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
 {
-    ntt_pass<LOGS, ROLE, FWD, CW>(a);
+    ntt_pass<LOGS, ROLE, FWD, CW, R16>(a);
 }
 
 // ---- ModUp base conversion fused into the forward column pass
@@ -879,13 +910,13 @@ __global__ __launch_bounds__(NT) void k_moddown_col(ModDownColArgs a)
 
 // The forward column pass with 8-wide tiles held to 64 VGPRs: 8 waves per SIMD
 // instead of 7 (the other passes reach 8 as they are, or would spill).
-template <int LOGS, int ROLE, bool FWD, int CW>
+template <int LOGS, int ROLE, bool FWD, int CW, bool R16>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void k_ntt_w8(NttKArgs a)
 {
-    ntt_pass<LOGS, ROLE, FWD, CW>(a);
+    ntt_pass<LOGS, ROLE, FWD, CW, R16>(a);
 }
 
-template <int LOGS, int ROLE, bool FWD, int CW = 16>
+template <int LOGS, int ROLE, bool FWD, int CW = 16, bool R16 = false>
 void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
 {
     using G = Geometry<LOGS, ROLE, CW>;
@@ -905,10 +936,10 @@ void launch_one(const NttKArgs &a, uint32_t batch, hipStream_t s)
     const double epi_bytes = epi ? 8.0 * n * (1.0 + epi_reads) : 0.0;
     ProfScope prof(epi ? "ntt_fwd_sub_mul" : names[FWD ? 1 : 0][ROLE], s,
                    (double)(batch - a.skipped) * (16.0 * n + a.tw_bytes * tw + epi_bytes));
-    if constexpr (ROLE == kCol && FWD && CW == 8)
-        hipLaunchKernelGGL((k_ntt_w8<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
+    if constexpr (ROLE == kCol && FWD && CW == 8 && !R16)
+        hipLaunchKernelGGL((k_ntt_w8<LOGS, ROLE, FWD, CW, R16>), grid, dim3(kThreads), 0, s, a);
     else
-        hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW>), grid, dim3(kThreads), 0, s, a);
+        hipLaunchKernelGGL((k_ntt<LOGS, ROLE, FWD, CW, R16>), grid, dim3(kThreads), 0, s, a);
 }
 
 template <bool FWD>
@@ -931,6 +962,9 @@ std::atomic<int> g_epi_int{0};
 // Tile widths chosen by the autotuner: [log n - 14][pass][direction][batch bucket],
 // 0 = the default (tiles of 8).  Written by ntt_set_tile, read at every launch.
 std::atomic<int> g_tile[kTuneLogN][2][2][kTuneBuckets];
+// Radix of the double-precision passes, same indexing: 0 = radix-8 steps, 1 =
+// radix-16 steps (the integer path always takes radix 8).  Set by the autotuner.
+std::atomic<int> g_radix[kTuneLogN][2][2][kTuneBuckets];
 
 int tune_bucket(uint32_t batch)
 {
@@ -939,19 +973,31 @@ int tune_bucket(uint32_t batch)
     return b;  // floor(log2(batch)), capped
 }
 
+template <int LOGS, int ROLE, bool FWD, bool R16>
+void launch_width(int cw, const NttKArgs &a, uint32_t batch, hipStream_t s)
+{
+    if (cw == 4)
+        launch_one<LOGS, ROLE, FWD, 4, R16>(a, batch, s);
+    else if (cw == 8)
+        launch_one<LOGS, ROLE, FWD, 8, R16>(a, batch, s);
+    else
+        launch_one<LOGS, ROLE, FWD, 16, R16>(a, batch, s);
+}
+
 template <int LOGS, int ROLE, bool FWD>
 void launch_sized(const NttKArgs &a, uint32_t batch, hipStream_t s)
 {
-    int cw = 0;
+    int cw = 0, r16 = 0;
     const int li = (int)a.logn - kTuneLogNMin;
-    if (li >= 0 && li < kTuneLogN) cw = g_tile[li][ROLE == kCol ? 0 : 1][FWD ? 1 : 0][tune_bucket(batch)].load();
+    if (li >= 0 && li < kTuneLogN) {
+        cw = g_tile[li][ROLE == kCol ? 0 : 1][FWD ? 1 : 0][tune_bucket(batch)].load();
+        r16 = g_radix[li][ROLE == kCol ? 0 : 1][FWD ? 1 : 0][tune_bucket(batch)].load();
+    }
     if (!cw) cw = 8;
-    if (cw == 4)
-        launch_one<LOGS, ROLE, FWD, 4>(a, batch, s);
-    else if (cw == 8)
-        launch_one<LOGS, ROLE, FWD, 8>(a, batch, s);
+    if (r16)
+        launch_width<LOGS, ROLE, FWD, true>(cw, a, batch, s);
     else
-        launch_one<LOGS, ROLE, FWD, 16>(a, batch, s);
+        launch_width<LOGS, ROLE, FWD, false>(cw, a, batch, s);
 }
 
 template <int ROLE, bool FWD>
@@ -1032,6 +1078,20 @@ void ntt_set_tile(int logn, int pass, bool fwd, uint32_t batch, int width)
     if (li < 0 || li >= kTuneLogN || pass < 0 || pass > 1) return;
     if (width != 0 && width != 4 && width != 8 && width != 16) return;
     g_tile[li][pass][fwd ? 1 : 0][tune_bucket(batch)].store(width);
+}
+
+void ntt_set_radix16(int logn, int pass, bool fwd, uint32_t batch, bool on)
+{
+    const int li = logn - kTuneLogNMin;
+    if (li < 0 || li >= kTuneLogN || pass < 0 || pass > 1) return;
+    g_radix[li][pass][fwd ? 1 : 0][tune_bucket(batch)].store(on ? 1 : 0);
+}
+
+bool ntt_get_radix16(int logn, int pass, bool fwd, uint32_t batch)
+{
+    const int li = logn - kTuneLogNMin;
+    if (li < 0 || li >= kTuneLogN || pass < 0 || pass > 1) return false;
+    return g_radix[li][pass][fwd ? 1 : 0][tune_bucket(batch)].load() != 0;
 }
 
 int ntt_get_tile(int logn, int pass, bool fwd, uint32_t batch)

@@ -30,6 +30,8 @@ def test_autotune_then_parity(logn, chain):
     assert len(rep["ntt"]) == 20  # 2 directions x batches 1..512
     for e in rep["ntt"]:
         assert e["col"] in (0, 4, 8, 16) and e["row"] in (0, 4, 8)
+        assert e["radix16"][0] in (0, 1) and e["radix16"][1] in (0, 1)
+        assert e["radix16_matched"] == [1, 1]  # radix-16 passes give the default's words
     assert rep["dot_plain"] and all(d["layout"] in (0, 1) for d in rep["dot_plain"])
     # NTT round trip and forward parity with the oracle under the tuned shapes
     rng = np.random.default_rng(logn)
