@@ -51,6 +51,12 @@ public:
 
     struct Options {
         int baby = 32;                       // baby steps of the linear maps (power of two)
+        // token rotations rot(x, 256 t) and within-head offset rotations rot(x, d)
+        // hoisted directly from one ModUp of x (a key per step: about 250 more
+        // Galois keys, 50 GB at N = 2^16) instead of baby and giant steps; the
+        // layer call measured the same either way (the key switches, not the
+        // ModUps, dominate these stages), so the baby-giant form is the default
+        bool direct_rotations = false;
         double ln_prescale = 5.477225575051661;  // sqrt(30)
         bool stage_timing = false;           // synchronise and time each stage of forward()
     };

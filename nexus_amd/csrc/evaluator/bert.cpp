@@ -66,14 +66,21 @@ std::vector<int> BertLayer::rotation_steps(size_t slots, const BertShape &shape,
     for (int b = 1; b < opt.baby; b++) s.push_back(b);
     for (int a = floor_div(-(kGroup - 1), opt.baby); a <= floor_div(kGroup - 1, opt.baby); a++)
         if (a) s.push_back(opt.baby * a);
-    // token rotations: 256 b and 256 tb a
-    const int tb = sqrt_pow2((size_t)T);
-    for (int b = 1; b < tb && b < T; b++) s.push_back(kGroup * b);
-    for (int a = 1; a * tb < T; a++) s.push_back(kGroup * tb * a);
-    // within-head offsets d in [-63, 63]: babies 1..7, giants 8 a
-    for (int b = 1; b < kOffBaby; b++) s.push_back(b);
-    for (int a = floor_div(-(kHeadDim - 1), kOffBaby); a <= floor_div(kHeadDim - 1, kOffBaby); a++)
-        if (a) s.push_back(kOffBaby * a);
+    if (opt.direct_rotations) {
+        // every token rotation 256 t and every within-head offset d in [-63, 63]
+        for (int t = 1; t < T; t++) s.push_back(kGroup * t);
+        for (int d = -(kHeadDim - 1); d < kHeadDim; d++)
+            if (d) s.push_back(d);
+    } else {
+        // token rotations: 256 b and 256 tb a
+        const int tb = sqrt_pow2((size_t)T);
+        for (int b = 1; b < tb && b < T; b++) s.push_back(kGroup * b);
+        for (int a = 1; a * tb < T; a++) s.push_back(kGroup * tb * a);
+        // within-head offsets d in [-63, 63]: babies 1..7, giants 8 a
+        for (int b = 1; b < kOffBaby; b++) s.push_back(b);
+        for (int a = floor_div(-(kHeadDim - 1), kOffBaby); a <= floor_div(kHeadDim - 1, kOffBaby); a++)
+            if (a) s.push_back(kOffBaby * a);
+    }
     // sums and broadcasts within 64-slot blocks and 256-slot groups
     for (int st = 1; st < kGroup; st <<= 1) {
         s.push_back(st);
@@ -222,32 +229,66 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
 
 std::vector<Ciphertext> BertLayer::token_rotations(const Ciphertext &x, int t0, int count)
 {
-    // rot(x, 256 t) for t in [t0, t0 + count), t = tb a + b: hoisted babies
-    // 256 b, then per baby hoisted giants 256 tb a
+    // rot(x, 256 t) for t in [t0, t0 + count), t = tb a + b.  A range of up to
+    // 64 tokens has about 4 giants a and 16 babies b, so the giants 256 tb a
+    // come first, hoisted from x together with the babies of a = 0 (one ModUp
+    // of x), then each giant's babies 256 b hoisted from it: 1 + (giants - 1)
+    // ModUps instead of one per baby, the same number of key switches.
     const Evaluator &ev = ck_.evaluator;
-    const int tb = sqrt_pow2(T_);
+    // direct: every rotation hoisted from one ModUp of x (tb = T: a = 0 only)
+    const int tb = opt_.direct_rotations ? (int)T_ : sqrt_pow2(T_);
     std::vector<Ciphertext> out(count);
-    std::vector<int> need_b;
+    std::vector<int> need_a;
     for (int t = t0; t < t0 + count; t++)
-        if (std::find(need_b.begin(), need_b.end(), t % tb) == need_b.end()) need_b.push_back(t % tb);
-    std::sort(need_b.begin(), need_b.end());
-    std::vector<int> bsteps;
-    for (int b : need_b)
-        if (b) bsteps.push_back(kGroup * b);
-    std::vector<Ciphertext> br;
-    if (!bsteps.empty()) ev.rotate_vector_hoisted(x, bsteps, *ck_.galois_keys, br);
-    size_t bi = 0;
-    for (int b : need_b) {
-        const Ciphertext &base = b ? br[bi++] : x;
-        std::vector<int> as, gsteps;
+        if (std::find(need_a.begin(), need_a.end(), t / tb) == need_a.end()) need_a.push_back(t / tb);
+    std::sort(need_a.begin(), need_a.end());
+    auto babies_of = [&](int a) {
+        std::vector<int> bs;
         for (int t = t0; t < t0 + count; t++)
-            if (t % tb == b) as.push_back(t / tb);
-        for (int a : as)
-            if (a) gsteps.push_back(kGroup * tb * a);
-        std::vector<Ciphertext> gr;
-        if (!gsteps.empty()) ev.rotate_vector_hoisted(base, gsteps, *ck_.galois_keys, gr);
-        size_t gi = 0;
-        for (int a : as) out[tb * a + b - t0] = a ? std::move(gr[gi++]) : base;
+            if (t / tb == a) bs.push_back(t % tb);
+        return bs;
+    };
+    // from x: every giant a != 0 and the babies of a = 0
+    std::vector<int> xsteps;
+    std::vector<std::pair<int, int>> xwhat;  // (a, b) of each x rotation
+    for (int a : need_a) {
+        if (a) {
+            xsteps.push_back(kGroup * tb * a);
+            xwhat.push_back({a, 0});
+        } else {
+            for (int b : babies_of(0))
+                if (b) {
+                    xsteps.push_back(kGroup * b);
+                    xwhat.push_back({0, b});
+                }
+        }
+    }
+    std::vector<Ciphertext> xr;
+    if (!xsteps.empty()) ev.rotate_vector_hoisted(x, xsteps, *ck_.galois_keys, xr);
+    std::vector<Ciphertext> giant(need_a.empty() ? 1 : need_a.back() + 1);
+    for (size_t i = 0; i < xsteps.size(); i++) {
+        const int a = xwhat[i].first, b = xwhat[i].second;
+        if (b)
+            out[tb * a + b - t0] = std::move(xr[i]);
+        else
+            giant[a] = std::move(xr[i]);
+    }
+    for (int a : need_a) {
+        const Ciphertext &base = a ? giant[a] : x;
+        std::vector<int> bsteps;
+        std::vector<int> bs = babies_of(a);
+        for (int b : bs)
+            if (b && a) bsteps.push_back(kGroup * b);
+        std::vector<Ciphertext> br;
+        if (!bsteps.empty()) ev.rotate_vector_hoisted(base, bsteps, *ck_.galois_keys, br);
+        size_t bi = 0;
+        for (int b : bs) {
+            Ciphertext &o = out[tb * a + b - t0];
+            if (!b)
+                o = a ? std::move(giant[a]) : x;  // the giant itself (its babies are already rotated)
+            else if (a)
+                o = std::move(br[bi++]);
+        }
     }
     return out;
 }
@@ -304,6 +345,17 @@ std::vector<Ciphertext> BertLayer::offset_rotations(const Ciphertext &x, int dmi
     // rot(x, d) for d in [dmin, dmax], d = 8 a + b
     const Evaluator &ev = ck_.evaluator;
     std::vector<Ciphertext> out(dmax - dmin + 1);
+    if (opt_.direct_rotations) {
+        // every offset hoisted from one ModUp of x
+        std::vector<int> steps;
+        for (int d = dmin; d <= dmax; d++)
+            if (d) steps.push_back(d);
+        std::vector<Ciphertext> r;
+        if (!steps.empty()) ev.rotate_vector_hoisted(x, steps, *ck_.galois_keys, r);
+        size_t i = 0;
+        for (int d = dmin; d <= dmax; d++) out[d - dmin] = d ? std::move(r[i++]) : x;
+        return out;
+    }
     std::vector<int> bsteps;
     for (int b = 1; b < kOffBaby; b++) bsteps.push_back(b);
     std::vector<Ciphertext> br;
