@@ -155,7 +155,12 @@ private:
     std::shared_ptr<const Encoded> diagonals(Matrix m, double factor, size_t chain_index, double scale);
     Ciphertext rotate_tokens(const Ciphertext &x, int t);
     std::vector<Ciphertext> token_rotations(const Ciphertext &x, int t0, int count);
+    // rot(x, 256 t) for every x of xs and t of ts (distinct offsets in [0, T)), batched
+    std::vector<std::vector<Ciphertext>> token_rotations_many(const std::vector<const Ciphertext *> &xs,
+                                                              const std::vector<int> &ts);
     std::vector<Ciphertext> offset_rotations(const Ciphertext &x, int dmin, int dmax);
+    std::vector<std::vector<Ciphertext>> offset_rotations_many(const std::vector<const Ciphertext *> &xs, int dmin,
+                                                               int dmax);
     // the 64 one-hot masks A_p (1 at slots = p mod 64), period 64, as periodic plaintexts
     const PeriodicPlaintexts &offset_masks(size_t limbs, double scale);
     // per-token segment index (-1 outside every segment) and its 1 / length

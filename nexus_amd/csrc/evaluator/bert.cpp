@@ -186,13 +186,18 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
     const int B = opt_.baby, n_a = d.a_max - d.a_min + 1;
     std::vector<int> bsteps;
     for (int b = 1; b < B; b++) bsteps.push_back(b);
+    // the babies of every input in one batched hoisted call (one ModUp launch)
     std::vector<std::vector<Ciphertext>> baby(x.size());
-    for (size_t c = 0; c < x.size(); c++) {
-        std::vector<Ciphertext> r;
-        ev.rotate_vector_hoisted(in[c], bsteps, *ck_.galois_keys, r);
-        baby[c].reserve(B);
-        baby[c].push_back(in[c]);
-        for (auto &t : r) baby[c].push_back(std::move(t));
+    {
+        std::vector<const Ciphertext *> ip;
+        for (auto &c : in) ip.push_back(&c);
+        std::vector<std::vector<Ciphertext>> r;
+        ev.rotate_vector_hoisted_many(ip, bsteps, *ck_.galois_keys, r);
+        for (size_t c = 0; c < x.size(); c++) {
+            baby[c].reserve(B);
+            baby[c].push_back(std::move(in[c]));
+            for (auto &t : r[c]) baby[c].push_back(std::move(t));
+        }
     }
     std::vector<Ciphertext> out(d.n_out);
     for (int o = 0; o < d.n_out; o++) {
@@ -229,26 +234,43 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
 
 std::vector<Ciphertext> BertLayer::token_rotations(const Ciphertext &x, int t0, int count)
 {
-    // rot(x, 256 t) for t in [t0, t0 + count), t = tb a + b.  A range of up to
-    // 64 tokens has about 4 giants a and 16 babies b, so the giants 256 tb a
-    // come first, hoisted from x together with the babies of a = 0 (one ModUp
-    // of x), then each giant's babies 256 b hoisted from it: 1 + (giants - 1)
-    // ModUps instead of one per baby, the same number of key switches.
+    std::vector<int> ts;
+    for (int t = t0; t < t0 + count; t++) ts.push_back(t);
+    std::vector<std::vector<Ciphertext>> r = token_rotations_many({&x}, ts);
+    return std::move(r[0]);
+}
+
+std::vector<std::vector<Ciphertext>> BertLayer::token_rotations_many(const std::vector<const Ciphertext *> &xs,
+                                                                     const std::vector<int> &ts)
+{
+    // rot(x, 256 t) for every x and every t of ts (distinct, in [0, T)), t = tb a + b.
+    // A range of 64 tokens has about 4 giants a and 16 babies b, so the giants
+    // 256 tb a come first, hoisted from each x together with the babies of a = 0
+    // (one ModUp of each x, all x in one batched call), then every giant's
+    // babies 256 b in one more batched call: far fewer ModUps than one per
+    // baby, the same number of key switches.
     const Evaluator &ev = ck_.evaluator;
     // direct: every rotation hoisted from one ModUp of x (tb = T: a = 0 only)
     const int tb = opt_.direct_rotations ? (int)T_ : sqrt_pow2(T_);
-    std::vector<Ciphertext> out(count);
+    const size_t X = xs.size();
+    std::vector<std::vector<Ciphertext>> out(X, std::vector<Ciphertext>(ts.size()));
+    std::vector<int> pos(T_, -1);  // t -> its index in ts
+    for (size_t k = 0; k < ts.size(); k++) {
+        if (ts[k] < 0 || (size_t)ts[k] >= T_ || pos[ts[k]] >= 0)
+            throw std::invalid_argument("BertLayer: token offsets must be distinct and in [0, T)");
+        pos[ts[k]] = (int)k;
+    }
     std::vector<int> need_a;
-    for (int t = t0; t < t0 + count; t++)
+    for (int t : ts)
         if (std::find(need_a.begin(), need_a.end(), t / tb) == need_a.end()) need_a.push_back(t / tb);
     std::sort(need_a.begin(), need_a.end());
     auto babies_of = [&](int a) {
         std::vector<int> bs;
-        for (int t = t0; t < t0 + count; t++)
+        for (int t : ts)
             if (t / tb == a) bs.push_back(t % tb);
         return bs;
     };
-    // from x: every giant a != 0 and the babies of a = 0
+    // from each x: every giant a != 0 and the babies of a = 0
     std::vector<int> xsteps;
     std::vector<std::pair<int, int>> xwhat;  // (a, b) of each x rotation
     for (int a : need_a) {
@@ -263,33 +285,52 @@ std::vector<Ciphertext> BertLayer::token_rotations(const Ciphertext &x, int t0, 
                 }
         }
     }
-    std::vector<Ciphertext> xr;
-    if (!xsteps.empty()) ev.rotate_vector_hoisted(x, xsteps, *ck_.galois_keys, xr);
-    std::vector<Ciphertext> giant(need_a.empty() ? 1 : need_a.back() + 1);
-    for (size_t i = 0; i < xsteps.size(); i++) {
-        const int a = xwhat[i].first, b = xwhat[i].second;
-        if (b)
-            out[tb * a + b - t0] = std::move(xr[i]);
-        else
-            giant[a] = std::move(xr[i]);
-    }
-    for (int a : need_a) {
-        const Ciphertext &base = a ? giant[a] : x;
+    std::vector<std::vector<Ciphertext>> xr;
+    if (!xsteps.empty()) ev.rotate_vector_hoisted_many(xs, xsteps, *ck_.galois_keys, xr);
+    const int amax = need_a.empty() ? 0 : need_a.back();
+    std::vector<std::vector<Ciphertext>> giant(X, std::vector<Ciphertext>(amax + 1));
+    for (size_t x = 0; x < X && !xsteps.empty(); x++)
+        for (size_t i = 0; i < xsteps.size(); i++) {
+            const int a = xwhat[i].first, b = xwhat[i].second;
+            if (b)
+                out[x][pos[tb * a + b]] = std::move(xr[x][i]);
+            else
+                giant[x][a] = std::move(xr[x][i]);
+        }
+    xr.clear();
+    // every giant's babies in one batched hoisted call; the union of their baby
+    // steps, the few outside a giant's offsets rotated and dropped
+    std::vector<const Ciphertext *> gs;
+    std::vector<std::pair<size_t, int>> gwhat;  // (x, a) of each giant
+    std::vector<int> ub;
+    for (size_t x = 0; x < X; x++)
+        for (int a : need_a)
+            if (a) {
+                gs.push_back(&giant[x][a]);
+                gwhat.push_back({x, a});
+                for (int b : babies_of(a))
+                    if (b && std::find(ub.begin(), ub.end(), b) == ub.end()) ub.push_back(b);
+            }
+    std::sort(ub.begin(), ub.end());
+    std::vector<std::vector<Ciphertext>> gr;
+    if (!gs.empty() && !ub.empty()) {
         std::vector<int> bsteps;
-        std::vector<int> bs = babies_of(a);
-        for (int b : bs)
-            if (b && a) bsteps.push_back(kGroup * b);
-        std::vector<Ciphertext> br;
-        if (!bsteps.empty()) ev.rotate_vector_hoisted(base, bsteps, *ck_.galois_keys, br);
-        size_t bi = 0;
-        for (int b : bs) {
-            Ciphertext &o = out[tb * a + b - t0];
-            if (!b)
-                o = a ? std::move(giant[a]) : x;  // the giant itself (its babies are already rotated)
-            else if (a)
-                o = std::move(br[bi++]);
+        for (int b : ub) bsteps.push_back(kGroup * b);
+        ev.rotate_vector_hoisted_many(gs, bsteps, *ck_.galois_keys, gr);
+    }
+    for (size_t k = 0; k < gwhat.size() && !gr.empty(); k++) {
+        const size_t x = gwhat[k].first;
+        const int a = gwhat[k].second;
+        for (int b : babies_of(a)) {
+            if (!b) continue;
+            const size_t j = std::find(ub.begin(), ub.end(), b) - ub.begin();
+            out[x][pos[tb * a + b]] = std::move(gr[k][j]);
         }
     }
+    for (size_t x = 0; x < X; x++)
+        for (int a : need_a)
+            for (int b : babies_of(a))
+                if (!b) out[x][pos[tb * a]] = a ? std::move(giant[x][a]) : *xs[x];  // the giant itself
     return out;
 }
 
@@ -342,35 +383,68 @@ std::vector<Ciphertext> BertLayer::extract_rows(const std::vector<Ciphertext> &x
 
 std::vector<Ciphertext> BertLayer::offset_rotations(const Ciphertext &x, int dmin, int dmax)
 {
-    // rot(x, d) for d in [dmin, dmax], d = 8 a + b
+    std::vector<std::vector<Ciphertext>> r = offset_rotations_many({&x}, dmin, dmax);
+    return std::move(r[0]);
+}
+
+std::vector<std::vector<Ciphertext>> BertLayer::offset_rotations_many(const std::vector<const Ciphertext *> &xs,
+                                                                      int dmin, int dmax)
+{
+    // rot(x, d) for every x and d in [dmin, dmax], d = 8 a + b
     const Evaluator &ev = ck_.evaluator;
-    std::vector<Ciphertext> out(dmax - dmin + 1);
+    const size_t X = xs.size();
+    std::vector<std::vector<Ciphertext>> out(X, std::vector<Ciphertext>(dmax - dmin + 1));
     if (opt_.direct_rotations) {
-        // every offset hoisted from one ModUp of x
+        // every offset hoisted from one ModUp of each x
         std::vector<int> steps;
         for (int d = dmin; d <= dmax; d++)
             if (d) steps.push_back(d);
-        std::vector<Ciphertext> r;
-        if (!steps.empty()) ev.rotate_vector_hoisted(x, steps, *ck_.galois_keys, r);
-        size_t i = 0;
-        for (int d = dmin; d <= dmax; d++) out[d - dmin] = d ? std::move(r[i++]) : x;
+        std::vector<std::vector<Ciphertext>> r;
+        if (!steps.empty()) ev.rotate_vector_hoisted_many(xs, steps, *ck_.galois_keys, r);
+        for (size_t x = 0; x < X; x++) {
+            size_t i = 0;
+            for (int d = dmin; d <= dmax; d++) out[x][d - dmin] = d ? std::move(r[x][i++]) : *xs[x];
+        }
         return out;
     }
+    // babies 1..7 of every x in one batched hoisted call, then the giants of
+    // every x and every baby (8 bases each) in one more: the union of their
+    // giant steps, the few outside a base's offsets rotated and dropped
     std::vector<int> bsteps;
     for (int b = 1; b < kOffBaby; b++) bsteps.push_back(b);
-    std::vector<Ciphertext> br;
-    ev.rotate_vector_hoisted(x, bsteps, *ck_.galois_keys, br);
-    for (int b = 0; b < kOffBaby; b++) {
-        const Ciphertext &base = b ? br[b - 1] : x;
-        std::vector<int> as, gsteps;
+    std::vector<std::vector<Ciphertext>> br;
+    ev.rotate_vector_hoisted_many(xs, bsteps, *ck_.galois_keys, br);
+    auto giants_of = [&](int b) {
+        std::vector<int> as;
         for (int d = dmin; d <= dmax; d++)
             if (((d % kOffBaby) + kOffBaby) % kOffBaby == b) as.push_back(floor_div(d, kOffBaby));
-        for (int a : as)
-            if (a) gsteps.push_back(kOffBaby * a);
-        std::vector<Ciphertext> gr;
-        if (!gsteps.empty()) ev.rotate_vector_hoisted(base, gsteps, *ck_.galois_keys, gr);
-        size_t gi = 0;
-        for (int a : as) out[kOffBaby * a + b - dmin] = a ? std::move(gr[gi++]) : base;
+        return as;
+    };
+    std::vector<int> ua;
+    for (int b = 0; b < kOffBaby; b++)
+        for (int a : giants_of(b))
+            if (a && std::find(ua.begin(), ua.end(), a) == ua.end()) ua.push_back(a);
+    std::sort(ua.begin(), ua.end());
+    std::vector<const Ciphertext *> bases;  // base (x, b) at x kOffBaby + b
+    for (size_t x = 0; x < X; x++)
+        for (int b = 0; b < kOffBaby; b++) bases.push_back(b ? &br[x][b - 1] : xs[x]);
+    std::vector<std::vector<Ciphertext>> gr;
+    if (!ua.empty()) {
+        std::vector<int> gsteps;
+        for (int a : ua) gsteps.push_back(kOffBaby * a);
+        ev.rotate_vector_hoisted_many(bases, gsteps, *ck_.galois_keys, gr);
+    }
+    for (size_t x = 0; x < X; x++) {
+        for (int b = 0; b < kOffBaby; b++)
+            for (int a : giants_of(b)) {
+                if (!a) continue;
+                const size_t j = std::find(ua.begin(), ua.end(), a) - ua.begin();
+                out[x][kOffBaby * a + b - dmin] = std::move(gr[x * kOffBaby + b][j]);
+            }
+        for (int b = 0; b < kOffBaby; b++) {
+            const int d = b;  // giant a = 0: the base itself
+            if (d >= dmin && d <= dmax) out[x][d - dmin] = b ? std::move(br[x][b - 1]) : *xs[x];
+        }
     }
     return out;
 }
@@ -565,35 +639,70 @@ BertLayer::AttnState BertLayer::attention_probs(const std::vector<Ciphertext> &q
     // probabilities of every ciphertext and active half, refreshed together
     std::vector<std::vector<Ciphertext>> &Pe = st.Pe;
     Pe.assign(n, std::vector<Ciphertext>(halves));
+    // q and k of every ciphertext at one level; the token rotations of every k
+    // over every active half's offsets in one batched call
+    std::vector<Ciphertext> qs(n), ks(n);
     for (size_t c = 0; c < n; c++) {
-        Ciphertext qc = q[c], kc = k[c];
-        if (kc.coeff_modulus_size() > qc.coeff_modulus_size()) ev.mod_switch_to_inplace(kc, qc.params_id());
-        if (qc.coeff_modulus_size() > kc.coeff_modulus_size()) ev.mod_switch_to_inplace(qc, kc.params_id());
-        // ---- scores, compacted: S^eta[256 i + 64 h + t'] = q_i . k_(i + 64 eta + t') on head h
-        std::vector<Ciphertext> S(halves);
-        for (int eta : active) {
-            const int t0 = kHeadDim * eta, lo = tlo[eta], hi = thi[eta];
-            std::vector<Ciphertext> kr = token_rotations(kc, t0 + lo, hi - lo);
-            mark("attention.k_rotations");
-            std::vector<Ciphertext> P;  // P[tau - lo] = q (.) rot(k, 256 tau), relinearised and rescaled
-            {
-                std::vector<const Ciphertext *> qa(hi - lo, &qc), kb;
-                for (auto &x : kr) kb.push_back(&x);
-                ev.multiply_relin_rescale_many(qa, kb, rk, P, true);
+        qs[c] = q[c];
+        ks[c] = k[c];
+        if (ks[c].coeff_modulus_size() > qs[c].coeff_modulus_size()) ev.mod_switch_to_inplace(ks[c], qs[c].params_id());
+        if (qs[c].coeff_modulus_size() > ks[c].coeff_modulus_size()) ev.mod_switch_to_inplace(qs[c], ks[c].params_id());
+    }
+    std::vector<int> ts;
+    for (int eta : active)
+        for (int t = kHeadDim * eta + tlo[eta]; t < kHeadDim * eta + thi[eta]; t++) ts.push_back(t);
+    std::vector<std::vector<Ciphertext>> KR;
+    {
+        std::vector<const Ciphertext *> kp;
+        for (auto &x : ks) kp.push_back(&x);
+        KR = token_rotations_many(kp, ts);
+    }
+    ks.clear();
+    mark("attention.k_rotations");
+    // ---- scores, compacted: S^eta[256 i + 64 h + t'] = q_i . k_(i + 64 eta + t') on head h.
+    // Every stage over every (ciphertext, active half) at once: the products in
+    // one batched call, a block mix per (ciphertext, half), the partial rotation
+    // sums Y_a of all of them in one call, their final sums in one more.
+    std::vector<std::vector<Ciphertext>> P(n * halves);  // [c halves + eta][tau - lo]
+    {
+        std::vector<const Ciphertext *> qa, kb;
+        for (size_t c = 0; c < n; c++) {
+            size_t tpos = 0;
+            for (int eta : active) {
+                for (int i = 0; i < thi[eta] - tlo[eta]; i++) {
+                    qa.push_back(&qs[c]);
+                    kb.push_back(&KR[c][tpos + i]);
+                }
+                tpos += thi[eta] - tlo[eta];
             }
-            kr.clear();
-            mark("attention.qk_products");
-            const size_t lp = P[0].coeff_modulus_size();
+        }
+        std::vector<Ciphertext> prod;
+        ev.multiply_relin_rescale_many(qa, kb, rk, prod, true);
+        KR.clear();
+        size_t k = 0;
+        for (size_t c = 0; c < n; c++)
+            for (int eta : active)
+                for (int i = 0; i < thi[eta] - tlo[eta]; i++) P[c * halves + eta].push_back(std::move(prod[k++]));
+    }
+    mark("attention.qk_products");
+    // S = sum_d rot(Z_d, d), Z_d = sum_tau A_(tau + d) (.) P_tau, d = 8 a + b:
+    // Y_a = sum_b rot(Z_(8a+b), b), S = sum_a rot(Y_a, 8 a), each sum of
+    // rotations with a single ModDown
+    std::vector<std::vector<Ciphertext>> Zall(n * halves);
+    std::vector<std::vector<const Ciphertext *>> zgroups;
+    std::vector<std::vector<int>> zsteps, ysteps(n * halves);
+    std::vector<size_t> ygroup;  // (c, eta) of each Y group
+    for (size_t c = 0; c < n; c++)
+        for (int eta : active) {
+            const int lo = tlo[eta], hi = thi[eta];
+            std::vector<Ciphertext> &Pc = P[c * halves + eta];
+            const size_t lp = Pc[0].coeff_modulus_size();
             const double ms = (double)prime(lp);
-            // S = sum_d rot(Z_d, d), Z_d = sum_tau A_(tau + d) (.) P_tau, d = 8 a + b:
-            // Y_a = sum_b rot(Z_(8a+b), b), S = sum_a rot(Y_a, 8 a), each sum of
-            // rotations with a single ModDown
             const int dmin = -(hi - 1), dmax = kHeadDim - 1 - lo;
             // every Z_d in one block mix: each P_tau is read once for all offsets
-            std::vector<Ciphertext> Zall;
             {
                 std::vector<const Ciphertext *> in;
-                for (auto &p : P) in.push_back(&p);
+                for (auto &p : Pc) in.push_back(&p);
                 const int K = hi - lo, M = dmax - dmin + 1;
                 std::vector<int32_t> index((size_t)M * K, -1);
                 for (int m = 0; m < M; m++)
@@ -601,38 +710,59 @@ BertLayer::AttnState BertLayer::attention_probs(const std::vector<Ciphertext> &q
                         const int pidx = lo + k + dmin + m;  // tau + d
                         if (pidx >= 0 && pidx < kHeadDim) index[(size_t)m * K + k] = pidx;
                     }
-                ev.multiply_periodic_mix(in, offset_masks(lp, ms), index, Zall);
+                ev.multiply_periodic_mix(in, offset_masks(lp, ms), index, Zall[c * halves + eta]);
             }
-            P.clear();
-            std::vector<Ciphertext> Ys;
-            std::vector<int> ysteps;
+            Pc.clear();
             for (int a = floor_div(dmin, kOffBaby); a <= floor_div(dmax, kOffBaby); a++) {
                 std::vector<const Ciphertext *> zp;
-                std::vector<int> zsteps;
+                std::vector<int> zs;
                 for (int b = 0; b < kOffBaby; b++) {
                     const int d = kOffBaby * a + b;
                     if (d < dmin || d > dmax) continue;
-                    zp.push_back(&Zall[d - dmin]);
-                    zsteps.push_back(b);
+                    zp.push_back(&Zall[c * halves + eta][d - dmin]);
+                    zs.push_back(b);
                 }
                 if (zp.empty()) continue;
-                Ciphertext Y;
-                ev.rotate_sum(zp, zsteps, gk, Y);
-                Ys.push_back(std::move(Y));
-                ysteps.push_back(kOffBaby * a);
+                zgroups.push_back(std::move(zp));
+                zsteps.push_back(std::move(zs));
+                ysteps[c * halves + eta].push_back(kOffBaby * a);
+                ygroup.push_back(c * halves + eta);
             }
-            std::vector<const Ciphertext *> yp;
-            for (const auto &y : Ys) yp.push_back(&y);
-            ev.rotate_sum(yp, ysteps, gk, S[eta]);
-            ev.rescale_to_next_inplace(S[eta]);
-            mark("attention.compaction");
         }
+    std::vector<Ciphertext> Ys;
+    ev.rotate_sum_many(zgroups, zsteps, gk, Ys);
+    zgroups.clear();
+    Zall.clear();
+    std::vector<std::vector<Ciphertext>> S(n, std::vector<Ciphertext>(halves));
+    {
+        std::vector<std::vector<const Ciphertext *>> ygroups(n * halves);
+        for (size_t g = 0; g < Ys.size(); g++) ygroups[ygroup[g]].push_back(&Ys[g]);
+        std::vector<std::vector<const Ciphertext *>> gp;
+        std::vector<std::vector<int>> gs;
+        std::vector<size_t> which;
+        for (size_t ce = 0; ce < n * halves; ce++)
+            if (!ygroups[ce].empty()) {
+                gp.push_back(ygroups[ce]);
+                gs.push_back(ysteps[ce]);
+                which.push_back(ce);
+            }
+        std::vector<Ciphertext> sums;
+        ev.rotate_sum_many(gp, gs, gk, sums);
+        std::vector<const Ciphertext *> sp;
+        for (auto &x : sums) sp.push_back(&x);
+        std::vector<Ciphertext> resc;
+        ev.rescale_to_next_many(sp, resc);
+        for (size_t k = 0; k < which.size(); k++) S[which[k] / halves][which[k] % halves] = std::move(resc[k]);
+    }
+    Ys.clear();
+    mark("attention.compaction");
+    for (size_t c = 0; c < n; c++) {
         // ---- softmax over t for every (query, head): exp, key mask, sum, 1 / sum
         std::vector<Ciphertext> E(halves);
         Ciphertext tot;
         bool have_tot = false;
         for (int eta : active) {
-            E[eta] = ck_.exp(S[eta]);
+            E[eta] = ck_.exp(S[c][eta]);
             const int full = std::min(kHeadDim, T - kHeadDim * eta);
             ev.multiply_plain_inplace(E[eta], key_mask(eta, full, seg, E[eta].coeff_modulus_size()));
             ev.rescale_to_next_inplace(E[eta]);
@@ -684,23 +814,51 @@ std::vector<Ciphertext> BertLayer::attention_values(AttnState &st, const std::ve
     const std::vector<int> &tlo = st.tlo, &thi = st.thi, &active = st.active;
     std::vector<std::vector<Ciphertext>> &Pe = st.Pe;
     // ---- expansion over the 64 features of each head; times the rotated v
+    // the token rotations of every v (at one level below the probabilities)
+    // over every active half's offsets in one batched call
+    std::vector<int> ts;
+    for (int eta : active)
+        for (int t = kHeadDim * eta + tlo[eta]; t < kHeadDim * eta + thi[eta]; t++) ts.push_back(t);
+    std::vector<std::vector<Ciphertext>> VR;
+    if (!active.empty()) {
+        const size_t lb = Pe[0][active[0]].coeff_modulus_size();
+        std::vector<Ciphertext> vs(n);
+        for (size_t c = 0; c < n; c++) {
+            if (Pe[c][active[0]].coeff_modulus_size() != lb)
+                throw std::logic_error("BertLayer: probabilities at different levels");
+            vs[c] = v[c];
+            if (vs[c].coeff_modulus_size() > lb - 1) ev.mod_switch_to_inplace(vs[c], chain_of(lb - 1));
+        }
+        std::vector<const Ciphertext *> vp;
+        for (auto &x : vs) vp.push_back(&x);
+        VR = token_rotations_many(vp, ts);
+    }
+    mark("attention.v_rotations");
+    // the offset rotations of every ciphertext's probabilities, per half in one batched call
+    std::vector<std::vector<std::vector<Ciphertext>>> PR(st.halves);
+    for (int eta : active) {
+        std::vector<const Ciphertext *> pp;
+        for (size_t c = 0; c < n; c++) pp.push_back(&Pe[c][eta]);
+        PR[eta] = offset_rotations_many(pp, tlo[eta] - (kHeadDim - 1), thi[eta] - 1);
+    }
+    mark("attention.p_offsets");
     std::vector<Ciphertext> out(n);
     for (size_t c = 0; c < n; c++) {
         Ciphertext acc;
         bool have_acc = false;
+        size_t tpos = 0;  // this half's first offset in ts
         for (int eta : active) {
             const int t0 = kHeadDim * eta, lo = tlo[eta], hi = thi[eta];
             const Ciphertext &Pb = Pe[c][eta];
             // Pb_tau[256 i + 64 h + f] = P[256 i + 64 h + tau] = sum_d A_(tau - d) (.) rot(P, d)
             const int dlo = lo - (kHeadDim - 1);
-            std::vector<Ciphertext> R = offset_rotations(Pb, dlo, hi - 1);
-            mark("attention.p_offsets");
+            std::vector<Ciphertext> R = std::move(PR[eta][c]);
             const size_t lb = Pb.coeff_modulus_size();
             const double ms = (double)prime(lb);
-            Ciphertext vlow = v[c];
-            if (vlow.coeff_modulus_size() > lb - 1) ev.mod_switch_to_inplace(vlow, chain_of(lb - 1));
-            std::vector<Ciphertext> vr = token_rotations(vlow, t0 + lo, hi - lo);
-            mark("attention.v_rotations");
+            std::vector<Ciphertext> vr(hi - lo);
+            for (int i = 0; i < hi - lo; i++) vr[i] = std::move(VR[c][tpos + i]);
+            tpos += hi - lo;
+            (void)t0;
             // every Pb_tau in one block mix over the rotations R_d
             std::vector<Ciphertext> pbs;
             {
