@@ -541,7 +541,12 @@ def main():
     ap.add_argument("--no-served", action="store_true", help="skip the serving measurement")
     ap.add_argument("--lockstep-steps", type=int, default=2,
                     help="steps of the cross-set lockstep mode (every set of a step in one forward_sets call)")
-    ap.add_argument("--requests", type=int, default=100, help="served BERT requests (p50/p99 samples)")
+    ap.add_argument("--requests", type=int, default=100, help="served BERT requests of the latency pass "
+                                                                 "(p50/p99 samples)")
+    ap.add_argument("--capacity-requests", type=int, default=48,
+                    help="served BERT requests queued at once for the capacity pass")
+    ap.add_argument("--load", type=float, default=0.8,
+                    help="arrival rate of the served latency pass as a fraction of the measured capacity")
     ap.add_argument("--served-plain-requests", type=int, default=0,
                     help="also serve the first k requests through the plain per-call drop-in")
     ap.add_argument("--nonlinear-requests", type=int, default=12, help="requests of the served nonlinear extra")
@@ -823,44 +828,52 @@ def main():
 
 
 def served_bert(nx, B, args, plain_rate, batched_rate):
-    """Poisson arrivals of single-sequence BERT-layer requests, lengths
-    ~U{16..128}, at 1.25x the batched path's sequence rate (requests queue up):
-    the served path (continuous batching of queued requests into 128-token
-    sets) over --requests requests, and optionally (--served-plain-requests)
-    the plain per-call drop-in serving the first of them (one lane, per-op
-    mode, one call per request) at the same arrival times."""
+    """Single-sequence BERT-layer requests, lengths ~U{16..128}, served with
+    continuous batching (queued requests packed into 128-token sets), in two
+    passes: (1) capacity -- --capacity-requests requests all queued at once,
+    the sustained sequence-layers/s; (2) latency -- --requests requests with
+    Poisson arrivals at --load times that capacity, p50/p99 from submission to
+    completion.  Optionally (--served-plain-requests) the plain per-call drop-in
+    serves the first requests of the latency pass (one lane, per-op mode, one
+    call per request) at the same arrival times."""
     rng = np.random.default_rng(77)
     n = args.requests
-    lens = [int(x) for x in rng.integers(16, BERT_TOKENS + 1, n)]
+    n_cap = args.capacity_requests
+    count_all = max(n, n_cap)
+    lens = [int(x) for x in rng.integers(16, BERT_TOKENS + 1, count_all)]
     reqs = []
     for L_ in lens:
         rows = np.zeros((BERT_TOKENS, 768))
         rows[:L_] = rng.normal(0, 1, (L_, 768))
         reqs.append((L_, bert_encrypt(nx, B, rows)))
-    rate = 1.25 * batched_rate
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
-    arrivals -= arrivals[0]
     B["ctx"].synchronize()
     import threading
 
-    def run(count, lanes, pack, per_op):
+    def run(count, lanes, pack, per_op, arrivals):
         B["ev"].set_per_op(per_op)
         srv = nx.Server(B["ev"], lanes=lanes, bert=B["layer"], pack=pack, memory_budget=96 << 30)
         srv.set_bert_concurrency(lanes)
         B["ev"].set_per_op(False)
         ids = [None] * count
+        if arrivals is None:  # everything queued before the server admits anything
+            srv.pause(True)
         t0 = time.perf_counter()
 
         def submitter():
             for i in range(count):
                 L_, cts = reqs[i]
-                delay = arrivals[i] - (time.perf_counter() - t0)
-                if delay > 0:
-                    time.sleep(delay)
+                if arrivals is not None:
+                    delay = arrivals[i] - (time.perf_counter() - t0)
+                    if delay > 0:
+                        time.sleep(delay)
                 ids[i] = srv.submit_bert(cts, L_)
 
         th = threading.Thread(target=submitter)
         th.start()
+        if arrivals is None:
+            th.join()
+            t0 = time.perf_counter()
+            srv.pause(False)
         done = 0
         for i in range(count):  # progress from the work itself
             while ids[i] is None:
@@ -877,27 +890,39 @@ def served_bert(nx, B, args, plain_rate, batched_rate):
         del srv
         return span, st
 
-    log("served BERT: %d requests, Poisson arrivals at %.3f/s" % (n, rate))
-    spanb, stb = run(n, args.bert_lanes, True, False)
+    log("served BERT capacity: %d requests queued at once" % n_cap)
+    span_c, st_c = run(n_cap, args.bert_lanes, True, False, None)
+    capacity = n_cap / span_c
+    log("served capacity (%d lanes, packing): %.1f s, %.3f sequence-layers/s, %d packed calls carrying %d requests"
+        % (args.bert_lanes, span_c, capacity, st_c["packed_calls"], st_c["packed_requests"]))
+    rate = args.load * capacity
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, n))
+    arrivals -= arrivals[0]
+    log("served BERT latency: %d requests, Poisson arrivals at %.3f/s (%.2f of capacity)" % (n, rate, args.load))
+    spanb, stb = run(n, args.bert_lanes, True, False, arrivals)
     log("served (%d lanes, packing): %.1f s, p50 %.0f ms, p99 %.0f ms, %d packed calls carrying %d requests"
         % (args.bert_lanes, spanb, stb["p50_ms"], stb["p99_ms"], stb["packed_calls"], stb["packed_requests"]))
     res = {
-        "workload": "single-sequence BERT-base layer requests, lengths ~U{16..128}, Poisson arrivals at %.3f "
-                    "requests/s (1.25x the batched path's rate); served = %d lanes, continuous batching of queued "
-                    "requests into 128-token sets" % (rate, args.bert_lanes),
-        "requests": n, "lanes": args.bert_lanes, "mean_length": float(np.mean(lens)),
-        "served": {"layers_per_s": n / spanb, "p50_ms": stb["p50_ms"], "p99_ms": stb["p99_ms"],
+        "workload": "single-sequence BERT-base layer requests, lengths ~U{16..128}; served = %d lanes, continuous "
+                    "batching of queued requests into 128-token sets; capacity: %d requests queued at once; "
+                    "latency: %d requests with Poisson arrivals at %.3f requests/s (%.2f of capacity)"
+                    % (args.bert_lanes, n_cap, n, rate, args.load),
+        "requests": n, "lanes": args.bert_lanes, "mean_length": float(np.mean(lens[:n])),
+        "capacity": {"layers_per_s": capacity, "requests": n_cap, "makespan_s": span_c,
+                     "packed_calls": st_c["packed_calls"], "packed_requests": st_c["packed_requests"]},
+        "served": {"layers_per_s": n / spanb, "arrival_rate": rate, "p50_ms": stb["p50_ms"], "p99_ms": stb["p99_ms"],
                    "mean_ms": stb["mean_ms"], "makespan_s": spanb, "packed_calls": stb["packed_calls"],
                    "packed_requests": stb["packed_requests"], "peak_concurrent": stb["peak_concurrent"],
                    "latency_samples": stb["completed"]},
         "plain_per_call_rate": plain_rate,
-        "throughput_ratio": (n / spanb) / plain_rate,
-        "throughput_ratio_definition": "served sequence-layers/s over the plain per-call drop-in's "
-                                       "sequence-layers/s (one call at a time, per-op mode)",
+        "throughput_ratio": capacity / plain_rate,
+        "throughput_ratio_definition": "served capacity (sequence-layers/s with continuous batching, requests "
+                                       "queued) over the plain per-call drop-in's sequence-layers/s (one call at "
+                                       "a time, per-op mode)",
     }
     k = min(args.served_plain_requests, n)
     if k:
-        span1, st1 = run(k, 1, False, True)
+        span1, st1 = run(k, 1, False, True, arrivals)
         log("plain per-call served (%d requests): %.1f s, p50 %.0f ms, p99 %.0f ms"
             % (k, span1, st1["p50_ms"], st1["p99_ms"]))
         res["plain_per_call_served"] = {"requests": k, "layers_per_s": k / span1, "p50_ms": st1["p50_ms"],

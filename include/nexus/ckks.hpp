@@ -350,6 +350,10 @@ public:
     Ciphertext invert_sqrt(const Ciphertext &x, int d_newt = 5, int d_gold = 2);
     Ciphertext exp(const Ciphertext &x);
     Ciphertext inverse(const Ciphertext &x, int iter = 4);
+    // exp / inverse over inputs of one level and scale in lockstep (every
+    // multiplication one batched call); word for word the single-input results
+    std::vector<Ciphertext> exp_many(const std::vector<const Ciphertext *> &x);
+    std::vector<Ciphertext> inverse_many(const std::vector<const Ciphertext *> &x, int iter = 4);
     void eval_odd_deg9_poly(const std::vector<double> &a, const Ciphertext &x, Ciphertext &dest);
     // the same steps over a batch of inputs of one level and scale in lockstep
     // (every multiplication with its relinearisation and rescale one batched

@@ -480,6 +480,78 @@ Ciphertext CKKSEvaluator::exp(const Ciphertext &x_in)
     return x;
 }
 
+// exp() over inputs of one level and scale in lockstep: every squaring of all
+// inputs one batched multiply-relinearise-rescale; word for word exp()
+std::vector<Ciphertext> CKKSEvaluator::exp_many(const std::vector<const Ciphertext *> &xs)
+{
+    const Evaluator &ev = evaluator;
+    std::vector<Ciphertext> x(xs.size());
+    if (xs.empty()) return x;
+    for (size_t k = 0; k < xs.size(); k++) {
+        if (xs[k]->params_id() != xs[0]->params_id() || xs[k]->scale() != xs[0]->scale())
+            throw std::invalid_argument("exp_many: inputs must share the level and scale");
+        x[k] = *xs[k];
+    }
+    Plaintext one, inv128;
+    encoder.encode(0.0078125, x[0].params_id(), x[0].scale(), inv128);
+    for (auto &c : x) ev.multiply_plain_inplace(c, inv128);
+    std::vector<const Ciphertext *> p;
+    for (auto &c : x) p.push_back(&c);
+    std::vector<Ciphertext> y;
+    ev.rescale_to_next_many(p, y);
+    encoder.encode(1.0, y[0].params_id(), y[0].scale(), one);
+    for (auto &c : y) ev.add_plain_inplace(c, one);
+    for (int i = 0; i < 7; i++) {
+        std::vector<const Ciphertext *> q;
+        for (auto &c : y) q.push_back(&c);
+        std::vector<Ciphertext> z;
+        ev.multiply_relin_rescale_many(q, q, *relin_keys, z);
+        y = std::move(z);
+    }
+    return y;
+}
+
+// inverse() over inputs of one level and scale in lockstep; word for word inverse()
+std::vector<Ciphertext> CKKSEvaluator::inverse_many(const std::vector<const Ciphertext *> &xs, int iter)
+{
+    const Evaluator &ev = evaluator;
+    const size_t K = xs.size();
+    std::vector<Ciphertext> y(K), res(K);
+    if (!K) return res;
+    for (size_t k = 0; k < K; k++)
+        if (xs[k]->params_id() != xs[0]->params_id() || xs[k]->scale() != xs[0]->scale())
+            throw std::invalid_argument("inverse_many: inputs must share the level and scale");
+    Plaintext one;
+    encoder.encode(1.0, xs[0]->params_id(), xs[0]->scale(), one);
+    for (size_t k = 0; k < K; k++) {
+        ev.sub_plain(*xs[k], one, y[k]);
+        ev.negate_inplace(y[k]);
+        ev.add_plain(y[k], one, res[k]);
+    }
+    for (int i = 0; i < iter; i++) {
+        std::vector<const Ciphertext *> yp;
+        for (auto &c : y) yp.push_back(&c);
+        std::vector<Ciphertext> y2;
+        ev.multiply_relin_rescale_many(yp, yp, *relin_keys, y2);
+        y = std::move(y2);
+        encoder.encode(1.0, y[0].params_id(), y[0].scale(), one);
+        std::vector<Ciphertext> tmp(K);
+        for (size_t k = 0; k < K; k++) {
+            ev.add_plain(y[k], one, tmp[k]);
+            ev.mod_switch_to_inplace(res[k], tmp[k].params_id());
+        }
+        std::vector<const Ciphertext *> rp, tp;
+        for (size_t k = 0; k < K; k++) {
+            rp.push_back(&res[k]);
+            tp.push_back(&tmp[k]);
+        }
+        std::vector<Ciphertext> r2;
+        ev.multiply_relin_rescale_many(rp, tp, *relin_keys, r2);
+        res = std::move(r2);
+    }
+    return res;
+}
+
 double CKKSEvaluator::calculate_MAE(const std::vector<double> &y_true, const Ciphertext &ct, size_t n)
 {
     Plaintext p;
