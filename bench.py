@@ -640,6 +640,15 @@ def main():
     layer.forward(enc_sets[0], segments=set_segs[0])
     ctx.synchronize()
     ranking = sorted(((nx.profile_read(c)[1], c) for c in nx.profile_classes()), reverse=True)
+    # every top class's rates over that one call (events around each launch)
+    class_rates = []
+    for t, c in ranking[:10]:
+        nl, ms_c, by = nx.profile_read(c)
+        fl = nx.profile_read_flops(c)
+        class_rates.append({"class": c, "ms_per_call": ms_c, "launches": nl,
+                            "algorithmic_gbs": by / (ms_c * 1e-3) / 1e9 if ms_c else 0.0,
+                            "hbm_frac": by / (ms_c * 1e-3) / 1e9 / HBM_PEAK_GBS if ms_c else 0.0,
+                            "fp64_tflops": fl / (ms_c * 1e-3) / 1e12 if ms_c and fl else None})
     nx.profile_select("")
     dominant = ranking[0][1]
     log("kernel time ranking of one call (ms): " + ", ".join("%s=%.0f" % (c, t) for t, c in ranking[:10]))
@@ -686,6 +695,7 @@ def main():
     # step over all sets, odd ciphertexts paired across sets); latency = the step
     lock = None
     if args.lockstep_steps > 0:
+        ctx.trim()  # the lockstep working set differs from the per-set calls'
         layer.forward_sets(enc_sets[:2], set_segs[:2])  # warm the lockstep working set
         ctx.synchronize()
         tl = time.perf_counter()
@@ -814,9 +824,11 @@ def main():
             "launches": launches,
             "avg_launch_us": avg_launch_ms * 1e3,
             "kernel_share_of_step": k_ms / (elapsed * 1e3),
+            "top_classes_one_call": class_rates,
         },
     }
     if not args.no_served:
+        ctx.trim()  # the serving lane starts on a device without the lockstep phase's cached blocks
         res["served"] = served_bert(nx, B, args, len(lens) / plain_s, len(lens) * 1000.0 / ms_per_step)
     del enc_sets, enc_single, outs, layer, B, ctx
     if not args.no_extra:

@@ -93,6 +93,12 @@ int nx_context_bind_lane(nx_context *ctx, size_t lane);
 int nx_galois_elt_from_step(const nx_context *ctx, int step, uint32_t *elt);
 /* Device memory held by the context's cache: in use / cached, bytes. */
 int nx_context_memory(const nx_context *ctx, size_t *in_use, size_t *cached);
+/* Release the cached (free) device blocks of every lane of the context, after
+ * a device synchronisation: between workload phases whose working sets differ,
+ * so that the next phase does not meet a device filled by the last one's
+ * cache.  No counterpart in the reference (its allocator is
+ * cuda/thirdparty/phantom-fhe/include/util/cuda_wrapper.cuh). */
+int nx_context_trim(nx_context *ctx);
 
 /* ---- ciphertexts / plaintexts (value types with deep copy; reference
  * PhantomCiphertext / seal::Ciphertext) */

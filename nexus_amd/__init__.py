@@ -61,6 +61,7 @@ _sig("nx_context_moduli", [vp, u64p])
 _sig("nx_context_synchronize", [vp])
 _sig("nx_galois_elt_from_step", [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_uint32)])
 _sig("nx_context_memory", [vp, ctypes.POINTER(sz), ctypes.POINTER(sz)])
+_sig("nx_context_trim", [vp])
 for _n in ("ciphertext", "plaintext"):
     _sig("nx_%s_create" % _n, [vp, ctypes.POINTER(vp)])
     _sig("nx_%s_destroy" % _n, [vp])
@@ -190,6 +191,10 @@ class Context:
         a, b = sz(), sz()
         _check(_lib.nx_context_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def trim(self):
+        """release the cached device blocks of every lane (after a device synchronisation)"""
+        _check(_lib.nx_context_trim(self.h))
 
 
 class Ciphertext:

@@ -193,6 +193,14 @@ int nx_context_memory(const nx_context *ctx, size_t *in_use, size_t *cached)
     });
 }
 
+int nx_context_trim(nx_context *ctx)
+{
+    return guard([&] {
+        need(ctx, "ctx");
+        ctx->impl.trim_caches();
+    });
+}
+
 // ---------------------------------------------------------------- ciphertexts
 int nx_ciphertext_create(nx_context *ctx, nx_ciphertext **out)
 {

@@ -264,6 +264,36 @@ __device__ __forceinline__ void epi_store_f(const NttEpilogue &e, uint32_t p, ui
     *o = (uint64_t)(r < 0 ? r + q : r);
 }
 
+// The butterflies of one forward radix-2^K group in registers: local stages
+// s .. s+K-1 of block `blockid`, twiddle row mrow (1 for columns).
+template <int K, bool LAZY>
+__device__ __forceinline__ void fwd_group_f(double *v, int s, uint32_t blockid, uint32_t mrow, const double *tw,
+                                            double q, double qinv)
+{
+#pragma unroll
+    for (int d = 0; d < K; d++) {
+        const int h = 1 << (K - 1 - d);
+#pragma unroll
+        for (int j = 0; j < (1 << d); j++) {
+            const uint32_t b = blockid * (1u << d) + j;
+            const double w = tw[((uint32_t)1 << (s + d)) * mrow + b];
+#pragma unroll
+            for (int u = 0; u < h; u++) {
+                const int t = j * 2 * h + u;
+                const double x = v[t];
+                const double y = mulmod_f(v[t + h], w, q, qinv);
+                if (LAZY) {
+                    v[t] = x + y;
+                    v[t + h] = x - y;
+                } else {
+                    v[t] = reduce_f(x + y, q, qinv);
+                    v[t + h] = reduce_f(x - y, q, qinv);
+                }
+            }
+        }
+    }
+}
+
 // Forward radix-2^K step in double precision (values |v| <= q/2 (1 + eps)).
 // LAZY: the sums and differences are left unreduced (see ntt_tile_f).
 template <int LOGS, int ROLE, int K, int CW, bool LAZY, int NT = kThreads>
@@ -295,28 +325,7 @@ __device__ __forceinline__ void fwd_step_f(double *lds, int s, const double *tw,
             addr[t] = lds_addr<ROLE, G::S, G::C>(cc, blockid * B + o + t * D);
             v[t] = lds[addr[t]];
         }
-#pragma unroll
-        for (int d = 0; d < K; d++) {
-            const int h = 1 << (K - 1 - d);
-#pragma unroll
-            for (int j = 0; j < (1 << d); j++) {
-                const uint32_t b = (uint32_t)blockid * (1u << d) + j;
-                const double w = tw[((uint32_t)1 << (s + d)) * mrow + b];
-#pragma unroll
-                for (int u = 0; u < h; u++) {
-                    const int t = j * 2 * h + u;
-                    const double x = v[t];
-                    const double y = mulmod_f(v[t + h], w, q, qinv);
-                    if (LAZY) {
-                        v[t] = x + y;
-                        v[t + h] = x - y;
-                    } else {
-                        v[t] = reduce_f(x + y, q, qinv);
-                        v[t + h] = reduce_f(x - y, q, qinv);
-                    }
-                }
-            }
-        }
+        fwd_group_f<K, LAZY>(v, s, (uint32_t)blockid, mrow, tw, q, qinv);
 #pragma unroll
         for (int t = 0; t < R; t++) lds[addr[t]] = v[t];
     }
@@ -654,6 +663,62 @@ __global__ __launch_bounds__(kThreads) void k_ntt(NttKArgs a)
 // products are the exact double-precision ones of the unfused conversion
 // (k_modup_conv_a), and the column stages those of the double-precision pass,
 // so the stored words equal the unfused conversion followed by the column pass.
+// The column stages of one tile whose values v (this thread's positions
+// e = threadIdx.x + NT t, reduced) are in registers, stored column-transformed
+// into out (positions gidx).  For 2^8-row columns, 8-column tiles and 256
+// threads each thread's 8 positions are exactly one radix-8 group of the first
+// three stages, so those run in registers before the values go to LDS, and
+// the last two stages run in registers on the way out: three LDS passes and
+// three barriers instead of eight passes and four.  The words are the same.
+template <int LOGS, int CW, int NT, int PER, bool LAZY>
+__device__ __forceinline__ void col_stages_store(double *v, double *lds, const double *tw, double q, double qinv,
+                                                 uint32_t n, uint32_t N1, uint32_t N2, uint32_t col0, uint64_t *out,
+                                                 const uint32_t *gidx)
+{
+    if constexpr (LOGS == 8 && CW == 8 && NT == 256 && PER == 8) {
+        fwd_group_f<3, LAZY>(v, 0, 0, 1, tw, q, qinv);  // rows gg + 32 t of column threadIdx.x % 8
+#pragma unroll
+        for (int t = 0; t < PER; t++) lds[threadIdx.x + t * NT] = v[t];
+        __syncthreads();
+        fwd_step_f<LOGS, kCol, 3, CW, LAZY, NT>(lds, 3, tw, q, qinv, 1, col0);
+        __syncthreads();
+        double w[2][4];
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int g = threadIdx.x + k * NT, cc = g % CW, gg = g / CW;
+#pragma unroll
+            for (int t = 0; t < 4; t++) w[k][t] = lds[(4 * gg + t) * CW + cc];
+        }
+        __syncthreads();  // every read of this tile done before the next one is written
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int g = threadIdx.x + k * NT, cc = g % CW, gg = g / CW;
+            fwd_group_f<2, LAZY>(w[k], 6, (uint32_t)gg, 1, tw, q, qinv);
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                double x = w[k][t];
+                if (LAZY) x = reduce_f(x, q, qinv);
+                x = x < 0 ? x + q : x;
+                out[(uint64_t)(4 * gg + t) * N2 + col0 + cc] = (uint64_t)x;
+            }
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < PER; t++) lds[threadIdx.x + t * NT] = v[t];
+        __syncthreads();
+        stages_f<LOGS, kCol, true, CW, LAZY, NT>(lds, tw, q, qinv, n, N1, col0);
+#pragma unroll
+        for (int t = 0; t < PER; t++) {
+            double x = lds[threadIdx.x + t * NT];
+            if (LAZY) x = reduce_f(x, q, qinv);
+            x = x < 0 ? x + q : x;
+            out[gidx[t]] = (uint64_t)x;
+        }
+        // each thread rewrites only the LDS words it has just read: the next
+        // tile's stages start after the barrier that follows those writes
+    }
+}
+
 struct ModUpColArgs {
     uint64_t *T;
     const uint64_t *coef;
@@ -716,6 +781,7 @@ __global__ __launch_bounds__(NT) void k_modup_col(ModUpColArgs a)
         double f[A];
 #pragma unroll
         for (int i = 0; i < A; i++) f[i] = fd[(uint64_t)i * a.key_limbs];
+        double cv[PER];
 #pragma unroll
         for (int t = 0; t < PER; t++) {
             double acc = 0.0;
@@ -724,25 +790,14 @@ __global__ __launch_bounds__(NT) void k_modup_col(ModUpColArgs a)
                 acc += mulmod_f(y[i][t], f[i], q, qinv);
                 if (i & 1) acc = reduce_f(acc, q, qinv);
             }
-            lds[threadIdx.x + t * NT] = reduce_f(acc, q, qinv);  // |v| <= q/2 (1 + eps)
+            cv[t] = reduce_f(acc, q, qinv);  // |v| <= q/2 (1 + eps)
         }
-        __syncthreads();
         const double *tw = a.twd + (uint64_t)kp * n;
-        const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
-        if (lazy)
-            stages_f<LOGS, kCol, true, CW, true, NT>(lds, tw, q, qinv, n, N1, col0);
-        else
-            stages_f<LOGS, kCol, true, CW, false, NT>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.T + (((uint64_t)b * a.nout + I) * a.digits + j) * n;
-#pragma unroll
-        for (int t = 0; t < PER; t++) {
-            double v = lds[threadIdx.x + t * NT];
-            if (lazy) v = reduce_f(v, q, qinv);
-            v = v < 0 ? v + q : v;
-            out[gidx[t]] = (uint64_t)v;
-        }
-        // each thread rewrites only the LDS words it has just read: the next
-        // output's stages start after the barrier that follows those writes
+        if (q * (0.5 + 1.25 * LOGS) <= 0x1p50)  // uniform
+            col_stages_store<LOGS, CW, NT, PER, true>(cv, lds, tw, q, qinv, n, N1, N2, col0, out, gidx);
+        else
+            col_stages_store<LOGS, CW, NT, PER, false>(cv, lds, tw, q, qinv, n, N1, N2, col0, out, gidx);
     }
 }
 
@@ -802,23 +857,15 @@ __global__ __launch_bounds__(kThreads) void k_center_col(CenterColArgs a)
     }
     for (uint32_t i = i0; i < i1; i++) {
         const double q = a.qd[i].x, qinv = a.qd[i].y;
+        double cv[PER];
 #pragma unroll
-        for (int t = 0; t < PER; t++) lds[threadIdx.x + t * kThreads] = reduce_f(v[t], q, qinv);
-        __syncthreads();
+        for (int t = 0; t < PER; t++) cv[t] = reduce_f(v[t], q, qinv);
         const double *tw = a.twd + (uint64_t)i * n;
-        const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
-        if (lazy)
-            stages_f<LOGS, kCol, true, 8, true>(lds, tw, q, qinv, n, N1, col0);
-        else
-            stages_f<LOGS, kCol, true, 8, false>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.z + ((uint64_t)p * a.limbs + i) * n;
-#pragma unroll
-        for (int t = 0; t < PER; t++) {
-            double y = lds[threadIdx.x + t * kThreads];
-            if (lazy) y = reduce_f(y, q, qinv);
-            y = y < 0 ? y + q : y;
-            out[gidx[t]] = (uint64_t)y;
-        }
+        if (q * (0.5 + 1.25 * LOGS) <= 0x1p50)  // uniform
+            col_stages_store<LOGS, 8, kThreads, PER, true>(cv, lds, tw, q, qinv, n, N1, N2, col0, out, gidx);
+        else
+            col_stages_store<LOGS, 8, kThreads, PER, false>(cv, lds, tw, q, qinv, n, N1, N2, col0, out, gidx);
     }
 }
 
@@ -879,6 +926,7 @@ __global__ __launch_bounds__(NT) void k_moddown_col(ModDownColArgs a)
         double f[A + 1];
 #pragma unroll
         for (int k = 0; k <= A; k++) f[k] = a.convd[(uint64_t)k * a.L + i];
+        double cv[PER];
 #pragma unroll
         for (int t = 0; t < PER; t++) {
             // - r (P mod q_i) as r (q_i - P mod q_i)
@@ -888,23 +936,14 @@ __global__ __launch_bounds__(NT) void k_moddown_col(ModDownColArgs a)
                 s += mulmod_f(v[k][t], f[k], q, qinv);
                 if (k & 1) s = reduce_f(s, q, qinv);
             }
-            lds[threadIdx.x + t * NT] = reduce_f(s, q, qinv);  // |v| <= q/2 (1 + eps)
+            cv[t] = reduce_f(s, q, qinv);  // |v| <= q/2 (1 + eps)
         }
-        __syncthreads();
         const double *tw = a.twd + (uint64_t)i * n;
-        const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform
-        if (lazy)
-            stages_f<LOGS, kCol, true, CW, true, NT>(lds, tw, q, qinv, n, N1, col0);
-        else
-            stages_f<LOGS, kCol, true, CW, false, NT>(lds, tw, q, qinv, n, N1, col0);
         uint64_t *out = a.z + ((uint64_t)p * a.lo + i) * n;
-#pragma unroll
-        for (int t = 0; t < PER; t++) {
-            double x = lds[threadIdx.x + t * NT];
-            if (lazy) x = reduce_f(x, q, qinv);
-            x = x < 0 ? x + q : x;
-            out[gidx[t]] = (uint64_t)x;
-        }
+        if (q * (0.5 + 1.25 * LOGS) <= 0x1p50)  // uniform
+            col_stages_store<LOGS, CW, NT, PER, true>(cv, lds, tw, q, qinv, n, N1, N2, col0, out, gidx);
+        else
+            col_stages_store<LOGS, CW, NT, PER, false>(cv, lds, tw, q, qinv, n, N1, N2, col0, out, gidx);
     }
 }
 
