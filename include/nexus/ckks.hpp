@@ -256,6 +256,9 @@ public:
     void rescale_to_next_many(const std::vector<const Ciphertext *> &cts, std::vector<Ciphertext> &dest) const;
     void multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
                                      const RelinKeys &rk, std::vector<Ciphertext> &dest, bool fused = false) const;
+    // dest = sum_i a[i] * b[i] (size 3): word for word multiply and add per pair
+    void multiply_sum(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                      Ciphertext &dest) const;
     // out[m] = sum_k w[index[m K + k]] (.) cts[k] (index -1: no term), M = index.size() / K
     void multiply_periodic_mix(const std::vector<const Ciphertext *> &cts, const PeriodicPlaintexts &w,
                                const std::vector<int32_t> &index, std::vector<Ciphertext> &out) const;
@@ -350,8 +353,8 @@ public:
     Ciphertext invert_sqrt(const Ciphertext &x, int d_newt = 5, int d_gold = 2);
     Ciphertext exp(const Ciphertext &x);
     Ciphertext inverse(const Ciphertext &x, int iter = 4);
-    // exp / inverse over inputs of one level and scale in lockstep (every
-    // multiplication one batched call); word for word the single-input results
+    // exp / inverse over inputs of one level and scale in lockstep: the
+    // single-input operation sequence, every multiplication one batched call
     std::vector<Ciphertext> exp_many(const std::vector<const Ciphertext *> &x);
     std::vector<Ciphertext> inverse_many(const std::vector<const Ciphertext *> &x, int iter = 4);
     void eval_odd_deg9_poly(const std::vector<double> &a, const Ciphertext &x, Ciphertext &dest);

@@ -173,6 +173,11 @@ int nx_rescale_to_next(const nx_ciphertext *a, nx_ciphertext *dest);
 int nx_rescale_to_next_many(const nx_ciphertext *const *cts, size_t count, nx_ciphertext *const *dests);
 int nx_multiply_relin_rescale_many(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count,
                                    const nx_kswitch_keys *relin_keys, nx_ciphertext *const *dests);
+/* dest = sum_i a[i] * b[i] as one size-3 ciphertext: word for word multiply and
+ * add of every pair (the reference's loop of evaluator.multiply + add,
+ * e.g. cuda/src/matrix_mul.cu:202-275), with no tensor written out.  Pairs of
+ * one level and product scale, size 2, primes of at most 60 bits. */
+int nx_multiply_sum(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count, nx_ciphertext *dest);
 /* The same products with relinearisation and rescale in one rounding (no
  * counterpart in the reference): with hybrid key switching the key-switch
  * accumulator plus P (c0, c1) is divided by P q_(l-1) in one centred

@@ -96,6 +96,7 @@ for _n in ("add_plain", "sub_plain", "multiply_plain"):
     _sig("nx_" + _n, [vp, vp, vp])
 for _n in ("negate", "square", "rescale_to_next", "mod_switch_to_next", "transform_to_ntt", "transform_from_ntt"):
     _sig("nx_" + _n, [vp, vp])
+_sig("nx_multiply_sum", [ctypes.POINTER(vp), ctypes.POINTER(vp), sz, vp])
 _sig("nx_relinearize", [vp, vp, vp])
 _sig("nx_mod_switch_to", [vp, sz, vp])
 _sig("nx_plain_mod_switch_to", [vp, sz, vp])
@@ -666,6 +667,13 @@ def rescale_to_next_many(cts):
     _check(_lib.nx_rescale_to_next_many((vp * len(cts))(*[c.h for c in cts]), len(cts),
                                         (vp * len(outs))(*[o.h for o in outs])))
     return outs
+
+
+def multiply_sum(a, b, dest=None):
+    """sum_i a[i] * b[i] as one size-3 ciphertext (word for word multiply + add)"""
+    d = dest or Ciphertext(a[0].ctx)
+    _check(_lib.nx_multiply_sum((vp * len(a))(*[c.h for c in a]), (vp * len(b))(*[c.h for c in b]), len(a), d.h))
+    return d
 
 
 def multiply_relin_rescale_many(a, b, relin_keys, fused=False):

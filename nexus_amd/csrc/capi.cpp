@@ -641,6 +641,23 @@ int nx_multiply_relin_rescale_fused_many(const nx_ciphertext *const *a, const nx
     return multiply_relin_rescale_many(a, b, count, relin_keys, dests, true);
 }
 
+int nx_multiply_sum(const nx_ciphertext *const *a, const nx_ciphertext *const *b, size_t count, nx_ciphertext *dest)
+{
+    return guard([&] {
+        need(a, "a");
+        need(b, "b");
+        need(dest, "dest");
+        std::vector<const nx::Ciphertext *> x, y;
+        for (size_t i = 0; i < count; i++) {
+            need(a[i], "a[i]");
+            need(b[i], "b[i]");
+            x.push_back(&a[i]->impl);
+            y.push_back(&b[i]->impl);
+        }
+        nx::op_multiply_sum(x, y, dest->impl);
+    });
+}
+
 int nx_relinearize(const nx_ciphertext *a, const nx_kswitch_keys *relin_keys, nx_ciphertext *dest)
 {
     return guard([&] {

@@ -927,15 +927,20 @@ std::vector<Ciphertext> BertLayer::attention_values(AttnState &st, const std::ve
                 for (auto &x : pbs) pp.push_back(&x);
                 ev.rescale_to_next_many(pp, pbs);
             }
-            for (int tau = lo; tau < hi; tau++) {
-                Ciphertext &pb = pbs[tau - lo];
-                Ciphertext prod;
-                ev.multiply(pb, vr[tau - lo], prod);
+            // sum_tau Pb_tau (x) rot(v, 256 tau) in one pass (no product written out)
+            {
+                std::vector<const Ciphertext *> pa, pv;
+                for (int tau = lo; tau < hi; tau++) {
+                    pa.push_back(&pbs[tau - lo]);
+                    pv.push_back(&vr[tau - lo]);
+                }
+                Ciphertext part;
+                ev.multiply_sum(pa, pv, part);
                 if (!have_acc) {
-                    acc = std::move(prod);
+                    acc = std::move(part);
                     have_acc = true;
                 } else {
-                    ev.add_inplace(acc, prod);
+                    ev.add_inplace(acc, part);
                 }
             }
             mark("attention.expansion");

@@ -480,8 +480,8 @@ Ciphertext CKKSEvaluator::exp(const Ciphertext &x_in)
     return x;
 }
 
-// exp() over inputs of one level and scale in lockstep: every squaring of all
-// inputs one batched multiply-relinearise-rescale; word for word exp()
+// exp() over inputs of one level and scale in lockstep: exp()'s operation
+// sequence, every squaring of all inputs one batched multiply-relinearise-rescale
 std::vector<Ciphertext> CKKSEvaluator::exp_many(const std::vector<const Ciphertext *> &xs)
 {
     const Evaluator &ev = evaluator;
@@ -511,7 +511,8 @@ std::vector<Ciphertext> CKKSEvaluator::exp_many(const std::vector<const Cipherte
     return y;
 }
 
-// inverse() over inputs of one level and scale in lockstep; word for word inverse()
+// inverse() over inputs of one level and scale in lockstep: inverse()'s
+// operation sequence, every multiplication one batched call
 std::vector<Ciphertext> CKKSEvaluator::inverse_many(const std::vector<const Ciphertext *> &xs, int iter)
 {
     const Evaluator &ev = evaluator;

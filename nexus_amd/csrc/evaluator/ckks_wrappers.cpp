@@ -542,6 +542,29 @@ void Evaluator::rescale_to_next_many(const std::vector<const Ciphertext *> &cts,
     dest = std::move(res);
 }
 
+void Evaluator::multiply_sum(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                             Ciphertext &dest) const
+{
+    if (a.empty() || a.size() != b.size()) throw std::invalid_argument("multiply_sum: operand counts differ");
+    if (per_op_) {
+        Ciphertext acc, t;
+        for (size_t i = 0; i < a.size(); i++) {
+            multiply(*a[i], *b[i], i ? t : acc);
+            if (i) add_inplace(acc, t);
+        }
+        dest = std::move(acc);
+        return;
+    }
+    std::vector<const nx_ciphertext *> x, y;
+    for (size_t i = 0; i < a.size(); i++) {
+        x.push_back(a[i]->get());
+        y.push_back(b[i]->get());
+    }
+    Ciphertext out(*ctx_);
+    check(nx_multiply_sum(x.data(), y.data(), x.size(), out.get()));
+    dest = std::move(out);
+}
+
 void Evaluator::multiply_relin_rescale_many(const std::vector<const Ciphertext *> &a,
                                             const std::vector<const Ciphertext *> &b, const RelinKeys &rk,
                                             std::vector<Ciphertext> &dest, bool fused) const

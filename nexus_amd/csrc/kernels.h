@@ -139,6 +139,10 @@ struct PairPtrs {
 // c01[r][2][l][n] = (a_r0 b_r0, a_r0 b_r1 + a_r1 b_r0), c2[r][l][n] = a_r1 b_r1
 void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const uint64_t *const *a,
                       const uint64_t *const *b, uint32_t count, uint32_t limbs, hipStream_t s);
+// out[3][limbs][n] (+)= sum_r a_r (x) b_r, the size-3 tensors of count <= kPtrMax
+// pairs of [2][limbs][n] ciphertexts (primes of at most 60 bits)
+void poly_tensor_sum(const DeviceTables &t, uint64_t *out, const uint64_t *const *a, const uint64_t *const *b,
+                     uint32_t count, uint32_t limbs, bool accumulate, hipStream_t s);
 // acc[p][limb][c] += d[p][limb][c] * w (Shoup constant of prime `limb`) for polys p
 void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stride, const uint64_t *d,
                          uint64_t d_stride, uint32_t limb, ulonglong2 w, uint32_t polys, hipStream_t s);

@@ -133,6 +133,41 @@ __global__ void k_tensor_many(uint64_t *c01, uint64_t *c2, PairPtrs p, const Mod
     c2[(uint64_t)r * ps + idx] = mul_mod(a1, b1, m);
 }
 
+// out[0..2][limb] (+)= sum_r a_r (x) b_r (the size-3 tensor of each pair),
+// count <= kPtrMax pairs of [2][limbs][n] ciphertexts: 128-bit sums of the
+// products (each below 2^120; at most 2 count <= 64 of them per output word)
+// and one Barrett reduction
+__global__ void k_tensor_sum(uint64_t *out, PairPtrs p, uint32_t count, const ModConst *mod, uint32_t limbs,
+                             uint32_t n, int accumulate)
+{
+    const uint32_t c = blockIdx.x * kBlock + threadIdx.x;
+    if (c >= n) return;
+    const uint32_t limb = blockIdx.y;
+    const ModConst m = mod[limb];
+    const uint64_t ps = (uint64_t)limbs * n;
+    const uint64_t idx = (uint64_t)limb * n + c;
+    uint64_t lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    auto mac = [&](int k, uint64_t x, uint64_t y) {
+        const uint64_t pl = x * y, ph = mulhi64(x, y);
+        lo[k] += pl;
+        hi[k] += ph + (lo[k] < pl);
+    };
+    for (uint32_t r = 0; r < count; r++) {
+        const uint64_t *a = p.x[r], *b = p.y[r];
+        const uint64_t a0 = a[idx], a1 = a[ps + idx], b0 = b[idx], b1 = b[ps + idx];
+        mac(0, a0, b0);
+        mac(1, a0, b1);
+        mac(1, a1, b0);
+        mac(2, a1, b1);
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        uint64_t v = barrett128(hi[k], lo[k], m);
+        if (accumulate) v = add_mod(v, out[k * ps + idx], m.q);
+        out[k * ps + idx] = v;
+    }
+}
+
 // out_r[p][limb] = (x_r[p][limb] - tmp[r * polys + p][limb]) inv[limb] for the
 // first l_out limbs; x_r has x_limbs limbs per poly (pointer lists p.y = x, p.x
 // unused, outs = out pointers)
@@ -437,6 +472,21 @@ void poly_tensor_many(const DeviceTables &t, uint64_t *c01, uint64_t *c2, const 
     ProfScope prof("tensor", s, 56.0 * (1u << t.logn) * limbs * count);
     hipLaunchKernelGGL(k_tensor_many, grid_for(t.logn, count * limbs), dim3(kBlock), 0, s, c01, c2, p, t.mod, limbs,
                        1u << t.logn);
+}
+
+void poly_tensor_sum(const DeviceTables &t, uint64_t *out, const uint64_t *const *a, const uint64_t *const *b,
+                     uint32_t count, uint32_t limbs, bool accumulate, hipStream_t s)
+{
+    if (count == 0 || count > (uint32_t)kPtrMax) throw std::invalid_argument("poly_tensor_sum: 1..32 pairs");
+    if (t.max_bits > 60) throw std::invalid_argument("poly_tensor_sum: primes of at most 60 bits");
+    PairPtrs p;
+    for (uint32_t i = 0; i < kPtrMax; i++) {
+        p.x[i] = i < count ? a[i] : nullptr;
+        p.y[i] = i < count ? b[i] : nullptr;
+    }
+    ProfScope prof("tensor", s, 8.0 * (1u << t.logn) * limbs * (4.0 * count + (accumulate ? 6.0 : 3.0)));
+    hipLaunchKernelGGL(k_tensor_sum, dim3(((1u << t.logn) + kBlock - 1) / kBlock, limbs), dim3(kBlock), 0, s, out, p,
+                       count, t.mod, limbs, 1u << t.logn, accumulate ? 1 : 0);
 }
 
 void rns_add_limb_scaled(const DeviceTables &t, uint64_t *acc, uint64_t acc_stride, const uint64_t *d,

@@ -214,6 +214,49 @@ void op_multiply(const Ciphertext &a, const Ciphertext &b, Ciphertext &dst)
 
 void op_square(const Ciphertext &a, Ciphertext &dst) { op_multiply(a, a, dst); }
 
+// sum_r a_r b_r as one size-3 ciphertext: multiply and add of every pair, each
+// tensor never written out (word for word the multiply / add sequence)
+void op_multiply_sum(const std::vector<const Ciphertext *> &a, const std::vector<const Ciphertext *> &b,
+                     Ciphertext &dst)
+{
+    require(!a.empty() && a.size() == b.size(), "operand counts must match");
+    const Ciphertext &a0 = *a[0];
+    valid(a0);
+    Context &c = *a0.ctx;
+    const size_t l = a0.limbs;
+    const double scale = a0.scale * b[0]->scale;
+    for (size_t r = 0; r < a.size(); r++) {
+        valid(*a[r]);
+        valid(*b[r]);
+        same_params(*a[r], *b[r]);
+        require(a[r]->ctx == &c && a[r]->limbs == l, "operands must share the level");
+        require(a[r]->ntt && b[r]->ntt, "encrypted1 or encrypted2 must be in NTT form");
+        require(a[r]->size == 2 && b[r]->size == 2, "multiply expects two size-2 ciphertexts (relinearize first)");
+        require(are_close(a[r]->scale * b[r]->scale, scale), "scale mismatch");
+    }
+    require(scale_within_bounds(c, scale, l), "scale out of bounds");
+    tally("tensor", l, (double)a.size());
+    tally("add", l, (double)(a.size() - 1));
+    DevBuf buf(&c, 3 * poly_words(c, l));
+    for (size_t base = 0; base < a.size(); base += (size_t)kPtrMax) {
+        const size_t cnt = std::min<size_t>(kPtrMax, a.size() - base);
+        std::vector<const uint64_t *> pa(cnt), pb(cnt);
+        for (size_t r = 0; r < cnt; r++) {
+            pa[r] = a[base + r]->data.get();
+            pb[r] = b[base + r]->data.get();
+        }
+        poly_tensor_sum(c.tables, buf.get(), pa.data(), pb.data(), (uint32_t)cnt, (uint32_t)l, base > 0, c.stream);
+    }
+    Ciphertext tmp;
+    tmp.ctx = &c;
+    tmp.size = 3;
+    tmp.limbs = l;
+    tmp.scale = scale;
+    tmp.ntt = true;
+    tmp.data = std::move(buf);
+    dst = std::move(tmp);
+}
+
 // ---------------------------------------------------------------- key switching
 // Adds key_switch(target) into (ct0, ct1); evaluator.cpp:2724-3021.
 // zero_c1: write instead of accumulate into ct1 (apply_galois clears c1 first).

@@ -156,3 +156,18 @@ def test_batched_rejects_mixed_levels(env):
         nx.multiply_relin_rescale_many([a[0], low], [a[0], low], rk)
     with pytest.raises(nx.NexusError):
         nx.multiply_relin_rescale_many([a[0], low], [a[0], low], rk, fused=True)
+
+
+def test_multiply_sum_matches(env):
+    """sum_i a_i * b_i in one pass equals multiply and add per pair word for
+    word (more pairs than one launch takes)"""
+    ctx, sk, rk = env
+    _note("multiply_sum logn=%d" % ctx.log_n)
+    a = _cts(ctx, sk, 37, 410)
+    b = _cts(ctx, sk, 37, 460)
+    got = nx.multiply_sum(a, b)
+    want = nx.multiply(a[0], b[0])
+    for x, y in zip(a[1:], b[1:]):
+        want = nx.add(want, nx.multiply(x, y))
+    assert got.info()["size"] == 3 and got.info()["scale"] == want.info()["scale"]
+    assert np.array_equal(got.to_array(), want.to_array())
