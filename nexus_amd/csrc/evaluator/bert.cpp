@@ -199,12 +199,14 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
             for (auto &t : r[c]) baby[c].push_back(std::move(t));
         }
     }
-    std::vector<Ciphertext> out(d.n_out);
+    // every output's giant-step accumulators as block mixes over the babies of
+    // each input group (the periodic diagonals are 512 words a limb); then the
+    // giant rotations of every output summed in one call (one ModDown launch
+    // for all outputs) and all outputs rescaled together
+    std::vector<std::vector<Ciphertext>> accs(d.n_out);
+    std::vector<std::vector<const Ciphertext *>> ap(d.n_out);
+    std::vector<std::vector<int>> gsteps(d.n_out);
     for (int o = 0; o < d.n_out; o++) {
-        // every giant step's multiply-accumulate as block mixes over the
-        // babies of each input group (the periodic diagonals are 512 words a
-        // limb); the giant rotations summed with a single ModDown
-        std::vector<Ciphertext> accs;
         for (int gi = 0; gi < enc->groups; gi++) {
             const int c0 = gi * enc->group, c1 = std::min(d.n_in, c0 + enc->group);
             std::vector<const Ciphertext *> terms;
@@ -213,22 +215,25 @@ std::vector<Ciphertext> BertLayer::linear(Matrix m, const std::vector<Ciphertext
             std::vector<Ciphertext> part;
             ev.multiply_periodic_mix(terms, *enc->sets[(size_t)o * enc->groups + gi],
                                      enc->index[(size_t)o * enc->groups + gi], part);
-            if (accs.empty())
-                accs = std::move(part);
+            if (accs[o].empty())
+                accs[o] = std::move(part);
             else
-                for (int a = 0; a < n_a; a++) ev.add_inplace(accs[a], part[a]);
+                for (int a = 0; a < n_a; a++) ev.add_inplace(accs[o][a], part[a]);
         }
-        std::vector<const Ciphertext *> ap;
-        std::vector<int> gsteps;
         for (int a = d.a_min; a <= d.a_max; a++)
             if (enc->giant_any[(size_t)o * n_a + (a - d.a_min)]) {
-                ap.push_back(&accs[a - d.a_min]);
-                gsteps.push_back(B * a);
+                ap[o].push_back(&accs[o][a - d.a_min]);
+                gsteps[o].push_back(B * a);
             }
-        if (ap.empty()) throw std::logic_error("BertLayer: all-zero weight block");
-        ev.rotate_sum(ap, gsteps, *ck_.galois_keys, out[o]);
-        ev.rescale_to_next_inplace(out[o]);
+        if (ap[o].empty()) throw std::logic_error("BertLayer: all-zero weight block");
     }
+    std::vector<Ciphertext> sums;
+    ev.rotate_sum_many(ap, gsteps, *ck_.galois_keys, sums);
+    accs.clear();
+    std::vector<const Ciphertext *> sp;
+    for (auto &x : sums) sp.push_back(&x);
+    std::vector<Ciphertext> out;
+    ev.rescale_to_next_many(sp, out);
     return out;
 }
 
@@ -961,16 +966,15 @@ std::vector<Ciphertext> BertLayer::layer_norm(const std::vector<Ciphertext> &r)
     if (r.size() != n) throw std::invalid_argument("BertLayer: layer_norm needs hidden / 256 ciphertexts");
     const Evaluator &ev = ck_.evaluator;
     const GaloisKeys &gk = *ck_.galois_keys;
+    // the squares of every ciphertext in one batched multiply-relinearise-rescale
     Ciphertext tot;
-    for (size_t c = 0; c < n; c++) {
-        Ciphertext sq;
-        ev.square(r[c], sq);
-        ev.relinearize_inplace(sq, *ck_.relin_keys);
-        ev.rescale_to_next_inplace(sq);
-        if (c == 0)
-            tot = std::move(sq);
-        else
-            ev.add_inplace(tot, sq);
+    {
+        std::vector<const Ciphertext *> rp;
+        for (auto &x : r) rp.push_back(&x);
+        std::vector<Ciphertext> sq;
+        ev.multiply_relin_rescale_many(rp, rp, *ck_.relin_keys, sq);
+        tot = std::move(sq[0]);
+        for (size_t c = 1; c < n; c++) ev.add_inplace(tot, sq[c]);
     }
     for (int st = 1; st < kGroup; st <<= 1) {
         Ciphertext t;
@@ -991,14 +995,15 @@ std::vector<Ciphertext> BertLayer::layer_norm(const std::vector<Ciphertext> &r)
         ev.add_inplace(tot, t);
     }
     Ciphertext y = ck_.invert_sqrt(tot, 4, 2);
-    std::vector<Ciphertext> out(n);
+    // every ciphertext times y in one batched multiply-relinearise-rescale
+    std::vector<Ciphertext> xs(n);
+    std::vector<const Ciphertext *> yp(n, &y), xp;
     for (size_t c = 0; c < n; c++) {
-        Ciphertext x = r[c];
-        ev.mod_switch_to_inplace(x, y.params_id());
-        ev.multiply(y, x, out[c]);
-        ev.relinearize_inplace(out[c], *ck_.relin_keys);
-        ev.rescale_to_next_inplace(out[c]);
+        ev.mod_switch_to(r[c], y.params_id(), xs[c]);
+        xp.push_back(&xs[c]);
     }
+    std::vector<Ciphertext> out;
+    ev.multiply_relin_rescale_many(yp, xp, *ck_.relin_keys, out);
     return out;
 }
 
