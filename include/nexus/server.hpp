@@ -55,6 +55,10 @@ public:
         bool per_op = false;         // the lanes' evaluators run in per-op mode (Evaluator::set_per_op)
         bool pack = true;            // continuous batching of BERT requests
         int bert_concurrency = 1;    // BERT layer calls in flight at once (one call fills the GPU)
+        // sets of queued BERT requests run in one lockstep layer call when more
+        // are queued than one set holds (BertLayer::forward_sets: bootstrapping
+        // steps over all sets, 1.06x the throughput of one call per set)
+        int bert_sets_per_call = 2;
         size_t bert_footprint = size_t(8) << 30;  // working set of one BERT layer call
     };
     struct Stats {

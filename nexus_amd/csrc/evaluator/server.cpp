@@ -209,6 +209,7 @@ void Server::run_lane(int lane)
     for (;;) {
         // a request, plus (continuous batching) queued BERT requests packed with it
         std::vector<std::shared_ptr<Request>> group;
+        std::vector<int> set_of;  // set of each member (sets run in one lockstep call)
         bool bert_call = false;
         {
             std::unique_lock<std::mutex> lk(mu_);
@@ -235,6 +236,7 @@ void Server::run_lane(int lane)
                     queue_.erase(pick);
                     r->charged = r->footprint;
                     group.push_back(r);
+                    set_of.push_back(0);
                     for (const Job &j : r->jobs)
                         if (j.kind == JobKind::Bert) {
                             bert_call = true;
@@ -242,21 +244,39 @@ void Server::run_lane(int lane)
                         }
                     if (bert_call) bert_running_++;
                     if (opt_.pack && bert && packable(*r)) {
-                        // later BERT requests whose rows still fit ride in the same set
+                        // later BERT requests whose rows still fit ride in the same set;
+                        // with more queued, further sets run in the same lockstep call
                         const Ciphertext &c0 = r->jobs[0].inputs[0];
+                        auto compatible = [&](const Request &q) {
+                            return packable(q) && q.jobs[0].inputs[0].scale() == c0.scale() &&
+                                   q.jobs[0].inputs[0].coeff_modulus_size() == c0.coeff_modulus_size();
+                        };
                         size_t used = (size_t)r->jobs[0].len;
-                        for (auto it = queue_.begin(); it != queue_.end() && used < tokens_;) {
-                            const Request &q = **it;
-                            if (packable(q) && used + (size_t)q.jobs[0].len <= tokens_ &&
-                                q.jobs[0].inputs[0].scale() == c0.scale() &&
-                                q.jobs[0].inputs[0].coeff_modulus_size() == c0.coeff_modulus_size()) {
-                                used += (size_t)q.jobs[0].len;
-                                (*it)->charged = 0;  // shares the set's working memory
-                                group.push_back(*it);
-                                it = queue_.erase(it);
-                            } else {
-                                ++it;
+                        int set = 0;
+                        for (;;) {
+                            for (auto it = queue_.begin(); it != queue_.end() && used < tokens_;) {
+                                const Request &q = **it;
+                                if (compatible(q) && used + (size_t)q.jobs[0].len <= tokens_) {
+                                    used += (size_t)q.jobs[0].len;
+                                    (*it)->charged = 0;  // shares the call's working memory
+                                    group.push_back(*it);
+                                    set_of.push_back(set);
+                                    it = queue_.erase(it);
+                                } else {
+                                    ++it;
+                                }
                             }
+                            if (set + 1 >= opt_.bert_sets_per_call) break;
+                            // the next set starts with the oldest compatible request left
+                            auto nx = std::find_if(queue_.begin(), queue_.end(),
+                                                   [&](const std::shared_ptr<Request> &q) { return compatible(*q); });
+                            if (nx == queue_.end()) break;
+                            set++;
+                            used = (size_t)(*nx)->jobs[0].len;
+                            (*nx)->charged = 0;
+                            group.push_back(*nx);
+                            set_of.push_back(set);
+                            queue_.erase(nx);
                         }
                     }
                     for (auto &g : group) {
@@ -278,22 +298,33 @@ void Server::run_lane(int lane)
         std::string err;
         try {
             if (group.size() > 1) {
-                // pack: member k at rows [off_k, off_k + len_k), one layer call, unpack
-                std::vector<Ciphertext> set;
-                std::vector<BertLayer::Segment> segs;
-                int off = 0;
-                for (auto &g : group) {
-                    const Job &j = g->jobs[0];
-                    std::vector<Ciphertext> sh = bert->shift_rows(j.inputs, off);
-                    if (set.empty())
-                        set = std::move(sh);
+                // pack: member k at rows [off_k, off_k + len_k) of its set, one
+                // layer call (sets in lockstep when there are several), unpack:
+                // every member of the call gets its rows moved back, the others cleared
+                const int nsets = set_of.back() + 1;
+                std::vector<std::vector<Ciphertext>> sets(nsets);
+                std::vector<std::vector<BertLayer::Segment>> segs(nsets);
+                std::vector<int> off(nsets, 0);
+                std::vector<BertLayer::Segment> where(group.size());
+                for (size_t k = 0; k < group.size(); k++) {
+                    const int st = set_of[k];
+                    const Job &j = group[k]->jobs[0];
+                    std::vector<Ciphertext> sh = off[st] ? bert->shift_rows(j.inputs, off[st]) : j.inputs;
+                    if (sets[st].empty())
+                        sets[st] = std::move(sh);
                     else
-                        for (size_t c = 0; c < set.size(); c++) ck.evaluator.add_inplace(set[c], sh[c]);
-                    segs.push_back({off, j.len});
-                    off += j.len;
+                        for (size_t c = 0; c < sets[st].size(); c++) ck.evaluator.add_inplace(sets[st][c], sh[c]);
+                    where[k] = {off[st], j.len};
+                    segs[st].push_back(where[k]);
+                    off[st] += j.len;
                 }
-                std::vector<Ciphertext> y = bert->forward(set, segs);
-                for (size_t k = 0; k < group.size(); k++) outs[k] = bert->extract_rows(y, segs[k].start, segs[k].len);
+                std::vector<std::vector<Ciphertext>> ys;
+                if (nsets > 1)
+                    ys = bert->forward_sets(sets, segs);
+                else
+                    ys.push_back(bert->forward(sets[0], segs[0]));
+                for (size_t k = 0; k < group.size(); k++)
+                    outs[k] = bert->extract_rows(ys[set_of[k]], where[k].start, where[k].len);
             } else {
                 for (Job &j : group[0]->jobs) {
                     if (j.kind == JobKind::Bert) {
