@@ -760,7 +760,8 @@ void op_multiply_relin_rescale_fused_many(const std::vector<const Ciphertext *> 
     ks_out_primes(c, l, outp);
     const size_t A = alpha + 1, lo = l - 1;
     std::vector<Ciphertext> res(a.size());
-    const size_t chunk = (size_t)kNttSrcMax / 2;
+    // every buffer of a chunk is contiguous: up to one epilogue output per product
+    const size_t chunk = (size_t)std::min(kEpiMax, kPtrMax);
     for (size_t base = 0; base < a.size(); base += chunk) {
         const size_t cnt = std::min(chunk, a.size() - base);
         std::vector<const uint64_t *> pa, pb;
