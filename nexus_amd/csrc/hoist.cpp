@@ -190,12 +190,20 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
     DevBuf outs(&c, G * 2 * pw), acc(&c, G * 2 * nout * n);
     uint8_t outp[kMaxPrimes];
     ks_out_primes(c, l, outp);
-    std::vector<char> switched(G, 0);
+    // the switched inputs of every group, in group order: their ModUps run in
+    // chunks of up to kMaxHoist inputs across group boundaries, and each group's
+    // inner products read its inputs' digits from the chunks that hold them
+    struct Sw {
+        size_t g;
+        const uint64_t *key, *seed, *c1;
+        const uint32_t *perm;
+    };
+    std::vector<Sw> sw;
     for (size_t g = 0; g < G; g++) {
         const auto &in = groups[g];
         require(!in.empty() && in.size() == steps[g].size(), "inputs and steps must match");
-        std::vector<const uint64_t *> ins, c1s, keys, seeds;
-        std::vector<const uint32_t *> perms, rperms;
+        std::vector<const uint64_t *> ins;
+        std::vector<const uint32_t *> perms;
         for (size_t i = 0; i < in.size(); i++) {
             check_input(*in[i], gk);
             require(in[i]->ctx == &c && in[i]->limbs == l, "inputs must share the level");
@@ -208,10 +216,7 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
             Prepared p = prepare(c, std::vector<int>{steps[g][i]}, gk);
             tally("ks_hoisted", l);
             perms.push_back(p.perms[0]);
-            rperms.push_back(p.perms[0]);
-            keys.push_back(p.keys[0]);
-            seeds.push_back(p.seeds[0]);
-            c1s.push_back(in[i]->data.get() + pw);
+            sw.push_back({g, p.keys[0], p.seeds[0], in[i]->data.get() + pw, p.perms[0]});
         }
         uint64_t *og = outs.get() + g * 2 * pw;
         DevBuf part;
@@ -221,22 +226,38 @@ void op_rotate_sum_many(const std::vector<std::vector<const Ciphertext *>> &grou
             rot_base(c.tables, o, ins.data() + base, perms.data() + base, (uint32_t)cnt, (uint32_t)l, c.stream);
             if (base) poly_add(c.tables, og, og, o, 2, (uint32_t)l, c.stream);
         }
-        uint64_t *ag = acc.get() + g * 2 * nout * n;
-        if (keys.empty()) {
-            // no switched term: a zero accumulator leaves the base unchanged
-            std::vector<uint64_t> zeros(nout, 0);
-            poly_fill_limbs(ag, zeros.data(), 2, (uint32_t)nout, c.logn, c.stream);
-            continue;
-        }
-        switched[g] = 1;
-        for (size_t base = 0; base < keys.size(); base += kMaxHoist) {
-            const size_t cnt = std::min<size_t>(kMaxHoist, keys.size() - base);
-            DevBuf T = ks_modup_list(c, c1s.data() + base, cnt, l);
-            ks_inner_product_rot_sum(c.tables, ag, T.get(), keys.data() + base, rperms.data() + base,
-                                     (uint32_t)cnt, (uint32_t)digits, (uint32_t)c.key_limbs(), outp, (uint32_t)nout,
-                                     base > 0, c.stream, nout * digits * n, seeds.data() + base);
+    }
+    std::vector<char> started(G, 0);
+    for (size_t base = 0; base < sw.size(); base += kMaxHoist) {
+        const size_t cnt = std::min<size_t>(kMaxHoist, sw.size() - base);
+        std::vector<const uint64_t *> c1s(cnt);
+        for (size_t r = 0; r < cnt; r++) c1s[r] = sw[base + r].c1;
+        DevBuf T = ks_modup_list(c, c1s.data(), cnt, l);
+        for (size_t r0 = 0; r0 < cnt;) {
+            const size_t g = sw[base + r0].g;
+            size_t r1 = r0;
+            while (r1 < cnt && sw[base + r1].g == g) r1++;
+            std::vector<const uint64_t *> keys, seeds;
+            std::vector<const uint32_t *> rperms;
+            for (size_t r = r0; r < r1; r++) {
+                keys.push_back(sw[base + r].key);
+                seeds.push_back(sw[base + r].seed);
+                rperms.push_back(sw[base + r].perm);
+            }
+            ks_inner_product_rot_sum(c.tables, acc.get() + g * 2 * nout * n, T.get() + r0 * nout * digits * n,
+                                     keys.data(), rperms.data(), (uint32_t)(r1 - r0), (uint32_t)digits,
+                                     (uint32_t)c.key_limbs(), outp, (uint32_t)nout, started[g] != 0, c.stream,
+                                     nout * digits * n, seeds.data());
+            started[g] = 1;
+            r0 = r1;
         }
     }
+    for (size_t g = 0; g < G; g++)
+        if (!started[g]) {
+            // no switched term: a zero accumulator leaves the base unchanged
+            std::vector<uint64_t> zeros(nout, 0);
+            poly_fill_limbs(acc.get() + g * 2 * nout * n, zeros.data(), 2, (uint32_t)nout, c.logn, c.stream);
+        }
     ks_moddown(c, outs.get(), acc.get(), G, l, false);
     for (size_t g = 0; g < G; g++) {
         Ciphertext r;
