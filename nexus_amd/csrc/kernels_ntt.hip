@@ -483,24 +483,55 @@ __device__ __forceinline__ void ntt_tile_f(const NttKArgs &a, uint64_t *lds_word
     const double q = (double)a.mod[prime].q;
     const double qinv = 1.0 / q;
     const double *tw = a.twd + (uint64_t)prime * n;
-    for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
-        uint64_t gidx;
-        if (ROLE == kCol) {
-            const int k = e / G::C, cc = e % G::C;
-            gidx = (uint64_t)k * N2 + first + cc;
-        } else if (ROLE == kRow) {
-            gidx = (uint64_t)first * G::S + e;
-        } else {
-            gidx = e;
-        }
-        lds[e] = reduce_f((double)src[gidx], q, qinv);
-    }
-    __syncthreads();
     const bool lazy = q * (0.5 + 1.25 * LOGS) <= 0x1p50;  // uniform: one prime per workgroup
-    if (lazy)
-        stages_f<LOGS, ROLE, FWD, CW, true, kThreads, R16>(lds, tw, q, qinv, n, N1, first);
-    else
-        stages_f<LOGS, ROLE, FWD, CW, false, kThreads, R16>(lds, tw, q, qinv, n, N1, first);
+    if constexpr (FWD && ROLE == kRow && LOGS == 8 && G::C == 8 && !R16) {
+        // rows of 256, 8 rows per tile, 256 threads: each thread loads one
+        // radix-8 group of the first three stages (row g / 32, positions
+        // g % 32 + 32 t) and runs it in registers; one LDS pass and one barrier
+        // fewer, the rest as below
+        {
+            const int g = threadIdx.x, cc = g / 32, o = g % 32;
+            double v[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+                v[t] = reduce_f((double)src[(uint64_t)(first + cc) * G::S + o + 32 * t], q, qinv);
+            if (lazy)
+                fwd_group_f<3, true>(v, 0, 0, N1 + first + cc, tw, q, qinv);
+            else
+                fwd_group_f<3, false>(v, 0, 0, N1 + first + cc, tw, q, qinv);
+#pragma unroll
+            for (int t = 0; t < 8; t++) lds[cc * G::S + o + 32 * t] = v[t];
+        }
+        __syncthreads();
+        if (lazy) {
+            fwd_step_f<LOGS, ROLE, 3, CW, true>(lds, 3, tw, q, qinv, N1, first);
+            __syncthreads();
+            fwd_step_f<LOGS, ROLE, 2, CW, true>(lds, 6, tw, q, qinv, N1, first);
+        } else {
+            fwd_step_f<LOGS, ROLE, 3, CW, false>(lds, 3, tw, q, qinv, N1, first);
+            __syncthreads();
+            fwd_step_f<LOGS, ROLE, 2, CW, false>(lds, 6, tw, q, qinv, N1, first);
+        }
+        __syncthreads();
+    } else {
+        for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
+            uint64_t gidx;
+            if (ROLE == kCol) {
+                const int k = e / G::C, cc = e % G::C;
+                gidx = (uint64_t)k * N2 + first + cc;
+            } else if (ROLE == kRow) {
+                gidx = (uint64_t)first * G::S + e;
+            } else {
+                gidx = e;
+            }
+            lds[e] = reduce_f((double)src[gidx], q, qinv);
+        }
+        __syncthreads();
+        if (lazy)
+            stages_f<LOGS, ROLE, FWD, CW, true, kThreads, R16>(lds, tw, q, qinv, n, N1, first);
+        else
+            stages_f<LOGS, ROLE, FWD, CW, false, kThreads, R16>(lds, tw, q, qinv, n, N1, first);
+    }
     const bool scale_out = !FWD && ROLE != kRow;  // the inverse's last pass
     const double ni = a.ninvd[prime];
     for (int e = threadIdx.x; e < G::TILE; e += kThreads) {
